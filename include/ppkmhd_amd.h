@@ -1,0 +1,221 @@
+/*
+ * ppkmhd_amd.h -- C ABI of the MI355X (gfx950) MUSCL-Hancock + HLLD + constrained-transport
+ * MHD time step.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never throws, and reports
+ * failure through an integer status (PPK_OK == 0) plus a per-handle message
+ * (ppk_mhd_last_error).  One handle owns one simulation: its device buffers, its HIP stream
+ * and its time-loop counters.  A handle is not thread-safe; calls on one handle are
+ * stream-ordered.
+ *
+ * Array layout used across this ABI ("reference layout"): 8 FP64 variables per cell in the
+ * order ID=0 (density), IP=1 (total energy), IU=2, IV=3, IW=4 (momentum), IA=5, IB=6, IC=7
+ * (magnetic field), stored variable-major with i fastest:
+ *     U[((v*ksize + k)*jsize + j)*isize + i]
+ * over the ghost-padded box isize = nx + 2*gw (same for j, k; ksize = 1 in 2D), gw = 3.
+ * IA/IB/IC hold the face-centred field on the LEFT face of the cell (x-face i-1/2, y-face
+ * j-1/2, z-face k-1/2); in 2D IC is a cell-centred Bz.  This is the LayoutLeft View
+ * DataArray{2d,3d}(isize, jsize[, ksize], nbvar) of the reference
+ * (src/shared/kokkos_shared.h:22-42, var order src/shared/enums.h:17-34).
+ */
+#ifndef PPKMHD_AMD_H_
+#define PPKMHD_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+enum ppk_status {
+  PPK_OK = 0,
+  PPK_ERR_INVALID_ARGUMENT = 1, /* bad pointer, size, enum value or unsupported option   */
+  PPK_ERR_HIP = 2,              /* a HIP runtime call failed (message names it)           */
+  PPK_ERR_OUT_OF_MEMORY = 3,
+  PPK_ERR_UNSUPPORTED = 4,      /* option the reference does not define for this path    */
+  PPK_ERR_CONFIG = 5            /* .ini file missing / unreadable / unknown solver name   */
+};
+
+/* ---- enums, same integer values as the reference ----------------------------------------- */
+/* src/shared/enums.h:141-148 (RiemannSolverType) */
+enum ppk_riemann { PPK_RIEMANN_APPROX = 0, PPK_RIEMANN_LLF = 1, PPK_RIEMANN_HLL = 2,
+                   PPK_RIEMANN_HLLC = 3, PPK_RIEMANN_HLLD = 4 };
+/* src/shared/enums.h:152-159 (BoundaryConditionType) */
+enum ppk_bc { PPK_BC_UNDEFINED = 0, PPK_BC_DIRICHLET = 1, PPK_BC_NEUMANN = 2,
+              PPK_BC_PERIODIC = 3 };
+/* arithmetic of the step.  PPK_FP64: double, the compiler may contract a*b+c into FMA.
+ * PPK_FP64_STRICT: double with every product and sum rounded separately, in the reference's
+ * operand order (bit-identical to the reference's double arithmetic).  PPK_FP32_BULK: trace
+ * and Riemann solves in float; state, CT/EMF update and dt stay double. */
+enum ppk_precision { PPK_FP64 = 0, PPK_FP32_BULK = 1, PPK_FP64_STRICT = 2 };
+/* kernel organisation of the step */
+enum ppk_kernel_path {
+  PPK_PATH_AUTO = 0,   /* fused kernels where available, else staged                         */
+  PPK_PATH_STAGED = 1, /* one kernel per reference stage, intermediates in HBM                */
+  PPK_PATH_FUSED = 2   /* LDS-tiled fused kernels                                             */
+};
+/* dt semantics of ppk_mhd_step */
+enum ppk_dt_mode {
+  PPK_DT_FIXED = 0, /* use the dt argument as given (parity runs at fixed dt)                 */
+  PPK_DT_CFL = 1    /* reference semantics: dt = cfl / max(sum_d (c_f,d+|v_d|)/dx_d) from the
+                       current state, clamped to tEnd - t (src/shared/SolverBase.cpp:149-179) */
+};
+/* initial conditions (src/muscl/SolverMHDMuscl.h:649-713 problem names) */
+enum ppk_problem_kind { PPK_PROBLEM_ORSZAG_TANG = 0, PPK_PROBLEM_BLAST = 1,
+                        PPK_PROBLEM_KELVIN_HELMHOLTZ = 2, PPK_PROBLEM_WAVE = 3,
+                        PPK_PROBLEM_IMPLODE = 4 };
+
+/*
+ * Solver parameters.  Mirrors the fields of HydroParams / HydroSettings that the MHD MUSCL
+ * path reads (src/shared/HydroParams.h:28-234); derived fields (dx, smallp, smallpp, gamma6)
+ * follow HydroParams::init (src/shared/HydroParams.cpp:421-453).  ppk_mhd_params_from_ini
+ * fills it exactly as HydroParams::setup does, including the float rounding of .ini reals.
+ */
+typedef struct ppk_mhd_params {
+  int32_t dim;           /* 2 or 3                                                     */
+  int32_t nx, ny, nz;    /* interior cells (nz = 1 in 2D)                              */
+  int32_t ghost_width;   /* 3 for MHD (src/shared/HydroParams.cpp:56-71)               */
+  int32_t riemann;       /* enum ppk_riemann: HLLD, HLL or LLF                         */
+  int32_t bc[6];         /* enum ppk_bc for xmin, xmax, ymin, ymax, zmin, zmax         */
+  int32_t n_step_max;    /* [run] nStepmax                                             */
+  int32_t precision;     /* enum ppk_precision                                         */
+  int32_t kernel_path;   /* enum ppk_kernel_path                                       */
+  int32_t reserved;      /* must be 0                                                  */
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  double dx, dy, dz;
+  double gamma0, gamma6, cfl, slope_type, smallr, smallc, smallp, smallpp, cIso;
+  double tEnd;
+} ppk_mhd_params;
+
+/*
+ * Initial-condition parameters (problems used by the MHD path: Orszag-Tang, blast,
+ * Kelvin-Helmholtz, linear wave, implode).  Field defaults are the reference's
+ * (src/shared/problems/{OrszagTang,Blast,KH,Wave,Implode}Params.h).  Extensions beyond the
+ * reference are marked "ext" and default to the reference behaviour.
+ */
+typedef struct ppk_mhd_problem {
+  int32_t kind;          /* enum ppk_problem_kind                                      */
+  int32_t kh_mode;       /* KH perturbation mode                                        */
+  int32_t kh_p_sine;     /* KH perturbation_sine                                        */
+  int32_t kh_p_sine_rob; /* KH perturbation_sine_robertson                              */
+  int32_t kh_p_rand;     /* KH perturbation_rand (counter-based RNG, see below)         */
+  int32_t kh_seed;       /* KH rand_seed                                                */
+  int32_t kh_axis;       /* ext: shear-normal axis for the 3D layer (2 = z, reference; 1 = y) */
+  int32_t wave_type;     /* linear wave 0 fast, 1 Alfven, 2 slow, 3 contact              */
+  int32_t implode_shape; /* implode shape                                               */
+  int32_t pad;
+  double ot_kt;          /* OrszagTang kt (3D only)                                     */
+  double ot_b0_scale;    /* ext: multiplies B0 (1.0 = reference)                        */
+  double blast_radius, blast_center_x, blast_center_y, blast_center_z;
+  double blast_density_in, blast_density_out, blast_pressure_in, blast_pressure_out;
+  double blast_bx, blast_by, blast_bz; /* ext: uniform B (0.5, 0.5, 0.5 = reference)     */
+  double kh_d_in, kh_d_out, kh_pressure, kh_vflow_in, kh_vflow_out, kh_amplitude;
+  double kh_w0, kh_delta;
+  double kh_bx, kh_by, kh_bz; /* ext: uniform B of the 3D random layer (0 = reference)  */
+  double wave_amplitude;
+  /* WaveParams re-reads the mesh extents and gamma0 with its own defaults (3.0, 1.5, 1.5,
+   * 1.66667; src/shared/problems/WaveParams.h:33-45), so they are carried separately */
+  double wave_gamma0, wave_lx, wave_ly, wave_lz;
+  /* implode inner / outer state: rho, p, u, v, w, Bx, By, Bz
+   * (src/shared/problems/ImplodeParams.h:40-58) */
+  double implode_in[8], implode_out[8];
+} ppk_mhd_problem;
+
+typedef struct ppk_mhd_solver ppk_mhd_t;
+
+/* ---- library ----------------------------------------------------------------------------- */
+int ppk_mhd_params_size(void);  /* sizeof(ppk_mhd_params), for FFI layout checks  */
+int ppk_mhd_problem_size(void); /* sizeof(ppk_mhd_problem)                         */
+const char *ppk_mhd_version(void);
+/* number of HIP devices visible (0 when none); never fails */
+int ppk_mhd_device_count(void);
+
+/*
+ * Parse an .ini file the way the reference does: case-insensitive sections and keys
+ * (src/utils/config/inih/INIReader.cpp:100-105), reals through strtof, i.e. rounded to float
+ * (src/utils/config/ConfigMap.cpp:37-46), HydroParams::setup defaults
+ * (src/shared/HydroParams.cpp:28-217) and the problem parameter defaults.  solver_name
+ * receives [run] solver_name (NUL-terminated, truncated to name_len).
+ * Replaces: HydroParams::setup + broadcast_parameters (src/main.cpp:133-137).
+ */
+int ppk_mhd_params_from_ini(const char *ini_path, ppk_mhd_params *params,
+                            ppk_mhd_problem *problem, char *solver_name, int name_len);
+/* Same, from an in-memory .ini text. */
+int ppk_mhd_params_from_ini_text(const char *ini_text, ppk_mhd_params *params,
+                                 ppk_mhd_problem *problem, char *solver_name, int name_len);
+/* Fill the derived fields (dx, dy, dz, smallp, smallpp, gamma6) from the primary ones,
+ * as HydroParams::init does (src/shared/HydroParams.cpp:421-453). */
+int ppk_mhd_params_init(ppk_mhd_params *params);
+
+/* ---- solver handle ----------------------------------------------------------------------- */
+/*
+ * Create a solver on the current HIP device with its own stream.  Allocates the state
+ * (two ping-pong copies, as U / U2 of src/muscl/SolverMHDMuscl.h:73-77) and workspace.
+ * Replaces: SolverMHDMuscl<dim>::SolverMHDMuscl allocation part
+ * (src/muscl/SolverMHDMuscl.h:212-387) registered under MHD_Muscl_2D / MHD_Muscl_3D.
+ */
+int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle);
+void ppk_mhd_destroy(ppk_mhd_t *handle);
+/* Message of the last failed call on this handle ("" when none).  handle may be NULL for
+ * the last failure of ppk_mhd_create / params parsing on this thread. */
+const char *ppk_mhd_last_error(const ppk_mhd_t *handle);
+
+/* Initial condition computed by the library (src/muscl/MHDInitFunctors{2D,3D}.h), then
+ * ghost fill, as SolverMHDMuscl ctor does (src/muscl/SolverMHDMuscl.h:389-402).
+ * Resets t = params.tCurrent (0), iteration = 0. */
+int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem);
+
+/* Host <-> device copies of the current state in the reference layout (count =
+ * 8*isize*jsize*ksize doubles).  download synchronizes the handle's stream. */
+int ppk_mhd_upload(ppk_mhd_t *h, const double *U, size_t count);
+int ppk_mhd_download(ppk_mhd_t *h, double *U, size_t count);
+/* Device pointer of the current state (same layout); valid until the next step. */
+int ppk_mhd_device_state(ppk_mhd_t *h, double **dU);
+/* The handle's HIP stream (hipStream_t), for callers that order their own work on it. */
+int ppk_mhd_stream(ppk_mhd_t *h, void **stream);
+
+/* Ghost fill of the current state, faces X then Y then Z over the full transverse range.
+ * Replaces: make_boundaries (src/muscl/SolverMHDMuscl.cpp:22-64,
+ * src/shared/SolverBase.cpp:339-537, src/shared/BoundariesFunctors.h:534-1057). */
+int ppk_mhd_make_boundaries(ppk_mhd_t *h);
+
+/* CFL time step of the current state: dt = cfl / invDt, NOT clamped to tEnd (the clamp is
+ * part of the time loop).  Synchronizes.
+ * Replaces: SolverMHDMuscl::compute_dt_local (src/muscl/SolverMHDMuscl.h:722-741). */
+int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt);
+
+/*
+ * One time step (godunov_unsplit_impl, src/muscl/SolverMHDMuscl.cpp:373-572): ghost fill,
+ * primitives, dt (per dt_mode), trace, fluxes, EMFs, flux update and CT update, written to
+ * the other ping-pong buffer; then t += dt, iteration += 1
+ * (src/shared/SolverBase.cpp:206-220).  Stream-ordered; does not synchronize.
+ * Replaces: SolverBase::next_iteration -> SolverMHDMuscl::next_iteration_impl (the IO part
+ * of next_iteration_impl is not on the path).
+ */
+int ppk_mhd_step(ppk_mhd_t *h, double dt, int dt_mode);
+/* nsteps calls of ppk_mhd_step(h, dt, dt_mode) without host synchronization, stopping on
+ * the device once t >= tEnd - 1e-14 or iteration >= n_step_max (SolverBase::finished,
+ * src/shared/SolverBase.cpp:195-201); steps past that point do no work. */
+int ppk_mhd_run(ppk_mhd_t *h, int nsteps, double dt, int dt_mode);
+/* Current time, last dt and iteration counter (synchronizes). */
+int ppk_mhd_get_time(ppk_mhd_t *h, double *t, double *dt, int64_t *iteration);
+/* Set time and iteration (restart); iteration parity selects the ping-pong buffer. */
+int ppk_mhd_set_time(ppk_mhd_t *h, double t, int64_t iteration);
+/* 1 when SolverBase::finished() holds (synchronizes). */
+int ppk_mhd_finished(ppk_mhd_t *h, int *finished);
+/* Wait for all work of the handle. */
+int ppk_mhd_synchronize(ppk_mhd_t *h);
+
+/* Diagnostics (not on the reference path): max over interior cells of |div B| computed
+ * from the face-centred field, and the interior sums of the 8 conservative variables.
+ * Synchronizes. */
+int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max);
+int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPKMHD_AMD_H_ */

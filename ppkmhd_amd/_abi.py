@@ -1,0 +1,136 @@
+"""ctypes mirror of the plain-data structs in include/ppkmhd_amd.h.
+
+The field order and types must match the C header exactly; tests compare ``ctypes.sizeof``
+against ``ppk_mhd_params_size()`` / ``ppk_mhd_problem_size()`` exported by the library.
+"""
+import ctypes as C
+
+# variable indexes, reference src/shared/enums.h:17-34
+ID, IP, IU, IV, IW, IA, IB, IC = range(8)
+VAR_NAMES = ("rho", "energy", "rho_vx", "rho_vy", "rho_vz", "bx", "by", "bz")
+
+# enum values, reference src/shared/enums.h:141-159
+RIEMANN_APPROX, RIEMANN_LLF, RIEMANN_HLL, RIEMANN_HLLC, RIEMANN_HLLD = range(5)
+BC_UNDEFINED, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
+FP64, FP32_BULK, FP64_STRICT = 0, 1, 2
+PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
+DT_FIXED, DT_CFL = 0, 1
+PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE = range(5)
+
+PPK_OK = 0
+
+
+class Params(C.Structure):
+    """ppk_mhd_params (mirror of the HydroParams fields the MHD MUSCL path reads)."""
+
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("ghost_width", C.c_int32),
+        ("riemann", C.c_int32),
+        ("bc", C.c_int32 * 6),
+        ("n_step_max", C.c_int32),
+        ("precision", C.c_int32),
+        ("kernel_path", C.c_int32),
+        ("reserved", C.c_int32),
+        ("xmin", C.c_double),
+        ("xmax", C.c_double),
+        ("ymin", C.c_double),
+        ("ymax", C.c_double),
+        ("zmin", C.c_double),
+        ("zmax", C.c_double),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("dz", C.c_double),
+        ("gamma0", C.c_double),
+        ("gamma6", C.c_double),
+        ("cfl", C.c_double),
+        ("slope_type", C.c_double),
+        ("smallr", C.c_double),
+        ("smallc", C.c_double),
+        ("smallp", C.c_double),
+        ("smallpp", C.c_double),
+        ("cIso", C.c_double),
+        ("tEnd", C.c_double),
+    ]
+
+    @property
+    def gw(self):
+        return self.ghost_width
+
+    @property
+    def isize(self):
+        return self.nx + 2 * self.ghost_width
+
+    @property
+    def jsize(self):
+        return self.ny + 2 * self.ghost_width
+
+    @property
+    def ksize(self):
+        return self.nz + 2 * self.ghost_width if self.dim == 3 else 1
+
+    @property
+    def shape(self):
+        """numpy shape of a state array in the reference layout: (8, ksize, jsize, isize)."""
+        return (8, self.ksize, self.jsize, self.isize)
+
+    @property
+    def count(self):
+        return 8 * self.ksize * self.jsize * self.isize
+
+    def interior(self):
+        """numpy index of the interior cells of a (8, ksize, jsize, isize) array."""
+        g = self.ghost_width
+        ks = slice(g, g + self.nz) if self.dim == 3 else slice(0, 1)
+        return (slice(None), ks, slice(g, g + self.ny), slice(g, g + self.nx))
+
+
+class Problem(C.Structure):
+    """ppk_mhd_problem (initial-condition parameters)."""
+
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("kh_mode", C.c_int32),
+        ("kh_p_sine", C.c_int32),
+        ("kh_p_sine_rob", C.c_int32),
+        ("kh_p_rand", C.c_int32),
+        ("kh_seed", C.c_int32),
+        ("kh_axis", C.c_int32),
+        ("wave_type", C.c_int32),
+        ("implode_shape", C.c_int32),
+        ("pad", C.c_int32),
+        ("ot_kt", C.c_double),
+        ("ot_b0_scale", C.c_double),
+        ("blast_radius", C.c_double),
+        ("blast_center_x", C.c_double),
+        ("blast_center_y", C.c_double),
+        ("blast_center_z", C.c_double),
+        ("blast_density_in", C.c_double),
+        ("blast_density_out", C.c_double),
+        ("blast_pressure_in", C.c_double),
+        ("blast_pressure_out", C.c_double),
+        ("blast_bx", C.c_double),
+        ("blast_by", C.c_double),
+        ("blast_bz", C.c_double),
+        ("kh_d_in", C.c_double),
+        ("kh_d_out", C.c_double),
+        ("kh_pressure", C.c_double),
+        ("kh_vflow_in", C.c_double),
+        ("kh_vflow_out", C.c_double),
+        ("kh_amplitude", C.c_double),
+        ("kh_w0", C.c_double),
+        ("kh_delta", C.c_double),
+        ("kh_bx", C.c_double),
+        ("kh_by", C.c_double),
+        ("kh_bz", C.c_double),
+        ("wave_amplitude", C.c_double),
+        ("wave_gamma0", C.c_double),
+        ("wave_lx", C.c_double),
+        ("wave_ly", C.c_double),
+        ("wave_lz", C.c_double),
+        ("implode_in", C.c_double * 8),
+        ("implode_out", C.c_double * 8),
+    ]

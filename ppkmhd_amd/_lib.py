@@ -1,0 +1,80 @@
+"""ctypes binding of libppkmhd_amd.so (the C ABI declared in include/ppkmhd_amd.h).
+
+The library is loaded from the package directory (built in-tree by ``python -m
+ppkmhd_amd.build`` or ``__graft_entry__.build()``).  There is no fallback: if the library is
+missing or a call fails, an exception is raised.
+"""
+import ctypes as C
+import os
+
+from . import _abi as A
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libppkmhd_amd.so")
+
+# every entry point of include/ppkmhd_amd.h: name -> (restype, argtypes)
+_P = C.POINTER(A.Params)
+_PR = C.POINTER(A.Problem)
+_H = C.c_void_p
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "ppk_mhd_params_size": (C.c_int, []),
+    "ppk_mhd_problem_size": (C.c_int, []),
+    "ppk_mhd_version": (C.c_char_p, []),
+    "ppk_mhd_device_count": (C.c_int, []),
+    "ppk_mhd_params_from_ini": (C.c_int, [C.c_char_p, _P, _PR, C.c_char_p, C.c_int]),
+    "ppk_mhd_params_from_ini_text": (C.c_int, [C.c_char_p, _P, _PR, C.c_char_p, C.c_int]),
+    "ppk_mhd_params_init": (C.c_int, [_P]),
+    "ppk_mhd_create": (C.c_int, [_P, C.POINTER(_H)]),
+    "ppk_mhd_destroy": (None, [_H]),
+    "ppk_mhd_last_error": (C.c_char_p, [_H]),
+    "ppk_mhd_init_problem": (C.c_int, [_H, _PR]),
+    "ppk_mhd_upload": (C.c_int, [_H, _DP, C.c_size_t]),
+    "ppk_mhd_download": (C.c_int, [_H, _DP, C.c_size_t]),
+    "ppk_mhd_device_state": (C.c_int, [_H, C.POINTER(_DP)]),
+    "ppk_mhd_stream": (C.c_int, [_H, C.POINTER(C.c_void_p)]),
+    "ppk_mhd_make_boundaries": (C.c_int, [_H]),
+    "ppk_mhd_compute_dt": (C.c_int, [_H, _DP]),
+    "ppk_mhd_step": (C.c_int, [_H, C.c_double, C.c_int]),
+    "ppk_mhd_run": (C.c_int, [_H, C.c_int, C.c_double, C.c_int]),
+    "ppk_mhd_get_time": (C.c_int, [_H, _DP, _DP, C.POINTER(C.c_int64)]),
+    "ppk_mhd_set_time": (C.c_int, [_H, C.c_double, C.c_int64]),
+    "ppk_mhd_finished": (C.c_int, [_H, C.POINTER(C.c_int)]),
+    "ppk_mhd_synchronize": (C.c_int, [_H]),
+    "ppk_mhd_divb_max": (C.c_int, [_H, _DP]),
+    "ppk_mhd_conserved_sums": (C.c_int, [_H, _DP]),
+}
+
+_lib = None
+
+
+class PPKError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"ppkmhd_amd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} not found: build it with `python -m ppkmhd_amd.build` "
+                "(the HIP kernels are required; there is no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(rc, handle=None):
+    if rc != A.PPK_OK:
+        msg = lib().ppk_mhd_last_error(handle)
+        raise PPKError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,479 @@
+// C ABI of the library: solver handles, state transfers and the time loop
+// (include/ppkmhd_amd.h).  Host orchestration only; kernels live in the .hip files.
+//
+// Time-loop state lives on the device (ctrl block: t, dt, CFL accumulator, iteration), so a
+// run of many steps is launched without any host synchronization; the host reads the block
+// back only when asked (get_time, finished, compute_dt).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "fused.hpp"
+#include "kernels.hpp"
+#include "ppkmhd_amd.h"
+
+namespace ppk {
+
+namespace {
+thread_local std::string g_thread_error;
+
+const KernelTable kFast = {fast::staged_boundaries, fast::staged_prim_dt, fast::staged_begin,
+                           fast::staged_set_dt,     fast::staged_end,     fast::staged_godunov};
+const KernelTable kStrict = {strict::staged_boundaries, strict::staged_prim_dt,
+                             strict::staged_begin,      strict::staged_set_dt,
+                             strict::staged_end,        strict::staged_godunov};
+}  // namespace
+
+void set_thread_error(const std::string &msg) { g_thread_error = msg; }
+const char *thread_error() { return g_thread_error.c_str(); }
+
+// diagnostics (diag.hip)
+hipError_t diag_divb(const DevParams &p, const double *U, double *out, hipStream_t s);
+hipError_t diag_sums(const DevParams &p, const double *U, double *partials, int nblocks,
+                     hipStream_t s);
+constexpr int kDiagBlocks = 512;
+
+}  // namespace ppk
+
+struct ppk_mhd_solver {
+  ppk_mhd_params p{};
+  ppk::DevParams dp{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double *U[2] = {nullptr, nullptr};
+  double *Q = nullptr;
+  double *ctrl = nullptr;       // device control block
+  double *ctrl_host = nullptr;  // pinned mirror
+  double *diag = nullptr;       // diagnostics scratch (device)
+  ppk::StagedScratch w;
+  ppk::FusedScratch fw;
+  bool fused = false;
+  const ppk::KernelTable *K = nullptr;
+  long long iter_host = 0;  // iteration count as far as the host knows (buffer parity)
+  std::string err;
+
+  double *cur() { return U[iter_host & 1]; }
+  double *nxt() { return U[(iter_host + 1) & 1]; }
+  long long cells() const { return (long long)dp.isize * dp.jsize * dp.ksize; }
+  size_t state_bytes() const { return (size_t)cells() * 8 * sizeof(double); }
+};
+
+namespace {
+
+int fail(ppk_mhd_t *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    ppk::set_thread_error(msg);
+  return code;
+}
+
+#define HIP_CHECK(h, expr)                                                               \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return fail((h), PPK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+  } while (0)
+
+int validate(const ppk_mhd_params *p, std::string &why) {
+  if (!p) { why = "params is NULL"; return PPK_ERR_INVALID_ARGUMENT; }
+  if (p->dim != 2 && p->dim != 3) { why = "dim must be 2 or 3"; return PPK_ERR_INVALID_ARGUMENT; }
+  if (p->ghost_width != 3) {
+    why = "ghost_width must be 3 for MHD (HydroParams.cpp:56-71)";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->nx < 3 || p->ny < 3 || (p->dim == 3 && p->nz < 3)) {
+    why = "nx, ny (and nz in 3D) must be >= ghost_width";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->dim == 2 && p->nz != 1) { why = "nz must be 1 in 2D"; return PPK_ERR_INVALID_ARGUMENT; }
+  long long cells = (long long)(p->nx + 6) * (p->ny + 6) * (p->dim == 3 ? p->nz + 6 : 1);
+  if (cells > (1LL << 31)) { why = "grid too large (more than 2^31 padded cells)"; return PPK_ERR_INVALID_ARGUMENT; }
+  if (p->riemann != PPK_RIEMANN_HLLD && p->riemann != PPK_RIEMANN_HLL &&
+      p->riemann != PPK_RIEMANN_LLF) {
+    why = "riemann must be hlld, hll or llf for MHD (RiemannSolvers_MHD.h:372-392)";
+    return PPK_ERR_UNSUPPORTED;
+  }
+  if (p->slope_type != 1.0 && p->slope_type != 2.0) {
+    why = "slope_type must be 1 (minmod) or 2 (MC)";
+    return PPK_ERR_UNSUPPORTED;
+  }
+  int nf = p->dim == 3 ? 6 : 4;
+  for (int f = 0; f < nf; ++f)
+    if (p->bc[f] != PPK_BC_DIRICHLET && p->bc[f] != PPK_BC_NEUMANN && p->bc[f] != PPK_BC_PERIODIC) {
+      why = "boundary types must be 1 (Dirichlet), 2 (Neumann) or 3 (periodic)";
+      return PPK_ERR_INVALID_ARGUMENT;
+    }
+  if (p->precision != PPK_FP64 && p->precision != PPK_FP64_STRICT && p->precision != PPK_FP32_BULK) {
+    why = "unknown precision";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->kernel_path < PPK_PATH_AUTO || p->kernel_path > PPK_PATH_FUSED) {
+    why = "unknown kernel_path";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (!(p->gamma0 > 1.0) || !(p->dx > 0) || !(p->dy > 0) || (p->dim == 3 && !(p->dz > 0))) {
+    why = "gamma0 must be > 1 and dx, dy, dz > 0 (call ppk_mhd_params_init)";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  return PPK_OK;
+}
+
+ppk::DevParams make_dev(const ppk_mhd_params &p) {
+  ppk::DevParams d{};
+  d.dim = p.dim;
+  d.nx = p.nx;
+  d.ny = p.ny;
+  d.nz = p.dim == 3 ? p.nz : 1;
+  d.gw = p.ghost_width;
+  d.isize = p.nx + 2 * d.gw;
+  d.jsize = p.ny + 2 * d.gw;
+  d.ksize = p.dim == 3 ? p.nz + 2 * d.gw : 1;
+  d.riemann = p.riemann;
+  for (int f = 0; f < 6; ++f) d.bc[f] = p.bc[f];
+  d.slope12 = (p.slope_type == 1.0 || p.slope_type == 2.0) ? 1 : 0;
+  d.plane = (long long)d.isize * d.jsize;
+  d.vstride = d.plane * d.ksize;
+  d.dx = p.dx;
+  d.dy = p.dy;
+  d.dz = p.dz;
+  d.gamma0 = p.gamma0;
+  d.slope_type = p.slope_type;
+  d.smallr = p.smallr;
+  d.smallc = p.smallc;
+  d.smallp = p.smallp;
+  d.cIso = p.cIso;
+  d.cfl = p.cfl;
+  return d;
+}
+
+int write_ctrl(ppk_mhd_t *h) {
+  HIP_CHECK(h, hipMemcpyAsync(h->ctrl, h->ctrl_host, sizeof(double) * ppk::CTRL_SLOTS,
+                              hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int read_ctrl(ppk_mhd_t *h) {
+  HIP_CHECK(h, hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(double) * ppk::CTRL_SLOTS,
+                              hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+// one time step from cur() into nxt() (godunov_unsplit_impl), optionally gated on finished()
+int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
+  const ppk::KernelTable &K = *h->K;
+  const ppk::DevParams &p = h->dp;
+  hipStream_t s = h->stream;
+  double *in = h->cur(), *out = h->nxt();
+  HIP_CHECK(h, K.begin(h->ctrl, check_finished, s));
+  if (h->fused) {
+    HIP_CHECK(h, ppk::fused_step(p, in, out, h->fw, h->ctrl, dt, dt_mode == PPK_DT_CFL,
+                                 h->p.precision, s));
+  } else {
+    HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s));
+    HIP_CHECK(h, K.prim_dt(p, in, h->Q, h->ctrl, dt_mode == PPK_DT_CFL, s));
+    HIP_CHECK(h, K.set_dt(h->ctrl, dt, dt_mode == PPK_DT_CFL, h->p.cfl, s));
+    HIP_CHECK(h, K.godunov(p, in, out, h->Q, h->w, h->ctrl, s));
+  }
+  HIP_CHECK(h, K.end(h->ctrl, s));
+  return PPK_OK;
+}
+
+void free_all(ppk_mhd_t *h) {
+  auto F = [](void *ptr) {
+    if (ptr) (void)hipFree(ptr);
+  };
+  F(h->U[0]);
+  F(h->U[1]);
+  F(h->Q);
+  F(h->ctrl);
+  F(h->diag);
+  F(h->w.faces);
+  F(h->w.edges);
+  F(h->w.flux);
+  F(h->w.emf);
+  F(h->w.elec);
+  F(h->w.dbf);
+  ppk::fused_free(h->fw);
+  if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ppk_mhd_params_size(void) { return (int)sizeof(ppk_mhd_params); }
+int ppk_mhd_problem_size(void) { return (int)sizeof(ppk_mhd_problem); }
+const char *ppk_mhd_version(void) { return "ppkmhd_amd 0.1 (gfx950)"; }
+
+int ppk_mhd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char *ppk_mhd_last_error(const ppk_mhd_t *h) {
+  return h ? h->err.c_str() : ppk::thread_error();
+}
+
+int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
+  if (!handle) return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "handle is NULL");
+  *handle = nullptr;
+  std::string why;
+  int rc = validate(params, why);
+  if (rc != PPK_OK) return fail(nullptr, rc, why);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, PPK_ERR_HIP, "no HIP device available");
+
+  ppk_mhd_t *h = new (std::nothrow) ppk_mhd_t();
+  if (!h) return fail(nullptr, PPK_ERR_OUT_OF_MEMORY, "host allocation failed");
+  h->p = *params;
+  h->dp = make_dev(*params);
+  h->K = params->precision == PPK_FP64_STRICT ? &ppk::kStrict : &ppk::kFast;
+  h->fused = ppk::fused_supported(h->dp, params->precision, params->kernel_path);
+  if (params->kernel_path == PPK_PATH_FUSED && !h->fused) {
+    delete h;
+    return fail(nullptr, PPK_ERR_UNSUPPORTED, "fused kernels do not support these parameters");
+  }
+  if (params->precision == PPK_FP32_BULK && !h->fused) {
+    delete h;
+    return fail(nullptr, PPK_ERR_UNSUPPORTED, "FP32 bulk arithmetic needs the fused kernels");
+  }
+  auto bail = [&](int code, const std::string &m) {
+    free_all(h);
+    delete h;
+    return fail(nullptr, code, m);
+  };
+  if (hipGetDevice(&h->device) != hipSuccess) return bail(PPK_ERR_HIP, "hipGetDevice failed");
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(PPK_ERR_HIP, "hipStreamCreateWithFlags failed");
+
+  const size_t n = (size_t)h->cells();
+  auto dalloc = [&](double **ptr, size_t count) {
+    return hipMalloc((void **)ptr, count * sizeof(double)) == hipSuccess;
+  };
+  bool ok = dalloc(&h->U[0], 8 * n) && dalloc(&h->U[1], 8 * n) &&
+            dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->diag, ppk::kDiagBlocks * 8 + 8);
+  if (ok && !h->fused) {
+    const int d = params->dim;
+    ok = dalloc(&h->Q, 8 * n) && dalloc(&h->w.faces, (size_t)2 * d * 8 * n) &&
+         dalloc(&h->w.edges, (size_t)4 * (d == 3 ? 3 : 1) * 8 * n) &&
+         dalloc(&h->w.flux, (size_t)d * 8 * n) && dalloc(&h->w.emf, (size_t)(d == 3 ? 3 : 1) * n);
+    if (ok && d == 3) ok = dalloc(&h->w.elec, 3 * n) && dalloc(&h->w.dbf, 6 * n);
+  }
+  if (ok && h->fused) ok = ppk::fused_alloc(h->dp, params->precision, h->fw) == hipSuccess;
+  if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
+  if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * ppk::CTRL_SLOTS) != hipSuccess)
+    return bail(PPK_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+  if (hipMemsetAsync(h->U[0], 0, 8 * n * sizeof(double), h->stream) != hipSuccess ||
+      hipMemsetAsync(h->U[1], 0, 8 * n * sizeof(double), h->stream) != hipSuccess)
+    return bail(PPK_ERR_HIP, "hipMemsetAsync failed");
+  std::fill(h->ctrl_host, h->ctrl_host + ppk::CTRL_SLOTS, 0.0);
+  // SolverBase::read_config: t = tCurrent (0), dt = tEnd (src/shared/SolverBase.cpp:119-121)
+  h->ctrl_host[ppk::CTRL_T] = 0.0;
+  h->ctrl_host[ppk::CTRL_DT] = params->tEnd;
+  h->ctrl_host[ppk::CTRL_ITER] = 0.0;
+  h->ctrl_host[ppk::CTRL_TEND] = params->tEnd;
+  h->ctrl_host[ppk::CTRL_NSTEPMAX] = (double)params->n_step_max;
+  h->ctrl_host[ppk::CTRL_ACTIVE] = 1.0;
+  if (write_ctrl(h) != PPK_OK) {
+    std::string m = h->err;
+    return bail(PPK_ERR_HIP, m);
+  }
+  *handle = h;
+  return PPK_OK;
+}
+
+void ppk_mhd_destroy(ppk_mhd_t *h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+}
+
+int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
+  if (!h || !problem) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind");
+  std::vector<double> host((size_t)h->cells() * 8);
+  ppk::init_problem_host(h->p, *problem, host.data());
+  h->iter_host = 0;
+  HIP_CHECK(h, hipMemcpyAsync(h->cur(), host.data(), h->state_bytes(), hipMemcpyHostToDevice,
+                              h->stream));
+  // SolverMHDMuscl ctor: boundaries, U2 = U, primitives, compute_dt (SolverMHDMuscl.h:389-402)
+  h->ctrl_host[ppk::CTRL_T] = 0.0;
+  h->ctrl_host[ppk::CTRL_ITER] = 0.0;
+  h->ctrl_host[ppk::CTRL_ACTIVE] = 1.0;
+  int rc = write_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  double dt = 0.0;
+  rc = ppk_mhd_compute_dt(h, &dt);
+  if (rc != PPK_OK) return rc;
+  HIP_CHECK(h, hipMemcpyAsync(h->nxt(), h->cur(), h->state_bytes(), hipMemcpyDeviceToDevice,
+                              h->stream));
+  h->ctrl_host[ppk::CTRL_DT] = (0.0 + dt > h->p.tEnd) ? h->p.tEnd - 0.0 : dt;
+  return write_ctrl(h);
+}
+
+int ppk_mhd_upload(ppk_mhd_t *h, const double *U, size_t count) {
+  if (!h || !U) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (count != (size_t)h->cells() * 8)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "count must be 8*isize*jsize*ksize");
+  HIP_CHECK(h, hipMemcpyAsync(h->cur(), U, h->state_bytes(), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int ppk_mhd_download(ppk_mhd_t *h, double *U, size_t count) {
+  if (!h || !U) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (count != (size_t)h->cells() * 8)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "count must be 8*isize*jsize*ksize");
+  HIP_CHECK(h, hipMemcpyAsync(U, h->cur(), h->state_bytes(), hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int ppk_mhd_device_state(ppk_mhd_t *h, double **dU) {
+  if (!h || !dU) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *dU = h->cur();
+  return PPK_OK;
+}
+
+int ppk_mhd_stream(ppk_mhd_t *h, void **stream) {
+  if (!h || !stream) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *stream = (void *)h->stream;
+  return PPK_OK;
+}
+
+int ppk_mhd_make_boundaries(ppk_mhd_t *h) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream));
+  return PPK_OK;
+}
+
+int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
+  if (!h || !dt) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  // the dt the next step would take: ghost fill, primitives, CFL max (no tEnd clamp)
+  HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream));
+  if (h->fused) {
+    HIP_CHECK(h, ppk::fused_cfl(h->dp, h->cur(), h->fw, h->ctrl, h->stream));
+  } else {
+    HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->ctrl, 1, h->stream));
+  }
+  int rc = read_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  *dt = h->p.cfl / h->ctrl_host[ppk::CTRL_INVDT];
+  return PPK_OK;
+}
+
+int ppk_mhd_step(ppk_mhd_t *h, double dt, int dt_mode) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (dt_mode != PPK_DT_FIXED && dt_mode != PPK_DT_CFL)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "dt_mode must be PPK_DT_FIXED or PPK_DT_CFL");
+  int rc = launch_step(h, dt, dt_mode, 0);
+  if (rc != PPK_OK) return rc;
+  h->iter_host += 1;
+  return PPK_OK;
+}
+
+int ppk_mhd_run(ppk_mhd_t *h, int nsteps, double dt, int dt_mode) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (nsteps < 0) return fail(h, PPK_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  if (dt_mode != PPK_DT_FIXED && dt_mode != PPK_DT_CFL)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "dt_mode must be PPK_DT_FIXED or PPK_DT_CFL");
+  // steps after finished() are no-ops on the device; the buffer parity is re-read at the end
+  long long start = h->iter_host;
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = launch_step(h, dt, dt_mode, 1);
+    if (rc != PPK_OK) return rc;
+    h->iter_host += 1;
+  }
+  int rc = read_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  h->iter_host = (long long)h->ctrl_host[ppk::CTRL_ITER];
+  (void)start;
+  return PPK_OK;
+}
+
+int ppk_mhd_get_time(ppk_mhd_t *h, double *t, double *dt, int64_t *iteration) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  int rc = read_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  if (t) *t = h->ctrl_host[ppk::CTRL_T];
+  if (dt) *dt = h->ctrl_host[ppk::CTRL_DT];
+  if (iteration) *iteration = (int64_t)h->ctrl_host[ppk::CTRL_ITER];
+  return PPK_OK;
+}
+
+int ppk_mhd_set_time(ppk_mhd_t *h, double t, int64_t iteration) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (iteration < 0) return fail(h, PPK_ERR_INVALID_ARGUMENT, "iteration must be >= 0");
+  int rc = read_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  // keep the current data where it is: swap the ping-pong buffers if the parity changes
+  if (((long long)iteration & 1) != (h->iter_host & 1)) std::swap(h->U[0], h->U[1]);
+  h->iter_host = iteration;
+  h->ctrl_host[ppk::CTRL_T] = t;
+  h->ctrl_host[ppk::CTRL_ITER] = (double)iteration;
+  return write_ctrl(h);
+}
+
+int ppk_mhd_finished(ppk_mhd_t *h, int *finished) {
+  if (!h || !finished) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = read_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  // SolverBase::finished, src/shared/SolverBase.cpp:195-201
+  *finished = (h->ctrl_host[ppk::CTRL_T] >= (h->p.tEnd - 1e-14) ||
+               h->ctrl_host[ppk::CTRL_ITER] >= (double)h->p.n_step_max)
+                  ? 1
+                  : 0;
+  return PPK_OK;
+}
+
+int ppk_mhd_synchronize(ppk_mhd_t *h) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max) {
+  if (!h || !divb_max) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = ppk_mhd_make_boundaries(h);
+  if (rc != PPK_OK) return rc;
+  HIP_CHECK(h, ppk::diag_divb(h->dp, h->cur(), h->diag, h->stream));
+  HIP_CHECK(h, hipMemcpyAsync(h->ctrl_host + ppk::CTRL_SLOTS - 1, h->diag, sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  *divb_max = h->ctrl_host[ppk::CTRL_SLOTS - 1];
+  return PPK_OK;
+}
+
+int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]) {
+  if (!h || !sums) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  HIP_CHECK(h, ppk::diag_sums(h->dp, h->cur(), h->diag, ppk::kDiagBlocks, h->stream));
+  std::vector<double> part((size_t)ppk::kDiagBlocks * 8);
+  HIP_CHECK(h, hipMemcpyAsync(part.data(), h->diag, part.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  for (int v = 0; v < 8; ++v) {
+    double s = 0.0;
+    for (int b = 0; b < ppk::kDiagBlocks; ++b) s += part[(size_t)b * 8 + v];
+    sums[v] = s;
+  }
+  return PPK_OK;
+}
+
+}  // extern "C"

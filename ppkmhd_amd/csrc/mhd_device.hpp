@@ -1,0 +1,534 @@
+// Device-side MHD building blocks of the MUSCL-Hancock + HLLD + CT step (gfx950).
+//
+// Every function is a __device__ inline on values held in registers (no local arrays indexed
+// at run time, so nothing spills to scratch).  Arithmetic is written in the operand order of
+// the reference so that a build with -ffp-contract=off reproduces the reference's double
+// results bit for bit; the default build lets the compiler contract a*b+c into FMA.
+//
+// Variable order ID, IP, IU, IV, IW, IA, IB, IC: reference src/shared/enums.h:17-34.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ppkmhd_amd.h"
+
+namespace ppk {
+
+enum { ID = 0, IP = 1, IU = 2, IV = 3, IW = 4, IA = 5, IB = 6, IC = 7 };
+enum { IX = 0, IY = 1, IZ = 2 };
+enum { IRT = 0, IRB = 1, ILT = 2, ILB = 3 };
+enum { EMFX = 0, EMFY = 1, EMFZ = 2 };
+enum { I_EMFZ = 0, I_EMFY = 1, I_EMFX = 2 };
+
+// POD copy of the solver parameters passed by value to every kernel (as the reference passes
+// HydroParams by value into every functor).
+struct DevParams {
+  int dim, nx, ny, nz, gw;
+  int isize, jsize, ksize;
+  int riemann;
+  int bc[6];
+  int slope12;          // slope_type is 1 or 2 (hydro slopes active)
+  long long plane;      // isize * jsize
+  long long vstride;    // isize * jsize * ksize (distance between variables)
+  double dx, dy, dz;
+  double gamma0, slope_type, smallr, smallc, smallp, cIso, cfl;
+};
+
+template <class R>
+struct St {
+  R v[8];
+  __device__ __forceinline__ R &operator[](int i) { return v[i]; }
+  __device__ __forceinline__ const R &operator[](int i) const { return v[i]; }
+};
+
+template <class R>
+__device__ __forceinline__ R dmax(R a, R b) { return fmax(a, b); }
+template <class R>
+__device__ __forceinline__ R dmin(R a, R b) { return fmin(a, b); }
+
+// compare-and-select maxima used by the 2D magnetic Riemann solver (FMAX4/FMIN4/FMAX5,
+// reference src/shared/mhd_utils.h:36-77)
+template <class R>
+__device__ __forceinline__ R max4(R a0, R a1, R a2, R a3) {
+  R r = a0;
+  r = (a1 > r) ? a1 : r;
+  r = (a2 > r) ? a2 : r;
+  r = (a3 > r) ? a3 : r;
+  return r;
+}
+template <class R>
+__device__ __forceinline__ R min4(R a0, R a1, R a2, R a3) {
+  R r = a0;
+  r = (a1 < r) ? a1 : r;
+  r = (a2 < r) ? a2 : r;
+  r = (a3 < r) ? a3 : r;
+  return r;
+}
+template <class R>
+__device__ __forceinline__ R max5(R a0, R a1, R a2, R a3, R a4) {
+  R r = max4(a0, a1, a2, a3);
+  r = (a4 > r) ? a4 : r;
+  return r;
+}
+
+// fast magnetosonic speed along dir, reference src/shared/mhd_utils.h:89-117
+template <class R>
+__device__ __forceinline__ R speed_fast(const St<R> &q, int dir, R gamma0) {
+  R d = q[ID], p = q[IP], a = q[IA], b = q[IB], c = q[IC];
+  R b2 = a * a + b * b + c * c;
+  R c2 = gamma0 * p / d;
+  R d2 = R(0.5) * (b2 / d + c2);
+  R bn = dir == IX ? a : (dir == IY ? b : c);
+  return sqrt(d2 + sqrt(d2 * d2 - c2 * bn * bn / d));
+}
+
+// per-cell CFL term sum_d (c_f,d + |v_d|)/dx_d, reference src/shared/mhd_utils.h:319-366 and
+// src/muscl/MHDRunFunctors{2D,3D}.h:73/76
+__device__ __forceinline__ double inv_dt_cell(const St<double> &q, const DevParams &p) {
+  const double gamma0 = p.gamma0;
+  double d = q[ID], pp = q[IP], a = q[IA], b = q[IB], c = q[IC];
+  double b2 = a * a + b * b + c * c;
+  double c2 = gamma0 * pp / d;
+  double d2 = 0.5 * (b2 / d + c2);
+  double cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
+  double vx = cf + fabs(q[IU]);
+  cf = sqrt(d2 + sqrt(d2 * d2 - c2 * b * b / d));
+  double vy = cf + fabs(q[IV]);
+  if (p.dim == 3) {
+    cf = sqrt(d2 + sqrt(d2 * d2 - c2 * c * c / d));
+    double vz = cf + fabs(q[IW]);
+    return vx / p.dx + vy / p.dy + vz / p.dz;
+  }
+  return vx / p.dx + vy / p.dy;
+}
+
+// conservative -> primitive with the face-centred field averaged to the cell centre,
+// reference src/muscl/MHDBaseFunctor3D.h:192-242
+template <class R>
+__device__ __forceinline__ void constoprim(const St<R> &u, R magx, R magy, R magz, St<R> &q,
+                                           const DevParams &p) {
+  q[ID] = dmax(u[ID], R(p.smallr));
+  q[IU] = u[IU] / q[ID];
+  q[IV] = u[IV] / q[ID];
+  q[IW] = u[IW] / q[ID];
+  q[IA] = R(0.5) * (u[IA] + magx);
+  q[IB] = R(0.5) * (u[IB] + magy);
+  q[IC] = R(0.5) * (u[IC] + magz);
+  R eken = R(0.5) * (q[IU] * q[IU] + q[IV] * q[IV] + q[IW] * q[IW]);
+  R emag = R(0.5) * (q[IA] * q[IA] + q[IB] * q[IB] + q[IC] * q[IC]);
+  if (p.cIso > 0) {
+    q[IP] = q[ID] * R(p.cIso) * R(p.cIso);
+  } else {
+    R eint = (u[IP] - emag) / q[ID] - eken;
+    q[IP] = dmax((R(p.gamma0) - R(1.0)) * q[ID] * eint, q[ID] * R(p.smallp));
+  }
+}
+
+// minmod (slope_type 1) / MC (slope_type 2) limited slope,
+// reference src/muscl/MHDBaseFunctor2D.h:308-327
+template <class R>
+__device__ __forceinline__ R limited_slope(R st, R q, R qplus, R qminus) {
+  R dlft = st * (q - qminus);
+  R drgt = st * (qplus - q);
+  R dcen = R(0.5) * (qplus - qminus);
+  R dsgn = (dcen >= R(0)) ? R(1) : R(-1);
+  R slop = dmin(fabs(dlft), fabs(drgt));
+  R dlim = slop;
+  if ((dlft * drgt) <= R(0)) dlim = R(0);
+  return dsgn * dmin(dlim, fabs(dcen));
+}
+
+// ------------------------------------------------------------------------------------------
+// 1D Riemann solvers, reference src/shared/RiemannSolvers_MHD.h.  Inputs are rotated so that
+// IU / IA are normal to the face; the flux keeps that rotation.
+// ------------------------------------------------------------------------------------------
+// HLLD, Miyoshi & Kusano (2005) as written in the reference (:133-367)
+template <class R>
+__device__ __forceinline__ void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
+                                             const DevParams &par) {
+  const R gamma0 = R(par.gamma0);
+  const R entho = R(1.0) / (gamma0 - R(1.0));
+
+  R a = R(0.5) * (qleft[IA] + qright[IA]);
+  R sgnm = (a >= R(0)) ? R(1) : R(-1);
+  qleft[IA] = a;
+  qright[IA] = a;
+  if (par.cIso > 0) {
+    qleft[IP] = qleft[ID] * R(par.cIso) * R(par.cIso);
+    qright[IP] = qright[ID] * R(par.cIso) * R(par.cIso);
+  }
+
+  R rl = qleft[ID], pl = qleft[IP], ul = qleft[IU], vl = qleft[IV], wl = qleft[IW];
+  R bl = qleft[IB], cl = qleft[IC];
+  R ecinl = R(0.5) * (ul * ul + vl * vl + wl * wl) * rl;
+  R emagl = R(0.5) * (a * a + bl * bl + cl * cl);
+  R etotl = pl * entho + ecinl + emagl;
+  R ptotl = pl + emagl;
+  R vdotbl = ul * a + vl * bl + wl * cl;
+
+  R rr = qright[ID], pr = qright[IP], ur = qright[IU], vr = qright[IV], wr = qright[IW];
+  R br = qright[IB], cr = qright[IC];
+  R ecinr = R(0.5) * (ur * ur + vr * vr + wr * wr) * rr;
+  R emagr = R(0.5) * (a * a + br * br + cr * cr);
+  R etotr = pr * entho + ecinr + emagr;
+  R ptotr = pr + emagr;
+  R vdotbr = ur * a + vr * br + wr * cr;
+
+  R cfastl = speed_fast(qleft, IX, gamma0);
+  R cfastr = speed_fast(qright, IX, gamma0);
+
+  R sl = dmin(ul, ur) - dmax(cfastl, cfastr);
+  R sr = dmax(ul, ur) + dmax(cfastl, cfastr);
+
+  R rcl = rl * (ul - sl);
+  R rcr = rr * (sr - ur);
+
+  R ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
+  R ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
+
+  R rstarl = rl * (sl - ul) / (sl - ustar);
+  R estar = rl * (sl - ul) * (sl - ustar) - a * a;
+  R el = rl * (sl - ul) * (sl - ul) - a * a;
+  R vstarl, wstarl, bstarl, cstarl;
+  if (a * a > R(0) && fabs(estar / (a * a) - R(1)) <= R(1e-8)) {
+    vstarl = vl;
+    bstarl = bl;
+    wstarl = wl;
+    cstarl = cl;
+  } else {
+    vstarl = vl - a * bl * (ustar - ul) / estar;
+    bstarl = bl * el / estar;
+    wstarl = wl - a * cl * (ustar - ul) / estar;
+    cstarl = cl * el / estar;
+  }
+  R vdotbstarl = ustar * a + vstarl * bstarl + wstarl * cstarl;
+  R etotstarl = ((sl - ul) * etotl - ptotl * ul + ptotstar * ustar + a * (vdotbl - vdotbstarl)) /
+                (sl - ustar);
+  R sqrrstarl = sqrt(rstarl);
+  R calfvenl = fabs(a) / sqrrstarl;
+  R sal = ustar - calfvenl;
+
+  R rstarr = rr * (sr - ur) / (sr - ustar);
+  estar = rr * (sr - ur) * (sr - ustar) - a * a;
+  R er = rr * (sr - ur) * (sr - ur) - a * a;
+  R vstarr, wstarr, bstarr, cstarr;
+  if (a * a > R(0) && fabs(estar / (a * a) - R(1)) <= R(1e-8)) {
+    vstarr = vr;
+    bstarr = br;
+    wstarr = wr;
+    cstarr = cr;
+  } else {
+    vstarr = vr - a * br * (ustar - ur) / estar;
+    bstarr = br * er / estar;
+    wstarr = wr - a * cr * (ustar - ur) / estar;
+    cstarr = cr * er / estar;
+  }
+  R vdotbstarr = ustar * a + vstarr * bstarr + wstarr * cstarr;
+  R etotstarr = ((sr - ur) * etotr - ptotr * ur + ptotstar * ustar + a * (vdotbr - vdotbstarr)) /
+                (sr - ustar);
+  R sqrrstarr = sqrt(rstarr);
+  R calfvenr = fabs(a) / sqrrstarr;
+  R sar = ustar + calfvenr;
+
+  R vstarstar =
+      (sqrrstarl * vstarl + sqrrstarr * vstarr + sgnm * (bstarr - bstarl)) / (sqrrstarl + sqrrstarr);
+  R wstarstar =
+      (sqrrstarl * wstarl + sqrrstarr * wstarr + sgnm * (cstarr - cstarl)) / (sqrrstarl + sqrrstarr);
+  R bstarstar =
+      (sqrrstarl * bstarr + sqrrstarr * bstarl + sgnm * sqrrstarl * sqrrstarr * (vstarr - vstarl)) /
+      (sqrrstarl + sqrrstarr);
+  R cstarstar =
+      (sqrrstarl * cstarr + sqrrstarr * cstarl + sgnm * sqrrstarl * sqrrstarr * (wstarr - wstarl)) /
+      (sqrrstarl + sqrrstarr);
+  R vdotbstarstar = ustar * a + vstarstar * bstarstar + wstarstar * cstarstar;
+  R etotstarstarl = etotstarl - sgnm * sqrrstarl * (vdotbstarl - vdotbstarstar);
+  R etotstarstarr = etotstarr + sgnm * sqrrstarr * (vdotbstarr - vdotbstarstar);
+
+  R ro, uo, vo, wo, bo, co, ptoto, etoto, vdotbo;
+  if (sl > R(0)) {
+    ro = rl; uo = ul; vo = vl; wo = wl; bo = bl; co = cl;
+    ptoto = ptotl; etoto = etotl; vdotbo = vdotbl;
+  } else if (sal > R(0)) {
+    ro = rstarl; uo = ustar; vo = vstarl; wo = wstarl; bo = bstarl; co = cstarl;
+    ptoto = ptotstar; etoto = etotstarl; vdotbo = vdotbstarl;
+  } else if (ustar > R(0)) {
+    ro = rstarl; uo = ustar; vo = vstarstar; wo = wstarstar; bo = bstarstar; co = cstarstar;
+    ptoto = ptotstar; etoto = etotstarstarl; vdotbo = vdotbstarstar;
+  } else if (sar > R(0)) {
+    ro = rstarr; uo = ustar; vo = vstarstar; wo = wstarstar; bo = bstarstar; co = cstarstar;
+    ptoto = ptotstar; etoto = etotstarstarr; vdotbo = vdotbstarstar;
+  } else if (sr > R(0)) {
+    ro = rstarr; uo = ustar; vo = vstarr; wo = wstarr; bo = bstarr; co = cstarr;
+    ptoto = ptotstar; etoto = etotstarr; vdotbo = vdotbstarr;
+  } else {
+    ro = rr; uo = ur; vo = vr; wo = wr; bo = br; co = cr;
+    ptoto = ptotr; etoto = etotr; vdotbo = vdotbr;
+  }
+
+  flux[ID] = ro * uo;
+  flux[IP] = (etoto + ptoto) * uo - a * vdotbo;
+  flux[IU] = ro * uo * uo - a * a + ptoto;
+  flux[IV] = ro * uo * vo - a * bo;
+  flux[IW] = ro * uo * wo - a * co;
+  flux[IA] = R(0);
+  flux[IB] = bo * uo - a * vo;
+  flux[IC] = co * uo - a * wo;
+}
+
+// ideal MHD flux and conservative state of a primitive state, reference
+// src/shared/mhd_utils.h:173-231 (used by HLL and LLF)
+template <class R>
+__device__ __forceinline__ void mhd_flux(const St<R> &q, St<R> &cvar, St<R> &ff,
+                                         const DevParams &par) {
+  R gamma0 = R(par.gamma0);
+  R p = par.cIso > 0 ? q[ID] * R(par.cIso) * R(par.cIso) : q[IP];
+  R entho = R(1.0) / (gamma0 - R(1.0));
+  R d = q[ID], u = q[IU], v = q[IV], w = q[IW], a = q[IA], b = q[IB], c = q[IC];
+  R ecin = R(0.5) * (u * u + v * v + w * w) * d;
+  R emag = R(0.5) * (a * a + b * b + c * c);
+  R etot = p * entho + ecin + emag;
+  R ptot = p + emag;
+  cvar[ID] = d; cvar[IP] = etot; cvar[IU] = d * u; cvar[IV] = d * v; cvar[IW] = d * w;
+  cvar[IA] = a; cvar[IB] = b; cvar[IC] = c;
+  ff[ID] = d * u;
+  ff[IP] = (etot + ptot) * u - a * (a * u + b * v + c * w);
+  ff[IU] = d * u * u - a * a + ptot;
+  ff[IV] = d * u * v - a * b;
+  ff[IW] = d * u * w - a * c;
+  ff[IA] = R(0);
+  ff[IB] = b * u - a * v;
+  ff[IC] = c * u - a * w;
+}
+
+// HLL, reference src/shared/RiemannSolvers_MHD.h:27-67
+template <class R>
+__device__ __forceinline__ void riemann_hll(St<R> &ql, St<R> &qr, St<R> &flux,
+                                            const DevParams &par) {
+  R bx_mean = R(0.5) * (ql[IA] + qr[IA]);
+  ql[IA] = bx_mean;
+  qr[IA] = bx_mean;
+  St<R> ul, fl, ur, fr;
+  mhd_flux(ql, ul, fl, par);
+  mhd_flux(qr, ur, fr, par);
+  R cfl = speed_fast(ql, IX, R(par.gamma0));
+  R cfr = speed_fast(qr, IX, R(par.gamma0));
+  R sl = dmin(dmin(ql[IU], qr[IU]) - dmax(cfl, cfr), R(0));
+  R sr = dmax(dmax(ql[IU], qr[IU]) + dmax(cfl, cfr), R(0));
+#pragma unroll
+  for (int v = 0; v < 8; ++v)
+    flux[v] = (sr * fl[v] - sl * fr[v] + sr * sl * (ur[v] - ul[v])) / (sr - sl);
+}
+
+// LLF, reference src/shared/RiemannSolvers_MHD.h:81-111
+template <class R>
+__device__ __forceinline__ void riemann_llf(St<R> &ql, St<R> &qr, St<R> &flux,
+                                            const DevParams &par) {
+  R bx_mean = R(0.5) * (ql[IA] + qr[IA]);
+  ql[IA] = bx_mean;
+  qr[IA] = bx_mean;
+  St<R> ul, fl, ur, fr;
+  mhd_flux(ql, ul, fl, par);
+  mhd_flux(qr, ur, fr, par);
+#pragma unroll
+  for (int v = 0; v < 8; ++v) flux[v] = (fl[v] + fr[v]) / R(2);
+  auto info = [&](const St<R> &q) {
+    R d = q[ID], pp = q[IP], a = q[IA], b = q[IB], c = q[IC];
+    R b2 = a * a + b * b + c * c;
+    R c2 = R(par.gamma0) * pp / d;
+    R d2 = R(0.5) * (b2 / d + c2);
+    R cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
+    return cf + fabs(q[IU]);
+  };
+  R vel_info = dmax(info(ql), info(qr));
+#pragma unroll
+  for (int v = 0; v < 8; ++v) flux[v] = flux[v] - vel_info * (ur[v] - ul[v]) / R(2);
+}
+
+template <class R>
+__device__ __forceinline__ void riemann_mhd(St<R> &ql, St<R> &qr, St<R> &flux,
+                                            const DevParams &p) {
+  if (p.riemann == PPK_RIEMANN_HLLD)
+    riemann_hlld(ql, qr, flux, p);
+  else if (p.riemann == PPK_RIEMANN_HLL)
+    riemann_hll(ql, qr, flux, p);
+  else
+    riemann_llf(ql, qr, flux, p);
+}
+
+// ------------------------------------------------------------------------------------------
+// 2D magnetic Riemann solver (EMF at a cell edge), reference src/shared/RiemannSolvers_MHD.h
+// ------------------------------------------------------------------------------------------
+// mag_riemann2d_hlld (:398-630), with the reference's branch-free 0/1 blend (:569-596): the
+// region masks come from the sign bit (a -0 speed selects the "negative" side).
+template <class R>
+__device__ __forceinline__ R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
+                                               const St<R> &qLR, const St<R> &qRR, R ELL, R ERL,
+                                               R ELR, R ERR, const DevParams &p) {
+  const R gamma0 = R(p.gamma0);
+  R rLL = qLL[ID], pLL = qLL[IP], uLL = qLL[IU], vLL = qLL[IV], aLL = qLL[IA], bLL = qLL[IB], cLL = qLL[IC];
+  R rLR = qLR[ID], pLR = qLR[IP], uLR = qLR[IU], vLR = qLR[IV], aLR = qLR[IA], bLR = qLR[IB], cLR = qLR[IC];
+  R rRL = qRL[ID], pRL = qRL[IP], uRL = qRL[IU], vRL = qRL[IV], aRL = qRL[IA], bRL = qRL[IB], cRL = qRL[IC];
+  R rRR = qRR[ID], pRR = qRR[IP], uRR = qRR[IU], vRR = qRR[IV], aRR = qRR[IA], bRR = qRR[IB], cRR = qRR[IC];
+
+  R cFastLLx = speed_fast(qLL, IX, gamma0);
+  R cFastLRx = speed_fast(qLR, IX, gamma0);
+  R cFastRLx = speed_fast(qRL, IX, gamma0);
+  R cFastRRx = speed_fast(qRR, IX, gamma0);
+  R cFastLLy = speed_fast(qLL, IY, gamma0);
+  R cFastLRy = speed_fast(qLR, IY, gamma0);
+  R cFastRLy = speed_fast(qRL, IY, gamma0);
+  R cFastRRy = speed_fast(qRR, IY, gamma0);
+
+  R SL = min4(uLL, uLR, uRL, uRR) - max4(cFastLLx, cFastLRx, cFastRLx, cFastRRx);
+  R SR = max4(uLL, uLR, uRL, uRR) + max4(cFastLLx, cFastLRx, cFastRLx, cFastRRx);
+  R SB = min4(vLL, vLR, vRL, vRR) - max4(cFastLLy, cFastLRy, cFastRLy, cFastRRy);
+  R ST = max4(vLL, vLR, vRL, vRR) + max4(cFastLLy, cFastLRy, cFastRLy, cFastRRy);
+
+  R PtotLL = pLL + R(0.5) * (aLL * aLL + bLL * bLL + cLL * cLL);
+  R PtotLR = pLR + R(0.5) * (aLR * aLR + bLR * bLR + cLR * cLR);
+  R PtotRL = pRL + R(0.5) * (aRL * aRL + bRL * bRL + cRL * cRL);
+  R PtotRR = pRR + R(0.5) * (aRR * aRR + bRR * bRR + cRR * cRR);
+
+  R rcLLx = rLL * (uLL - SL);
+  R rcRLx = rRL * (SR - uRL);
+  R rcLRx = rLR * (uLR - SL);
+  R rcRRx = rRR * (SR - uRR);
+  R rcLLy = rLL * (vLL - SB);
+  R rcLRy = rLR * (ST - vLR);
+  R rcRLy = rRL * (vRL - SB);
+  R rcRRy = rRR * (ST - vRR);
+
+  R ustar = (rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR +
+             (PtotLL - PtotRL + PtotLR - PtotRR)) /
+            (rcLLx + rcLRx + rcRLx + rcRRx);
+  R vstar = (rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR +
+             (PtotLL - PtotLR + PtotRL - PtotRR)) /
+            (rcLLy + rcLRy + rcRLy + rcRRy);
+
+  R rstarLLx = rLL * (SL - uLL) / (SL - ustar);
+  R BstarLL = bLL * (SL - uLL) / (SL - ustar);
+  R rstarLLy = rLL * (SB - vLL) / (SB - vstar);
+  R AstarLL = aLL * (SB - vLL) / (SB - vstar);
+  R rstarLL = rLL * (SL - uLL) / (SL - ustar) * (SB - vLL) / (SB - vstar);
+  R EstarLLx = ustar * BstarLL - vLL * aLL;
+  R EstarLLy = uLL * bLL - vstar * AstarLL;
+  R EstarLL = ustar * BstarLL - vstar * AstarLL;
+
+  R rstarLRx = rLR * (SL - uLR) / (SL - ustar);
+  R BstarLR = bLR * (SL - uLR) / (SL - ustar);
+  R rstarLRy = rLR * (ST - vLR) / (ST - vstar);
+  R AstarLR = aLR * (ST - vLR) / (ST - vstar);
+  R rstarLR = rLR * (SL - uLR) / (SL - ustar) * (ST - vLR) / (ST - vstar);
+  R EstarLRx = ustar * BstarLR - vLR * aLR;
+  R EstarLRy = uLR * bLR - vstar * AstarLR;
+  R EstarLR = ustar * BstarLR - vstar * AstarLR;
+
+  R rstarRLx = rRL * (SR - uRL) / (SR - ustar);
+  R BstarRL = bRL * (SR - uRL) / (SR - ustar);
+  R rstarRLy = rRL * (SB - vRL) / (SB - vstar);
+  R AstarRL = aRL * (SB - vRL) / (SB - vstar);
+  R rstarRL = rRL * (SR - uRL) / (SR - ustar) * (SB - vRL) / (SB - vstar);
+  R EstarRLx = ustar * BstarRL - vRL * aRL;
+  R EstarRLy = uRL * bRL - vstar * AstarRL;
+  R EstarRL = ustar * BstarRL - vstar * AstarRL;
+
+  R rstarRRx = rRR * (SR - uRR) / (SR - ustar);
+  R BstarRR = bRR * (SR - uRR) / (SR - ustar);
+  R rstarRRy = rRR * (ST - vRR) / (ST - vstar);
+  R AstarRR = aRR * (ST - vRR) / (ST - vstar);
+  R rstarRR = rRR * (SR - uRR) / (SR - ustar) * (ST - vRR) / (ST - vstar);
+  R EstarRRx = ustar * BstarRR - vRR * aRR;
+  R EstarRRy = uRR * bRR - vstar * AstarRR;
+  R EstarRR = ustar * BstarRR - vstar * AstarRR;
+
+  const R smallc = R(p.smallc);
+  R calfvenL = max5(fabs(aLR) / sqrt(rstarLRx), fabs(AstarLR) / sqrt(rstarLR),
+                    fabs(aLL) / sqrt(rstarLLx), fabs(AstarLL) / sqrt(rstarLL), smallc);
+  R calfvenR = max5(fabs(aRR) / sqrt(rstarRRx), fabs(AstarRR) / sqrt(rstarRR),
+                    fabs(aRL) / sqrt(rstarRLx), fabs(AstarRL) / sqrt(rstarRL), smallc);
+  R calfvenB = max5(fabs(bLL) / sqrt(rstarLLy), fabs(BstarLL) / sqrt(rstarLL),
+                    fabs(bRL) / sqrt(rstarRLy), fabs(BstarRL) / sqrt(rstarRL), smallc);
+  R calfvenT = max5(fabs(bLR) / sqrt(rstarLRy), fabs(BstarLR) / sqrt(rstarLR),
+                    fabs(bRR) / sqrt(rstarRRy), fabs(BstarRR) / sqrt(rstarRR), smallc);
+
+  R SAL = dmin(ustar - calfvenL, R(0));
+  R SAR = dmax(ustar + calfvenR, R(0));
+  R SAB = dmin(vstar - calfvenB, R(0));
+  R SAT = dmax(vstar + calfvenT, R(0));
+
+  R AstarT = (SAR * AstarRR - SAL * AstarLR) / (SAR - SAL);
+  R AstarB = (SAR * AstarRL - SAL * AstarLL) / (SAR - SAL);
+  R BstarR = (SAT * BstarRR - SAB * BstarRL) / (SAT - SAB);
+  R BstarL = (SAT * BstarLR - SAB * BstarLL) / (SAT - SAB);
+
+  // 0/1 region masks from the sign bit (copysign(1, S) > 0)
+  const R SB_pos = signbit(SB) ? R(0) : R(1), SB_neg = R(1) - SB_pos;
+  const R ST_pos = signbit(ST) ? R(0) : R(1), ST_neg = R(1) - ST_pos;
+  const R SL_pos = signbit(SL) ? R(0) : R(1), SL_neg = R(1) - SL_pos;
+  const R SR_pos = signbit(SR) ? R(0) : R(1), SR_neg = R(1) - SR_pos;
+
+  R E = R(0), tmpE;
+  tmpE = (SAL * SAB * EstarRR - SAL * SAT * EstarRL - SAR * SAB * EstarLR + SAR * SAT * EstarLL) /
+             (SAR - SAL) / (SAT - SAB) -
+         SAT * SAB / (SAT - SAB) * (AstarT - AstarB) + SAR * SAL / (SAR - SAL) * (BstarR - BstarL);
+  E += (SB_neg * ST_pos * SL_neg * SR_pos) * tmpE;
+
+  tmpE = (SAR * EstarLLx - SAL * EstarRLx + SAR * SAL * (bRL - bLL)) / (SAR - SAL);
+  tmpE = SL_pos * ELL + SL_neg * SR_neg * ERL + SL_neg * SR_pos * tmpE;
+  E += SB_pos * tmpE;
+
+  tmpE = (SAR * EstarLRx - SAL * EstarRRx + SAR * SAL * (bRR - bLR)) / (SAR - SAL);
+  tmpE = SL_pos * ELR + SL_neg * SR_neg * ERR + SL_neg * SR_pos * tmpE;
+  E += (SB_neg * ST_neg) * tmpE;
+
+  tmpE = (SAT * EstarLLy - SAB * EstarLRy - SAT * SAB * (aLR - aLL)) / (SAT - SAB);
+  E += (SB_neg * ST_pos * SL_pos) * tmpE;
+
+  tmpE = (SAT * EstarRLy - SAB * EstarRRy - SAT * SAB * (aRR - aRL)) / (SAT - SAB);
+  E += (SB_neg * ST_pos * SL_neg * SR_neg) * tmpE;
+  return E;
+}
+
+// compute_emf<emfDir>: map the 4 edge states (RT, RB, LT, LB) to LL/RL/LR/RR with the
+// parallel field averaged, then the 2D HLLD solver, reference
+// src/shared/RiemannSolvers_MHD.h:651-874.  Template parameter = (iu, iv, iw, ia, ib, ic)
+// permutation: EMFZ (u,v,w), EMFY (w,u,v), EMFX (v,w,u).
+template <int EDIR, class R>
+__device__ __forceinline__ R compute_emf(const St<R> &qRT, const St<R> &qRB, const St<R> &qLT,
+                                         const St<R> &qLB, const DevParams &p) {
+  constexpr int iu = EDIR == EMFZ ? IU : (EDIR == EMFY ? IW : IV);
+  constexpr int iv = EDIR == EMFZ ? IV : (EDIR == EMFY ? IU : IW);
+  constexpr int iw = EDIR == EMFZ ? IW : (EDIR == EMFY ? IV : IU);
+  constexpr int ia = EDIR == EMFZ ? IA : (EDIR == EMFY ? IC : IB);
+  constexpr int ib = EDIR == EMFZ ? IB : (EDIR == EMFY ? IA : IC);
+  constexpr int ic = EDIR == EMFZ ? IC : (EDIR == EMFY ? IB : IA);
+  St<R> qLL, qRL, qLR, qRR;
+  qLL[ID] = qRT[ID]; qRL[ID] = qLT[ID]; qLR[ID] = qRB[ID]; qRR[ID] = qLB[ID];
+  if (p.cIso > 0) {
+    qLL[IP] = qLL[ID] * R(p.cIso) * R(p.cIso);
+    qRL[IP] = qRL[ID] * R(p.cIso) * R(p.cIso);
+    qLR[IP] = qLR[ID] * R(p.cIso) * R(p.cIso);
+    qRR[IP] = qRR[ID] * R(p.cIso) * R(p.cIso);
+  } else {
+    qLL[IP] = qRT[IP]; qRL[IP] = qLT[IP]; qLR[IP] = qRB[IP]; qRR[IP] = qLB[IP];
+  }
+  qLL[IU] = qRT[iu]; qRL[IU] = qLT[iu]; qLR[IU] = qRB[iu]; qRR[IU] = qLB[iu];
+  qLL[IV] = qRT[iv]; qRL[IV] = qLT[iv]; qLR[IV] = qRB[iv]; qRR[IV] = qLB[iv];
+  const R ax_top = R(0.5) * (qRT[ia] + qLT[ia]);
+  const R ax_bot = R(0.5) * (qRB[ia] + qLB[ia]);
+  qLL[IA] = ax_top; qRL[IA] = ax_top; qLR[IA] = ax_bot; qRR[IA] = ax_bot;
+  const R by_right = R(0.5) * (qRT[ib] + qRB[ib]);
+  const R by_left = R(0.5) * (qLT[ib] + qLB[ib]);
+  qLL[IB] = by_right; qRL[IB] = by_left; qLR[IB] = by_right; qRR[IB] = by_left;
+  qLL[IW] = qRT[iw]; qRL[IW] = qLT[iw]; qLR[IW] = qRB[iw]; qRR[IW] = qLB[iw];
+  qLL[IC] = qRT[ic]; qRL[IC] = qLT[ic]; qLR[IC] = qRB[ic]; qRR[IC] = qLB[ic];
+  R ELL = qLL[IU] * qLL[IB] - qLL[IV] * qLL[IA];
+  R ERL = qRL[IU] * qRL[IB] - qRL[IV] * qRL[IA];
+  R ELR = qLR[IU] * qLR[IB] - qLR[IV] * qLR[IA];
+  R ERR = qRR[IU] * qRR[IB] - qRR[IV] * qRR[IA];
+  return mag_riemann2d_hlld(qLL, qRL, qLR, qRR, ELL, ERL, ELR, ERR, p);
+}
+
+}  // namespace ppk

@@ -1,0 +1,174 @@
+"""Python face of the solver, shaped like the reference's ``SolverBase`` / ``SolverFactory``.
+
+``SolverFactory.Instance().create("MHD_Muscl_3D", params, configMap)`` returns a solver with
+``next_iteration()``, ``compute_dt()``, ``finished()``, ``m_t``, ``m_dt``, ``m_iteration``,
+``m_nCells`` (reference src/shared/SolverBase.h:53-157, src/shared/SolverFactory.h:43-116).
+Every compute call goes through the HIP library's C ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from ._lib import PPKError, check, lib
+from .params import default_problem
+
+
+def params_from_ini(path=None, text=None):
+    """(params, problem, solver_name) from an .ini file or text, with the reference's rules."""
+    p, pr = A.Params(), A.Problem()
+    name = C.create_string_buffer(128)
+    if text is not None:
+        rc = lib().ppk_mhd_params_from_ini_text(text.encode(), C.byref(p), C.byref(pr), name, 128)
+    else:
+        rc = lib().ppk_mhd_params_from_ini(str(path).encode(), C.byref(p), C.byref(pr), name, 128)
+    check(rc)
+    return p, pr, name.value.decode()
+
+
+class SolverMHDMuscl:
+    """MHD MUSCL-Hancock solver on one GPU (one C-ABI handle)."""
+
+    def __init__(self, params, problem=None):
+        self.params = params
+        h = C.c_void_p()
+        check(lib().ppk_mhd_create(C.byref(params), C.byref(h)))
+        self._h = h
+        self.m_nCells = params.isize * params.jsize * params.ksize  # ghost cells included (H10)
+        self.m_nCells_interior = params.nx * params.ny * (params.nz if params.dim == 3 else 1)
+        if problem is not None:
+            self.init_problem(problem)
+
+    # ---- lifetime -------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ppk_mhd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        return check(rc, self._h)
+
+    # ---- state ----------------------------------------------------------------------------
+    def init_problem(self, problem):
+        self._check(lib().ppk_mhd_init_problem(self._h, C.byref(problem)))
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.size != self.params.count:
+            raise ValueError(f"state must have {self.params.count} doubles, got {U.size}")
+        self._check(lib().ppk_mhd_upload(self._h, U.ctypes.data_as(C.POINTER(C.c_double)), U.size))
+
+    def download(self):
+        U = np.empty(self.params.shape, dtype=np.float64)
+        self._check(lib().ppk_mhd_download(self._h, U.ctypes.data_as(C.POINTER(C.c_double)),
+                                           U.size))
+        return U
+
+    def device_state_ptr(self):
+        ptr = C.POINTER(C.c_double)()
+        self._check(lib().ppk_mhd_device_state(self._h, C.byref(ptr)))
+        return C.cast(ptr, C.c_void_p).value
+
+    def stream_ptr(self):
+        s = C.c_void_p()
+        self._check(lib().ppk_mhd_stream(self._h, C.byref(s)))
+        return s.value
+
+    # ---- time loop (SolverBase) -------------------------------------------------------------
+    def make_boundaries(self):
+        self._check(lib().ppk_mhd_make_boundaries(self._h))
+
+    def compute_dt_local(self):
+        dt = C.c_double()
+        self._check(lib().ppk_mhd_compute_dt(self._h, C.byref(dt)))
+        return dt.value
+
+    def step(self, dt=0.0, dt_mode=A.DT_CFL):
+        self._check(lib().ppk_mhd_step(self._h, float(dt), int(dt_mode)))
+
+    def next_iteration(self):
+        """SolverBase::next_iteration with reference dt semantics."""
+        self.step(0.0, A.DT_CFL)
+
+    def run(self, nsteps, dt=0.0, dt_mode=A.DT_CFL):
+        self._check(lib().ppk_mhd_run(self._h, int(nsteps), float(dt), int(dt_mode)))
+
+    def time(self):
+        t, dt, it = C.c_double(), C.c_double(), C.c_int64()
+        self._check(lib().ppk_mhd_get_time(self._h, C.byref(t), C.byref(dt), C.byref(it)))
+        return t.value, dt.value, it.value
+
+    def set_time(self, t, iteration):
+        self._check(lib().ppk_mhd_set_time(self._h, float(t), int(iteration)))
+
+    @property
+    def m_t(self):
+        return self.time()[0]
+
+    @property
+    def m_dt(self):
+        return self.time()[1]
+
+    @property
+    def m_iteration(self):
+        return self.time()[2]
+
+    def finished(self):
+        f = C.c_int()
+        self._check(lib().ppk_mhd_finished(self._h, C.byref(f)))
+        return bool(f.value)
+
+    def synchronize(self):
+        self._check(lib().ppk_mhd_synchronize(self._h))
+
+    # ---- diagnostics ------------------------------------------------------------------------
+    def divb_max(self):
+        d = C.c_double()
+        self._check(lib().ppk_mhd_divb_max(self._h, C.byref(d)))
+        return d.value
+
+    def conserved_sums(self):
+        s = (C.c_double * 8)()
+        self._check(lib().ppk_mhd_conserved_sums(self._h, s))
+        return np.array(list(s))
+
+
+class SolverFactory:
+    """Registry of solver constructors keyed by the reference's solver names
+    (src/shared/SolverFactory.cpp:29-30)."""
+
+    _instance = None
+
+    def __init__(self):
+        self._map = {}
+        self.registerSolver("MHD_Muscl_2D", SolverMHDMuscl)
+        self.registerSolver("MHD_Muscl_3D", SolverMHDMuscl)
+
+    @classmethod
+    def Instance(cls):
+        if cls._instance is None:
+            cls._instance = SolverFactory()
+        return cls._instance
+
+    def registerSolver(self, key, fn):
+        self._map[key] = fn
+
+    def create(self, solver_name, params, problem=None):
+        fn = self._map.get(solver_name)
+        if fn is None:
+            raise PPKError(A.PPK_OK + 5, f"{solver_name}: is not recognized as a valid application "
+                           f"name key; valid names: {sorted(self._map)}")
+        return fn(params, problem if problem is not None else default_problem(A.PROBLEM_ORSZAG_TANG,
+                                                                               params))

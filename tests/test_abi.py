@@ -1,0 +1,182 @@
+"""The C-ABI library (CPU checks: no GPU compute).
+
+The library must load, export every function declared in include/ppkmhd_amd.h, agree with
+the ctypes mirror of its structs, and parse .ini files with the reference's rules.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd import f32, make_params
+from ppkmhd_amd._lib import LIB_PATH, SIGNATURES, PPKError, lib
+from ppkmhd_amd.solver import params_from_ini
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "ppkmhd_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(ppk_mhd_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB_PATH), "build the library first (python -m ppkmhd_amd.build)"
+    L = C.CDLL(LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding covers the whole header
+    assert set(names) == set(SIGNATURES), set(names) ^ set(SIGNATURES)
+
+
+def test_struct_layouts_match():
+    L = lib()
+    assert L.ppk_mhd_params_size() == C.sizeof(A.Params)
+    assert L.ppk_mhd_problem_size() == C.sizeof(A.Problem)
+    assert L.ppk_mhd_version().decode().startswith("ppkmhd_amd")
+
+
+def test_params_init_matches_reference_formulas():
+    p = make_params(3, 16, 8, 4, xmax=2.0, ymax=1.0, zmax=0.5, gamma0=5 / 3, smallc=1e-7,
+                    smallr=1e-8)
+    q = A.Params.from_buffer_copy(p)
+    q.dx = q.dy = q.dz = q.smallp = q.smallpp = q.gamma6 = 0.0
+    assert lib().ppk_mhd_params_init(C.byref(q)) == 0
+    for f in ("dx", "dy", "dz", "smallp", "smallpp", "gamma6"):
+        assert getattr(q, f) == getattr(p, f), f
+
+
+OT_INI = """
+[run]
+solver_name=MHD_Muscl_2D
+tEnd=0.5
+nStepmax=1000
+nOutput=0
+
+[mesh]
+nx=256
+ny=256
+boundary_type_xmin=3
+boundary_type_xmax=3
+boundary_type_ymin=3
+boundary_type_ymax=3
+
+[hydro]
+gamma0=1.666666666667
+cfl=0.8
+slope_type=1
+problem=orszag_tang
+riemann=hlld
+smallr=1e-7
+smallc=1e-7
+
+[OTHER]
+implementationVersion=0
+"""
+
+
+def test_ini_reference_semantics():
+    p, pr, name = params_from_ini(text=OT_INI)
+    assert name == "MHD_Muscl_2D"
+    assert p.dim == 2 and p.nx == 256 and p.ny == 256 and p.nz == 1
+    assert p.ghost_width == 3
+    assert p.riemann == A.RIEMANN_HLLD
+    assert list(p.bc)[:4] == [3, 3, 3, 3]
+    # reals come back float-rounded (ConfigMap::getFloat)
+    assert p.gamma0 == f32(1.666666666667) and p.gamma0 != 5 / 3
+    assert p.cfl == f32(0.8) and p.tEnd == f32(0.5)
+    assert p.smallc == f32(1e-7) and p.smallr == f32(1e-7)
+    ref = make_params(2, 256, 256, gamma0=1.666666666667, cfl=0.8, slope_type=1, smallr=1e-7,
+                      smallc=1e-7, tEnd=0.5, riemann=A.RIEMANN_HLLD)
+    for f in ("dx", "dy", "smallp", "smallpp", "gamma6", "xmin", "xmax"):
+        assert getattr(p, f) == getattr(ref, f), f
+    assert pr.kind == A.PROBLEM_ORSZAG_TANG
+
+
+def test_ini_case_insensitive_keys_and_comments():
+    txt = OT_INI.replace("[run]", "[RUN]").replace("tEnd=0.5", "TEND = 0.25 ; inline comment")
+    txt = txt.replace("[mesh]", "# a comment line\n[Mesh]").replace("nx=256", "NX=128")
+    p, _, _ = params_from_ini(text=txt)
+    assert p.tEnd == f32(0.25)
+    assert p.nx == 128
+
+
+def test_ini_defaults_and_problem_params():
+    txt = """[run]
+solver_name=MHD_Muscl_3D
+[mesh]
+nx=8
+ny=8
+nz=8
+[hydro]
+problem=blast
+riemann=hll
+[blast]
+density_out=1.0
+"""
+    p, pr, _ = params_from_ini(text=txt)
+    assert p.dim == 3
+    # HydroParams::setup defaults (src/shared/HydroParams.cpp)
+    assert p.gamma0 == f32(1.4) and p.cfl == f32(0.5) and p.slope_type == 1.0
+    assert p.smallc == f32(1e-10) and p.n_step_max == 1000
+    assert list(p.bc) == [A.BC_DIRICHLET] * 6
+    assert p.riemann == A.RIEMANN_HLL
+    assert pr.kind == A.PROBLEM_BLAST
+    assert pr.blast_density_out == 1.0 and pr.blast_density_in == 1.0
+    assert pr.blast_radius == f32(0.05) and pr.blast_pressure_in == 10.0
+    assert pr.blast_pressure_out == f32(0.1)
+    assert pr.blast_bx == 0.5
+
+
+def test_ini_unknown_solver_is_an_error():
+    with pytest.raises(PPKError) as e:
+        params_from_ini(text="[run]\nsolver_name=Hydro_Muscl_2D\n")
+    assert e.value.code == 5
+    assert "Hydro_Muscl_2D" in str(e.value)
+
+
+def test_ini_missing_file_is_an_error(tmp_path):
+    with pytest.raises(PPKError) as e:
+        params_from_ini(path=tmp_path / "nope.ini")
+    assert e.value.code == 5
+
+
+def test_reference_settings_files_parse():
+    """The MHD .ini files shipped with the reference parse (their keys exist)."""
+    txt = """[run]
+solver_name=MHD_Muscl_2D
+tEnd=1.0
+nStepmax=1000
+nOutput=10
+[mesh]
+nx=256
+ny=256
+boundary_type_xmin=3
+boundary_type_xmax=3
+boundary_type_ymin=3
+boundary_type_ymax=3
+[hydro]
+gamma0=1.666
+cfl=0.8
+niter_riemann=10
+iorder=2
+slope_type=2
+problem=orszag_tang
+riemann=hlld
+smallr=1e-7
+smallc=1e-7
+[output]
+outputPrefix=orszag_tang_2d
+outputVtkAscii=false
+hdf5_enabled=yes
+[other]
+implementationVersion=2
+"""
+    p, pr, name = params_from_ini(text=txt)
+    assert p.slope_type == 2.0 and p.gamma0 == f32(1.666)
