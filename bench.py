@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: MHD cell-updates/s of the MUSCL-Hancock + HLLD + CT step on one MI355X.
+
+Workload (default): 3D MHD blast, FP64, 256^3 periodic unit cube, rho = 1 inside and outside,
+p = 10 inside a sphere of radius 0.1 at the centre and 0.1 outside, v = 0, uniform
+B = (1,1,1)/sqrt(3), gamma = 5/3, HLLD + CT, minmod slopes, CFL 0.8 (dt from the CFL condition
+every step, as the reference does).  A step is one full time step of the solver over the grid.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--n 256] [--no-cpu]
+
+Prints one JSON line on stdout; progress goes to stderr.  The state is resident in HBM for the
+whole timed region (no host transfers); the host does not synchronize between steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ppkmhd_amd import _abi as A  # noqa: E402
+from ppkmhd_amd import default_problem, make_params  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E, GB/s
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix), TFLOP/s
+
+# Algorithmic FP64 operations (add + mul + div + sqrt, counted by the instrumented oracle) per
+# work item of each stage, 3D; the item ranges are those of the reference's functors.
+FLOP_PER_ITEM_3D = {
+    "primitives_dt": 57.5, "elec_field": 45.0, "mag_slopes": 48.0, "trace": 639.0,
+    "fluxes": 855.0, "emf": 1575.0, "update_ct": 78.0,
+}
+# Algorithmic HBM bytes per work item of each staged kernel (inputs read once + outputs
+# written once, FP64): e.g. emf reads 12 edge states x 8 variables and writes 3 EMFs.
+BYTES_PER_ITEM_3D = {
+    "primitives_dt": 8 * (8 + 3 + 8), "elec_field": 8 * (6 + 4 + 3), "mag_slopes": 8 * (3 + 6),
+    "trace": 8 * (8 + 6 + 12 + 12 + 18 * 8), "fluxes": 8 * (6 * 8 + 3 * 8),
+    "emf": 8 * (12 * 8 + 3), "update_ct": 8 * (8 + 6 * 5 + 7 + 8),
+}
+# the fused step: each cell's 8 FP64 conservative variables read once and written once
+STEP_BYTES_PER_CELL = 128.0
+
+
+def items(stage, n):
+    return {"primitives_dt": (n + 5) ** 3, "elec_field": (n + 4) ** 3, "mag_slopes": (n + 4) ** 3,
+            "trace": (n + 3) ** 3, "fluxes": (n + 1) ** 3, "emf": (n + 1) ** 3,
+            "update_ct": n ** 3, "fused_half": (n + 2) ** 3, "fused_update": n ** 3}.get(stage)
+
+
+def stage_flops(stage, n):
+    if stage in FLOP_PER_ITEM_3D:
+        return FLOP_PER_ITEM_3D[stage] * items(stage, n)
+    if stage == "fused_update":
+        return (FLOP_PER_ITEM_3D["fluxes"] + FLOP_PER_ITEM_3D["emf"] +
+                FLOP_PER_ITEM_3D["update_ct"] + FLOP_PER_ITEM_3D["primitives_dt"]) * n ** 3
+    if stage == "fused_half":
+        return (FLOP_PER_ITEM_3D["primitives_dt"] + FLOP_PER_ITEM_3D["elec_field"] +
+                FLOP_PER_ITEM_3D["mag_slopes"] + FLOP_PER_ITEM_3D["trace"]) * n ** 3
+    return None
+
+
+def stage_bytes(stage, n):
+    if stage in BYTES_PER_ITEM_3D:
+        return BYTES_PER_ITEM_3D[stage] * items(stage, n)
+    if stage in ("fused_update", "fused_half"):
+        return None
+    return None
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_params(n, path, precision):
+    p = make_params(3, n, n, n, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
+                    bc=A.BC_PERIODIC, tEnd=1.0e9, n_step_max=1 << 30, kernel_path=path,
+                    precision=precision)
+    pr = default_problem(A.PROBLEM_BLAST, p)
+    pr.blast_radius = 0.1
+    pr.blast_center_x = pr.blast_center_y = pr.blast_center_z = 0.5
+    pr.blast_density_in = pr.blast_density_out = 1.0
+    pr.blast_pressure_in = 10.0
+    pr.blast_pressure_out = 0.1
+    b = 1.0 / np.sqrt(3.0)
+    pr.blast_bx = pr.blast_by = pr.blast_bz = b
+    return p, pr
+
+
+def cpu_baseline(n, steps=1):
+    """The oracle (C++ OpenMP restatement of the reference's step) on the host cores, on a
+    bounded sample of the same workload: one warm-up step, then `steps` timed steps."""
+    from oracle import pyoracle as O
+
+    O.build()
+    p, pr = workload_params(n, A.PATH_AUTO, A.FP64)
+    s = O.Solver(p)
+    s.init_problem(pr)
+    s.step(0, A.DT_CFL)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(0, A.DT_CFL)
+    dt = time.perf_counter() - t0
+    s.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (C++ OpenMP restatement of the reference impl-0 step), same "
+                      f"{n}^3 blast workload, 1 warm-up + {steps} timed step(s) "
+                      f"({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--path", choices=("auto", "staged", "fused"), default="auto")
+    ap.add_argument("--precision", choices=("fp64", "fp64_strict", "fp32_bulk"), default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the MHD step is measured on a single MI355X "
+              "(multi-GPU halo exchange is not part of this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from ppkmhd_amd.solver import SolverMHDMuscl
+
+    path = {"auto": A.PATH_AUTO, "staged": A.PATH_STAGED, "fused": A.PATH_FUSED}[a.path]
+    prec = {"fp64": A.FP64, "fp64_strict": A.FP64_STRICT, "fp32_bulk": A.FP32_BULK}[a.precision]
+    n = a.n
+    p, pr = workload_params(n, path, prec)
+    log(f"[bench] creating {n}^3 solver (path={a.path}, precision={a.precision})")
+    s = SolverMHDMuscl(p, pr)
+    log(f"[bench] warmup {a.warmup} steps")
+    s.run(a.warmup)
+    s.synchronize()
+
+    # short profiled pass: find the dominant kernel stage
+    s.timers_enable(None)
+    s.run(3)
+    prof = s.timers_read()
+    s.timers_enable([])
+    dom = max(prof, key=lambda k: prof[k][0] / prof[k][1] * (1 if k != "control" else 0))
+    log("[bench] stage ms/launch: " + ", ".join(f"{k} {v[0] / v[1]:.3f}" for k, v in prof.items()))
+
+    # timed region: only the dominant stage is bracketed with events
+    s.timers_enable([dom])
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(a.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    dom_ms, dom_cnt = s.timers_read()[dom]
+    t, dt, it = s.time()
+    s.timers_enable([])
+    elapsed = t1 - t0
+    ms_step = elapsed / a.steps * 1e3
+    cells = n ** 3
+    value = cells * a.steps / elapsed
+    log(f"[bench] {a.steps} steps in {elapsed:.3f} s: {ms_step:.3f} ms/step, "
+        f"{value / 1e9:.3f} G cell-updates/s; t={t:.6g} it={it}")
+
+    avg_ms = dom_ms / dom_cnt
+    flops = stage_flops(dom, n)
+    nbytes = stage_bytes(dom, n)
+    roof = None
+    if flops is not None:
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": dom,
+                "kernel_ms": avg_ms, "flop_per_launch": flops,
+                "note": "FP64 VALU-bound stage; peak is the MI355X FP64 figure"}
+        if nbytes is not None:
+            gbs = nbytes / (avg_ms * 1e-3) / 1e9
+            roof["hbm_achieved_GBs"] = gbs
+            roof["hbm_frac"] = gbs / PEAK_HBM_GBS
+    step_hbm = STEP_BYTES_PER_CELL * value / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        log(f"[bench] CPU baseline: oracle on {n}^3, {a.cpu_steps} timed step(s)")
+        try:
+            cpu = cpu_baseline(n, a.cpu_steps)
+        except Exception as e:  # reported, never fatal for the GPU figure
+            log(f"[bench] CPU baseline failed: {e!r}")
+
+    out = {
+        "metric": "MHD cell-updates/s (MUSCL-HLLD, 3D 256^3 FP64)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec != A.FP32_BULK else "f64 state / f32 bulk",
+        "data": "synthetic (analytic blast initial condition)",
+        "config": {"workload": f"3D MHD blast {n}^3 FP64, periodic, HLLD+CT, minmod, CFL 0.8",
+                   "global_batch": 1, "seq_len": None, "parallelism": "single GPU",
+                   "kernel_path": a.path, "precision": a.precision},
+        "value_ghost_inclusive": (n + 6) ** 3 * a.steps / elapsed,
+        "step_hbm_GBs_at_128B_per_cell": step_hbm,
+        "step_hbm_frac": step_hbm / PEAK_HBM_GBS,
+        "stage_ms": {k: v[0] / v[1] for k, v in prof.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

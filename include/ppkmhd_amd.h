@@ -208,6 +208,18 @@ int ppk_mhd_finished(ppk_mhd_t *h, int *finished);
 /* Wait for all work of the handle. */
 int ppk_mhd_synchronize(ppk_mhd_t *h);
 
+/*
+ * Per-stage GPU timers, the counterpart of the reference's phase timers (TIMER_BOUNDARIES,
+ * TIMER_DT, TIMER_NUM_SCHEME; src/shared/SolverBase.h:35-42, 150-157): when a stage's bit is
+ * set in stage_mask, every launch of that stage is bracketed by a pair of hipEvents recorded
+ * on the handle's stream.  Enabling (or disabling) clears the accumulated times.
+ */
+int ppk_mhd_timers_enable(ppk_mhd_t *h, uint32_t stage_mask);
+/* Sum of the recorded durations (ms) and the number of launches of one stage (synchronizes). */
+int ppk_mhd_timers_read(ppk_mhd_t *h, int stage, double *total_ms, int64_t *launches);
+int ppk_mhd_timer_count(void);
+const char *ppk_mhd_timer_name(int stage);
+
 /* Diagnostics (not on the reference path): max over interior cells of |div B| computed
  * from the face-centred field, and the interior sums of the 8 conservative variables.
  * Synchronizes. */

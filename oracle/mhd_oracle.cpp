@@ -38,7 +38,11 @@ namespace {
 struct OpCounts {
   long long add = 0, mul = 0, div = 0, sqrt = 0, sel = 0;
 };
-OpCounts g_ops;
+// counts per stage of the step (index set by the solver before each stage)
+constexpr int kStages = 10;
+OpCounts g_stage_ops[kStages];
+int g_stage = 0;
+#define g_ops (g_stage_ops[g_stage])
 
 struct real {
   double v;
@@ -64,7 +68,9 @@ inline real FMAX(real a, real b) { ++g_ops.sel; return real(std::fmax(a.v, b.v))
 inline real FMIN(real a, real b) { ++g_ops.sel; return real(std::fmin(a.v, b.v)); }
 inline real COPYSIGN(real a, real b) { return real(std::copysign(a.v, b.v)); }
 inline double D(real a) { return a.v; }
+#define ORA_STAGE(n) (g_stage = (n))
 #else
+#define ORA_STAGE(n) ((void)0)
 typedef double real;
 inline double SQRT(double a) { return std::sqrt(a); }
 inline double FABS(double a) { return std::fabs(a); }
@@ -1167,8 +1173,10 @@ struct Solver {
 
   // godunov_unsplit_impl, src/muscl/SolverMHDMuscl.cpp:373-458 (2D) and :465-572 (3D)
   void godunov(double *din, double *dout, double fixed_dt, int dt_mode) {
+    ORA_STAGE(0);
     fill_boundaries(p, din);
     std::memcpy(dout, din, sizeof(double) * b.cells() * 8);
+    ORA_STAGE(1);
     primitives(p, din, Q.data());
     if (dt_mode == PPK_DT_CFL)
       compute_dt();
@@ -1183,6 +1191,7 @@ struct Solver {
 
   void step2d(const double *Uin, double *Uout, real dtdx, real dtdy) {
     const int gw = b.gw, is = b.is, js = b.js;
+    ORA_STAGE(4);
     // ComputeTraceFunctor2D_MHD range, src/muscl/MHDRunFunctors2D.h:857-858
 #pragma omp parallel for if (!ORA_COUNTING)
     for (int j = gw - 2; j < js - gw + 1; ++j)
@@ -1198,6 +1207,7 @@ struct Solver {
         set_state(b, QE[ILT][0].data(), i, j, 0, tr.qEdge[ILT]);
         set_state(b, QE[ILB][0].data(), i, j, 0, tr.qEdge[ILB]);
       }
+    ORA_STAGE(5);
     // ComputeFluxesAndStoreFunctor2D_MHD, src/muscl/MHDRunFunctors2D.h:420-464
 #pragma omp parallel for if (!ORA_COUNTING)
     for (int j = gw; j < js - gw + 1; ++j)
@@ -1216,6 +1226,7 @@ struct Solver {
         riemann_mhd(ql, qr, fl, p);
         set_state(b, F[1].data(), i, j, 0, fl);
       }
+    ORA_STAGE(6);
     // ComputeEmfAndStoreFunctor2D, src/muscl/MHDRunFunctors2D.h:654-680
 #pragma omp parallel for if (!ORA_COUNTING)
     for (int j = gw; j < js - gw + 1; ++j)
@@ -1227,6 +1238,7 @@ struct Solver {
         get_state(b, QE[ILB][0].data(), i, j, 0, qe[ILB]);
         Emf[b.at(i, j, 0, 0)] = D(compute_emf(EMFZ, qe, p));
       }
+    ORA_STAGE(7);
     // UpdateFunctor2D_MHD, src/muscl/MHDRunFunctors2D.h:1574-1634
 #pragma omp parallel for if (!ORA_COUNTING)
     for (int j = gw; j < js - gw; ++j)
@@ -1264,6 +1276,7 @@ struct Solver {
   void step3d(const double *Uin, double *Uout, real dtdx, real dtdy, real dtdz) {
     const int gw = b.gw, is = b.is, js = b.js, ks = b.ks;
     const double *Qd = Q.data();
+    ORA_STAGE(2);
     // ComputeElecFieldFunctor3D, src/muscl/MHDRunFunctors3D.h:303-356
 #pragma omp parallel for collapse(2) if (!ORA_COUNTING)
     for (int k = 1; k < ks - 1; ++k)
@@ -1288,6 +1301,7 @@ struct Solver {
           B = 0.5 * (uu(i - 1, j, k, IB) + uu(i, j, k, IB));
           Elec[b.at(i, j, k, IZ)] = D(u * B - v * A);
         }
+    ORA_STAGE(3);
     // ComputeMagSlopesFunctor3D + slope_unsplit_mhd_3d, src/muscl/MHDRunFunctors3D.h:472-532,
     // src/muscl/MHDBaseFunctor3D.h:561-668 (xslope_type = min(slope_type, 2))
     const real xst = FMIN(real(p.slope_type), 2.0);
@@ -1313,6 +1327,7 @@ struct Solver {
           dC[b.at(i, j, k, 1)] = D(dCy);
           dC[b.at(i, j, k, 2)] = 0.0;
         }
+    ORA_STAGE(4);
     // ComputeTraceFunctor3D_MHD range, src/muscl/MHDRunFunctors3D.h:678-680
 #pragma omp parallel for collapse(2) if (!ORA_COUNTING)
     for (int k = gw - 2; k < ks - gw + 1; ++k)
@@ -1328,6 +1343,7 @@ struct Solver {
           for (int c = 0; c < 4; ++c)
             for (int d = 0; d < 3; ++d) set_state(b, QE[c][d].data(), i, j, k, tr.qEdge[c][d]);
         }
+    ORA_STAGE(5);
     // ComputeFluxesAndStoreFunctor3D_MHD, src/muscl/MHDRunFunctors3D.h:1839-1902
 #pragma omp parallel for collapse(2) if (!ORA_COUNTING)
     for (int k = gw; k < ks - gw + 1; ++k)
@@ -1355,6 +1371,7 @@ struct Solver {
           riemann_mhd(ql, qr, fl, p);
           set_state(b, F[2].data(), i, j, k, fl);
         }
+    ORA_STAGE(6);
     // ComputeEmfAndStoreFunctor3D, src/muscl/MHDRunFunctors3D.h:2183-2230 (EMFY takes RB and
     // LT swapped, :2212-2217)
 #pragma omp parallel for collapse(2) if (!ORA_COUNTING)
@@ -1378,6 +1395,7 @@ struct Solver {
           get_state(b, QE[ILB][0].data(), i, j, k, qe[ILB]);
           Emf[b.at(i, j, k, I_EMFX)] = D(compute_emf(EMFX, qe, p));
         }
+    ORA_STAGE(7);
     // UpdateFunctor3D_MHD, src/muscl/MHDRunFunctors3D.h:2471-2538
 #pragma omp parallel for collapse(2) if (!ORA_COUNTING)
     for (int k = gw; k < ks - gw; ++k)
@@ -1894,19 +1912,27 @@ const double *ora_scratch(const ora_solver *s, int which) {
 }
 
 void ora_op_counts(long long counts[5]) {
-#ifdef ORA_COUNT
-  counts[0] = g_ops.add;
-  counts[1] = g_ops.mul;
-  counts[2] = g_ops.div;
-  counts[3] = g_ops.sqrt;
-  counts[4] = g_ops.sel;
-#else
+  ora_op_counts_stage(-1, counts);
+}
+void ora_op_counts_stage(int stage, long long counts[5]) {
   for (int c = 0; c < 5; ++c) counts[c] = 0;
+#ifdef ORA_COUNT
+  for (int st = 0; st < kStages; ++st) {
+    if (stage >= 0 && st != stage) continue;
+    counts[0] += g_stage_ops[st].add;
+    counts[1] += g_stage_ops[st].mul;
+    counts[2] += g_stage_ops[st].div;
+    counts[3] += g_stage_ops[st].sqrt;
+    counts[4] += g_stage_ops[st].sel;
+  }
+#else
+  (void)stage;
 #endif
 }
 void ora_op_counts_reset(void) {
 #ifdef ORA_COUNT
-  g_ops = OpCounts();
+  for (int st = 0; st < kStages; ++st) g_stage_ops[st] = OpCounts();
+  g_stage = 0;
 #endif
 }
 int ora_is_counting_build(void) {

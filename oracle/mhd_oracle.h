@@ -60,6 +60,9 @@ const double *ora_scratch(const ora_solver *s, int which);
 /* operation counts of one step (instrumented build only; zeros otherwise):
  * counts[0] add/sub, [1] mul, [2] div, [3] sqrt, [4] min/max/abs/compare-select */
 void ora_op_counts(long long counts[5]);
+/* same, for one stage of the step: 0 boundaries+copy, 1 primitives+dt, 2 E field (3D),
+ * 3 magnetic slopes (3D), 4 trace, 5 fluxes, 6 EMFs, 7 update + CT */
+void ora_op_counts_stage(int stage, long long counts[5]);
 void ora_op_counts_reset(void);
 int ora_is_counting_build(void);
 

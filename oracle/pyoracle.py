@@ -67,6 +67,7 @@ def _lib(counting=False):
     lib.ora_scratch.restype = dp
     lib.ora_op_counts.argtypes = [C.POINTER(C.c_longlong)]
     lib.ora_op_counts_reset.argtypes = []
+    lib.ora_op_counts_stage.argtypes = [C.c_int, C.POINTER(C.c_longlong)]
     lib.ora_is_counting_build.restype = C.c_int
     _LIBS[name] = lib
     return lib
@@ -185,10 +186,15 @@ class Solver:
         return a.reshape(ncomp, self.p.ksize, self.p.jsize, self.p.isize)
 
 
-def op_counts():
+STAGES = ("boundaries", "primitives_dt", "elec_field", "mag_slopes", "trace", "fluxes", "emf",
+          "update_ct")
+
+
+def op_counts(stage=None):
+    """Operation counts accumulated by the instrumented build (all stages, or one by name)."""
     lib = _lib(counting=True)
     c = (C.c_longlong * 5)()
-    lib.ora_op_counts(c)
+    lib.ora_op_counts_stage(-1 if stage is None else STAGES.index(stage), c)
     return dict(zip(("add", "mul", "div", "sqrt", "select"), list(c)))
 
 

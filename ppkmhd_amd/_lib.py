@@ -41,6 +41,10 @@ SIGNATURES = {
     "ppk_mhd_set_time": (C.c_int, [_H, C.c_double, C.c_int64]),
     "ppk_mhd_finished": (C.c_int, [_H, C.POINTER(C.c_int)]),
     "ppk_mhd_synchronize": (C.c_int, [_H]),
+    "ppk_mhd_timers_enable": (C.c_int, [_H, C.c_uint32]),
+    "ppk_mhd_timers_read": (C.c_int, [_H, C.c_int, _DP, C.POINTER(C.c_int64)]),
+    "ppk_mhd_timer_count": (C.c_int, []),
+    "ppk_mhd_timer_name": (C.c_char_p, [C.c_int]),
     "ppk_mhd_divb_max": (C.c_int, [_H, _DP]),
     "ppk_mhd_conserved_sums": (C.c_int, [_H, _DP]),
 }

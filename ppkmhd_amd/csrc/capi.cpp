@@ -40,6 +40,90 @@ hipError_t diag_sums(const DevParams &p, const double *U, double *partials, int 
                      hipStream_t s);
 constexpr int kDiagBlocks = 512;
 
+const char *const kStageNames[ST_COUNT] = {
+    "boundaries", "primitives_dt", "elec_field", "mag_slopes", "trace",        "fluxes",
+    "emf",        "update_ct",     "control",    "fused_half", "fused_update", "fused_cfl"};
+
+// hipEvent pairs per stage; events are pooled and resolved when the times are read
+class EventTimer : public StageTimer {
+ public:
+  explicit EventTimer(hipStream_t s) : stream_(s) {}
+  ~EventTimer() override {
+    for (auto e : pool_) (void)hipEventDestroy(e);
+  }
+  void set_mask(uint32_t m) {
+    mask_ = m;
+    clear();
+  }
+  uint32_t mask() const { return mask_; }
+  void mark(int stage, bool start) override {
+    if (!(mask_ & (1u << stage))) return;
+    hipEvent_t e = take();
+    if (!e) return;
+    (void)hipEventRecord(e, stream_);
+    if (start) {
+      open_[stage] = e;
+    } else if (open_[stage]) {
+      recs_.push_back({stage, open_[stage], e});
+      open_[stage] = nullptr;
+    }
+  }
+  // resolve all recorded pairs into totals; recycles the events
+  hipError_t resolve() {
+    hipError_t err = hipStreamSynchronize(stream_);
+    if (err != hipSuccess) return err;
+    for (auto &r : recs_) {
+      float ms = 0.f;
+      err = hipEventElapsedTime(&ms, r.a, r.b);
+      if (err != hipSuccess) return err;
+      total_ms_[r.stage] += ms;
+      count_[r.stage] += 1;
+      free_.push_back(r.a);
+      free_.push_back(r.b);
+    }
+    recs_.clear();
+    return hipSuccess;
+  }
+  double total(int st) const { return total_ms_[st]; }
+  long long count(int st) const { return count_[st]; }
+
+ private:
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  hipEvent_t take() {
+    if (!free_.empty()) {
+      hipEvent_t e = free_.back();
+      free_.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    pool_.push_back(e);
+    return e;
+  }
+  void clear() {
+    for (auto &r : recs_) {
+      free_.push_back(r.a);
+      free_.push_back(r.b);
+    }
+    recs_.clear();
+    for (int i = 0; i < ST_COUNT; ++i) {
+      total_ms_[i] = 0.0;
+      count_[i] = 0;
+      open_[i] = nullptr;
+    }
+  }
+  hipStream_t stream_;
+  uint32_t mask_ = 0;
+  std::vector<hipEvent_t> pool_, free_;
+  std::vector<Rec> recs_;
+  hipEvent_t open_[ST_COUNT] = {};
+  double total_ms_[ST_COUNT] = {};
+  long long count_[ST_COUNT] = {};
+};
+
 }  // namespace ppk
 
 struct ppk_mhd_solver {
@@ -56,9 +140,11 @@ struct ppk_mhd_solver {
   ppk::FusedScratch fw;
   bool fused = false;
   const ppk::KernelTable *K = nullptr;
+  ppk::EventTimer *timer = nullptr;
   long long iter_host = 0;  // iteration count as far as the host knows (buffer parity)
   std::string err;
 
+  ppk::StageTimer *tm() { return (timer && timer->mask()) ? timer : nullptr; }
   double *cur() { return U[iter_host & 1]; }
   double *nxt() { return U[(iter_host + 1) & 1]; }
   long long cells() const { return (long long)dp.isize * dp.jsize * dp.ksize; }
@@ -174,15 +260,16 @@ int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
   const ppk::DevParams &p = h->dp;
   hipStream_t s = h->stream;
   double *in = h->cur(), *out = h->nxt();
+  ppk::StageTimer *tm = h->tm();
   HIP_CHECK(h, K.begin(h->ctrl, check_finished, s));
   if (h->fused) {
     HIP_CHECK(h, ppk::fused_step(p, in, out, h->fw, h->ctrl, dt, dt_mode == PPK_DT_CFL,
-                                 h->p.precision, s));
+                                 h->p.precision, s, tm));
   } else {
-    HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s));
-    HIP_CHECK(h, K.prim_dt(p, in, h->Q, h->ctrl, dt_mode == PPK_DT_CFL, s));
+    HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s, tm));
+    HIP_CHECK(h, K.prim_dt(p, in, h->Q, h->ctrl, dt_mode == PPK_DT_CFL, s, tm));
     HIP_CHECK(h, K.set_dt(h->ctrl, dt, dt_mode == PPK_DT_CFL, h->p.cfl, s));
-    HIP_CHECK(h, K.godunov(p, in, out, h->Q, h->w, h->ctrl, s));
+    HIP_CHECK(h, K.godunov(p, in, out, h->Q, h->w, h->ctrl, s, tm));
   }
   HIP_CHECK(h, K.end(h->ctrl, s));
   return PPK_OK;
@@ -204,6 +291,8 @@ void free_all(ppk_mhd_t *h) {
   F(h->w.elec);
   F(h->w.dbf);
   ppk::fused_free(h->fw);
+  delete h->timer;
+  h->timer = nullptr;
   if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
   if (h->stream) (void)hipStreamDestroy(h->stream);
 }
@@ -258,6 +347,8 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   if (hipGetDevice(&h->device) != hipSuccess) return bail(PPK_ERR_HIP, "hipGetDevice failed");
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(PPK_ERR_HIP, "hipStreamCreateWithFlags failed");
+  h->timer = new (std::nothrow) ppk::EventTimer(h->stream);
+  if (!h->timer) return bail(PPK_ERR_OUT_OF_MEMORY, "host allocation failed");
 
   const size_t n = (size_t)h->cells();
   auto dalloc = [&](double **ptr, size_t count) {
@@ -359,7 +450,7 @@ int ppk_mhd_stream(ppk_mhd_t *h, void **stream) {
 int ppk_mhd_make_boundaries(ppk_mhd_t *h) {
   if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
   HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
-  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
   return PPK_OK;
 }
 
@@ -367,11 +458,11 @@ int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
   if (!h || !dt) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   // the dt the next step would take: ghost fill, primitives, CFL max (no tEnd clamp)
   HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
-  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
   if (h->fused) {
-    HIP_CHECK(h, ppk::fused_cfl(h->dp, h->cur(), h->fw, h->ctrl, h->stream));
+    HIP_CHECK(h, ppk::fused_cfl(h->dp, h->cur(), h->fw, h->ctrl, h->stream, h->tm()));
   } else {
-    HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->ctrl, 1, h->stream));
+    HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->ctrl, 1, h->stream, h->tm()));
   }
   int rc = read_ctrl(h);
   if (rc != PPK_OK) return rc;
@@ -447,6 +538,29 @@ int ppk_mhd_synchronize(ppk_mhd_t *h) {
   if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
   HIP_CHECK(h, hipStreamSynchronize(h->stream));
   return PPK_OK;
+}
+
+int ppk_mhd_timers_enable(ppk_mhd_t *h, uint32_t stage_mask) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  h->timer->set_mask(stage_mask);
+  return PPK_OK;
+}
+
+int ppk_mhd_timers_read(ppk_mhd_t *h, int stage, double *total_ms, int64_t *launches) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (stage < 0 || stage >= ppk::ST_COUNT) return fail(h, PPK_ERR_INVALID_ARGUMENT, "bad stage");
+  HIP_CHECK(h, h->timer->resolve());
+  if (total_ms) *total_ms = h->timer->total(stage);
+  if (launches) *launches = h->timer->count(stage);
+  return PPK_OK;
+}
+
+int ppk_mhd_timer_count(void) { return ppk::ST_COUNT; }
+
+const char *ppk_mhd_timer_name(int stage) {
+  if (stage < 0 || stage >= ppk::ST_COUNT) return "";
+  return ppk::kStageNames[stage];
 }
 
 int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max) {

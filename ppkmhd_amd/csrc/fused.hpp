@@ -19,9 +19,10 @@ hipError_t fused_alloc(const DevParams &p, int precision, FusedScratch &w);
 void fused_free(FusedScratch &w);
 // whole step: ghost fill of Uin, dt (fixed or CFL, clamped to tEnd - t), update into Uout
 hipError_t fused_step(const DevParams &p, double *Uin, double *Uout, FusedScratch &w,
-                      double *ctrl, double fixed_dt, int cfl_mode, int precision, hipStream_t s);
+                      double *ctrl, double fixed_dt, int cfl_mode, int precision, hipStream_t s,
+                      StageTimer *tm);
 // CFL accumulator of the current state into ctrl[CTRL_INVDT] (ghosts already filled)
 hipError_t fused_cfl(const DevParams &p, const double *U, FusedScratch &w, double *ctrl,
-                     hipStream_t s);
+                     hipStream_t s, StageTimer *tm);
 
 }  // namespace ppk

@@ -22,6 +22,34 @@ enum {
   CTRL_SLOTS = 8
 };
 
+// kernel stages bracketed by the optional per-stage GPU timers (ppk_mhd_timers_*)
+enum Stage {
+  ST_BOUNDARIES = 0,
+  ST_PRIM_DT,
+  ST_ELEC,
+  ST_MAGSLOPES,
+  ST_TRACE,
+  ST_FLUXES,
+  ST_EMF,
+  ST_UPDATE,
+  ST_CONTROL,
+  ST_FUSED_HALF,     // fused: primitives + E + slopes + half step (compact trace data)
+  ST_FUSED_UPDATE,   // fused: Riemann solves + EMFs + update + CT (+ next CFL)
+  ST_FUSED_CFL,      // fused: standalone CFL reduction
+  ST_COUNT
+};
+// host-side hook the launchers call around each stage (implemented by the driver)
+struct StageTimer {
+  virtual void mark(int stage, bool start) = 0;
+  virtual ~StageTimer() {}
+};
+#define PPK_TIMED(tm, stage, launch)   \
+  do {                                 \
+    if (tm) (tm)->mark((stage), true); \
+    launch;                            \
+    if (tm) (tm)->mark((stage), false); \
+  } while (0)
+
 // scratch of the stage-per-kernel form (all over the padded box, SoA)
 struct StagedScratch {
   double *faces = nullptr;  // 2*dim face states x 8 vars
@@ -34,16 +62,17 @@ struct StagedScratch {
 
 #define PPK_DECLARE_KERNELS(NS)                                                                  \
   namespace NS {                                                                                 \
-  hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s); \
+  hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s, \
+                               StageTimer *tm);                                                  \
   hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *ctrl,        \
-                            int do_dt, hipStream_t s);                                           \
+                            int do_dt, hipStream_t s, StageTimer *tm);                           \
   hipError_t staged_begin(double *ctrl, int check_finished, hipStream_t s);                      \
   hipError_t staged_set_dt(double *ctrl, double fixed_dt, int cfl_mode, double cfl,              \
                            hipStream_t s);                                                       \
   hipError_t staged_end(double *ctrl, hipStream_t s);                                            \
   hipError_t staged_godunov(const DevParams &p, const double *Uin, double *Uout,                 \
                             const double *Q, StagedScratch &w, const double *ctrl,               \
-                            hipStream_t s);                                                      \
+                            hipStream_t s, StageTimer *tm);                                      \
   }
 
 PPK_DECLARE_KERNELS(fast)

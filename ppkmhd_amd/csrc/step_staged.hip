@@ -469,20 +469,24 @@ int grid_for(long long n) {
 // -------------------------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------------------------
-hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s) {
+hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s,
+                             StageTimer *tm) {
+  if (tm) tm->mark(ST_BOUNDARIES, true);
   for (int dir = 0; dir < p.dim; ++dir) {
     const int s1 = dir == 0 ? p.jsize : p.isize;
     const int s2 = dir == 2 ? p.jsize : p.ksize;
     long long n = 2LL * p.gw * s1 * s2;
     hipLaunchKernelGGL(k_boundaries, dim3(grid_for(n)), dim3(kBlock), 0, s, U, p, dir, ctrl);
   }
+  if (tm) tm->mark(ST_BOUNDARIES, false);
   return hipGetLastError();
 }
 
 hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *ctrl, int do_dt,
-                          hipStream_t s) {
+                          hipStream_t s, StageTimer *tm) {
   long long n = (long long)p.isize * p.jsize * p.ksize;
-  hipLaunchKernelGGL(k_prim_dt, dim3(grid_for(n)), dim3(kBlock), 0, s, U, Q, p, ctrl, do_dt);
+  PPK_TIMED(tm, ST_PRIM_DT,
+            hipLaunchKernelGGL(k_prim_dt, dim3(grid_for(n)), dim3(kBlock), 0, s, U, Q, p, ctrl, do_dt));
   return hipGetLastError();
 }
 
@@ -500,19 +504,22 @@ hipError_t staged_end(double *ctrl, hipStream_t s) {
 }
 
 hipError_t staged_godunov(const DevParams &p, const double *Uin, double *Uout, const double *Q,
-                          StagedScratch &w, const double *ctrl, hipStream_t s) {
+                          StagedScratch &w, const double *ctrl, hipStream_t s, StageTimer *tm) {
   long long n = (long long)p.isize * p.jsize * p.ksize;
   dim3 g(grid_for(n)), b(kBlock);
   if (p.dim == 3) {
-    hipLaunchKernelGGL(k_elec3d, g, b, 0, s, Uin, Q, w.elec, p, ctrl);
-    hipLaunchKernelGGL(k_magslopes3d, g, b, 0, s, Uin, w.dbf, p, ctrl);
-    hipLaunchKernelGGL(k_trace3d, g, b, 0, s, Uin, Q, w.elec, w.dbf, w.faces, w.edges, p, ctrl);
+    PPK_TIMED(tm, ST_ELEC, hipLaunchKernelGGL(k_elec3d, g, b, 0, s, Uin, Q, w.elec, p, ctrl));
+    PPK_TIMED(tm, ST_MAGSLOPES, hipLaunchKernelGGL(k_magslopes3d, g, b, 0, s, Uin, w.dbf, p, ctrl));
+    PPK_TIMED(tm, ST_TRACE, hipLaunchKernelGGL(k_trace3d, g, b, 0, s, Uin, Q, w.elec, w.dbf, w.faces,
+                                               w.edges, p, ctrl));
   } else {
-    hipLaunchKernelGGL(k_trace2d, g, b, 0, s, Uin, Q, w.faces, w.edges, p, ctrl);
+    PPK_TIMED(tm, ST_TRACE,
+              hipLaunchKernelGGL(k_trace2d, g, b, 0, s, Uin, Q, w.faces, w.edges, p, ctrl));
   }
-  hipLaunchKernelGGL(k_fluxes, g, b, 0, s, w.faces, w.flux, p, ctrl);
-  hipLaunchKernelGGL(k_emf, g, b, 0, s, w.edges, w.emf, p, ctrl);
-  hipLaunchKernelGGL(k_update, g, b, 0, s, Uin, Uout, w.flux, w.emf, p, ctrl);
+  PPK_TIMED(tm, ST_FLUXES, hipLaunchKernelGGL(k_fluxes, g, b, 0, s, w.faces, w.flux, p, ctrl));
+  PPK_TIMED(tm, ST_EMF, hipLaunchKernelGGL(k_emf, g, b, 0, s, w.edges, w.emf, p, ctrl));
+  PPK_TIMED(tm, ST_UPDATE,
+            hipLaunchKernelGGL(k_update, g, b, 0, s, Uin, Uout, w.flux, w.emf, p, ctrl));
   return hipGetLastError();
 }
 

@@ -133,6 +133,33 @@ class SolverMHDMuscl:
     def synchronize(self):
         self._check(lib().ppk_mhd_synchronize(self._h))
 
+    # ---- per-stage GPU timers -------------------------------------------------------------
+    @staticmethod
+    def timer_names():
+        L = lib()
+        return [L.ppk_mhd_timer_name(i).decode() for i in range(L.ppk_mhd_timer_count())]
+
+    def timers_enable(self, stages=None):
+        """Bracket the given stages (names or indexes; None = all, [] = off) with hipEvents."""
+        names = self.timer_names()
+        if stages is None:
+            mask = (1 << len(names)) - 1
+        else:
+            mask = 0
+            for st in stages:
+                mask |= 1 << (names.index(st) if isinstance(st, str) else int(st))
+        self._check(lib().ppk_mhd_timers_enable(self._h, mask))
+
+    def timers_read(self):
+        """{stage name: (total ms, launches)} for stages with at least one launch."""
+        out = {}
+        for i, n in enumerate(self.timer_names()):
+            ms, cnt = C.c_double(), C.c_int64()
+            self._check(lib().ppk_mhd_timers_read(self._h, i, C.byref(ms), C.byref(cnt)))
+            if cnt.value:
+                out[n] = (ms.value, cnt.value)
+        return out
+
     # ---- diagnostics ------------------------------------------------------------------------
     def divb_max(self):
         d = C.c_double()

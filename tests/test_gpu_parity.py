@@ -1,0 +1,223 @@
+"""Parity of the HIP path with the oracle (GPU).
+
+Bar: PPK_FP64_STRICT is bit-identical to the oracle (same IEEE operations in the same order);
+PPK_FP64 (FMA contraction allowed) is within a relative L-inf of 1e-12 per step at fixed dt and
+a relative L1 of 1e-10 after 1000 steps.  Every kernel path (staged, fused when available) is
+checked.  Golden fixtures come from tests/golden/.
+"""
+import numpy as np
+import pytest
+
+from parity import rel_l1, rel_linf
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd import default_problem, make_params
+from test_golden_cpu import load_case
+
+pytestmark = pytest.mark.gpu
+
+PATHS = [A.PATH_STAGED, A.PATH_AUTO]
+
+
+def solver(p, pr=None):
+    from ppkmhd_amd.solver import SolverMHDMuscl
+    return SolverMHDMuscl(p, pr)
+
+
+def with_(p, **kw):
+    q = A.Params.from_buffer_copy(p)
+    for k, v in kw.items():
+        setattr(q, k, v)
+    return q
+
+
+def blast3d(n, **kw):
+    p = make_params(3, n, n, n, gamma0=5 / 3, cfl=0.8, slope_type=kw.pop("slope_type", 1.0),
+                    tEnd=1.0, **kw)
+    pr = default_problem(A.PROBLEM_BLAST, p)
+    pr.blast_radius = 0.2
+    pr.blast_density_out = 1.0
+    pr.blast_bx = pr.blast_by = pr.blast_bz = 1.0 / np.sqrt(3.0)
+    return p, pr
+
+
+def ot2d(nx, ny, **kw):
+    p = make_params(2, nx, ny, gamma0=5 / 3, cfl=0.8, slope_type=kw.pop("slope_type", 1.0),
+                    tEnd=0.5, **kw)
+    return p, default_problem(A.PROBLEM_ORSZAG_TANG, p)
+
+
+def run_both(oracle, p, pr, nsteps, dt=0.0, mode=A.DT_CFL):
+    o = oracle.Solver(p)
+    o.init_problem(pr)
+    g = solver(p, pr)
+    assert np.array_equal(g.download(), o.get_state()), "initial conditions differ"
+    for _ in range(nsteps):
+        o.step(dt, mode)
+        g.step(dt, mode)
+    return o, g
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("case", ["ot2d_64x64", "ot2d_48x40", "blast3d_16", "blast3d_12x10x14"])
+def test_strict_is_bit_identical(oracle, gpu, case, path):
+    if case.startswith("ot2d"):
+        nx, ny = map(int, case.split("_")[1].split("x"))
+        p, pr = ot2d(nx, ny, precision=A.FP64_STRICT, kernel_path=path)
+        steps = 10
+    else:
+        dims = case.split("_")[1].split("x")
+        if len(dims) == 1:
+            p, pr = blast3d(int(dims[0]), precision=A.FP64_STRICT, kernel_path=path)
+        else:
+            nx, ny, nz = map(int, dims)
+            p = make_params(3, nx, ny, nz, gamma0=5 / 3, cfl=0.8, tEnd=1.0,
+                            precision=A.FP64_STRICT, kernel_path=path)
+            pr = default_problem(A.PROBLEM_BLAST, p)
+            pr.blast_radius = 0.25
+        steps = 5
+    o, g = run_both(oracle, p, pr, steps)
+    assert np.array_equal(g.download()[p.interior()], o.get_state()[p.interior()])
+    tg, dtg, itg = g.time()
+    to, dto, ito = o.time()
+    assert (tg, dtg, itg) == (to, dto, ito)
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("variant", ["hll", "llf", "mc", "dirichlet_neumann", "kh_rand"])
+def test_strict_variants(oracle, gpu, variant, path):
+    kw = dict(precision=A.FP64_STRICT, kernel_path=path)
+    if variant == "hll":
+        p, pr = ot2d(32, 32, riemann=A.RIEMANN_HLL, **kw)
+    elif variant == "llf":
+        p, pr = blast3d(12, riemann=A.RIEMANN_LLF, **kw)
+    elif variant == "mc":
+        p, pr = blast3d(12, slope_type=2.0, **kw)
+    elif variant == "dirichlet_neumann":
+        p = make_params(3, 12, 10, 8, gamma0=1.4, cfl=0.5, tEnd=1.0, bc=[1, 2, 2, 1, 1, 3], **kw)
+        pr = default_problem(A.PROBLEM_IMPLODE, p)
+        pr.implode_in[5] = 0.3
+        pr.implode_out[6] = 0.2
+    else:
+        p = make_params(3, 16, 16, 16, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5,
+                        zmax=0.5, gamma0=1.4, cfl=0.8, tEnd=1.0, **kw)
+        pr = default_problem(A.PROBLEM_KELVIN_HELMHOLTZ, p)
+        pr.kh_p_rand, pr.kh_p_sine_rob, pr.kh_axis = 1, 0, 1
+        pr.kh_d_in, pr.kh_d_out, pr.kh_pressure, pr.kh_amplitude = 2.0, 1.0, 2.5, 0.01
+        pr.kh_bx = 0.1
+    o, g = run_both(oracle, p, pr, 4)
+    assert np.array_equal(g.download()[p.interior()], o.get_state()[p.interior()])
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fast_fixed_dt_per_step(oracle, gpu, dim, path):
+    """FMA-contracted FP64: relative L-inf <= 1e-12 after one step at fixed dt, from the same
+    state, for several consecutive steps (each step restarted from the oracle state)."""
+    if dim == 2:
+        p, pr = ot2d(64, 64, precision=A.FP64, kernel_path=path)
+        dt = 2e-3
+    else:
+        p, pr = blast3d(16, precision=A.FP64, kernel_path=path)
+        dt = 5e-3
+    o = oracle.Solver(p)
+    o.init_problem(pr)
+    g = solver(p, pr)
+    worst = 0.0
+    for _ in range(5):
+        U = o.get_state()
+        g.upload(U)
+        g.set_time(*o.time()[::2])
+        o.step(dt, A.DT_FIXED)
+        g.step(dt, A.DT_FIXED)
+        worst = max(worst, rel_linf(g.download(), o.get_state(), p))
+    assert worst <= 1e-12, worst
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_fast_1000_steps_l1(oracle, gpu, path):
+    """Orszag-Tang 2D, 1000 CFL steps: relative L1 <= 1e-10 against the oracle."""
+    p, pr = ot2d(64, 64, precision=A.FP64, kernel_path=path, n_step_max=1000)
+    p.tEnd = 10.0
+    o = oracle.Solver(p)
+    o.init_problem(pr)
+    g = solver(p, pr)
+    for _ in range(1000):
+        o.step(0, A.DT_CFL)
+    g.run(1000)
+    assert g.m_iteration == 1000
+    e = rel_l1(g.download(), o.get_state(), p)
+    assert e <= 1e-10, e
+    assert abs(g.m_t - o.time()[0]) <= 1e-12 * o.time()[0]
+
+
+@pytest.mark.parametrize("name", ["ot2d_16", "blast3d_8"])
+@pytest.mark.parametrize("precision", [A.FP64_STRICT, A.FP64])
+def test_golden_fixtures(gpu, name, precision):
+    p, z = load_case(name)
+    p.precision = precision
+    g = solver(p)
+    g.upload(z["U0"])
+    g.step(0, A.DT_CFL)
+    U1 = g.download()
+    for _ in range(9):
+        g.step(0, A.DT_CFL)
+    U10 = g.download()
+    if precision == A.FP64_STRICT:
+        assert np.array_equal(U1[p.interior()], z["U1"][p.interior()])
+        assert np.array_equal(U10[p.interior()], z["U10"][p.interior()])
+        assert g.time()[:2] == tuple(z["time10"])
+    else:
+        assert rel_linf(U1, z["U1"], p) <= 1e-12
+        assert rel_linf(U10, z["U10"], p) <= 1e-11
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_divb_roundoff(gpu, path):
+    """CT keeps the discrete div B of a solenoidal initial field at roundoff (Orszag-Tang 3D)."""
+    p = make_params(3, 32, 32, 32, gamma0=5 / 3, cfl=0.8, tEnd=10.0, kernel_path=path)
+    pr = default_problem(A.PROBLEM_ORSZAG_TANG, p)
+    pr.ot_kt = 1.0
+    g = solver(p, pr)
+    d0 = g.divb_max()
+    g.run(50)
+    d1 = g.divb_max()
+    bscale = 1.0 / np.sqrt(4 * np.pi) / p.dx
+    assert d0 < 1e-12 * bscale and d1 < 1e-12 * bscale, (d0, d1, bscale)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_conservation_periodic(gpu, path):
+    p, pr = blast3d(24, kernel_path=path)
+    p.tEnd = 10.0
+    g = solver(p, pr)
+    s0 = g.conserved_sums()
+    g.run(20)
+    s1 = g.conserved_sums()
+    # mass, energy, momentum and the face-field totals are conserved to roundoff
+    np.testing.assert_allclose(s1, s0, rtol=1e-12, atol=1e-9)
+
+
+def test_run_stops_at_finished(gpu):
+    p, pr = ot2d(32, 32, n_step_max=7)
+    p.tEnd = 10.0
+    g = solver(p, pr)
+    g.run(20)
+    t, dt, it = g.time()
+    assert it == 7 and g.finished()
+    # tEnd clamp: the last dt lands exactly on tEnd
+    p2, pr2 = ot2d(32, 32, n_step_max=100000)
+    p2.tEnd = 0.01
+    g2 = solver(p2, pr2)
+    g2.run(1000)
+    t2, dt2, it2 = g2.time()
+    assert t2 == pytest.approx(0.01, abs=1e-15) and g2.finished()
+
+
+def test_compute_dt_matches_oracle(oracle, gpu):
+    p, pr = blast3d(16, precision=A.FP64_STRICT)
+    o = oracle.Solver(p)
+    o.init_problem(pr)
+    g = solver(p, pr)
+    Q = oracle.primitives(p, oracle.fill_boundaries(p, o.get_state()))
+    assert g.compute_dt_local() == p.cfl / oracle.inv_dt(p, Q)
+    assert g.time()[1] == o.time()[1]
