@@ -41,33 +41,40 @@ BYTES_PER_ITEM_3D = {
     "trace": 8 * (8 + 6 + 12 + 12 + 18 * 8), "fluxes": 8 * (6 * 8 + 3 * 8),
     "emf": 8 * (12 * 8 + 3), "update_ct": 8 * (8 + 6 * 5 + 7 + 8),
 }
-# the fused step: each cell's 8 FP64 conservative variables read once and written once
+# Fused kernels, per work item (3D): the Riemann kernel solves 3 HLLD problems (fluxes stage,
+# 855) and 3 2D-HLLD EMFs (emf stage, 1575) per cell of [gw, size-gw]^3; the adds that rebuild
+# face and edge states from the records are not counted.  The half-step kernel: E field (45)
+# and face-field slopes (48) per cell plus the trace without the 210 adds of state assembly
+# (639 - 210).  The update kernel: update + CT (78) and the CFL term of the new state (57.5).
+FLOP_PER_ITEM_3D.update({"fused_riemann": 855.0 + 1575.0, "fused_half": 45.0 + 48.0 + 429.0,
+                         "fused_update": 78.0 + 57.5})
+# the step: each cell's 8 FP64 conservative variables read once and written once
 STEP_BYTES_PER_CELL = 128.0
 
 
 def items(stage, n):
     return {"primitives_dt": (n + 5) ** 3, "elec_field": (n + 4) ** 3, "mag_slopes": (n + 4) ** 3,
             "trace": (n + 3) ** 3, "fluxes": (n + 1) ** 3, "emf": (n + 1) ** 3,
-            "update_ct": n ** 3, "fused_half": (n + 2) ** 3, "fused_update": n ** 3}.get(stage)
+            "update_ct": n ** 3, "fused_half": (n + 2) ** 3, "fused_riemann": (n + 1) ** 3,
+            "fused_update": n ** 3}.get(stage)
+
+
+def step_flops_per_cell(n):
+    """Algorithmic FP64 operations of one step per interior cell-update (staged item ranges)."""
+    return sum(FLOP_PER_ITEM_3D[k] * items(k, n) for k in
+               ("primitives_dt", "elec_field", "mag_slopes", "trace", "fluxes", "emf",
+                "update_ct")) / n ** 3
 
 
 def stage_flops(stage, n):
     if stage in FLOP_PER_ITEM_3D:
         return FLOP_PER_ITEM_3D[stage] * items(stage, n)
-    if stage == "fused_update":
-        return (FLOP_PER_ITEM_3D["fluxes"] + FLOP_PER_ITEM_3D["emf"] +
-                FLOP_PER_ITEM_3D["update_ct"] + FLOP_PER_ITEM_3D["primitives_dt"]) * n ** 3
-    if stage == "fused_half":
-        return (FLOP_PER_ITEM_3D["primitives_dt"] + FLOP_PER_ITEM_3D["elec_field"] +
-                FLOP_PER_ITEM_3D["mag_slopes"] + FLOP_PER_ITEM_3D["trace"]) * n ** 3
     return None
 
 
 def stage_bytes(stage, n):
     if stage in BYTES_PER_ITEM_3D:
         return BYTES_PER_ITEM_3D[stage] * items(stage, n)
-    if stage in ("fused_update", "fused_half"):
-        return None
     return None
 
 
@@ -121,7 +128,7 @@ def main():
     ap.add_argument("--path", choices=("auto", "staged", "fused"), default="auto")
     ap.add_argument("--precision", choices=("fp64", "fp64_strict", "fp32_bulk"), default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the MHD step is measured on a single MI355X "
@@ -180,6 +187,7 @@ def main():
             roof["hbm_achieved_GBs"] = gbs
             roof["hbm_frac"] = gbs / PEAK_HBM_GBS
     step_hbm = STEP_BYTES_PER_CELL * value / 1e9
+    step_tflops = step_flops_per_cell(n) * value / 1e12
 
     cpu = None
     if not a.no_cpu:
@@ -208,6 +216,9 @@ def main():
         "value_ghost_inclusive": (n + 6) ** 3 * a.steps / elapsed,
         "step_hbm_GBs_at_128B_per_cell": step_hbm,
         "step_hbm_frac": step_hbm / PEAK_HBM_GBS,
+        "step_flop_per_cell": step_flops_per_cell(n),
+        "step_fp64_TFLOPs": step_tflops,
+        "step_fp64_frac": step_tflops / PEAK_FP64_TFLOPS,
         "stage_ms": {k: v[0] / v[1] for k, v in prof.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

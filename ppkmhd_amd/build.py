@@ -39,7 +39,8 @@ COMMON = [
 UNITS = [
     ("step_staged.hip", "step_staged_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("step_staged.hip", "step_staged_strict.o", ["-DPPK_NS=strict", "-ffp-contract=off"]),
-    ("fused.hip", "fused.o", ["-ffp-contract=fast"]),
+    ("fused.hip", "fused_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
+    ("fused.hip", "fused_strict.o", ["-DPPK_NS=strict", "-ffp-contract=off"]),
     ("diag.hip", "diag.o", []),
     ("capi.cpp", "capi.o", ["-x", "hip"]),
     ("config.cpp", "config.o", []),
@@ -48,13 +49,7 @@ UNITS = [
 
 
 def _units():
-    units = []
-    for src, obj, extra in UNITS:
-        if src == "fused.hip" and not os.path.exists(os.path.join(CSRC, src)):
-            units.append(("fused_stub.cpp", "fused_stub.o", ["-x", "hip"]))
-            continue
-        units.append((src, obj, extra))
-    return units
+    return list(UNITS)
 
 
 def _compile(unit, verbose=False):

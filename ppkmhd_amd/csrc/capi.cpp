@@ -15,7 +15,6 @@
 #include <vector>
 
 #include "capi_internal.hpp"
-#include "fused.hpp"
 #include "kernels.hpp"
 #include "ppkmhd_amd.h"
 
@@ -25,10 +24,12 @@ namespace {
 thread_local std::string g_thread_error;
 
 const KernelTable kFast = {fast::staged_boundaries, fast::staged_prim_dt, fast::staged_begin,
-                           fast::staged_set_dt,     fast::staged_end,     fast::staged_godunov};
+                           fast::staged_set_dt,     fast::staged_end,     fast::staged_godunov,
+                           fast::fused_godunov,     fast::fused_cfl};
 const KernelTable kStrict = {strict::staged_boundaries, strict::staged_prim_dt,
                              strict::staged_begin,      strict::staged_set_dt,
-                             strict::staged_end,        strict::staged_godunov};
+                             strict::staged_end,        strict::staged_godunov,
+                             strict::fused_godunov,     strict::fused_cfl};
 }  // namespace
 
 void set_thread_error(const std::string &msg) { g_thread_error = msg; }
@@ -41,8 +42,16 @@ hipError_t diag_sums(const DevParams &p, const double *U, double *partials, int 
 constexpr int kDiagBlocks = 512;
 
 const char *const kStageNames[ST_COUNT] = {
-    "boundaries", "primitives_dt", "elec_field", "mag_slopes", "trace",        "fluxes",
-    "emf",        "update_ct",     "control",    "fused_half", "fused_update", "fused_cfl"};
+    "boundaries", "primitives_dt", "elec_field", "mag_slopes",    "trace",        "fluxes", "emf",
+    "update_ct",  "control",       "fused_half", "fused_riemann", "fused_update", "fused_cfl"};
+static_assert(ST_COUNT == 13, "stage names");
+
+// the fused kernels serve FP64 / strict FP64 in 2D and 3D unless the staged path is asked for
+bool fused_supported(const DevParams &p, int precision, int kernel_path) {
+  (void)p;
+  if (kernel_path == PPK_PATH_STAGED) return false;
+  return precision == PPK_FP64 || precision == PPK_FP64_STRICT;
+}
 
 // hipEvent pairs per stage; events are pooled and resolved when the times are read
 class EventTimer : public StageTimer {
@@ -134,11 +143,15 @@ struct ppk_mhd_solver {
   double *U[2] = {nullptr, nullptr};
   double *Q = nullptr;
   double *ctrl = nullptr;       // device control block
-  double *ctrl_host = nullptr;  // pinned mirror
+  double *ctrl_host = nullptr;  // pinned mirror (CTRL_SLOTS) + one diagnostics slot
   double *diag = nullptr;       // diagnostics scratch (device)
+  double *parts = nullptr;      // per-block CFL partial maxima (device, kCflMaxBlocks)
   ppk::StagedScratch w;
   ppk::FusedScratch fw;
   bool fused = false;
+  // parts holds the CFL maxima of cur() (ghosts as the next fill makes them): set by a fused
+  // CFL-mode step or compute_dt, cleared by anything that changes cur() behind the kernels
+  bool cfl_valid = false;
   const ppk::KernelTable *K = nullptr;
   ppk::EventTimer *timer = nullptr;
   long long iter_host = 0;  // iteration count as far as the host knows (buffer parity)
@@ -261,14 +274,20 @@ int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
   hipStream_t s = h->stream;
   double *in = h->cur(), *out = h->nxt();
   ppk::StageTimer *tm = h->tm();
+  const bool cfl = dt_mode == PPK_DT_CFL;
+  const int nparts = ppk::cfl_blocks(p);
+  const int mode = cfl ? ppk::SETDT_CFL : ppk::SETDT_FIXED;
   HIP_CHECK(h, K.begin(h->ctrl, check_finished, s));
+  HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s, tm));
   if (h->fused) {
-    HIP_CHECK(h, ppk::fused_step(p, in, out, h->fw, h->ctrl, dt, dt_mode == PPK_DT_CFL,
-                                 h->p.precision, s, tm));
+    // the previous fused step left the CFL maxima of its output (= in) in parts
+    if (cfl && !h->cfl_valid) HIP_CHECK(h, K.fused_cfl(p, in, h->parts, h->ctrl, s, tm));
+    HIP_CHECK(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s));
+    HIP_CHECK(h, K.fused_godunov(p, in, out, h->fw, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
+    h->cfl_valid = cfl;
   } else {
-    HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s, tm));
-    HIP_CHECK(h, K.prim_dt(p, in, h->Q, h->ctrl, dt_mode == PPK_DT_CFL, s, tm));
-    HIP_CHECK(h, K.set_dt(h->ctrl, dt, dt_mode == PPK_DT_CFL, h->p.cfl, s));
+    HIP_CHECK(h, K.prim_dt(p, in, h->Q, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
+    HIP_CHECK(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s));
     HIP_CHECK(h, K.godunov(p, in, out, h->Q, h->w, h->ctrl, s, tm));
   }
   HIP_CHECK(h, K.end(h->ctrl, s));
@@ -284,13 +303,16 @@ void free_all(ppk_mhd_t *h) {
   F(h->Q);
   F(h->ctrl);
   F(h->diag);
+  F(h->parts);
   F(h->w.faces);
   F(h->w.edges);
   F(h->w.flux);
   F(h->w.emf);
   F(h->w.elec);
   F(h->w.dbf);
-  ppk::fused_free(h->fw);
+  F(h->fw.rec);
+  F(h->fw.flux);
+  F(h->fw.emf);
   delete h->timer;
   h->timer = nullptr;
   if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
@@ -355,7 +377,8 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
     return hipMalloc((void **)ptr, count * sizeof(double)) == hipSuccess;
   };
   bool ok = dalloc(&h->U[0], 8 * n) && dalloc(&h->U[1], 8 * n) &&
-            dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->diag, ppk::kDiagBlocks * 8 + 8);
+            dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->diag, ppk::kDiagBlocks * 8 + 8) &&
+            dalloc(&h->parts, ppk::kCflMaxBlocks);
   if (ok && !h->fused) {
     const int d = params->dim;
     ok = dalloc(&h->Q, 8 * n) && dalloc(&h->w.faces, (size_t)2 * d * 8 * n) &&
@@ -363,9 +386,14 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
          dalloc(&h->w.flux, (size_t)d * 8 * n) && dalloc(&h->w.emf, (size_t)(d == 3 ? 3 : 1) * n);
     if (ok && d == 3) ok = dalloc(&h->w.elec, 3 * n) && dalloc(&h->w.dbf, 6 * n);
   }
-  if (ok && h->fused) ok = ppk::fused_alloc(h->dp, params->precision, h->fw) == hipSuccess;
+  if (ok && h->fused) {
+    const int d = params->dim;
+    ok = dalloc(&h->fw.rec, (size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n) &&
+         dalloc(&h->fw.flux, (size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n) &&
+         dalloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n);
+  }
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
-  if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * ppk::CTRL_SLOTS) != hipSuccess)
+  if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * (ppk::CTRL_SLOTS + 1)) != hipSuccess)
     return bail(PPK_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
   if (hipMemsetAsync(h->U[0], 0, 8 * n * sizeof(double), h->stream) != hipSuccess ||
       hipMemsetAsync(h->U[1], 0, 8 * n * sizeof(double), h->stream) != hipSuccess)
@@ -400,6 +428,7 @@ int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
   std::vector<double> host((size_t)h->cells() * 8);
   ppk::init_problem_host(h->p, *problem, host.data());
   h->iter_host = 0;
+  h->cfl_valid = false;
   HIP_CHECK(h, hipMemcpyAsync(h->cur(), host.data(), h->state_bytes(), hipMemcpyHostToDevice,
                               h->stream));
   // SolverMHDMuscl ctor: boundaries, U2 = U, primitives, compute_dt (SolverMHDMuscl.h:389-402)
@@ -421,6 +450,7 @@ int ppk_mhd_upload(ppk_mhd_t *h, const double *U, size_t count) {
   if (!h || !U) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (count != (size_t)h->cells() * 8)
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "count must be 8*isize*jsize*ksize");
+  h->cfl_valid = false;
   HIP_CHECK(h, hipMemcpyAsync(h->cur(), U, h->state_bytes(), hipMemcpyHostToDevice, h->stream));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));
   return PPK_OK;
@@ -437,6 +467,7 @@ int ppk_mhd_download(ppk_mhd_t *h, double *U, size_t count) {
 
 int ppk_mhd_device_state(ppk_mhd_t *h, double **dU) {
   if (!h || !dU) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  h->cfl_valid = false;  // the caller may write the state
   *dU = h->cur();
   return PPK_OK;
 }
@@ -460,10 +491,13 @@ int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
   HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
   HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
   if (h->fused) {
-    HIP_CHECK(h, ppk::fused_cfl(h->dp, h->cur(), h->fw, h->ctrl, h->stream, h->tm()));
+    HIP_CHECK(h, h->K->fused_cfl(h->dp, h->cur(), h->parts, h->ctrl, h->stream, h->tm()));
+    h->cfl_valid = true;
   } else {
-    HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->ctrl, 1, h->stream, h->tm()));
+    HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->parts, h->ctrl, 1, h->stream, h->tm()));
   }
+  HIP_CHECK(h, h->K->set_dt(h->ctrl, h->parts, ppk::cfl_blocks(h->dp), 0.0,
+                            ppk::SETDT_REDUCE_ONLY, h->p.cfl, h->stream));
   int rc = read_ctrl(h);
   if (rc != PPK_OK) return rc;
   *dt = h->p.cfl / h->ctrl_host[ppk::CTRL_INVDT];
@@ -486,7 +520,8 @@ int ppk_mhd_run(ppk_mhd_t *h, int nsteps, double dt, int dt_mode) {
   if (dt_mode != PPK_DT_FIXED && dt_mode != PPK_DT_CFL)
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "dt_mode must be PPK_DT_FIXED or PPK_DT_CFL");
   // steps after finished() are no-ops on the device; the buffer parity is re-read at the end
-  long long start = h->iter_host;
+  const long long start = h->iter_host;
+  const bool cfl_valid0 = h->cfl_valid;
   for (int s = 0; s < nsteps; ++s) {
     int rc = launch_step(h, dt, dt_mode, 1);
     if (rc != PPK_OK) return rc;
@@ -495,7 +530,8 @@ int ppk_mhd_run(ppk_mhd_t *h, int nsteps, double dt, int dt_mode) {
   int rc = read_ctrl(h);
   if (rc != PPK_OK) return rc;
   h->iter_host = (long long)h->ctrl_host[ppk::CTRL_ITER];
-  (void)start;
+  // no step ran (already finished): the CFL partials are as they were before the call
+  if (h->iter_host == start) h->cfl_valid = cfl_valid0;
   return PPK_OK;
 }
 
@@ -516,6 +552,7 @@ int ppk_mhd_set_time(ppk_mhd_t *h, double t, int64_t iteration) {
   if (rc != PPK_OK) return rc;
   // keep the current data where it is: swap the ping-pong buffers if the parity changes
   if (((long long)iteration & 1) != (h->iter_host & 1)) std::swap(h->U[0], h->U[1]);
+  h->cfl_valid = false;
   h->iter_host = iteration;
   h->ctrl_host[ppk::CTRL_T] = t;
   h->ctrl_host[ppk::CTRL_ITER] = (double)iteration;
@@ -568,10 +605,10 @@ int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max) {
   int rc = ppk_mhd_make_boundaries(h);
   if (rc != PPK_OK) return rc;
   HIP_CHECK(h, ppk::diag_divb(h->dp, h->cur(), h->diag, h->stream));
-  HIP_CHECK(h, hipMemcpyAsync(h->ctrl_host + ppk::CTRL_SLOTS - 1, h->diag, sizeof(double),
+  HIP_CHECK(h, hipMemcpyAsync(h->ctrl_host + ppk::CTRL_SLOTS, h->diag, sizeof(double),
                               hipMemcpyDeviceToHost, h->stream));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));
-  *divb_max = h->ctrl_host[ppk::CTRL_SLOTS - 1];
+  *divb_max = h->ctrl_host[ppk::CTRL_SLOTS];
   return PPK_OK;
 }
 

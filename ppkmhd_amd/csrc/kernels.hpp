@@ -13,12 +13,12 @@ namespace ppk {
 enum {
   CTRL_T = 0,        // current time
   CTRL_DT = 1,       // dt of the step in flight
-  CTRL_INVDT = 2,    // CFL accumulator (bit pattern of a non-negative double, atomicMax)
+  CTRL_INVDT = 2,    // CFL maximum max_cells sum_d (c_f + |v_d|)/dx_d of the state (set_dt)
   CTRL_ITER = 3,     // iteration counter (exact in a double up to 2^53)
   CTRL_TEND = 4,
   CTRL_NSTEPMAX = 5,
   CTRL_ACTIVE = 6,   // 0 once SolverBase::finished() held before a checked step
-  CTRL_INVDT_NEXT = 7,  // CFL accumulator of the state a fused step wrote
+  CTRL_SPARE = 7,
   CTRL_SLOTS = 8
 };
 
@@ -33,8 +33,9 @@ enum Stage {
   ST_EMF,
   ST_UPDATE,
   ST_CONTROL,
-  ST_FUSED_HALF,     // fused: primitives + E + slopes + half step (compact trace data)
-  ST_FUSED_UPDATE,   // fused: Riemann solves + EMFs + update + CT (+ next CFL)
+  ST_FUSED_HALF,     // fused: primitives + E + slopes + half step (compact records)
+  ST_FUSED_RIEMANN,  // fused: Riemann problems at faces + EMFs at edges
+  ST_FUSED_UPDATE,   // fused: update + CT + ghost copy (+ CFL of the new state)
   ST_FUSED_CFL,      // fused: standalone CFL reduction
   ST_COUNT
 };
@@ -60,19 +61,50 @@ struct StagedScratch {
   double *dbf = nullptr;    // 3D: 6 transverse face-field slopes
 };
 
+// scratch of the fused form (padded-box indexing, SoA)
+constexpr int kRec3 = 38;   // half-step record per cell, 3D
+constexpr int kRec2 = 26;   // half-step record per cell, 2D
+constexpr int kFluxComp3 = 5;  // ID IP IU IV IW per face direction, 3D
+constexpr int kFluxComp2 = 6;  // ID IP IU IV IW IC per face direction, 2D
+struct FusedScratch {
+  double *rec = nullptr;   // kRec{2,3} x cells
+  double *flux = nullptr;  // dim x kFluxComp{2,3} x cells
+  double *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
+};
+
+// dt modes of the set_dt kernel
+enum { SETDT_FIXED = 0, SETDT_CFL = 1, SETDT_REDUCE_ONLY = 2 };
+
+// The CFL maximum is produced as per-block partial maxima (no same-address atomics): every
+// producer kernel (staged primitives, fused update, fused CFL) runs exactly cfl_blocks(p)
+// blocks of kCflBlock threads and each block writes its slot; set_dt reduces the slots.
+constexpr int kCflBlock = 256;
+constexpr int kCflMaxBlocks = 4096;
+inline int cfl_blocks(const DevParams &p) {
+  long long cells = (long long)p.isize * p.jsize * p.ksize;
+  long long g = (cells + kCflBlock - 1) / kCflBlock;
+  if (g > kCflMaxBlocks) g = kCflMaxBlocks;
+  return g < 1 ? 1 : (int)g;
+}
+
 #define PPK_DECLARE_KERNELS(NS)                                                                  \
   namespace NS {                                                                                 \
   hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s, \
                                StageTimer *tm);                                                  \
-  hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *ctrl,        \
-                            int do_dt, hipStream_t s, StageTimer *tm);                           \
+  hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *parts,       \
+                            const double *ctrl, int do_dt, hipStream_t s, StageTimer *tm);       \
   hipError_t staged_begin(double *ctrl, int check_finished, hipStream_t s);                      \
-  hipError_t staged_set_dt(double *ctrl, double fixed_dt, int cfl_mode, double cfl,              \
-                           hipStream_t s);                                                       \
+  hipError_t staged_set_dt(double *ctrl, const double *parts, int nparts, double fixed_dt,       \
+                           int mode, double cfl, hipStream_t s);                                 \
   hipError_t staged_end(double *ctrl, hipStream_t s);                                            \
   hipError_t staged_godunov(const DevParams &p, const double *Uin, double *Uout,                 \
                             const double *Q, StagedScratch &w, const double *ctrl,               \
                             hipStream_t s, StageTimer *tm);                                      \
+  hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout,                  \
+                           FusedScratch &w, double *parts, const double *ctrl, int fold_cfl,     \
+                           hipStream_t s, StageTimer *tm);                                       \
+  hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const double *ctrl,   \
+                       hipStream_t s, StageTimer *tm);                                           \
   }
 
 PPK_DECLARE_KERNELS(fast)
@@ -86,6 +118,21 @@ struct KernelTable {
   decltype(&fast::staged_set_dt) set_dt;
   decltype(&fast::staged_end) end;
   decltype(&fast::staged_godunov) godunov;
+  decltype(&fast::fused_godunov) fused_godunov;
+  decltype(&fast::fused_cfl) fused_cfl;
 };
+
+// block-wide maximum of one double per thread (any block size that is a multiple of 64, up
+// to 1024); every thread gets the result
+__device__ __forceinline__ double block_max(double v, double *smem32) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) smem32[wave] = v;
+  __syncthreads();
+  double m = smem32[0];
+  for (int w = 1; w < nw; ++w) m = fmax(m, smem32[w]);
+  __syncthreads();
+  return m;
+}
 
 }  // namespace ppk

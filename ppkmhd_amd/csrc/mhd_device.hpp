@@ -1,6 +1,6 @@
 // Device-side MHD building blocks of the MUSCL-Hancock + HLLD + CT step (gfx950).
 //
-// Every function is a __device__ inline on values held in registers (no local arrays indexed
+// Every function is an inline on values held in registers (no local arrays indexed
 // at run time, so nothing spills to scratch).  Arithmetic is written in the operand order of
 // the reference so that a build with -ffp-contract=off reproduces the reference's double
 // results bit for bit; the default build lets the compiler contract a*b+c into FMA.
@@ -13,6 +13,9 @@
 #include <cstdint>
 
 #include "ppkmhd_amd.h"
+
+// host+device inline: the kernels' per-thread code also builds for the CPU (index checks)
+#define PPK_HD __host__ __device__ __forceinline__
 
 namespace ppk {
 
@@ -39,19 +42,22 @@ struct DevParams {
 template <class R>
 struct St {
   R v[8];
-  __device__ __forceinline__ R &operator[](int i) { return v[i]; }
-  __device__ __forceinline__ const R &operator[](int i) const { return v[i]; }
+  PPK_HD R &operator[](int i) { return v[i]; }
+  PPK_HD const R &operator[](int i) const { return v[i]; }
 };
 
 template <class R>
-__device__ __forceinline__ R dmax(R a, R b) { return fmax(a, b); }
+PPK_HD R dmax(R a, R b) { return fmax(a, b); }
+// sign bit test (true for -0 and negative values), host and device
 template <class R>
-__device__ __forceinline__ R dmin(R a, R b) { return fmin(a, b); }
+PPK_HD bool sign_bit(R x) { return __builtin_signbit(x); }
+template <class R>
+PPK_HD R dmin(R a, R b) { return fmin(a, b); }
 
 // compare-and-select maxima used by the 2D magnetic Riemann solver (FMAX4/FMIN4/FMAX5,
 // reference src/shared/mhd_utils.h:36-77)
 template <class R>
-__device__ __forceinline__ R max4(R a0, R a1, R a2, R a3) {
+PPK_HD R max4(R a0, R a1, R a2, R a3) {
   R r = a0;
   r = (a1 > r) ? a1 : r;
   r = (a2 > r) ? a2 : r;
@@ -59,7 +65,7 @@ __device__ __forceinline__ R max4(R a0, R a1, R a2, R a3) {
   return r;
 }
 template <class R>
-__device__ __forceinline__ R min4(R a0, R a1, R a2, R a3) {
+PPK_HD R min4(R a0, R a1, R a2, R a3) {
   R r = a0;
   r = (a1 < r) ? a1 : r;
   r = (a2 < r) ? a2 : r;
@@ -67,7 +73,7 @@ __device__ __forceinline__ R min4(R a0, R a1, R a2, R a3) {
   return r;
 }
 template <class R>
-__device__ __forceinline__ R max5(R a0, R a1, R a2, R a3, R a4) {
+PPK_HD R max5(R a0, R a1, R a2, R a3, R a4) {
   R r = max4(a0, a1, a2, a3);
   r = (a4 > r) ? a4 : r;
   return r;
@@ -75,7 +81,7 @@ __device__ __forceinline__ R max5(R a0, R a1, R a2, R a3, R a4) {
 
 // fast magnetosonic speed along dir, reference src/shared/mhd_utils.h:89-117
 template <class R>
-__device__ __forceinline__ R speed_fast(const St<R> &q, int dir, R gamma0) {
+PPK_HD R speed_fast(const St<R> &q, int dir, R gamma0) {
   R d = q[ID], p = q[IP], a = q[IA], b = q[IB], c = q[IC];
   R b2 = a * a + b * b + c * c;
   R c2 = gamma0 * p / d;
@@ -86,7 +92,7 @@ __device__ __forceinline__ R speed_fast(const St<R> &q, int dir, R gamma0) {
 
 // per-cell CFL term sum_d (c_f,d + |v_d|)/dx_d, reference src/shared/mhd_utils.h:319-366 and
 // src/muscl/MHDRunFunctors{2D,3D}.h:73/76
-__device__ __forceinline__ double inv_dt_cell(const St<double> &q, const DevParams &p) {
+PPK_HD double inv_dt_cell(const St<double> &q, const DevParams &p) {
   const double gamma0 = p.gamma0;
   double d = q[ID], pp = q[IP], a = q[IA], b = q[IB], c = q[IC];
   double b2 = a * a + b * b + c * c;
@@ -107,7 +113,7 @@ __device__ __forceinline__ double inv_dt_cell(const St<double> &q, const DevPara
 // conservative -> primitive with the face-centred field averaged to the cell centre,
 // reference src/muscl/MHDBaseFunctor3D.h:192-242
 template <class R>
-__device__ __forceinline__ void constoprim(const St<R> &u, R magx, R magy, R magz, St<R> &q,
+PPK_HD void constoprim(const St<R> &u, R magx, R magy, R magz, St<R> &q,
                                            const DevParams &p) {
   q[ID] = dmax(u[ID], R(p.smallr));
   q[IU] = u[IU] / q[ID];
@@ -129,7 +135,7 @@ __device__ __forceinline__ void constoprim(const St<R> &u, R magx, R magy, R mag
 // minmod (slope_type 1) / MC (slope_type 2) limited slope,
 // reference src/muscl/MHDBaseFunctor2D.h:308-327
 template <class R>
-__device__ __forceinline__ R limited_slope(R st, R q, R qplus, R qminus) {
+PPK_HD R limited_slope(R st, R q, R qplus, R qminus) {
   R dlft = st * (q - qminus);
   R drgt = st * (qplus - q);
   R dcen = R(0.5) * (qplus - qminus);
@@ -146,7 +152,7 @@ __device__ __forceinline__ R limited_slope(R st, R q, R qplus, R qminus) {
 // ------------------------------------------------------------------------------------------
 // HLLD, Miyoshi & Kusano (2005) as written in the reference (:133-367)
 template <class R>
-__device__ __forceinline__ void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
+PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
                                              const DevParams &par) {
   const R gamma0 = R(par.gamma0);
   const R entho = R(1.0) / (gamma0 - R(1.0));
@@ -280,7 +286,7 @@ __device__ __forceinline__ void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> 
 // ideal MHD flux and conservative state of a primitive state, reference
 // src/shared/mhd_utils.h:173-231 (used by HLL and LLF)
 template <class R>
-__device__ __forceinline__ void mhd_flux(const St<R> &q, St<R> &cvar, St<R> &ff,
+PPK_HD void mhd_flux(const St<R> &q, St<R> &cvar, St<R> &ff,
                                          const DevParams &par) {
   R gamma0 = R(par.gamma0);
   R p = par.cIso > 0 ? q[ID] * R(par.cIso) * R(par.cIso) : q[IP];
@@ -304,7 +310,7 @@ __device__ __forceinline__ void mhd_flux(const St<R> &q, St<R> &cvar, St<R> &ff,
 
 // HLL, reference src/shared/RiemannSolvers_MHD.h:27-67
 template <class R>
-__device__ __forceinline__ void riemann_hll(St<R> &ql, St<R> &qr, St<R> &flux,
+PPK_HD void riemann_hll(St<R> &ql, St<R> &qr, St<R> &flux,
                                             const DevParams &par) {
   R bx_mean = R(0.5) * (ql[IA] + qr[IA]);
   ql[IA] = bx_mean;
@@ -323,7 +329,7 @@ __device__ __forceinline__ void riemann_hll(St<R> &ql, St<R> &qr, St<R> &flux,
 
 // LLF, reference src/shared/RiemannSolvers_MHD.h:81-111
 template <class R>
-__device__ __forceinline__ void riemann_llf(St<R> &ql, St<R> &qr, St<R> &flux,
+PPK_HD void riemann_llf(St<R> &ql, St<R> &qr, St<R> &flux,
                                             const DevParams &par) {
   R bx_mean = R(0.5) * (ql[IA] + qr[IA]);
   ql[IA] = bx_mean;
@@ -347,7 +353,7 @@ __device__ __forceinline__ void riemann_llf(St<R> &ql, St<R> &qr, St<R> &flux,
 }
 
 template <class R>
-__device__ __forceinline__ void riemann_mhd(St<R> &ql, St<R> &qr, St<R> &flux,
+PPK_HD void riemann_mhd(St<R> &ql, St<R> &qr, St<R> &flux,
                                             const DevParams &p) {
   if (p.riemann == PPK_RIEMANN_HLLD)
     riemann_hlld(ql, qr, flux, p);
@@ -363,7 +369,7 @@ __device__ __forceinline__ void riemann_mhd(St<R> &ql, St<R> &qr, St<R> &flux,
 // mag_riemann2d_hlld (:398-630), with the reference's branch-free 0/1 blend (:569-596): the
 // region masks come from the sign bit (a -0 speed selects the "negative" side).
 template <class R>
-__device__ __forceinline__ R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
+PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
                                                const St<R> &qLR, const St<R> &qRR, R ELL, R ERL,
                                                R ELR, R ERR, const DevParams &p) {
   const R gamma0 = R(p.gamma0);
@@ -464,10 +470,10 @@ __device__ __forceinline__ R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &q
   R BstarL = (SAT * BstarLR - SAB * BstarLL) / (SAT - SAB);
 
   // 0/1 region masks from the sign bit (copysign(1, S) > 0)
-  const R SB_pos = signbit(SB) ? R(0) : R(1), SB_neg = R(1) - SB_pos;
-  const R ST_pos = signbit(ST) ? R(0) : R(1), ST_neg = R(1) - ST_pos;
-  const R SL_pos = signbit(SL) ? R(0) : R(1), SL_neg = R(1) - SL_pos;
-  const R SR_pos = signbit(SR) ? R(0) : R(1), SR_neg = R(1) - SR_pos;
+  const R SB_pos = sign_bit(SB) ? R(0) : R(1), SB_neg = R(1) - SB_pos;
+  const R ST_pos = sign_bit(ST) ? R(0) : R(1), ST_neg = R(1) - ST_pos;
+  const R SL_pos = sign_bit(SL) ? R(0) : R(1), SL_neg = R(1) - SL_pos;
+  const R SR_pos = sign_bit(SR) ? R(0) : R(1), SR_neg = R(1) - SR_pos;
 
   R E = R(0), tmpE;
   tmpE = (SAL * SAB * EstarRR - SAL * SAT * EstarRL - SAR * SAB * EstarLR + SAR * SAT * EstarLL) /
@@ -496,7 +502,7 @@ __device__ __forceinline__ R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &q
 // src/shared/RiemannSolvers_MHD.h:651-874.  Template parameter = (iu, iv, iw, ia, ib, ic)
 // permutation: EMFZ (u,v,w), EMFY (w,u,v), EMFX (v,w,u).
 template <int EDIR, class R>
-__device__ __forceinline__ R compute_emf(const St<R> &qRT, const St<R> &qRB, const St<R> &qLT,
+PPK_HD R compute_emf(const St<R> &qRT, const St<R> &qRB, const St<R> &qLT,
                                          const St<R> &qLB, const DevParams &p) {
   constexpr int iu = EDIR == EMFZ ? IU : (EDIR == EMFY ? IW : IV);
   constexpr int iv = EDIR == EMFZ ? IV : (EDIR == EMFY ? IU : IW);

@@ -84,11 +84,14 @@ __global__ void __launch_bounds__(kBlock) k_boundaries(double *__restrict__ U, D
 }
 
 // ---- primitives over [0, size-1) per direction + CFL max over the interior ----------------
-// reference src/muscl/MHDRunFunctors3D.h:88-163 and :16-83 (reduction fused here)
+// reference src/muscl/MHDRunFunctors3D.h:88-163 and :16-83 (reduction fused here); the max is
+// left as one partial per block in parts[blockIdx.x] (grid = cfl_blocks)
 __global__ void __launch_bounds__(kBlock) k_prim_dt(const double *__restrict__ U,
                                                     double *__restrict__ Q, DevParams p,
-                                                    double *ctrl, int do_dt) {
+                                                    double *__restrict__ parts,
+                                                    const double *ctrl, int do_dt) {
   if (!active(ctrl)) return;
+  __shared__ double red[kBlock / 64];
   const long long total = (long long)p.isize * p.jsize * p.ksize;
   double best = 0.0;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
@@ -111,11 +114,8 @@ __global__ void __launch_bounds__(kBlock) k_prim_dt(const double *__restrict__ U
     }
   }
   if (!do_dt) return;
-  // wavefront max with DPP/permute shuffles, then one atomic per wave on the bit pattern
-  // (non-negative doubles order like their unsigned 64-bit patterns)
-  for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off, 64));
-  if ((threadIdx.x & 63) == 0)
-    atomicMax((unsigned long long *)&ctrl[CTRL_INVDT], (unsigned long long)__double_as_longlong(best));
+  best = block_max(best, red);
+  if (threadIdx.x == 0) parts[blockIdx.x] = best;
 }
 
 // ---- E field at cell edges (3D), reference src/muscl/MHDRunFunctors3D.h:303-356 ----------
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_update(const double *__restrict__ Ui
 
 // ---- time-loop control (single thread) ------------------------------------------------------
 // before a step: stop when SolverBase::finished() holds (src/shared/SolverBase.cpp:195-201)
-// if the step was asked to check; clear the CFL accumulator
+// if the step was asked to check
 __global__ void k_begin(double *ctrl, int check_finished) {
   if (threadIdx.x != 0) return;
   if (check_finished) {
@@ -435,20 +435,28 @@ __global__ void k_begin(double *ctrl, int check_finished) {
   } else {
     ctrl[CTRL_ACTIVE] = 1.0;
   }
-  ctrl[CTRL_INVDT] = 0.0;
 }
-// dt from the CFL reduction or the fixed value, clamped to tEnd - t,
-// src/muscl/SolverMHDMuscl.h:737 and src/shared/SolverBase.cpp:173-177
-__global__ void k_set_dt(double *ctrl, double fixed_dt, int cfl_mode, double cfl) {
-  if (threadIdx.x != 0 || ctrl[CTRL_ACTIVE] == 0.0) return;
-  double dt;
-  if (cfl_mode) {
-    dt = cfl / ctrl[CTRL_INVDT];
-    if (ctrl[CTRL_T] + dt > ctrl[CTRL_TEND]) dt = ctrl[CTRL_TEND] - ctrl[CTRL_T];
-  } else {
-    dt = fixed_dt;
+// dt from the CFL partial maxima or the fixed value, clamped to tEnd - t,
+// src/muscl/SolverMHDMuscl.h:737 and src/shared/SolverBase.cpp:173-177.  One block.
+__global__ void __launch_bounds__(kBlock) k_set_dt(double *ctrl, const double *__restrict__ parts,
+                                                   int nparts, double fixed_dt, int mode,
+                                                   double cfl) {
+  __shared__ double red[kBlock / 64];
+  if (ctrl[CTRL_ACTIVE] == 0.0) return;
+  if (mode == SETDT_FIXED) {
+    if (threadIdx.x == 0) ctrl[CTRL_DT] = fixed_dt;
+    return;
   }
-  ctrl[CTRL_DT] = dt;
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, parts[b]);
+  m = block_max(m, red);
+  if (threadIdx.x != 0) return;
+  ctrl[CTRL_INVDT] = m;
+  if (mode == SETDT_CFL) {
+    double dt = cfl / m;
+    if (ctrl[CTRL_T] + dt > ctrl[CTRL_TEND]) dt = ctrl[CTRL_TEND] - ctrl[CTRL_T];
+    ctrl[CTRL_DT] = dt;
+  }
 }
 // after a step: ++iteration; t += dt (src/shared/SolverBase.cpp:216-218)
 __global__ void k_end(double *ctrl) {
@@ -482,11 +490,12 @@ hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, 
   return hipGetLastError();
 }
 
-hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *ctrl, int do_dt,
-                          hipStream_t s, StageTimer *tm) {
-  long long n = (long long)p.isize * p.jsize * p.ksize;
+hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *parts,
+                          const double *ctrl, int do_dt, hipStream_t s, StageTimer *tm) {
+  static_assert(kBlock == kCflBlock, "CFL partials assume kCflBlock threads per block");
   PPK_TIMED(tm, ST_PRIM_DT,
-            hipLaunchKernelGGL(k_prim_dt, dim3(grid_for(n)), dim3(kBlock), 0, s, U, Q, p, ctrl, do_dt));
+            hipLaunchKernelGGL(k_prim_dt, dim3(cfl_blocks(p)), dim3(kBlock), 0, s, U, Q, p, parts,
+                               ctrl, do_dt));
   return hipGetLastError();
 }
 
@@ -494,8 +503,10 @@ hipError_t staged_begin(double *ctrl, int check_finished, hipStream_t s) {
   hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, ctrl, check_finished);
   return hipGetLastError();
 }
-hipError_t staged_set_dt(double *ctrl, double fixed_dt, int cfl_mode, double cfl, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, s, ctrl, fixed_dt, cfl_mode, cfl);
+hipError_t staged_set_dt(double *ctrl, const double *parts, int nparts, double fixed_dt, int mode,
+                         double cfl, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(kBlock), 0, s, ctrl, parts, nparts, fixed_dt, mode,
+                     cfl);
   return hipGetLastError();
 }
 hipError_t staged_end(double *ctrl, hipStream_t s) {

@@ -8,37 +8,40 @@
 
 namespace ppk {
 
-// outputs of the 2D trace: qm/qp per direction, edge states RT, RB, LT, LB
+// Compact half-step data of one 2D cell: cell centre at t^{n+1/2}, halved slopes, face
+// fields at t^{n+1/2} and their transverse slopes.  Every face and edge state of the cell is
+// an add/sub of these plus the floors.
 template <class R>
-struct Trace2D {
-  St<R> qm[2], qp[2], qe[4];
+struct Half2D {
+  R r, pp, u, v, w, A, B, C;
+  R drx, dpx, dux, dvx, dwx, dBx, dCx;
+  R dry, dpy, duy, dvy, dwy, dAy, dCy;
+  R AL, AR, BL, BR;
+  R dALy, dBLx, dARy, dBRx;
 };
 
-// qNb[di][dj] = Q(i+di-1, j+dj-1); bf[di][dj][c] = face field c of cell (i+di-1, j+dj-1)
+// E_z at the lower-left corner of cell (0,0) from the four cells around it (index m = -1,
+// first index x): u, v averaged over the cells, A, B over the two faces touching the corner
 template <class R>
-__device__ __forceinline__ void trace_cell_2d(const St<R> (&qNb)[3][3], const R (&bf)[4][4][3],
-                                              R dtdx, R dtdy, const DevParams &p,
-                                              Trace2D<R> &out) {
+PPK_HD R elec_z_2d(R umm, R um0, R u0m, R u00, R vmm, R vm0, R v0m, R v00, R A0m, R A00, R Bm0,
+                   R B00) {
+  R u = R(0.25) * (umm + um0 + u0m + u00);
+  R v = R(0.25) * (vmm + vm0 + v0m + v00);
+  R A = R(0.5) * (A0m + A00);
+  R B = R(0.5) * (Bm0 + B00);
+  return u * B - v * A;
+}
+
+// Hancock half step of one 2D cell.  qNb[di][dj] = Q(i+di-1, j+dj-1); bf[di][dj][c] = face
+// field c of cell (i+di-1, j+dj-1); Ez = corner fields ELL (i,j), ELR (i,j+1), ERL (i+1,j),
+// ERR (i+1,j+1).
+template <class R>
+PPK_HD void half_step_2d(const St<R> (&qNb)[3][3], const R (&bf)[4][4][3], R ELL, R ELR, R ERL,
+                         R ERR, R dtdx, R dtdy, const DevParams &p, Half2D<R> &h) {
   const int C = 1;
-  const R smallR = R(p.smallr), smallp = R(p.smallp), gamma = R(p.gamma0);
+  const R gamma = R(p.gamma0);
   const R st = R(p.slope_type);
   const St<R> &q = qNb[C][C];
-
-  R Ez[2][2];
-#pragma unroll
-  for (int di = 0; di < 2; ++di)
-#pragma unroll
-    for (int dj = 0; dj < 2; ++dj) {
-      const int cx = C + di, cy = C + dj;
-      R u = R(0.25) * (qNb[cx - 1][cy - 1][IU] + qNb[cx - 1][cy][IU] + qNb[cx][cy - 1][IU] +
-                       qNb[cx][cy][IU]);
-      R v = R(0.25) * (qNb[cx - 1][cy - 1][IV] + qNb[cx - 1][cy][IV] + qNb[cx][cy - 1][IV] +
-                       qNb[cx][cy][IV]);
-      R A = R(0.5) * (bf[cx][cy - 1][0] + bf[cx][cy][0]);
-      R B = R(0.5) * (bf[cx - 1][cy][1] + bf[cx][cy][1]);
-      Ez[di][dj] = u * B - v * A;
-    }
-  const R ELL = Ez[0][0], ELR = Ez[0][1], ERL = Ez[1][0], ERR = Ez[1][1];
 
   R r = q[ID], pp = q[IP], u = q[IU], v = q[IV], w = q[IW], A = q[IA], B = q[IB], Cc = q[IC];
   R AL = bf[C][C][0], AR = bf[C + 1][C][0], BL = bf[C][C][1], BR = bf[C][C + 1][1];
@@ -60,10 +63,10 @@ __device__ __forceinline__ void trace_cell_2d(const St<R> (&qNb)[3][3], const R 
   R dAy = R(0.5) * limited_slope(st, q[IA], qNb[1][2][IA], qNb[1][0][IA]);
 
   // transverse face-field slopes at (i,j), (i+1,j), (i,j+1) (slope_unsplit_mhd_2d)
-  R dALy = R(0.5) * limited_slope(st, bf[C][C][0], bf[C][C + 1][0], bf[C][C - 1][0]);
-  R dBLx = R(0.5) * limited_slope(st, bf[C][C][1], bf[C + 1][C][1], bf[C - 1][C][1]);
-  R dARy = R(0.5) * limited_slope(st, bf[C + 1][C][0], bf[C + 1][C + 1][0], bf[C + 1][C - 1][0]);
-  R dBRx = R(0.5) * limited_slope(st, bf[C][C + 1][1], bf[C + 1][C + 1][1], bf[C - 1][C + 1][1]);
+  h.dALy = R(0.5) * limited_slope(st, bf[C][C][0], bf[C][C + 1][0], bf[C][C - 1][0]);
+  h.dBLx = R(0.5) * limited_slope(st, bf[C][C][1], bf[C + 1][C][1], bf[C - 1][C][1]);
+  h.dARy = R(0.5) * limited_slope(st, bf[C + 1][C][0], bf[C + 1][C + 1][0], bf[C + 1][C - 1][0]);
+  h.dBRx = R(0.5) * limited_slope(st, bf[C][C + 1][1], bf[C + 1][C + 1][1], bf[C - 1][C + 1][1]);
 
   R dAx = R(0.5) * (AR - AL);
   R dBy = R(0.5) * (BR - BL);
@@ -81,37 +84,90 @@ __device__ __forceinline__ void trace_cell_2d(const St<R> (&qNb)[3][3], const R 
   R sBL0 = -(ERL - ELL) * R(0.5) * dtdx;
   R sBR0 = -(ERR - ELR) * R(0.5) * dtdx;
 
-  r = r + sr0; u = u + su0; v = v + sv0; w = w + sw0; pp = pp + sp0;
-  A = A + sA0; B = B + sB0; Cc = Cc + sC0;
-  AL = AL + sAL0; AR = AR + sAR0; BL = BL + sBL0; BR = BR + sBR0;
+  h.r = r + sr0; h.u = u + su0; h.v = v + sv0; h.w = w + sw0; h.pp = pp + sp0;
+  h.A = A + sA0; h.B = B + sB0; h.C = Cc + sC0;
+  h.AL = AL + sAL0; h.AR = AR + sAR0; h.BL = BL + sBL0; h.BR = BR + sBR0;
+  h.drx = drx; h.dpx = dpx; h.dux = dux; h.dvx = dvx; h.dwx = dwx; h.dBx = dBx; h.dCx = dCx;
+  h.dry = dry; h.dpy = dpy; h.duy = duy; h.dvy = dvy; h.dwy = dwy; h.dAy = dAy; h.dCy = dCy;
+}
 
-  auto fl = [&](St<R> &s) {
-    s[ID] = dmax(smallR, s[ID]);
-    s[IP] = dmax(smallp * s[ID], s[IP]);
-  };
-  St<R> &qp0 = out.qp[0], &qm0 = out.qm[0], &qp1 = out.qp[1], &qm1 = out.qm[1];
-  qp0[ID] = r - drx; qp0[IU] = u - dux; qp0[IV] = v - dvx; qp0[IW] = w - dwx;
-  qp0[IP] = pp - dpx; qp0[IA] = AL; qp0[IB] = B - dBx; qp0[IC] = Cc - dCx; fl(qp0);
-  qm0[ID] = r + drx; qm0[IU] = u + dux; qm0[IV] = v + dvx; qm0[IW] = w + dwx;
-  qm0[IP] = pp + dpx; qm0[IA] = AR; qm0[IB] = B + dBx; qm0[IC] = Cc + dCx; fl(qm0);
-  qp1[ID] = r - dry; qp1[IU] = u - duy; qp1[IV] = v - dvy; qp1[IW] = w - dwy;
-  qp1[IP] = pp - dpy; qp1[IA] = A - dAy; qp1[IB] = BL; qp1[IC] = Cc - dCy; fl(qp1);
-  qm1[ID] = r + dry; qm1[IU] = u + duy; qm1[IV] = v + dvy; qm1[IW] = w + dwy;
-  qm1[IP] = pp + dpy; qm1[IA] = A + dAy; qm1[IB] = BR; qm1[IC] = Cc + dCy; fl(qm1);
+// 2D floors: density with smallr, pressure with smallp * density
+template <class R>
+PPK_HD void floor2(St<R> &s, const DevParams &p) {
+  s[ID] = dmax(R(p.smallr), s[ID]);
+  s[IP] = dmax(R(p.smallp) * s[ID], s[IP]);
+}
 
-  St<R> &qRT = out.qe[IRT], &qRB = out.qe[IRB], &qLT = out.qe[ILT], &qLB = out.qe[ILB];
-  qRT[ID] = r + (+drx + dry); qRT[IU] = u + (+dux + duy); qRT[IV] = v + (+dvx + dvy);
-  qRT[IW] = w + (+dwx + dwy); qRT[IP] = pp + (+dpx + dpy); qRT[IA] = AR + (+dARy);
-  qRT[IB] = BR + (+dBRx); qRT[IC] = Cc + (+dCx + dCy); fl(qRT);
-  qRB[ID] = r + (+drx - dry); qRB[IU] = u + (+dux - duy); qRB[IV] = v + (+dvx - dvy);
-  qRB[IW] = w + (+dwx - dwy); qRB[IP] = pp + (+dpx - dpy); qRB[IA] = AR + (-dARy);
-  qRB[IB] = BL + (+dBLx); qRB[IC] = Cc + (+dCx - dCy); fl(qRB);
-  qLB[ID] = r + (-drx - dry); qLB[IU] = u + (-dux - duy); qLB[IV] = v + (-dvx - dvy);
-  qLB[IW] = w + (-dwx - dwy); qLB[IP] = pp + (-dpx - dpy); qLB[IA] = AL + (-dALy);
-  qLB[IB] = BL + (-dBLx); qLB[IC] = Cc + (-dCx - dCy); fl(qLB);
-  qLT[ID] = r + (-drx + dry); qLT[IU] = u + (-dux + duy); qLT[IV] = v + (-dvx + dvy);
-  qLT[IW] = w + (-dwx + dwy); qLT[IP] = pp + (-dpx + dpy); qLT[IA] = AL + (+dALy);
-  qLT[IB] = BR + (-dBRx); qLT[IC] = Cc + (-dCx + dCy); fl(qLT);
+// 2D face states: dir 0/1, side -1 = qp (right state at the cell's left interface), +1 = qm
+template <class R>
+PPK_HD St<R> face_state_2d(const Half2D<R> &h, int dir, int side, const DevParams &p) {
+  St<R> s;
+  if (dir == 0) {
+    if (side < 0) {
+      s[ID] = h.r - h.drx; s[IU] = h.u - h.dux; s[IV] = h.v - h.dvx; s[IW] = h.w - h.dwx;
+      s[IP] = h.pp - h.dpx; s[IA] = h.AL; s[IB] = h.B - h.dBx; s[IC] = h.C - h.dCx;
+    } else {
+      s[ID] = h.r + h.drx; s[IU] = h.u + h.dux; s[IV] = h.v + h.dvx; s[IW] = h.w + h.dwx;
+      s[IP] = h.pp + h.dpx; s[IA] = h.AR; s[IB] = h.B + h.dBx; s[IC] = h.C + h.dCx;
+    }
+  } else {
+    if (side < 0) {
+      s[ID] = h.r - h.dry; s[IU] = h.u - h.duy; s[IV] = h.v - h.dvy; s[IW] = h.w - h.dwy;
+      s[IP] = h.pp - h.dpy; s[IA] = h.A - h.dAy; s[IB] = h.BL; s[IC] = h.C - h.dCy;
+    } else {
+      s[ID] = h.r + h.dry; s[IU] = h.u + h.duy; s[IV] = h.v + h.dvy; s[IW] = h.w + h.dwy;
+      s[IP] = h.pp + h.dpy; s[IA] = h.A + h.dAy; s[IB] = h.BR; s[IC] = h.C + h.dCy;
+    }
+  }
+  floor2(s, p);
+  return s;
+}
+
+// 2D edge (corner) states RT, RB, LT, LB: x is the first transverse direction, y the second
+template <class R>
+PPK_HD St<R> edge_state_2d(const Half2D<R> &h, int c, const DevParams &p) {
+  const bool r1 = (c == IRT || c == IRB);
+  const bool r2 = (c == IRT || c == ILT);
+  auto cmb = [&](R x0, R d1, R d2) { return x0 + ((r1 ? +d1 : -d1) + (r2 ? d2 : -d2)); };
+  St<R> s;
+  s[ID] = cmb(h.r, h.drx, h.dry); s[IU] = cmb(h.u, h.dux, h.duy); s[IV] = cmb(h.v, h.dvx, h.dvy);
+  s[IW] = cmb(h.w, h.dwx, h.dwy); s[IP] = cmb(h.pp, h.dpx, h.dpy);
+  s[IA] = r1 ? (h.AR + (r2 ? +h.dARy : -h.dARy)) : (h.AL + (r2 ? +h.dALy : -h.dALy));
+  s[IB] = r2 ? (h.BR + (r1 ? +h.dBRx : -h.dBRx)) : (h.BL + (r1 ? +h.dBLx : -h.dBLx));
+  s[IC] = cmb(h.C, h.dCx, h.dCy);
+  floor2(s, p);
+  return s;
+}
+
+// outputs of the 2D trace: qm/qp per direction, edge states RT, RB, LT, LB
+template <class R>
+struct Trace2D {
+  St<R> qm[2], qp[2], qe[4];
+};
+
+// whole 2D trace of one cell from its 3x3 primitive and 4x4 face-field neighbourhoods
+template <class R>
+PPK_HD void trace_cell_2d(const St<R> (&qNb)[3][3], const R (&bf)[4][4][3], R dtdx, R dtdy,
+                          const DevParams &p, Trace2D<R> &out) {
+  R Ez[2][2];
+#pragma unroll
+  for (int di = 0; di < 2; ++di)
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int cx = 1 + di, cy = 1 + dj;
+      Ez[di][dj] = elec_z_2d(qNb[cx - 1][cy - 1][IU], qNb[cx - 1][cy][IU], qNb[cx][cy - 1][IU],
+                             qNb[cx][cy][IU], qNb[cx - 1][cy - 1][IV], qNb[cx - 1][cy][IV],
+                             qNb[cx][cy - 1][IV], qNb[cx][cy][IV], bf[cx][cy - 1][0],
+                             bf[cx][cy][0], bf[cx - 1][cy][1], bf[cx][cy][1]);
+    }
+  Half2D<R> h;
+  half_step_2d(qNb, bf, Ez[0][0], Ez[0][1], Ez[1][0], Ez[1][1], dtdx, dtdy, p, h);
+  out.qp[0] = face_state_2d(h, 0, -1, p);
+  out.qm[0] = face_state_2d(h, 0, +1, p);
+  out.qp[1] = face_state_2d(h, 1, -1, p);
+  out.qm[1] = face_state_2d(h, 1, +1, p);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out.qe[c] = edge_state_2d(h, c, p);
 }
 
 // Compact half-step data of one 3D cell, from which every face state and edge state of the
@@ -131,7 +187,7 @@ struct Half3D {
 // (dbf, layout of MHDRunFunctors3D.h:726-738), edge electric fields E[x|y|z][2][2]
 // (Ex: ELL ELR ERL ERR, Ey: FLL FLR FRL FRR, Ez: GLL GLR GRL GRR)
 template <class R>
-__device__ __forceinline__ void half_step_3d(const St<R> &q, const St<R> &qPX, const St<R> &qMX,
+PPK_HD void half_step_3d(const St<R> &q, const St<R> &qPX, const St<R> &qMX,
                                              const St<R> &qPY, const St<R> &qMY, const St<R> &qPZ,
                                              const St<R> &qMZ, const R (&bfNb)[6], const R (&dbf)[12],
                                              const R (&Ex)[4], const R (&Ey)[4], const R (&Ez)[4],
@@ -197,14 +253,14 @@ __device__ __forceinline__ void half_step_3d(const St<R> &q, const St<R> &qPX, c
 }
 
 template <class R>
-__device__ __forceinline__ void floor3(St<R> &s, const DevParams &p) {
+PPK_HD void floor3(St<R> &s, const DevParams &p) {
   s[ID] = dmax(R(p.smallr), s[ID]);
   s[IP] = dmax(R(p.smallp), s[IP]);  // 3D floors with smallp (not smallp*rho)
 }
 
 // face states: dir 0/1/2, side -1 = qp (right state at left interface), +1 = qm
 template <class R>
-__device__ __forceinline__ St<R> face_state_3d(const Half3D<R> &h, int dir, int side,
+PPK_HD St<R> face_state_3d(const Half3D<R> &h, int dir, int side,
                                                const DevParams &p) {
   St<R> s;
   if (dir == 0) {
@@ -238,7 +294,7 @@ __device__ __forceinline__ St<R> face_state_3d(const Half3D<R> &h, int dir, int 
 
 // edge states: edge direction e (0 X, 1 Y, 2 Z), corner c in IRT, IRB, ILT, ILB
 template <class R>
-__device__ __forceinline__ St<R> edge_state_3d(const Half3D<R> &h, int e, int c,
+PPK_HD St<R> edge_state_3d(const Half3D<R> &h, int e, int c,
                                                const DevParams &p) {
   St<R> s;
   // signs of the two transverse half-slopes for RT, RB, LT, LB: (s1, s2)
