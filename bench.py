@@ -82,6 +82,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_calibration(solver, nbytes):
+    """One device-to-device copy of nbytes of the state through HIP's own copy kernel: a plain
+    copy of known size, for scaling the profiler's FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    dst = C.c_void_p()
+    if hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMalloc failed")
+    try:
+        rc = hip.hipMemcpy(dst, C.c_void_p(solver.device_state_ptr()), C.c_size_t(nbytes), 3)
+        if rc != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("device-to-device copy failed")
+    finally:
+        hip.hipFree(dst)
+
+
 def workload_params(n, path, precision):
     p = make_params(3, n, n, n, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
                     bc=A.BC_PERIODIC, tEnd=1.0e9, n_step_max=1 << 30, kernel_path=path,
@@ -129,6 +146,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp64_strict", "fp32_bulk"), default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one device-to-device copy of the state (profiler calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the MHD step is measured on a single MI355X "
@@ -143,6 +162,10 @@ def main():
     p, pr = workload_params(n, path, prec)
     log(f"[bench] creating {n}^3 solver (path={a.path}, precision={a.precision})")
     s = SolverMHDMuscl(p, pr)
+    if a.copy_calibration:
+        nb = 8 * p.isize * p.jsize * p.ksize * 8
+        copy_calibration(s, nb)
+        log(f"[bench] copy calibration: {nb} bytes device to device")
     log(f"[bench] warmup {a.warmup} steps")
     s.run(a.warmup)
     s.synchronize()

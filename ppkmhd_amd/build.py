@@ -3,8 +3,9 @@
     python -m ppkmhd_amd.build [--jobs N] [--verbose]
 
 Kernel files are compiled once per arithmetic variant: ``fast`` (the compiler may contract
-a*b+c into FMA) and ``strict`` (-ffp-contract=off, bit-identical to the reference's double
-arithmetic).  Every HIP compile disables packed FP32 instructions.
+a*b+c into FMA; repeated denominators are inverted once) and ``strict`` (-ffp-contract=off,
+every division as written: bit-identical to the reference's double arithmetic).  Every HIP
+compile disables packed FP32 instructions.
 """
 import argparse
 import concurrent.futures as cf
@@ -38,9 +39,10 @@ COMMON = [
 # (source, object, extra flags)
 UNITS = [
     ("step_staged.hip", "step_staged_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
-    ("step_staged.hip", "step_staged_strict.o", ["-DPPK_NS=strict", "-ffp-contract=off"]),
+    ("step_staged.hip", "step_staged_strict.o",
+     ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("fused.hip", "fused_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
-    ("fused.hip", "fused_strict.o", ["-DPPK_NS=strict", "-ffp-contract=off"]),
+    ("fused.hip", "fused_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("diag.hip", "diag.o", []),
     ("capi.cpp", "capi.o", ["-x", "hip"]),
     ("config.cpp", "config.o", []),

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -145,7 +146,7 @@ struct ppk_mhd_solver {
   double *ctrl = nullptr;       // device control block
   double *ctrl_host = nullptr;  // pinned mirror (CTRL_SLOTS) + one diagnostics slot
   double *diag = nullptr;       // diagnostics scratch (device)
-  double *parts = nullptr;      // per-block CFL partial maxima (device, kCflMaxBlocks)
+  double *parts = nullptr;      // per-block CFL partial maxima (device)
   ppk::StagedScratch w;
   ppk::FusedScratch fw;
   bool fused = false;
@@ -250,6 +251,9 @@ ppk::DevParams make_dev(const ppk_mhd_params &p) {
   d.smallp = p.smallp;
   d.cIso = p.cIso;
   d.cfl = p.cfl;
+  // tuning knob: PPKMHD_HALF_TILE selects one of the compiled 3D half-step tile shapes
+  const char *tile = std::getenv("PPKMHD_HALF_TILE");
+  d.tile3 = tile ? std::atoi(tile) : 0;
   return d;
 }
 
@@ -275,7 +279,7 @@ int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
   double *in = h->cur(), *out = h->nxt();
   ppk::StageTimer *tm = h->tm();
   const bool cfl = dt_mode == PPK_DT_CFL;
-  const int nparts = ppk::cfl_blocks(p);
+  const int nparts = h->fused ? ppk::fused_cfl_parts(p) : ppk::cfl_blocks(p);
   const int mode = cfl ? ppk::SETDT_CFL : ppk::SETDT_FIXED;
   HIP_CHECK(h, K.begin(h->ctrl, check_finished, s));
   HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s, tm));
@@ -378,7 +382,7 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   };
   bool ok = dalloc(&h->U[0], 8 * n) && dalloc(&h->U[1], 8 * n) &&
             dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->diag, ppk::kDiagBlocks * 8 + 8) &&
-            dalloc(&h->parts, ppk::kCflMaxBlocks);
+            dalloc(&h->parts, std::max<size_t>(ppk::kCflMaxBlocks, ppk::fused_cfl_parts(h->dp)));
   if (ok && !h->fused) {
     const int d = params->dim;
     ok = dalloc(&h->Q, 8 * n) && dalloc(&h->w.faces, (size_t)2 * d * 8 * n) &&
@@ -496,7 +500,8 @@ int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
   } else {
     HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->parts, h->ctrl, 1, h->stream, h->tm()));
   }
-  HIP_CHECK(h, h->K->set_dt(h->ctrl, h->parts, ppk::cfl_blocks(h->dp), 0.0,
+  const int nparts = h->fused ? ppk::fused_cfl_parts(h->dp) : ppk::cfl_blocks(h->dp);
+  HIP_CHECK(h, h->K->set_dt(h->ctrl, h->parts, nparts, 0.0,
                             ppk::SETDT_REDUCE_ONLY, h->p.cfl, h->stream));
   int rc = read_ctrl(h);
   if (rc != PPK_OK) return rc;

@@ -1,12 +1,14 @@
-// Fused form of the MHD step: three kernels per time step in place of the seven stages.
+// Fused form of the MHD step: three stages per time step in place of the reference's seven.
 //
 //   k_half3d / k_half2d        LDS-tiled: primitives, edge electric field, transverse
 //                              face-field slopes and the Hancock half step of a tile of cells;
 //                              one compact record per cell (kRec3 = 38 / kRec2 = 26 doubles)
 //                              instead of the 18 (3D) / 8 (2D) full face and edge states
-//   k_riemann3d / k_riemann2d  per cell: the Riemann problems of its lower faces and the 2D
-//                              HLLD EMFs of its lower edges, the states rebuilt from the
-//                              records of the cell and its lower neighbours.  The right-side
+//   k_face{3,2}d<D>, k_edge{3,2}d<E>
+//                              per cell, one launch per direction: the Riemann problem of its
+//                              lower face, the 2D HLLD EMF of its lower edge, the states
+//                              rebuilt from the records of the cell and its lower neighbours
+//                              (one solve per thread keeps register use low).  The right-side
 //                              face values of a cell (AR, dARy, ...) are bit for bit the
 //                              left-side values of its +1 neighbour, so records hold only the
 //                              left ones.
@@ -137,13 +139,20 @@ struct HalfTile3 {
   PPK_HD static int q(int x, int y, int z) { return x + QX * (y + QY * z); }
   PPK_HD static int e(int x, int y, int z) { return x + EX * (y + EY * z); }
 };
-using Tile3 = HalfTile3<16, 4, 4>;
-static_assert(Tile3::LDS * 8 <= 160 * 1024, "3D half-step tile exceeds the LDS of a CU");
+// compiled 3D tile shapes, selected by DevParams::tile3 (0 is the default)
+using Tile3 = HalfTile3<16, 4, 4>;   // 256 threads, 74 KB LDS
+using Tile3b = HalfTile3<8, 8, 8>;   // 512 threads, 113 KB
+using Tile3c = HalfTile3<16, 8, 4>;  // 512 threads, 123 KB
+using Tile3d = HalfTile3<8, 8, 4>;   // 256 threads, 68 KB
+using Tile3e = HalfTile3<32, 4, 2>;  // 256 threads, 94 KB
+static_assert(Tile3c::LDS * 8 <= 160 * 1024, "3D half-step tile exceeds the LDS of a CU");
 
-// phase 1: face fields A, B, C of cells [b0-1, b0+T+1] (clipped to the box)
+// phase 1, all global loads of the tile: face fields A, B, C of cells [b0-1, b0+T+1]
+// (clipped to the box) into sF, and the five other conservative variables of the primitive
+// cells [b0-1, b0+T] (up to size-gw+1) into their slots of sQ
 template <class T>
-PPK_HD void half3_faces(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
-                        int b0z, int tid, double *sF) {
+PPK_HD void half3_load(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
+                       int b0z, int tid, double *sF, double *sQ) {
   const long long vs = p.vstride;
   for (int n = tid; n < T::NF; n += T::NT) {
     const int x = n % T::FX, r = n / T::FX, y = r % T::FY, z = r / T::FY;
@@ -154,25 +163,30 @@ PPK_HD void half3_faces(const double *__restrict__ U, const DevParams &p, int b0
     sF[T::NF + n] = U[t + IB * vs];
     sF[2 * T::NF + n] = U[t + IC * vs];
   }
-}
-
-// phase 2: primitives of cells [b0-1, b0+T] up to size-gw+1 (MHDRunFunctors3D.h:88-163)
-template <class T>
-PPK_HD void half3_prims(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
-                        int b0z, int tid, const double *sF, double *sQ) {
-  const long long vs = p.vstride;
-  const int hi = 1 - p.gw;  // last cell: size + hi
+  const int hi = 1 - p.gw;  // last primitive cell: size + hi
   for (int n = tid; n < T::NQ; n += T::NT) {
     const int x = n % T::QX, r = n / T::QX, y = r % T::QY, z = r / T::QY;
     const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
     if (i > p.isize + hi || j > p.jsize + hi || k > p.ksize + hi) continue;
     const long long t = cidx(p, i, j, k);
+#pragma unroll
+    for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = U[t + v * vs];
+  }
+}
+
+// phase 2: primitives of cells [b0-1, b0+T] up to size-gw+1 (MHDRunFunctors3D.h:88-163), in
+// place: each thread reads the raw slots of its own cells only
+template <class T>
+PPK_HD void half3_prims(const DevParams &p, int b0x, int b0y, int b0z, int tid, const double *sF,
+                        double *sQ) {
+  const int hi = 1 - p.gw;
+  for (int n = tid; n < T::NQ; n += T::NT) {
+    const int x = n % T::QX, r = n / T::QX, y = r % T::QY, z = r / T::QY;
+    const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
+    if (i > p.isize + hi || j > p.jsize + hi || k > p.ksize + hi) continue;
     St<double> u, q;
-    u[ID] = U[t + ID * vs];
-    u[IP] = U[t + IP * vs];
-    u[IU] = U[t + IU * vs];
-    u[IV] = U[t + IV * vs];
-    u[IW] = U[t + IW * vs];
+#pragma unroll
+    for (int v = ID; v <= IW; ++v) u[v] = sQ[v * T::NQ + n];
     const int f0 = T::f(x, y, z);
     u[IA] = sF[f0];
     u[IB] = sF[T::NF + f0];
@@ -287,9 +301,11 @@ struct HalfTile2 {
 };
 using Tile2 = HalfTile2<32, 8>;
 
+// all global loads of the tile: face fields A, B into sF, the six cell-centred variables
+// (ID..IW and the 2D IC) of the primitive cells into their slots of sQ
 template <class T>
-PPK_HD void half2_faces(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
-                        int tid, double *sF) {
+PPK_HD void half2_load(const double *__restrict__ U, const DevParams &p, int b0x, int b0y, int tid,
+                       double *sF, double *sQ) {
   const long long vs = p.vstride;
   for (int n = tid; n < T::NF; n += T::NT) {
     const int x = n % T::FX, y = n / T::FX;
@@ -299,27 +315,32 @@ PPK_HD void half2_faces(const double *__restrict__ U, const DevParams &p, int b0
     sF[n] = U[t + IA * vs];
     sF[T::NF + n] = U[t + IB * vs];
   }
-}
-
-// primitives (MHDRunFunctors2D.h ComputePrimitivesFunctor): Bz is cell-centred in 2D and is
-// averaged with 0
-template <class T>
-PPK_HD void half2_prims(const double *__restrict__ U, const DevParams &p, int b0x, int b0y, int tid,
-                        const double *sF, double *sQ) {
-  const long long vs = p.vstride;
   const int hi = 1 - p.gw;
   for (int n = tid; n < T::NQ; n += T::NT) {
     const int x = n % T::QX, y = n / T::QX;
     const int i = b0x - 1 + x, j = b0y - 1 + y;
     if (i > p.isize + hi || j > p.jsize + hi) continue;
     const long long t = cidx(p, i, j, 0);
+#pragma unroll
+    for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = U[t + v * vs];
+    sQ[IC * T::NQ + n] = U[t + IC * vs];
+  }
+}
+
+// primitives in place (MHDRunFunctors2D.h ComputePrimitivesFunctor): Bz is cell-centred in 2D
+// and is averaged with 0
+template <class T>
+PPK_HD void half2_prims(const DevParams &p, int b0x, int b0y, int tid, const double *sF,
+                        double *sQ) {
+  const int hi = 1 - p.gw;
+  for (int n = tid; n < T::NQ; n += T::NT) {
+    const int x = n % T::QX, y = n / T::QX;
+    const int i = b0x - 1 + x, j = b0y - 1 + y;
+    if (i > p.isize + hi || j > p.jsize + hi) continue;
     St<double> u, q;
-    u[ID] = U[t + ID * vs];
-    u[IP] = U[t + IP * vs];
-    u[IU] = U[t + IU * vs];
-    u[IV] = U[t + IV * vs];
-    u[IW] = U[t + IW * vs];
-    u[IC] = U[t + IC * vs];
+#pragma unroll
+    for (int v = ID; v <= IW; ++v) u[v] = sQ[v * T::NQ + n];
+    u[IC] = sQ[IC * T::NQ + n];
     const int f0 = T::f(x, y);
     u[IA] = sF[f0];
     u[IB] = sF[T::NF + f0];
@@ -391,143 +412,146 @@ PPK_HD void swap_dir(St<double> &s, int a, int b) {
   s[b] = t;
 }
 
-// reference src/muscl/MHDRunFunctors3D.h:1839-1902 (fluxes), :2183-2230 (EMFs)
-PPK_HD void riemann3_cell(const double *__restrict__ rec, double *__restrict__ flux,
-                          double *__restrict__ emf, const DevParams &p, int i, int j, int k) {
-  const long long vs = p.vstride, sx = 1, sy = p.isize, sz = p.plane;
-  const long long c = cidx(p, i, j, k);
-  auto R = [&](long long t, int f) { return rec[t + f * vs]; };
-  auto store_flux = [&](int d, const St<double> &f) {
-    double *o = flux + (long long)d * kFluxComp3 * vs + c;
-    o[0] = f[ID];
-    o[vs] = f[IP];
-    o[2 * vs] = f[IU];
-    o[3 * vs] = f[IV];
-    o[4 * vs] = f[IW];
-  };
-  Half3D<double> hc;
-  load_rec3(rec, vs, c, hc);
-  // x face: qm of cell c-x (its AR is AL of c) against qp of cell c
-  {
-    Half3D<double> hl;
-    load_rec3(rec, vs, c - sx, hl);
-    hl.AR = R(c, R3_AL);
-    St<double> ql = face_state_3d(hl, 0, +1, p), qr = face_state_3d(hc, 0, -1, p), f;
-    riemann_mhd(ql, qr, f, p);
-    store_flux(0, f);
-  }
-  {
-    Half3D<double> hl;
-    load_rec3(rec, vs, c - sy, hl);
-    hl.BR = R(c, R3_BL);
-    St<double> ql = face_state_3d(hl, 1, +1, p), qr = face_state_3d(hc, 1, -1, p), f;
+// One face of cell c = (i,j,k) (its lower face in direction D): qm of cell c-D (whose right
+// face field is the left one of c) against qp of cell c, rotated so that the normal is x,
+// then the 1D Riemann solver; the 5 (3D) or 6 (2D) flux components are stored rotated.
+// Reference src/muscl/MHDRunFunctors3D.h:1839-1902 and MHDRunFunctors2D.h.
+template <int D>
+PPK_HD void face3(const double *__restrict__ rec, double *__restrict__ flux, const DevParams &p,
+                  long long c) {
+  const long long vs = p.vstride;
+  const long long sd = D == 0 ? 1 : (D == 1 ? (long long)p.isize : p.plane);
+  Half3D<double> hl, hr;
+  load_rec3(rec, vs, c - sd, hl);
+  load_rec3(rec, vs, c, hr);
+  if (D == 0) hl.AR = rec[c + R3_AL * vs];
+  if (D == 1) hl.BR = rec[c + R3_BL * vs];
+  if (D == 2) hl.CR = rec[c + R3_CL * vs];
+  St<double> ql = face_state_3d(hl, D, +1, p), qr = face_state_3d(hr, D, -1, p), f;
+  if (D == 1) {
     swap_dir(ql, IU, IV); swap_dir(ql, IA, IB);
     swap_dir(qr, IU, IV); swap_dir(qr, IA, IB);
-    riemann_mhd(ql, qr, f, p);
-    store_flux(1, f);
-  }
-  {
-    Half3D<double> hl;
-    load_rec3(rec, vs, c - sz, hl);
-    hl.CR = R(c, R3_CL);
-    St<double> ql = face_state_3d(hl, 2, +1, p), qr = face_state_3d(hc, 2, -1, p), f;
+  } else if (D == 2) {
     swap_dir(ql, IU, IW); swap_dir(ql, IA, IC);
     swap_dir(qr, IU, IW); swap_dir(qr, IA, IC);
-    riemann_mhd(ql, qr, f, p);
-    store_flux(2, f);
   }
-  // z edge: RT of c-x-y, RB of c-x, LT of c-y, LB of c
-  {
-    Half3D<double> a, b, d;
+  riemann_mhd(ql, qr, f, p);
+  double *o = flux + (long long)D * kFluxComp3 * vs + c;
+  o[0] = f[ID];
+  o[vs] = f[IP];
+  o[2 * vs] = f[IU];
+  o[3 * vs] = f[IV];
+  o[4 * vs] = f[IW];
+}
+
+// One edge of cell c (its lower edge along direction E = 0 x, 1 y, 2 z): the four edge states
+// of the cells around it, then compute_emf.  Reference src/muscl/MHDRunFunctors3D.h:2183-2230
+// (the y edge takes RB and LT swapped, :2212-2217).
+template <int E>
+PPK_HD void edge3(const double *__restrict__ rec, double *__restrict__ emf, const DevParams &p,
+                  long long c) {
+  const long long vs = p.vstride, sx = 1, sy = p.isize, sz = p.plane;
+  auto R = [&](long long t, int f) { return rec[t + f * vs]; };
+  Half3D<double> a, b, d, h;
+  load_rec3(rec, vs, c, h);
+  if (E == 2) {  // RT of c-x-y, RB of c-x, LT of c-y, LB of c
     load_rec3(rec, vs, c - sx - sy, a);
     a.AR = R(c - sy, R3_AL); a.dARy = R(c - sy, R3_DALY);
     a.BR = R(c - sx, R3_BL); a.dBRx = R(c - sx, R3_DBLX);
     load_rec3(rec, vs, c - sx, b);
-    b.AR = R(c, R3_AL); b.dARy = R(c, R3_DALY);
+    b.AR = h.AL; b.dARy = h.dALy;
     load_rec3(rec, vs, c - sy, d);
-    d.BR = R(c, R3_BL); d.dBRx = R(c, R3_DBLX);
+    d.BR = h.BL; d.dBRx = h.dBLx;
     emf[c + I_EMFZ * vs] =
         compute_emf<EMFZ>(edge_state_3d(a, 2, IRT, p), edge_state_3d(b, 2, IRB, p),
-                          edge_state_3d(d, 2, ILT, p), edge_state_3d(hc, 2, ILB, p), p);
-  }
-  // y edge: RT of c-x-z, LT of c-z, RB of c-x, LB of c (RB and LT swapped, :2212-2217)
-  {
-    Half3D<double> a, b, d;
+                          edge_state_3d(d, 2, ILT, p), edge_state_3d(h, 2, ILB, p), p);
+  } else if (E == 1) {  // RT of c-x-z, LT of c-z, RB of c-x, LB of c
     load_rec3(rec, vs, c - sx - sz, a);
     a.AR = R(c - sz, R3_AL); a.dARz = R(c - sz, R3_DALZ);
     a.CR = R(c - sx, R3_CL); a.dCRx = R(c - sx, R3_DCLX);
     load_rec3(rec, vs, c - sz, b);
-    b.CR = R(c, R3_CL); b.dCRx = R(c, R3_DCLX);
+    b.CR = h.CL; b.dCRx = h.dCLx;
     load_rec3(rec, vs, c - sx, d);
-    d.AR = R(c, R3_AL); d.dARz = R(c, R3_DALZ);
+    d.AR = h.AL; d.dARz = h.dALz;
     emf[c + I_EMFY * vs] =
         compute_emf<EMFY>(edge_state_3d(a, 1, IRT, p), edge_state_3d(b, 1, ILT, p),
-                          edge_state_3d(d, 1, IRB, p), edge_state_3d(hc, 1, ILB, p), p);
-  }
-  // x edge: RT of c-y-z, RB of c-y, LT of c-z, LB of c
-  {
-    Half3D<double> a, b, d;
+                          edge_state_3d(d, 1, IRB, p), edge_state_3d(h, 1, ILB, p), p);
+  } else {  // RT of c-y-z, RB of c-y, LT of c-z, LB of c
     load_rec3(rec, vs, c - sy - sz, a);
     a.BR = R(c - sz, R3_BL); a.dBRz = R(c - sz, R3_DBLZ);
     a.CR = R(c - sy, R3_CL); a.dCRy = R(c - sy, R3_DCLY);
     load_rec3(rec, vs, c - sy, b);
-    b.BR = R(c, R3_BL); b.dBRz = R(c, R3_DBLZ);
+    b.BR = h.BL; b.dBRz = h.dBLz;
     load_rec3(rec, vs, c - sz, d);
-    d.CR = R(c, R3_CL); d.dCRy = R(c, R3_DCLY);
+    d.CR = h.CL; d.dCRy = h.dCLy;
     emf[c + I_EMFX * vs] =
         compute_emf<EMFX>(edge_state_3d(a, 0, IRT, p), edge_state_3d(b, 0, IRB, p),
-                          edge_state_3d(d, 0, ILT, p), edge_state_3d(hc, 0, ILB, p), p);
+                          edge_state_3d(d, 0, ILT, p), edge_state_3d(h, 0, ILB, p), p);
   }
 }
 
-// reference src/muscl/MHDRunFunctors2D.h ComputeFluxesAndStoreFunctor and
-// ComputeEmfAndStoreFunctor (:654-680)
-PPK_HD void riemann2_cell(const double *__restrict__ rec, double *__restrict__ flux,
-                          double *__restrict__ emf, const DevParams &p, int i, int j) {
-  const long long vs = p.vstride, sx = 1, sy = p.isize;
-  const long long c = cidx(p, i, j, 0);
-  auto R = [&](long long t, int f) { return rec[t + f * vs]; };
-  auto store_flux = [&](int d, const St<double> &f) {
-    double *o = flux + (long long)d * kFluxComp2 * vs + c;
-    o[0] = f[ID];
-    o[vs] = f[IP];
-    o[2 * vs] = f[IU];
-    o[3 * vs] = f[IV];
-    o[4 * vs] = f[IW];
-    o[5 * vs] = f[IC];
-  };
-  Half2D<double> hc;
-  load_rec2(rec, vs, c, hc);
-  {
-    Half2D<double> hl;
-    load_rec2(rec, vs, c - sx, hl);
-    hl.AR = R(c, R2_AL);
-    St<double> ql = face_state_2d(hl, 0, +1, p), qr = face_state_2d(hc, 0, -1, p), f;
-    riemann_mhd(ql, qr, f, p);
-    store_flux(0, f);
-  }
-  {
-    Half2D<double> hl;
-    load_rec2(rec, vs, c - sy, hl);
-    hl.BR = R(c, R2_BL);
-    St<double> ql = face_state_2d(hl, 1, +1, p), qr = face_state_2d(hc, 1, -1, p), f;
+// 2D faces (D = 0, 1) and the z edge, reference src/muscl/MHDRunFunctors2D.h
+// ComputeFluxesAndStoreFunctor and ComputeEmfAndStoreFunctor (:654-680)
+template <int D>
+PPK_HD void face2(const double *__restrict__ rec, double *__restrict__ flux, const DevParams &p,
+                  long long c) {
+  const long long vs = p.vstride;
+  const long long sd = D == 0 ? 1 : (long long)p.isize;
+  Half2D<double> hl, hr;
+  load_rec2(rec, vs, c - sd, hl);
+  load_rec2(rec, vs, c, hr);
+  if (D == 0) hl.AR = rec[c + R2_AL * vs];
+  if (D == 1) hl.BR = rec[c + R2_BL * vs];
+  St<double> ql = face_state_2d(hl, D, +1, p), qr = face_state_2d(hr, D, -1, p), f;
+  if (D == 1) {
     swap_dir(ql, IU, IV); swap_dir(ql, IA, IB);
     swap_dir(qr, IU, IV); swap_dir(qr, IA, IB);
-    riemann_mhd(ql, qr, f, p);
-    store_flux(1, f);
   }
-  {
-    Half2D<double> a, b, d;
-    load_rec2(rec, vs, c - sx - sy, a);
-    a.AR = R(c - sy, R2_AL); a.dARy = R(c - sy, R2_DALY);
-    a.BR = R(c - sx, R2_BL); a.dBRx = R(c - sx, R2_DBLX);
-    load_rec2(rec, vs, c - sx, b);
-    b.AR = R(c, R2_AL); b.dARy = R(c, R2_DALY);
-    load_rec2(rec, vs, c - sy, d);
-    d.BR = R(c, R2_BL); d.dBRx = R(c, R2_DBLX);
-    emf[c] = compute_emf<EMFZ>(edge_state_2d(a, IRT, p), edge_state_2d(b, IRB, p),
-                               edge_state_2d(d, ILT, p), edge_state_2d(hc, ILB, p), p);
-  }
+  riemann_mhd(ql, qr, f, p);
+  double *o = flux + (long long)D * kFluxComp2 * vs + c;
+  o[0] = f[ID];
+  o[vs] = f[IP];
+  o[2 * vs] = f[IU];
+  o[3 * vs] = f[IV];
+  o[4 * vs] = f[IW];
+  o[5 * vs] = f[IC];
+}
+
+PPK_HD void edge2(const double *__restrict__ rec, double *__restrict__ emf, const DevParams &p,
+                  long long c) {
+  const long long vs = p.vstride, sx = 1, sy = p.isize;
+  auto R = [&](long long t, int f) { return rec[t + f * vs]; };
+  Half2D<double> a, b, d, h;
+  load_rec2(rec, vs, c, h);
+  load_rec2(rec, vs, c - sx - sy, a);
+  a.AR = R(c - sy, R2_AL); a.dARy = R(c - sy, R2_DALY);
+  a.BR = R(c - sx, R2_BL); a.dBRx = R(c - sx, R2_DBLX);
+  load_rec2(rec, vs, c - sx, b);
+  b.AR = h.AL; b.dARy = h.dALy;
+  load_rec2(rec, vs, c - sy, d);
+  d.BR = h.BL; d.dBRx = h.dBLx;
+  emf[c] = compute_emf<EMFZ>(edge_state_2d(a, IRT, p), edge_state_2d(b, IRB, p),
+                             edge_state_2d(d, ILT, p), edge_state_2d(h, ILB, p), p);
+}
+
+// cell (i,j,k) of thread tid of brick L; false past the covered range
+PPK_HD bool brick_cell(const Brick &b, int L, int tid, int &i, int &j, int &k) {
+  const int ux = L % b.nx, r = L / b.nx, uy = r % b.ny, uz = r / b.ny;
+  i = ux * b.bx + tid % b.bx;
+  j = uy * b.by + (tid / b.bx) % b.by;
+  k = uz * b.bz + tid / (b.bx * b.by);
+  return i < b.mx && j < b.my && k < b.mz;
+}
+
+// the sub-box [gw, size-gw] per direction of the face and edge kernels, in bricks
+inline Brick sub_bricks(const DevParams &p) {
+  return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1);
+}
+// padded-box index of thread tid of sub-box brick L, or -1
+PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int tid) {
+  int i, j, k;
+  if (!brick_cell(b, L, tid, i, j, k)) return -1;
+  return cidx(p, p.gw + i, p.gw + j, p.dim == 3 ? p.gw + k : 0);
 }
 
 // ===========================================================================================
@@ -703,25 +727,26 @@ constexpr int kRiemannBlock = 256;
 
 __device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
 
-__global__ void __launch_bounds__(Tile3::NT) k_half3d(const double *__restrict__ U,
-                                                      double *__restrict__ rec, DevParams p,
-                                                      const double *ctrl, int ntx, int nty) {
+template <class T>
+__global__ void __launch_bounds__(T::NT) k_half3d(const double *__restrict__ U,
+                                                  double *__restrict__ rec, DevParams p,
+                                                  const double *ctrl, int ntx, int nty) {
   if (!active(ctrl)) return;
-  __shared__ double lds[Tile3::LDS];
-  double *sF = lds, *sQ = lds + 3 * Tile3::NF, *sE = sQ + 8 * Tile3::NQ;
+  __shared__ double lds[T::LDS];
+  double *sF = lds, *sQ = lds + 3 * T::NF, *sE = sQ + 8 * T::NQ;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int b0x = p.gw - 1 + (L % ntx) * Tile3::TX;
-  const int b0y = p.gw - 1 + ((L / ntx) % nty) * Tile3::TY;
-  const int b0z = p.gw - 1 + (L / (ntx * nty)) * Tile3::TZ;
+  const int b0x = p.gw - 1 + (L % ntx) * T::TX;
+  const int b0y = p.gw - 1 + ((L / ntx) % nty) * T::TY;
+  const int b0z = p.gw - 1 + (L / (ntx * nty)) * T::TZ;
   const double dt = ctrl[CTRL_DT];
   const int tid = threadIdx.x;
-  half3_faces<Tile3>(U, p, b0x, b0y, b0z, tid, sF);
+  half3_load<T>(U, p, b0x, b0y, b0z, tid, sF, sQ);
   __syncthreads();
-  half3_prims<Tile3>(U, p, b0x, b0y, b0z, tid, sF, sQ);
+  half3_prims<T>(p, b0x, b0y, b0z, tid, sF, sQ);
   __syncthreads();
-  half3_elec<Tile3>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
+  half3_elec<T>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
   __syncthreads();
-  half3_record<Tile3>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
+  half3_record<T>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
 }
 
 __global__ void __launch_bounds__(Tile2::NT) k_half2d(const double *__restrict__ U,
@@ -735,54 +760,64 @@ __global__ void __launch_bounds__(Tile2::NT) k_half2d(const double *__restrict__
   const int b0y = p.gw - 1 + (L / ntx) * Tile2::TY;
   const double dt = ctrl[CTRL_DT];
   const int tid = threadIdx.x;
-  half2_faces<Tile2>(U, p, b0x, b0y, tid, sF);
+  half2_load<Tile2>(U, p, b0x, b0y, tid, sF, sQ);
   __syncthreads();
-  half2_prims<Tile2>(U, p, b0x, b0y, tid, sF, sQ);
+  half2_prims<Tile2>(p, b0x, b0y, tid, sF, sQ);
   __syncthreads();
   half2_elec<Tile2>(p, b0x, b0y, tid, sF, sQ, sE);
   __syncthreads();
   half2_record<Tile2>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
 }
 
-__global__ void __launch_bounds__(kRiemannBlock) k_riemann3d(const double *__restrict__ rec,
-                                                             double *__restrict__ flux,
-                                                             double *__restrict__ emf,
-                                                             DevParams p, const double *ctrl) {
+// one thread per cell of the sub-box (bricks, XCD-contiguous); a launch covers one direction
+template <int D>
+__global__ void __launch_bounds__(kRiemannBlock) k_face3d(const double *__restrict__ rec,
+                                                          double *__restrict__ flux, DevParams p,
+                                                          Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
-  const long long n = (long long)xcd_remap(blockIdx.x, gridDim.x) * kRiemannBlock + threadIdx.x;
-  const int mx = p.nx + 1, my = p.ny + 1, mz = p.nz + 1;
-  if (n >= (long long)mx * my * mz) return;
-  const int i = (int)(n % mx);
-  const long long r = n / mx;
-  riemann3_cell(rec, flux, emf, p, p.gw + i, p.gw + (int)(r % my), p.gw + (int)(r / my));
+  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
+  if (c >= 0) face3<D>(rec, flux, p, c);
+}
+// 4 waves per SIMD: the EMF solve fits 128 VGPRs without spilling
+template <int E>
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_edge3d(const double *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
+         const double *ctrl) {
+  if (!active(ctrl)) return;
+  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
+  if (c >= 0) edge3<E>(rec, emf, p, c);
+}
+template <int D>
+__global__ void __launch_bounds__(kRiemannBlock) k_face2d(const double *__restrict__ rec,
+                                                          double *__restrict__ flux, DevParams p,
+                                                          Brick bk, const double *ctrl) {
+  if (!active(ctrl)) return;
+  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
+  if (c >= 0) face2<D>(rec, flux, p, c);
+}
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_edge2d(const double *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
+         const double *ctrl) {
+  if (!active(ctrl)) return;
+  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
+  if (c >= 0) edge2(rec, emf, p, c);
 }
 
-__global__ void __launch_bounds__(kRiemannBlock) k_riemann2d(const double *__restrict__ rec,
-                                                             double *__restrict__ flux,
-                                                             double *__restrict__ emf,
-                                                             DevParams p, const double *ctrl) {
-  if (!active(ctrl)) return;
-  const long long n = (long long)xcd_remap(blockIdx.x, gridDim.x) * kRiemannBlock + threadIdx.x;
-  const int mx = p.nx + 1, my = p.ny + 1;
-  if (n >= (long long)mx * my) return;
-  riemann2_cell(rec, flux, emf, p, p.gw + (int)(n % mx), p.gw + (int)(n / mx));
-}
-
+// one block per brick of the padded box; the block's CFL partial goes to parts[blockIdx.x]
 __global__ void __launch_bounds__(kCflBlock) k_update(const double *__restrict__ Uin,
                                                       double *__restrict__ Uout,
                                                       const double *__restrict__ flux,
                                                       const double *__restrict__ emf,
                                                       double *__restrict__ parts, DevParams p,
-                                                      const double *ctrl, int fold) {
+                                                      Brick bk, const double *ctrl, int fold) {
   if (!active(ctrl)) return;
   __shared__ double red[kCflBlock / 64];
   const double dt = ctrl[CTRL_DT];
   const double dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = p.dim == 3 ? dt / p.dz : 0.0;
-  const long long total = (long long)p.isize * p.jsize * p.ksize;
+  int i, j, k;
   double best = 0.0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x)
-    best = fmax(best, update_cell(Uin, Uout, flux, emf, p, t, dtdx, dtdy, dtdz, fold != 0));
+  if (brick_cell(bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x, i, j, k))
+    best = update_cell(Uin, Uout, flux, emf, p, cidx(p, i, j, k), dtdx, dtdy, dtdz, fold != 0);
   if (!fold) return;
   best = block_max(best, red);
   if (threadIdx.x == 0) parts[blockIdx.x] = best;
@@ -790,19 +825,27 @@ __global__ void __launch_bounds__(kCflBlock) k_update(const double *__restrict__
 
 __global__ void __launch_bounds__(kCflBlock) k_cfl(const double *__restrict__ U,
                                                    double *__restrict__ parts, DevParams p,
-                                                   const double *ctrl) {
+                                                   Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
   __shared__ double red[kCflBlock / 64];
-  const long long total = (long long)p.isize * p.jsize * p.ksize;
+  int i, j, k;
   double best = 0.0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x)
-    best = fmax(best, cfl_cell(U, p, t));
+  if (brick_cell(bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x, i, j, k))
+    best = cfl_cell(U, p, cidx(p, i, j, k));
   best = block_max(best, red);
   if (threadIdx.x == 0) parts[blockIdx.x] = best;
 }
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+template <class T>
+void launch_half3(const DevParams &p, const double *Uin, double *rec, const double *ctrl,
+                  hipStream_t s) {
+  const int ntx = ceil_div(p.nx + 2, T::TX), nty = ceil_div(p.ny + 2, T::TY);
+  const int ntz = ceil_div(p.nz + 2, T::TZ);
+  hipLaunchKernelGGL(k_half3d<T>, dim3(ntx * nty * ntz), dim3(T::NT), 0, s, Uin, rec, p, ctrl,
+                     ntx, nty);
+}
 
 }  // namespace fusedk
 
@@ -814,28 +857,41 @@ hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout, Fu
                          StageTimer *tm) {
   using namespace fusedk;
   if (p.dim == 3) {
-    const int ntx = ceil_div(p.nx + 2, Tile3::TX), nty = ceil_div(p.ny + 2, Tile3::TY);
-    const int ntz = ceil_div(p.nz + 2, Tile3::TZ);
-    PPK_TIMED(tm, ST_FUSED_HALF,
-              hipLaunchKernelGGL(k_half3d, dim3(ntx * nty * ntz), dim3(Tile3::NT), 0, s, Uin,
-                                 w.rec, p, ctrl, ntx, nty));
-    const long long m = (long long)(p.nx + 1) * (p.ny + 1) * (p.nz + 1);
-    PPK_TIMED(tm, ST_FUSED_RIEMANN,
-              hipLaunchKernelGGL(k_riemann3d, dim3(ceil_div(m, kRiemannBlock)),
-                                 dim3(kRiemannBlock), 0, s, w.rec, w.flux, w.emf, p, ctrl));
+    if (tm) tm->mark(ST_FUSED_HALF, true);
+    switch (p.tile3) {
+      case 1: launch_half3<Tile3b>(p, Uin, w.rec, ctrl, s); break;
+      case 2: launch_half3<Tile3c>(p, Uin, w.rec, ctrl, s); break;
+      case 3: launch_half3<Tile3d>(p, Uin, w.rec, ctrl, s); break;
+      case 4: launch_half3<Tile3e>(p, Uin, w.rec, ctrl, s); break;
+      default: launch_half3<Tile3>(p, Uin, w.rec, ctrl, s); break;
+    }
+    if (tm) tm->mark(ST_FUSED_HALF, false);
+    const Brick bk = sub_bricks(p);
+    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
+    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
+    hipLaunchKernelGGL(k_face3d<0>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
+    hipLaunchKernelGGL(k_face3d<1>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
+    hipLaunchKernelGGL(k_face3d<2>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
+    hipLaunchKernelGGL(k_edge3d<2>, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
+    hipLaunchKernelGGL(k_edge3d<1>, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
+    hipLaunchKernelGGL(k_edge3d<0>, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
+    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
   } else {
     const int ntx = ceil_div(p.nx + 2, Tile2::TX), nty = ceil_div(p.ny + 2, Tile2::TY);
     PPK_TIMED(tm, ST_FUSED_HALF,
               hipLaunchKernelGGL(k_half2d, dim3(ntx * nty), dim3(Tile2::NT), 0, s, Uin, w.rec,
                                  p, ctrl, ntx));
-    const long long m = (long long)(p.nx + 1) * (p.ny + 1);
-    PPK_TIMED(tm, ST_FUSED_RIEMANN,
-              hipLaunchKernelGGL(k_riemann2d, dim3(ceil_div(m, kRiemannBlock)),
-                                 dim3(kRiemannBlock), 0, s, w.rec, w.flux, w.emf, p, ctrl));
+    const Brick bk = sub_bricks(p);
+    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
+    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
+    hipLaunchKernelGGL(k_face2d<0>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
+    hipLaunchKernelGGL(k_face2d<1>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
+    hipLaunchKernelGGL(k_edge2d, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
+    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
   }
   PPK_TIMED(tm, ST_FUSED_UPDATE,
-            hipLaunchKernelGGL(k_update, dim3(cfl_blocks(p)), dim3(kCflBlock), 0, s, Uin, Uout,
-                               w.flux, w.emf, parts, p, ctrl, fold_cfl));
+            hipLaunchKernelGGL(k_update, dim3(fused_cfl_parts(p)), dim3(kCflBlock), 0, s, Uin,
+                               Uout, w.flux, w.emf, parts, p, padded_bricks(p), ctrl, fold_cfl));
   return hipGetLastError();
 }
 
@@ -843,8 +899,8 @@ hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const d
                      hipStream_t s, StageTimer *tm) {
   using namespace fusedk;
   PPK_TIMED(tm, ST_FUSED_CFL,
-            hipLaunchKernelGGL(k_cfl, dim3(cfl_blocks(p)), dim3(kCflBlock), 0, s, U, parts, p,
-                               ctrl));
+            hipLaunchKernelGGL(k_cfl, dim3(fused_cfl_parts(p)), dim3(kCflBlock), 0, s, U, parts, p,
+                               padded_bricks(p), ctrl));
   return hipGetLastError();
 }
 

@@ -75,9 +75,10 @@ struct FusedScratch {
 // dt modes of the set_dt kernel
 enum { SETDT_FIXED = 0, SETDT_CFL = 1, SETDT_REDUCE_ONLY = 2 };
 
-// The CFL maximum is produced as per-block partial maxima (no same-address atomics): every
-// producer kernel (staged primitives, fused update, fused CFL) runs exactly cfl_blocks(p)
-// blocks of kCflBlock threads and each block writes its slot; set_dt reduces the slots.
+// The CFL maximum is produced as per-block partial maxima (no same-address atomics): each
+// block of a producer kernel writes its slot, set_dt reduces as many slots as the producer had
+// blocks.  Staged primitives: cfl_blocks(p) blocks (grid-stride); fused update and fused CFL:
+// one block per brick of the padded box, fused_cfl_parts(p).
 constexpr int kCflBlock = 256;
 constexpr int kCflMaxBlocks = 4096;
 inline int cfl_blocks(const DevParams &p) {
@@ -86,6 +87,32 @@ inline int cfl_blocks(const DevParams &p) {
   if (g > kCflMaxBlocks) g = kCflMaxBlocks;
   return g < 1 ? 1 : (int)g;
 }
+
+// Bricks: the fused per-cell kernels give each block of 256 threads a brick of cells,
+// 16 x 4 x 4 in 3D (a wave is a 16 x 4 slab of one z plane: four 128-byte rows per array) and
+// 64 x 4 in 2D, so that most lower neighbours a cell reads belong to its own brick.
+struct Brick {
+  int bx, by, bz;  // brick extents
+  int nx, ny, nz;  // bricks per direction
+  int mx, my, mz;  // cells covered per direction
+  long long count() const { return (long long)nx * ny * nz; }
+};
+inline Brick make_brick(int dim, int mx, int my, int mz) {
+  Brick b;
+  b.bx = dim == 3 ? 16 : 64;
+  b.by = 4;
+  b.bz = dim == 3 ? 4 : 1;
+  b.mx = mx;
+  b.my = my;
+  b.mz = mz;
+  b.nx = (mx + b.bx - 1) / b.bx;
+  b.ny = (my + b.by - 1) / b.by;
+  b.nz = (mz + b.bz - 1) / b.bz;
+  return b;
+}
+// bricks of the padded box (fused update, fused CFL)
+inline Brick padded_bricks(const DevParams &p) { return make_brick(p.dim, p.isize, p.jsize, p.ksize); }
+inline int fused_cfl_parts(const DevParams &p) { return (int)padded_bricks(p).count(); }
 
 #define PPK_DECLARE_KERNELS(NS)                                                                  \
   namespace NS {                                                                                 \

@@ -37,6 +37,7 @@ struct DevParams {
   long long vstride;    // isize * jsize * ksize (distance between variables)
   double dx, dy, dz;
   double gamma0, slope_type, smallr, smallc, smallp, cIso, cfl;
+  int tile3;            // 3D half-step tile shape (index into the compiled shapes), tuning only
 };
 
 template <class R>
@@ -51,6 +52,20 @@ PPK_HD R dmax(R a, R b) { return fmax(a, b); }
 // sign bit test (true for -0 and negative values), host and device
 template <class R>
 PPK_HD bool sign_bit(R x) { return __builtin_signbit(x); }
+
+// Division.  Built with PPK_EXACT_DIV (the strict variant) every a / b is evaluated as
+// written.  Otherwise a denominator used several times is inverted once (one correctly
+// rounded 1/b) and each quotient becomes a * (1/b): within one rounding of a / b, and an FP64
+// division costs about ten instructions on CDNA against one for a multiply.
+#ifdef PPK_EXACT_DIV
+constexpr bool kSharedRecip = false;
+#else
+constexpr bool kSharedRecip = true;
+#endif
+template <class R>
+PPK_HD R recip(R b) { return kSharedRecip ? R(1) / b : R(0); }
+template <class R>
+PPK_HD R divr(R a, R b, R inv_b) { return kSharedRecip ? a * inv_b : a / b; }
 template <class R>
 PPK_HD R dmin(R a, R b) { return fmin(a, b); }
 
@@ -83,11 +98,12 @@ PPK_HD R max5(R a0, R a1, R a2, R a3, R a4) {
 template <class R>
 PPK_HD R speed_fast(const St<R> &q, int dir, R gamma0) {
   R d = q[ID], p = q[IP], a = q[IA], b = q[IB], c = q[IC];
+  const R id = recip(d);
   R b2 = a * a + b * b + c * c;
-  R c2 = gamma0 * p / d;
-  R d2 = R(0.5) * (b2 / d + c2);
+  R c2 = divr(gamma0 * p, d, id);
+  R d2 = R(0.5) * (divr(b2, d, id) + c2);
   R bn = dir == IX ? a : (dir == IY ? b : c);
-  return sqrt(d2 + sqrt(d2 * d2 - c2 * bn * bn / d));
+  return sqrt(d2 + sqrt(d2 * d2 - divr(c2 * bn * bn, d, id)));
 }
 
 // per-cell CFL term sum_d (c_f,d + |v_d|)/dx_d, reference src/shared/mhd_utils.h:319-366 and
@@ -116,9 +132,10 @@ template <class R>
 PPK_HD void constoprim(const St<R> &u, R magx, R magy, R magz, St<R> &q,
                                            const DevParams &p) {
   q[ID] = dmax(u[ID], R(p.smallr));
-  q[IU] = u[IU] / q[ID];
-  q[IV] = u[IV] / q[ID];
-  q[IW] = u[IW] / q[ID];
+  const R ir = recip(q[ID]);
+  q[IU] = divr(u[IU], q[ID], ir);
+  q[IV] = divr(u[IV], q[ID], ir);
+  q[IW] = divr(u[IW], q[ID], ir);
   q[IA] = R(0.5) * (u[IA] + magx);
   q[IB] = R(0.5) * (u[IB] + magy);
   q[IC] = R(0.5) * (u[IC] + magz);
@@ -127,7 +144,7 @@ PPK_HD void constoprim(const St<R> &u, R magx, R magy, R magz, St<R> &q,
   if (p.cIso > 0) {
     q[IP] = q[ID] * R(p.cIso) * R(p.cIso);
   } else {
-    R eint = (u[IP] - emag) / q[ID] - eken;
+    R eint = divr(u[IP] - emag, q[ID], ir) - eken;
     q[IP] = dmax((R(p.gamma0) - R(1.0)) * q[ID] * eint, q[ID] * R(p.smallp));
   }
 }
@@ -191,63 +208,70 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   R rcl = rl * (ul - sl);
   R rcr = rr * (sr - ur);
 
-  R ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
-  R ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
+  const R irc = recip(rcr + rcl);
+  R ustar = divr(rcr * ur + rcl * ul + (ptotl - ptotr), rcr + rcl, irc);
+  R ptotstar = divr(rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur), rcr + rcl, irc);
 
-  R rstarl = rl * (sl - ul) / (sl - ustar);
+  const R ia2 = recip(a * a);
+  const R isl = recip(sl - ustar);
+  R rstarl = divr(rl * (sl - ul), sl - ustar, isl);
   R estar = rl * (sl - ul) * (sl - ustar) - a * a;
   R el = rl * (sl - ul) * (sl - ul) - a * a;
   R vstarl, wstarl, bstarl, cstarl;
-  if (a * a > R(0) && fabs(estar / (a * a) - R(1)) <= R(1e-8)) {
+  if (a * a > R(0) && fabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
     vstarl = vl;
     bstarl = bl;
     wstarl = wl;
     cstarl = cl;
   } else {
-    vstarl = vl - a * bl * (ustar - ul) / estar;
-    bstarl = bl * el / estar;
-    wstarl = wl - a * cl * (ustar - ul) / estar;
-    cstarl = cl * el / estar;
+    const R ie = recip(estar);
+    vstarl = vl - divr(a * bl * (ustar - ul), estar, ie);
+    bstarl = divr(bl * el, estar, ie);
+    wstarl = wl - divr(a * cl * (ustar - ul), estar, ie);
+    cstarl = divr(cl * el, estar, ie);
   }
   R vdotbstarl = ustar * a + vstarl * bstarl + wstarl * cstarl;
-  R etotstarl = ((sl - ul) * etotl - ptotl * ul + ptotstar * ustar + a * (vdotbl - vdotbstarl)) /
-                (sl - ustar);
+  R etotstarl = divr((sl - ul) * etotl - ptotl * ul + ptotstar * ustar + a * (vdotbl - vdotbstarl),
+                     sl - ustar, isl);
   R sqrrstarl = sqrt(rstarl);
   R calfvenl = fabs(a) / sqrrstarl;
   R sal = ustar - calfvenl;
 
-  R rstarr = rr * (sr - ur) / (sr - ustar);
+  const R isr = recip(sr - ustar);
+  R rstarr = divr(rr * (sr - ur), sr - ustar, isr);
   estar = rr * (sr - ur) * (sr - ustar) - a * a;
   R er = rr * (sr - ur) * (sr - ur) - a * a;
   R vstarr, wstarr, bstarr, cstarr;
-  if (a * a > R(0) && fabs(estar / (a * a) - R(1)) <= R(1e-8)) {
+  if (a * a > R(0) && fabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
     vstarr = vr;
     bstarr = br;
     wstarr = wr;
     cstarr = cr;
   } else {
-    vstarr = vr - a * br * (ustar - ur) / estar;
-    bstarr = br * er / estar;
-    wstarr = wr - a * cr * (ustar - ur) / estar;
-    cstarr = cr * er / estar;
+    const R ie = recip(estar);
+    vstarr = vr - divr(a * br * (ustar - ur), estar, ie);
+    bstarr = divr(br * er, estar, ie);
+    wstarr = wr - divr(a * cr * (ustar - ur), estar, ie);
+    cstarr = divr(cr * er, estar, ie);
   }
   R vdotbstarr = ustar * a + vstarr * bstarr + wstarr * cstarr;
-  R etotstarr = ((sr - ur) * etotr - ptotr * ur + ptotstar * ustar + a * (vdotbr - vdotbstarr)) /
-                (sr - ustar);
+  R etotstarr = divr((sr - ur) * etotr - ptotr * ur + ptotstar * ustar + a * (vdotbr - vdotbstarr),
+                     sr - ustar, isr);
   R sqrrstarr = sqrt(rstarr);
   R calfvenr = fabs(a) / sqrrstarr;
   R sar = ustar + calfvenr;
 
-  R vstarstar =
-      (sqrrstarl * vstarl + sqrrstarr * vstarr + sgnm * (bstarr - bstarl)) / (sqrrstarl + sqrrstarr);
-  R wstarstar =
-      (sqrrstarl * wstarl + sqrrstarr * wstarr + sgnm * (cstarr - cstarl)) / (sqrrstarl + sqrrstarr);
-  R bstarstar =
-      (sqrrstarl * bstarr + sqrrstarr * bstarl + sgnm * sqrrstarl * sqrrstarr * (vstarr - vstarl)) /
-      (sqrrstarl + sqrrstarr);
-  R cstarstar =
-      (sqrrstarl * cstarr + sqrrstarr * cstarl + sgnm * sqrrstarl * sqrrstarr * (wstarr - wstarl)) /
-      (sqrrstarl + sqrrstarr);
+  const R isq = recip(sqrrstarl + sqrrstarr);
+  R vstarstar = divr(sqrrstarl * vstarl + sqrrstarr * vstarr + sgnm * (bstarr - bstarl),
+                     sqrrstarl + sqrrstarr, isq);
+  R wstarstar = divr(sqrrstarl * wstarl + sqrrstarr * wstarr + sgnm * (cstarr - cstarl),
+                     sqrrstarl + sqrrstarr, isq);
+  R bstarstar = divr(sqrrstarl * bstarr + sqrrstarr * bstarl +
+                         sgnm * sqrrstarl * sqrrstarr * (vstarr - vstarl),
+                     sqrrstarl + sqrrstarr, isq);
+  R cstarstar = divr(sqrrstarl * cstarr + sqrrstarr * cstarl +
+                         sgnm * sqrrstarl * sqrrstarr * (wstarr - wstarl),
+                     sqrrstarl + sqrrstarr, isq);
   R vdotbstarstar = ustar * a + vstarstar * bstarstar + wstarstar * cstarstar;
   R etotstarstarl = etotstarl - sgnm * sqrrstarl * (vdotbstarl - vdotbstarstar);
   R etotstarstarr = etotstarr + sgnm * sqrrstarr * (vdotbstarr - vdotbstarstar);
@@ -413,38 +437,41 @@ PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
              (PtotLL - PtotLR + PtotRL - PtotRR)) /
             (rcLLy + rcLRy + rcRLy + rcRRy);
 
-  R rstarLLx = rLL * (SL - uLL) / (SL - ustar);
-  R BstarLL = bLL * (SL - uLL) / (SL - ustar);
-  R rstarLLy = rLL * (SB - vLL) / (SB - vstar);
-  R AstarLL = aLL * (SB - vLL) / (SB - vstar);
-  R rstarLL = rLL * (SL - uLL) / (SL - ustar) * (SB - vLL) / (SB - vstar);
+  const R iSLu = recip(SL - ustar), iSRu = recip(SR - ustar);
+  const R iSBv = recip(SB - vstar), iSTv = recip(ST - vstar);
+
+  R rstarLLx = divr(rLL * (SL - uLL), SL - ustar, iSLu);
+  R BstarLL = divr(bLL * (SL - uLL), SL - ustar, iSLu);
+  R rstarLLy = divr(rLL * (SB - vLL), SB - vstar, iSBv);
+  R AstarLL = divr(aLL * (SB - vLL), SB - vstar, iSBv);
+  R rstarLL = divr(divr(rLL * (SL - uLL), SL - ustar, iSLu) * (SB - vLL), SB - vstar, iSBv);
   R EstarLLx = ustar * BstarLL - vLL * aLL;
   R EstarLLy = uLL * bLL - vstar * AstarLL;
   R EstarLL = ustar * BstarLL - vstar * AstarLL;
 
-  R rstarLRx = rLR * (SL - uLR) / (SL - ustar);
-  R BstarLR = bLR * (SL - uLR) / (SL - ustar);
-  R rstarLRy = rLR * (ST - vLR) / (ST - vstar);
-  R AstarLR = aLR * (ST - vLR) / (ST - vstar);
-  R rstarLR = rLR * (SL - uLR) / (SL - ustar) * (ST - vLR) / (ST - vstar);
+  R rstarLRx = divr(rLR * (SL - uLR), SL - ustar, iSLu);
+  R BstarLR = divr(bLR * (SL - uLR), SL - ustar, iSLu);
+  R rstarLRy = divr(rLR * (ST - vLR), ST - vstar, iSTv);
+  R AstarLR = divr(aLR * (ST - vLR), ST - vstar, iSTv);
+  R rstarLR = divr(divr(rLR * (SL - uLR), SL - ustar, iSLu) * (ST - vLR), ST - vstar, iSTv);
   R EstarLRx = ustar * BstarLR - vLR * aLR;
   R EstarLRy = uLR * bLR - vstar * AstarLR;
   R EstarLR = ustar * BstarLR - vstar * AstarLR;
 
-  R rstarRLx = rRL * (SR - uRL) / (SR - ustar);
-  R BstarRL = bRL * (SR - uRL) / (SR - ustar);
-  R rstarRLy = rRL * (SB - vRL) / (SB - vstar);
-  R AstarRL = aRL * (SB - vRL) / (SB - vstar);
-  R rstarRL = rRL * (SR - uRL) / (SR - ustar) * (SB - vRL) / (SB - vstar);
+  R rstarRLx = divr(rRL * (SR - uRL), SR - ustar, iSRu);
+  R BstarRL = divr(bRL * (SR - uRL), SR - ustar, iSRu);
+  R rstarRLy = divr(rRL * (SB - vRL), SB - vstar, iSBv);
+  R AstarRL = divr(aRL * (SB - vRL), SB - vstar, iSBv);
+  R rstarRL = divr(divr(rRL * (SR - uRL), SR - ustar, iSRu) * (SB - vRL), SB - vstar, iSBv);
   R EstarRLx = ustar * BstarRL - vRL * aRL;
   R EstarRLy = uRL * bRL - vstar * AstarRL;
   R EstarRL = ustar * BstarRL - vstar * AstarRL;
 
-  R rstarRRx = rRR * (SR - uRR) / (SR - ustar);
-  R BstarRR = bRR * (SR - uRR) / (SR - ustar);
-  R rstarRRy = rRR * (ST - vRR) / (ST - vstar);
-  R AstarRR = aRR * (ST - vRR) / (ST - vstar);
-  R rstarRR = rRR * (SR - uRR) / (SR - ustar) * (ST - vRR) / (ST - vstar);
+  R rstarRRx = divr(rRR * (SR - uRR), SR - ustar, iSRu);
+  R BstarRR = divr(bRR * (SR - uRR), SR - ustar, iSRu);
+  R rstarRRy = divr(rRR * (ST - vRR), ST - vstar, iSTv);
+  R AstarRR = divr(aRR * (ST - vRR), ST - vstar, iSTv);
+  R rstarRR = divr(divr(rRR * (SR - uRR), SR - ustar, iSRu) * (ST - vRR), ST - vstar, iSTv);
   R EstarRRx = ustar * BstarRR - vRR * aRR;
   R EstarRRy = uRR * bRR - vstar * AstarRR;
   R EstarRR = ustar * BstarRR - vstar * AstarRR;
@@ -464,10 +491,11 @@ PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
   R SAB = dmin(vstar - calfvenB, R(0));
   R SAT = dmax(vstar + calfvenT, R(0));
 
-  R AstarT = (SAR * AstarRR - SAL * AstarLR) / (SAR - SAL);
-  R AstarB = (SAR * AstarRL - SAL * AstarLL) / (SAR - SAL);
-  R BstarR = (SAT * BstarRR - SAB * BstarRL) / (SAT - SAB);
-  R BstarL = (SAT * BstarLR - SAB * BstarLL) / (SAT - SAB);
+  const R iSAx = recip(SAR - SAL), iSAy = recip(SAT - SAB);
+  R AstarT = divr(SAR * AstarRR - SAL * AstarLR, SAR - SAL, iSAx);
+  R AstarB = divr(SAR * AstarRL - SAL * AstarLL, SAR - SAL, iSAx);
+  R BstarR = divr(SAT * BstarRR - SAB * BstarRL, SAT - SAB, iSAy);
+  R BstarL = divr(SAT * BstarLR - SAB * BstarLL, SAT - SAB, iSAy);
 
   // 0/1 region masks from the sign bit (copysign(1, S) > 0)
   const R SB_pos = sign_bit(SB) ? R(0) : R(1), SB_neg = R(1) - SB_pos;
@@ -476,23 +504,26 @@ PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
   const R SR_pos = sign_bit(SR) ? R(0) : R(1), SR_neg = R(1) - SR_pos;
 
   R E = R(0), tmpE;
-  tmpE = (SAL * SAB * EstarRR - SAL * SAT * EstarRL - SAR * SAB * EstarLR + SAR * SAT * EstarLL) /
-             (SAR - SAL) / (SAT - SAB) -
-         SAT * SAB / (SAT - SAB) * (AstarT - AstarB) + SAR * SAL / (SAR - SAL) * (BstarR - BstarL);
+  tmpE = divr(divr(SAL * SAB * EstarRR - SAL * SAT * EstarRL - SAR * SAB * EstarLR +
+                        SAR * SAT * EstarLL,
+                    SAR - SAL, iSAx),
+              SAT - SAB, iSAy) -
+         divr(SAT * SAB, SAT - SAB, iSAy) * (AstarT - AstarB) +
+         divr(SAR * SAL, SAR - SAL, iSAx) * (BstarR - BstarL);
   E += (SB_neg * ST_pos * SL_neg * SR_pos) * tmpE;
 
-  tmpE = (SAR * EstarLLx - SAL * EstarRLx + SAR * SAL * (bRL - bLL)) / (SAR - SAL);
+  tmpE = divr(SAR * EstarLLx - SAL * EstarRLx + SAR * SAL * (bRL - bLL), SAR - SAL, iSAx);
   tmpE = SL_pos * ELL + SL_neg * SR_neg * ERL + SL_neg * SR_pos * tmpE;
   E += SB_pos * tmpE;
 
-  tmpE = (SAR * EstarLRx - SAL * EstarRRx + SAR * SAL * (bRR - bLR)) / (SAR - SAL);
+  tmpE = divr(SAR * EstarLRx - SAL * EstarRRx + SAR * SAL * (bRR - bLR), SAR - SAL, iSAx);
   tmpE = SL_pos * ELR + SL_neg * SR_neg * ERR + SL_neg * SR_pos * tmpE;
   E += (SB_neg * ST_neg) * tmpE;
 
-  tmpE = (SAT * EstarLLy - SAB * EstarLRy - SAT * SAB * (aLR - aLL)) / (SAT - SAB);
+  tmpE = divr(SAT * EstarLLy - SAB * EstarLRy - SAT * SAB * (aLR - aLL), SAT - SAB, iSAy);
   E += (SB_neg * ST_pos * SL_pos) * tmpE;
 
-  tmpE = (SAT * EstarRLy - SAB * EstarRRy - SAT * SAB * (aRR - aRL)) / (SAT - SAB);
+  tmpE = divr(SAT * EstarRLy - SAB * EstarRRy - SAT * SAB * (aRR - aRL), SAT - SAB, iSAy);
   E += (SB_neg * ST_pos * SL_neg * SR_neg) * tmpE;
   return E;
 }
