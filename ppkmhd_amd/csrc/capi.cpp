@@ -254,6 +254,9 @@ ppk::DevParams make_dev(const ppk_mhd_params &p) {
   // tuning knob: PPKMHD_HALF_TILE selects one of the compiled 3D half-step tile shapes
   const char *tile = std::getenv("PPKMHD_HALF_TILE");
   d.tile3 = tile ? std::atoi(tile) : 0;
+  // tuning knob: PPKMHD_BRICK selects the block shape of the fused per-cell kernels
+  const char *brick = std::getenv("PPKMHD_BRICK");
+  d.brick = brick ? std::atoi(brick) : 0;
   return d;
 }
 

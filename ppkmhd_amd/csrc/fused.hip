@@ -536,6 +536,14 @@ PPK_HD void edge2(const double *__restrict__ rec, double *__restrict__ emf, cons
 
 // cell (i,j,k) of thread tid of brick L; false past the covered range
 PPK_HD bool brick_cell(const Brick &b, int L, int tid, int &i, int &j, int &k) {
+  if (b.bx == 0) {  // flattened runs of 256 cells
+    const long long n = (long long)L * 256 + tid;
+    i = (int)(n % b.mx);
+    const long long r = n / b.mx;
+    j = (int)(r % b.my);
+    k = (int)(r / b.my);
+    return k < b.mz;
+  }
   const int ux = L % b.nx, r = L / b.nx, uy = r % b.ny, uz = r / b.ny;
   i = ux * b.bx + tid % b.bx;
   j = uy * b.by + (tid / b.bx) % b.by;
@@ -545,7 +553,7 @@ PPK_HD bool brick_cell(const Brick &b, int L, int tid, int &i, int &j, int &k) {
 
 // the sub-box [gw, size-gw] per direction of the face and edge kernels, in bricks
 inline Brick sub_bricks(const DevParams &p) {
-  return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1);
+  return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1, p.brick);
 }
 // padded-box index of thread tid of sub-box brick L, or -1
 PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int tid) {

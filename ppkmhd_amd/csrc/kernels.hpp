@@ -88,30 +88,40 @@ inline int cfl_blocks(const DevParams &p) {
   return g < 1 ? 1 : (int)g;
 }
 
-// Bricks: the fused per-cell kernels give each block of 256 threads a brick of cells,
-// 16 x 4 x 4 in 3D (a wave is a 16 x 4 slab of one z plane: four 128-byte rows per array) and
-// 64 x 4 in 2D, so that most lower neighbours a cell reads belong to its own brick.
+// Bricks: the fused per-cell kernels give each block of 256 threads a brick of cells, so
+// that lower neighbours a cell reads mostly belong to its own brick; shape 0 (default) is a
+// run of 256 consecutive cells of the flattened range (one wave = 64 cells of one row), shape
+// 1 is 64 x 2 x 2, shape 2 16 x 4 x 4, shape 3 32 x 4 x 2 (2D: 64 x 4 for shapes 1-3).
 struct Brick {
-  int bx, by, bz;  // brick extents
-  int nx, ny, nz;  // bricks per direction
+  int bx, by, bz;  // brick extents (bx = 0: flattened runs of 256 cells)
+  int nx, ny, nz;  // bricks per direction (flattened: nx = number of runs)
   int mx, my, mz;  // cells covered per direction
   long long count() const { return (long long)nx * ny * nz; }
 };
-inline Brick make_brick(int dim, int mx, int my, int mz) {
+inline Brick make_brick(int dim, int mx, int my, int mz, int shape) {
   Brick b;
-  b.bx = dim == 3 ? 16 : 64;
-  b.by = 4;
-  b.bz = dim == 3 ? 4 : 1;
   b.mx = mx;
   b.my = my;
   b.mz = mz;
+  if (shape <= 0 || shape > 3) {
+    b.bx = b.by = b.bz = 0;
+    b.nx = (int)(((long long)mx * my * mz + 255) / 256);
+    b.ny = b.nz = 1;
+    return b;
+  }
+  static const int shp[4][3] = {{0, 0, 0}, {64, 2, 2}, {16, 4, 4}, {32, 4, 2}};
+  b.bx = dim == 3 ? shp[shape][0] : 64;
+  b.by = dim == 3 ? shp[shape][1] : 4;
+  b.bz = dim == 3 ? shp[shape][2] : 1;
   b.nx = (mx + b.bx - 1) / b.bx;
   b.ny = (my + b.by - 1) / b.by;
   b.nz = (mz + b.bz - 1) / b.bz;
   return b;
 }
 // bricks of the padded box (fused update, fused CFL)
-inline Brick padded_bricks(const DevParams &p) { return make_brick(p.dim, p.isize, p.jsize, p.ksize); }
+inline Brick padded_bricks(const DevParams &p) {
+  return make_brick(p.dim, p.isize, p.jsize, p.ksize, p.brick);
+}
 inline int fused_cfl_parts(const DevParams &p) { return (int)padded_bricks(p).count(); }
 
 #define PPK_DECLARE_KERNELS(NS)                                                                  \

@@ -438,22 +438,32 @@ __global__ void k_begin(double *ctrl, int check_finished) {
 }
 // dt from the CFL partial maxima or the fixed value, clamped to tEnd - t,
 // src/muscl/SolverMHDMuscl.h:737 and src/shared/SolverBase.cpp:173-177.  One block.
-__global__ void __launch_bounds__(kBlock) k_set_dt(double *ctrl, const double *__restrict__ parts,
-                                                   int nparts, double fixed_dt, int mode,
-                                                   double cfl) {
-  __shared__ double red[kBlock / 64];
+constexpr int kSetDtBlock = 1024;
+__global__ void __launch_bounds__(kSetDtBlock) k_set_dt(double *ctrl,
+                                                        const double *__restrict__ parts,
+                                                        int nparts, double fixed_dt, int mode,
+                                                        double cfl) {
+  __shared__ double red[kSetDtBlock / 64];
   if (ctrl[CTRL_ACTIVE] == 0.0) return;
   if (mode == SETDT_FIXED) {
     if (threadIdx.x == 0) ctrl[CTRL_DT] = fixed_dt;
     return;
   }
-  double m = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, parts[b]);
-  m = block_max(m, red);
+  // eight independent loads in flight per thread: the partials can number ~10^5
+  double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int b = threadIdx.x;
+  for (; b + 7 * kSetDtBlock < nparts; b += 8 * kSetDtBlock) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], parts[b + u * kSetDtBlock]);
+  }
+  for (; b < nparts; b += kSetDtBlock) m[0] = fmax(m[0], parts[b]);
+#pragma unroll
+  for (int u = 1; u < 8; ++u) m[0] = fmax(m[0], m[u]);
+  const double mx = block_max(m[0], red);
   if (threadIdx.x != 0) return;
-  ctrl[CTRL_INVDT] = m;
+  ctrl[CTRL_INVDT] = mx;
   if (mode == SETDT_CFL) {
-    double dt = cfl / m;
+    double dt = cfl / mx;
     if (ctrl[CTRL_T] + dt > ctrl[CTRL_TEND]) dt = ctrl[CTRL_TEND] - ctrl[CTRL_T];
     ctrl[CTRL_DT] = dt;
   }
@@ -505,8 +515,8 @@ hipError_t staged_begin(double *ctrl, int check_finished, hipStream_t s) {
 }
 hipError_t staged_set_dt(double *ctrl, const double *parts, int nparts, double fixed_dt, int mode,
                          double cfl, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(kBlock), 0, s, ctrl, parts, nparts, fixed_dt, mode,
-                     cfl);
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(kSetDtBlock), 0, s, ctrl, parts, nparts, fixed_dt,
+                     mode, cfl);
   return hipGetLastError();
 }
 hipError_t staged_end(double *ctrl, hipStream_t s) {
