@@ -231,7 +231,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec != A.FP32_BULK else "f64 state / f32 bulk",
+        "dtype": "f64" if prec != A.FP32_BULK else "f32 bulk (f64 state, update, CT, dt)",
         "data": "synthetic (analytic blast initial condition)",
         "config": {"workload": f"3D MHD blast {n}^3 FP64, periodic, HLLD+CT, minmod, CFL 0.8",
                    "global_batch": 1, "seq_len": None, "parallelism": "single GPU",

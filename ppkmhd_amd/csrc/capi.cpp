@@ -47,11 +47,11 @@ const char *const kStageNames[ST_COUNT] = {
     "update_ct",  "control",       "fused_half", "fused_riemann", "fused_update", "fused_cfl"};
 static_assert(ST_COUNT == 13, "stage names");
 
-// the fused kernels serve FP64 / strict FP64 in 2D and 3D unless the staged path is asked for
+// the fused kernels serve every precision in 2D and 3D unless the staged path is asked for
 bool fused_supported(const DevParams &p, int precision, int kernel_path) {
   (void)p;
   if (kernel_path == PPK_PATH_STAGED) return false;
-  return precision == PPK_FP64 || precision == PPK_FP64_STRICT;
+  return precision == PPK_FP64 || precision == PPK_FP64_STRICT || precision == PPK_FP32_BULK;
 }
 
 // hipEvent pairs per stage; events are pooled and resolved when the times are read
@@ -257,6 +257,9 @@ ppk::DevParams make_dev(const ppk_mhd_params &p) {
   // tuning knob: PPKMHD_BRICK selects the block shape of the fused per-cell kernels
   const char *brick = std::getenv("PPKMHD_BRICK");
   d.brick = brick ? std::atoi(brick) : 0;
+  // tuning knob: PPKMHD_RIEMANN_MIX=0 launches the six face / edge kernels one by one
+  const char *mix = std::getenv("PPKMHD_RIEMANN_MIX");
+  d.riemann_mix = mix ? std::atoi(mix) : 1;
   return d;
 }
 
@@ -359,6 +362,7 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   h->p = *params;
   h->dp = make_dev(*params);
   h->K = params->precision == PPK_FP64_STRICT ? &ppk::kStrict : &ppk::kFast;
+  h->dp.bulk32 = params->precision == PPK_FP32_BULK ? 1 : 0;
   h->fused = ppk::fused_supported(h->dp, params->precision, params->kernel_path);
   if (params->kernel_path == PPK_PATH_FUSED && !h->fused) {
     delete h;
@@ -395,9 +399,13 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   }
   if (ok && h->fused) {
     const int d = params->dim;
-    ok = dalloc(&h->fw.rec, (size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n) &&
-         dalloc(&h->fw.flux, (size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n) &&
-         dalloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n);
+    const size_t es = h->dp.bulk32 ? sizeof(float) : sizeof(double);
+    auto balloc = [&](void **ptr, size_t count) {
+      return hipMalloc(ptr, count * es) == hipSuccess;
+    };
+    ok = balloc(&h->fw.rec, (size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n) &&
+         balloc(&h->fw.flux, (size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n) &&
+         balloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n);
   }
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
   if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * (ppk::CTRL_SLOTS + 1)) != hipSuccess)

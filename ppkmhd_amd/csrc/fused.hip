@@ -62,8 +62,9 @@ PPK_HD int xcd_remap(int b, int nblocks) {
   return x * q + (x < r ? x : r) + b / 8;
 }
 
-PPK_HD void store_rec3(double *__restrict__ rec, long long vs, long long t, const Half3D<double> &h) {
-  double *o = rec + t;
+template <class R>
+PPK_HD void store_rec3(R *__restrict__ rec, long long vs, long long t, const Half3D<R> &h) {
+  R *o = rec + t;
   o[R3_R * vs] = h.r;     o[R3_P * vs] = h.pp;    o[R3_U * vs] = h.u;     o[R3_V * vs] = h.v;
   o[R3_W * vs] = h.w;     o[R3_A * vs] = h.A;     o[R3_B * vs] = h.B;     o[R3_C * vs] = h.C;
   o[R3_DRX * vs] = h.drx; o[R3_DPX * vs] = h.dpx; o[R3_DUX * vs] = h.dux; o[R3_DVX * vs] = h.dvx;
@@ -79,8 +80,9 @@ PPK_HD void store_rec3(double *__restrict__ rec, long long vs, long long t, cons
 
 // left-side fields of the record at t; the right-side ones are set by the caller from the +1
 // neighbours where a state needs them
-PPK_HD void load_rec3(const double *__restrict__ rec, long long vs, long long t, Half3D<double> &h) {
-  const double *a = rec + t;
+template <class R>
+PPK_HD void load_rec3(const R *__restrict__ rec, long long vs, long long t, Half3D<R> &h) {
+  const R *a = rec + t;
   h.r = a[R3_R * vs];     h.pp = a[R3_P * vs];    h.u = a[R3_U * vs];     h.v = a[R3_V * vs];
   h.w = a[R3_W * vs];     h.A = a[R3_A * vs];     h.B = a[R3_B * vs];     h.C = a[R3_C * vs];
   h.drx = a[R3_DRX * vs]; h.dpx = a[R3_DPX * vs]; h.dux = a[R3_DUX * vs]; h.dvx = a[R3_DVX * vs];
@@ -92,12 +94,13 @@ PPK_HD void load_rec3(const double *__restrict__ rec, long long vs, long long t,
   h.AL = a[R3_AL * vs];   h.BL = a[R3_BL * vs];   h.CL = a[R3_CL * vs];
   h.dALy = a[R3_DALY * vs]; h.dALz = a[R3_DALZ * vs]; h.dBLx = a[R3_DBLX * vs];
   h.dBLz = a[R3_DBLZ * vs]; h.dCLx = a[R3_DCLX * vs]; h.dCLy = a[R3_DCLY * vs];
-  h.AR = h.BR = h.CR = 0.0;
-  h.dARy = h.dARz = h.dBRx = h.dBRz = h.dCRx = h.dCRy = 0.0;
+  h.AR = h.BR = h.CR = R(0);
+  h.dARy = h.dARz = h.dBRx = h.dBRz = h.dCRx = h.dCRy = R(0);
 }
 
-PPK_HD void store_rec2(double *__restrict__ rec, long long vs, long long t, const Half2D<double> &h) {
-  double *o = rec + t;
+template <class R>
+PPK_HD void store_rec2(R *__restrict__ rec, long long vs, long long t, const Half2D<R> &h) {
+  R *o = rec + t;
   o[R2_R * vs] = h.r;     o[R2_P * vs] = h.pp;    o[R2_U * vs] = h.u;     o[R2_V * vs] = h.v;
   o[R2_W * vs] = h.w;     o[R2_A * vs] = h.A;     o[R2_B * vs] = h.B;     o[R2_C * vs] = h.C;
   o[R2_DRX * vs] = h.drx; o[R2_DPX * vs] = h.dpx; o[R2_DUX * vs] = h.dux; o[R2_DVX * vs] = h.dvx;
@@ -108,8 +111,9 @@ PPK_HD void store_rec2(double *__restrict__ rec, long long vs, long long t, cons
   o[R2_DALY * vs] = h.dALy; o[R2_DBLX * vs] = h.dBLx;
 }
 
-PPK_HD void load_rec2(const double *__restrict__ rec, long long vs, long long t, Half2D<double> &h) {
-  const double *a = rec + t;
+template <class R>
+PPK_HD void load_rec2(const R *__restrict__ rec, long long vs, long long t, Half2D<R> &h) {
+  const R *a = rec + t;
   h.r = a[R2_R * vs];     h.pp = a[R2_P * vs];    h.u = a[R2_U * vs];     h.v = a[R2_V * vs];
   h.w = a[R2_W * vs];     h.A = a[R2_A * vs];     h.B = a[R2_B * vs];     h.C = a[R2_C * vs];
   h.drx = a[R2_DRX * vs]; h.dpx = a[R2_DPX * vs]; h.dux = a[R2_DUX * vs]; h.dvx = a[R2_DVX * vs];
@@ -118,8 +122,8 @@ PPK_HD void load_rec2(const double *__restrict__ rec, long long vs, long long t,
   h.dwy = a[R2_DWY * vs]; h.dAy = a[R2_DAY * vs]; h.dCy = a[R2_DCY * vs];
   h.AL = a[R2_AL * vs];   h.BL = a[R2_BL * vs];
   h.dALy = a[R2_DALY * vs]; h.dBLx = a[R2_DBLX * vs];
-  h.AR = h.BR = 0.0;
-  h.dARy = h.dBRx = 0.0;
+  h.AR = h.BR = R(0);
+  h.dARy = h.dBRx = R(0);
 }
 
 // ===========================================================================================
@@ -134,7 +138,7 @@ struct HalfTile3 {
   static constexpr int FX = TX + 3, FY = TY + 3, FZ = TZ + 3, NF = FX * FY * FZ;
   static constexpr int QX = TX + 2, QY = TY + 2, QZ = TZ + 2, NQ = QX * QY * QZ;
   static constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1, NE = EX * EY * EZ;
-  static constexpr int LDS = 3 * NF + 8 * NQ + 3 * NE;  // doubles
+  static constexpr int LDS = 3 * NF + 8 * NQ + 3 * NE;  // elements of the arithmetic type
   PPK_HD static int f(int x, int y, int z) { return x + FX * (y + FY * z); }
   PPK_HD static int q(int x, int y, int z) { return x + QX * (y + QY * z); }
   PPK_HD static int e(int x, int y, int z) { return x + EX * (y + EY * z); }
@@ -145,55 +149,83 @@ using Tile3b = HalfTile3<8, 8, 8>;   // 512 threads, 113 KB
 using Tile3c = HalfTile3<16, 8, 4>;  // 512 threads, 123 KB
 using Tile3d = HalfTile3<8, 8, 4>;   // 256 threads, 68 KB
 using Tile3e = HalfTile3<32, 4, 2>;  // 256 threads, 94 KB
-static_assert(Tile3c::LDS * 8 <= 160 * 1024, "3D half-step tile exceeds the LDS of a CU");
+static_assert(Tile3c::LDS * sizeof(double) <= 160 * 1024, "3D tile exceeds the LDS of a CU");
 
 // phase 1, all global loads of the tile: face fields A, B, C of cells [b0-1, b0+T+1]
 // (clipped to the box) into sF, and the five other conservative variables of the primitive
-// cells [b0-1, b0+T] (up to size-gw+1) into their slots of sQ
-template <class T>
+// cells [b0-1, b0+T] (up to size-gw+1) into their slots of sQ.  Every load of a thread is
+// issued before its first LDS store, so the tile pays one memory latency, not one per
+// strided pass.
+template <class T, class R>
 PPK_HD void half3_load(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
-                       int b0z, int tid, double *sF, double *sQ) {
+                       int b0z, int tid, R *sF, R *sQ) {
+  constexpr int IF = (T::NF + T::NT - 1) / T::NT, IQ = (T::NQ + T::NT - 1) / T::NT;
   const long long vs = p.vstride;
-  for (int n = tid; n < T::NF; n += T::NT) {
+  const int hi = 1 - p.gw;  // last primitive cell: size + hi
+  double fv[IF][3], qv[IQ][5];
+  bool fok[IF], qok[IQ];
+#pragma unroll
+  for (int it = 0; it < IF; ++it) {
+    const int n = tid + it * T::NT;
     const int x = n % T::FX, r = n / T::FX, y = r % T::FY, z = r / T::FY;
     const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
-    if (i >= p.isize || j >= p.jsize || k >= p.ksize) continue;
-    const long long t = cidx(p, i, j, k);
-    sF[n] = U[t + IA * vs];
-    sF[T::NF + n] = U[t + IB * vs];
-    sF[2 * T::NF + n] = U[t + IC * vs];
+    fok[it] = n < T::NF && i < p.isize && j < p.jsize && k < p.ksize;
+    // unconditional load at a clamped (in-box) address keeps all loads in one batch
+    const long long t = cidx(p, min(i, p.isize - 1), min(j, p.jsize - 1), min(k, p.ksize - 1));
+    fv[it][0] = U[t + IA * vs];
+    fv[it][1] = U[t + IB * vs];
+    fv[it][2] = U[t + IC * vs];
   }
-  const int hi = 1 - p.gw;  // last primitive cell: size + hi
-  for (int n = tid; n < T::NQ; n += T::NT) {
+#pragma unroll
+  for (int it = 0; it < IQ; ++it) {
+    const int n = tid + it * T::NT;
     const int x = n % T::QX, r = n / T::QX, y = r % T::QY, z = r / T::QY;
     const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
-    if (i > p.isize + hi || j > p.jsize + hi || k > p.ksize + hi) continue;
-    const long long t = cidx(p, i, j, k);
+    qok[it] = n < T::NQ && i <= p.isize + hi && j <= p.jsize + hi && k <= p.ksize + hi;
+    const long long t =
+        cidx(p, min(i, p.isize + hi), min(j, p.jsize + hi), min(k, p.ksize + hi));
 #pragma unroll
-    for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = U[t + v * vs];
+    for (int v = ID; v <= IW; ++v) qv[it][v] = U[t + v * vs];
+  }
+#pragma unroll
+  for (int it = 0; it < IF; ++it) {
+    const int n = tid + it * T::NT;
+    if (fok[it]) {
+      sF[n] = R(fv[it][0]);
+      sF[T::NF + n] = R(fv[it][1]);
+      sF[2 * T::NF + n] = R(fv[it][2]);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < IQ; ++it) {
+    const int n = tid + it * T::NT;
+    if (qok[it]) {
+#pragma unroll
+      for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = R(qv[it][v]);
+    }
   }
 }
 
 // phase 2: primitives of cells [b0-1, b0+T] up to size-gw+1 (MHDRunFunctors3D.h:88-163), in
 // place: each thread reads the raw slots of its own cells only
-template <class T>
-PPK_HD void half3_prims(const DevParams &p, int b0x, int b0y, int b0z, int tid, const double *sF,
-                        double *sQ) {
+template <class T, class R>
+PPK_HD void half3_prims(const DevParams &p, int b0x, int b0y, int b0z, int tid, const R *sF,
+                        R *sQ) {
   const int hi = 1 - p.gw;
   for (int n = tid; n < T::NQ; n += T::NT) {
     const int x = n % T::QX, r = n / T::QX, y = r % T::QY, z = r / T::QY;
     const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
     if (i > p.isize + hi || j > p.jsize + hi || k > p.ksize + hi) continue;
-    St<double> u, q;
+    St<R> u, q;
 #pragma unroll
     for (int v = ID; v <= IW; ++v) u[v] = sQ[v * T::NQ + n];
     const int f0 = T::f(x, y, z);
     u[IA] = sF[f0];
     u[IB] = sF[T::NF + f0];
     u[IC] = sF[2 * T::NF + f0];
-    const double mx = sF[T::f(x + 1, y, z)];
-    const double my = sF[T::NF + T::f(x, y + 1, z)];
-    const double mz = sF[2 * T::NF + T::f(x, y, z + 1)];
+    const R mx = sF[T::f(x + 1, y, z)];
+    const R my = sF[T::NF + T::f(x, y + 1, z)];
+    const R mz = sF[2 * T::NF + T::f(x, y, z + 1)];
     constoprim(u, mx, my, mz, q, p);
 #pragma unroll
     for (int v = 0; v < 8; ++v) sQ[v * T::NQ + n] = q[v];
@@ -202,9 +234,9 @@ PPK_HD void half3_prims(const DevParams &p, int b0x, int b0y, int b0z, int tid, 
 
 // phase 3: edge electric field E = v x B of cells [b0, b0+T] up to size-gw+1, same
 // operations as the staged E kernel (MHDRunFunctors3D.h:303-356)
-template <class T>
-PPK_HD void half3_elec(const DevParams &p, int b0x, int b0y, int b0z, int tid, const double *sF,
-                       const double *sQ, double *sE) {
+template <class T, class R>
+PPK_HD void half3_elec(const DevParams &p, int b0x, int b0y, int b0z, int tid, const R *sF,
+                       const R *sQ, R *sE) {
   const int hi = 1 - p.gw;
   for (int n = tid; n < T::NE; n += T::NT) {
     const int x = n % T::EX, r = n / T::EX, y = r % T::EY, z = r / T::EY;
@@ -216,41 +248,40 @@ PPK_HD void half3_elec(const DevParams &p, int b0x, int b0y, int b0z, int tid, c
     auto u = [&](int dx, int dy, int dz, int c) {
       return sF[c * T::NF + T::f(X + dx, Y + dy, Z + dz)];
     };
-    double v_, w_, u_, A, B, C;
-    v_ = 0.25 * (q(0, -1, -1, IV) + q(0, -1, 0, IV) + q(0, 0, -1, IV) + q(0, 0, 0, IV));
-    w_ = 0.25 * (q(0, -1, -1, IW) + q(0, -1, 0, IW) + q(0, 0, -1, IW) + q(0, 0, 0, IW));
-    B = 0.5 * (u(0, 0, -1, 1) + u(0, 0, 0, 1));
-    C = 0.5 * (u(0, -1, 0, 2) + u(0, 0, 0, 2));
+    R v_, w_, u_, A, B, C;
+    v_ = R(0.25) * (q(0, -1, -1, IV) + q(0, -1, 0, IV) + q(0, 0, -1, IV) + q(0, 0, 0, IV));
+    w_ = R(0.25) * (q(0, -1, -1, IW) + q(0, -1, 0, IW) + q(0, 0, -1, IW) + q(0, 0, 0, IW));
+    B = R(0.5) * (u(0, 0, -1, 1) + u(0, 0, 0, 1));
+    C = R(0.5) * (u(0, -1, 0, 2) + u(0, 0, 0, 2));
     sE[n] = v_ * C - w_ * B;
-    u_ = 0.25 * (q(-1, 0, -1, IU) + q(-1, 0, 0, IU) + q(0, 0, -1, IU) + q(0, 0, 0, IU));
-    w_ = 0.25 * (q(-1, 0, -1, IW) + q(-1, 0, 0, IW) + q(0, 0, -1, IW) + q(0, 0, 0, IW));
-    A = 0.5 * (u(0, 0, -1, 0) + u(0, 0, 0, 0));
-    C = 0.5 * (u(-1, 0, 0, 2) + u(0, 0, 0, 2));
+    u_ = R(0.25) * (q(-1, 0, -1, IU) + q(-1, 0, 0, IU) + q(0, 0, -1, IU) + q(0, 0, 0, IU));
+    w_ = R(0.25) * (q(-1, 0, -1, IW) + q(-1, 0, 0, IW) + q(0, 0, -1, IW) + q(0, 0, 0, IW));
+    A = R(0.5) * (u(0, 0, -1, 0) + u(0, 0, 0, 0));
+    C = R(0.5) * (u(-1, 0, 0, 2) + u(0, 0, 0, 2));
     sE[T::NE + n] = w_ * A - u_ * C;
-    u_ = 0.25 * (q(-1, -1, 0, IU) + q(-1, 0, 0, IU) + q(0, -1, 0, IU) + q(0, 0, 0, IU));
-    v_ = 0.25 * (q(-1, -1, 0, IV) + q(-1, 0, 0, IV) + q(0, -1, 0, IV) + q(0, 0, 0, IV));
-    A = 0.5 * (u(0, -1, 0, 0) + u(0, 0, 0, 0));
-    B = 0.5 * (u(-1, 0, 0, 1) + u(0, 0, 0, 1));
+    u_ = R(0.25) * (q(-1, -1, 0, IU) + q(-1, 0, 0, IU) + q(0, -1, 0, IU) + q(0, 0, 0, IU));
+    v_ = R(0.25) * (q(-1, -1, 0, IV) + q(-1, 0, 0, IV) + q(0, -1, 0, IV) + q(0, 0, 0, IV));
+    A = R(0.5) * (u(0, -1, 0, 0) + u(0, 0, 0, 0));
+    B = R(0.5) * (u(-1, 0, 0, 1) + u(0, 0, 0, 1));
     sE[2 * T::NE + n] = u_ * B - v_ * A;
   }
 }
 
 // phase 4: one record cell per thread: slopes, transverse face-field slopes
 // (MHDRunFunctors3D.h:472-532) and the half step (MHDBaseFunctor3D.h:688-1114)
-template <class T>
-PPK_HD void half3_record(const DevParams &p, int b0x, int b0y, int b0z, int tid, const double *sF,
-                         const double *sQ, const double *sE, double dt,
-                         double *__restrict__ rec) {
+template <class T, class R>
+PPK_HD void half3_record(const DevParams &p, int b0x, int b0y, int b0z, int tid, const R *sF,
+                         const R *sQ, const R *sE, double dt, R *__restrict__ rec) {
   const int x = tid % T::TX, y = (tid / T::TX) % T::TY, z = tid / (T::TX * T::TY);
   const int i = b0x + x, j = b0y + y, k = b0z + z;
   if (i > p.isize - p.gw || j > p.jsize - p.gw || k > p.ksize - p.gw) return;
   const int X = x + 1, Y = y + 1, Z = z + 1;
-  auto ld = [&](int dx, int dy, int dz, St<double> &s) {
+  auto ld = [&](int dx, int dy, int dz, St<R> &s) {
     const int n = T::q(X + dx, Y + dy, Z + dz);
 #pragma unroll
     for (int v = 0; v < 8; ++v) s[v] = sQ[v * T::NQ + n];
   };
-  St<double> q, qPX, qMX, qPY, qMY, qPZ, qMZ;
+  St<R> q, qPX, qMX, qPY, qMY, qPZ, qMZ;
   ld(0, 0, 0, q);
   ld(1, 0, 0, qPX);
   ld(-1, 0, 0, qMX);
@@ -261,26 +292,26 @@ PPK_HD void half3_record(const DevParams &p, int b0x, int b0y, int b0z, int tid,
   auto F = [&](int dx, int dy, int dz, int c) {
     return sF[c * T::NF + T::f(X + dx, Y + dy, Z + dz)];
   };
-  const double bfNb[6] = {F(0, 0, 0, 0), F(1, 0, 0, 0), F(0, 0, 0, 1),
+  const R bfNb[6] = {F(0, 0, 0, 0), F(1, 0, 0, 0), F(0, 0, 0, 1),
                           F(0, 1, 0, 1), F(0, 0, 0, 2), F(0, 0, 1, 2)};
-  const double xst = fmin(p.slope_type, 2.0);
+  const R xst = R(fmin(p.slope_type, 2.0));
   // the left-side transverse slopes; the right-side ones (second half) are never used
-  const double d12[12] = {limited_slope(xst, F(0, 0, 0, 0), F(0, 1, 0, 0), F(0, -1, 0, 0)),
+  const R d12[12] = {limited_slope(xst, F(0, 0, 0, 0), F(0, 1, 0, 0), F(0, -1, 0, 0)),
                           limited_slope(xst, F(0, 0, 0, 0), F(0, 0, 1, 0), F(0, 0, -1, 0)),
                           limited_slope(xst, F(0, 0, 0, 1), F(1, 0, 0, 1), F(-1, 0, 0, 1)),
                           limited_slope(xst, F(0, 0, 0, 1), F(0, 0, 1, 1), F(0, 0, -1, 1)),
                           limited_slope(xst, F(0, 0, 0, 2), F(1, 0, 0, 2), F(-1, 0, 0, 2)),
                           limited_slope(xst, F(0, 0, 0, 2), F(0, 1, 0, 2), F(0, -1, 0, 2)),
-                          0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                          R(0), R(0), R(0), R(0), R(0), R(0)};
   auto E = [&](int dx, int dy, int dz, int c) {
     return sE[c * T::NE + T::e(x + dx, y + dy, z + dz)];
   };
   // only the fields of the left faces' edges enter the record (ERR, FRR, GRR do not)
-  const double Ex[4] = {E(0, 0, 0, 0), E(0, 0, 1, 0), E(0, 1, 0, 0), 0.0};
-  const double Ey[4] = {E(0, 0, 0, 1), E(0, 0, 1, 1), E(1, 0, 0, 1), 0.0};
-  const double Ez[4] = {E(0, 0, 0, 2), E(0, 1, 0, 2), E(1, 0, 0, 2), 0.0};
-  const double dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = dt / p.dz;
-  Half3D<double> h;
+  const R Ex[4] = {E(0, 0, 0, 0), E(0, 0, 1, 0), E(0, 1, 0, 0), R(0)};
+  const R Ey[4] = {E(0, 0, 0, 1), E(0, 0, 1, 1), E(1, 0, 0, 1), R(0)};
+  const R Ez[4] = {E(0, 0, 0, 2), E(0, 1, 0, 2), E(1, 0, 0, 2), R(0)};
+  const R dtdx = R(dt / p.dx), dtdy = R(dt / p.dy), dtdz = R(dt / p.dz);
+  Half3D<R> h;
   half_step_3d(q, qPX, qMX, qPY, qMY, qPZ, qMZ, bfNb, d12, Ex, Ey, Ez, dtdx, dtdy, dtdz, p, h);
   store_rec3(rec, p.vstride, cidx(p, i, j, k), h);
 }
@@ -302,60 +333,84 @@ struct HalfTile2 {
 using Tile2 = HalfTile2<32, 8>;
 
 // all global loads of the tile: face fields A, B into sF, the six cell-centred variables
-// (ID..IW and the 2D IC) of the primitive cells into their slots of sQ
-template <class T>
+// (ID..IW and the 2D IC) of the primitive cells into their slots of sQ; loads first, then
+// the LDS stores
+template <class T, class R>
 PPK_HD void half2_load(const double *__restrict__ U, const DevParams &p, int b0x, int b0y, int tid,
-                       double *sF, double *sQ) {
+                       R *sF, R *sQ) {
+  constexpr int IF = (T::NF + T::NT - 1) / T::NT, IQ = (T::NQ + T::NT - 1) / T::NT;
   const long long vs = p.vstride;
-  for (int n = tid; n < T::NF; n += T::NT) {
+  const int hi = 1 - p.gw;
+  double fv[IF][2], qv[IQ][6];
+  bool fok[IF], qok[IQ];
+#pragma unroll
+  for (int it = 0; it < IF; ++it) {
+    const int n = tid + it * T::NT;
     const int x = n % T::FX, y = n / T::FX;
     const int i = b0x - 1 + x, j = b0y - 1 + y;
-    if (i >= p.isize || j >= p.jsize) continue;
-    const long long t = cidx(p, i, j, 0);
-    sF[n] = U[t + IA * vs];
-    sF[T::NF + n] = U[t + IB * vs];
+    fok[it] = n < T::NF && i < p.isize && j < p.jsize;
+    const long long t = cidx(p, min(i, p.isize - 1), min(j, p.jsize - 1), 0);
+    fv[it][0] = U[t + IA * vs];
+    fv[it][1] = U[t + IB * vs];
   }
-  const int hi = 1 - p.gw;
-  for (int n = tid; n < T::NQ; n += T::NT) {
+#pragma unroll
+  for (int it = 0; it < IQ; ++it) {
+    const int n = tid + it * T::NT;
     const int x = n % T::QX, y = n / T::QX;
     const int i = b0x - 1 + x, j = b0y - 1 + y;
-    if (i > p.isize + hi || j > p.jsize + hi) continue;
-    const long long t = cidx(p, i, j, 0);
+    qok[it] = n < T::NQ && i <= p.isize + hi && j <= p.jsize + hi;
+    const long long t = cidx(p, min(i, p.isize + hi), min(j, p.jsize + hi), 0);
 #pragma unroll
-    for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = U[t + v * vs];
-    sQ[IC * T::NQ + n] = U[t + IC * vs];
+    for (int v = ID; v <= IW; ++v) qv[it][v] = U[t + v * vs];
+    qv[it][5] = U[t + IC * vs];
+  }
+#pragma unroll
+  for (int it = 0; it < IF; ++it) {
+    const int n = tid + it * T::NT;
+    if (fok[it]) {
+      sF[n] = R(fv[it][0]);
+      sF[T::NF + n] = R(fv[it][1]);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < IQ; ++it) {
+    const int n = tid + it * T::NT;
+    if (qok[it]) {
+#pragma unroll
+      for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = R(qv[it][v]);
+      sQ[IC * T::NQ + n] = R(qv[it][5]);
+    }
   }
 }
 
 // primitives in place (MHDRunFunctors2D.h ComputePrimitivesFunctor): Bz is cell-centred in 2D
 // and is averaged with 0
-template <class T>
-PPK_HD void half2_prims(const DevParams &p, int b0x, int b0y, int tid, const double *sF,
-                        double *sQ) {
+template <class T, class R>
+PPK_HD void half2_prims(const DevParams &p, int b0x, int b0y, int tid, const R *sF, R *sQ) {
   const int hi = 1 - p.gw;
   for (int n = tid; n < T::NQ; n += T::NT) {
     const int x = n % T::QX, y = n / T::QX;
     const int i = b0x - 1 + x, j = b0y - 1 + y;
     if (i > p.isize + hi || j > p.jsize + hi) continue;
-    St<double> u, q;
+    St<R> u, q;
 #pragma unroll
     for (int v = ID; v <= IW; ++v) u[v] = sQ[v * T::NQ + n];
     u[IC] = sQ[IC * T::NQ + n];
     const int f0 = T::f(x, y);
     u[IA] = sF[f0];
     u[IB] = sF[T::NF + f0];
-    const double mx = sF[T::f(x + 1, y)];
-    const double my = sF[T::NF + T::f(x, y + 1)];
-    constoprim(u, mx, my, 0.0, q, p);
+    const R mx = sF[T::f(x + 1, y)];
+    const R my = sF[T::NF + T::f(x, y + 1)];
+    constoprim(u, mx, my, R(0), q, p);
 #pragma unroll
     for (int v = 0; v < 8; ++v) sQ[v * T::NQ + n] = q[v];
   }
 }
 
 // E_z at the lower-left corner of cells [b0, b0+T] up to size-gw+1
-template <class T>
-PPK_HD void half2_elec(const DevParams &p, int b0x, int b0y, int tid, const double *sF,
-                       const double *sQ, double *sE) {
+template <class T, class R>
+PPK_HD void half2_elec(const DevParams &p, int b0x, int b0y, int tid, const R *sF, const R *sQ,
+                       R *sE) {
   const int hi = 1 - p.gw;
   for (int n = tid; n < T::NE; n += T::NT) {
     const int x = n % T::EX, y = n / T::EX;
@@ -370,14 +425,14 @@ PPK_HD void half2_elec(const DevParams &p, int b0x, int b0y, int tid, const doub
 }
 
 // one record cell per thread (MHDBaseFunctor2D.h:774-1109)
-template <class T>
-PPK_HD void half2_record(const DevParams &p, int b0x, int b0y, int tid, const double *sF,
-                         const double *sQ, const double *sE, double dt, double *__restrict__ rec) {
+template <class T, class R>
+PPK_HD void half2_record(const DevParams &p, int b0x, int b0y, int tid, const R *sF, const R *sQ,
+                         const R *sE, double dt, R *__restrict__ rec) {
   const int x = tid % T::TX, y = tid / T::TX;
   const int i = b0x + x, j = b0y + y;
   if (i > p.isize - p.gw || j > p.jsize - p.gw) return;
-  St<double> qNb[3][3];
-  double bf[4][4][3];
+  St<R> qNb[3][3];
+  R bf[4][4][3];
 #pragma unroll
   for (int di = 0; di < 3; ++di)
 #pragma unroll
@@ -393,12 +448,12 @@ PPK_HD void half2_record(const DevParams &p, int b0x, int b0y, int tid, const do
       const int n = T::f(x + di, y + dj);
       bf[di][dj][0] = sF[n];
       bf[di][dj][1] = sF[T::NF + n];
-      bf[di][dj][2] = 0.0;
+      bf[di][dj][2] = R(0);
     }
-  const double ELL = sE[T::e(x, y)], ELR = sE[T::e(x, y + 1)];
-  const double ERL = sE[T::e(x + 1, y)], ERR = sE[T::e(x + 1, y + 1)];
-  Half2D<double> h;
-  half_step_2d(qNb, bf, ELL, ELR, ERL, ERR, dt / p.dx, dt / p.dy, p, h);
+  const R ELL = sE[T::e(x, y)], ELR = sE[T::e(x, y + 1)];
+  const R ERL = sE[T::e(x + 1, y)], ERR = sE[T::e(x + 1, y + 1)];
+  Half2D<R> h;
+  half_step_2d(qNb, bf, ELL, ELR, ERL, ERR, R(dt / p.dx), R(dt / p.dy), p, h);
   store_rec2(rec, p.vstride, cidx(p, i, j, 0), h);
 }
 
@@ -406,8 +461,9 @@ PPK_HD void half2_record(const DevParams &p, int b0x, int b0y, int tid, const do
 // Riemann problems at the lower faces and EMFs at the lower edges of cell (i,j,k), for cells
 // [gw, size-gw] per direction (the ranges of the reference's flux and EMF functors)
 // ===========================================================================================
-PPK_HD void swap_dir(St<double> &s, int a, int b) {
-  double t = s[a];
+template <class R>
+PPK_HD void swap_dir(St<R> &s, int a, int b) {
+  R t = s[a];
   s[a] = s[b];
   s[b] = t;
 }
@@ -416,18 +472,18 @@ PPK_HD void swap_dir(St<double> &s, int a, int b) {
 // face field is the left one of c) against qp of cell c, rotated so that the normal is x,
 // then the 1D Riemann solver; the 5 (3D) or 6 (2D) flux components are stored rotated.
 // Reference src/muscl/MHDRunFunctors3D.h:1839-1902 and MHDRunFunctors2D.h.
-template <int D>
-PPK_HD void face3(const double *__restrict__ rec, double *__restrict__ flux, const DevParams &p,
+template <int D, class R>
+PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
                   long long c) {
   const long long vs = p.vstride;
   const long long sd = D == 0 ? 1 : (D == 1 ? (long long)p.isize : p.plane);
-  Half3D<double> hl, hr;
+  Half3D<R> hl, hr;
   load_rec3(rec, vs, c - sd, hl);
   load_rec3(rec, vs, c, hr);
   if (D == 0) hl.AR = rec[c + R3_AL * vs];
   if (D == 1) hl.BR = rec[c + R3_BL * vs];
   if (D == 2) hl.CR = rec[c + R3_CL * vs];
-  St<double> ql = face_state_3d(hl, D, +1, p), qr = face_state_3d(hr, D, -1, p), f;
+  St<R> ql = face_state_3d(hl, D, +1, p), qr = face_state_3d(hr, D, -1, p), f;
   if (D == 1) {
     swap_dir(ql, IU, IV); swap_dir(ql, IA, IB);
     swap_dir(qr, IU, IV); swap_dir(qr, IA, IB);
@@ -436,7 +492,7 @@ PPK_HD void face3(const double *__restrict__ rec, double *__restrict__ flux, con
     swap_dir(qr, IU, IW); swap_dir(qr, IA, IC);
   }
   riemann_mhd(ql, qr, f, p);
-  double *o = flux + (long long)D * kFluxComp3 * vs + c;
+  R *o = flux + (long long)D * kFluxComp3 * vs + c;
   o[0] = f[ID];
   o[vs] = f[IP];
   o[2 * vs] = f[IU];
@@ -447,17 +503,17 @@ PPK_HD void face3(const double *__restrict__ rec, double *__restrict__ flux, con
 // One edge of cell c (its lower edge along direction E = 0 x, 1 y, 2 z): the four edge states
 // of the cells around it, then compute_emf.  Reference src/muscl/MHDRunFunctors3D.h:2183-2230
 // (the y edge takes RB and LT swapped, :2212-2217).
-template <int E>
-PPK_HD void edge3(const double *__restrict__ rec, double *__restrict__ emf, const DevParams &p,
+template <int E, class R>
+PPK_HD void edge3(const R *__restrict__ rec, R *__restrict__ emf, const DevParams &p,
                   long long c) {
   const long long vs = p.vstride, sx = 1, sy = p.isize, sz = p.plane;
-  auto R = [&](long long t, int f) { return rec[t + f * vs]; };
-  Half3D<double> a, b, d, h;
+  auto RF = [&](long long t, int f) { return rec[t + f * vs]; };
+  Half3D<R> a, b, d, h;
   load_rec3(rec, vs, c, h);
   if (E == 2) {  // RT of c-x-y, RB of c-x, LT of c-y, LB of c
     load_rec3(rec, vs, c - sx - sy, a);
-    a.AR = R(c - sy, R3_AL); a.dARy = R(c - sy, R3_DALY);
-    a.BR = R(c - sx, R3_BL); a.dBRx = R(c - sx, R3_DBLX);
+    a.AR = RF(c - sy, R3_AL); a.dARy = RF(c - sy, R3_DALY);
+    a.BR = RF(c - sx, R3_BL); a.dBRx = RF(c - sx, R3_DBLX);
     load_rec3(rec, vs, c - sx, b);
     b.AR = h.AL; b.dARy = h.dALy;
     load_rec3(rec, vs, c - sy, d);
@@ -467,8 +523,8 @@ PPK_HD void edge3(const double *__restrict__ rec, double *__restrict__ emf, cons
                           edge_state_3d(d, 2, ILT, p), edge_state_3d(h, 2, ILB, p), p);
   } else if (E == 1) {  // RT of c-x-z, LT of c-z, RB of c-x, LB of c
     load_rec3(rec, vs, c - sx - sz, a);
-    a.AR = R(c - sz, R3_AL); a.dARz = R(c - sz, R3_DALZ);
-    a.CR = R(c - sx, R3_CL); a.dCRx = R(c - sx, R3_DCLX);
+    a.AR = RF(c - sz, R3_AL); a.dARz = RF(c - sz, R3_DALZ);
+    a.CR = RF(c - sx, R3_CL); a.dCRx = RF(c - sx, R3_DCLX);
     load_rec3(rec, vs, c - sz, b);
     b.CR = h.CL; b.dCRx = h.dCLx;
     load_rec3(rec, vs, c - sx, d);
@@ -478,8 +534,8 @@ PPK_HD void edge3(const double *__restrict__ rec, double *__restrict__ emf, cons
                           edge_state_3d(d, 1, IRB, p), edge_state_3d(h, 1, ILB, p), p);
   } else {  // RT of c-y-z, RB of c-y, LT of c-z, LB of c
     load_rec3(rec, vs, c - sy - sz, a);
-    a.BR = R(c - sz, R3_BL); a.dBRz = R(c - sz, R3_DBLZ);
-    a.CR = R(c - sy, R3_CL); a.dCRy = R(c - sy, R3_DCLY);
+    a.BR = RF(c - sz, R3_BL); a.dBRz = RF(c - sz, R3_DBLZ);
+    a.CR = RF(c - sy, R3_CL); a.dCRy = RF(c - sy, R3_DCLY);
     load_rec3(rec, vs, c - sy, b);
     b.BR = h.BL; b.dBRz = h.dBLz;
     load_rec3(rec, vs, c - sz, d);
@@ -492,23 +548,23 @@ PPK_HD void edge3(const double *__restrict__ rec, double *__restrict__ emf, cons
 
 // 2D faces (D = 0, 1) and the z edge, reference src/muscl/MHDRunFunctors2D.h
 // ComputeFluxesAndStoreFunctor and ComputeEmfAndStoreFunctor (:654-680)
-template <int D>
-PPK_HD void face2(const double *__restrict__ rec, double *__restrict__ flux, const DevParams &p,
+template <int D, class R>
+PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
                   long long c) {
   const long long vs = p.vstride;
   const long long sd = D == 0 ? 1 : (long long)p.isize;
-  Half2D<double> hl, hr;
+  Half2D<R> hl, hr;
   load_rec2(rec, vs, c - sd, hl);
   load_rec2(rec, vs, c, hr);
   if (D == 0) hl.AR = rec[c + R2_AL * vs];
   if (D == 1) hl.BR = rec[c + R2_BL * vs];
-  St<double> ql = face_state_2d(hl, D, +1, p), qr = face_state_2d(hr, D, -1, p), f;
+  St<R> ql = face_state_2d(hl, D, +1, p), qr = face_state_2d(hr, D, -1, p), f;
   if (D == 1) {
     swap_dir(ql, IU, IV); swap_dir(ql, IA, IB);
     swap_dir(qr, IU, IV); swap_dir(qr, IA, IB);
   }
   riemann_mhd(ql, qr, f, p);
-  double *o = flux + (long long)D * kFluxComp2 * vs + c;
+  R *o = flux + (long long)D * kFluxComp2 * vs + c;
   o[0] = f[ID];
   o[vs] = f[IP];
   o[2 * vs] = f[IU];
@@ -517,15 +573,16 @@ PPK_HD void face2(const double *__restrict__ rec, double *__restrict__ flux, con
   o[5 * vs] = f[IC];
 }
 
-PPK_HD void edge2(const double *__restrict__ rec, double *__restrict__ emf, const DevParams &p,
+template <class R>
+PPK_HD void edge2(const R *__restrict__ rec, R *__restrict__ emf, const DevParams &p,
                   long long c) {
   const long long vs = p.vstride, sx = 1, sy = p.isize;
-  auto R = [&](long long t, int f) { return rec[t + f * vs]; };
-  Half2D<double> a, b, d, h;
+  auto RF = [&](long long t, int f) { return rec[t + f * vs]; };
+  Half2D<R> a, b, d, h;
   load_rec2(rec, vs, c, h);
   load_rec2(rec, vs, c - sx - sy, a);
-  a.AR = R(c - sy, R2_AL); a.dARy = R(c - sy, R2_DALY);
-  a.BR = R(c - sx, R2_BL); a.dBRx = R(c - sx, R2_DBLX);
+  a.AR = RF(c - sy, R2_AL); a.dARy = RF(c - sy, R2_DALY);
+  a.BR = RF(c - sx, R2_BL); a.dBRx = RF(c - sx, R2_DBLX);
   load_rec2(rec, vs, c - sx, b);
   b.AR = h.AL; b.dARy = h.dALy;
   load_rec2(rec, vs, c - sy, d);
@@ -535,14 +592,22 @@ PPK_HD void edge2(const double *__restrict__ rec, double *__restrict__ emf, cons
 }
 
 // cell (i,j,k) of thread tid of brick L; false past the covered range
+// row r of an enumeration of (j, k) rows in patches of pj x pk rows, npj patches along j
+PPK_HD void patch_row(long long r, int pj, int pk, int npj, int &j, int &k) {
+  const int lj = (int)(r % pj);
+  r /= pj;
+  const int lk = (int)(r % pk);
+  r /= pk;
+  j = (int)(r % npj) * pj + lj;
+  k = (int)(r / npj) * pk + lk;
+}
+
 PPK_HD bool brick_cell(const Brick &b, int L, int tid, int &i, int &j, int &k) {
-  if (b.bx == 0) {  // flattened runs of 256 cells
+  if (b.bx == 0) {  // flattened runs of 256 cells, rows in patches
     const long long n = (long long)L * 256 + tid;
     i = (int)(n % b.mx);
-    const long long r = n / b.mx;
-    j = (int)(r % b.my);
-    k = (int)(r / b.my);
-    return k < b.mz;
+    patch_row(n / b.mx, b.by, b.bz, b.ny, j, k);
+    return j < b.my && k < b.mz && n < (long long)b.mx * b.ny * b.by * b.nz * b.bz;
   }
   const int ux = L % b.nx, r = L / b.nx, uy = r % b.ny, uz = r / b.ny;
   i = ux * b.bx + tid % b.bx;
@@ -566,33 +631,38 @@ PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int t
 // update + CT + ghost copy + CFL of the new state
 // ===========================================================================================
 // CT of one face field (MHDRunFunctors3D.h:2600-2620); each component in the reference's order
-PPK_HD double ct3_a(double a, const double *__restrict__ emf, long long vs, long long t,
+template <class R>
+PPK_HD double ct3_a(double a, const R *__restrict__ emf, long long vs, long long t,
                     long long sy, long long sz, double dtdy, double dtdz) {
-  a += (emf[t + sy + I_EMFZ * vs] - emf[t + I_EMFZ * vs]) * dtdy;
-  a -= (emf[t + sz + I_EMFY * vs] - emf[t + I_EMFY * vs]) * dtdz;
+  a += (double(emf[t + sy + I_EMFZ * vs]) - double(emf[t + I_EMFZ * vs])) * dtdy;
+  a -= (double(emf[t + sz + I_EMFY * vs]) - double(emf[t + I_EMFY * vs])) * dtdz;
   return a;
 }
-PPK_HD double ct3_b(double b, const double *__restrict__ emf, long long vs, long long t,
+template <class R>
+PPK_HD double ct3_b(double b, const R *__restrict__ emf, long long vs, long long t,
                     long long sx, long long sz, double dtdx, double dtdz) {
-  b -= (emf[t + sx + I_EMFZ * vs] - emf[t + I_EMFZ * vs]) * dtdx;
-  b += (emf[t + sz + I_EMFX * vs] - emf[t + I_EMFX * vs]) * dtdz;
+  b -= (double(emf[t + sx + I_EMFZ * vs]) - double(emf[t + I_EMFZ * vs])) * dtdx;
+  b += (double(emf[t + sz + I_EMFX * vs]) - double(emf[t + I_EMFX * vs])) * dtdz;
   return b;
 }
-PPK_HD double ct3_c(double c, const double *__restrict__ emf, long long vs, long long t,
+template <class R>
+PPK_HD double ct3_c(double c, const R *__restrict__ emf, long long vs, long long t,
                     long long sx, long long sy, double dtdx, double dtdy) {
-  c += (emf[t + sx + I_EMFY * vs] - emf[t + I_EMFY * vs]) * dtdx;
-  c -= (emf[t + sy + I_EMFX * vs] - emf[t + I_EMFX * vs]) * dtdy;
+  c += (double(emf[t + sx + I_EMFY * vs]) - double(emf[t + I_EMFY * vs])) * dtdx;
+  c -= (double(emf[t + sy + I_EMFX * vs]) - double(emf[t + I_EMFX * vs])) * dtdy;
   return c;
 }
 // 2D CT (MHDRunFunctors2D.h:1690-1691)
-PPK_HD double ct2_a(double a, const double *__restrict__ emf, long long t, long long sy,
+template <class R>
+PPK_HD double ct2_a(double a, const R *__restrict__ emf, long long t, long long sy,
                     double dtdy) {
-  a += (emf[t + sy] - emf[t]) * dtdy;
+  a += (double(emf[t + sy]) - double(emf[t])) * dtdy;
   return a;
 }
-PPK_HD double ct2_b(double b, const double *__restrict__ emf, long long t, long long sx,
+template <class R>
+PPK_HD double ct2_b(double b, const R *__restrict__ emf, long long t, long long sx,
                     double dtdx) {
-  b -= (emf[t + sx] - emf[t]) * dtdx;
+  b -= (double(emf[t + sx]) - double(emf[t])) * dtdx;
   return b;
 }
 
@@ -601,8 +671,9 @@ PPK_HD double ct2_b(double b, const double *__restrict__ emf, long long t, long 
 // primitive of an interior cell needs the new face field of its +1 neighbour: the CT value of
 // that face, or past the last interior cell the value the next ghost fill puts there
 // (periodic image, or +-own field, BoundariesFunctors.h).
+template <class R>
 PPK_HD double update_cell(const double *__restrict__ Uin, double *__restrict__ Uout,
-                          const double *__restrict__ flux, const double *__restrict__ emf,
+                          const R *__restrict__ flux, const R *__restrict__ emf,
                           const DevParams &p, long long t, double dtdx, double dtdy, double dtdz,
                           bool fold) {
   const long long vs = p.vstride, sx = 1, sy = p.isize, sz = p.plane;
@@ -619,7 +690,7 @@ PPK_HD double update_cell(const double *__restrict__ Uin, double *__restrict__ U
   if (in) {
     if (p.dim == 3) {
       auto F = [&](int d, int m, long long off) {
-        return flux[((long long)d * kFluxComp3 + m) * vs + t + off];
+        return double(flux[((long long)d * kFluxComp3 + m) * vs + t + off]);
       };
       u[ID] += F(0, 0, 0) * dtdx; u[IP] += F(0, 1, 0) * dtdx; u[IU] += F(0, 2, 0) * dtdx;
       u[IV] += F(0, 3, 0) * dtdx; u[IW] += F(0, 4, 0) * dtdx;
@@ -668,7 +739,7 @@ PPK_HD double update_cell(const double *__restrict__ Uin, double *__restrict__ U
       }
     } else {
       auto F = [&](int d, int m, long long off) {
-        return flux[((long long)d * kFluxComp2 + m) * vs + t + off];
+        return double(flux[((long long)d * kFluxComp2 + m) * vs + t + off]);
       };
       u[ID] += F(0, 0, 0) * dtdx; u[IP] += F(0, 1, 0) * dtdx; u[IU] += F(0, 2, 0) * dtdx;
       u[IV] += F(0, 3, 0) * dtdx; u[IW] += F(0, 4, 0) * dtdx; u[IC] += F(0, 5, 0) * dtdx;
@@ -735,76 +806,119 @@ constexpr int kRiemannBlock = 256;
 
 __device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
 
-template <class T>
+template <class T, class R>
 __global__ void __launch_bounds__(T::NT) k_half3d(const double *__restrict__ U,
-                                                  double *__restrict__ rec, DevParams p,
-                                                  const double *ctrl, int ntx, int nty) {
+                                                  R *__restrict__ rec, DevParams p,
+                                                  const double *ctrl, int ntx, int nty, int ntz) {
   if (!active(ctrl)) return;
-  __shared__ double lds[T::LDS];
-  double *sF = lds, *sQ = lds + 3 * T::NF, *sE = sQ + 8 * T::NQ;
+  __shared__ R lds[T::LDS];
+  R *sF = lds, *sQ = lds + 3 * T::NF, *sE = sQ + 8 * T::NQ;
+  // tiles: x fastest, (y, z) tile rows in patches, each XCD a contiguous range
   const int L = xcd_remap(blockIdx.x, gridDim.x);
+  int ty, tz;
+  patch_row(L / ntx, kPatch, kPatch, (nty + kPatch - 1) / kPatch, ty, tz);
+  if (ty >= nty || tz >= ntz) return;  // padding tiles of the last patches
   const int b0x = p.gw - 1 + (L % ntx) * T::TX;
-  const int b0y = p.gw - 1 + ((L / ntx) % nty) * T::TY;
-  const int b0z = p.gw - 1 + (L / (ntx * nty)) * T::TZ;
+  const int b0y = p.gw - 1 + ty * T::TY;
+  const int b0z = p.gw - 1 + tz * T::TZ;
   const double dt = ctrl[CTRL_DT];
   const int tid = threadIdx.x;
-  half3_load<T>(U, p, b0x, b0y, b0z, tid, sF, sQ);
+  half3_load<T, R>(U, p, b0x, b0y, b0z, tid, sF, sQ);
   __syncthreads();
-  half3_prims<T>(p, b0x, b0y, b0z, tid, sF, sQ);
+  half3_prims<T, R>(p, b0x, b0y, b0z, tid, sF, sQ);
   __syncthreads();
-  half3_elec<T>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
+  half3_elec<T, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
   __syncthreads();
-  half3_record<T>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
+  half3_record<T, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
 }
 
+template <class R>
 __global__ void __launch_bounds__(Tile2::NT) k_half2d(const double *__restrict__ U,
-                                                      double *__restrict__ rec, DevParams p,
+                                                      R *__restrict__ rec, DevParams p,
                                                       const double *ctrl, int ntx) {
   if (!active(ctrl)) return;
-  __shared__ double lds[Tile2::LDS];
-  double *sF = lds, *sQ = lds + 2 * Tile2::NF, *sE = sQ + 8 * Tile2::NQ;
+  __shared__ R lds[Tile2::LDS];
+  R *sF = lds, *sQ = lds + 2 * Tile2::NF, *sE = sQ + 8 * Tile2::NQ;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int b0x = p.gw - 1 + (L % ntx) * Tile2::TX;
   const int b0y = p.gw - 1 + (L / ntx) * Tile2::TY;
   const double dt = ctrl[CTRL_DT];
   const int tid = threadIdx.x;
-  half2_load<Tile2>(U, p, b0x, b0y, tid, sF, sQ);
+  half2_load<Tile2, R>(U, p, b0x, b0y, tid, sF, sQ);
   __syncthreads();
-  half2_prims<Tile2>(p, b0x, b0y, tid, sF, sQ);
+  half2_prims<Tile2, R>(p, b0x, b0y, tid, sF, sQ);
   __syncthreads();
-  half2_elec<Tile2>(p, b0x, b0y, tid, sF, sQ, sE);
+  half2_elec<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE);
   __syncthreads();
-  half2_record<Tile2>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
+  half2_record<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
 }
 
 // one thread per cell of the sub-box (bricks, XCD-contiguous); a launch covers one direction
-template <int D>
-__global__ void __launch_bounds__(kRiemannBlock) k_face3d(const double *__restrict__ rec,
-                                                          double *__restrict__ flux, DevParams p,
+template <int D, class R>
+__global__ void __launch_bounds__(kRiemannBlock) k_face3d(const R *__restrict__ rec,
+                                                          R *__restrict__ flux, DevParams p,
                                                           Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
   if (c >= 0) face3<D>(rec, flux, p, c);
 }
 // 4 waves per SIMD: the EMF solve fits 128 VGPRs without spilling
-template <int E>
+template <int E, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_edge3d(const double *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
+k_edge3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
          const double *ctrl) {
   if (!active(ctrl)) return;
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
   if (c >= 0) edge3<E>(rec, emf, p, c);
 }
-template <int D>
-__global__ void __launch_bounds__(kRiemannBlock) k_face2d(const double *__restrict__ rec,
-                                                          double *__restrict__ flux, DevParams p,
+// All solves of the step in one launch: block B does solve type B % NTYPE (3D: faces x, y, z,
+// edges z, y, x; 2D: faces x, y, edge z) for the cells of run B / NTYPE.  Blocks are renumbered
+// per XCD first, so the six solves of a run execute on one XCD close in time: each record is
+// read from HBM about once and then from that XCD's L2, and every CU interleaves the
+// memory-bound face waves with the VALU-bound edge waves.
+template <int W, class R>
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
+k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux,
+                R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl) {
+  if (!active(ctrl)) return;
+  const int B = xcd_remap(blockIdx.x, gridDim.x);
+  const long long c = sub_brick_cell(p, bk, B / 6, threadIdx.x);
+  if (c < 0) return;
+  switch (B % 6) {
+    case 0: face3<0>(rec, flux, p, c); break;
+    case 1: face3<1>(rec, flux, p, c); break;
+    case 2: face3<2>(rec, flux, p, c); break;
+    case 3: edge3<2>(rec, emf, p, c); break;
+    case 4: edge3<1>(rec, emf, p, c); break;
+    default: edge3<0>(rec, emf, p, c); break;
+  }
+}
+template <class R>
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
+                R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl) {
+  if (!active(ctrl)) return;
+  const int B = xcd_remap(blockIdx.x, gridDim.x);
+  const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
+  if (c < 0) return;
+  switch (B % 3) {
+    case 0: face2<0>(rec, flux, p, c); break;
+    case 1: face2<1>(rec, flux, p, c); break;
+    default: edge2(rec, emf, p, c); break;
+  }
+}
+
+template <int D, class R>
+__global__ void __launch_bounds__(kRiemannBlock) k_face2d(const R *__restrict__ rec,
+                                                          R *__restrict__ flux, DevParams p,
                                                           Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
   if (c >= 0) face2<D>(rec, flux, p, c);
 }
+template <class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_edge2d(const double *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
+k_edge2d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
          const double *ctrl) {
   if (!active(ctrl)) return;
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
@@ -812,10 +926,11 @@ k_edge2d(const double *__restrict__ rec, double *__restrict__ emf, DevParams p, 
 }
 
 // one block per brick of the padded box; the block's CFL partial goes to parts[blockIdx.x]
+template <class R>
 __global__ void __launch_bounds__(kCflBlock) k_update(const double *__restrict__ Uin,
                                                       double *__restrict__ Uout,
-                                                      const double *__restrict__ flux,
-                                                      const double *__restrict__ emf,
+                                                      const R *__restrict__ flux,
+                                                      const R *__restrict__ emf,
                                                       double *__restrict__ parts, DevParams p,
                                                       Brick bk, const double *ctrl, int fold) {
   if (!active(ctrl)) return;
@@ -846,13 +961,72 @@ __global__ void __launch_bounds__(kCflBlock) k_cfl(const double *__restrict__ U,
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-template <class T>
-void launch_half3(const DevParams &p, const double *Uin, double *rec, const double *ctrl,
+template <class T, class R>
+void launch_half3(const DevParams &p, const double *Uin, R *rec, const double *ctrl,
                   hipStream_t s) {
   const int ntx = ceil_div(p.nx + 2, T::TX), nty = ceil_div(p.ny + 2, T::TY);
   const int ntz = ceil_div(p.nz + 2, T::TZ);
-  hipLaunchKernelGGL(k_half3d<T>, dim3(ntx * nty * ntz), dim3(T::NT), 0, s, Uin, rec, p, ctrl,
-                     ntx, nty);
+  const int rows = ceil_div(nty, kPatch) * kPatch * ceil_div(ntz, kPatch) * kPatch;
+  hipLaunchKernelGGL((k_half3d<T, R>), dim3(ntx * rows), dim3(T::NT), 0, s, Uin, rec, p, ctrl,
+                     ntx, nty, ntz);
+}
+
+// the step's three stages in arithmetic type R (double, or float for the FP32-bulk form);
+// rec, flux and emf hold R
+template <class R>
+hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *rec, R *flux,
+                        R *emf, double *parts, const double *ctrl, int fold_cfl, hipStream_t s,
+                        StageTimer *tm) {
+  if (p.dim == 3) {
+    if (tm) tm->mark(ST_FUSED_HALF, true);
+    switch (p.tile3) {
+      case 1: launch_half3<Tile3b>(p, Uin, rec, ctrl, s); break;
+      case 2: launch_half3<Tile3c>(p, Uin, rec, ctrl, s); break;
+      case 3: launch_half3<Tile3d>(p, Uin, rec, ctrl, s); break;
+      case 4: launch_half3<Tile3e>(p, Uin, rec, ctrl, s); break;
+      default: launch_half3<Tile3>(p, Uin, rec, ctrl, s); break;
+    }
+    if (tm) tm->mark(ST_FUSED_HALF, false);
+    const Brick bk = sub_bricks(p);
+    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
+    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
+    if (p.riemann_mix == 2) {  // 3 waves per SIMD, no spill
+      hipLaunchKernelGGL((k_riemann_mix3d<3, R>), dim3((unsigned)(6 * bk.count())), b, 0, s, rec,
+                         flux, emf, p, bk, ctrl);
+    } else if (p.riemann_mix) {
+      hipLaunchKernelGGL((k_riemann_mix3d<4, R>), dim3((unsigned)(6 * bk.count())), b, 0, s, rec,
+                         flux, emf, p, bk, ctrl);
+    } else {
+      hipLaunchKernelGGL((k_face3d<0, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
+      hipLaunchKernelGGL((k_face3d<1, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
+      hipLaunchKernelGGL((k_face3d<2, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edge3d<2, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edge3d<1, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edge3d<0, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+    }
+    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
+  } else {
+    const int ntx = ceil_div(p.nx + 2, Tile2::TX), nty = ceil_div(p.ny + 2, Tile2::TY);
+    PPK_TIMED(tm, ST_FUSED_HALF,
+              hipLaunchKernelGGL((k_half2d<R>), dim3(ntx * nty), dim3(Tile2::NT), 0, s, Uin, rec,
+                                 p, ctrl, ntx));
+    const Brick bk = sub_bricks(p);
+    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
+    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
+    if (p.riemann_mix) {
+      hipLaunchKernelGGL((k_riemann_mix2d<R>), dim3((unsigned)(3 * bk.count())), b, 0, s, rec,
+                         flux, emf, p, bk, ctrl);
+    } else {
+      hipLaunchKernelGGL((k_face2d<0, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
+      hipLaunchKernelGGL((k_face2d<1, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edge2d<R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+    }
+    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
+  }
+  PPK_TIMED(tm, ST_FUSED_UPDATE,
+            hipLaunchKernelGGL((k_update<R>), dim3(fused_cfl_parts(p)), dim3(kCflBlock), 0, s,
+                               Uin, Uout, flux, emf, parts, p, padded_bricks(p), ctrl, fold_cfl));
+  return hipGetLastError();
 }
 
 }  // namespace fusedk
@@ -864,43 +1038,13 @@ hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout, Fu
                          double *parts, const double *ctrl, int fold_cfl, hipStream_t s,
                          StageTimer *tm) {
   using namespace fusedk;
-  if (p.dim == 3) {
-    if (tm) tm->mark(ST_FUSED_HALF, true);
-    switch (p.tile3) {
-      case 1: launch_half3<Tile3b>(p, Uin, w.rec, ctrl, s); break;
-      case 2: launch_half3<Tile3c>(p, Uin, w.rec, ctrl, s); break;
-      case 3: launch_half3<Tile3d>(p, Uin, w.rec, ctrl, s); break;
-      case 4: launch_half3<Tile3e>(p, Uin, w.rec, ctrl, s); break;
-      default: launch_half3<Tile3>(p, Uin, w.rec, ctrl, s); break;
-    }
-    if (tm) tm->mark(ST_FUSED_HALF, false);
-    const Brick bk = sub_bricks(p);
-    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
-    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
-    hipLaunchKernelGGL(k_face3d<0>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
-    hipLaunchKernelGGL(k_face3d<1>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
-    hipLaunchKernelGGL(k_face3d<2>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
-    hipLaunchKernelGGL(k_edge3d<2>, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
-    hipLaunchKernelGGL(k_edge3d<1>, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
-    hipLaunchKernelGGL(k_edge3d<0>, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
-    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
-  } else {
-    const int ntx = ceil_div(p.nx + 2, Tile2::TX), nty = ceil_div(p.ny + 2, Tile2::TY);
-    PPK_TIMED(tm, ST_FUSED_HALF,
-              hipLaunchKernelGGL(k_half2d, dim3(ntx * nty), dim3(Tile2::NT), 0, s, Uin, w.rec,
-                                 p, ctrl, ntx));
-    const Brick bk = sub_bricks(p);
-    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
-    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
-    hipLaunchKernelGGL(k_face2d<0>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
-    hipLaunchKernelGGL(k_face2d<1>, g, b, 0, s, w.rec, w.flux, p, bk, ctrl);
-    hipLaunchKernelGGL(k_edge2d, g, b, 0, s, w.rec, w.emf, p, bk, ctrl);
-    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
-  }
-  PPK_TIMED(tm, ST_FUSED_UPDATE,
-            hipLaunchKernelGGL(k_update, dim3(fused_cfl_parts(p)), dim3(kCflBlock), 0, s, Uin,
-                               Uout, w.flux, w.emf, parts, p, padded_bricks(p), ctrl, fold_cfl));
-  return hipGetLastError();
+  if (p.bulk32)
+    return godunov_impl<float>(p, Uin, Uout, static_cast<float *>(w.rec),
+                               static_cast<float *>(w.flux), static_cast<float *>(w.emf), parts,
+                               ctrl, fold_cfl, s, tm);
+  return godunov_impl<double>(p, Uin, Uout, static_cast<double *>(w.rec),
+                              static_cast<double *>(w.flux), static_cast<double *>(w.emf), parts,
+                              ctrl, fold_cfl, s, tm);
 }
 
 hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const double *ctrl,

@@ -66,10 +66,11 @@ constexpr int kRec3 = 38;   // half-step record per cell, 3D
 constexpr int kRec2 = 26;   // half-step record per cell, 2D
 constexpr int kFluxComp3 = 5;  // ID IP IU IV IW per face direction, 3D
 constexpr int kFluxComp2 = 6;  // ID IP IU IV IW IC per face direction, 2D
+// element type: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {
-  double *rec = nullptr;   // kRec{2,3} x cells
-  double *flux = nullptr;  // dim x kFluxComp{2,3} x cells
-  double *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
+  void *rec = nullptr;   // kRec{2,3} x cells
+  void *flux = nullptr;  // dim x kFluxComp{2,3} x cells
+  void *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
 };
 
 // dt modes of the set_dt kernel
@@ -88,15 +89,18 @@ inline int cfl_blocks(const DevParams &p) {
   return g < 1 ? 1 : (int)g;
 }
 
-// Bricks: the fused per-cell kernels give each block of 256 threads a brick of cells, so
-// that lower neighbours a cell reads mostly belong to its own brick; shape 0 (default) is a
-// run of 256 consecutive cells of the flattened range (one wave = 64 cells of one row), shape
-// 1 is 64 x 2 x 2, shape 2 16 x 4 x 4, shape 3 32 x 4 x 2 (2D: 64 x 4 for shapes 1-3).
+// Bricks: the fused per-cell kernels give each block of 256 threads a brick of cells.
+// Shape 0 (default) is a run of 256 consecutive cells of the flattened range (one wave = 64
+// cells of one x row); its rows are numbered in patches of 4 x 4 (y, z) rows in 3D, so the
+// rows a cell's y and z neighbours live in were visited a few rows earlier (L2 hits).
+// Shapes 1-3 are 3D bricks 64 x 2 x 2, 16 x 4 x 4, 32 x 4 x 2 (2D: 64 x 4), kept for tuning.
+constexpr int kPatch = 4;
 struct Brick {
-  int bx, by, bz;  // brick extents (bx = 0: flattened runs of 256 cells)
-  int nx, ny, nz;  // bricks per direction (flattened: nx = number of runs)
+  int bx, by, bz;  // brick extents; bx = 0: flattened runs, by x bz = row patch
+  int nx, ny, nz;  // bricks per direction; flattened: patches per direction in ny, nz
   int mx, my, mz;  // cells covered per direction
-  long long count() const { return (long long)nx * ny * nz; }
+  long long nblocks;
+  long long count() const { return nblocks; }
 };
 inline Brick make_brick(int dim, int mx, int my, int mz, int shape) {
   Brick b;
@@ -104,9 +108,14 @@ inline Brick make_brick(int dim, int mx, int my, int mz, int shape) {
   b.my = my;
   b.mz = mz;
   if (shape <= 0 || shape > 3) {
-    b.bx = b.by = b.bz = 0;
-    b.nx = (int)(((long long)mx * my * mz + 255) / 256);
-    b.ny = b.nz = 1;
+    b.bx = 0;
+    b.by = dim == 3 ? kPatch : 1;
+    b.bz = dim == 3 ? kPatch : 1;
+    b.ny = (my + b.by - 1) / b.by;
+    b.nz = (mz + b.bz - 1) / b.bz;
+    const long long rows = (long long)b.ny * b.by * b.nz * b.bz;
+    b.nx = 0;
+    b.nblocks = (rows * mx + 255) / 256;
     return b;
   }
   static const int shp[4][3] = {{0, 0, 0}, {64, 2, 2}, {16, 4, 4}, {32, 4, 2}};
@@ -116,6 +125,7 @@ inline Brick make_brick(int dim, int mx, int my, int mz, int shape) {
   b.nx = (mx + b.bx - 1) / b.bx;
   b.ny = (my + b.by - 1) / b.by;
   b.nz = (mz + b.bz - 1) / b.bz;
+  b.nblocks = (long long)b.nx * b.ny * b.nz;
   return b;
 }
 // bricks of the padded box (fused update, fused CFL)

@@ -39,6 +39,8 @@ struct DevParams {
   double gamma0, slope_type, smallr, smallc, smallp, cIso, cfl;
   int tile3;            // 3D half-step tile shape (index into the compiled shapes), tuning only
   int brick;            // block-to-cells shape of the fused per-cell kernels, tuning only
+  int riemann_mix;      // 1: all face and edge solves in one interleaved launch
+  int bulk32;           // FP32-bulk form: half step, records, face/edge solves in float
 };
 
 template <class R>
@@ -50,6 +52,9 @@ struct St {
 
 template <class R>
 PPK_HD R dmax(R a, R b) { return fmax(a, b); }
+// |x| in the argument's own type (host fabs(float) would widen to double)
+PPK_HD double dabs(double x) { return __builtin_fabs(x); }
+PPK_HD float dabs(float x) { return __builtin_fabsf(x); }
 // sign bit test (true for -0 and negative values), host and device
 template <class R>
 PPK_HD bool sign_bit(R x) { return __builtin_signbit(x); }
@@ -108,6 +113,17 @@ PPK_HD R msqrt(R x) { return kSharedRecip ? R(fast_sqrt(double(x))) : sqrt(x); }
 // a / sqrt(r)
 template <class R>
 PPK_HD R over_sqrt(R a, R r) { return kSharedRecip ? a * R(fast_rsq(double(r))) : a / sqrt(r); }
+// float (FP32-bulk form): single-instruction reciprocal, sqrt and rsq on the GPU (about
+// 1 ulp of float, far inside that form's error budget); IEEE float elsewhere
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(PPK_EXACT_DIV)
+PPK_HD float recip(float b) { return __builtin_amdgcn_rcpf(b); }
+PPK_HD float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+PPK_HD float over_sqrt(float a, float r) { return a * __builtin_amdgcn_rsqf(r); }
+#else
+PPK_HD float recip(float b) { return 1.0f / b; }
+PPK_HD float msqrt(float x) { return sqrtf(x); }
+PPK_HD float over_sqrt(float a, float r) { return a / sqrtf(r); }
+#endif
 template <class R>
 PPK_HD R dmin(R a, R b) { return fmin(a, b); }
 
@@ -157,12 +173,12 @@ PPK_HD double inv_dt_cell(const St<double> &q, const DevParams &p) {
   double c2 = gamma0 * pp / d;
   double d2 = 0.5 * (b2 / d + c2);
   double cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
-  double vx = cf + fabs(q[IU]);
+  double vx = cf + dabs(q[IU]);
   cf = sqrt(d2 + sqrt(d2 * d2 - c2 * b * b / d));
-  double vy = cf + fabs(q[IV]);
+  double vy = cf + dabs(q[IV]);
   if (p.dim == 3) {
     cf = sqrt(d2 + sqrt(d2 * d2 - c2 * c * c / d));
-    double vz = cf + fabs(q[IW]);
+    double vz = cf + dabs(q[IW]);
     return vx / p.dx + vy / p.dy + vz / p.dz;
   }
   return vx / p.dx + vy / p.dy;
@@ -199,10 +215,10 @@ PPK_HD R limited_slope(R st, R q, R qplus, R qminus) {
   R drgt = st * (qplus - q);
   R dcen = R(0.5) * (qplus - qminus);
   R dsgn = (dcen >= R(0)) ? R(1) : R(-1);
-  R slop = dmin(fabs(dlft), fabs(drgt));
+  R slop = dmin(dabs(dlft), dabs(drgt));
   R dlim = slop;
   if ((dlft * drgt) <= R(0)) dlim = R(0);
-  return dsgn * dmin(dlim, fabs(dcen));
+  return dsgn * dmin(dlim, dabs(dcen));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -260,7 +276,7 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   R estar = rl * (sl - ul) * (sl - ustar) - a * a;
   R el = rl * (sl - ul) * (sl - ul) - a * a;
   R vstarl, wstarl, bstarl, cstarl;
-  if (a * a > R(0) && fabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
+  if (a * a > R(0) && dabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
     vstarl = vl;
     bstarl = bl;
     wstarl = wl;
@@ -276,7 +292,7 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   R etotstarl = divr((sl - ul) * etotl - ptotl * ul + ptotstar * ustar + a * (vdotbl - vdotbstarl),
                      sl - ustar, isl);
   R sqrrstarl = msqrt(rstarl);
-  R calfvenl = over_sqrt(fabs(a), rstarl);
+  R calfvenl = over_sqrt(dabs(a), rstarl);
   R sal = ustar - calfvenl;
 
   const R isr = recip(sr - ustar);
@@ -284,7 +300,7 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   estar = rr * (sr - ur) * (sr - ustar) - a * a;
   R er = rr * (sr - ur) * (sr - ur) - a * a;
   R vstarr, wstarr, bstarr, cstarr;
-  if (a * a > R(0) && fabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
+  if (a * a > R(0) && dabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
     vstarr = vr;
     bstarr = br;
     wstarr = wr;
@@ -300,7 +316,7 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   R etotstarr = divr((sr - ur) * etotr - ptotr * ur + ptotstar * ustar + a * (vdotbr - vdotbstarr),
                      sr - ustar, isr);
   R sqrrstarr = msqrt(rstarr);
-  R calfvenr = over_sqrt(fabs(a), rstarr);
+  R calfvenr = over_sqrt(dabs(a), rstarr);
   R sar = ustar + calfvenr;
 
   const R isq = recip(sqrrstarl + sqrrstarr);
@@ -411,7 +427,7 @@ PPK_HD void riemann_llf(St<R> &ql, St<R> &qr, St<R> &flux,
     R c2 = R(par.gamma0) * pp / d;
     R d2 = R(0.5) * (b2 / d + c2);
     R cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
-    return cf + fabs(q[IU]);
+    return cf + dabs(q[IU]);
   };
   R vel_info = dmax(info(ql), info(qr));
 #pragma unroll
@@ -519,14 +535,14 @@ PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
   R EstarRR = ustar * BstarRR - vstar * AstarRR;
 
   const R smallc = R(p.smallc);
-  R calfvenL = max5(over_sqrt(fabs(aLR), rstarLRx), over_sqrt(fabs(AstarLR), rstarLR),
-                    over_sqrt(fabs(aLL), rstarLLx), over_sqrt(fabs(AstarLL), rstarLL), smallc);
-  R calfvenR = max5(over_sqrt(fabs(aRR), rstarRRx), over_sqrt(fabs(AstarRR), rstarRR),
-                    over_sqrt(fabs(aRL), rstarRLx), over_sqrt(fabs(AstarRL), rstarRL), smallc);
-  R calfvenB = max5(over_sqrt(fabs(bLL), rstarLLy), over_sqrt(fabs(BstarLL), rstarLL),
-                    over_sqrt(fabs(bRL), rstarRLy), over_sqrt(fabs(BstarRL), rstarRL), smallc);
-  R calfvenT = max5(over_sqrt(fabs(bLR), rstarLRy), over_sqrt(fabs(BstarLR), rstarLR),
-                    over_sqrt(fabs(bRR), rstarRRy), over_sqrt(fabs(BstarRR), rstarRR), smallc);
+  R calfvenL = max5(over_sqrt(dabs(aLR), rstarLRx), over_sqrt(dabs(AstarLR), rstarLR),
+                    over_sqrt(dabs(aLL), rstarLLx), over_sqrt(dabs(AstarLL), rstarLL), smallc);
+  R calfvenR = max5(over_sqrt(dabs(aRR), rstarRRx), over_sqrt(dabs(AstarRR), rstarRR),
+                    over_sqrt(dabs(aRL), rstarRLx), over_sqrt(dabs(AstarRL), rstarRL), smallc);
+  R calfvenB = max5(over_sqrt(dabs(bLL), rstarLLy), over_sqrt(dabs(BstarLL), rstarLL),
+                    over_sqrt(dabs(bRL), rstarRLy), over_sqrt(dabs(BstarRL), rstarRL), smallc);
+  R calfvenT = max5(over_sqrt(dabs(bLR), rstarLRy), over_sqrt(dabs(BstarLR), rstarLR),
+                    over_sqrt(dabs(bRR), rstarRRy), over_sqrt(dabs(BstarRR), rstarRR), smallc);
 
   R SAL = dmin(ustar - calfvenL, R(0));
   R SAR = dmax(ustar + calfvenR, R(0));

@@ -72,9 +72,12 @@ PPK_HD void half_step_2d(const St<R> (&qNb)[3][3], const R (&bf)[4][4][3], R ELL
   R dBy = R(0.5) * (BR - BL);
 
   R sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy;
-  R su0 = (-u * dux - dpx / r - B * dBx / r - Cc * dCx / r) * dtdx + (-v * duy + B * dAy / r) * dtdy;
-  R sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - dpy / r - A * dAy / r - Cc * dCy / r) * dtdy;
-  R sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy;
+  const R ir = recip(r);
+  R su0 = (-u * dux - divr(dpx, r, ir) - divr(B * dBx, r, ir) - divr(Cc * dCx, r, ir)) * dtdx +
+          (-v * duy + divr(B * dAy, r, ir)) * dtdy;
+  R sv0 = (-u * dvx + divr(A * dBx, r, ir)) * dtdx +
+          (-v * dvy - divr(dpy, r, ir) - divr(A * dAy, r, ir) - divr(Cc * dCy, r, ir)) * dtdy;
+  R sw0 = (-u * dwx + divr(A * dCx, r, ir)) * dtdx + (-v * dwy + divr(B * dCy, r, ir)) * dtdy;
   R sp0 = (-u * dpx - dux * gamma * pp) * dtdx + (-v * dpy - dvy * gamma * pp) * dtdy;
   R sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy;
   R sB0 = (-u * dBx - B * dux + v * dAx + A * dvx) * dtdx;
@@ -219,12 +222,14 @@ PPK_HD void half_step_3d(const St<R> &q, const St<R> &qPX, const St<R> &qMX,
   const R dCz = R(0.5) * (CR - CL);
 
   R sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-w * drz - dwz * r) * dtdz;
-  R su0 = (-u * dux - (dpx + B * dBx + C * dCx) / r) * dtdx + (-v * duy + B * dAy / r) * dtdy +
-          (-w * duz + C * dAz / r) * dtdz;
-  R sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - (dpy + A * dAy + C * dCy) / r) * dtdy +
-          (-w * dvz + C * dBz / r) * dtdz;
-  R sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy +
-          (-w * dwz - (dpz + A * dAz + B * dBz) / r) * dtdz;
+  const R ir = recip(r);
+  R su0 = (-u * dux - divr(dpx + B * dBx + C * dCx, r, ir)) * dtdx +
+          (-v * duy + divr(B * dAy, r, ir)) * dtdy + (-w * duz + divr(C * dAz, r, ir)) * dtdz;
+  R sv0 = (-u * dvx + divr(A * dBx, r, ir)) * dtdx +
+          (-v * dvy - divr(dpy + A * dAy + C * dCy, r, ir)) * dtdy +
+          (-w * dvz + divr(C * dBz, r, ir)) * dtdz;
+  R sw0 = (-u * dwx + divr(A * dCx, r, ir)) * dtdx + (-v * dwy + divr(B * dCy, r, ir)) * dtdy +
+          (-w * dwz - divr(dpz + A * dAz + B * dBz, r, ir)) * dtdz;
   R sp0 = (-u * dpx - dux * gamma * pp) * dtdx + (-v * dpy - dvy * gamma * pp) * dtdy +
           (-w * dpz - dwz * gamma * pp) * dtdz;
   R sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
