@@ -226,6 +226,19 @@ const char *ppk_mhd_timer_name(int stage);
 int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max);
 int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]);
 
+/* ---- output ---------------------------------------------------------------------------- */
+/* Write a state (host array, reference layout) as VTK ImageData <dir>/<prefix>_<istep>.vti
+ * (istep zero-padded to 7 digits), byte for byte as the reference does: CellData arrays rho,
+ * energy, rho_vx, rho_vy, rho_vz, bx, by, bz over the interior cells, appended raw binary
+ * (ascii = 0) or ascii text (ascii = 1, the reference's [output] outputVtkAscii).
+ * Replaces: save_VTK_2D / save_VTK_3D (src/utils/io/IO_VTK.cpp:23-410) with the variable
+ * names of src/shared/SolverBase.cpp:48-55. */
+int ppk_mhd_write_vtk(const ppk_mhd_params *params, const double *U, const char *dir,
+                      const char *prefix, int istep, int ascii);
+/* The same for the handle's current state (copies it to the host; synchronizes).
+ * Replaces: SolverMHDMuscl::save_solution_impl (src/muscl/SolverMHDMuscl.h:894-906). */
+int ppk_mhd_save_vtk(ppk_mhd_t *h, const char *dir, const char *prefix, int istep, int ascii);
+
 #ifdef __cplusplus
 }
 #endif

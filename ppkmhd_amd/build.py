@@ -47,6 +47,7 @@ UNITS = [
     ("capi.cpp", "capi.o", ["-x", "hip"]),
     ("config.cpp", "config.o", []),
     ("ics.cpp", "ics.o", ["-ffp-contract=off"]),
+    ("io_vtk.cpp", "io_vtk.o", []),
 ]
 
 

@@ -628,6 +628,16 @@ int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max) {
   return PPK_OK;
 }
 
+int ppk_mhd_save_vtk(ppk_mhd_t *h, const char *dir, const char *prefix, int istep, int ascii) {
+  if (!h || !dir || !prefix) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::vector<double> host((size_t)h->cells() * 8);
+  int rc = ppk_mhd_download(h, host.data(), host.size());
+  if (rc != PPK_OK) return rc;
+  rc = ppk_mhd_write_vtk(&h->p, host.data(), dir, prefix, istep, ascii);
+  if (rc != PPK_OK) return fail(h, rc, ppk::thread_error());
+  return PPK_OK;
+}
+
 int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]) {
   if (!h || !sums) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   HIP_CHECK(h, ppk::diag_sums(h->dp, h->cur(), h->diag, ppk::kDiagBlocks, h->stream));

@@ -171,6 +171,24 @@ class SolverMHDMuscl:
         self._check(lib().ppk_mhd_conserved_sums(self._h, s))
         return np.array(list(s))
 
+    def save_vtk(self, directory=".", prefix="output", istep=None, ascii=False):
+        """Write the current state as <directory>/<prefix>_<istep>.vti in the reference's VTK
+        format (istep defaults to the iteration count)."""
+        if istep is None:
+            istep = self.m_iteration
+        self._check(lib().ppk_mhd_save_vtk(self._h, str(directory).encode(), prefix.encode(),
+                                           int(istep), 1 if ascii else 0))
+
+
+def write_vtk(params, U, directory=".", prefix="output", istep=0, ascii=False):
+    """Write a host state array (reference layout) in the reference's VTK format (no GPU)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    if U.size != params.count:
+        raise ValueError("state must hold 8*isize*jsize*ksize doubles")
+    check(lib().ppk_mhd_write_vtk(C.byref(params), U.ctypes.data_as(C.POINTER(C.c_double)),
+                                  str(directory).encode(), prefix.encode(), int(istep),
+                                  1 if ascii else 0))
+
 
 class SolverFactory:
     """Registry of solver constructors keyed by the reference's solver names

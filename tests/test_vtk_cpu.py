@@ -1,0 +1,92 @@
+"""VTK output in the reference's format (src/utils/io/IO_VTK.cpp), from a host array: no GPU.
+
+The expected headers below are the reference writer's text for these grids (stream-formatted
+numbers, version 1.0 in 2D and 0.1 in 3D, offsets of UInt64 byte count + data per variable);
+the appended block is parsed back by its offsets and compared with the interior cells.
+"""
+import struct
+
+import numpy as np
+
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd import make_params
+from ppkmhd_amd.solver import write_vtk
+
+NAMES = ["rho", "energy", "rho_vx", "rho_vy", "rho_vz", "bx", "by", "bz"]
+
+
+def _state(p, seed):
+    return np.random.default_rng(seed).normal(size=p.shape)
+
+
+def _split_binary(data):
+    head, _, tail = data.partition(b"  <AppendedData encoding=\"raw\">\n_")
+    body, _, foot = tail.rpartition(b"  </AppendedData>\n</VTKFile>\n")
+    assert foot == b""
+    return head.decode(), body
+
+
+def test_vtk_2d_binary(tmp_path):
+    p = make_params(2, 5, 4, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0)
+    U = _state(p, 1)
+    write_vtk(p, U, tmp_path, "out", 12)
+    data = (tmp_path / "out_0000012.vti").read_bytes()
+    head, body = _split_binary(data)
+    expect = (
+        '<VTKFile type="ImageData" version="1.0" byte_order="LittleEndian" header_type="UInt64">\n'
+        '  <ImageData WholeExtent="0 5 0 4 0 0" Origin="0 0 0" Spacing="0.2 0.25 0">\n'
+        '  <Piece Extent="0 5 0 4 0 0 ">\n'
+        "    <PointData>\n"
+        "    </PointData>\n"
+        "    <CellData>\n"
+        + "".join(f'     <DataArray type="Float64" Name="{n}" format="appended" offset="{v * 168}" />\n'
+                  for v, n in enumerate(NAMES))
+        + "    </CellData>\n  </Piece>\n  </ImageData>\n")
+    assert head == expect
+    interior = U[p.interior()]
+    pos = 0
+    for v in range(8):
+        (nbytes,) = struct.unpack_from("<Q", body, pos)
+        assert nbytes == 5 * 4 * 8
+        vals = np.frombuffer(body, dtype="<f8", count=20, offset=pos + 8)
+        assert np.array_equal(vals, interior[v].reshape(-1))
+        pos += 8 + nbytes
+    assert pos == len(body)
+
+
+def test_vtk_3d_binary(tmp_path):
+    p = make_params(3, 4, 3, 2, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+    U = _state(p, 2)
+    write_vtk(p, U, tmp_path, "blast", 0)
+    head, body = _split_binary((tmp_path / "blast_0000000.vti").read_bytes())
+    lines = head.split("\n")
+    assert lines[0] == ('<VTKFile type="ImageData" version="0.1" byte_order="LittleEndian" '
+                        'header_type="UInt64">')
+    assert lines[1] == ('  <ImageData WholeExtent="0 4 0 3 0 2" Origin="-0.5 -0.5 -0.5" '
+                        'Spacing="0.25 0.333333 0.5">')
+    assert lines[2] == '  <Piece Extent="0 4 0 3 0 2 ">'
+    ncell = 4 * 3 * 2
+    offsets = [int(l.split('offset="')[1].split('"')[0]) for l in lines if "offset=" in l]
+    assert offsets == [v * (ncell * 8 + 8) for v in range(8)]
+    interior = U[p.interior()]
+    for v in range(8):
+        (nbytes,) = struct.unpack_from("<Q", body, offsets[v])
+        assert nbytes == ncell * 8
+        vals = np.frombuffer(body, dtype="<f8", count=ncell, offset=offsets[v] + 8)
+        # i fastest, then j, then k
+        assert np.array_equal(vals, interior[v].reshape(-1))
+
+
+def test_vtk_ascii(tmp_path):
+    p = make_params(2, 3, 2)
+    U = _state(p, 3)
+    write_vtk(p, U, tmp_path, "a", 7, ascii=True)
+    text = (tmp_path / "a_0000007.vti").read_text()
+    assert text.startswith('<?xml version="1.0"?>\n<VTKFile type="ImageData" version="1.0"')
+    interior = U[p.interior()]
+    for v, name in enumerate(NAMES):
+        block = text.split(f'Name="{name}" format="ascii" >\n')[1].split("\n    </DataArray>")[0]
+        vals = np.array([float(x) for x in block.split()])
+        # default stream precision: 6 significant digits
+        np.testing.assert_allclose(vals, interior[v].reshape(-1), rtol=1e-5, atol=0)
+    assert text.endswith("    </CellData>\n  </Piece>\n  </ImageData>\n</VTKFile>\n")
