@@ -239,6 +239,35 @@ int ppk_mhd_write_vtk(const ppk_mhd_params *params, const double *U, const char 
  * Replaces: SolverMHDMuscl::save_solution_impl (src/muscl/SolverMHDMuscl.h:894-906). */
 int ppk_mhd_save_vtk(ppk_mhd_t *h, const char *dir, const char *prefix, int istep, int ascii);
 
+/* ---- batched ensembles ----------------------------------------------------------------------
+ * Many independent simulations of one dimension and precision (any sizes, gamma, cfl,
+ * boundaries, tEnd, n_step_max) advanced together: every stage of a step is one launch over
+ * all members, and steps after the first replay one captured hipGraph.  Each member follows
+ * its own time loop exactly as one ppk_mhd_t would with dt_mode PPK_DT_CFL (its own CFL dt,
+ * clamp to tEnd, finished() test), and its state after each step equals that handle's bit for
+ * bit in the same precision (the fused path's per-cell code is shared).
+ * Replaces: a loop over the reference's driver, one SolverMHDMuscl per member
+ * (src/main.cpp:130-160 with src/shared/SolverBase.cpp:161-219), for PPK_PATH_FUSED. */
+typedef struct ppk_mhd_batch ppk_mhd_batch;
+typedef ppk_mhd_batch ppk_mhd_batch_t;
+/* params: nmembers parameter blocks (kernel_path is ignored: the batch always runs fused). */
+int ppk_mhd_batch_create(const ppk_mhd_params *params, int nmembers, ppk_mhd_batch_t **handle);
+void ppk_mhd_batch_destroy(ppk_mhd_batch_t *h);
+const char *ppk_mhd_batch_last_error(const ppk_mhd_batch_t *h);
+int ppk_mhd_batch_size(const ppk_mhd_batch_t *h);
+/* problems: nmembers problem descriptions; t = 0, iteration = 0 for every member.
+ * Replaces: SolverMHDMuscl::init (src/muscl/SolverMHDMuscl.h:868-889), per member. */
+int ppk_mhd_batch_init_problem(ppk_mhd_batch_t *h, const ppk_mhd_problem *problems);
+/* Member state in the reference layout (count = 8*isize*jsize*ksize of that member). */
+int ppk_mhd_batch_upload(ppk_mhd_batch_t *h, int member, const double *U, size_t count);
+int ppk_mhd_batch_download(ppk_mhd_batch_t *h, int member, double *U, size_t count);
+/* nsteps steps of every member that has not finished; synchronizes at the end.
+ * Replaces: SolverBase::next_iteration (src/shared/SolverBase.cpp:161-219), per member. */
+int ppk_mhd_batch_run(ppk_mhd_batch_t *h, int nsteps);
+int ppk_mhd_batch_get_time(ppk_mhd_batch_t *h, int member, double *t, double *dt,
+                           int64_t *iteration);
+int ppk_mhd_batch_synchronize(ppk_mhd_batch_t *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ DT_FIXED, DT_CFL = 0, 1
 PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE = range(5)
 
 PPK_OK = 0
+PPK_ERR_INVALID_ARGUMENT, PPK_ERR_HIP, PPK_ERR_OUT_OF_MEMORY, PPK_ERR_UNSUPPORTED, PPK_ERR_CONFIG = 1, 2, 3, 4, 5
 
 
 class Params(C.Structure):

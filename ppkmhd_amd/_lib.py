@@ -50,6 +50,16 @@ SIGNATURES = {
     "ppk_mhd_write_vtk": (C.c_int, [C.POINTER(A.Params), _DP, C.c_char_p, C.c_char_p, C.c_int,
                                     C.c_int]),
     "ppk_mhd_save_vtk": (C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_int, C.c_int]),
+    "ppk_mhd_batch_create": (C.c_int, [_P, C.c_int, C.POINTER(_H)]),
+    "ppk_mhd_batch_destroy": (None, [_H]),
+    "ppk_mhd_batch_last_error": (C.c_char_p, [_H]),
+    "ppk_mhd_batch_size": (C.c_int, [_H]),
+    "ppk_mhd_batch_init_problem": (C.c_int, [_H, _PR]),
+    "ppk_mhd_batch_upload": (C.c_int, [_H, C.c_int, _DP, C.c_size_t]),
+    "ppk_mhd_batch_download": (C.c_int, [_H, C.c_int, _DP, C.c_size_t]),
+    "ppk_mhd_batch_run": (C.c_int, [_H, C.c_int]),
+    "ppk_mhd_batch_get_time": (C.c_int, [_H, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
+    "ppk_mhd_batch_synchronize": (C.c_int, [_H]),
 }
 
 _lib = None
@@ -83,5 +93,12 @@ def lib():
 def check(rc, handle=None):
     if rc != A.PPK_OK:
         msg = lib().ppk_mhd_last_error(handle)
+        raise PPKError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def check_batch(rc, handle=None):
+    if rc != A.PPK_OK:
+        msg = lib().ppk_mhd_batch_last_error(handle)
         raise PPKError(rc, msg.decode() if msg else "")
     return rc

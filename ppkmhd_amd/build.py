@@ -43,11 +43,14 @@ UNITS = [
      ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("fused.hip", "fused_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("fused.hip", "fused_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
+    ("batch.hip", "batch_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
+    ("batch.hip", "batch_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("diag.hip", "diag.o", []),
-    ("capi.cpp", "capi.o", ["-x", "hip"]),
+    ("capi.cpp", "capi.o", ["--offload-host-only"]),
     ("config.cpp", "config.o", []),
     ("ics.cpp", "ics.o", ["-ffp-contract=off"]),
     ("io_vtk.cpp", "io_vtk.o", []),
+    ("batch_capi.cpp", "batch_capi.o", ["--offload-host-only"]),
 ]
 
 

@@ -1,0 +1,277 @@
+// Batched ensemble of independent grids (same dimension, any sizes and parameters): every
+// stage of the fused step is one launch over the blocks of all members, each member with its
+// own time, dt (own CFL condition), iteration count and finished() test.  A block finds its
+// member by binary search on the members' first-block table; the member's current state is
+// U[iteration & 1], read from its own control block, so the same launch sequence serves every
+// step and is captured once in a hipGraph by the host.
+//
+// The per-thread bodies are those of the single-grid fused step (fused_body.hpp), so every
+// member's result equals the single-grid result bit for bit in the same arithmetic variant.
+#include <hip/hip_runtime.h>
+
+#include "fused_body.hpp"
+
+namespace ppk {
+namespace PPK_NS {
+namespace batchk {
+
+using namespace fusedk;
+
+constexpr int kBlock = 256;
+
+// member owning block b of stage st: the last member whose first block is <= b
+PPK_HD int find_member(const BatchMember *__restrict__ m, int nm, int st, long long b) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (m[mid].begin[st] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool member_active(const BatchMember &m) {
+  return m.ctrl[CTRL_ACTIVE] != 0.0;
+}
+__device__ __forceinline__ int parity(const BatchMember &m) {
+  return ((long long)m.ctrl[CTRL_ITER]) & 1;
+}
+
+// ---- per-member control: finished() test (SolverBase.cpp:195-201), dt, end of step ---------
+__global__ void k_bbegin(const BatchMember *__restrict__ mem, int nm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  double *ctrl = mem[i].ctrl;
+  const bool done = ctrl[CTRL_T] >= (ctrl[CTRL_TEND] - 1e-14) || ctrl[CTRL_ITER] >= ctrl[CTRL_NSTEPMAX];
+  ctrl[CTRL_ACTIVE] = done ? 0.0 : 1.0;
+}
+
+// one block per member: reduce its CFL partials, dt = cfl / max, clamped to tEnd - t
+__global__ void __launch_bounds__(kBlock) k_bset_dt(const BatchMember *__restrict__ mem) {
+  __shared__ double red[kBlock / 64];
+  const BatchMember &m = mem[blockIdx.x];
+  double *ctrl = m.ctrl;
+  if (ctrl[CTRL_ACTIVE] == 0.0) return;
+  double mx = 0.0;
+  for (int b = threadIdx.x; b < m.nparts; b += kBlock) mx = fmax(mx, m.parts[b]);
+  mx = block_max(mx, red);
+  if (threadIdx.x != 0) return;
+  ctrl[CTRL_INVDT] = mx;
+  double dt = m.p.cfl / mx;
+  if (ctrl[CTRL_T] + dt > ctrl[CTRL_TEND]) dt = ctrl[CTRL_TEND] - ctrl[CTRL_T];
+  ctrl[CTRL_DT] = dt;
+}
+
+__global__ void k_bend(const BatchMember *__restrict__ mem, int nm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  double *ctrl = mem[i].ctrl;
+  if (ctrl[CTRL_ACTIVE] == 0.0) return;
+  ctrl[CTRL_ITER] += 1.0;
+  ctrl[CTRL_T] += ctrl[CTRL_DT];
+}
+
+// ---- ghost fill of the current state, direction dir ---------------------------------------
+__global__ void __launch_bounds__(kBlock) k_bbnd(const BatchMember *__restrict__ mem, int nm,
+                                                 int dir) {
+  const long long gb = blockIdx.x;
+  const BatchMember &m = mem[find_member(mem, nm, BS_BND0 + dir, gb)];
+  if (!member_active(m)) return;
+  const long long t = (gb - m.begin[BS_BND0 + dir]) * kBlock + threadIdx.x;
+  if (t < bnd_elements(m.p, dir)) bnd_element(m.U[parity(m)], m.p, dir, t);
+}
+
+// ---- standalone CFL partials of the current state ------------------------------------------
+__global__ void __launch_bounds__(kCflBlock) k_bcfl(const BatchMember *__restrict__ mem, int nm) {
+  __shared__ double red[kCflBlock / 64];
+  const long long gb = blockIdx.x;
+  const BatchMember &m = mem[find_member(mem, nm, BS_CFL, gb)];
+  if (!member_active(m)) return;
+  const int L = (int)(gb - m.begin[BS_CFL]);
+  int i, j, k;
+  double best = 0.0;
+  if (brick_cell(m.pad, L, threadIdx.x, i, j, k))
+    best = cfl_cell(m.U[parity(m)], m.p, cidx(m.p, i, j, k));
+  best = block_max(best, red);
+  if (threadIdx.x == 0) m.parts[L] = best;
+}
+
+// ---- half step -----------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(Tile2::NT) k_bhalf2d(const BatchMember *__restrict__ mem,
+                                                       int nm) {
+  const long long gb = blockIdx.x;
+  const BatchMember &m = mem[find_member(mem, nm, BS_HALF, gb)];
+  if (!member_active(m)) return;
+  __shared__ R lds[Tile2::LDS];
+  R *sF = lds, *sQ = lds + 2 * Tile2::NF, *sE = sQ + 8 * Tile2::NQ;
+  const DevParams &p = m.p;
+  const int L = (int)(gb - m.begin[BS_HALF]);
+  const int ntx = (p.nx + 2 + Tile2::TX - 1) / Tile2::TX;
+  const int b0x = p.gw - 1 + (L % ntx) * Tile2::TX;
+  const int b0y = p.gw - 1 + (L / ntx) * Tile2::TY;
+  const double dt = m.ctrl[CTRL_DT];
+  const double *U = m.U[parity(m)];
+  R *rec = static_cast<R *>(m.rec);
+  const int tid = threadIdx.x;
+  half2_load<Tile2, R>(U, p, b0x, b0y, tid, sF, sQ);
+  __syncthreads();
+  half2_prims<Tile2, R>(p, b0x, b0y, tid, sF, sQ);
+  __syncthreads();
+  half2_elec<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE);
+  __syncthreads();
+  half2_record<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
+}
+
+template <class R>
+__global__ void __launch_bounds__(Tile3::NT) k_bhalf3d(const BatchMember *__restrict__ mem,
+                                                       int nm) {
+  const long long gb = blockIdx.x;
+  const BatchMember &m = mem[find_member(mem, nm, BS_HALF, gb)];
+  if (!member_active(m)) return;
+  __shared__ R lds[Tile3::LDS];
+  R *sF = lds, *sQ = lds + 3 * Tile3::NF, *sE = sQ + 8 * Tile3::NQ;
+  const DevParams &p = m.p;
+  const int L = (int)(gb - m.begin[BS_HALF]);
+  const int ntx = (p.nx + 2 + Tile3::TX - 1) / Tile3::TX;
+  const int nty = (p.ny + 2 + Tile3::TY - 1) / Tile3::TY;
+  const int b0x = p.gw - 1 + (L % ntx) * Tile3::TX;
+  const int b0y = p.gw - 1 + ((L / ntx) % nty) * Tile3::TY;
+  const int b0z = p.gw - 1 + (L / (ntx * nty)) * Tile3::TZ;
+  const double dt = m.ctrl[CTRL_DT];
+  const double *U = m.U[parity(m)];
+  R *rec = static_cast<R *>(m.rec);
+  const int tid = threadIdx.x;
+  half3_load<Tile3, R>(U, p, b0x, b0y, b0z, tid, sF, sQ);
+  __syncthreads();
+  half3_prims<Tile3, R>(p, b0x, b0y, b0z, tid, sF, sQ);
+  __syncthreads();
+  half3_elec<Tile3, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
+  __syncthreads();
+  half3_record<Tile3, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
+}
+
+// ---- face and edge solves (interleaved types as in the single-grid step) ---------------------
+template <class R>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_briemann(const BatchMember *__restrict__ mem, int nm) {
+  const long long gb = blockIdx.x;
+  const BatchMember &m = mem[find_member(mem, nm, BS_RIEMANN, gb)];
+  if (!member_active(m)) return;
+  const DevParams &p = m.p;
+  const int nt = p.dim == 3 ? 6 : 3;
+  const long long B = gb - m.begin[BS_RIEMANN];
+  const long long c = sub_brick_cell(p, m.sub, (int)(B / nt), threadIdx.x);
+  if (c < 0) return;
+  const R *rec = static_cast<const R *>(m.rec);
+  R *flux = static_cast<R *>(m.flux);
+  R *emf = static_cast<R *>(m.emf);
+  if (p.dim == 3) {
+    switch ((int)(B % 6)) {
+      case 0: face3<0>(rec, flux, p, c); break;
+      case 1: face3<1>(rec, flux, p, c); break;
+      case 2: face3<2>(rec, flux, p, c); break;
+      case 3: edge3<2>(rec, emf, p, c); break;
+      case 4: edge3<1>(rec, emf, p, c); break;
+      default: edge3<0>(rec, emf, p, c); break;
+    }
+  } else {
+    switch ((int)(B % 3)) {
+      case 0: face2<0>(rec, flux, p, c); break;
+      case 1: face2<1>(rec, flux, p, c); break;
+      default: edge2(rec, emf, p, c); break;
+    }
+  }
+}
+
+// ---- update + CT + ghost copy + CFL partials of the new state -----------------------------
+template <class R>
+__global__ void __launch_bounds__(kCflBlock) k_bupdate(const BatchMember *__restrict__ mem,
+                                                       int nm) {
+  __shared__ double red[kCflBlock / 64];
+  const long long gb = blockIdx.x;
+  const BatchMember &m = mem[find_member(mem, nm, BS_UPDATE, gb)];
+  if (!member_active(m)) return;
+  const DevParams &p = m.p;
+  const int L = (int)(gb - m.begin[BS_UPDATE]);
+  const double dt = m.ctrl[CTRL_DT];
+  const double dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = p.dim == 3 ? dt / p.dz : 0.0;
+  const int par = parity(m);
+  int i, j, k;
+  double best = 0.0;
+  if (brick_cell(m.pad, L, threadIdx.x, i, j, k))
+    best = update_cell(m.U[par], m.U[par ^ 1], static_cast<const R *>(m.flux),
+                       static_cast<const R *>(m.emf), p, cidx(p, i, j, k), dtdx, dtdy, dtdz, true);
+  best = block_max(best, red);
+  if (threadIdx.x == 0) m.parts[L] = best;
+}
+
+}  // namespace batchk
+
+// blocks per stage of one member (the host builds the first-block tables from these)
+void batch_block_counts(const DevParams &p, long long counts[BS_COUNT]) {
+  using namespace fusedk;
+  for (int d = 0; d < 3; ++d)
+    counts[BS_BND0 + d] = d < p.dim ? (bnd_elements(p, d) + batchk::kBlock - 1) / batchk::kBlock : 0;
+  counts[BS_CFL] = fused_cfl_parts(p);
+  if (p.dim == 3) {
+    const long long ntx = (p.nx + 2 + Tile3::TX - 1) / Tile3::TX;
+    const long long nty = (p.ny + 2 + Tile3::TY - 1) / Tile3::TY;
+    const long long ntz = (p.nz + 2 + Tile3::TZ - 1) / Tile3::TZ;
+    counts[BS_HALF] = ntx * nty * ntz;
+  } else {
+    const long long ntx = (p.nx + 2 + Tile2::TX - 1) / Tile2::TX;
+    const long long nty = (p.ny + 2 + Tile2::TY - 1) / Tile2::TY;
+    counts[BS_HALF] = ntx * nty;
+  }
+  counts[BS_RIEMANN] = (p.dim == 3 ? 6 : 3) * sub_bricks(p).count();
+  counts[BS_UPDATE] = fused_cfl_parts(p);
+}
+
+// one step of every member: finished test, ghost fill, (standalone CFL on request), dt, the
+// three fused stages, end of step.  Stream-ordered, no host synchronization, no memset: safe
+// to capture in a graph and replay.
+hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32,
+                      const long long totals[BS_COUNT], int with_cfl, hipStream_t s) {
+  using namespace batchk;
+  const dim3 b(kBlock), bm((nm + 63) / 64 * 64 > 1024 ? 1024 : (nm + 63) / 64 * 64);
+  const dim3 gm((nm + (int)bm.x - 1) / (int)bm.x);
+  hipLaunchKernelGGL(k_bbegin, gm, bm, 0, s, members, nm);
+  for (int d = 0; d < dim; ++d)
+    if (totals[BS_BND0 + d] > 0)
+      hipLaunchKernelGGL(k_bbnd, dim3((unsigned)totals[BS_BND0 + d]), b, 0, s, members, nm, d);
+  if (with_cfl)
+    hipLaunchKernelGGL(k_bcfl, dim3((unsigned)totals[BS_CFL]), dim3(kCflBlock), 0, s, members,
+                       nm);
+  hipLaunchKernelGGL(k_bset_dt, dim3(nm), b, 0, s, members);
+  if (bulk32) {
+    if (dim == 3)
+      hipLaunchKernelGGL(k_bhalf3d<float>, dim3((unsigned)totals[BS_HALF]), dim3(Tile3::NT), 0,
+                         s, members, nm);
+    else
+      hipLaunchKernelGGL(k_bhalf2d<float>, dim3((unsigned)totals[BS_HALF]), dim3(Tile2::NT), 0,
+                         s, members, nm);
+    hipLaunchKernelGGL(k_briemann<float>, dim3((unsigned)totals[BS_RIEMANN]), b, 0, s, members,
+                       nm);
+    hipLaunchKernelGGL(k_bupdate<float>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
+                       s, members, nm);
+  } else {
+    if (dim == 3)
+      hipLaunchKernelGGL(k_bhalf3d<double>, dim3((unsigned)totals[BS_HALF]), dim3(Tile3::NT), 0,
+                         s, members, nm);
+    else
+      hipLaunchKernelGGL(k_bhalf2d<double>, dim3((unsigned)totals[BS_HALF]), dim3(Tile2::NT), 0,
+                         s, members, nm);
+    hipLaunchKernelGGL(k_briemann<double>, dim3((unsigned)totals[BS_RIEMANN]), b, 0, s, members,
+                       nm);
+    hipLaunchKernelGGL(k_bupdate<double>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
+                       s, members, nm);
+  }
+  hipLaunchKernelGGL(k_bend, gm, bm, 0, s, members, nm);
+  return hipGetLastError();
+}
+
+}  // namespace PPK_NS
+}  // namespace ppk

@@ -24,14 +24,16 @@ namespace ppk {
 namespace {
 thread_local std::string g_thread_error;
 
-const KernelTable kFast = {fast::staged_boundaries, fast::staged_prim_dt, fast::staged_begin,
-                           fast::staged_set_dt,     fast::staged_end,     fast::staged_godunov,
-                           fast::fused_godunov,     fast::fused_cfl};
-const KernelTable kStrict = {strict::staged_boundaries, strict::staged_prim_dt,
-                             strict::staged_begin,      strict::staged_set_dt,
-                             strict::staged_end,        strict::staged_godunov,
-                             strict::fused_godunov,     strict::fused_cfl};
 }  // namespace
+const KernelTable kFast = {fast::staged_boundaries,  fast::staged_prim_dt, fast::staged_begin,
+                           fast::staged_set_dt,      fast::staged_end,     fast::staged_godunov,
+                           fast::fused_godunov,      fast::fused_cfl,      fast::batch_block_counts,
+                           fast::batch_step};
+const KernelTable kStrict = {strict::staged_boundaries,  strict::staged_prim_dt,
+                             strict::staged_begin,       strict::staged_set_dt,
+                             strict::staged_end,         strict::staged_godunov,
+                             strict::fused_godunov,      strict::fused_cfl,
+                             strict::batch_block_counts, strict::batch_step};
 
 void set_thread_error(const std::string &msg) { g_thread_error = msg; }
 const char *thread_error() { return g_thread_error.c_str(); }
@@ -330,6 +332,11 @@ void free_all(ppk_mhd_t *h) {
 }
 
 }  // namespace
+
+namespace ppk {
+int validate_params(const ppk_mhd_params *p, std::string &why) { return validate(p, why); }
+DevParams make_dev_params(const ppk_mhd_params &p) { return make_dev(p); }
+}  // namespace ppk
 
 extern "C" {
 

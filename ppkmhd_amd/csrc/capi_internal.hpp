@@ -11,6 +11,11 @@ namespace ppk {
 void set_thread_error(const std::string &msg);
 const char *thread_error();
 
+// parameter checks and device-side parameter block of a solver (capi.cpp)
+int validate_params(const ppk_mhd_params *p, std::string &why);
+struct DevParams;
+DevParams make_dev_params(const ppk_mhd_params &p);
+
 // host initial conditions in the reference layout (ics.cpp)
 void init_problem_host(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U);
 

@@ -133,6 +133,31 @@ inline Brick padded_bricks(const DevParams &p) {
   return make_brick(p.dim, p.isize, p.jsize, p.ksize, p.brick);
 }
 inline int fused_cfl_parts(const DevParams &p) { return (int)padded_bricks(p).count(); }
+// the sub-box [gw, size-gw] per direction of the face and edge kernels, in bricks
+inline Brick sub_bricks(const DevParams &p) {
+  return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1, p.brick);
+}
+
+// ---- batched ensembles: many independent grids of one dimension advanced by the same
+// launches.  Each batched launch covers the blocks of every member back to back; a member
+// owns blocks [begin[stage], begin[stage] + its count) of that stage.
+enum BatchStage {
+  BS_BND0 = 0, BS_BND1, BS_BND2,  // ghost fill, one launch per direction
+  BS_CFL,                          // standalone CFL of the current state (first step)
+  BS_HALF, BS_RIEMANN, BS_UPDATE,  // the three fused stages
+  BS_COUNT
+};
+struct BatchMember {
+  DevParams p;
+  double *U[2];        // ping-pong states; the current one is U[iteration & 1]
+  void *rec, *flux, *emf;
+  double *ctrl;        // the member's control block (CTRL_SLOTS)
+  double *parts;       // the member's CFL partial maxima
+  Brick pad, sub;      // bricks of the padded box and of the face/edge sub-box
+  long long begin[BS_COUNT];
+  int nparts;
+  int unused;
+};
 
 #define PPK_DECLARE_KERNELS(NS)                                                                  \
   namespace NS {                                                                                 \
@@ -152,6 +177,10 @@ inline int fused_cfl_parts(const DevParams &p) { return (int)padded_bricks(p).co
                            hipStream_t s, StageTimer *tm);                                       \
   hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const double *ctrl,   \
                        hipStream_t s, StageTimer *tm);                                           \
+  /* batched ensemble: counts of blocks per stage for one member, then the batched step */      \
+  void batch_block_counts(const DevParams &p, long long counts[BS_COUNT]);                      \
+  hipError_t batch_step(const BatchMember *members, int nmembers, int dim, int bulk32,           \
+                        const long long totals[BS_COUNT], int with_cfl, hipStream_t s);         \
   }
 
 PPK_DECLARE_KERNELS(fast)
@@ -167,7 +196,55 @@ struct KernelTable {
   decltype(&fast::staged_godunov) godunov;
   decltype(&fast::fused_godunov) fused_godunov;
   decltype(&fast::fused_cfl) fused_cfl;
+  decltype(&fast::batch_block_counts) batch_block_counts;
+  decltype(&fast::batch_step) batch_step;
 };
+
+// entry-point tables of the two arithmetic variants (capi.cpp)
+extern const KernelTable kFast;
+extern const KernelTable kStrict;
+
+// ghost fill of one element of direction dir (both faces), thread index t over
+// 2 * gw * s1 * s2 elements; reference src/shared/BoundariesFunctors.h:534-1057 (x, then y,
+// then z; Dirichlet flips the normal velocity and normal field)
+PPK_HD void bnd_element(double *__restrict__ U, const DevParams &p, int dir, long long t) {
+  const int gw = p.gw;
+  const int n = dir == 0 ? p.nx : (dir == 1 ? p.ny : p.nz);
+  const int s1 = dir == 0 ? p.jsize : p.isize;
+  const int s2 = dir == 2 ? p.jsize : p.ksize;
+  int g = (int)(t % gw);
+  long long r = t / gw;
+  int t1 = (int)(r % s1);
+  r /= s1;
+  int t2 = (int)(r % s2);
+  int upper = (int)(r / s2);
+  int face = 2 * dir + upper;
+  int bc = p.bc[face];
+  int c = upper ? g + n + gw : g;
+  int c0;
+  if (bc == PPK_BC_DIRICHLET)
+    c0 = upper ? 2 * n + 2 * gw - 1 - c : 2 * gw - 1 - c;
+  else if (bc == PPK_BC_NEUMANN)
+    c0 = upper ? n + gw - 1 : gw;
+  else
+    c0 = upper ? c - n : n + c;
+  int i, j, k, i0, j0, k0;
+  if (dir == 0) { i = c; j = t1; k = t2; i0 = c0; j0 = j; k0 = k; }
+  else if (dir == 1) { i = t1; j = c; k = t2; i0 = i; j0 = c0; k0 = k; }
+  else { i = t1; j = t2; k = c; i0 = i; j0 = j; k0 = c0; }
+  const long long dst = ((long long)k * p.jsize + j) * p.isize + i;
+  const long long src = ((long long)k0 * p.jsize + j0) * p.isize + i0;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    double sign = (bc == PPK_BC_DIRICHLET && (v == IU + dir || v == IA + dir)) ? -1.0 : 1.0;
+    U[dst + v * p.vstride] = U[src + v * p.vstride] * sign;
+  }
+}
+PPK_HD long long bnd_elements(const DevParams &p, int dir) {
+  const int s1 = dir == 0 ? p.jsize : p.isize;
+  const int s2 = dir == 2 ? p.jsize : p.ksize;
+  return 2LL * p.gw * s1 * s2;
+}
 
 // block-wide maximum of one double per thread (any block size that is a multiple of 64, up
 // to 1024); every thread gets the result

@@ -43,44 +43,14 @@ __device__ __forceinline__ void decompose(const DevParams &p, long long t, int &
   k = (int)(r / p.jsize);
 }
 
-// ---- ghost fill, one direction (both faces), reference src/shared/BoundariesFunctors.h ----
+// ---- ghost fill, one direction (both faces): bnd_element (kernels.hpp) per thread ---------
 __global__ void __launch_bounds__(kBlock) k_boundaries(double *__restrict__ U, DevParams p,
                                                        int dir, const double *ctrl) {
   if (!active(ctrl)) return;
-  const int gw = p.gw;
-  const int n = dir == 0 ? p.nx : (dir == 1 ? p.ny : p.nz);
-  const int s1 = dir == 0 ? p.jsize : p.isize;
-  const int s2 = dir == 2 ? p.jsize : p.ksize;
-  long long total = 2LL * gw * s1 * s2;
+  const long long total = bnd_elements(p, dir);
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    int g = (int)(t % gw);
-    long long r = t / gw;
-    int t1 = (int)(r % s1);
-    r /= s1;
-    int t2 = (int)(r % s2);
-    int upper = (int)(r / s2);
-    int face = 2 * dir + upper;
-    int bc = p.bc[face];
-    int c = upper ? g + n + gw : g;
-    int c0;
-    if (bc == PPK_BC_DIRICHLET)
-      c0 = upper ? 2 * n + 2 * gw - 1 - c : 2 * gw - 1 - c;
-    else if (bc == PPK_BC_NEUMANN)
-      c0 = upper ? n + gw - 1 : gw;
-    else
-      c0 = upper ? c - n : n + c;
-    int i, j, k, i0, j0, k0;
-    if (dir == 0) { i = c; j = t1; k = t2; i0 = c0; j0 = j; k0 = k; }
-    else if (dir == 1) { i = t1; j = c; k = t2; i0 = i; j0 = c0; k0 = k; }
-    else { i = t1; j = t2; k = c; i0 = i; j0 = j; k0 = c0; }
-    long long dst = idx(p, i, j, k), src = idx(p, i0, j0, k0);
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      double sign = (bc == PPK_BC_DIRICHLET && (v == IU + dir || v == IA + dir)) ? -1.0 : 1.0;
-      U[dst + v * p.vstride] = U[src + v * p.vstride] * sign;
-    }
-  }
+       t += (long long)gridDim.x * blockDim.x)
+    bnd_element(U, p, dir, t);
 }
 
 // ---- primitives over [0, size-1) per direction + CFL max over the interior ----------------
@@ -491,9 +461,7 @@ hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, 
                              StageTimer *tm) {
   if (tm) tm->mark(ST_BOUNDARIES, true);
   for (int dir = 0; dir < p.dim; ++dir) {
-    const int s1 = dir == 0 ? p.jsize : p.isize;
-    const int s2 = dir == 2 ? p.jsize : p.ksize;
-    long long n = 2LL * p.gw * s1 * s2;
+    const long long n = bnd_elements(p, dir);
     hipLaunchKernelGGL(k_boundaries, dim3(grid_for(n)), dim3(kBlock), 0, s, U, p, dir, ctrl);
   }
   if (tm) tm->mark(ST_BOUNDARIES, false);

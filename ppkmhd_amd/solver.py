@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi as A
-from ._lib import PPKError, check, lib
+from ._lib import PPKError, check, check_batch, lib
 from .params import default_problem
 
 
@@ -188,6 +188,78 @@ def write_vtk(params, U, directory=".", prefix="output", istep=0, ascii=False):
     check(lib().ppk_mhd_write_vtk(C.byref(params), U.ctypes.data_as(C.POINTER(C.c_double)),
                                   str(directory).encode(), prefix.encode(), int(istep),
                                   1 if ascii else 0))
+
+
+class BatchSolver:
+    """An ensemble of independent MHD simulations advanced together on one GPU (one C-ABI batch
+    handle): each member has its own grid size, parameters, time, CFL dt and finished() test,
+    and evolves exactly as a ``SolverMHDMuscl`` with the same parameters would in DT_CFL mode.
+    """
+
+    def __init__(self, params_list, problems=None):
+        self.params = list(params_list)
+        arr = (A.Params * len(self.params))(*self.params)
+        h = C.c_void_p()
+        check_batch(lib().ppk_mhd_batch_create(arr, len(self.params), C.byref(h)))
+        self._h = h
+        if problems is not None:
+            self.init_problem(problems)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ppk_mhd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.params)
+
+    def _check(self, rc):
+        return check_batch(rc, self._h)
+
+    def init_problem(self, problems):
+        if len(problems) != len(self.params):
+            raise ValueError("one problem per member")
+        arr = (A.Problem * len(problems))(*problems)
+        self._check(lib().ppk_mhd_batch_init_problem(self._h, arr))
+
+    def upload(self, member, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.size != self.params[member].count:
+            raise ValueError(f"state must have {self.params[member].count} doubles, got {U.size}")
+        self._check(lib().ppk_mhd_batch_upload(self._h, member,
+                                               U.ctypes.data_as(C.POINTER(C.c_double)), U.size))
+
+    def download(self, member):
+        U = np.empty(self.params[member].shape, dtype=np.float64)
+        self._check(lib().ppk_mhd_batch_download(self._h, member,
+                                                 U.ctypes.data_as(C.POINTER(C.c_double)), U.size))
+        return U
+
+    def run(self, nsteps):
+        """nsteps steps of every member not yet finished (synchronizes)."""
+        self._check(lib().ppk_mhd_batch_run(self._h, int(nsteps)))
+
+    def time(self, member):
+        """(t, dt of the last step, iteration) of one member."""
+        t, dt, it = C.c_double(), C.c_double(), C.c_int64()
+        self._check(lib().ppk_mhd_batch_get_time(self._h, member, C.byref(t), C.byref(dt),
+                                                 C.byref(it)))
+        return t.value, dt.value, it.value
+
+    def synchronize(self):
+        self._check(lib().ppk_mhd_batch_synchronize(self._h))
 
 
 class SolverFactory:

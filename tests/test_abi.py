@@ -180,3 +180,23 @@ implementationVersion=2
 """
     p, pr, name = params_from_ini(text=txt)
     assert p.slope_type == 2.0 and p.gamma0 == f32(1.666)
+
+
+def test_batch_create_rejects_bad_members_before_any_device_call():
+    """Parameter checks of ppk_mhd_batch_create run on the host: mixed dimensions, mixed
+    precisions, bad sizes and an empty batch are refused with a message naming the member."""
+    L = lib()
+    a = make_params(2, 16, 16)
+    b = make_params(3, 8, 8, 8)
+    c = make_params(2, 16, 16, precision=A.FP32_BULK)
+    d = make_params(2, 16, 16)
+    d.nx = 0
+    h = C.c_void_p()
+    for members, what in (([a, b], "share dim"), ([a, c], "share dim"), ([a, d], "member 1")):
+        arr = (A.Params * len(members))(*members)
+        rc = L.ppk_mhd_batch_create(arr, len(members), C.byref(h))
+        assert rc == A.PPK_ERR_INVALID_ARGUMENT
+        assert not h.value
+        assert what in L.ppk_mhd_batch_last_error(None).decode()
+    arr = (A.Params * 1)(a)
+    assert L.ppk_mhd_batch_create(arr, 0, C.byref(h)) == A.PPK_ERR_INVALID_ARGUMENT

@@ -1,0 +1,124 @@
+"""Batched ensembles on the GPU: every member must evolve exactly as a single-grid solver with
+the same parameters (same per-cell code, own CFL dt, own finished() test), bit for bit, in each
+precision; the strict members are also compared with the oracle.
+"""
+import numpy as np
+import pytest
+
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd import default_problem, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def single(p, pr):
+    from ppkmhd_amd.solver import SolverMHDMuscl
+    return SolverMHDMuscl(p, pr)
+
+
+def batch(ps, prs):
+    from ppkmhd_amd.solver import BatchSolver
+    return BatchSolver(ps, prs)
+
+
+def ot_member(n, gamma, b0, precision, tEnd=10.0, n_step_max=1000, bc=A.BC_PERIODIC):
+    p = make_params(2, n, n, gamma0=gamma, cfl=0.8, tEnd=tEnd, n_step_max=n_step_max,
+                    precision=precision, bc=bc)
+    pr = default_problem(A.PROBLEM_ORSZAG_TANG, p)
+    pr.ot_b0_scale = b0
+    return p, pr
+
+
+def blast_member(nx, ny, nz, precision, radius=0.2):
+    p = make_params(3, nx, ny, nz, gamma0=5 / 3, cfl=0.8, tEnd=10.0, precision=precision)
+    pr = default_problem(A.PROBLEM_BLAST, p)
+    pr.blast_radius = radius
+    pr.blast_density_out = 1.0
+    pr.blast_bx = pr.blast_by = pr.blast_bz = 1.0 / np.sqrt(3.0)
+    return p, pr
+
+
+def compare(members, nsteps, chunks=(1, 2, 3)):
+    """Run the batch and one single-grid solver per member; the states, times and iteration
+    counts must be identical after every chunk of steps."""
+    ps = [m[0] for m in members]
+    prs = [m[1] for m in members]
+    b = batch(ps, prs)
+    ss = [single(p, pr) for p, pr in members]
+    for m, s in enumerate(ss):
+        assert np.array_equal(b.download(m), s.download()), f"member {m}: initial state"
+    done = 0
+    for c in list(chunks) + [nsteps - sum(chunks)]:
+        b.run(c)
+        for s in ss:
+            s.run(c)
+        done += c
+        for m, s in enumerate(ss):
+            t, dt, it = b.time(m)
+            st, sdt, sit = s.time()
+            assert (t, it) == (st, sit), f"member {m} after {done} steps: {(t, it)} vs {(st, sit)}"
+            assert dt == sdt, f"member {m}: dt {dt} vs {sdt}"
+            Ub, Us = b.download(m), s.download()
+            assert np.array_equal(Ub, Us), (
+                f"member {m} after {done} steps: max |diff| {np.abs(Ub - Us).max():.3e}")
+    return b, ss
+
+
+@pytest.mark.parametrize("precision", [A.FP64_STRICT, A.FP64, A.FP32_BULK])
+def test_batch_2d_uneven_members_equal_single_grids(gpu, precision):
+    rng = np.random.default_rng(7)
+    sizes = [32, 64, 48, 96, 40, 128, 24, 72]
+    members = [ot_member(n, float(rng.uniform(1.4, 2.0)), float(rng.uniform(0.5, 2.0)), precision)
+               for n in sizes]
+    compare(members, 12)
+
+
+@pytest.mark.parametrize("precision", [A.FP64_STRICT, A.FP64])
+def test_batch_3d_uneven_members_equal_single_grids(gpu, precision):
+    members = [blast_member(24, 16, 20, precision), blast_member(32, 32, 32, precision, 0.3),
+               blast_member(17, 23, 9, precision)]
+    compare(members, 6)
+
+
+def test_batch_members_finish_independently(gpu):
+    """Different tEnd / n_step_max: finished members stop, the others go on, as single grids do."""
+    members = [ot_member(32, 5 / 3, 1.0, A.FP64, tEnd=0.01),
+               ot_member(48, 1.6, 1.2, A.FP64, n_step_max=5),
+               ot_member(40, 1.5, 0.8, A.FP64),
+               ot_member(32, 5 / 3, 1.0, A.FP64, bc=A.BC_NEUMANN)]
+    b, ss = compare(members, 30, chunks=(4, 7))
+    its = [b.time(m)[2] for m in range(len(members))]
+    assert its[1] == 5 and its[2] == 30 and its[0] < 30, its
+    assert b.time(0)[0] == pytest.approx(0.01, abs=1e-14)
+
+
+def test_batch_strict_equals_oracle(oracle, gpu):
+    members = [ot_member(32, 1.7, 1.3, A.FP64_STRICT), ot_member(40, 1.45, 0.6, A.FP64_STRICT)]
+    b = batch([m[0] for m in members], [m[1] for m in members])
+    os_ = []
+    for p, pr in members:
+        o = oracle.Solver(p)
+        o.init_problem(pr)
+        os_.append(o)
+    b.run(5)
+    for o in os_:
+        for _ in range(5):
+            o.step(0, A.DT_CFL)
+    for m, o in enumerate(os_):
+        assert np.array_equal(b.download(m), o.get_state()), f"member {m}"
+        assert b.time(m)[0] == o.time()[0]
+
+
+def test_batch_upload_download_roundtrip_and_replay(gpu):
+    """Upload a member's state mid-run: the captured step graph replays on the new state."""
+    members = [ot_member(32, 5 / 3, 1.0, A.FP64), ot_member(48, 1.6, 1.1, A.FP64)]
+    b = batch([m[0] for m in members], [m[1] for m in members])
+    b.run(3)
+    U1 = b.download(1)
+    s = single(*members[1])
+    s.upload(U1)
+    s.set_time(*b.time(1)[::2])
+    b.upload(1, U1)
+    b.run(4)
+    s.run(4)
+    assert np.array_equal(b.download(1), s.download())
