@@ -136,6 +136,73 @@ def cpu_baseline(n, steps=1):
                       f"({dt:.1f} s)"}
 
 
+def bench_ensemble(a):
+    """Batched ensemble: 64 2D Orszag-Tang runs of uneven sizes advanced together (one step
+    graph for all); the same members stepped one solver after another are timed beside it."""
+    from ppkmhd_amd.ensemble import ot_ensemble
+    from ppkmhd_amd.solver import BatchSolver, SolverMHDMuscl
+
+    prec = {"fp64": A.FP64, "fp64_strict": A.FP64_STRICT, "fp32_bulk": A.FP32_BULK}[a.precision]
+    ps, prs, draws = ot_ensemble(a.members, a.seed, a.warmup + a.steps, prec)
+    cells = sum(p.nx * p.ny for p in ps)
+    log(f"[bench] ensemble: {len(ps)} members, {cells} cells, sides "
+        f"{sorted(set(d[0] for d in draws))}")
+    b = BatchSolver(ps, prs)
+    tile, brick, tuned_ms = b.autotune(a.tune_steps)
+    log(f"[bench] autotune: tile {tile}, brick {brick}, {tuned_ms:.3f} ms/step")
+    b.run(a.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(a.steps)
+    b.synchronize()
+    elapsed = time.perf_counter() - t0
+    its = [b.time(m)[2] for m in range(len(ps))]
+    assert min(its) == a.warmup + a.steps, "a member finished early"
+    value = cells * a.steps / elapsed
+    log(f"[bench] {a.steps} steps in {elapsed:.3f} s: {elapsed / a.steps * 1e3:.3f} ms/step, "
+        f"{value / 1e9:.3f} G cell-updates/s")
+    b.close()
+
+    # the same members, one single-grid solver after another (a short sample of each)
+    seq_s = 0.0
+    nseq = a.seq_steps
+    for p, pr in zip(ps, prs):
+        s = SolverMHDMuscl(p, pr)
+        s.run(2)
+        s.synchronize()
+        t = time.perf_counter()
+        s.run(nseq)
+        s.synchronize()
+        seq_s += (time.perf_counter() - t) / nseq
+        s.close()
+    seq_value = cells / seq_s
+    log(f"[bench] sequential single grids: {seq_s * 1e3:.3f} ms per ensemble step, "
+        f"{seq_value / 1e9:.3f} G cell-updates/s")
+    out = {
+        "metric": "MHD cell-updates/s (batched ensemble, 2D Orszag-Tang, uneven sizes)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec != A.FP32_BULK else "f32 bulk (f64 state, update, CT, dt)",
+        "data": "synthetic (analytic Orszag-Tang initial conditions, seeded parameters)",
+        "config": {"workload": f"{len(ps)} x 2D Orszag-Tang, sides from {{64..512}}, seed "
+                               f"{a.seed}, gamma U[1.4,2], B0 x U[0.5,2], HLLD+CT, CFL 0.8",
+                   "members": len(ps), "cells_per_step": cells, "precision": a.precision,
+                   "tile": tile, "brick": brick},
+        "sequential_single_grid_value": seq_value,
+        "batch_over_sequential": value / seq_value,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,11 +215,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the state (profiler calibration)")
+    ap.add_argument("--workload", choices=("blast3d", "ensemble2d"), default="blast3d",
+                    help="blast3d: the 3D blast step (default); ensemble2d: batched ensemble")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tune-steps", type=int, default=3)
+    ap.add_argument("--seq-steps", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the MHD step is measured on a single MI355X "
               "(multi-GPU halo exchange is not part of this build); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "ensemble2d":
+        return bench_ensemble(a)
 
     from ppkmhd_amd.solver import SolverMHDMuscl
 

@@ -267,6 +267,18 @@ int ppk_mhd_batch_run(ppk_mhd_batch_t *h, int nsteps);
 int ppk_mhd_batch_get_time(ppk_mhd_batch_t *h, int member, double *t, double *dt,
                            int64_t *iteration);
 int ppk_mhd_batch_synchronize(ppk_mhd_batch_t *h);
+/* Block organisation of the batched kernels: tile = half-step tile shape (2D: 0 = 32x8,
+ * 1 = 64x4, 2 = 16x16; 3D: 0 = 16x4x4, 1 = 8x8x4, 2 = 32x4x2), brick = cell order of the
+ * update and face/edge kernels (0 = flattened rows in 4x4 patches; 2D: 1 = 64x4 bricks; 3D:
+ * 1 = 64x2x2, 2 = 16x4x4, 3 = 32x4x2).  Results do not depend on the choice.  No counterpart
+ * in the reference (Kokkos picks its own team shapes). */
+int ppk_mhd_batch_set_tiles(ppk_mhd_batch_t *h, int tile, int brick);
+int ppk_mhd_batch_get_tiles(const ppk_mhd_batch_t *h, int *tile, int *brick);
+/* Autotuner: times trial_steps replays of the step for every tile x brick candidate on the
+ * batch's own members, restores every member's state and time afterwards, and keeps the
+ * fastest (returned with its ms per step).  No counterpart in the reference. */
+int ppk_mhd_batch_autotune(ppk_mhd_batch_t *h, int trial_steps, int *tile, int *brick,
+                           double *ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,10 @@ SIGNATURES = {
     "ppk_mhd_batch_run": (C.c_int, [_H, C.c_int]),
     "ppk_mhd_batch_get_time": (C.c_int, [_H, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
     "ppk_mhd_batch_synchronize": (C.c_int, [_H]),
+    "ppk_mhd_batch_set_tiles": (C.c_int, [_H, C.c_int, C.c_int]),
+    "ppk_mhd_batch_get_tiles": (C.c_int, [_H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ppk_mhd_batch_autotune": (C.c_int, [_H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         _DP]),
 }
 
 _lib = None

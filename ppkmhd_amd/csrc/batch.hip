@@ -99,58 +99,56 @@ __global__ void __launch_bounds__(kCflBlock) k_bcfl(const BatchMember *__restric
 }
 
 // ---- half step -----------------------------------------------------------------------------
-template <class R>
-__global__ void __launch_bounds__(Tile2::NT) k_bhalf2d(const BatchMember *__restrict__ mem,
-                                                       int nm) {
+template <class T, class R>
+__global__ void __launch_bounds__(T::NT) k_bhalf2d(const BatchMember *__restrict__ mem, int nm) {
   const long long gb = blockIdx.x;
   const BatchMember &m = mem[find_member(mem, nm, BS_HALF, gb)];
   if (!member_active(m)) return;
-  __shared__ R lds[Tile2::LDS];
-  R *sF = lds, *sQ = lds + 2 * Tile2::NF, *sE = sQ + 8 * Tile2::NQ;
+  __shared__ R lds[T::LDS];
+  R *sF = lds, *sQ = lds + 2 * T::NF, *sE = sQ + 8 * T::NQ;
   const DevParams &p = m.p;
   const int L = (int)(gb - m.begin[BS_HALF]);
-  const int ntx = (p.nx + 2 + Tile2::TX - 1) / Tile2::TX;
-  const int b0x = p.gw - 1 + (L % ntx) * Tile2::TX;
-  const int b0y = p.gw - 1 + (L / ntx) * Tile2::TY;
+  const int ntx = (p.nx + 2 + T::TX - 1) / T::TX;
+  const int b0x = p.gw - 1 + (L % ntx) * T::TX;
+  const int b0y = p.gw - 1 + (L / ntx) * T::TY;
   const double dt = m.ctrl[CTRL_DT];
   const double *U = m.U[parity(m)];
   R *rec = static_cast<R *>(m.rec);
   const int tid = threadIdx.x;
-  half2_load<Tile2, R>(U, p, b0x, b0y, tid, sF, sQ);
+  half2_load<T, R>(U, p, b0x, b0y, tid, sF, sQ);
   __syncthreads();
-  half2_prims<Tile2, R>(p, b0x, b0y, tid, sF, sQ);
+  half2_prims<T, R>(p, b0x, b0y, tid, sF, sQ);
   __syncthreads();
-  half2_elec<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE);
+  half2_elec<T, R>(p, b0x, b0y, tid, sF, sQ, sE);
   __syncthreads();
-  half2_record<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
+  half2_record<T, R>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
 }
 
-template <class R>
-__global__ void __launch_bounds__(Tile3::NT) k_bhalf3d(const BatchMember *__restrict__ mem,
-                                                       int nm) {
+template <class T, class R>
+__global__ void __launch_bounds__(T::NT) k_bhalf3d(const BatchMember *__restrict__ mem, int nm) {
   const long long gb = blockIdx.x;
   const BatchMember &m = mem[find_member(mem, nm, BS_HALF, gb)];
   if (!member_active(m)) return;
-  __shared__ R lds[Tile3::LDS];
-  R *sF = lds, *sQ = lds + 3 * Tile3::NF, *sE = sQ + 8 * Tile3::NQ;
+  __shared__ R lds[T::LDS];
+  R *sF = lds, *sQ = lds + 3 * T::NF, *sE = sQ + 8 * T::NQ;
   const DevParams &p = m.p;
   const int L = (int)(gb - m.begin[BS_HALF]);
-  const int ntx = (p.nx + 2 + Tile3::TX - 1) / Tile3::TX;
-  const int nty = (p.ny + 2 + Tile3::TY - 1) / Tile3::TY;
-  const int b0x = p.gw - 1 + (L % ntx) * Tile3::TX;
-  const int b0y = p.gw - 1 + ((L / ntx) % nty) * Tile3::TY;
-  const int b0z = p.gw - 1 + (L / (ntx * nty)) * Tile3::TZ;
+  const int ntx = (p.nx + 2 + T::TX - 1) / T::TX;
+  const int nty = (p.ny + 2 + T::TY - 1) / T::TY;
+  const int b0x = p.gw - 1 + (L % ntx) * T::TX;
+  const int b0y = p.gw - 1 + ((L / ntx) % nty) * T::TY;
+  const int b0z = p.gw - 1 + (L / (ntx * nty)) * T::TZ;
   const double dt = m.ctrl[CTRL_DT];
   const double *U = m.U[parity(m)];
   R *rec = static_cast<R *>(m.rec);
   const int tid = threadIdx.x;
-  half3_load<Tile3, R>(U, p, b0x, b0y, b0z, tid, sF, sQ);
+  half3_load<T, R>(U, p, b0x, b0y, b0z, tid, sF, sQ);
   __syncthreads();
-  half3_prims<Tile3, R>(p, b0x, b0y, b0z, tid, sF, sQ);
+  half3_prims<T, R>(p, b0x, b0y, b0z, tid, sF, sQ);
   __syncthreads();
-  half3_elec<Tile3, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
+  half3_elec<T, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE);
   __syncthreads();
-  half3_record<Tile3, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
+  half3_record<T, R>(p, b0x, b0y, b0z, tid, sF, sQ, sE, dt, rec);
 }
 
 // ---- face and edge solves (interleaved types as in the single-grid step) ---------------------
@@ -210,22 +208,30 @@ __global__ void __launch_bounds__(kCflBlock) k_bupdate(const BatchMember *__rest
 
 }  // namespace batchk
 
+// half-step tiles of one member
+template <int TX_, int TY_>
+long long half_tiles2(const DevParams &p) {
+  return (long long)((p.nx + 2 + TX_ - 1) / TX_) * ((p.ny + 2 + TY_ - 1) / TY_);
+}
+template <class T>
+long long half_tiles3(const DevParams &p) {
+  return (long long)((p.nx + 2 + T::TX - 1) / T::TX) * ((p.ny + 2 + T::TY - 1) / T::TY) *
+         ((p.nz + 2 + T::TZ - 1) / T::TZ);
+}
+
 // blocks per stage of one member (the host builds the first-block tables from these)
-void batch_block_counts(const DevParams &p, long long counts[BS_COUNT]) {
+void batch_block_counts(const DevParams &p, int tile, long long counts[BS_COUNT]) {
   using namespace fusedk;
   for (int d = 0; d < 3; ++d)
     counts[BS_BND0 + d] = d < p.dim ? (bnd_elements(p, d) + batchk::kBlock - 1) / batchk::kBlock : 0;
   counts[BS_CFL] = fused_cfl_parts(p);
-  if (p.dim == 3) {
-    const long long ntx = (p.nx + 2 + Tile3::TX - 1) / Tile3::TX;
-    const long long nty = (p.ny + 2 + Tile3::TY - 1) / Tile3::TY;
-    const long long ntz = (p.nz + 2 + Tile3::TZ - 1) / Tile3::TZ;
-    counts[BS_HALF] = ntx * nty * ntz;
-  } else {
-    const long long ntx = (p.nx + 2 + Tile2::TX - 1) / Tile2::TX;
-    const long long nty = (p.ny + 2 + Tile2::TY - 1) / Tile2::TY;
-    counts[BS_HALF] = ntx * nty;
-  }
+  if (p.dim == 3)
+    counts[BS_HALF] = tile == 1 ? half_tiles3<Tile3d>(p)
+                                : (tile == 2 ? half_tiles3<Tile3e>(p) : half_tiles3<Tile3>(p));
+  else
+    counts[BS_HALF] = tile == 1 ? half_tiles2<Tile2b::TX, Tile2b::TY>(p)
+                                : (tile == 2 ? half_tiles2<Tile2c::TX, Tile2c::TY>(p)
+                                             : half_tiles2<Tile2::TX, Tile2::TY>(p));
   counts[BS_RIEMANN] = (p.dim == 3 ? 6 : 3) * sub_bricks(p).count();
   counts[BS_UPDATE] = fused_cfl_parts(p);
 }
@@ -233,7 +239,29 @@ void batch_block_counts(const DevParams &p, long long counts[BS_COUNT]) {
 // one step of every member: finished test, ghost fill, (standalone CFL on request), dt, the
 // three fused stages, end of step.  Stream-ordered, no host synchronization, no memset: safe
 // to capture in a graph and replay.
-hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32,
+template <class R>
+void launch_bhalf(const BatchMember *members, int nm, int dim, int tile, long long blocks,
+                  hipStream_t s) {
+  using namespace batchk;
+  const dim3 g((unsigned)blocks);
+  if (dim == 3) {
+    if (tile == 1)
+      hipLaunchKernelGGL((k_bhalf3d<Tile3d, R>), g, dim3(Tile3d::NT), 0, s, members, nm);
+    else if (tile == 2)
+      hipLaunchKernelGGL((k_bhalf3d<Tile3e, R>), g, dim3(Tile3e::NT), 0, s, members, nm);
+    else
+      hipLaunchKernelGGL((k_bhalf3d<Tile3, R>), g, dim3(Tile3::NT), 0, s, members, nm);
+  } else {
+    if (tile == 1)
+      hipLaunchKernelGGL((k_bhalf2d<Tile2b, R>), g, dim3(Tile2b::NT), 0, s, members, nm);
+    else if (tile == 2)
+      hipLaunchKernelGGL((k_bhalf2d<Tile2c, R>), g, dim3(Tile2c::NT), 0, s, members, nm);
+    else
+      hipLaunchKernelGGL((k_bhalf2d<Tile2, R>), g, dim3(Tile2::NT), 0, s, members, nm);
+  }
+}
+
+hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32, int tile,
                       const long long totals[BS_COUNT], int with_cfl, hipStream_t s) {
   using namespace batchk;
   const dim3 b(kBlock), bm((nm + 63) / 64 * 64 > 1024 ? 1024 : (nm + 63) / 64 * 64);
@@ -247,23 +275,13 @@ hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32,
                        nm);
   hipLaunchKernelGGL(k_bset_dt, dim3(nm), b, 0, s, members);
   if (bulk32) {
-    if (dim == 3)
-      hipLaunchKernelGGL(k_bhalf3d<float>, dim3((unsigned)totals[BS_HALF]), dim3(Tile3::NT), 0,
-                         s, members, nm);
-    else
-      hipLaunchKernelGGL(k_bhalf2d<float>, dim3((unsigned)totals[BS_HALF]), dim3(Tile2::NT), 0,
-                         s, members, nm);
+    launch_bhalf<float>(members, nm, dim, tile, totals[BS_HALF], s);
     hipLaunchKernelGGL(k_briemann<float>, dim3((unsigned)totals[BS_RIEMANN]), b, 0, s, members,
                        nm);
     hipLaunchKernelGGL(k_bupdate<float>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
                        s, members, nm);
   } else {
-    if (dim == 3)
-      hipLaunchKernelGGL(k_bhalf3d<double>, dim3((unsigned)totals[BS_HALF]), dim3(Tile3::NT), 0,
-                         s, members, nm);
-    else
-      hipLaunchKernelGGL(k_bhalf2d<double>, dim3((unsigned)totals[BS_HALF]), dim3(Tile2::NT), 0,
-                         s, members, nm);
+    launch_bhalf<double>(members, nm, dim, tile, totals[BS_HALF], s);
     hipLaunchKernelGGL(k_briemann<double>, dim3((unsigned)totals[BS_RIEMANN]), b, 0, s, members,
                        nm);
     hipLaunchKernelGGL(k_bupdate<double>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,

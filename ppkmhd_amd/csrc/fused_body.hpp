@@ -315,7 +315,9 @@ struct HalfTile2 {
   PPK_HD static int q(int x, int y) { return x + QX * y; }
   PPK_HD static int e(int x, int y) { return x + EX * y; }
 };
-using Tile2 = HalfTile2<32, 8>;
+using Tile2 = HalfTile2<32, 8>;   // 256 threads, 30 KB LDS
+using Tile2b = HalfTile2<64, 4>;  // 256 threads, 35 KB
+using Tile2c = HalfTile2<16, 16>; // 256 threads, 29 KB
 
 // all global loads of the tile: face fields A, B into sF, the six cell-centred variables
 // (ID..IW and the 2D IC) of the primitive cells into their slots of sQ; loads first, then

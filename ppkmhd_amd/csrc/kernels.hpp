@@ -138,6 +138,10 @@ inline Brick sub_bricks(const DevParams &p) {
   return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1, p.brick);
 }
 
+// half-step tile shapes of a batch (batch_block_counts / batch_step `tile`): 2D 32x8, 64x4,
+// 16x16; 3D 16x4x4, 8x8x4, 32x4x2
+constexpr int kBatchTiles = 3;
+
 // ---- batched ensembles: many independent grids of one dimension advanced by the same
 // launches.  Each batched launch covers the blocks of every member back to back; a member
 // owns blocks [begin[stage], begin[stage] + its count) of that stage.
@@ -178,8 +182,8 @@ struct BatchMember {
   hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const double *ctrl,   \
                        hipStream_t s, StageTimer *tm);                                           \
   /* batched ensemble: counts of blocks per stage for one member, then the batched step */      \
-  void batch_block_counts(const DevParams &p, long long counts[BS_COUNT]);                      \
-  hipError_t batch_step(const BatchMember *members, int nmembers, int dim, int bulk32,           \
+  void batch_block_counts(const DevParams &p, int tile, long long counts[BS_COUNT]);            \
+  hipError_t batch_step(const BatchMember *members, int nmembers, int dim, int bulk32, int tile, \
                         const long long totals[BS_COUNT], int with_cfl, hipStream_t s);         \
   }
 

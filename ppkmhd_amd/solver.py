@@ -261,6 +261,23 @@ class BatchSolver:
     def synchronize(self):
         self._check(lib().ppk_mhd_batch_synchronize(self._h))
 
+    def set_tiles(self, tile, brick):
+        """Half-step tile shape and cell order of the batched kernels (results are unchanged)."""
+        self._check(lib().ppk_mhd_batch_set_tiles(self._h, int(tile), int(brick)))
+
+    def tiles(self):
+        t, b = C.c_int(), C.c_int()
+        self._check(lib().ppk_mhd_batch_get_tiles(self._h, C.byref(t), C.byref(b)))
+        return t.value, b.value
+
+    def autotune(self, trial_steps=5):
+        """Time every tile x brick candidate on the members themselves (their states and times
+        are restored), keep the fastest; returns (tile, brick, ms per step)."""
+        t, b, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(lib().ppk_mhd_batch_autotune(self._h, int(trial_steps), C.byref(t), C.byref(b),
+                                                 C.byref(ms)))
+        return t.value, b.value, ms.value
+
 
 class SolverFactory:
     """Registry of solver constructors keyed by the reference's solver names

@@ -122,3 +122,46 @@ def test_batch_upload_download_roundtrip_and_replay(gpu):
     b.run(4)
     s.run(4)
     assert np.array_equal(b.download(1), s.download())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batch_tile_shapes_give_identical_results(gpu, dim):
+    """Every tile x brick organisation of the batched kernels gives the same states."""
+    if dim == 2:
+        members = [ot_member(40, 1.6, 1.2, A.FP64), ot_member(72, 1.9, 0.7, A.FP64)]
+        bricks = 2
+    else:
+        members = [blast_member(20, 12, 18, A.FP64), blast_member(16, 16, 16, A.FP64, 0.3)]
+        bricks = 4
+    ref = None
+    for tile in range(3):
+        for brick in range(bricks):
+            b = batch([m[0] for m in members], [m[1] for m in members])
+            b.set_tiles(tile, brick)
+            assert b.tiles() == (tile, brick)
+            b.run(4)
+            out = [b.download(m) for m in range(len(members))]
+            if ref is None:
+                ref = out
+            for m in range(len(members)):
+                assert np.array_equal(out[m], ref[m]), (tile, brick, m)
+
+
+def test_batch_autotune_restores_state(gpu):
+    members = [ot_member(n, 1.4 + 0.1 * i, 1.0, A.FP64) for i, n in enumerate((32, 64, 48))]
+    b = batch([m[0] for m in members], [m[1] for m in members])
+    b.run(2)
+    before = [b.download(m) for m in range(3)]
+    times = [b.time(m) for m in range(3)]
+    tile, brick, ms = b.autotune(2)
+    assert b.tiles() == (tile, brick) and ms > 0
+    for m in range(3):
+        assert np.array_equal(b.download(m), before[m])
+        assert b.time(m) == times[m]
+    # and the run continues exactly as an untuned batch
+    c = batch([m[0] for m in members], [m[1] for m in members])
+    c.run(2)
+    b.run(3)
+    c.run(3)
+    for m in range(3):
+        assert np.array_equal(b.download(m), c.download(m))
