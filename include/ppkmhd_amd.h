@@ -280,6 +280,61 @@ int ppk_mhd_batch_get_tiles(const ppk_mhd_batch_t *h, int *tile, int *brick);
 int ppk_mhd_batch_autotune(ppk_mhd_batch_t *h, int trial_steps, int *tile, int *brick,
                            double *ms_per_step);
 
+/* ---- surrogate form of the face solves (learned HLLD flux) -----------------------------------
+ * No counterpart in the reference: ppkMHD solves every face with HLLD
+ * (src/shared/RiemannSolvers_MHD.h:133-367).  A network 16 -> 128 -> 128 -> 8 (ReLU, bf16
+ * weights and activations, FP32 accumulation on the matrix cores) learns the difference
+ * between the HLLD and LLF fluxes of a face in dimensionless form; a face whose density and
+ * pressure jumps are below the gate thresholds takes LLF + the learned correction if that flux
+ * keeps the neighbouring cell updates positive, and exact HLLD otherwise.  Training samples
+ * are dumped from the solver's own faces; training runs on the GPU with the library's own
+ * forward, backward and Adam kernels.  All calls below are synchronous. */
+typedef struct ppk_mlp ppk_mlp;
+typedef ppk_mlp ppk_mlp_t;
+typedef struct ppk_faceset ppk_faceset;
+typedef ppk_faceset ppk_faceset_t;
+/* FP32 parameter count: W1[128][16] b1[128] W2[128][128] b2[128] W3[8][128] b3[8], then the
+ * standardisation mu_in[16] isig_in[16] mu_out[8] sig_out[8] (19768). */
+int ppk_mlp_param_count(void);
+/* He-uniform initial weights from `seed`, zero biases, identity standardisation. */
+int ppk_mlp_create(uint64_t seed, ppk_mlp_t **net);
+void ppk_mlp_destroy(ppk_mlp_t *net);
+const char *ppk_mlp_last_error(const ppk_mlp_t *net);
+int ppk_mlp_get_params(ppk_mlp_t *net, float *params, size_t count);
+int ppk_mlp_set_params(ppk_mlp_t *net, const float *params, size_t count);
+/* Training samples on the device: 16 features and 8 targets per face. */
+int ppk_faceset_create(size_t capacity, ppk_faceset_t **set);
+void ppk_faceset_destroy(ppk_faceset_t *set);
+const char *ppk_faceset_last_error(const ppk_faceset_t *set);
+int ppk_faceset_size(const ppk_faceset_t *set, size_t *n);
+int ppk_faceset_clear(ppk_faceset_t *set);
+int ppk_faceset_append(ppk_faceset_t *set, const float *X, const float *T, size_t n);
+int ppk_faceset_read(ppk_faceset_t *set, size_t first, size_t n, float *X, float *T);
+/* Append the samples of every stride-th face (from `phase`) that the solver's next CFL step
+ * will solve: ghost fill, CFL dt, half step, then per face the features and the target
+ * (F_HLLD - F_LLF) / scale.  Needs the fused path. */
+int ppk_mhd_dump_faces(ppk_mhd_t *h, ppk_faceset_t *set, int stride, int phase, size_t *appended);
+/* Feature and target means and standard deviations of a face set into the network. */
+int ppk_mlp_standardize(ppk_mlp_t *net, const ppk_faceset_t *set);
+/* `steps` Adam steps (lr, betas 0.9 / 0.999) on minibatches of `batch` samples (a multiple of
+ * 256) drawn uniformly from the set with `seed`; the loss is the mean squared error of the
+ * standardised outputs. */
+int ppk_mlp_train(ppk_mlp_t *net, const ppk_faceset_t *set, int steps, int batch, float lr,
+                  uint64_t seed);
+/* Gradient of that loss on n given samples (n a multiple of 256) w.r.t. the 19720 trained
+ * parameters, without updating them. */
+int ppk_mlp_gradient(ppk_mlp_t *net, const float *X, const float *T, size_t n, float *grad,
+                     size_t count);
+/* Standardised outputs Y[n][8] of the network for features X[n][16]. */
+int ppk_mlp_infer(ppk_mlp_t *net, const float *X, size_t n, float *Y);
+/* Mean squared error per standardised output over a face set (8 doubles). */
+int ppk_mlp_evaluate(ppk_mlp_t *net, const ppk_faceset_t *set, double *mse);
+/* Use the network for the face solves of this solver (net = NULL: plain HLLD again).  The
+ * network must outlive its use; gate thresholds are relative density / pressure jumps. */
+int ppk_mhd_set_surrogate(ppk_mhd_t *h, const ppk_mlp_t *net, double gate_rho, double gate_p);
+/* Faces solved by the network and by HLLD since the last reset. */
+int ppk_mhd_surrogate_counts(ppk_mhd_t *h, int64_t *by_network, int64_t *by_hlld, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ _P = C.POINTER(A.Params)
 _PR = C.POINTER(A.Problem)
 _H = C.c_void_p
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "ppk_mhd_params_size": (C.c_int, []),
     "ppk_mhd_problem_size": (C.c_int, []),
@@ -64,6 +65,28 @@ SIGNATURES = {
     "ppk_mhd_batch_get_tiles": (C.c_int, [_H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ppk_mhd_batch_autotune": (C.c_int, [_H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          _DP]),
+    "ppk_mlp_param_count": (C.c_int, []),
+    "ppk_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(_H)]),
+    "ppk_mlp_destroy": (None, [_H]),
+    "ppk_mlp_last_error": (C.c_char_p, [_H]),
+    "ppk_mlp_get_params": (C.c_int, [_H, _FP, C.c_size_t]),
+    "ppk_mlp_set_params": (C.c_int, [_H, _FP, C.c_size_t]),
+    "ppk_faceset_create": (C.c_int, [C.c_size_t, C.POINTER(_H)]),
+    "ppk_faceset_destroy": (None, [_H]),
+    "ppk_faceset_last_error": (C.c_char_p, [_H]),
+    "ppk_faceset_size": (C.c_int, [_H, C.POINTER(C.c_size_t)]),
+    "ppk_faceset_clear": (C.c_int, [_H]),
+    "ppk_faceset_append": (C.c_int, [_H, _FP, _FP, C.c_size_t]),
+    "ppk_faceset_read": (C.c_int, [_H, C.c_size_t, C.c_size_t, _FP, _FP]),
+    "ppk_mhd_dump_faces": (C.c_int, [_H, _H, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "ppk_mlp_standardize": (C.c_int, [_H, _H]),
+    "ppk_mlp_train": (C.c_int, [_H, _H, C.c_int, C.c_int, C.c_float, C.c_uint64]),
+    "ppk_mlp_gradient": (C.c_int, [_H, _FP, _FP, C.c_size_t, _FP, C.c_size_t]),
+    "ppk_mlp_infer": (C.c_int, [_H, _FP, C.c_size_t, _FP]),
+    "ppk_mlp_evaluate": (C.c_int, [_H, _H, _DP]),
+    "ppk_mhd_set_surrogate": (C.c_int, [_H, _H, C.c_double, C.c_double]),
+    "ppk_mhd_surrogate_counts": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                           C.c_int]),
 }
 
 _lib = None
@@ -97,6 +120,13 @@ def lib():
 def check(rc, handle=None):
     if rc != A.PPK_OK:
         msg = lib().ppk_mhd_last_error(handle)
+        raise PPKError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def check_with(rc, handle, last_error):
+    if rc != A.PPK_OK:
+        msg = last_error(handle)
         raise PPKError(rc, msg.decode() if msg else "")
     return rc
 

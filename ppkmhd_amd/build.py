@@ -45,12 +45,17 @@ UNITS = [
     ("fused.hip", "fused_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("batch.hip", "batch_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("batch.hip", "batch_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
+    ("surrogate.hip", "surrogate_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
+    ("surrogate.hip", "surrogate_strict.o",
+     ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
+    ("mlp.hip", "mlp.o", []),
     ("diag.hip", "diag.o", []),
     ("capi.cpp", "capi.o", ["--offload-host-only"]),
     ("config.cpp", "config.o", []),
     ("ics.cpp", "ics.o", ["-ffp-contract=off"]),
     ("io_vtk.cpp", "io_vtk.o", []),
     ("batch_capi.cpp", "batch_capi.o", ["--offload-host-only"]),
+    ("mlp_capi.cpp", "mlp_capi.o", ["--offload-host-only"]),
 ]
 
 

@@ -152,8 +152,8 @@ __global__ void __launch_bounds__(T::NT) k_bhalf3d(const BatchMember *__restrict
 }
 
 // ---- face and edge solves (interleaved types as in the single-grid step) ---------------------
-template <class R>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+template <int W, class R>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 k_briemann(const BatchMember *__restrict__ mem, int nm) {
   const long long gb = blockIdx.x;
   const BatchMember &m = mem[find_member(mem, nm, BS_RIEMANN, gb)];
@@ -276,14 +276,19 @@ hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32, i
   hipLaunchKernelGGL(k_bset_dt, dim3(nm), b, 0, s, members);
   if (bulk32) {
     launch_bhalf<float>(members, nm, dim, tile, totals[BS_HALF], s);
-    hipLaunchKernelGGL(k_briemann<float>, dim3((unsigned)totals[BS_RIEMANN]), b, 0, s, members,
-                       nm);
+    hipLaunchKernelGGL((k_briemann<4, float>), dim3((unsigned)totals[BS_RIEMANN]), b, 0, s,
+                       members, nm);
     hipLaunchKernelGGL(k_bupdate<float>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
                        s, members, nm);
   } else {
     launch_bhalf<double>(members, nm, dim, tile, totals[BS_HALF], s);
-    hipLaunchKernelGGL(k_briemann<double>, dim3((unsigned)totals[BS_RIEMANN]), b, 0, s, members,
-                       nm);
+    // 3 waves per SIMD keep the 3D double solves free of spills (the single-grid default)
+    if (dim == 3)
+      hipLaunchKernelGGL((k_briemann<3, double>), dim3((unsigned)totals[BS_RIEMANN]), b, 0, s,
+                         members, nm);
+    else
+      hipLaunchKernelGGL((k_briemann<4, double>), dim3((unsigned)totals[BS_RIEMANN]), b, 0, s,
+                         members, nm);
     hipLaunchKernelGGL(k_bupdate<double>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
                        s, members, nm);
   }

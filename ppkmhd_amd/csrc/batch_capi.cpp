@@ -219,22 +219,43 @@ const char *ppk_mhd_batch_last_error(const ppk_mhd_batch_t *h) {
 
 int ppk_mhd_batch_init_problem(ppk_mhd_batch_t *h, const ppk_mhd_problem *problems) {
   if (!h || !problems) return bfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  for (int m = 0; m < h->nm; ++m)
+    if (problems[m].kind < PPK_PROBLEM_ORSZAG_TANG || problems[m].kind > PPK_PROBLEM_IMPLODE)
+      return bfail(h, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind (member " +
+                                                    std::to_string(m) + ")");
   int rc = read_ctrl(h);
   if (rc != PPK_OK) return rc;
+  for (int m = 0; m < h->nm; ++m) {
+    double *c = h->ctrl_host + (size_t)m * ppk::CTRL_SLOTS;
+    c[ppk::CTRL_T] = 0.0;
+    c[ppk::CTRL_ITER] = 0.0;
+    c[ppk::CTRL_ACTIVE] = 1.0;
+  }
+  rc = write_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  // as a single solver's init: the state, its ghost fill, U2 = U, and the CFL dt of t = 0
   for (int m = 0; m < h->nm; ++m) {
     const ppk::BatchMember &bm = h->members[m];
     std::vector<double> host((size_t)bm.p.vstride * 8);
     ppk::init_problem_host(h->params[m], problems[m], host.data());
     BHIP(h, hipMemcpyAsync(bm.U[0], host.data(), host.size() * sizeof(double),
                            hipMemcpyHostToDevice, h->stream));
+    BHIP(h, h->K->boundaries(bm.p, bm.U[0], bm.ctrl, h->stream, nullptr));
+    BHIP(h, h->K->fused_cfl(bm.p, bm.U[0], bm.parts, bm.ctrl, h->stream, nullptr));
+    BHIP(h, h->K->set_dt(bm.ctrl, bm.parts, bm.nparts, 0.0, ppk::SETDT_REDUCE_ONLY,
+                         h->params[m].cfl, h->stream));
+    BHIP(h, hipMemcpyAsync(bm.U[1], bm.U[0], host.size() * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
     BHIP(h, hipStreamSynchronize(h->stream));
-    double *c = h->ctrl_host + (size_t)m * ppk::CTRL_SLOTS;
-    c[ppk::CTRL_T] = 0.0;
-    c[ppk::CTRL_ITER] = 0.0;
-    c[ppk::CTRL_DT] = h->params[m].tEnd;
-    c[ppk::CTRL_ACTIVE] = 1.0;
   }
-  h->cfl_valid = false;
+  rc = read_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  for (int m = 0; m < h->nm; ++m) {
+    double *c = h->ctrl_host + (size_t)m * ppk::CTRL_SLOTS;
+    const double dt = h->params[m].cfl / c[ppk::CTRL_INVDT];
+    c[ppk::CTRL_DT] = (0.0 + dt > h->params[m].tEnd) ? h->params[m].tEnd - 0.0 : dt;
+  }
+  h->cfl_valid = true;
   return write_ctrl(h);
 }
 

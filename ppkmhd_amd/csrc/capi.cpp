@@ -25,15 +25,20 @@ namespace {
 thread_local std::string g_thread_error;
 
 }  // namespace
-const KernelTable kFast = {fast::staged_boundaries,  fast::staged_prim_dt, fast::staged_begin,
-                           fast::staged_set_dt,      fast::staged_end,     fast::staged_godunov,
-                           fast::fused_godunov,      fast::fused_cfl,      fast::batch_block_counts,
+const KernelTable kFast = {fast::staged_boundaries, fast::staged_prim_dt,
+                           fast::staged_begin,      fast::staged_set_dt,
+                           fast::staged_end,        fast::staged_godunov,
+                           fast::fused_godunov,     fast::fused_cfl,
+                           fast::fused_half,        fast::surrogate_dump,
+                           fast::surrogate_dump_slots, fast::batch_block_counts,
                            fast::batch_step};
-const KernelTable kStrict = {strict::staged_boundaries,  strict::staged_prim_dt,
-                             strict::staged_begin,       strict::staged_set_dt,
-                             strict::staged_end,         strict::staged_godunov,
-                             strict::fused_godunov,      strict::fused_cfl,
-                             strict::batch_block_counts, strict::batch_step};
+const KernelTable kStrict = {strict::staged_boundaries, strict::staged_prim_dt,
+                             strict::staged_begin,      strict::staged_set_dt,
+                             strict::staged_end,        strict::staged_godunov,
+                             strict::fused_godunov,     strict::fused_cfl,
+                             strict::fused_half,        strict::surrogate_dump,
+                             strict::surrogate_dump_slots, strict::batch_block_counts,
+                             strict::batch_step};
 
 void set_thread_error(const std::string &msg) { g_thread_error = msg; }
 const char *thread_error() { return g_thread_error.c_str(); }
@@ -151,6 +156,8 @@ struct ppk_mhd_solver {
   double *parts = nullptr;      // per-block CFL partial maxima (device)
   ppk::StagedScratch w;
   ppk::FusedScratch fw;
+  ppk::SurrogateCfg sur;                   // surrogate face solves (fw.sur points here when on)
+  unsigned long long *sur_counts = nullptr;  // device: faces by the network, by HLLD
   bool fused = false;
   // parts holds the CFL maxima of cur() (ghosts as the next fill makes them): set by a fused
   // CFL-mode step or compute_dt, cleared by anything that changes cur() behind the kernels
@@ -259,9 +266,10 @@ ppk::DevParams make_dev(const ppk_mhd_params &p) {
   // tuning knob: PPKMHD_BRICK selects the block shape of the fused per-cell kernels
   const char *brick = std::getenv("PPKMHD_BRICK");
   d.brick = brick ? std::atoi(brick) : 0;
-  // tuning knob: PPKMHD_RIEMANN_MIX=0 launches the six face / edge kernels one by one
+  // tuning knob: PPKMHD_RIEMANN_MIX=0 launches the six face / edge kernels one by one, 1 and 2
+  // interleave them in one launch at 4 or 3 waves per SIMD (2: measured fastest in 3D FP64)
   const char *mix = std::getenv("PPKMHD_RIEMANN_MIX");
-  d.riemann_mix = mix ? std::atoi(mix) : 1;
+  d.riemann_mix = mix ? std::atoi(mix) : 2;
   return d;
 }
 
@@ -325,6 +333,7 @@ void free_all(ppk_mhd_t *h) {
   F(h->fw.rec);
   F(h->fw.flux);
   F(h->fw.emf);
+  F(h->sur_counts);
   delete h->timer;
   h->timer = nullptr;
   if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
@@ -524,6 +533,75 @@ int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
   int rc = read_ctrl(h);
   if (rc != PPK_OK) return rc;
   *dt = h->p.cfl / h->ctrl_host[ppk::CTRL_INVDT];
+  return PPK_OK;
+}
+
+int ppk_mhd_dump_faces(ppk_mhd_t *h, ppk_faceset_t *set, int stride, int phase,
+                       size_t *appended) {
+  if (!h || !set) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!h->fused) return fail(h, PPK_ERR_UNSUPPORTED, "face dumps need the fused kernel path");
+  if (stride < 1 || phase < 0 || phase >= stride)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "need stride >= 1 and 0 <= phase < stride");
+  int finished = 0;
+  int rc = ppk_mhd_finished(h, &finished);
+  if (rc != PPK_OK) return rc;
+  if (finished) return fail(h, PPK_ERR_INVALID_ARGUMENT, "the solver has finished");
+  const long long n = h->K->surrogate_dump_slots(h->dp, stride, phase);
+  if ((size_t)n > set->capacity - set->n)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "face set too small for this dump");
+  // the faces the next CFL step will solve: ghost fill, CFL dt, half-step records
+  HIP_CHECK(h, h->K->begin(h->ctrl, 1, h->stream));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
+  HIP_CHECK(h, h->K->fused_cfl(h->dp, h->cur(), h->parts, h->ctrl, h->stream, h->tm()));
+  HIP_CHECK(h, h->K->set_dt(h->ctrl, h->parts, ppk::fused_cfl_parts(h->dp), 0.0,
+                            ppk::SETDT_CFL, h->p.cfl, h->stream));
+  HIP_CHECK(h, h->K->fused_half(h->dp, h->cur(), h->fw.rec, h->ctrl, h->stream));
+  HIP_CHECK(h, h->K->surrogate_dump(h->dp, h->fw.rec, h->ctrl, set->X + set->n * ppk::kMlpIn,
+                                    set->T + set->n * ppk::kMlpOut, stride, phase, h->stream));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  h->cfl_valid = true;
+  set->n += (size_t)n;
+  if (appended) *appended = (size_t)n;
+  return PPK_OK;
+}
+
+int ppk_mhd_set_surrogate(ppk_mhd_t *h, const ppk_mlp_t *net, double gate_rho, double gate_p) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (!net) {
+    h->fw.sur = nullptr;
+    h->sur = ppk::SurrogateCfg{};
+    return PPK_OK;
+  }
+  if (!h->fused) return fail(h, PPK_ERR_UNSUPPORTED, "the surrogate needs the fused kernel path");
+  if (h->p.riemann != PPK_RIEMANN_HLLD)
+    return fail(h, PPK_ERR_UNSUPPORTED, "the surrogate replaces HLLD: riemann must be hlld");
+  if (!(gate_rho >= 0.0) || !(gate_p >= 0.0))
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "gate thresholds must be >= 0");
+  if (!h->sur_counts) {
+    HIP_CHECK(h, hipMalloc(&h->sur_counts, 2 * sizeof(unsigned long long)));
+    HIP_CHECK(h, hipMemsetAsync(h->sur_counts, 0, 2 * sizeof(unsigned long long), h->stream));
+  }
+  h->sur.net = net->packed;
+  h->sur.gate_rho = gate_rho;
+  h->sur.gate_p = gate_p;
+  h->sur.counters = h->sur_counts;
+  h->fw.sur = &h->sur;
+  return PPK_OK;
+}
+
+int ppk_mhd_surrogate_counts(ppk_mhd_t *h, int64_t *by_network, int64_t *by_hlld, int reset) {
+  if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  unsigned long long c[2] = {0, 0};
+  if (h->sur_counts) {
+    HIP_CHECK(h, hipMemcpyAsync(c, h->sur_counts, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    if (reset) {
+      HIP_CHECK(h, hipMemsetAsync(h->sur_counts, 0, sizeof(c), h->stream));
+      HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    }
+  }
+  if (by_network) *by_network = (int64_t)c[0];
+  if (by_hlld) *by_hlld = (int64_t)c[1];
   return PPK_OK;
 }
 

@@ -1,8 +1,13 @@
 // Internal declarations shared by the host-side translation units of the library.
 #pragma once
 
-#include <string>
+#include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mlp.hpp"
 #include "ppkmhd_amd.h"
 
 namespace ppk {
@@ -20,3 +25,23 @@ DevParams make_dev_params(const ppk_mhd_params &p);
 void init_problem_host(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U);
 
 }  // namespace ppk
+
+// the surrogate network: FP32 master parameters, Adam moments, bf16 packed copy (device)
+struct ppk_mlp {
+  float *P = nullptr, *m = nullptr, *v = nullptr;
+  ppk::MlpPacked *packed = nullptr;
+  long long adam_step = 0;
+  // training scratch, sized for the last minibatch
+  uint16_t *scratch = nullptr;
+  float *partials = nullptr;
+  long long scratch_batch = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+// training samples of the surrogate: features X[n][16], targets T[n][8] (device)
+struct ppk_faceset {
+  float *X = nullptr, *T = nullptr;
+  size_t n = 0, capacity = 0;
+  std::string err;
+};

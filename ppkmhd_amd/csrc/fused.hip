@@ -202,8 +202,8 @@ void launch_half3(const DevParams &p, const double *Uin, R *rec, const double *c
 // rec, flux and emf hold R
 template <class R>
 hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *rec, R *flux,
-                        R *emf, double *parts, const double *ctrl, int fold_cfl, hipStream_t s,
-                        StageTimer *tm) {
+                        R *emf, double *parts, const double *ctrl, int fold_cfl,
+                        const SurrogateCfg *sur, hipStream_t s, StageTimer *tm) {
   if (p.dim == 3) {
     if (tm) tm->mark(ST_FUSED_HALF, true);
     switch (p.tile3) {
@@ -217,7 +217,13 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
     const Brick bk = sub_bricks(p);
     const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
     if (tm) tm->mark(ST_FUSED_RIEMANN, true);
-    if (p.riemann_mix == 2) {  // 3 waves per SIMD, no spill
+    if (sur) {  // learned face fluxes, then the exact edges
+      const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_edge3d<2, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edge3d<1, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edge3d<0, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+    } else if (p.riemann_mix == 2) {  // 3 waves per SIMD, no spill
       hipLaunchKernelGGL((k_riemann_mix3d<3, R>), dim3((unsigned)(6 * bk.count())), b, 0, s, rec,
                          flux, emf, p, bk, ctrl);
     } else if (p.riemann_mix) {
@@ -240,7 +246,11 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
     const Brick bk = sub_bricks(p);
     const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
     if (tm) tm->mark(ST_FUSED_RIEMANN, true);
-    if (p.riemann_mix) {
+    if (sur) {
+      const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_edge2d<R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+    } else if (p.riemann_mix) {
       hipLaunchKernelGGL((k_riemann_mix2d<R>), dim3((unsigned)(3 * bk.count())), b, 0, s, rec,
                          flux, emf, p, bk, ctrl);
     } else {
@@ -268,10 +278,37 @@ hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout, Fu
   if (p.bulk32)
     return godunov_impl<float>(p, Uin, Uout, static_cast<float *>(w.rec),
                                static_cast<float *>(w.flux), static_cast<float *>(w.emf), parts,
-                               ctrl, fold_cfl, s, tm);
+                               ctrl, fold_cfl, w.sur, s, tm);
   return godunov_impl<double>(p, Uin, Uout, static_cast<double *>(w.rec),
                               static_cast<double *>(w.flux), static_cast<double *>(w.emf), parts,
-                              ctrl, fold_cfl, s, tm);
+                              ctrl, fold_cfl, w.sur, s, tm);
+}
+
+template <class R>
+hipError_t half_impl(const DevParams &p, const double *U, R *rec, const double *ctrl,
+                     hipStream_t s) {
+  using namespace fusedk;
+  if (p.dim == 3) {
+    switch (p.tile3) {
+      case 1: launch_half3<Tile3b>(p, U, rec, ctrl, s); break;
+      case 2: launch_half3<Tile3c>(p, U, rec, ctrl, s); break;
+      case 3: launch_half3<Tile3d>(p, U, rec, ctrl, s); break;
+      case 4: launch_half3<Tile3e>(p, U, rec, ctrl, s); break;
+      default: launch_half3<Tile3>(p, U, rec, ctrl, s); break;
+    }
+  } else {
+    const int ntx = ceil_div(p.nx + 2, Tile2::TX), nty = ceil_div(p.ny + 2, Tile2::TY);
+    hipLaunchKernelGGL((k_half2d<R>), dim3(ntx * nty), dim3(Tile2::NT), 0, s, U, rec, p, ctrl,
+                       ntx);
+  }
+  return hipGetLastError();
+}
+
+// the half-step records of U alone (the surrogate's training-sample dump reads them)
+hipError_t fused_half(const DevParams &p, const double *U, void *rec, const double *ctrl,
+                      hipStream_t s) {
+  if (p.bulk32) return half_impl<float>(p, U, static_cast<float *>(rec), ctrl, s);
+  return half_impl<double>(p, U, static_cast<double *>(rec), ctrl, s);
 }
 
 hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const double *ctrl,

@@ -460,8 +460,8 @@ PPK_HD void swap_dir(St<R> &s, int a, int b) {
 // then the 1D Riemann solver; the 5 (3D) or 6 (2D) flux components are stored rotated.
 // Reference src/muscl/MHDRunFunctors3D.h:1839-1902 and MHDRunFunctors2D.h.
 template <int D, class R>
-PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
-                  long long c) {
+PPK_HD void face_states3(const R *__restrict__ rec, const DevParams &p, long long c, St<R> &ql,
+                         St<R> &qr) {
   const long long vs = p.vstride;
   const long long sd = D == 0 ? 1 : (D == 1 ? (long long)p.isize : p.plane);
   Half3D<R> hl, hr;
@@ -470,7 +470,8 @@ PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevPara
   if (D == 0) hl.AR = rec[c + R3_AL * vs];
   if (D == 1) hl.BR = rec[c + R3_BL * vs];
   if (D == 2) hl.CR = rec[c + R3_CL * vs];
-  St<R> ql = face_state_3d(hl, D, +1, p), qr = face_state_3d(hr, D, -1, p), f;
+  ql = face_state_3d(hl, D, +1, p);
+  qr = face_state_3d(hr, D, -1, p);
   if (D == 1) {
     swap_dir(ql, IU, IV); swap_dir(ql, IA, IB);
     swap_dir(qr, IU, IV); swap_dir(qr, IA, IB);
@@ -478,13 +479,24 @@ PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevPara
     swap_dir(ql, IU, IW); swap_dir(ql, IA, IC);
     swap_dir(qr, IU, IW); swap_dir(qr, IA, IC);
   }
-  riemann_mhd(ql, qr, f, p);
+}
+template <int D, class R>
+PPK_HD void store_flux3(R *__restrict__ flux, const DevParams &p, long long c, const St<R> &f) {
+  const long long vs = p.vstride;
   R *o = flux + (long long)D * kFluxComp3 * vs + c;
   o[0] = f[ID];
   o[vs] = f[IP];
   o[2 * vs] = f[IU];
   o[3 * vs] = f[IV];
   o[4 * vs] = f[IW];
+}
+template <int D, class R>
+PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
+                  long long c) {
+  St<R> ql, qr, f;
+  face_states3<D>(rec, p, c, ql, qr);
+  riemann_mhd(ql, qr, f, p);
+  store_flux3<D>(flux, p, c, f);
 }
 
 // One edge of cell c (its lower edge along direction E = 0 x, 1 y, 2 z): the four edge states
@@ -536,8 +548,8 @@ PPK_HD void edge3(const R *__restrict__ rec, R *__restrict__ emf, const DevParam
 // 2D faces (D = 0, 1) and the z edge, reference src/muscl/MHDRunFunctors2D.h
 // ComputeFluxesAndStoreFunctor and ComputeEmfAndStoreFunctor (:654-680)
 template <int D, class R>
-PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
-                  long long c) {
+PPK_HD void face_states2(const R *__restrict__ rec, const DevParams &p, long long c, St<R> &ql,
+                         St<R> &qr) {
   const long long vs = p.vstride;
   const long long sd = D == 0 ? 1 : (long long)p.isize;
   Half2D<R> hl, hr;
@@ -545,12 +557,16 @@ PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevPara
   load_rec2(rec, vs, c, hr);
   if (D == 0) hl.AR = rec[c + R2_AL * vs];
   if (D == 1) hl.BR = rec[c + R2_BL * vs];
-  St<R> ql = face_state_2d(hl, D, +1, p), qr = face_state_2d(hr, D, -1, p), f;
+  ql = face_state_2d(hl, D, +1, p);
+  qr = face_state_2d(hr, D, -1, p);
   if (D == 1) {
     swap_dir(ql, IU, IV); swap_dir(ql, IA, IB);
     swap_dir(qr, IU, IV); swap_dir(qr, IA, IB);
   }
-  riemann_mhd(ql, qr, f, p);
+}
+template <int D, class R>
+PPK_HD void store_flux2(R *__restrict__ flux, const DevParams &p, long long c, const St<R> &f) {
+  const long long vs = p.vstride;
   R *o = flux + (long long)D * kFluxComp2 * vs + c;
   o[0] = f[ID];
   o[vs] = f[IP];
@@ -558,6 +574,14 @@ PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevPara
   o[3 * vs] = f[IV];
   o[4 * vs] = f[IW];
   o[5 * vs] = f[IC];
+}
+template <int D, class R>
+PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
+                  long long c) {
+  St<R> ql, qr, f;
+  face_states2<D>(rec, p, c, ql, qr);
+  riemann_mhd(ql, qr, f, p);
+  store_flux2<D>(flux, p, c, f);
 }
 
 template <class R>

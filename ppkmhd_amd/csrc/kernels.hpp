@@ -66,11 +66,18 @@ constexpr int kRec3 = 38;   // half-step record per cell, 3D
 constexpr int kRec2 = 26;   // half-step record per cell, 2D
 constexpr int kFluxComp3 = 5;  // ID IP IU IV IW per face direction, 3D
 constexpr int kFluxComp2 = 6;  // ID IP IU IV IW IC per face direction, 2D
+// the surrogate form of the face solves: network, gate, counters (device memory)
+struct SurrogateCfg {
+  const void *net = nullptr;               // packed network (MlpPacked, device)
+  double gate_rho = 0.0, gate_p = 0.0;     // max relative jumps |a_R - a_L| / min(a_L, a_R)
+  unsigned long long *counters = nullptr;  // [0] faces by the network, [1] by HLLD; or null
+};
 // element type: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {
   void *rec = nullptr;   // kRec{2,3} x cells
   void *flux = nullptr;  // dim x kFluxComp{2,3} x cells
   void *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
+  const SurrogateCfg *sur = nullptr;  // surrogate face solves when set
 };
 
 // dt modes of the set_dt kernel
@@ -181,6 +188,14 @@ struct BatchMember {
                            hipStream_t s, StageTimer *tm);                                       \
   hipError_t fused_cfl(const DevParams &p, const double *U, double *parts, const double *ctrl,   \
                        hipStream_t s, StageTimer *tm);                                           \
+  hipError_t fused_half(const DevParams &p, const double *U, void *rec, const double *ctrl,      \
+                        hipStream_t s);                                                          \
+  /* surrogate form: face solves, and the training-sample dump of the current records */        \
+  hipError_t surrogate_faces(const DevParams &p, const void *rec, void *flux,                    \
+                             const double *ctrl, const SurrogateCfg &cfg, hipStream_t s);        \
+  hipError_t surrogate_dump(const DevParams &p, const void *rec, const double *ctrl, float *X,   \
+                            float *T, long long stride, long long phase, hipStream_t s);         \
+  long long surrogate_dump_slots(const DevParams &p, long long stride, long long phase);         \
   /* batched ensemble: counts of blocks per stage for one member, then the batched step */      \
   void batch_block_counts(const DevParams &p, int tile, long long counts[BS_COUNT]);            \
   hipError_t batch_step(const BatchMember *members, int nmembers, int dim, int bulk32, int tile, \
@@ -200,6 +215,9 @@ struct KernelTable {
   decltype(&fast::staged_godunov) godunov;
   decltype(&fast::fused_godunov) fused_godunov;
   decltype(&fast::fused_cfl) fused_cfl;
+  decltype(&fast::fused_half) fused_half;
+  decltype(&fast::surrogate_dump) surrogate_dump;
+  decltype(&fast::surrogate_dump_slots) surrogate_dump_slots;
   decltype(&fast::batch_block_counts) batch_block_counts;
   decltype(&fast::batch_step) batch_step;
 };

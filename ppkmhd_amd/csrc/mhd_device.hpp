@@ -39,7 +39,7 @@ struct DevParams {
   double gamma0, slope_type, smallr, smallc, smallp, cIso, cfl;
   int tile3;            // 3D half-step tile shape (index into the compiled shapes), tuning only
   int brick;            // block-to-cells shape of the fused per-cell kernels, tuning only
-  int riemann_mix;      // 1: all face and edge solves in one interleaved launch
+  int riemann_mix;      // 1, 2: all face and edge solves in one interleaved launch (4 / 3 waves)
   int bulk32;           // FP32-bulk form: half step, records, face/edge solves in float
 };
 

@@ -1,0 +1,158 @@
+// Wave-level MLP on the CDNA4 matrix cores: one wavefront evaluates 64 samples as four blocks
+// of 16, each layer a row of 16x16 MFMA tiles (bf16 operands, FP32 accumulators).  Layout and
+// weight packing: mlp.hpp.  Device code only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mlp.hpp"
+
+namespace ppk {
+namespace mlpd {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// make LDS writes of this wavefront visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ v4f mfma_k16(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_k32(v8bf a, v8bf b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ short bf16s(float x) { return (short)bf16_bits(x); }
+__device__ __forceinline__ __bf16 bf16b(float x) {
+  const uint16_t u = bf16_bits(x);
+  __bf16 b;
+  __builtin_memcpy(&b, &u, 2);
+  return b;
+}
+__device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
+
+// the pieces of a packed network the wave functions read (LDS or global)
+struct MlpView {
+  const uint16_t *A1, *A2, *A3, *A3T, *A2T;
+  const float *b1, *b2, *b3, *mu_in, *isig_in, *mu_out, *sig_out;
+};
+__device__ __forceinline__ MlpView view_of(const MlpPacked *w) {
+  return {&w->A1[0][0][0], &w->A2[0][0][0][0], &w->A3[0][0][0], &w->A3T[0][0][0],
+          &w->A2T[0][0][0][0], w->b1, w->b2, w->b3, w->mu_in, w->isig_in, w->mu_out,
+          w->sig_out};
+}
+
+// Hide the weight pointers from loop-invariant code motion: every 16-sample block reloads its
+// operands (cache / LDS hits) instead of the compiler holding all ~40 KB of weight operands of
+// a wave in registers across the loop.
+__device__ __forceinline__ MlpView opaque(MlpView w) {
+  asm volatile("" : "+s"(w.A1), "+s"(w.A2), "+s"(w.A3), "+s"(w.A3T), "+s"(w.A2T));
+  asm volatile("" : "+s"(w.b1), "+s"(w.b2), "+s"(w.b3));
+  return w;
+}
+
+__device__ __forceinline__ v4s ld_a16(const uint16_t *A, int tile, int lane) {
+  return *reinterpret_cast<const v4s *>(A + ((size_t)tile * 64 + lane) * 4);
+}
+__device__ __forceinline__ v8bf ld_a32(const uint16_t *A, int tile, int lane) {
+  return *reinterpret_cast<const v8bf *>(A + ((size_t)tile * 64 + lane) * 8);
+}
+
+// bf16 B operand of a 16x16x32 step q from two accumulator tiles (features 32q .. 32q+31)
+__device__ __forceinline__ v8bf pack_pair(v4f lo, v4f hi) {
+  v8bf r;
+  r[0] = bf16b(lo[0]);
+  r[1] = bf16b(lo[1]);
+  r[2] = bf16b(lo[2]);
+  r[3] = bf16b(lo[3]);
+  r[4] = bf16b(hi[0]);
+  r[5] = bf16b(hi[1]);
+  r[6] = bf16b(hi[2]);
+  r[7] = bf16b(hi[3]);
+  return r;
+}
+
+// Forward pass of one block of 16 samples.  xb: B operand of layer 1 (lane l: the bf16
+// features 4(l/16) .. 4(l/16)+3 of sample l%16).  Each hidden tile is rounded to bf16 as soon
+// as it is complete (tile r: features 16r + 4(l/16) + j of sample l%16), packed in pairs into
+// the next layer's B operands h1b / h2b; the masks hold the signs of the pre-activations (bit
+// 4r + j).  y: outputs 4(l/16) + j (rows >= 8 are 0).
+__device__ __forceinline__ void mlp_forward16(const MlpView &w, v4s xb, v8bf h1b[4], v8bf h2b[4],
+                                              unsigned &mask1, unsigned &mask2, v4f &y) {
+  const int lane = lane_id(), g = lane >> 4;
+  mask1 = mask2 = 0u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v4f t[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 2 * q + h;
+      v4f c;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = w.b1[16 * r + 4 * g + j];
+      c = mfma_k16(ld_a16(w.A1, r, lane), xb, c);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        mask1 |= (c[j] > 0.f ? 1u : 0u) << (4 * r + j);
+        t[h][j] = relu(c[j]);
+      }
+    }
+    h1b[q] = pack_pair(t[0], t[1]);
+  }
+#pragma unroll
+  for (int q2 = 0; q2 < 4; ++q2) {
+    v4f t[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 2 * q2 + h;
+      v4f c;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = w.b2[16 * r + 4 * g + j];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) c = mfma_k32(ld_a32(w.A2, r * 4 + q, lane), h1b[q], c);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        mask2 |= (c[j] > 0.f ? 1u : 0u) << (4 * r + j);
+        t[h][j] = relu(c[j]);
+      }
+    }
+    h2b[q2] = pack_pair(t[0], t[1]);
+  }
+  v4f c;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) c[j] = w.b3[4 * g + j];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) c = mfma_k32(ld_a32(w.A3, q, lane), h2b[q], c);
+  y = c;
+}
+
+// Inference of the 64 samples of a wave: sX holds their standardised bf16 features
+// ([sample][16], this wave's slice of LDS), sY receives the 8 outputs per sample
+// ([sample][8] floats).  Every lane of the wave must call it.
+__device__ __forceinline__ void mlp_infer_wave(const MlpView &w, const uint16_t *sX, float *sY) {
+  const int lane = lane_id(), g = lane >> 4, col = lane & 15;
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    const v4s xb = *reinterpret_cast<const v4s *>(sX + (16 * f + col) * 16 + 4 * g);
+    v8bf h1b[4], h2b[4];
+    unsigned m1, m2;
+    v4f y;
+    mlp_forward16(opaque(w), xb, h1b, h2b, m1, m2, y);
+    if (g < 2) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sY[(16 * f + col) * 8 + 4 * g + j] = y[j];
+    }
+  }
+  wave_lds_sync();
+}
+
+}  // namespace mlpd
+}  // namespace ppk

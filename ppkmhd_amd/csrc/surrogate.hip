@@ -1,0 +1,237 @@
+// Surrogate form of the face solves, and the face-sample dump its network is trained on.
+//
+//   k_face_sur<DIM, D>  the face fluxes of direction D with the learned flux where the gate and
+//                       the admissibility check allow, exact HLLD elsewhere (per face).  A
+//                       block of 256 threads stages the network (40 KB of bf16 weights) in LDS
+//                       once and then walks bricks of the face sub-box; each wavefront runs the
+//                       network on the matrix cores for its 64 faces when any of them passed
+//                       the gate, and HLLD for the faces that did not or whose flux was refused.
+//   k_dump<DIM>         features and training targets of every stride-th face (all directions)
+//
+// The face states are those of the plain fused step (face_states2/3), so a face that falls
+// back gets the plain step's flux bit for bit.  Formulas of the surrogate: surrogate_body.hpp.
+#include <hip/hip_runtime.h>
+
+#include "fused_body.hpp"
+#include "mlp_device.hpp"
+#include "surrogate_body.hpp"
+
+namespace ppk {
+namespace PPK_NS {
+namespace surk {
+
+using namespace fusedk;
+using namespace mlpd;
+
+constexpr int kBlock = 256;
+constexpr int kFloatBlock = 128 + 128 + 16 + 16 + 16 + 8 + 8;  // b1 b2 b3 mu_in isig_in mu_out sig_out
+
+__device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
+
+template <int DIM, int D, class R>
+PPK_HD void states(const R *rec, const DevParams &p, long long c, St<R> &ql, St<R> &qr) {
+  if constexpr (DIM == 3)
+    face_states3<D>(rec, p, c, ql, qr);
+  else
+    face_states2<D>(rec, p, c, ql, qr);
+}
+template <int DIM, int D, class R>
+PPK_HD void store(R *flux, const DevParams &p, long long c, const St<R> &f) {
+  if constexpr (DIM == 3)
+    store_flux3<D>(flux, p, c, f);
+  else
+    store_flux2<D>(flux, p, c, f);
+}
+
+template <int DIM, int D, class R>
+__global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
+                                                     R *__restrict__ flux, DevParams p, Brick bk,
+                                                     const double *ctrl, SurrogateCfg cfg) {
+  if (!active(ctrl)) return;
+  __shared__ __attribute__((aligned(16))) uint16_t sW[kPackA1 + kPackA2 + kPackA3];
+  __shared__ float sFl[kFloatBlock];
+  __shared__ __attribute__((aligned(16))) uint16_t sXall[4][64 * 16];
+  __shared__ float sYall[4][64 * 8];
+  const MlpPacked *net = static_cast<const MlpPacked *>(cfg.net);
+  {  // stage the inference weights: A1, A2, A3 are the first kMlpInferBytes of the net
+    const uint4 *src = reinterpret_cast<const uint4 *>(net);
+    uint4 *dst = reinterpret_cast<uint4 *>(sW);
+    for (int n = threadIdx.x; n < kMlpInferBytes / 16; n += kBlock) dst[n] = src[n];
+    const float *fsrc = net->b1;  // b1 .. sig_out are contiguous
+    for (int n = threadIdx.x; n < kFloatBlock; n += kBlock) sFl[n] = fsrc[n];
+  }
+  __syncthreads();
+  MlpView w;
+  w.A1 = sW;
+  w.A2 = sW + kPackA1;
+  w.A3 = sW + kPackA1 + kPackA2;
+  w.A3T = w.A2T = nullptr;
+  w.b1 = sFl;
+  w.b2 = sFl + 128;
+  w.b3 = sFl + 256;
+  w.mu_in = sFl + 272;
+  w.isig_in = sFl + 288;
+  w.mu_out = sFl + 304;
+  w.sig_out = sFl + 312;
+  const int wave = threadIdx.x >> 6, lane = lane_id();
+  uint16_t *sX = sXall[wave];
+  float *sY = sYall[wave];
+  const R gamma0 = R(p.gamma0);
+  const long long nb = bk.nblocks;
+#pragma unroll 1
+  for (long long L = blockIdx.x; L < nb; L += gridDim.x) {
+    const long long c = sub_brick_cell(p, bk, (int)L, threadIdx.x);
+    St<R> ql, qr, F;
+    R s = R(1);
+    bool gate = false;
+    if (c >= 0) {
+      states<DIM, D>(rec, p, c, ql, qr);
+      St<R> ul, ur, fl, fr;
+      llf_parts(ql, qr, ul, ur, fl, fr, s, F, p);
+      gate = smooth_face(ql, qr, cfg.gate_rho, cfg.gate_p);
+    }
+    bool use = false;
+    if (__ballot(gate) != 0ull) {  // wave-uniform: the network runs for the whole wavefront
+      float x[kMlpIn];
+      R rb = R(1);
+      if (gate) {
+        face_features(ql, qr, s, gamma0, x, rb);
+      } else {
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) x[i] = w.mu_in[i];
+      }
+#pragma unroll
+      for (int i = 0; i < kMlpIn; ++i) sX[lane * 16 + i] = bf16_bits((x[i] - w.mu_in[i]) * w.isig_in[i]);
+      wave_lds_sync();
+      mlp_infer_wave(w, sX, sY);
+      if (gate) {
+        R sc[kMlpOut];
+        out_scales(rb, s, sc);
+        St<R> Fs;
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o)
+          Fs[o] = F[o] + sc[o] * R(w.mu_out[o] + w.sig_out[o] * sY[lane * 8 + o]);
+        St<R> ul, ur, fl, fr;
+        mhd_flux(ql, ul, fl, p);
+        mhd_flux(qr, ur, fr, p);
+        use = admissible(ul, ur, fl, fr, Fs, s, gamma0);
+        if (use) F = Fs;
+      }
+    }
+    if (c >= 0 && !use) riemann_hlld(ql, qr, F, p);
+    if (c >= 0) store<DIM, D>(flux, p, c, F);
+    if (cfg.counters) {
+      const unsigned long long nsur = __popcll(__ballot(c >= 0 && use));
+      const unsigned long long nall = __popcll(__ballot(c >= 0));
+      if (lane == 0 && nall) {
+        atomicAdd(&cfg.counters[0], nsur);
+        atomicAdd(&cfg.counters[1], nall - nsur);
+      }
+    }
+  }
+}
+
+// one thread per selected face: slot s is face t = s * stride + phase of the enumeration
+// (direction, then the sub-box cells, i fastest)
+template <int DIM, class R>
+__global__ void __launch_bounds__(kBlock) k_dump(const R *__restrict__ rec, DevParams p,
+                                                 const double *ctrl, float *X, float *T,
+                                                 long long nslots, long long stride,
+                                                 long long phase) {
+  if (!active(ctrl)) return;
+  const long long sl = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (sl >= nslots) return;
+  const long long t = sl * stride + phase;
+  const long long nxs = p.nx + 1, nys = p.ny + 1, nzs = DIM == 3 ? p.nz + 1 : 1;
+  const long long nsub = nxs * nys * nzs;
+  const int D = (int)(t / nsub);
+  const long long r = t % nsub;
+  const int i = p.gw + (int)(r % nxs), j = p.gw + (int)((r / nxs) % nys);
+  const int k = DIM == 3 ? p.gw + (int)(r / (nxs * nys)) : 0;
+  const long long c = cidx(p, i, j, k);
+  St<R> ql, qr;
+  if (D == 0)
+    states<DIM, 0>(rec, p, c, ql, qr);
+  else if (D == 1)
+    states<DIM, 1>(rec, p, c, ql, qr);
+  else
+    states<DIM, DIM == 3 ? 2 : 1>(rec, p, c, ql, qr);
+  St<R> fh;
+  {
+    St<R> a = ql, b = qr;
+    riemann_hlld(a, b, fh, p);
+  }
+  St<R> ul, ur, fl, fr, F;
+  R s;
+  llf_parts(ql, qr, ul, ur, fl, fr, s, F, p);
+  float x[kMlpIn], tg[kMlpOut];
+  R rb, sc[kMlpOut];
+  face_features(ql, qr, s, R(p.gamma0), x, rb);
+  out_scales(rb, s, sc);
+  face_target(fh, F, sc, tg);
+#pragma unroll
+  for (int n = 0; n < kMlpIn; ++n) X[sl * kMlpIn + n] = x[n];
+#pragma unroll
+  for (int n = 0; n < kMlpOut; ++n) T[sl * kMlpOut + n] = tg[n];
+}
+
+template <class R>
+hipError_t faces_impl(const DevParams &p, const R *rec, R *flux, const double *ctrl,
+                      const SurrogateCfg &cfg, hipStream_t s) {
+  const Brick bk = sub_bricks(p);
+  const long long nb = bk.count();
+  const dim3 g((unsigned)(nb < 512 ? nb : 512)), b(kBlock);
+  if (p.dim == 3) {
+    hipLaunchKernelGGL((k_face_sur<3, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+    hipLaunchKernelGGL((k_face_sur<3, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+    hipLaunchKernelGGL((k_face_sur<3, 2, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+  } else {
+    hipLaunchKernelGGL((k_face_sur<2, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+    hipLaunchKernelGGL((k_face_sur<2, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace surk
+
+long long surrogate_dump_slots(const DevParams &p, long long stride, long long phase) {
+  const long long nsub = (long long)(p.nx + 1) * (p.ny + 1) * (p.dim == 3 ? p.nz + 1 : 1);
+  const long long faces = p.dim * nsub;
+  if (stride < 1 || phase < 0 || phase >= stride || phase >= faces) return 0;
+  return (faces - phase + stride - 1) / stride;
+}
+
+hipError_t surrogate_faces(const DevParams &p, const void *rec, void *flux, const double *ctrl,
+                           const SurrogateCfg &cfg, hipStream_t s) {
+  if (p.bulk32)
+    return surk::faces_impl<float>(p, static_cast<const float *>(rec), static_cast<float *>(flux),
+                                   ctrl, cfg, s);
+  return surk::faces_impl<double>(p, static_cast<const double *>(rec),
+                                  static_cast<double *>(flux), ctrl, cfg, s);
+}
+
+hipError_t surrogate_dump(const DevParams &p, const void *rec, const double *ctrl, float *X,
+                          float *T, long long stride, long long phase, hipStream_t s) {
+  const long long n = surrogate_dump_slots(p, stride, phase);
+  if (n <= 0) return hipSuccess;
+  const dim3 g((unsigned)((n + surk::kBlock - 1) / surk::kBlock)), b(surk::kBlock);
+  if (p.dim == 3) {
+    if (p.bulk32)
+      hipLaunchKernelGGL((surk::k_dump<3, float>), g, b, 0, s, static_cast<const float *>(rec), p,
+                         ctrl, X, T, n, stride, phase);
+    else
+      hipLaunchKernelGGL((surk::k_dump<3, double>), g, b, 0, s, static_cast<const double *>(rec),
+                         p, ctrl, X, T, n, stride, phase);
+  } else {
+    if (p.bulk32)
+      hipLaunchKernelGGL((surk::k_dump<2, float>), g, b, 0, s, static_cast<const float *>(rec), p,
+                         ctrl, X, T, n, stride, phase);
+    else
+      hipLaunchKernelGGL((surk::k_dump<2, double>), g, b, 0, s, static_cast<const double *>(rec),
+                         p, ctrl, X, T, n, stride, phase);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace PPK_NS
+}  // namespace ppk

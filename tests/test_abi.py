@@ -21,7 +21,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(ppk_mhd_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(ppk_(?:mhd|mlp|faceset)_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,205 @@
+"""The surrogate form on the GPU: the MFMA network kernels against the CPU oracle
+(inference, loss gradient), training, the face-sample dump against the oracle's own HLLD and
+LLF, the fallback and LLF limits of the surrogate step, and what a trained network gains over
+the same scheme without it."""
+import numpy as np
+import pytest
+
+from parity import rel_l1
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd import default_problem, make_params
+from test_surrogate_cpu import random_params
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(p, pr=None):
+    from ppkmhd_amd.solver import SolverMHDMuscl
+    return SolverMHDMuscl(p, pr)
+
+
+def S():
+    from oracle import surrogate_oracle
+    return surrogate_oracle
+
+
+def test_mlp_inference_matches_oracle(gpu):
+    from ppkmhd_amd.surrogate import MLP
+    P = random_params(11)
+    net = MLP(0)
+    net.set_params(P)
+    assert np.array_equal(net.get_params(), P)
+    X = np.random.default_rng(12).normal(size=(1000, 16)).astype(np.float32)
+    Y = net.infer(X)
+    ref, _ = S().mlp_forward(P, X, rounding=True)
+    exact, _ = S().mlp_forward(P, X, rounding=False)
+    err = np.abs(Y - ref).max() / np.abs(ref).max()
+    err_exact = np.abs(Y - exact).max() / np.abs(exact).max()
+    print(f"inference: max error vs bf16 oracle {err:.2e}, vs exact {err_exact:.2e}")
+    assert err <= 1e-2, err
+    assert err_exact <= 5e-2, err_exact
+
+
+def test_mlp_gradient_matches_oracle(gpu):
+    from ppkmhd_amd.surrogate import MLP
+    P = random_params(13)
+    net = MLP(0)
+    net.set_params(P)
+    rng = np.random.default_rng(14)
+    X = rng.normal(size=(512, 16)).astype(np.float32)
+    T = rng.normal(size=(512, 8)).astype(np.float32)
+    g = net.gradient(X, T)
+    ref = S().mlp_grad(P, X, T, rounding=True)
+    exact = S().mlp_grad(P, X, T, rounding=False)
+    rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    rel_exact = np.linalg.norm(g - exact) / np.linalg.norm(exact)
+    blocks = {"W1": (S().W1, S().B1), "W2": (S().W2, S().B2), "W3": (S().W3, S().B3),
+              "b": (S().B3, S().N_TRAINED)}
+    for name, (a, b) in blocks.items():
+        r = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
+        print(f"gradient {name}: relative L2 error vs bf16 oracle {r:.2e}")
+        assert r <= 2e-2, (name, r)
+    print(f"gradient: relative L2 vs bf16 oracle {rel:.2e}, vs exact {rel_exact:.2e}")
+    assert rel <= 1e-2 and rel_exact <= 5e-2
+    # the parameters did not move
+    assert np.array_equal(net.get_params(), P)
+
+
+def test_training_reduces_loss(gpu):
+    from ppkmhd_amd.surrogate import MLP, FaceSet
+    rng = np.random.default_rng(15)
+    n = 1 << 15
+    X = rng.uniform(-1, 1, size=(n, 16)).astype(np.float32)
+    M = rng.normal(size=(16, 8)) / 4
+    T = np.tanh(X @ M).astype(np.float32) + 0.1 * (X[:, :8] ** 2).astype(np.float32)
+    fs = FaceSet(n)
+    fs.append(X, T)
+    net = MLP(3)
+    net.standardize(fs)
+    before = net.evaluate(fs).mean()
+    net.train(fs, steps=400, batch=1024, lr=2e-3)
+    after = net.evaluate(fs).mean()
+    print(f"training: standardised MSE {before:.3e} -> {after:.3e}")
+    assert after < 0.05 * before
+
+
+def ot(n, precision=A.FP64, riemann=A.RIEMANN_HLLD):
+    p = make_params(2, n, n, gamma0=5 / 3, cfl=0.8, tEnd=0.5, riemann=riemann,
+                    precision=precision)
+    return p, default_problem(A.PROBLEM_ORSZAG_TANG, p)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dump_matches_oracle_fluxes(oracle, gpu, dim):
+    """Every dumped sample: rebuild the normalised face from its features; the oracle's HLLD
+    minus LLF of that face is the dumped target."""
+    from ppkmhd_amd.surrogate import FaceSet, dump_faces
+    if dim == 2:
+        p, pr = ot(32)
+    else:
+        p = make_params(3, 12, 10, 8, gamma0=5 / 3, cfl=0.8, tEnd=1.0)
+        pr = default_problem(A.PROBLEM_BLAST, p)
+        pr.blast_radius = 0.3
+    g = solver(p, pr)
+    g.run(3)
+    nfaces = dim * (p.nx + 1) * (p.ny + 1) * ((p.nz + 1) if dim == 3 else 1)
+    fs = FaceSet(nfaces)
+    assert dump_faces(g, fs, 1, 0) == nfaces
+    X, T = fs.read()
+    assert np.isfinite(X).all() and np.isfinite(T).all()
+    # rho_L + rho_R = 2 rho_bar, and |u| <= s
+    np.testing.assert_allclose(X[:, 0] + X[:, 8], 2.0, rtol=1e-6)
+    assert np.abs(X[:, [1, 9]]).max() <= 1.0 + 1e-6
+    pl = make_params(2, 8, 8, gamma0=5 / 3, riemann=A.RIEMANN_HLLD, float_round=False)
+    rng = np.random.default_rng(16)
+    worst = 0.0
+    for i in rng.choice(nfaces, 300, replace=False):
+        ql, qr, gam = S().states_from_features(X[i].astype(np.float64))
+        _, t = S().face_sample(lambda a, b: oracle.riemann(pl, a, b), ql, qr, gam)
+        worst = max(worst, np.abs(t - T[i]).max())
+    print(f"dump {dim}D: {nfaces} faces, worst target mismatch {worst:.2e}")
+    assert worst <= 1e-5
+    # a dump of every 7th face from phase 3 picks exactly those
+    fs2 = FaceSet(nfaces)
+    g2 = solver(p, pr)
+    g2.run(3)
+    k = dump_faces(g2, fs2, 7, 3)
+    X2, _ = fs2.read()
+    assert k == len(range(3, nfaces, 7))
+    np.testing.assert_array_equal(X2, X[3::7])
+
+
+@pytest.mark.parametrize("precision", [A.FP64, A.FP64_STRICT])
+def test_gate_zero_is_plain_hlld(gpu, precision):
+    from ppkmhd_amd.surrogate import MLP, attach, counts
+    p, pr = ot(48, precision)
+    a, b = solver(p, pr), solver(p, pr)
+    net = MLP(5)
+    attach(b, net, 0.0, 0.0)
+    a.run(10)
+    b.run(10)
+    assert np.array_equal(a.download(), b.download())
+    n_net, n_hlld = counts(b)
+    assert n_net == 0 and n_hlld == 10 * 2 * 49 * 49
+
+
+def test_zero_network_is_the_llf_scheme(oracle, gpu):
+    """A network that outputs 0 with an open gate gives F_LLF at every face: the strict step
+    then equals the oracle's step with riemann = LLF bit for bit (edges stay 2D HLLD)."""
+    from ppkmhd_amd.surrogate import MLP, attach, counts
+    p, pr = ot(40, A.FP64_STRICT)
+    P = np.zeros(S().N_PARAMS, dtype=np.float32)
+    P[S().ISIG_IN:S().MU_OUT] = 1.0
+    P[S().SIG_OUT:S().N_PARAMS] = 1.0
+    net = MLP(0)
+    net.set_params(P)
+    g = solver(p, pr)
+    attach(g, net, 1e30, 1e30)
+    q, _ = ot(40, A.FP64_STRICT, riemann=A.RIEMANN_LLF)
+    o = oracle.Solver(q)
+    o.init_problem(pr)
+    for _ in range(8):
+        g.step(0, A.DT_CFL)
+        o.step(0, A.DT_CFL)
+    assert np.array_equal(g.download(), o.get_state())
+    n_net, n_hlld = counts(g)
+    assert n_hlld == 0 and n_net == 8 * 2 * 41 * 41
+
+
+def test_trained_network_gains_over_llf(gpu):
+    """Train on faces of the solver's own Orszag-Tang run, then compare, against plain HLLD at
+    t = 0.2: the same gated scheme with the trained network and with no network (the LLF flux
+    at gated faces)."""
+    from ppkmhd_amd.surrogate import MLP, FaceSet, attach, counts, dump_faces
+    p, pr = ot(64)
+    src = solver(p, pr)
+    fs = FaceSet(2 * 65 * 65 * 80)
+    for k in range(80):
+        dump_faces(src, fs, 1, 0)
+        src.run(3)
+    net = MLP(7)
+    net.standardize(fs)
+    mse0 = net.evaluate(fs)
+    net.train(fs, steps=3000, batch=4096, lr=1e-3)
+    mse1 = net.evaluate(fs)
+    print(f"trained on {len(fs)} faces: standardised MSE {mse0.mean():.3e} -> {mse1.mean():.3e}")
+    zero = MLP(0)
+    P = np.zeros(S().N_PARAMS, dtype=np.float32)
+    P[S().ISIG_IN:S().MU_OUT] = 1.0
+    P[S().SIG_OUT:S().N_PARAMS] = 1.0
+    zero.set_params(P)
+    q, _ = ot(64)
+    q.tEnd = 0.2
+    ref = solver(q, pr)
+    ref.run(100000)
+    out = {}
+    for name, m in (("trained", net), ("no network", zero)):
+        s = solver(q, pr)
+        attach(s, m, 0.1, 0.1)
+        s.run(100000)
+        assert s.time()[0] == pytest.approx(0.2)
+        e = rel_l1(s.download(), ref.download(), q, var=0)
+        a, b = counts(s)
+        out[name] = e
+        print(f"{name}: density L1 vs HLLD {e:.3e}, faces by network {a / (a + b):.1%}")
+    assert out["trained"] < 0.5 * out["no network"], out
