@@ -244,8 +244,9 @@ int ppk_mhd_save_vtk(ppk_mhd_t *h, const char *dir, const char *prefix, int iste
  * boundaries, tEnd, n_step_max) advanced together: every stage of a step is one launch over
  * all members, and steps after the first replay one captured hipGraph.  Each member follows
  * its own time loop exactly as one ppk_mhd_t would with dt_mode PPK_DT_CFL (its own CFL dt,
- * clamp to tEnd, finished() test), and its state after each step equals that handle's bit for
- * bit in the same precision (the fused path's per-cell code is shared).
+ * clamp to tEnd, finished() test); the per-cell code is the fused path's, so its state after
+ * each step equals that handle's bit for bit in PPK_FP64_STRICT (to roundoff in the other
+ * precisions, where the compiler's FMA contractions may differ between the two kernels).
  * Replaces: a loop over the reference's driver, one SolverMHDMuscl per member
  * (src/main.cpp:130-160 with src/shared/SolverBase.cpp:161-219), for PPK_PATH_FUSED. */
 typedef struct ppk_mhd_batch ppk_mhd_batch;

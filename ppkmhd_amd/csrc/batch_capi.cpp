@@ -113,6 +113,24 @@ int set_tables(ppk_mhd_batch_t *h, int tile, int brick) {
   return PPK_OK;
 }
 
+// the step graph (one batch_step without the standalone CFL pass), captured once per table
+int capture_step(ppk_mhd_batch_t *h) {
+  if (h->exec) return PPK_OK;
+  BHIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = h->K->batch_step(h->d_members, h->nm, h->dim, h->bulk32, h->tile, h->totals, 0,
+                                  h->stream);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(h, PPK_ERR_HIP, std::string("step graph capture failed: ") +
+                                     hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  h->graph = g;
+  BHIP(h, hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  return PPK_OK;
+}
+
 int check_member(ppk_mhd_batch_t *h, int m) {
   if (m < 0 || m >= h->nm) return bfail(h, PPK_ERR_INVALID_ARGUMENT, "member index out of range");
   return PPK_OK;
@@ -301,19 +319,9 @@ int ppk_mhd_batch_run(ppk_mhd_batch_t *h, int nsteps) {
     h->cfl_valid = true;
     done = 1;
   }
-  if (done < nsteps && !h->exec) {
-    BHIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = h->K->batch_step(h->d_members, h->nm, h->dim, h->bulk32, h->tile, h->totals,
-                                    0, h->stream);
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
-    if (e != hipSuccess || e2 != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return bfail(h, PPK_ERR_HIP, std::string("step graph capture failed: ") +
-                                       hipGetErrorString(e != hipSuccess ? e : e2));
-    }
-    h->graph = g;
-    BHIP(h, hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  if (done < nsteps) {
+    const int rc = capture_step(h);
+    if (rc != PPK_OK) return rc;
   }
   for (; done < nsteps; ++done) BHIP(h, hipGraphLaunch(h->exec, h->stream));
   BHIP(h, hipStreamSynchronize(h->stream));
@@ -390,8 +398,9 @@ int ppk_mhd_batch_autotune(ppk_mhd_batch_t *h, int trial_steps, int *tile, int *
     for (int tl = 0; tl < ppk::kBatchTiles && status == PPK_OK; ++tl)
       for (int bk = 0; bk < brick_shapes(h->dim) && status == PPK_OK; ++bk) {
         if ((status = set_tables(h, tl, bk)) != PPK_OK) break;
-        // one step outside the timing (standalone CFL + graph capture), then the replays
+        // one step outside the timing (standalone CFL), the graph capture, then the replays
         if ((status = ppk_mhd_batch_run(h, 1)) != PPK_OK) break;
+        if ((status = capture_step(h)) != PPK_OK) break;
         float ms = 0.f;
         if (hipEventRecord(e0, h->stream) != hipSuccess) status = PPK_ERR_HIP;
         for (int s = 0; s < trial_steps && status == PPK_OK; ++s)

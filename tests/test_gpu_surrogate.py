@@ -5,7 +5,7 @@ the same scheme without it."""
 import numpy as np
 import pytest
 
-from parity import rel_l1
+from parity import rel_l1, rel_linf
 from ppkmhd_amd import _abi as A
 from ppkmhd_amd import default_problem, make_params
 from test_surrogate_cpu import random_params
@@ -138,7 +138,10 @@ def test_gate_zero_is_plain_hlld(gpu, precision):
     attach(b, net, 0.0, 0.0)
     a.run(10)
     b.run(10)
-    assert np.array_equal(a.download(), b.download())
+    if precision == A.FP64_STRICT:
+        assert np.array_equal(a.download(), b.download())
+    else:  # the FMA contractions of the two face kernels may differ
+        assert rel_linf(b.download(), a.download(), p) <= 1e-13
     n_net, n_hlld = counts(b)
     assert n_net == 0 and n_hlld == 10 * 2 * 49 * 49
 
@@ -172,9 +175,10 @@ def test_trained_network_gains_over_llf(gpu):
     at gated faces)."""
     from ppkmhd_amd.surrogate import MLP, FaceSet, attach, counts, dump_faces
     p, pr = ot(64)
+    p.tEnd = 10.0
     src = solver(p, pr)
-    fs = FaceSet(2 * 65 * 65 * 80)
-    for k in range(80):
+    fs = FaceSet(2 * 65 * 65 * 60)
+    for k in range(60):  # t from 0 to about 0.6
         dump_faces(src, fs, 1, 0)
         src.run(3)
     net = MLP(7)
