@@ -50,6 +50,11 @@ FLOP_PER_ITEM_3D.update({"fused_riemann": 855.0 + 1575.0, "fused_half": 45.0 + 4
                          "fused_update": 78.0 + 57.5})
 # the step: each cell's 8 FP64 conservative variables read once and written once
 STEP_BYTES_PER_CELL = 128.0
+# HBM bytes per work item measured with rocprofv3 FETCH_SIZE (x2) + WRITE_SIZE (x1) at 256^3,
+# FP64 (profiles/fused_blast3d_256_pmc.md): per launch 8.77 GB (riemann), 7.99 GB (half),
+# 5.35 GB (update), over the items of the table above
+MEASURED_BYTES_PER_ITEM_3D = {"fused_riemann": 8.77e9 / 257 ** 3, "fused_half": 7.99e9 / 258 ** 3,
+                              "fused_update": 5.35e9 / 256 ** 3}
 
 
 def items(stage, n):
@@ -203,6 +208,121 @@ def bench_ensemble(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_surrogate(a):
+    """Surrogate form, 2D: train the network on faces dumped from the solver's own runs (the
+    Orszag-Tang 256^2 run to t = 0.5, ensemble members, a 3D blast), then run Orszag-Tang at
+    n^2 to t = 0.5 with plain HLLD and with the gated surrogate, and compare time and density."""
+    from ppkmhd_amd.ensemble import ot_ensemble
+    from ppkmhd_amd.solver import SolverMHDMuscl
+    from ppkmhd_amd import surrogate as SG
+
+    def ot(n, tEnd):
+        p = make_params(2, n, n, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
+                        bc=A.BC_PERIODIC, tEnd=tEnd, n_step_max=1 << 30)
+        return p, default_problem(A.PROBLEM_ORSZAG_TANG, p)
+
+    t0 = time.perf_counter()
+    fs = SG.FaceSet(a.samples + (1 << 22))
+    # Orszag-Tang 256^2 to t = 0.5, every 4th step
+    p, pr = ot(256, 0.5)
+    s = SolverMHDMuscl(p, pr)
+    faces = 2 * 257 * 257
+    stride = max(1, -(-faces * 160 // max(1, a.samples // 2)))  # ~160 snapshots to t = 0.5
+    while not s.finished() and len(fs) + faces // stride + 1 <= a.samples // 2:
+        SG.dump_faces(s, fs, stride, 0)
+        s.run(4)
+    s.close()
+    n_ot = len(fs)
+    # ensemble members (uneven sizes, gamma, B0), four snapshots each
+    ps, prs, _ = ot_ensemble(16, a.seed, 1000)
+    per_snap = max(1, a.samples // 4 // 64)
+    for q, qr in zip(ps, prs):
+        m = SolverMHDMuscl(q, qr)
+        faces = 2 * (q.nx + 1) * (q.ny + 1)
+        stride = max(1, faces // per_snap)
+        for _ in range(4):
+            if len(fs) + faces // stride + 1 > a.samples * 3 // 4:
+                break
+            SG.dump_faces(m, fs, stride, 0)
+            m.run(60)
+        m.close()
+    n_ens = len(fs) - n_ot
+    # 3D blast 128^3, faces rotated to x (the network sees the normal direction as x)
+    q = make_params(3, 128, 128, 128, gamma0=5.0 / 3.0, cfl=0.8, tEnd=1.0)
+    qr = default_problem(A.PROBLEM_BLAST, q)
+    qr.blast_radius = 0.1
+    qr.blast_density_out = 1.0
+    qr.blast_pressure_in, qr.blast_pressure_out = 10.0, 0.1
+    qr.blast_bx = qr.blast_by = qr.blast_bz = 1.0 / np.sqrt(3.0)
+    b3 = SolverMHDMuscl(q, qr)
+    b3.run(40)
+    faces3 = 3 * 129 ** 3
+    room = a.samples - len(fs)
+    if room > 0:
+        SG.dump_faces(b3, fs, max(1, faces3 // room + 1), 0)
+    b3.close()
+    n_all = len(fs)
+    train_set = SG.select_gated(fs, a.gate, a.gate)
+    fs.close()
+    t_dump = time.perf_counter() - t0
+    log(f"[bench] samples: {n_all} dumped (Orszag-Tang {n_ot}, ensemble {n_ens}, blast "
+        f"{n_all - n_ot - n_ens}), {len(train_set)} pass the gate; {t_dump:.1f} s")
+    net = SG.MLP(a.seed)
+    net.standardize(train_set)
+    mse0 = net.evaluate(train_set).mean()
+    t0 = time.perf_counter()
+    SG.train_schedule(net, train_set, a.train_steps, batch=4096, lr=1e-3, seed=a.seed + 1)
+    t_train = time.perf_counter() - t0
+    mse1 = net.evaluate(train_set).mean()
+    log(f"[bench] training: {a.train_steps} steps of 4096 in {t_train:.1f} s, standardised MSE "
+        f"{mse0:.3e} -> {mse1:.3e}")
+
+    n = a.n if a.n != 256 else 1024
+    runs = {}
+    for name in ("plain", "surrogate"):
+        p, pr = ot(n, 0.5)
+        s = SolverMHDMuscl(p, pr)
+        if name == "surrogate":
+            SG.attach(s, net, a.gate, a.gate)
+        s.run(a.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        while not s.finished():
+            s.run(100)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        t, dt, it = s.time()
+        steps = it - a.warmup
+        runs[name] = (s.download(), el, steps, SG.counts(s) if name == "surrogate" else None)
+        log(f"[bench] {name}: {steps} steps to t={t:.4f} in {el:.3f} s, "
+            f"{el / steps * 1e3:.3f} ms/step, {n * n * steps / el / 1e9:.3f} G cell-updates/s")
+        s.close()
+    Up, ep, sp, _ = runs["plain"]
+    Us, es, ss, (by_net, by_hlld) = runs["surrogate"]
+    p, _ = ot(n, 0.5)
+    ii = p.interior()
+    l1 = float(np.abs(Us[ii][0] - Up[ii][0]).sum() / np.abs(Up[ii][0]).sum())
+    v_plain, v_sur = n * n * sp / ep, n * n * ss / es
+    log(f"[bench] density L1 (surrogate vs plain HLLD, t=0.5): {l1:.3e}; faces by network "
+        f"{by_net / max(1, by_net + by_hlld):.1%}; speed ratio {v_sur / v_plain:.3f}")
+    out = {
+        "metric": "MHD cell-updates/s (surrogate + fallback, 2D Orszag-Tang, FP64 state)",
+        "value": v_sur, "unit": "cell-updates/s", "n_gpus": 1, "steps": ss, "warmup": a.warmup,
+        "ms_per_step": es / ss * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 state; bf16 MLP with f32 accumulation",
+        "data": "synthetic (analytic initial conditions; training faces from the solver's runs)",
+        "config": {"workload": f"Orszag-Tang {n}^2 to t=0.5, gated MLP flux (gate {a.gate}) "
+                               f"with admissibility fallback to HLLD", "precision": "fp64"},
+        "plain_value": v_plain, "speed_ratio": v_sur / v_plain, "density_l1_vs_plain": l1,
+        "faces_by_network": by_net / max(1, by_net + by_hlld),
+        "training": {"samples_dumped": n_all, "samples_gated": len(train_set),
+                     "steps": a.train_steps, "batch": 4096, "seconds": t_train,
+                     "mse_standardised_before": mse0, "mse_standardised_after": mse1},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,8 +335,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the state (profiler calibration)")
-    ap.add_argument("--workload", choices=("blast3d", "ensemble2d"), default="blast3d",
-                    help="blast3d: the 3D blast step (default); ensemble2d: batched ensemble")
+    ap.add_argument("--workload", choices=("blast3d", "ensemble2d", "surrogate2d"),
+                    default="blast3d",
+                    help="blast3d: the 3D blast step (default); ensemble2d: batched ensemble; "
+                         "surrogate2d: learned flux on Orszag-Tang")
+    ap.add_argument("--samples", type=int, default=20_000_000)
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--gate", type=float, default=0.05)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tune-steps", type=int, default=3)
@@ -229,6 +354,8 @@ def main():
 
     if a.workload == "ensemble2d":
         return bench_ensemble(a)
+    if a.workload == "surrogate2d":
+        return bench_surrogate(a)
 
     from ppkmhd_amd.solver import SolverMHDMuscl
 
@@ -285,6 +412,9 @@ def main():
             gbs = nbytes / (avg_ms * 1e-3) / 1e9
             roof["hbm_achieved_GBs"] = gbs
             roof["hbm_frac"] = gbs / PEAK_HBM_GBS
+        if dom in MEASURED_BYTES_PER_ITEM_3D and prec == A.FP64:
+            roof["traffic"] = MEASURED_BYTES_PER_ITEM_3D[dom] * items(dom, n)
+            roof["traffic_GBs"] = roof["traffic"] / (avg_ms * 1e-3) / 1e9
     step_hbm = STEP_BYTES_PER_CELL * value / 1e9
     step_tflops = step_flops_per_cell(n) * value / 1e12
 

@@ -135,14 +135,17 @@ def llf(ql, qr, gamma0):
 
 
 def features(ql, qr, gamma0):
-    """16 dimensionless features, rho_bar and s of a face."""
+    """16 dimensionless features (means and jumps of the two states), rho_bar and s."""
     _, s, *_ = llf(ql, qr, gamma0)
     bx = 0.5 * (ql[IA] + qr[IA])
     rb = 0.5 * (ql[ID] + qr[ID])
     ib = 1.0 / (np.sqrt(rb) * s)
-    x = [ql[ID] / rb, ql[IU] / s, ql[IV] / s, ql[IW] / s, bx * ib, ql[IB] * ib, ql[IC] * ib,
-         ql[IP] / (rb * s * s), qr[ID] / rb, qr[IU] / s, qr[IV] / s, qr[IW] / s,
-         gamma0 - GAMMA_REF, qr[IB] * ib, qr[IC] * ib, qr[IP] / (rb * s * s)]
+    ip = 1.0 / (rb * s * s)
+    m = lambda v: 0.5 * (ql[v] + qr[v])
+    d = lambda v: qr[v] - ql[v]
+    x = [d(ID) / rb, m(IU) / s, m(IV) / s, m(IW) / s, bx * ib, m(IB) * ib, m(IC) * ib,
+         m(IP) * ip, gamma0 - GAMMA_REF, d(IU) / s, d(IV) / s, d(IW) / s, d(IB) * ib,
+         d(IC) * ib, d(IP) * ip, 0.0]
     return np.array(x), rb, s
 
 
@@ -163,9 +166,10 @@ def face_sample(hlld, ql, qr, gamma0):
 
 def states_from_features(x):
     """The normalised face (rho_bar = 1, s = 1) the features describe, and gamma."""
-    ql = np.array([x[0], x[7], x[1], x[2], x[3], x[4], x[5], x[6]], float)
-    qr = np.array([x[8], x[15], x[9], x[10], x[11], x[4], x[13], x[14]], float)
-    return ql, qr, float(x[12]) + GAMMA_REF
+    x = np.asarray(x, dtype=np.float64)
+    mean = np.array([1.0, x[7], x[1], x[2], x[3], x[4], x[5], x[6]])
+    jump = np.array([x[0], x[14], x[9], x[10], x[11], 0.0, x[12], x[13]])
+    return mean - 0.5 * jump, mean + 0.5 * jump, float(x[8]) + GAMMA_REF
 
 
 def admissible(ql, qr, F, gamma0):

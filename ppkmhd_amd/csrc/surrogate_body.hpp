@@ -10,11 +10,12 @@
 // use the similarity of ideal MHD under rho -> a rho, v -> b v, B -> sqrt(a) b B, p -> a b^2 p
 // (fluxes scale as a b, a b^2, a b^3, sqrt(a) b^2 for mass, momentum, energy, field): with
 // rho_bar = (rho_L + rho_R)/2 and s the LLF signal speed max(c_f + |u|) of the two states,
-//     x = [rho_L/rho_bar, u_L/s, v_L/s, w_L/s, Bx/(sqrt(rho_bar) s), By_L/(...), Bz_L/(...),
-//          p_L/(rho_bar s^2), rho_R/rho_bar, u_R/s, v_R/s, w_R/s, gamma - 5/3, By_R/(...),
-//          Bz_R/(...), p_R/(rho_bar s^2)]
-// Bx is the same on both sides (the solvers average it), so its right-hand slot carries the
-// adiabatic index instead: members of one ensemble differ in gamma.
+// the features are the dimensionless means and jumps of the two states,
+//     x = [(rho_R - rho_L)/rho_bar, mean u, v, w / s, Bx / (sqrt(rho_bar) s), mean By, Bz / (same),
+//          mean p / (rho_bar s^2), gamma - 5/3, jumps of u, v, w / s, of By, Bz / (sqrt(rho_bar) s),
+//          of p / (rho_bar s^2), 0]
+// (the same information as the two states: the mean density is the unit, Bx is one value
+// after the solvers' averaging, and members of one ensemble differ in gamma).
 //
 // A face uses the surrogate only if it passes the gate (density and pressure jumps below
 // thresholds) and its flux passes an a-posteriori admissibility check: the two states
@@ -59,29 +60,31 @@ PPK_HD void llf_parts(St<R> &ql, St<R> &qr, St<R> &ul, St<R> &ur, St<R> &fl, St<
   for (int v = 0; v < 8; ++v) F[v] = F[v] - s * (ur[v] - ul[v]) / R(2);
 }
 
-// dimensionless features of a face (after llf_parts: ql[IA] == qr[IA] == Bx)
+// dimensionless features of a face (after llf_parts: ql[IA] == qr[IA] == Bx): the means and
+// the jumps of the two states, so that the small jumps of smooth faces are not lost to the
+// bf16 rounding of the network inputs
 template <class R>
 PPK_HD void face_features(const St<R> &ql, const St<R> &qr, R s, R gamma0, float x[kMlpIn],
                           R &rho_bar) {
   rho_bar = R(0.5) * (ql[ID] + qr[ID]);
   const R iv = R(1) / s, ib = R(1) / (sqrt(rho_bar) * s), ip = R(1) / (rho_bar * s * s);
-  const R ir = R(1) / rho_bar;
-  x[0] = float(ql[ID] * ir);
-  x[1] = float(ql[IU] * iv);
-  x[2] = float(ql[IV] * iv);
-  x[3] = float(ql[IW] * iv);
+  const R ir = R(1) / rho_bar, h = R(0.5);
+  x[0] = float((qr[ID] - ql[ID]) * ir);
+  x[1] = float(h * (ql[IU] + qr[IU]) * iv);
+  x[2] = float(h * (ql[IV] + qr[IV]) * iv);
+  x[3] = float(h * (ql[IW] + qr[IW]) * iv);
   x[4] = float(ql[IA] * ib);
-  x[5] = float(ql[IB] * ib);
-  x[6] = float(ql[IC] * ib);
-  x[7] = float(ql[IP] * ip);
-  x[8] = float(qr[ID] * ir);
-  x[9] = float(qr[IU] * iv);
-  x[10] = float(qr[IV] * iv);
-  x[11] = float(qr[IW] * iv);
-  x[12] = float(gamma0 - R(kGammaRef));
-  x[13] = float(qr[IB] * ib);
-  x[14] = float(qr[IC] * ib);
-  x[15] = float(qr[IP] * ip);
+  x[5] = float(h * (ql[IB] + qr[IB]) * ib);
+  x[6] = float(h * (ql[IC] + qr[IC]) * ib);
+  x[7] = float(h * (ql[IP] + qr[IP]) * ip);
+  x[8] = float(gamma0 - R(kGammaRef));
+  x[9] = float((qr[IU] - ql[IU]) * iv);
+  x[10] = float((qr[IV] - ql[IV]) * iv);
+  x[11] = float((qr[IW] - ql[IW]) * iv);
+  x[12] = float((qr[IB] - ql[IB]) * ib);
+  x[13] = float((qr[IC] - ql[IC]) * ib);
+  x[14] = float((qr[IP] - ql[IP]) * ip);
+  x[15] = 0.f;
 }
 
 // physical scale of each output component

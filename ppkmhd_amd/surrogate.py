@@ -168,3 +168,38 @@ def counts(solver, reset=False):
     solver._check(lib().ppk_mhd_surrogate_counts(solver._h, C.byref(a), C.byref(b),
                                                  1 if reset else 0))
     return a.value, b.value
+
+
+def gated(X, gate_rho, gate_p):
+    """Mask of the samples whose face passes the gate of the face kernels (the relative jumps
+    are scale-free: x0 is the density jump over the mean density, x7 and x14 the mean and the
+    jump of the pressure in the same unit)."""
+    X = np.asarray(X, dtype=np.float64)
+    dr = np.abs(X[:, 0]) < gate_rho * (1.0 - 0.5 * np.abs(X[:, 0]))
+    dp = np.abs(X[:, 14]) < gate_p * (X[:, 7] - 0.5 * np.abs(X[:, 14]))
+    return dr & dp
+
+
+def select_gated(faceset, gate_rho, gate_p, chunk=1 << 22):
+    """A new FaceSet with the samples of `faceset` that pass the gate (the faces the network
+    will be asked about)."""
+    n = len(faceset)
+    keep_x, keep_t = [], []
+    for first in range(0, n, chunk):
+        X, T = faceset.read(first, min(chunk, n - first))
+        m = gated(X, gate_rho, gate_p)
+        keep_x.append(X[m])
+        keep_t.append(T[m])
+    X = np.concatenate(keep_x) if keep_x else np.zeros((0, N_IN), np.float32)
+    T = np.concatenate(keep_t) if keep_t else np.zeros((0, N_OUT), np.float32)
+    out = FaceSet(max(1, X.shape[0]))
+    if X.shape[0]:
+        out.append(X, T)
+    return out
+
+
+def train_schedule(net, faceset, steps, batch=4096, lr=1e-3, seed=1, phases=4):
+    """`steps` Adam steps in `phases` equal parts, the learning rate divided by 3 per part."""
+    per = max(1, steps // phases)
+    for k in range(phases):
+        net.train(faceset, per, batch=batch, lr=lr / 3 ** k, seed=seed + k)
