@@ -158,6 +158,7 @@ struct ppk_mhd_solver {
   ppk::FusedScratch fw;
   ppk::SurrogateCfg sur;                   // surrogate face solves (fw.sur points here when on)
   unsigned long long *sur_counts = nullptr;  // device: faces by the network, by HLLD
+  unsigned char *sur_flags = nullptr;        // device: faces left to the HLLD pass
   bool fused = false;
   // parts holds the CFL maxima of cur() (ghosts as the next fill makes them): set by a fused
   // CFL-mode step or compute_dt, cleared by anything that changes cur() behind the kernels
@@ -334,6 +335,7 @@ void free_all(ppk_mhd_t *h) {
   F(h->fw.flux);
   F(h->fw.emf);
   F(h->sur_counts);
+  F(h->sur_flags);
   delete h->timer;
   h->timer = nullptr;
   if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
@@ -578,9 +580,13 @@ int ppk_mhd_set_surrogate(ppk_mhd_t *h, const ppk_mlp_t *net, double gate_rho, d
   if (!(gate_rho >= 0.0) || !(gate_p >= 0.0))
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "gate thresholds must be >= 0");
   if (!h->sur_counts) {
-    HIP_CHECK(h, hipMalloc(&h->sur_counts, 2 * sizeof(unsigned long long)));
-    HIP_CHECK(h, hipMemsetAsync(h->sur_counts, 0, 2 * sizeof(unsigned long long), h->stream));
+    const size_t bytes = 2 * ppk::kSurrogateBlocks * sizeof(unsigned long long);
+    HIP_CHECK(h, hipMalloc(&h->sur_counts, bytes));
+    HIP_CHECK(h, hipMemsetAsync(h->sur_counts, 0, bytes, h->stream));
   }
+  if (!h->sur_flags)
+    HIP_CHECK(h, hipMalloc(&h->sur_flags, (size_t)h->dp.dim * h->cells()));
+  h->sur.flags = h->sur_flags;
   h->sur.net = net->packed;
   h->sur.gate_rho = gate_rho;
   h->sur.gate_p = gate_p;
@@ -593,10 +599,17 @@ int ppk_mhd_surrogate_counts(ppk_mhd_t *h, int64_t *by_network, int64_t *by_hlld
   if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
   unsigned long long c[2] = {0, 0};
   if (h->sur_counts) {
-    HIP_CHECK(h, hipMemcpyAsync(c, h->sur_counts, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    std::vector<unsigned long long> per(2 * ppk::kSurrogateBlocks);
+    const size_t bytes = per.size() * sizeof(unsigned long long);
+    HIP_CHECK(h, hipMemcpyAsync(per.data(), h->sur_counts, bytes, hipMemcpyDeviceToHost,
+                                h->stream));
     HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    for (int b = 0; b < ppk::kSurrogateBlocks; ++b) {
+      c[0] += per[2 * b];
+      c[1] += per[2 * b + 1];
+    }
     if (reset) {
-      HIP_CHECK(h, hipMemsetAsync(h->sur_counts, 0, sizeof(c), h->stream));
+      HIP_CHECK(h, hipMemsetAsync(h->sur_counts, 0, bytes, h->stream));
       HIP_CHECK(h, hipStreamSynchronize(h->stream));
     }
   }

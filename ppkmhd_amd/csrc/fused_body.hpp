@@ -65,8 +65,8 @@ PPK_HD void store_rec3(R *__restrict__ rec, long long vs, long long t, const Hal
 
 // left-side fields of the record at t; the right-side ones are set by the caller from the +1
 // neighbours where a state needs them
-template <class R>
-PPK_HD void load_rec3(const R *__restrict__ rec, long long vs, long long t, Half3D<R> &h) {
+template <class R, class T>
+PPK_HD void load_rec3(const R *__restrict__ rec, long long vs, long long t, Half3D<T> &h) {
   const R *a = rec + t;
   h.r = a[R3_R * vs];     h.pp = a[R3_P * vs];    h.u = a[R3_U * vs];     h.v = a[R3_V * vs];
   h.w = a[R3_W * vs];     h.A = a[R3_A * vs];     h.B = a[R3_B * vs];     h.C = a[R3_C * vs];
@@ -79,8 +79,8 @@ PPK_HD void load_rec3(const R *__restrict__ rec, long long vs, long long t, Half
   h.AL = a[R3_AL * vs];   h.BL = a[R3_BL * vs];   h.CL = a[R3_CL * vs];
   h.dALy = a[R3_DALY * vs]; h.dALz = a[R3_DALZ * vs]; h.dBLx = a[R3_DBLX * vs];
   h.dBLz = a[R3_DBLZ * vs]; h.dCLx = a[R3_DCLX * vs]; h.dCLy = a[R3_DCLY * vs];
-  h.AR = h.BR = h.CR = R(0);
-  h.dARy = h.dARz = h.dBRx = h.dBRz = h.dCRx = h.dCRy = R(0);
+  h.AR = h.BR = h.CR = T(0);
+  h.dARy = h.dARz = h.dBRx = h.dBRz = h.dCRx = h.dCRy = T(0);
 }
 
 template <class R>
@@ -96,8 +96,8 @@ PPK_HD void store_rec2(R *__restrict__ rec, long long vs, long long t, const Hal
   o[R2_DALY * vs] = h.dALy; o[R2_DBLX * vs] = h.dBLx;
 }
 
-template <class R>
-PPK_HD void load_rec2(const R *__restrict__ rec, long long vs, long long t, Half2D<R> &h) {
+template <class R, class T>
+PPK_HD void load_rec2(const R *__restrict__ rec, long long vs, long long t, Half2D<T> &h) {
   const R *a = rec + t;
   h.r = a[R2_R * vs];     h.pp = a[R2_P * vs];    h.u = a[R2_U * vs];     h.v = a[R2_V * vs];
   h.w = a[R2_W * vs];     h.A = a[R2_A * vs];     h.B = a[R2_B * vs];     h.C = a[R2_C * vs];
@@ -107,8 +107,8 @@ PPK_HD void load_rec2(const R *__restrict__ rec, long long vs, long long t, Half
   h.dwy = a[R2_DWY * vs]; h.dAy = a[R2_DAY * vs]; h.dCy = a[R2_DCY * vs];
   h.AL = a[R2_AL * vs];   h.BL = a[R2_BL * vs];
   h.dALy = a[R2_DALY * vs]; h.dBLx = a[R2_DBLX * vs];
-  h.AR = h.BR = R(0);
-  h.dARy = h.dBRx = R(0);
+  h.AR = h.BR = T(0);
+  h.dARy = h.dBRx = T(0);
 }
 
 // ===========================================================================================
@@ -459,12 +459,12 @@ PPK_HD void swap_dir(St<R> &s, int a, int b) {
 // face field is the left one of c) against qp of cell c, rotated so that the normal is x,
 // then the 1D Riemann solver; the 5 (3D) or 6 (2D) flux components are stored rotated.
 // Reference src/muscl/MHDRunFunctors3D.h:1839-1902 and MHDRunFunctors2D.h.
-template <int D, class R>
-PPK_HD void face_states3(const R *__restrict__ rec, const DevParams &p, long long c, St<R> &ql,
-                         St<R> &qr) {
+template <int D, class R, class T>
+PPK_HD void face_states3(const R *__restrict__ rec, const DevParams &p, long long c, St<T> &ql,
+                         St<T> &qr) {
   const long long vs = p.vstride;
   const long long sd = D == 0 ? 1 : (D == 1 ? (long long)p.isize : p.plane);
-  Half3D<R> hl, hr;
+  Half3D<T> hl, hr;
   load_rec3(rec, vs, c - sd, hl);
   load_rec3(rec, vs, c, hr);
   if (D == 0) hl.AR = rec[c + R3_AL * vs];
@@ -547,12 +547,12 @@ PPK_HD void edge3(const R *__restrict__ rec, R *__restrict__ emf, const DevParam
 
 // 2D faces (D = 0, 1) and the z edge, reference src/muscl/MHDRunFunctors2D.h
 // ComputeFluxesAndStoreFunctor and ComputeEmfAndStoreFunctor (:654-680)
-template <int D, class R>
-PPK_HD void face_states2(const R *__restrict__ rec, const DevParams &p, long long c, St<R> &ql,
-                         St<R> &qr) {
+template <int D, class R, class T>
+PPK_HD void face_states2(const R *__restrict__ rec, const DevParams &p, long long c, St<T> &ql,
+                         St<T> &qr) {
   const long long vs = p.vstride;
   const long long sd = D == 0 ? 1 : (long long)p.isize;
-  Half2D<R> hl, hr;
+  Half2D<T> hl, hr;
   load_rec2(rec, vs, c - sd, hl);
   load_rec2(rec, vs, c, hr);
   if (D == 0) hl.AR = rec[c + R2_AL * vs];

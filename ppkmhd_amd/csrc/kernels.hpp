@@ -67,10 +67,14 @@ constexpr int kRec2 = 26;   // half-step record per cell, 2D
 constexpr int kFluxComp3 = 5;  // ID IP IU IV IW per face direction, 3D
 constexpr int kFluxComp2 = 6;  // ID IP IU IV IW IC per face direction, 2D
 // the surrogate form of the face solves: network, gate, counters (device memory)
+constexpr int kSurrogateBlocks = 512;  // blocks of a surrogate face launch (at most)
 struct SurrogateCfg {
-  const void *net = nullptr;               // packed network (MlpPacked, device)
-  double gate_rho = 0.0, gate_p = 0.0;     // max relative jumps |a_R - a_L| / min(a_L, a_R)
-  unsigned long long *counters = nullptr;  // [0] faces by the network, [1] by HLLD; or null
+  const void *net = nullptr;            // packed network (MlpPacked, device)
+  double gate_rho = 0.0, gate_p = 0.0;  // max relative jumps |a_R - a_L| / min(a_L, a_R)
+  // per launch block b: [2b] faces by the network, [2b + 1] by HLLD (2 x kSurrogateBlocks
+  // counters, accumulated over launches), or null
+  unsigned long long *counters = nullptr;
+  unsigned char *flags = nullptr;  // dim x cells: faces left to the HLLD pass (device scratch)
 };
 // element type: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {

@@ -30,13 +30,9 @@ __device__ __forceinline__ v4f mfma_k32(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ short bf16s(float x) { return (short)bf16_bits(x); }
-__device__ __forceinline__ __bf16 bf16b(float x) {
-  const uint16_t u = bf16_bits(x);
-  __bf16 b;
-  __builtin_memcpy(&b, &u, 2);
-  return b;
-}
+// bf16 round to nearest even in hardware (v_cvt_pk_bf16_f32; the same rounding as bf16_bits)
+__device__ __forceinline__ __bf16 bf16b(float x) { return static_cast<__bf16>(x); }
+__device__ __forceinline__ short bf16s(float x) { return __builtin_bit_cast(short, bf16b(x)); }
 __device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
 
 // the pieces of a packed network the wave functions read (LDS or global)
@@ -53,9 +49,19 @@ __device__ __forceinline__ MlpView view_of(const MlpPacked *w) {
 // Hide the weight pointers from loop-invariant code motion: every 16-sample block reloads its
 // operands (cache / LDS hits) instead of the compiler holding all ~40 KB of weight operands of
 // a wave in registers across the loop.
+// The pointers move by an opaque zero rather than being hidden themselves, so the compiler
+// still sees where they point (LDS weights stay ds_read, not flat loads).
 __device__ __forceinline__ MlpView opaque(MlpView w) {
-  asm volatile("" : "+s"(w.A1), "+s"(w.A2), "+s"(w.A3), "+s"(w.A3T), "+s"(w.A2T));
-  asm volatile("" : "+s"(w.b1), "+s"(w.b2), "+s"(w.b3));
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  w.A1 += z;
+  w.A2 += z;
+  w.A3 += z;
+  if (w.A3T) w.A3T += z;
+  if (w.A2T) w.A2T += z;
+  w.b1 += z;
+  w.b2 += z;
+  w.b3 += z;
   return w;
 }
 
@@ -136,21 +142,85 @@ __device__ __forceinline__ void mlp_forward16(const MlpView &w, v4s xb, v8bf h1b
 
 // Inference of the 64 samples of a wave: sX holds their standardised bf16 features
 // ([sample][16], this wave's slice of LDS), sY receives the 8 outputs per sample
-// ([sample][8] floats).  Every lane of the wave must call it.
-__device__ __forceinline__ void mlp_infer_wave(const MlpView &w, const uint16_t *sX, float *sY) {
+// ([sample][8] floats).  Every lane of the wave must call it.  NB blocks of 16 samples share
+// each weight operand load (NB = 4: every A operand is read once per wave).
+template <int NB>
+__device__ __forceinline__ void mlp_infer_blocks(const MlpView &w, const uint16_t *sX, float *sY,
+                                                 int f0) {
   const int lane = lane_id(), g = lane >> 4, col = lane & 15;
-#pragma unroll 1
-  for (int f = 0; f < 4; ++f) {
-    const v4s xb = *reinterpret_cast<const v4s *>(sX + (16 * f + col) * 16 + 4 * g);
-    v8bf h1b[4], h2b[4];
-    unsigned m1, m2;
-    v4f y;
-    mlp_forward16(opaque(w), xb, h1b, h2b, m1, m2, y);
-    if (g < 2) {
+  v4s xb[NB];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) sY[(16 * f + col) * 8 + 4 * g + j] = y[j];
+  for (int f = 0; f < NB; ++f)
+    xb[f] = *reinterpret_cast<const v4s *>(sX + (16 * (f0 + f) + col) * 16 + 4 * g);
+  v8bf h1b[NB][4], h2b[NB][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v4f t[NB][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 2 * q + h;
+      v4f bias;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bias[j] = w.b1[16 * r + 4 * g + j];
+      const v4s a = ld_a16(w.A1, r, lane);
+#pragma unroll
+      for (int f = 0; f < NB; ++f) {
+        const v4f c = mfma_k16(a, xb[f], bias);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[f][h][j] = relu(c[j]);
+      }
     }
+#pragma unroll
+    for (int f = 0; f < NB; ++f) h1b[f][q] = pack_pair(t[f][0], t[f][1]);
   }
+#pragma unroll
+  for (int q2 = 0; q2 < 4; ++q2) {
+    v4f t[NB][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 2 * q2 + h;
+      v4f acc[NB];
+#pragma unroll
+      for (int f = 0; f < NB; ++f)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[f][j] = w.b2[16 * r + 4 * g + j];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const v8bf a = ld_a32(w.A2, r * 4 + q, lane);
+#pragma unroll
+        for (int f = 0; f < NB; ++f) acc[f] = mfma_k32(a, h1b[f][q], acc[f]);
+      }
+#pragma unroll
+      for (int f = 0; f < NB; ++f)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[f][h][j] = relu(acc[f][j]);
+    }
+#pragma unroll
+    for (int f = 0; f < NB; ++f) h2b[f][q2] = pack_pair(t[f][0], t[f][1]);
+  }
+  v4f y[NB];
+#pragma unroll
+  for (int f = 0; f < NB; ++f)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) y[f][j] = w.b3[4 * g + j];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const v8bf a = ld_a32(w.A3, q, lane);
+#pragma unroll
+    for (int f = 0; f < NB; ++f) y[f] = mfma_k32(a, h2b[f][q], y[f]);
+  }
+  if (g < 2) {
+#pragma unroll
+    for (int f = 0; f < NB; ++f)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sY[(16 * (f0 + f) + col) * 8 + 4 * g + j] = y[f][j];
+  }
+}
+
+template <int NB = 4>
+__device__ __forceinline__ void mlp_infer_wave(const MlpView &w, const uint16_t *sX, float *sY) {
+#pragma unroll 1
+  for (int f0 = 0; f0 < 4; f0 += NB) mlp_infer_blocks<NB>(opaque(w), sX, sY, f0);
   wave_lds_sync();
 }
 

@@ -1,11 +1,12 @@
 // Surrogate form of the face solves, and the face-sample dump its network is trained on.
 //
 //   k_face_sur<DIM, D>  the face fluxes of direction D with the learned flux where the gate and
-//                       the admissibility check allow, exact HLLD elsewhere (per face).  A
-//                       block of 256 threads stages the network (40 KB of bf16 weights) in LDS
-//                       once and then walks bricks of the face sub-box; each wavefront runs the
-//                       network on the matrix cores for its 64 faces when any of them passed
-//                       the gate, and HLLD for the faces that did not or whose flux was refused.
+//                       the admissibility check allow (per face).  A block of 256 threads
+//                       stages the network (40 KB of bf16 weights) in LDS once and then walks
+//                       bricks of the face sub-box; each wavefront runs the network on the
+//                       matrix cores for its 64 faces when any of them passed the gate, and
+//                       marks the faces that did not or whose flux was refused
+//   k_face_fallback     exact HLLD for the marked faces
 //   k_dump<DIM>         features and training targets of every stride-th face (all directions)
 //
 // The face states are those of the plain fused step (face_states2/3), so a face that falls
@@ -24,12 +25,16 @@ using namespace fusedk;
 using namespace mlpd;
 
 constexpr int kBlock = 256;
+#ifndef PPK_SUR_NB
+#define PPK_SUR_NB 2
+#endif
+constexpr int kNB = PPK_SUR_NB;  // 16-face blocks per weight-operand load
 constexpr int kFloatBlock = 128 + 128 + 16 + 16 + 16 + 8 + 8;  // b1 b2 b3 mu_in isig_in mu_out sig_out
 
 __device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
 
-template <int DIM, int D, class R>
-PPK_HD void states(const R *rec, const DevParams &p, long long c, St<R> &ql, St<R> &qr) {
+template <int DIM, int D, class R, class T>
+PPK_HD void states(const R *rec, const DevParams &p, long long c, St<T> &ql, St<T> &qr) {
   if constexpr (DIM == 3)
     face_states3<D>(rec, p, c, ql, qr);
   else
@@ -50,8 +55,10 @@ __global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
   if (!active(ctrl)) return;
   __shared__ __attribute__((aligned(16))) uint16_t sW[kPackA1 + kPackA2 + kPackA3];
   __shared__ float sFl[kFloatBlock];
+  // per wave: the bf16 features of its 64 faces, then (same bytes, 32 per face) the outputs:
+  // a pass of the network reads the features of its 16-face blocks before it writes their
+  // outputs, and later passes read only later blocks
   __shared__ __attribute__((aligned(16))) uint16_t sXall[4][64 * 16];
-  __shared__ float sYall[4][64 * 8];
   const MlpPacked *net = static_cast<const MlpPacked *>(cfg.net);
   {  // stage the inference weights: A1, A2, A3 are the first kMlpInferBytes of the net
     const uint4 *src = reinterpret_cast<const uint4 *>(net);
@@ -75,60 +82,98 @@ __global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
   w.sig_out = sFl + 312;
   const int wave = threadIdx.x >> 6, lane = lane_id();
   uint16_t *sX = sXall[wave];
-  float *sY = sYall[wave];
-  const R gamma0 = R(p.gamma0);
+  float *sY = reinterpret_cast<float *>(sXall[wave]);
+  const float gamma0 = float(p.gamma0);
   const long long nb = bk.nblocks;
+  unsigned long long n_net = 0, n_all = 0;  // faces of this wave by the network, in total
 #pragma unroll 1
   for (long long L = blockIdx.x; L < nb; L += gridDim.x) {
     const long long c = sub_brick_cell(p, bk, (int)L, threadIdx.x);
-    St<R> ql, qr, F;
-    R s = R(1);
+    // the face states, the LLF pieces, the features and the check in float: their rounding
+    // is far below the network's own error; faces left to HLLD are recomputed in R there
+    St<float> a, b;
+    St<R> F;
     bool gate = false;
     if (c >= 0) {
-      states<DIM, D>(rec, p, c, ql, qr);
-      St<R> ul, ur, fl, fr;
-      llf_parts(ql, qr, ul, ur, fl, fr, s, F, p);
-      gate = smooth_face(ql, qr, cfg.gate_rho, cfg.gate_p);
+      states<DIM, D>(rec, p, c, a, b);
+      gate = smooth_face(a, b, cfg.gate_rho, cfg.gate_p);
     }
     bool use = false;
-    if (__ballot(gate) != 0ull) {  // wave-uniform: the network runs for the whole wavefront
+    if (__ballot(gate) != 0ull) {  // the network runs for the whole wavefront (wave-uniform)
+      St<float> Fl;
+      float s = 1.f, rb = 1.f;
       float x[kMlpIn];
-      R rb = R(1);
       if (gate) {
-        face_features(ql, qr, s, gamma0, x, rb);
+        St<float> ul, ur, fl, fr;
+        llf_parts(a, b, ul, ur, fl, fr, s, Fl, p);
+        face_features(a, b, s, gamma0, x, rb);
       } else {
 #pragma unroll
         for (int i = 0; i < kMlpIn; ++i) x[i] = w.mu_in[i];
       }
 #pragma unroll
-      for (int i = 0; i < kMlpIn; ++i) sX[lane * 16 + i] = bf16_bits((x[i] - w.mu_in[i]) * w.isig_in[i]);
+      for (int i = 0; i < kMlpIn; ++i)
+        sX[lane * 16 + i] = __builtin_bit_cast(uint16_t, bf16b((x[i] - w.mu_in[i]) * w.isig_in[i]));
       wave_lds_sync();
-      mlp_infer_wave(w, sX, sY);
+      mlp_infer_wave<kNB>(w, sX, sY);
       if (gate) {
-        R sc[kMlpOut];
+        float sc[kMlpOut];
         out_scales(rb, s, sc);
-        St<R> Fs;
+        St<float> Fs, ul, ur, fl, fr;
 #pragma unroll
         for (int o = 0; o < kMlpOut; ++o)
-          Fs[o] = F[o] + sc[o] * R(w.mu_out[o] + w.sig_out[o] * sY[lane * 8 + o]);
-        St<R> ul, ur, fl, fr;
-        mhd_flux(ql, ul, fl, p);
-        mhd_flux(qr, ur, fr, p);
+          Fs[o] = Fl[o] + sc[o] * (w.mu_out[o] + w.sig_out[o] * sY[lane * 8 + o]);
+        mhd_flux(a, ul, fl, p);
+        mhd_flux(b, ur, fr, p);
         use = admissible(ul, ur, fl, fr, Fs, s, gamma0);
-        if (use) F = Fs;
+        if (use) {
+#pragma unroll
+          for (int o = 0; o < 8; ++o) F[o] = R(Fs[o]);
+        }
       }
     }
-    if (c >= 0 && !use) riemann_hlld(ql, qr, F, p);
-    if (c >= 0) store<DIM, D>(flux, p, c, F);
-    if (cfg.counters) {
-      const unsigned long long nsur = __popcll(__ballot(c >= 0 && use));
-      const unsigned long long nall = __popcll(__ballot(c >= 0));
-      if (lane == 0 && nall) {
-        atomicAdd(&cfg.counters[0], nsur);
-        atomicAdd(&cfg.counters[1], nall - nsur);
+    // the network's flux, or a mark for the HLLD pass (which keeps this kernel's registers
+    // free of the HLLD solver)
+    if (c >= 0) {
+      if (use) store<DIM, D>(flux, p, c, F);
+      cfg.flags[(long long)D * p.vstride + c] = use ? 0 : 1;
+    }
+    n_net += __popcll(__ballot(c >= 0 && use));
+    n_all += __popcll(__ballot(c >= 0));
+  }
+  // per-block counts (no same-address atomics): this block's slots, in stream order
+  if (cfg.counters) {
+    __shared__ unsigned long long cnt[2][kBlock / 64];
+    if (lane == 0) {
+      cnt[0][wave] = n_net;
+      cnt[1][wave] = n_all;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long a = 0, b = 0;
+      for (int i = 0; i < kBlock / 64; ++i) {
+        a += cnt[0][i];
+        b += cnt[1][i];
       }
+      cfg.counters[2 * blockIdx.x] += a;
+      cfg.counters[2 * blockIdx.x + 1] += b - a;
     }
   }
+}
+
+// exact HLLD for the faces of direction D that the surrogate pass marked
+template <int DIM, int D, class R>
+__global__ void __launch_bounds__(kBlock) k_face_fallback(const R *__restrict__ rec,
+                                                          R *__restrict__ flux, DevParams p,
+                                                          Brick bk, const double *ctrl,
+                                                          const unsigned char *flags) {
+  if (!active(ctrl)) return;
+  const long long c = sub_brick_cell(p, bk, (int)blockIdx.x, threadIdx.x);
+  if (c < 0 || !flags[(long long)D * p.vstride + c]) return;
+  St<R> ql, qr, F;
+  states<DIM, D>(rec, p, c, ql, qr);
+  riemann_hlld(ql, qr, F, p);
+  store<DIM, D>(flux, p, c, F);
 }
 
 // one thread per selected face: slot s is face t = s * stride + phase of the enumeration
@@ -180,14 +225,21 @@ hipError_t faces_impl(const DevParams &p, const R *rec, R *flux, const double *c
                       const SurrogateCfg &cfg, hipStream_t s) {
   const Brick bk = sub_bricks(p);
   const long long nb = bk.count();
-  const dim3 g((unsigned)(nb < 512 ? nb : 512)), b(kBlock);
+  const dim3 g((unsigned)(nb < kSurrogateBlocks ? nb : kSurrogateBlocks)), b(kBlock);
+  const dim3 gf((unsigned)nb);
+  const unsigned char *fl = cfg.flags;
   if (p.dim == 3) {
     hipLaunchKernelGGL((k_face_sur<3, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
     hipLaunchKernelGGL((k_face_sur<3, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
     hipLaunchKernelGGL((k_face_sur<3, 2, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+    hipLaunchKernelGGL((k_face_fallback<3, 0, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
+    hipLaunchKernelGGL((k_face_fallback<3, 1, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
+    hipLaunchKernelGGL((k_face_fallback<3, 2, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
   } else {
     hipLaunchKernelGGL((k_face_sur<2, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
     hipLaunchKernelGGL((k_face_sur<2, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+    hipLaunchKernelGGL((k_face_fallback<2, 0, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
+    hipLaunchKernelGGL((k_face_fallback<2, 1, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
   }
   return hipGetLastError();
 }

@@ -148,8 +148,9 @@ def test_gate_zero_is_plain_hlld(gpu, precision):
 
 
 def test_zero_network_is_the_llf_scheme(oracle, gpu):
-    """A network that outputs 0 with an open gate gives F_LLF at every face: the strict step
-    then equals the oracle's step with riemann = LLF bit for bit (edges stay 2D HLLD)."""
+    """A network that outputs 0 with an open gate gives F_LLF at every face (its pieces in
+    float): the step then follows the oracle's step with riemann = LLF (edges stay 2D HLLD) to
+    float rounding of the face fluxes."""
     from ppkmhd_amd.surrogate import MLP, attach, counts
     p, pr = ot(40, A.FP64_STRICT)
     P = np.zeros(S().N_PARAMS, dtype=np.float32)
@@ -165,7 +166,9 @@ def test_zero_network_is_the_llf_scheme(oracle, gpu):
     for _ in range(8):
         g.step(0, A.DT_CFL)
         o.step(0, A.DT_CFL)
-    assert np.array_equal(g.download(), o.get_state())
+    e = rel_linf(g.download(), o.get_state(), q)
+    print(f"zero network vs oracle LLF scheme after 8 steps: relative L-inf {e:.2e}")
+    assert e <= 1e-5
     n_net, n_hlld = counts(g)
     assert n_hlld == 0 and n_net == 8 * 2 * 41 * 41
 
