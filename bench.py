@@ -208,24 +208,20 @@ def bench_ensemble(a):
     print(json.dumps(out), flush=True)
 
 
-def bench_surrogate(a):
-    """Surrogate form, 2D: train the network on faces dumped from the solver's own runs (the
-    Orszag-Tang 256^2 run to t = 0.5, ensemble members, a 3D blast), then run Orszag-Tang at
-    n^2 to t = 0.5 with plain HLLD and with the gated surrogate, and compare time and density."""
+def train_surrogate(a):
+    """The surrogate network trained on faces dumped from the solver's own runs (the
+    Orszag-Tang 256^2 run to t = 0.5, ensemble members, a 3D blast), restricted to the faces
+    that pass the gate; returns (network, statistics)."""
     from ppkmhd_amd.ensemble import ot_ensemble
     from ppkmhd_amd.solver import SolverMHDMuscl
     from ppkmhd_amd import surrogate as SG
 
-    def ot(n, tEnd):
-        p = make_params(2, n, n, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
-                        bc=A.BC_PERIODIC, tEnd=tEnd, n_step_max=1 << 30)
-        return p, default_problem(A.PROBLEM_ORSZAG_TANG, p)
-
     t0 = time.perf_counter()
     fs = SG.FaceSet(a.samples + (1 << 22))
     # Orszag-Tang 256^2 to t = 0.5, every 4th step
-    p, pr = ot(256, 0.5)
-    s = SolverMHDMuscl(p, pr)
+    p = make_params(2, 256, 256, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0,
+                    riemann=A.RIEMANN_HLLD, bc=A.BC_PERIODIC, tEnd=0.5, n_step_max=1 << 30)
+    s = SolverMHDMuscl(p, default_problem(A.PROBLEM_ORSZAG_TANG, p))
     faces = 2 * 257 * 257
     stride = max(1, -(-faces * 160 // max(1, a.samples // 2)))  # ~160 snapshots to t = 0.5
     while not s.finished() and len(fs) + faces // stride + 1 <= a.samples // 2:
@@ -276,16 +272,28 @@ def bench_surrogate(a):
     mse1 = net.evaluate(train_set).mean()
     log(f"[bench] training: {a.train_steps} steps of 4096 in {t_train:.1f} s, standardised MSE "
         f"{mse0:.3e} -> {mse1:.3e}")
+    stats = {"samples_dumped": n_all, "samples_gated": len(train_set), "steps": a.train_steps,
+             "batch": 4096, "seconds": t_train, "mse_standardised_before": mse0,
+             "mse_standardised_after": mse1}
+    train_set.close()
+    return net, stats
 
-    n = a.n if a.n != 256 else 1024
+
+def run_pair(a, make, n_cells, net):
+    """The same run with plain HLLD and with the gated surrogate: (states, seconds, steps,
+    counts) per form; the timed part excludes `a.warmup` steps."""
+    from ppkmhd_amd.solver import SolverMHDMuscl
+    from ppkmhd_amd import surrogate as SG
+
     runs = {}
     for name in ("plain", "surrogate"):
-        p, pr = ot(n, 0.5)
+        p, pr = make()
         s = SolverMHDMuscl(p, pr)
         if name == "surrogate":
             SG.attach(s, net, a.gate, a.gate)
         s.run(a.warmup)
         s.synchronize()
+        SG.counts(s, reset=True)
         t0 = time.perf_counter()
         while not s.finished():
             s.run(100)
@@ -295,8 +303,23 @@ def bench_surrogate(a):
         steps = it - a.warmup
         runs[name] = (s.download(), el, steps, SG.counts(s) if name == "surrogate" else None)
         log(f"[bench] {name}: {steps} steps to t={t:.4f} in {el:.3f} s, "
-            f"{el / steps * 1e3:.3f} ms/step, {n * n * steps / el / 1e9:.3f} G cell-updates/s")
+            f"{el / steps * 1e3:.3f} ms/step, {n_cells * steps / el / 1e9:.3f} G cell-updates/s")
         s.close()
+    return runs
+
+
+def bench_surrogate(a):
+    """Surrogate form, 2D (config 3): Orszag-Tang at n^2 to t = 0.5 with plain HLLD and with
+    the gated surrogate; time and density L1 against plain."""
+    net, stats = train_surrogate(a)
+
+    def ot(n, tEnd):
+        p = make_params(2, n, n, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
+                        bc=A.BC_PERIODIC, tEnd=tEnd, n_step_max=1 << 30)
+        return p, default_problem(A.PROBLEM_ORSZAG_TANG, p)
+
+    n = a.n if a.n != 256 else 1024
+    runs = run_pair(a, lambda: ot(n, 0.5), n * n, net)
     Up, ep, sp, _ = runs["plain"]
     Us, es, ss, (by_net, by_hlld) = runs["surrogate"]
     p, _ = ot(n, 0.5)
@@ -315,9 +338,58 @@ def bench_surrogate(a):
                                f"with admissibility fallback to HLLD", "precision": "fp64"},
         "plain_value": v_plain, "speed_ratio": v_sur / v_plain, "density_l1_vs_plain": l1,
         "faces_by_network": by_net / max(1, by_net + by_hlld),
-        "training": {"samples_dumped": n_all, "samples_gated": len(train_set),
-                     "steps": a.train_steps, "batch": 4096, "seconds": t_train,
-                     "mse_standardised_before": mse0, "mse_standardised_after": mse1},
+        "training": stats,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_surrogate3d(a):
+    """Surrogate form, 3D (config 4): MHD Kelvin-Helmholtz at n^3 (default 512^3) on
+    [-1/2, 1/2]^3, FP32 bulk with FP64 state and CT, 300 steps, plain HLLD against the gated
+    surrogate; time and density L1 against plain."""
+    net, stats = train_surrogate(a)
+    n = a.n if a.n != 256 else 512
+    steps = 300
+
+    def kh():
+        p = make_params(3, n, n, n, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5,
+                        zmax=0.5, gamma0=1.4, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
+                        bc=A.BC_PERIODIC, tEnd=1.0e9, n_step_max=steps + a.warmup,
+                        precision=A.FP32_BULK)
+        pr = default_problem(A.PROBLEM_KELVIN_HELMHOLTZ, p)
+        pr.kh_p_rand, pr.kh_p_sine, pr.kh_p_sine_rob = 1, 0, 0
+        pr.kh_axis = 1                       # shear layer normal to y
+        pr.kh_d_in, pr.kh_d_out = 2.0, 1.0   # rho = 2 for |y| <= 0.25, 1 outside
+        pr.kh_vflow_in, pr.kh_vflow_out = -0.5, 0.5
+        pr.kh_pressure = 2.5
+        pr.kh_amplitude = 0.01
+        pr.kh_seed = a.seed
+        pr.kh_bx, pr.kh_by, pr.kh_bz = 0.1, 0.0, 0.0
+        return p, pr
+
+    t0 = time.perf_counter()
+    runs = run_pair(a, kh, n ** 3, net)
+    log(f"[bench] both runs (with initial conditions) {time.perf_counter() - t0:.1f} s")
+    Up, ep, sp, _ = runs["plain"]
+    Us, es, ss, (by_net, by_hlld) = runs["surrogate"]
+    p, _ = kh()
+    ii = p.interior()
+    l1 = float(np.abs(Us[ii][0] - Up[ii][0]).sum() / np.abs(Up[ii][0]).sum())
+    v_plain, v_sur = n ** 3 * sp / ep, n ** 3 * ss / es
+    log(f"[bench] density L1 (surrogate vs plain HLLD, {steps} steps): {l1:.3e}; faces by "
+        f"network {by_net / max(1, by_net + by_hlld):.1%}; speed ratio {v_sur / v_plain:.3f}")
+    out = {
+        "metric": "MHD cell-updates/s (surrogate + fallback, 3D Kelvin-Helmholtz, FP32 bulk)",
+        "value": v_sur, "unit": "cell-updates/s", "n_gpus": 1, "steps": ss, "warmup": a.warmup,
+        "ms_per_step": es / ss * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 bulk, f64 state and CT; bf16 MLP with f32 accumulation",
+        "data": "synthetic (seeded Kelvin-Helmholtz layer; training faces from the solver's runs)",
+        "config": {"workload": f"MHD Kelvin-Helmholtz {n}^3, {steps} steps, gated MLP flux "
+                               f"(gate {a.gate}) with admissibility fallback to HLLD",
+                   "precision": "fp32_bulk"},
+        "plain_value": v_plain, "speed_ratio": v_sur / v_plain, "density_l1_vs_plain": l1,
+        "faces_by_network": by_net / max(1, by_net + by_hlld), "training": stats,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -335,13 +407,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the state (profiler calibration)")
-    ap.add_argument("--workload", choices=("blast3d", "ensemble2d", "surrogate2d"),
+    ap.add_argument("--workload",
+                    choices=("blast3d", "ensemble2d", "surrogate2d", "surrogate3d"),
                     default="blast3d",
                     help="blast3d: the 3D blast step (default); ensemble2d: batched ensemble; "
-                         "surrogate2d: learned flux on Orszag-Tang")
+                         "surrogate2d / surrogate3d: learned flux on Orszag-Tang 2D / "
+                         "Kelvin-Helmholtz 3D")
     ap.add_argument("--samples", type=int, default=20_000_000)
-    ap.add_argument("--train-steps", type=int, default=20000)
-    ap.add_argument("--gate", type=float, default=0.05)
+    ap.add_argument("--train-steps", type=int, default=300000)
+    ap.add_argument("--gate", type=float, default=0.005)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tune-steps", type=int, default=3)
@@ -356,6 +430,8 @@ def main():
         return bench_ensemble(a)
     if a.workload == "surrogate2d":
         return bench_surrogate(a)
+    if a.workload == "surrogate3d":
+        return bench_surrogate3d(a)
 
     from ppkmhd_amd.solver import SolverMHDMuscl
 
