@@ -166,6 +166,16 @@ const char *ppk_mhd_last_error(const ppk_mhd_t *handle);
  * ghost fill, as SolverMHDMuscl ctor does (src/muscl/SolverMHDMuscl.h:389-402).
  * Resets t = params.tCurrent (0), iteration = 0. */
 int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem);
+/* The same with the initial state computed on the GPU (one thread per cell; no host array,
+ * no PCIe transfer).  The cells' sin / cos / exp come from the device math library, so a
+ * state can differ from ppk_mhd_init_problem's in the last bits where those appear.
+ * Replaces: the reference's init functors run on the device (src/muscl/MHDInitFunctors2D.h,
+ * src/muscl/MHDInitFunctors3D.h). */
+int ppk_mhd_init_problem_device(ppk_mhd_t *h, const ppk_mhd_problem *problem);
+/* The initial state of a problem computed on the host into U (count = 8*isize*jsize*ksize),
+ * no GPU needed: what ppk_mhd_init_problem uploads. */
+int ppk_mhd_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem *problem,
+                          double *U, size_t count);
 
 /* Host <-> device copies of the current state in the reference layout (count =
  * 8*isize*jsize*ksize doubles).  download synchronizes the handle's stream. */

@@ -30,6 +30,8 @@ SIGNATURES = {
     "ppk_mhd_destroy": (None, [_H]),
     "ppk_mhd_last_error": (C.c_char_p, [_H]),
     "ppk_mhd_init_problem": (C.c_int, [_H, _PR]),
+    "ppk_mhd_init_problem_device": (C.c_int, [_H, _PR]),
+    "ppk_mhd_initial_state": (C.c_int, [_P, _PR, _DP, C.c_size_t]),
     "ppk_mhd_upload": (C.c_int, [_H, _DP, C.c_size_t]),
     "ppk_mhd_download": (C.c_int, [_H, _DP, C.c_size_t]),
     "ppk_mhd_device_state": (C.c_int, [_H, C.POINTER(_DP)]),

@@ -53,6 +53,7 @@ UNITS = [
     ("capi.cpp", "capi.o", ["--offload-host-only"]),
     ("config.cpp", "config.o", []),
     ("ics.cpp", "ics.o", ["-ffp-contract=off"]),
+    ("ics_device.hip", "ics_device.o", ["-ffp-contract=off"]),
     ("io_vtk.cpp", "io_vtk.o", []),
     ("batch_capi.cpp", "batch_capi.o", ["--offload-host-only"]),
     ("mlp_capi.cpp", "mlp_capi.o", ["--offload-host-only"]),

@@ -454,6 +454,12 @@ void ppk_mhd_destroy(ppk_mhd_t *h) {
   delete h;
 }
 
+namespace {
+// after the initial state is in U[0]: the rest of the SolverMHDMuscl constructor
+// (boundaries, U2 = U, primitives, compute_dt; src/muscl/SolverMHDMuscl.h:389-402)
+int finish_init(ppk_mhd_t *h);
+}  // namespace
+
 int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
   if (!h || !problem) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
@@ -464,7 +470,41 @@ int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
   h->cfl_valid = false;
   HIP_CHECK(h, hipMemcpyAsync(h->cur(), host.data(), h->state_bytes(), hipMemcpyHostToDevice,
                               h->stream));
-  // SolverMHDMuscl ctor: boundaries, U2 = U, primitives, compute_dt (SolverMHDMuscl.h:389-402)
+  // the host array must stay alive until the copy has read it
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return finish_init(h);
+}
+
+int ppk_mhd_init_problem_device(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
+  if (!h || !problem) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
+    return fail(h, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind");
+  h->iter_host = 0;
+  h->cfl_valid = false;
+  // the other state buffer is the wave problem's vector-potential scratch (3 of its 8 planes)
+  HIP_CHECK(h, ppk::init_problem_device(h->p, *problem, h->cur(), h->nxt(), h->stream));
+  return finish_init(h);
+}
+
+int ppk_mhd_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem *problem,
+                          double *U, size_t count) {
+  std::string why;
+  int rc = ppk::validate_params(params, why);
+  if (rc != PPK_OK) return fail(nullptr, rc, why);
+  if (!problem || !U) return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
+    return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind");
+  const size_t cells = (size_t)(params->nx + 2 * params->ghost_width) *
+                       (params->ny + 2 * params->ghost_width) *
+                       (params->dim == 3 ? params->nz + 2 * params->ghost_width : 1);
+  if (count != 8 * cells)
+    return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "count must be 8*isize*jsize*ksize");
+  ppk::init_problem_host(*params, *problem, U);
+  return PPK_OK;
+}
+
+namespace {
+int finish_init(ppk_mhd_t *h) {
   h->ctrl_host[ppk::CTRL_T] = 0.0;
   h->ctrl_host[ppk::CTRL_ITER] = 0.0;
   h->ctrl_host[ppk::CTRL_ACTIVE] = 1.0;
@@ -478,6 +518,7 @@ int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
   h->ctrl_host[ppk::CTRL_DT] = (0.0 + dt > h->p.tEnd) ? h->p.tEnd - 0.0 : dt;
   return write_ctrl(h);
 }
+}  // namespace
 
 int ppk_mhd_upload(ppk_mhd_t *h, const double *U, size_t count) {
   if (!h || !U) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");

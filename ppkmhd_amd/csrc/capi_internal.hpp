@@ -23,6 +23,9 @@ DevParams make_dev_params(const ppk_mhd_params &p);
 
 // host initial conditions in the reference layout (ics.cpp)
 void init_problem_host(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U);
+// the same on the device, into U (8 x cells); A: 3 x cells of scratch (ics_device.hip)
+hipError_t init_problem_device(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U,
+                               double *A, hipStream_t s);
 
 }  // namespace ppk
 

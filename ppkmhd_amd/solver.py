@@ -61,8 +61,10 @@ class SolverMHDMuscl:
         return check(rc, self._h)
 
     # ---- state ----------------------------------------------------------------------------
-    def init_problem(self, problem):
-        self._check(lib().ppk_mhd_init_problem(self._h, C.byref(problem)))
+    def init_problem(self, problem, on_device=False):
+        """Initial conditions (host-computed and uploaded, or computed on the GPU)."""
+        fn = lib().ppk_mhd_init_problem_device if on_device else lib().ppk_mhd_init_problem
+        self._check(fn(self._h, C.byref(problem)))
 
     def upload(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
@@ -178,6 +180,14 @@ class SolverMHDMuscl:
             istep = self.m_iteration
         self._check(lib().ppk_mhd_save_vtk(self._h, str(directory).encode(), prefix.encode(),
                                            int(istep), 1 if ascii else 0))
+
+
+def initial_state(params, problem):
+    """The initial state of a problem computed on the host (reference layout), no GPU."""
+    U = np.empty(params.shape, dtype=np.float64)
+    check(lib().ppk_mhd_initial_state(C.byref(params), C.byref(problem),
+                                      U.ctypes.data_as(C.POINTER(C.c_double)), U.size))
+    return U
 
 
 def write_vtk(params, U, directory=".", prefix="output", istep=0, ascii=False):
