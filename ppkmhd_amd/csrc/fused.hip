@@ -120,6 +120,38 @@ k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux,
     default: edge3<0>(rec, emf, p, c); break;
   }
 }
+// The three face solves (one launch) and the three edge solves (another), for running the two
+// launches side by side on two streams: each keeps its own register budget (the faces need
+// fewer registers than the EMF solves, so more face waves fit on a SIMD).
+template <int W, class R>
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
+k_faces_mix3d(const R *__restrict__ rec, R *__restrict__ flux, DevParams p, Brick bk,
+              const double *ctrl) {
+  if (!active(ctrl)) return;
+  const int B = xcd_remap(blockIdx.x, gridDim.x);
+  const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
+  if (c < 0) return;
+  switch (B % 3) {
+    case 0: face3<0>(rec, flux, p, c); break;
+    case 1: face3<1>(rec, flux, p, c); break;
+    default: face3<2>(rec, flux, p, c); break;
+  }
+}
+template <int W, class R>
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
+k_edges_mix3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
+              const double *ctrl) {
+  if (!active(ctrl)) return;
+  const int B = xcd_remap(blockIdx.x, gridDim.x);
+  const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
+  if (c < 0) return;
+  switch (B % 3) {
+    case 0: edge3<2>(rec, emf, p, c); break;
+    case 1: edge3<1>(rec, emf, p, c); break;
+    default: edge3<0>(rec, emf, p, c); break;
+  }
+}
+
 template <class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
@@ -203,7 +235,8 @@ void launch_half3(const DevParams &p, const double *Uin, R *rec, const double *c
 template <class R>
 hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *rec, R *flux,
                         R *emf, double *parts, const double *ctrl, int fold_cfl,
-                        const SurrogateCfg *sur, hipStream_t s, StageTimer *tm) {
+                        const SurrogateCfg *sur, const AuxStream *aux, hipStream_t s,
+                        StageTimer *tm) {
   if (p.dim == 3) {
     if (tm) tm->mark(ST_FUSED_HALF, true);
     switch (p.tile3) {
@@ -223,6 +256,20 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
       hipLaunchKernelGGL((k_edge3d<2, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
       hipLaunchKernelGGL((k_edge3d<1, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
       hipLaunchKernelGGL((k_edge3d<0, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+    } else if (p.riemann_mix >= 3 && aux) {  // faces and edges side by side on two streams
+      const dim3 g3((unsigned)(3 * bk.count()));
+      hipError_t e = hipEventRecord(aux->fork, s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(aux->stream, aux->fork, 0);
+      if (e != hipSuccess) return e;
+      // faces at 4 waves per SIMD (3: 5, spilling 60 bytes), edges at 3 (no spill)
+      if (p.riemann_mix == 3)
+        hipLaunchKernelGGL((k_faces_mix3d<4, R>), g3, b, 0, aux->stream, rec, flux, p, bk, ctrl);
+      else
+        hipLaunchKernelGGL((k_faces_mix3d<5, R>), g3, b, 0, aux->stream, rec, flux, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edges_mix3d<3, R>), g3, b, 0, s, rec, emf, p, bk, ctrl);
+      e = hipEventRecord(aux->join, aux->stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, aux->join, 0);
+      if (e != hipSuccess) return e;
     } else if (p.riemann_mix == 2) {  // 3 waves per SIMD, no spill
       hipLaunchKernelGGL((k_riemann_mix3d<3, R>), dim3((unsigned)(6 * bk.count())), b, 0, s, rec,
                          flux, emf, p, bk, ctrl);
@@ -278,10 +325,10 @@ hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout, Fu
   if (p.bulk32)
     return godunov_impl<float>(p, Uin, Uout, static_cast<float *>(w.rec),
                                static_cast<float *>(w.flux), static_cast<float *>(w.emf), parts,
-                               ctrl, fold_cfl, w.sur, s, tm);
+                               ctrl, fold_cfl, w.sur, w.aux, s, tm);
   return godunov_impl<double>(p, Uin, Uout, static_cast<double *>(w.rec),
                               static_cast<double *>(w.flux), static_cast<double *>(w.emf), parts,
-                              ctrl, fold_cfl, w.sur, s, tm);
+                              ctrl, fold_cfl, w.sur, w.aux, s, tm);
 }
 
 template <class R>

@@ -136,6 +136,65 @@ using Tile3d = HalfTile3<8, 8, 4>;   // 256 threads, 68 KB
 using Tile3e = HalfTile3<32, 4, 2>;  // 256 threads, 94 KB
 static_assert(Tile3c::LDS * sizeof(double) <= 160 * 1024, "3D tile exceeds the LDS of a CU");
 
+// the global loads of phase 1, held in registers until stashed in LDS (so a persistent block
+// can issue the next tile's loads before it computes the current tile's records)
+template <class T>
+struct HalfFetch3 {
+  static constexpr int IF = (T::NF + T::NT - 1) / T::NT, IQ = (T::NQ + T::NT - 1) / T::NT;
+  double fv[IF][3], qv[IQ][5];
+  bool fok[IF], qok[IQ];
+};
+template <class T>
+PPK_HD void half3_fetch(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
+                        int b0z, int tid, HalfFetch3<T> &f) {
+  constexpr int IF = HalfFetch3<T>::IF, IQ = HalfFetch3<T>::IQ;
+  const long long vs = p.vstride;
+  const int hi = 1 - p.gw;  // last primitive cell: size + hi
+#pragma unroll
+  for (int it = 0; it < IF; ++it) {
+    const int n = tid + it * T::NT;
+    const int x = n % T::FX, r = n / T::FX, y = r % T::FY, z = r / T::FY;
+    const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
+    f.fok[it] = n < T::NF && i < p.isize && j < p.jsize && k < p.ksize;
+    // unconditional load at a clamped (in-box) address keeps all loads in one batch
+    const long long t = cidx(p, min(i, p.isize - 1), min(j, p.jsize - 1), min(k, p.ksize - 1));
+    f.fv[it][0] = U[t + IA * vs];
+    f.fv[it][1] = U[t + IB * vs];
+    f.fv[it][2] = U[t + IC * vs];
+  }
+#pragma unroll
+  for (int it = 0; it < IQ; ++it) {
+    const int n = tid + it * T::NT;
+    const int x = n % T::QX, r = n / T::QX, y = r % T::QY, z = r / T::QY;
+    const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
+    f.qok[it] = n < T::NQ && i <= p.isize + hi && j <= p.jsize + hi && k <= p.ksize + hi;
+    const long long t =
+        cidx(p, min(i, p.isize + hi), min(j, p.jsize + hi), min(k, p.ksize + hi));
+#pragma unroll
+    for (int v = ID; v <= IW; ++v) f.qv[it][v] = U[t + v * vs];
+  }
+}
+template <class T, class R>
+PPK_HD void half3_stash(const HalfFetch3<T> &f, int tid, R *sF, R *sQ) {
+  constexpr int IF = HalfFetch3<T>::IF, IQ = HalfFetch3<T>::IQ;
+#pragma unroll
+  for (int it = 0; it < IF; ++it) {
+    const int n = tid + it * T::NT;
+    if (f.fok[it]) {
+      sF[n] = R(f.fv[it][0]);
+      sF[T::NF + n] = R(f.fv[it][1]);
+      sF[2 * T::NF + n] = R(f.fv[it][2]);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < IQ; ++it) {
+    const int n = tid + it * T::NT;
+    if (f.qok[it]) {
+#pragma unroll
+      for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = R(f.qv[it][v]);
+    }
+  }
+}
 // phase 1, all global loads of the tile: face fields A, B, C of cells [b0-1, b0+T+1]
 // (clipped to the box) into sF, and the five other conservative variables of the primitive
 // cells [b0-1, b0+T] (up to size-gw+1) into their slots of sQ.  Every load of a thread is
@@ -144,51 +203,9 @@ static_assert(Tile3c::LDS * sizeof(double) <= 160 * 1024, "3D tile exceeds the L
 template <class T, class R>
 PPK_HD void half3_load(const double *__restrict__ U, const DevParams &p, int b0x, int b0y,
                        int b0z, int tid, R *sF, R *sQ) {
-  constexpr int IF = (T::NF + T::NT - 1) / T::NT, IQ = (T::NQ + T::NT - 1) / T::NT;
-  const long long vs = p.vstride;
-  const int hi = 1 - p.gw;  // last primitive cell: size + hi
-  double fv[IF][3], qv[IQ][5];
-  bool fok[IF], qok[IQ];
-#pragma unroll
-  for (int it = 0; it < IF; ++it) {
-    const int n = tid + it * T::NT;
-    const int x = n % T::FX, r = n / T::FX, y = r % T::FY, z = r / T::FY;
-    const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
-    fok[it] = n < T::NF && i < p.isize && j < p.jsize && k < p.ksize;
-    // unconditional load at a clamped (in-box) address keeps all loads in one batch
-    const long long t = cidx(p, min(i, p.isize - 1), min(j, p.jsize - 1), min(k, p.ksize - 1));
-    fv[it][0] = U[t + IA * vs];
-    fv[it][1] = U[t + IB * vs];
-    fv[it][2] = U[t + IC * vs];
-  }
-#pragma unroll
-  for (int it = 0; it < IQ; ++it) {
-    const int n = tid + it * T::NT;
-    const int x = n % T::QX, r = n / T::QX, y = r % T::QY, z = r / T::QY;
-    const int i = b0x - 1 + x, j = b0y - 1 + y, k = b0z - 1 + z;
-    qok[it] = n < T::NQ && i <= p.isize + hi && j <= p.jsize + hi && k <= p.ksize + hi;
-    const long long t =
-        cidx(p, min(i, p.isize + hi), min(j, p.jsize + hi), min(k, p.ksize + hi));
-#pragma unroll
-    for (int v = ID; v <= IW; ++v) qv[it][v] = U[t + v * vs];
-  }
-#pragma unroll
-  for (int it = 0; it < IF; ++it) {
-    const int n = tid + it * T::NT;
-    if (fok[it]) {
-      sF[n] = R(fv[it][0]);
-      sF[T::NF + n] = R(fv[it][1]);
-      sF[2 * T::NF + n] = R(fv[it][2]);
-    }
-  }
-#pragma unroll
-  for (int it = 0; it < IQ; ++it) {
-    const int n = tid + it * T::NT;
-    if (qok[it]) {
-#pragma unroll
-      for (int v = ID; v <= IW; ++v) sQ[v * T::NQ + n] = R(qv[it][v]);
-    }
-  }
+  HalfFetch3<T> f;
+  half3_fetch<T>(U, p, b0x, b0y, b0z, tid, f);
+  half3_stash<T, R>(f, tid, sF, sQ);
 }
 
 // phase 2: primitives of cells [b0-1, b0+T] up to size-gw+1 (MHDRunFunctors3D.h:88-163), in

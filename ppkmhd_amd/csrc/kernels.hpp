@@ -76,12 +76,18 @@ struct SurrogateCfg {
   unsigned long long *counters = nullptr;
   unsigned char *flags = nullptr;  // dim x cells: faces left to the HLLD pass (device scratch)
 };
+// a second stream of default priority and the two events that fork work onto it and join it
+struct AuxStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
 // element type: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {
   void *rec = nullptr;   // kRec{2,3} x cells
   void *flux = nullptr;  // dim x kFluxComp{2,3} x cells
   void *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
   const SurrogateCfg *sur = nullptr;  // surrogate face solves when set
+  const AuxStream *aux = nullptr;     // side stream for concurrent face / edge launches
 };
 
 // dt modes of the set_dt kernel
