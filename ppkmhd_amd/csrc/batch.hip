@@ -152,21 +152,21 @@ __global__ void __launch_bounds__(T::NT) k_bhalf3d(const BatchMember *__restrict
 }
 
 // ---- face and edge solves (interleaved types as in the single-grid step) ---------------------
-template <int W, class R>
+template <int DIM, int W, class R>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 k_briemann(const BatchMember *__restrict__ mem, int nm) {
   const long long gb = blockIdx.x;
   const BatchMember &m = mem[find_member(mem, nm, BS_RIEMANN, gb)];
   if (!member_active(m)) return;
   const DevParams &p = m.p;
-  const int nt = p.dim == 3 ? 6 : 3;
+  constexpr int nt = DIM == 3 ? 6 : 3;
   const long long B = gb - m.begin[BS_RIEMANN];
   const long long c = sub_brick_cell(p, m.sub, (int)(B / nt), threadIdx.x);
   if (c < 0) return;
   const R *rec = static_cast<const R *>(m.rec);
   R *flux = static_cast<R *>(m.flux);
   R *emf = static_cast<R *>(m.emf);
-  if (p.dim == 3) {
+  if constexpr (DIM == 3) {
     switch ((int)(B % 6)) {
       case 0: face3<0>(rec, flux, p, c); break;
       case 1: face3<1>(rec, flux, p, c); break;
@@ -185,7 +185,7 @@ k_briemann(const BatchMember *__restrict__ mem, int nm) {
 }
 
 // ---- update + CT + ghost copy + CFL partials of the new state -----------------------------
-template <class R>
+template <int DIM, class R>
 __global__ void __launch_bounds__(kCflBlock) k_bupdate(const BatchMember *__restrict__ mem,
                                                        int nm) {
   __shared__ double red[kCflBlock / 64];
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(kCflBlock) k_bupdate(const BatchMember *__rest
   const DevParams &p = m.p;
   const int L = (int)(gb - m.begin[BS_UPDATE]);
   const double dt = m.ctrl[CTRL_DT];
-  const double dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = p.dim == 3 ? dt / p.dz : 0.0;
+  const double dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = DIM == 3 ? dt / p.dz : 0.0;
   const int par = parity(m);
   int i, j, k;
   double best = 0.0;
@@ -261,6 +261,26 @@ void launch_bhalf(const BatchMember *members, int nm, int dim, int tile, long lo
   }
 }
 
+// half step, face/edge solves, update of every member (kernels specialised per dimension)
+template <class R>
+void launch_rest(const BatchMember *members, int nm, int dim, int tile,
+                 const long long totals[BS_COUNT], hipStream_t s) {
+  using namespace batchk;
+  const dim3 b(kBlock), gr((unsigned)totals[BS_RIEMANN]), gu((unsigned)totals[BS_UPDATE]);
+  launch_bhalf<R>(members, nm, dim, tile, totals[BS_HALF], s);
+  if (dim == 3) {
+    // 3 waves per SIMD keep the 3D double solves free of spills (the single-grid default)
+    if constexpr (sizeof(R) == sizeof(double))
+      hipLaunchKernelGGL((k_briemann<3, 3, R>), gr, b, 0, s, members, nm);
+    else
+      hipLaunchKernelGGL((k_briemann<3, 4, R>), gr, b, 0, s, members, nm);
+    hipLaunchKernelGGL((k_bupdate<3, R>), gu, dim3(kCflBlock), 0, s, members, nm);
+  } else {
+    hipLaunchKernelGGL((k_briemann<2, 4, R>), gr, b, 0, s, members, nm);
+    hipLaunchKernelGGL((k_bupdate<2, R>), gu, dim3(kCflBlock), 0, s, members, nm);
+  }
+}
+
 hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32, int tile,
                       const long long totals[BS_COUNT], int with_cfl, hipStream_t s) {
   using namespace batchk;
@@ -274,24 +294,10 @@ hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32, i
     hipLaunchKernelGGL(k_bcfl, dim3((unsigned)totals[BS_CFL]), dim3(kCflBlock), 0, s, members,
                        nm);
   hipLaunchKernelGGL(k_bset_dt, dim3(nm), b, 0, s, members);
-  if (bulk32) {
-    launch_bhalf<float>(members, nm, dim, tile, totals[BS_HALF], s);
-    hipLaunchKernelGGL((k_briemann<4, float>), dim3((unsigned)totals[BS_RIEMANN]), b, 0, s,
-                       members, nm);
-    hipLaunchKernelGGL(k_bupdate<float>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
-                       s, members, nm);
-  } else {
-    launch_bhalf<double>(members, nm, dim, tile, totals[BS_HALF], s);
-    // 3 waves per SIMD keep the 3D double solves free of spills (the single-grid default)
-    if (dim == 3)
-      hipLaunchKernelGGL((k_briemann<3, double>), dim3((unsigned)totals[BS_RIEMANN]), b, 0, s,
-                         members, nm);
-    else
-      hipLaunchKernelGGL((k_briemann<4, double>), dim3((unsigned)totals[BS_RIEMANN]), b, 0, s,
-                         members, nm);
-    hipLaunchKernelGGL(k_bupdate<double>, dim3((unsigned)totals[BS_UPDATE]), dim3(kCflBlock), 0,
-                       s, members, nm);
-  }
+  if (bulk32)
+    launch_rest<float>(members, nm, dim, tile, totals, s);
+  else
+    launch_rest<double>(members, nm, dim, tile, totals, s);
   hipLaunchKernelGGL(k_bend, gm, bm, 0, s, members, nm);
   return hipGetLastError();
 }
