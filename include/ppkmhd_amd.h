@@ -137,7 +137,9 @@ int ppk_mhd_device_count(void);
  * (src/utils/config/inih/INIReader.cpp:100-105), reals through strtof, i.e. rounded to float
  * (src/utils/config/ConfigMap.cpp:37-46), HydroParams::setup defaults
  * (src/shared/HydroParams.cpp:28-217) and the problem parameter defaults.  solver_name
- * receives [run] solver_name (NUL-terminated, truncated to name_len).
+ * receives [run] solver_name (NUL-terminated, truncated to name_len).  Solver names
+ * MHD_Muscl_2D/3D give ghost_width 3 (for ppk_mhd_create), Hydro_Muscl_2D/3D ghost_width 2
+ * (for ppk_hydro_create); any other name fails with PPK_ERR_CONFIG.
  * Replaces: HydroParams::setup + broadcast_parameters (src/main.cpp:133-137).
  */
 int ppk_mhd_params_from_ini(const char *ini_path, ppk_mhd_params *params,
@@ -345,6 +347,51 @@ int ppk_mlp_evaluate(ppk_mlp_t *net, const ppk_faceset_t *set, double *mse);
 int ppk_mhd_set_surrogate(ppk_mhd_t *h, const ppk_mlp_t *net, double gate_rho, double gate_p);
 /* Faces solved by the network and by HLLD since the last reset. */
 int ppk_mhd_surrogate_counts(ppk_mhd_t *h, int64_t *by_network, int64_t *by_hlld, int reset);
+
+/* ---- Euler (hydro) MUSCL-Hancock solvers ------------------------------------------------------
+ * The reference's Hydro_Muscl_2D / Hydro_Muscl_3D (src/muscl/SolverHydroMuscl.{h,cpp},
+ * implementation version 0, no gravity): nbvar 4 in 2D (ID, IP, IU, IV) and 5 in 3D (+ IW),
+ * ghost_width 2, same array layout as above with nbvar variables.  Parameters come in the
+ * ppk_mhd_params struct with ghost_width 2, riemann approx, llf, hllc or (2D only) hll,
+ * slope_type 0, 1 or 2, precision PPK_FP64 or PPK_FP64_STRICT, kernel_path staged or fused;
+ * the initial condition in the ppk_mhd_problem struct, kind implode or blast.  The time-loop semantics (CFL dt clamped to tEnd,
+ * U / U2 ping-pong, finished()) are those of the MHD handle.  PPK_FP64_STRICT results equal the
+ * reference's double arithmetic bit for bit. */
+typedef struct ppk_hydro_solver ppk_hydro_t;
+/* Replaces: SolverHydroMuscl<dim>::SolverHydroMuscl allocation part
+ * (src/muscl/SolverHydroMuscl.h:177-285), keys Hydro_Muscl_2D / Hydro_Muscl_3D
+ * (src/shared/SolverFactory.cpp:26-27). */
+int ppk_hydro_create(const ppk_mhd_params *params, ppk_hydro_t **handle);
+void ppk_hydro_destroy(ppk_hydro_t *handle);
+const char *ppk_hydro_last_error(const ppk_hydro_t *handle);
+/* nbvar of the handle (4 or 5); 0 for NULL */
+int ppk_hydro_nbvar(const ppk_hydro_t *handle);
+/* Initial condition, ghost fill, U2 = U and the first dt, as the constructor does
+ * (src/muscl/SolverHydroMuscl.h:286-300; implode HydroInitFunctors2D.h:35-125 /
+ * HydroInitFunctors3D.h:66-165, blast HydroInitFunctors2D.h:129-212 / 3D.h:170-262). */
+int ppk_hydro_init_problem(ppk_hydro_t *h, const ppk_mhd_problem *problem);
+/* The initial condition alone, on the host (count = nbvar*isize*jsize*ksize). */
+int ppk_hydro_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem *problem,
+                            double *U, size_t count);
+int ppk_hydro_upload(ppk_hydro_t *h, const double *U, size_t count);
+int ppk_hydro_download(ppk_hydro_t *h, double *U, size_t count);
+int ppk_hydro_device_state(ppk_hydro_t *h, double **dU);
+int ppk_hydro_stream(ppk_hydro_t *h, void **stream);
+/* Replaces: SolverHydroMuscl::make_boundaries (src/muscl/SolverHydroMuscl.cpp:22-64). */
+int ppk_hydro_make_boundaries(ppk_hydro_t *h);
+/* Replaces: SolverHydroMuscl::compute_dt_local (src/muscl/SolverHydroMuscl.h:650-691). */
+int ppk_hydro_compute_dt(ppk_hydro_t *h, double *dt);
+/* One step; dt_mode as ppk_mhd_step.
+ * Replaces: SolverHydroMuscl::next_iteration_impl -> godunov_unsplit_impl
+ * (src/muscl/SolverHydroMuscl.h:697-760, SolverHydroMuscl.cpp:257-405). */
+int ppk_hydro_step(ppk_hydro_t *h, double dt, int dt_mode);
+/* nsteps steps, each a no-op once finished() holds (as ppk_mhd_run). */
+int ppk_hydro_run(ppk_hydro_t *h, int nsteps, double dt, int dt_mode);
+int ppk_hydro_get_time(ppk_hydro_t *h, double *t, double *dt, int64_t *iteration);
+int ppk_hydro_set_time(ppk_hydro_t *h, double t, int64_t iteration);
+/* Replaces: SolverBase::finished (src/shared/SolverBase.cpp:195-201). */
+int ppk_hydro_finished(ppk_hydro_t *h, int *finished);
+int ppk_hydro_synchronize(ppk_hydro_t *h);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,484 @@
+// TEST INFRASTRUCTURE -- CPU restatement of the reference's hydro (Euler) MUSCL-Hancock step,
+// implementation version 0, used only as the checker of the library's hydro kernels.  It
+// follows the reference's staging: ghost fill, U2 = U, primitives of every cell, fluxes of the
+// left faces of every cell of [gw, size-gw] (slopes recomputed per face), then the update.
+// Reference: src/muscl/SolverHydroMuscl.cpp:282-405 (godunov_unsplit_impl),
+// src/muscl/HydroRunFunctors{2D,3D}.h, src/muscl/HydroBaseFunctor{2D,3D}.h,
+// src/shared/RiemannSolvers.h, src/shared/BoundariesFunctors.h:18-420,
+// src/muscl/HydroInitFunctors{2D,3D}.h (implode, blast).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../include/ppkmhd_amd.h"
+
+namespace {
+
+enum { ID = 0, IP = 1, IU = 2, IV = 3, IW = 4 };
+
+struct Grid {
+  int dim, nv, gw, nx, ny, nz, is, js, ks;
+  long long vs;
+  explicit Grid(const ppk_mhd_params &p)
+      : dim(p.dim), nv(p.dim == 3 ? 5 : 4), gw(p.ghost_width), nx(p.nx), ny(p.ny),
+        nz(p.dim == 3 ? p.nz : 1) {
+    is = nx + 2 * gw;
+    js = ny + 2 * gw;
+    ks = dim == 3 ? nz + 2 * gw : 1;
+    vs = (long long)is * js * ks;
+  }
+  long long at(int i, int j, int k, int v) const {
+    return (((long long)v * ks + k) * js + j) * (long long)is + i;
+  }
+};
+
+struct S5 {
+  double v[5] = {0, 0, 0, 0, 0};
+  double &operator[](int i) { return v[i]; }
+  const double &operator[](int i) const { return v[i]; }
+};
+
+// ---- boundaries (MakeBoundariesFunctor2D/3D, non-MHD: Dirichlet flips the normal velocity) --
+void boundaries(const ppk_mhd_params &p, double *U) {
+  const Grid g(p);
+  for (int dir = 0; dir < g.dim; ++dir) {
+    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    for (int side = 0; side < 2; ++side) {
+      const int bc = p.bc[2 * dir + side];
+      for (int k = 0; k < g.ks; ++k)
+        for (int j = 0; j < g.js; ++j)
+          for (int i = 0; i < g.is; ++i) {
+            const int c = dir == 0 ? i : dir == 1 ? j : k;
+            int c0;
+            if (side == 0) {
+              if (c >= g.gw) continue;
+              c0 = bc == PPK_BC_DIRICHLET ? 2 * g.gw - 1 - c : bc == PPK_BC_NEUMANN ? g.gw : n + c;
+            } else {
+              if (c < n + g.gw) continue;
+              c0 = bc == PPK_BC_DIRICHLET ? 2 * n + 2 * g.gw - 1 - c
+                   : bc == PPK_BC_NEUMANN ? n + g.gw - 1
+                                          : c - n;
+            }
+            const int i0 = dir == 0 ? c0 : i, j0 = dir == 1 ? c0 : j, k0 = dir == 2 ? c0 : k;
+            for (int v = 0; v < g.nv; ++v) {
+              const double sign = (bc == PPK_BC_DIRICHLET && v == IU + dir) ? -1.0 : 1.0;
+              U[g.at(i, j, k, v)] = U[g.at(i0, j0, k0, v)] * sign;
+            }
+          }
+    }
+  }
+}
+
+// ---- primitives (computePrimitives) ---------------------------------------------------------
+void to_prim(const ppk_mhd_params &p, int nv, const S5 &u, double &c, S5 &q) {
+  const double d = std::fmax(u[ID], p.smallr);
+  const double ux = u[IU] / d, uy = u[IV] / d;
+  const double uz = nv == 5 ? u[IW] / d : 0.0;
+  const double eken = nv == 5 ? 0.5 * (ux * ux + uy * uy + uz * uz) : 0.5 * (ux * ux + uy * uy);
+  const double e = u[IP] / d - eken;
+  const double pr = std::fmax((p.gamma0 - 1.0) * d * e, d * p.smallp);
+  c = std::sqrt(p.gamma0 * (pr) / d);
+  q[ID] = d;
+  q[IP] = pr;
+  q[IU] = ux;
+  q[IV] = uy;
+  if (nv == 5) q[IW] = uz;
+}
+
+// ---- slopes (slope_unsplit_hydro_2d/3d_scalar) ----------------------------------------------
+double minmod_slope(double st, double q, double qp, double qm) {
+  const double dlft = st * (q - qm), drgt = st * (qp - q), dcen = 0.5 * (qp - qm);
+  const double dsgn = dcen >= 0.0 ? 1.0 : -1.0;
+  double dlim = std::fmin(std::fabs(dlft), std::fabs(drgt));
+  if ((dlft * drgt) <= 0.0) dlim = 0.0;
+  return dsgn * std::fmin(dlim, std::fabs(dcen));
+}
+
+// ---- Riemann solvers (RiemannSolvers.h) -----------------------------------------------------
+void flux_of_gdnv(const ppk_mhd_params &p, int nv, const S5 &qg, S5 &f) {
+  f[ID] = qg[ID] * qg[IU];
+  f[IU] = f[ID] * qg[IU] + qg[IP];
+  f[IV] = f[ID] * qg[IV];
+  if (nv == 5) f[IW] = f[ID] * qg[IW];
+  const double entho = 1.0 / (p.gamma0 - 1.0);
+  double ekin = 0.5 * qg[ID] * (qg[IU] * qg[IU] + qg[IV] * qg[IV]);
+  if (nv == 5) ekin += 0.5 * qg[ID] * (qg[IW] * qg[IW]);
+  const double etot = qg[IP] * entho + ekin;
+  f[IP] = qg[IU] * (etot + qg[IP]);
+}
+
+void solve_approx(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, S5 &f) {
+  const double g0 = p.gamma0, g6 = p.gamma6;
+  const double rl = std::fmax(ql[ID], p.smallr), ul = ql[IU], pl = std::fmax(ql[IP], rl * p.smallp);
+  const double rr = std::fmax(qr[ID], p.smallr), ur = qr[IU], pr = std::fmax(qr[IP], rr * p.smallp);
+  const double cl = g0 * pl * rl, cr = g0 * pr * rr;
+  double wl = std::sqrt(cl), wr = std::sqrt(cr);
+  double pstar = std::fmax(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), 0.0);
+  double pold = pstar, conv = 1.0;
+  for (int it = 0; it < 10 && conv > 1e-6; ++it) {
+    const double wwl = std::sqrt(cl * (1.0 + g6 * (pold - pl) / pl));
+    const double wwr = std::sqrt(cr * (1.0 + g6 * (pold - pr) / pr));
+    const double a = 2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
+    const double b = 2.0 * wwr * wwr * wwr / (wwr * wwr + cr);
+    const double usl = ul - (pold - pl) / wwl, usr = ur + (pold - pr) / wwr;
+    const double delp = std::fmax(b * a / (b + a) * (usl - usr), -pold);
+    pold = pold + delp;
+    conv = std::fabs(delp / (pold + p.smallpp));
+  }
+  pstar = pold;
+  wl = std::sqrt(cl * (1.0 + g6 * (pstar - pl) / pl));
+  wr = std::sqrt(cr * (1.0 + g6 * (pstar - pr) / pr));
+  const double ustar = 0.5 * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr);
+  const double sgnm = std::copysign(1.0, ustar);
+  const bool left = sgnm > 0.0;
+  const double ro = left ? rl : rr, uo = left ? ul : ur, po = left ? pl : pr, wo = left ? wl : wr;
+  const double co = std::fmax(p.smallc, std::sqrt(std::fabs(g0 * po / ro)));
+  const double rstar = std::fmax(ro / (1.0 + ro * (po - pstar) / (wo * wo)), p.smallr);
+  const double cstar = std::fmax(p.smallc, std::sqrt(std::fabs(g0 * pstar / rstar)));
+  double spout = co - sgnm * uo, spin = cstar - sgnm * ustar;
+  const double ushock = wo / ro - sgnm * uo;
+  if (pstar >= po) spin = spout = ushock;
+  const double scr = std::fmax(spout - spin, p.smallc + std::fabs(spout + spin));
+  double frac = 0.5 * (1.0 + (spout + spin) / scr);
+  frac = (frac != frac) ? 0.0 : (frac >= 1.0 ? 1.0 : frac <= 0.0 ? 0.0 : frac);
+  S5 qg;
+  qg[ID] = frac * rstar + (1.0 - frac) * ro;
+  qg[IU] = frac * ustar + (1.0 - frac) * uo;
+  qg[IP] = frac * pstar + (1.0 - frac) * po;
+  if (spout < 0.0) { qg[ID] = ro; qg[IU] = uo; qg[IP] = po; }
+  if (spin > 0.0) { qg[ID] = rstar; qg[IU] = ustar; qg[IP] = pstar; }
+  qg[IV] = left ? ql[IV] : qr[IV];
+  if (nv == 5) qg[IW] = left ? ql[IW] : qr[IW];
+  flux_of_gdnv(p, nv, qg, f);
+}
+
+void cons_flux(int nv, const S5 &q, double entho, bool hll, S5 &u, S5 &f) {
+  u[ID] = q[ID];
+  u[IP] = q[IP] * entho + 0.5 * q[ID] * q[IU] * q[IU];
+  u[IP] += 0.5 * q[ID] * q[IV] * q[IV];
+  if (nv == 5) u[IP] += 0.5 * q[ID] * q[IW] * q[IW];
+  u[IU] = q[ID] * q[IU];
+  u[IV] = q[ID] * q[IV];
+  if (nv == 5) u[IW] = q[ID] * q[IW];
+  f[ID] = hll ? u[IU] : u[ID] * q[IU];
+  f[IP] = q[IU] * (u[IP] + q[IP]);
+  f[IU] = q[IP] + u[IU] * q[IU];
+  f[IV] = f[ID] * q[IV];
+  if (nv == 5) f[IW] = f[ID] * q[IW];
+}
+
+void solve_llf_hll(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, bool hll,
+                   S5 &f) {
+  const double g0 = p.gamma0, entho = 1.0 / (g0 - 1.0);
+  const double rl = std::fmax(ql[ID], p.smallr), ul = ql[IU], pl = std::fmax(ql[IP], rl * p.smallp);
+  const double rr = std::fmax(qr[ID], p.smallr), ur = qr[IU], pr = std::fmax(qr[IP], rr * p.smallp);
+  const double cl = std::sqrt(g0 * pl / rl), cr = std::sqrt(g0 * pr / rr);
+  S5 uL, uR, fL, fR;
+  cons_flux(nv, ql, entho, hll, uL, fL);
+  cons_flux(nv, qr, entho, hll, uR, fR);
+  if (!hll) {
+    const double cmax = std::fmax(std::fabs(ul) + cl, std::fabs(ur) + cr);
+    for (int n = 0; n < nv; ++n) f[n] = 0.5 * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
+  } else {
+    const double SL = std::fmin(std::fmin(ul, ur) - std::fmax(cl, cr), 0.0);
+    const double SR = std::fmax(std::fmax(ul, ur) + std::fmax(cl, cr), 0.0);
+    for (int n = 0; n < nv; ++n)
+      f[n] = (SR * fL[n] - SL * fR[n] + SR * SL * (uR[n] - uL[n])) / (SR - SL);
+  }
+}
+
+void solve_hllc(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, S5 &f) {
+  const double g0 = p.gamma0, entho = 1.0 / (g0 - 1.0);
+  const double rl = std::fmax(ql[ID], p.smallr), pl = std::fmax(ql[IP], rl * p.smallp), ul = ql[IU];
+  double ecinl = 0.5 * rl * ul * ul;
+  ecinl += 0.5 * rl * ql[IV] * ql[IV];
+  if (nv == 5) ecinl += 0.5 * rl * ql[IW] * ql[IW];
+  const double etotl = pl * entho + ecinl, ptotl = pl;
+  const double rr = std::fmax(qr[ID], p.smallr), pr = std::fmax(qr[IP], rr * p.smallp), ur = qr[IU];
+  double ecinr = 0.5 * rr * ur * ur;
+  ecinr += 0.5 * rr * qr[IV] * qr[IV];
+  if (nv == 5) ecinr += 0.5 * rr * qr[IW] * qr[IW];
+  const double etotr = pr * entho + ecinr, ptotr = pr;
+  const double cfl_ = std::sqrt(std::fmax(g0 * pl / rl, p.smallc * p.smallc));
+  const double cfr = std::sqrt(std::fmax(g0 * pr / rr, p.smallc * p.smallc));
+  const double SL = std::fmin(ul, ur) - std::fmax(cfl_, cfr);
+  const double SR = std::fmax(ul, ur) + std::fmax(cfl_, cfr);
+  const double rcl = rl * (ul - SL), rcr = rr * (SR - ur);
+  const double ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
+  const double ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
+  const double rstarl = rl * (SL - ul) / (SL - ustar);
+  const double etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar) / (SL - ustar);
+  const double rstarr = rr * (SR - ur) / (SR - ustar);
+  const double etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar) / (SR - ustar);
+  double ro, uo, ptoto, etoto;
+  if (SL > 0.0) { ro = rl; uo = ul; ptoto = ptotl; etoto = etotl; }
+  else if (ustar > 0.0) { ro = rstarl; uo = ustar; ptoto = ptotstar; etoto = etotstarl; }
+  else if (SR > 0.0) { ro = rstarr; uo = ustar; ptoto = ptotstar; etoto = etotstarr; }
+  else { ro = rr; uo = ur; ptoto = ptotr; etoto = etotr; }
+  f[ID] = ro * uo;
+  f[IU] = ro * uo * uo + ptoto;
+  f[IP] = (etoto + ptoto) * uo;
+  f[IV] = f[ID] > 0.0 ? f[ID] * ql[IV] : f[ID] * qr[IV];
+  if (nv == 5) f[IW] = f[ID] > 0.0 ? f[ID] * ql[IW] : f[ID] * qr[IW];
+}
+
+void solve(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, S5 &f) {
+  if (p.riemann == PPK_RIEMANN_APPROX) solve_approx(p, nv, ql, qr, f);
+  else if (p.riemann == PPK_RIEMANN_HLL) solve_llf_hll(p, nv, ql, qr, true, f);
+  else if (p.riemann == PPK_RIEMANN_HLLC) solve_hllc(p, nv, ql, qr, f);
+  else solve_llf_hll(p, nv, ql, qr, false, f);
+}
+
+// ---- the step -------------------------------------------------------------------------------
+struct HydroSolver {
+  ppk_mhd_params p;
+  Grid g;
+  std::vector<double> U, U2, Q, F[3];
+  double t = 0.0, dt = 0.0;
+  long long it = 0;
+  explicit HydroSolver(const ppk_mhd_params &pp) : p(pp), g(pp) {
+    U.assign(g.nv * g.vs, 0.0);
+    U2 = U;
+    Q = U;
+    for (int d = 0; d < 3; ++d) F[d] = U;
+  }
+  S5 cell(const std::vector<double> &A, int i, int j, int k) const {
+    S5 s;
+    for (int v = 0; v < g.nv; ++v) s[v] = A[g.at(i, j, k, v)];
+    return s;
+  }
+  // compute_dt_local (SolverHydroMuscl.h:650-690): interior cells of the current state
+  double inv_dt(const std::vector<double> &A) const {
+    double inv = 0.0;
+    for (int k = g.dim == 3 ? g.gw : 0; k < (g.dim == 3 ? g.ks - g.gw : 1); ++k)
+      for (int j = g.gw; j < g.js - g.gw; ++j)
+        for (int i = g.gw; i < g.is - g.gw; ++i) {
+          S5 q;
+          double c = 0.0;
+          to_prim(p, g.nv, cell(A, i, j, k), c, q);
+          const double vx = c + std::fabs(q[IU]), vy = c + std::fabs(q[IV]);
+          if (g.dim == 3) {
+            const double vz = c + std::fabs(q[IW]);
+            inv = std::fmax(inv, vx / p.dx + vy / p.dy + vz / p.dz);
+          } else {
+            inv = std::fmax(inv, vx / p.dx + vy / p.dy);
+          }
+        }
+    return inv;
+  }
+  // slopes of cell (i,j,k) of Q
+  void cell_slopes(int i, int j, int k, S5 &dx, S5 &dy, S5 &dz) const {
+    const S5 q = cell(Q, i, j, k);
+    if (p.slope_type == 0) { dx = dy = dz = S5(); return; }
+    const S5 qpx = cell(Q, i + 1, j, k), qmx = cell(Q, i - 1, j, k);
+    const S5 qpy = cell(Q, i, j + 1, k), qmy = cell(Q, i, j - 1, k);
+    for (int v = 0; v < g.nv; ++v) {
+      dx[v] = minmod_slope(p.slope_type, q[v], qpx[v], qmx[v]);
+      dy[v] = minmod_slope(p.slope_type, q[v], qpy[v], qmy[v]);
+    }
+    if (g.dim == 3) {
+      const S5 qpz = cell(Q, i, j, k + 1), qmz = cell(Q, i, j, k - 1);
+      for (int v = 0; v < g.nv; ++v) dz[v] = minmod_slope(p.slope_type, q[v], qpz[v], qmz[v]);
+    }
+  }
+  // trace_unsplit_{2d,3d}_along_dir of a cell to face (dir, side: 0 min, 1 max)
+  S5 trace(const S5 &q, const S5 &dx, const S5 &dy, const S5 &dz, int dir, int side,
+           double dtdx, double dtdy, double dtdz) const {
+    const double g0 = p.gamma0;
+    const double r = q[ID], pr = q[IP], u = q[IU], v = q[IV], w = g.dim == 3 ? q[IW] : 0.0;
+    double sr0, sp0, su0, sv0, sw0 = 0.0;
+    if (g.dim == 3) {
+      sr0 = -u * dx[ID] - v * dy[ID] - w * dz[ID] - (dx[IU] + dy[IV] + dz[IW]) * r;
+      sp0 = -u * dx[IP] - v * dy[IP] - w * dz[IP] - (dx[IU] + dy[IV] + dz[IW]) * g0 * pr;
+      su0 = -u * dx[IU] - v * dy[IU] - w * dz[IU] - (dx[IP]) / r;
+      sv0 = -u * dx[IV] - v * dy[IV] - w * dz[IV] - (dy[IP]) / r;
+      sw0 = -u * dx[IW] - v * dy[IW] - w * dz[IW] - (dz[IP]) / r;
+    } else {
+      sr0 = -u * dx[ID] - v * dy[ID] - (dx[IU] + dy[IV]) * r;
+      sp0 = -u * dx[IP] - v * dy[IP] - (dx[IU] + dy[IV]) * g0 * pr;
+      su0 = -u * dx[IU] - v * dy[IU] - (dx[IP]) / r;
+      sv0 = -u * dx[IV] - v * dy[IV] - (dy[IP]) / r;
+    }
+    const S5 &d = dir == 0 ? dx : dir == 1 ? dy : dz;
+    const double dtd = dir == 0 ? dtdx : dir == 1 ? dtdy : dtdz;
+    const double s0[5] = {sr0, sp0, su0, sv0, sw0};
+    const double c0[5] = {r, pr, u, v, w};
+    S5 out;
+    for (int n = 0; n < g.nv; ++n)
+      out[n] = side == 0 ? c0[n] - 0.5 * d[n] + s0[n] * dtd * 0.5
+                         : c0[n] + 0.5 * d[n] + s0[n] * dtd * 0.5;
+    out[ID] = std::fmax(p.smallr, out[ID]);
+    return out;
+  }
+  // godunov_unsplit_impl, implementation version 0 (SolverHydroMuscl.cpp:282-405)
+  void godunov(std::vector<double> &in, std::vector<double> &out, double dt_) {
+    boundaries(p, in.data());
+    out = in;
+    for (int k = 0; k < g.ks; ++k)
+      for (int j = 0; j < g.js; ++j)
+        for (int i = 0; i < g.is; ++i) {
+          S5 q;
+          double c;
+          to_prim(p, g.nv, cell(in, i, j, k), c, q);
+          for (int v = 0; v < g.nv; ++v) Q[g.at(i, j, k, v)] = q[v];
+        }
+    const double dtdx = dt_ / p.dx, dtdy = dt_ / p.dy, dtdz = g.dim == 3 ? dt_ / p.dz : 0.0;
+    const double dtd[3] = {dtdx, dtdy, dtdz};
+    const int k0 = g.dim == 3 ? g.gw : 0, k1 = g.dim == 3 ? g.ks - g.gw : 0;
+    // ComputeAndStoreFluxesFunctor: the left face of every cell of [gw, size-gw]
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = k0; k <= k1; ++k)
+      for (int j = g.gw; j <= g.js - g.gw; ++j)
+        for (int i = g.gw; i <= g.is - g.gw; ++i) {
+          S5 dx, dy, dz;
+          cell_slopes(i, j, k, dx, dy, dz);
+          const S5 qloc = cell(Q, i, j, k);
+          for (int dir = 0; dir < g.dim; ++dir) {
+            const int ni = i - (dir == 0), nj = j - (dir == 1), nk = k - (dir == 2);
+            S5 ndx, ndy, ndz;
+            cell_slopes(ni, nj, nk, ndx, ndy, ndz);
+            S5 qright = trace(qloc, dx, dy, dz, dir, 0, dtdx, dtdy, dtdz);
+            S5 qleft = trace(cell(Q, ni, nj, nk), ndx, ndy, ndz, dir, 1, dtdx, dtdy, dtdz);
+            const int nrm = IU + dir;  // rotate the normal velocity into IU
+            std::swap(qleft[IU], qleft[nrm]);
+            std::swap(qright[IU], qright[nrm]);
+            S5 f;
+            solve(p, g.nv, qleft, qright, f);
+            std::swap(f[IU], f[nrm]);  // and back
+            for (int v = 0; v < g.nv; ++v) F[dir][g.at(i, j, k, v)] = f[v] * dtd[dir];
+          }
+        }
+    // UpdateFunctor: +F(i) - F(i+1) per direction, in that order, variable by variable
+    for (int k = k0; k < (g.dim == 3 ? k1 : 1); ++k)
+      for (int j = g.gw; j < g.js - g.gw; ++j)
+        for (int i = g.gw; i < g.is - g.gw; ++i)
+          for (int v = 0; v < g.nv; ++v) {
+            double &o = out[g.at(i, j, k, v)];
+            o += F[0][g.at(i, j, k, v)];
+            o -= F[0][g.at(i + 1, j, k, v)];
+            o += F[1][g.at(i, j, k, v)];
+            o -= F[1][g.at(i, j + 1, k, v)];
+            if (g.dim == 3) {
+              o += F[2][g.at(i, j, k, v)];
+              o -= F[2][g.at(i, j, k + 1, v)];
+            }
+          }
+  }
+  // SolverBase::next_iteration (src/shared/SolverBase.cpp:161-219) with the hydro dt
+  void step(double dt_fixed, int dt_mode) {
+    std::vector<double> &cur = (it % 2 == 0) ? U : U2;
+    std::vector<double> &nxt = (it % 2 == 0) ? U2 : U;
+    if (dt_mode == PPK_DT_CFL) {
+      dt = p.cfl / inv_dt(cur);
+      if (t + dt > p.tEnd) dt = p.tEnd - t;
+    } else {
+      dt = dt_fixed;
+    }
+    godunov(cur, nxt, dt);
+    ++it;
+    t += dt;
+  }
+};
+
+// ---- initial conditions (HydroInitFunctors2D.h:35-230, HydroInitFunctors3D.h) ---------------
+void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
+  const Grid g(p);
+  std::fill(U, U + g.nv * g.vs, 0.0);
+  for (int k = 0; k < g.ks; ++k)
+    for (int j = 0; j < g.js; ++j)
+      for (int i = 0; i < g.is; ++i) {
+        const double x = p.xmin + p.dx / 2 + (i - g.gw) * p.dx;
+        const double y = p.ymin + p.dy / 2 + (j - g.gw) * p.dy;
+        const double z = g.dim == 3 ? p.zmin + p.dz / 2 + (k - g.gw) * p.dz : 0.0;
+        double d, e, u, v, w;
+        if (pr.kind == PPK_PROBLEM_BLAST) {
+          double d2 = (x - pr.blast_center_x) * (x - pr.blast_center_x) +
+                      (y - pr.blast_center_y) * (y - pr.blast_center_y);
+          if (g.dim == 3) d2 += (z - pr.blast_center_z) * (z - pr.blast_center_z);
+          const bool in = d2 < pr.blast_radius * pr.blast_radius;
+          d = in ? pr.blast_density_in : pr.blast_density_out;
+          e = (in ? pr.blast_pressure_in : pr.blast_pressure_out) / (p.gamma0 - 1.0);
+          u = v = w = 0.0;
+        } else {  // implode
+          bool out;
+          if (g.dim == 2)
+            out = pr.implode_shape == 1 ? (x + y * y > 0.5 && x + y * y < 1.5)
+                                        : (x + y > (p.xmin + p.xmax) / 2. + p.ymin);
+          else
+            out = pr.implode_shape == 1 ? (x + y + z > 0.5 && x + y + z < 2.5)
+                                        : (x + y + z > (p.xmin + p.xmax) / 2. + p.ymin + p.zmin);
+          const double *s = out ? pr.implode_out : pr.implode_in;
+          d = s[0];
+          u = s[2];
+          v = s[3];
+          w = s[4];
+          e = g.dim == 2 ? s[1] / (p.gamma0 - 1.0) + 0.5 * d * (u * u + v * v)
+                         : s[1] / (p.gamma0 - 1.0) + 0.5 * d * (u * u + v * v + w * w);
+        }
+        U[g.at(i, j, k, ID)] = d;
+        U[g.at(i, j, k, IP)] = e;
+        U[g.at(i, j, k, IU)] = u;  // the reference stores the velocity, not the momentum
+        U[g.at(i, j, k, IV)] = v;
+        if (g.dim == 3) U[g.at(i, j, k, IW)] = w;
+      }
+}
+
+}  // namespace
+
+extern "C" {
+
+struct hora_solver {
+  HydroSolver s;
+  explicit hora_solver(const ppk_mhd_params &p) : s(p) {}
+};
+
+hora_solver *hora_create(const ppk_mhd_params *p) { return new hora_solver(*p); }
+void hora_destroy(hora_solver *s) { delete s; }
+void hora_init_problem(hora_solver *s, const ppk_mhd_problem *pr) {
+  init(s->s.p, *pr, s->s.U.data());
+  boundaries(s->s.p, s->s.U.data());
+  s->s.U2 = s->s.U;
+  s->s.t = 0.0;
+  s->s.it = 0;
+}
+void hora_init_array(const ppk_mhd_params *p, const ppk_mhd_problem *pr, double *U) {
+  init(*p, *pr, U);
+}
+void hora_set_state(hora_solver *s, const double *U) {
+  std::vector<double> &cur = (s->s.it % 2 == 0) ? s->s.U : s->s.U2;
+  std::memcpy(cur.data(), U, sizeof(double) * cur.size());
+}
+void hora_get_state(hora_solver *s, double *U) {
+  const std::vector<double> &cur = (s->s.it % 2 == 0) ? s->s.U : s->s.U2;
+  std::memcpy(U, cur.data(), sizeof(double) * cur.size());
+}
+void hora_step(hora_solver *s, double dt, int mode) { s->s.step(dt, mode); }
+void hora_get_time(hora_solver *s, double *t, double *dt, long long *it) {
+  *t = s->s.t;
+  *dt = s->s.dt;
+  *it = s->s.it;
+}
+void hora_set_time(hora_solver *s, double t, long long it) {
+  // keep the current data in the current buffer
+  if ((it % 2) != (s->s.it % 2)) std::swap(s->s.U, s->s.U2);
+  s->s.t = t;
+  s->s.it = it;
+}
+void hora_boundaries(const ppk_mhd_params *p, double *U) { boundaries(*p, U); }
+void hora_riemann(const ppk_mhd_params *p, const double *ql, const double *qr, double *f) {
+  const int nv = p->dim == 3 ? 5 : 4;
+  S5 a, b, o;
+  for (int v = 0; v < nv; ++v) {
+    a[v] = ql[v];
+    b[v] = qr[v];
+  }
+  solve(*p, nv, a, b, o);
+  for (int v = 0; v < nv; ++v) f[v] = o[v];
+}
+double hora_inv_dt(hora_solver *s) {
+  const std::vector<double> &cur = (s->s.it % 2 == 0) ? s->s.U : s->s.U2;
+  return s->s.inv_dt(cur);
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+"""TEST INFRASTRUCTURE -- ctypes wrapper of the CPU hydro oracle (oracle/libhydro_oracle.so).
+
+The hydro oracle restates the reference's implementation-0 Euler MUSCL-Hancock step
+(src/muscl/SolverHydroMuscl.cpp:257-405); only tests/ and ``__graft_entry__.smoke()`` may use it,
+and only as the checker.  Arrays are numpy float64 in the reference layout (nbvar, ksize,
+jsize, isize) with nbvar = 4 in 2D (ID, IP, IU, IV) and 5 in 3D (+ IW).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ppkmhd_amd import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = []
+
+
+def build(force=False):
+    """Compile oracle/libhydro_oracle.so with the committed Makefile."""
+    out = os.path.join(_HERE, "libhydro_oracle.so")
+    if force or not os.path.exists(out):
+        cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["libhydro_oracle.so"]
+        subprocess.run(cmd, check=True)
+    return out
+
+
+def _lib():
+    if _LIB:
+        return _LIB[0]
+    path = os.path.join(_HERE, "libhydro_oracle.so")
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    P = C.POINTER(A.Params)
+    PR = C.POINTER(A.Problem)
+    dp = C.POINTER(C.c_double)
+    lib.hora_create.argtypes = [P]
+    lib.hora_create.restype = C.c_void_p
+    lib.hora_destroy.argtypes = [C.c_void_p]
+    lib.hora_init_problem.argtypes = [C.c_void_p, PR]
+    lib.hora_init_array.argtypes = [P, PR, dp]
+    lib.hora_set_state.argtypes = [C.c_void_p, dp]
+    lib.hora_get_state.argtypes = [C.c_void_p, dp]
+    lib.hora_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    lib.hora_get_time.argtypes = [C.c_void_p, dp, dp, C.POINTER(C.c_longlong)]
+    lib.hora_set_time.argtypes = [C.c_void_p, C.c_double, C.c_longlong]
+    lib.hora_boundaries.argtypes = [P, dp]
+    lib.hora_riemann.argtypes = [P, dp, dp, dp]
+    lib.hora_inv_dt.argtypes = [C.c_void_p]
+    lib.hora_inv_dt.restype = C.c_double
+    _LIB.append(lib)
+    return lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def nbvar(p):
+    return 5 if p.dim == 3 else 4
+
+
+def shape(p):
+    g = p.ghost_width
+    ks = p.nz + 2 * g if p.dim == 3 else 1
+    return (nbvar(p), ks, p.ny + 2 * g, p.nx + 2 * g)
+
+
+def zeros(p):
+    return np.zeros(shape(p), dtype=np.float64)
+
+
+def init_array(p, prob):
+    """Initial condition of the interior and ghosts, before any ghost fill."""
+    U = zeros(p)
+    _lib().hora_init_array(C.byref(p), C.byref(prob), _dp(U))
+    return U
+
+
+def fill_boundaries(p, U):
+    _lib().hora_boundaries(C.byref(p), _dp(U))
+    return U
+
+
+def riemann(p, ql, qr):
+    """Flux of the configured solver for primitive states (rho, p, u_normal, v[, w])."""
+    n = nbvar(p)
+    a = np.ascontiguousarray(ql, dtype=np.float64)
+    b = np.ascontiguousarray(qr, dtype=np.float64)
+    assert a.shape == (n,) and b.shape == (n,)
+    f = np.zeros(n)
+    _lib().hora_riemann(C.byref(p), _dp(a), _dp(b), _dp(f))
+    return f
+
+
+class Solver:
+    """The reference's Hydro_Muscl time loop (U/U2 ping-pong, dt from the CFL or fixed)."""
+
+    def __init__(self, p):
+        self.p = p
+        self._h = _lib().hora_create(C.byref(p))
+
+    def close(self):
+        if self._h:
+            _lib().hora_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_problem(self, prob):
+        _lib().hora_init_problem(self._h, C.byref(prob))
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == shape(self.p)
+        _lib().hora_set_state(self._h, _dp(U))
+
+    def get_state(self):
+        U = zeros(self.p)
+        _lib().hora_get_state(self._h, _dp(U))
+        return U
+
+    def step(self, dt=0.0, dt_mode=A.DT_FIXED):
+        _lib().hora_step(self._h, float(dt), int(dt_mode))
+
+    def time(self):
+        t, dt, it = C.c_double(), C.c_double(), C.c_longlong()
+        _lib().hora_get_time(self._h, C.byref(t), C.byref(dt), C.byref(it))
+        return t.value, dt.value, it.value
+
+    def set_time(self, t, iteration):
+        _lib().hora_set_time(self._h, float(t), int(iteration))
+
+    def inv_dt(self):
+        return _lib().hora_inv_dt(self._h)

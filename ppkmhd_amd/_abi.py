@@ -74,13 +74,21 @@ class Params(C.Structure):
         return self.nz + 2 * self.ghost_width if self.dim == 3 else 1
 
     @property
+    def nbvar(self):
+        """8 for MHD (ghost width 3); 4 (2D) or 5 (3D) for the hydro solvers (ghost width 2),
+        as HydroParams::setup assigns them (src/shared/HydroParams.cpp:42-71)."""
+        if self.ghost_width == 2:
+            return 5 if self.dim == 3 else 4
+        return 8
+
+    @property
     def shape(self):
-        """numpy shape of a state array in the reference layout: (8, ksize, jsize, isize)."""
-        return (8, self.ksize, self.jsize, self.isize)
+        """numpy shape of a state array in the reference layout: (nbvar, ksize, jsize, isize)."""
+        return (self.nbvar, self.ksize, self.jsize, self.isize)
 
     @property
     def count(self):
-        return 8 * self.ksize * self.jsize * self.isize
+        return self.nbvar * self.ksize * self.jsize * self.isize
 
     def interior(self):
         """numpy index of the interior cells of a (8, ksize, jsize, isize) array."""

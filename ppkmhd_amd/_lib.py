@@ -89,6 +89,24 @@ SIGNATURES = {
     "ppk_mhd_set_surrogate": (C.c_int, [_H, _H, C.c_double, C.c_double]),
     "ppk_mhd_surrogate_counts": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                            C.c_int]),
+    "ppk_hydro_create": (C.c_int, [_P, C.POINTER(_H)]),
+    "ppk_hydro_destroy": (None, [_H]),
+    "ppk_hydro_last_error": (C.c_char_p, [_H]),
+    "ppk_hydro_nbvar": (C.c_int, [_H]),
+    "ppk_hydro_init_problem": (C.c_int, [_H, _PR]),
+    "ppk_hydro_initial_state": (C.c_int, [_P, _PR, _DP, C.c_size_t]),
+    "ppk_hydro_upload": (C.c_int, [_H, _DP, C.c_size_t]),
+    "ppk_hydro_download": (C.c_int, [_H, _DP, C.c_size_t]),
+    "ppk_hydro_device_state": (C.c_int, [_H, C.POINTER(_DP)]),
+    "ppk_hydro_stream": (C.c_int, [_H, C.POINTER(C.c_void_p)]),
+    "ppk_hydro_make_boundaries": (C.c_int, [_H]),
+    "ppk_hydro_compute_dt": (C.c_int, [_H, _DP]),
+    "ppk_hydro_step": (C.c_int, [_H, C.c_double, C.c_int]),
+    "ppk_hydro_run": (C.c_int, [_H, C.c_int, C.c_double, C.c_int]),
+    "ppk_hydro_get_time": (C.c_int, [_H, _DP, _DP, C.POINTER(C.c_int64)]),
+    "ppk_hydro_set_time": (C.c_int, [_H, C.c_double, C.c_int64]),
+    "ppk_hydro_finished": (C.c_int, [_H, C.POINTER(C.c_int)]),
+    "ppk_hydro_synchronize": (C.c_int, [_H]),
 }
 
 _lib = None

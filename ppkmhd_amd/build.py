@@ -48,6 +48,8 @@ UNITS = [
     ("surrogate.hip", "surrogate_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("surrogate.hip", "surrogate_strict.o",
      ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
+    ("hydro.hip", "hydro_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
+    ("hydro.hip", "hydro_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("mlp.hip", "mlp.o", []),
     ("diag.hip", "diag.o", []),
     ("capi.cpp", "capi.o", ["--offload-host-only"]),
@@ -57,6 +59,7 @@ UNITS = [
     ("io_vtk.cpp", "io_vtk.o", []),
     ("batch_capi.cpp", "batch_capi.o", ["--offload-host-only"]),
     ("mlp_capi.cpp", "mlp_capi.o", ["--offload-host-only"]),
+    ("hydro_capi.cpp", "hydro_capi.o", ["--offload-host-only"]),
 ]
 
 

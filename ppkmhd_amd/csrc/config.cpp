@@ -120,13 +120,21 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
   if (solver_name && name_len > 0) {
     std::snprintf(solver_name, (size_t)name_len, "%s", sname.c_str());
   }
-  if (sname == "MHD_Muscl_2D")
-    P.dim = 2;
-  else if (sname == "MHD_Muscl_3D")
-    P.dim = 3;
-  else
-    P.dim = 0;
+  // HydroParams::setup, src/shared/HydroParams.cpp:42-71: solver name -> dim, ghostWidth
   P.ghost_width = 3;
+  if (sname == "MHD_Muscl_2D") {
+    P.dim = 2;
+  } else if (sname == "MHD_Muscl_3D") {
+    P.dim = 3;
+  } else if (sname == "Hydro_Muscl_2D") {
+    P.dim = 2;
+    P.ghost_width = 2;
+  } else if (sname == "Hydro_Muscl_3D") {
+    P.dim = 3;
+    P.ghost_width = 2;
+  } else {
+    P.dim = 0;
+  }
   // [mesh]
   P.nx = (int)m.get_integer("mesh", "nx", 1);
   P.ny = (int)m.get_integer("mesh", "ny", 1);
@@ -171,7 +179,15 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     else if (prob == "implode") R.kind = PPK_PROBLEM_IMPLODE;
     else if (prob == "kelvin_helmholtz") R.kind = PPK_PROBLEM_KELVIN_HELMHOLTZ;
     else if (prob == "wave") R.kind = PPK_PROBLEM_WAVE;
-    else R.kind = PPK_PROBLEM_ORSZAG_TANG;  // reference falls back to Orszag-Tang (:701-709)
+    // hydro problems of the reference this library does not provide: kind -1, which
+    // ppk_hydro_init_problem rejects (SolverHydroMuscl.cpp:140-168)
+    else if (P.ghost_width == 2 &&
+             (prob == "gresho_vortex" || prob == "four_quadrant" || prob == "isentropic_vortex" ||
+              prob == "rayleigh_taylor" || prob == "rising_bubble" || prob == "disk"))
+      R.kind = -1;
+    // unknown names: the MHD solver falls back to Orszag-Tang (SolverMHDMuscl.h:701-709),
+    // the hydro solver to implode (SolverHydroMuscl.cpp:170-177)
+    else R.kind = P.ghost_width == 2 ? PPK_PROBLEM_IMPLODE : PPK_PROBLEM_ORSZAG_TANG;
     // OrszagTangParams
     R.ot_kt = m.get_float("OrszagTang", "kt", 0.0f);
     R.ot_b0_scale = m.get_float("OrszagTang", "b0_scale", 1.0f);
@@ -235,7 +251,7 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
   }
   if (P.dim == 0) {
     set_thread_error("solver_name '" + sname +
-                     "' is not handled by this library (MHD_Muscl_2D / MHD_Muscl_3D)");
+                     "' is not handled by this library (MHD_Muscl_2D/3D, Hydro_Muscl_2D/3D)");
     return PPK_ERR_CONFIG;
   }
   return PPK_OK;

@@ -1,0 +1,591 @@
+// Euler (hydro) MUSCL-Hancock step of the reference's Hydro_Muscl_2D / Hydro_Muscl_3D,
+// implementation version 0 (src/muscl/SolverHydroMuscl.cpp:257-405):
+//   ghost fill -> primitives -> fluxes of the left faces of [gw, size-gw] -> update.
+// Two forms:
+//  - staged: one kernel per reference stage, primitives and fluxes in HBM (the structural twin
+//    of the reference, kept as the yardstick);
+//  - fused: one marching kernel per step.  A wave owns 64 consecutive x columns (60 of them
+//    output) and walks along y (2D) or z (3D, with 8 waves of a block stacked along y and
+//    exchanging rows through LDS).  The primitives, slopes and Hancock sources of a cell are
+//    computed once and kept in registers while the march passes it, each face is solved once,
+//    the update of a cell is finished one row / plane later (its upper flux), and the CFL
+//    maximum of the new state is folded into the same pass.  Per cell HBM traffic is one read
+//    and one write of the state.
+// Every face state, flux and update uses the reference's expressions in its order, so the
+// strict build (-ffp-contract=off) gives the reference's double results bit for bit.
+//
+// This file is compiled twice: PPK_NS=fast (FMA contraction allowed) and PPK_NS=strict.
+#include <hip/hip_runtime.h>
+
+#include "hydro_kernels.hpp"
+
+namespace ppk {
+namespace PPK_NS {
+
+namespace {
+
+using hyd::HSt;
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
+
+int grid_for(long long n) {
+  long long g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <int NV>
+__device__ __forceinline__ void load(const double *__restrict__ a, long long vs, long long c,
+                                     HSt<NV> &s) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = a[c + v * vs];
+}
+
+template <int NV>
+__device__ __forceinline__ void swap_normal(HSt<NV> &q, int dir) {
+  if (dir == 0) return;
+  const double t = q[IU];
+  q[IU] = q[IU + dir];
+  q[IU + dir] = t;
+}
+
+// CFL term of one cell (ComputeDtFunctor2D/3D, HydroRunFunctors2D.h:46-76, 3D.h:46-81)
+template <int NV>
+__device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &h) {
+  HSt<NV> q;
+  double c;
+  hyd::prims(u, h.P, c, q);
+  const double vx = c + fabs(q[IU]);
+  const double vy = c + fabs(q[IV]);
+  if constexpr (NV == 5) {
+    const double vz = c + fabs(q[IW]);
+    return vx / h.dx + vy / h.dy + vz / h.dz;
+  } else {
+    return vx / h.dx + vy / h.dy;
+  }
+}
+
+// ---- ghost fill, one direction, both faces (MakeBoundariesFunctor2D/3D,
+// src/shared/BoundariesFunctors.h:18-531): Dirichlet reflects and flips the normal velocity,
+// Neumann copies the first interior cell, periodic wraps
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_hbnd(double *__restrict__ U, double *__restrict__ C,
+                                                 HydroDev h, int dir, const double *ctrl) {
+  if (!active(ctrl)) return;
+  const int gw = h.gw;
+  const int n = dir == 0 ? h.nx : (dir == 1 ? h.ny : h.nz);
+  const int s1 = dir == 0 ? h.jsize : h.isize;
+  const int s2 = dir == 2 ? h.jsize : h.ksize;
+  const long long total = 2LL * gw * s1 * s2;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int g = (int)(t % gw);
+    long long r = t / gw;
+    const int t1 = (int)(r % s1);
+    r /= s1;
+    const int t2 = (int)(r % s2);
+    const int upper = (int)(r / s2);
+    const int bc = h.bc[2 * dir + upper];
+    const int c = upper ? g + n + gw : g;
+    int c0;
+    if (bc == PPK_BC_DIRICHLET)
+      c0 = upper ? 2 * n + 2 * gw - 1 - c : 2 * gw - 1 - c;
+    else if (bc == PPK_BC_NEUMANN)
+      c0 = upper ? n + gw - 1 : gw;
+    else
+      c0 = upper ? c - n : n + c;
+    int i, j, k, i0, j0, k0;
+    if (dir == 0) { i = c; j = t1; k = t2; i0 = c0; j0 = j; k0 = k; }
+    else if (dir == 1) { i = t1; j = c; k = t2; i0 = i; j0 = c0; k0 = k; }
+    else { i = t1; j = t2; k = c; i0 = i; j0 = j; k0 = c0; }
+    const long long dst = ((long long)k * h.jsize + j) * h.isize + i;
+    const long long src = ((long long)k0 * h.jsize + j0) * h.isize + i0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double sign = (bc == PPK_BC_DIRICHLET && v == IU + dir) ? -1.0 : 1.0;
+      const double x = U[src + v * h.vstride] * sign;
+      U[dst + v * h.vstride] = x;
+      if (C) C[dst + v * h.vstride] = x;
+    }
+  }
+}
+
+// ---- primitives of every cell (ConvertToPrimitivesFunctor, SolverHydroMuscl.h) and the CFL
+// maximum over the interior, one partial per block
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_hprim(const double *__restrict__ U,
+                                                  double *__restrict__ Q, HydroDev h,
+                                                  double *__restrict__ parts, const double *ctrl,
+                                                  int do_dt) {
+  if (!active(ctrl)) return;
+  __shared__ double red[kBlock / 64];
+  const long long total = (long long)h.isize * h.jsize * h.ksize;
+  double best = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    HSt<NV> u, q;
+    load(U, h.vstride, t, u);
+    double c;
+    hyd::prims(u, h.P, c, q);
+    if (Q) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) Q[t + v * h.vstride] = q[v];
+    }
+    if (do_dt) {
+      const int i = (int)(t % h.isize);
+      const long long r = t / h.isize;
+      const int j = (int)(r % h.jsize);
+      const int k = (int)(r / h.jsize);
+      const int g = h.gw;
+      const bool inner = i >= g && i < h.isize - g && j >= g && j < h.jsize - g &&
+                         (NV == 4 || (k >= g && k < h.ksize - g));
+      if (inner) {
+        const double vx = c + fabs(q[IU]);
+        const double vy = c + fabs(q[IV]);
+        double w;
+        if constexpr (NV == 5) {
+          const double vz = c + fabs(q[IW]);
+          w = vx / h.dx + vy / h.dy + vz / h.dz;
+        } else {
+          w = vx / h.dx + vy / h.dy;
+        }
+        best = fmax(best, w);
+      }
+    }
+  }
+  if (!do_dt) return;
+  best = block_max(best, red);
+  if (threadIdx.x == 0) parts[blockIdx.x] = best;
+}
+
+// ---- staged fluxes (ComputeAndStoreFluxesFunctor2D/3D, HydroRunFunctors2D.h:490-750,
+// HydroRunFunctors3D.h:300-720): for every cell of [gw, size-gw] the flux through its left
+// face in each direction, slopes of the cell and of its left neighbour recomputed from Q,
+// the normal velocity rotated into IU for the solver and back, times dt/dx
+template <int DIM, int NV>
+__device__ __forceinline__ void cell_slopes(const double *__restrict__ Q, const HydroDev &h,
+                                            long long c, HSt<NV> &q, HSt<NV> &dX, HSt<NV> &dY,
+                                            HSt<NV> &dZ) {
+  HSt<NV> qpx, qmx, qpy, qmy, qpz, qmz;
+  load(Q, h.vstride, c, q);
+  load(Q, h.vstride, c + 1, qpx);
+  load(Q, h.vstride, c - 1, qmx);
+  load(Q, h.vstride, c + h.isize, qpy);
+  load(Q, h.vstride, c - h.isize, qmy);
+  if constexpr (DIM == 3) {
+    load(Q, h.vstride, c + h.plane, qpz);
+    load(Q, h.vstride, c - h.plane, qmz);
+  } else {
+    qpz = q;
+    qmz = q;
+  }
+  hyd::slopes(q, qpx, qmx, qpy, qmy, qpz, qmz, h.P, dX, dY, dZ);
+}
+
+template <int DIM, int NV>
+__global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
+                                                  double *__restrict__ F, HydroDev h,
+                                                  const double *ctrl) {
+  if (!active(ctrl)) return;
+  const double dt = ctrl[CTRL_DT];
+  const double dtd[3] = {dt / h.dx, dt / h.dy, DIM == 3 ? dt / h.dz : 0.0};
+  const int nsx = h.nx + 1, nsy = h.ny + 1, nsz = DIM == 3 ? h.nz + 1 : 1;
+  const long long total = (long long)nsx * nsy * nsz;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = h.gw + (int)(t % nsx);
+    const long long r = t / nsx;
+    const int j = h.gw + (int)(r % nsy);
+    const int k = DIM == 3 ? h.gw + (int)(r / nsy) : 0;
+    const long long c = ((long long)k * h.jsize + j) * h.isize + i;
+    HSt<NV> q, dX, dY, dZ, s0;
+    cell_slopes<DIM>(Q, h, c, q, dX, dY, dZ);
+    hyd::sources(q, dX, dY, dZ, h.P.gamma0, s0);
+#pragma unroll 1
+    for (int d = 0; d < DIM; ++d) {
+      const long long st = d == 0 ? 1 : (d == 1 ? (long long)h.isize : h.plane);
+      HSt<NV> qn, nX, nY, nZ, ns0;
+      cell_slopes<DIM>(Q, h, c - st, qn, nX, nY, nZ);
+      hyd::sources(qn, nX, nY, nZ, h.P.gamma0, ns0);
+      HSt<NV> qr, ql, f;
+      hyd::face_state(q, d == 0 ? dX : (d == 1 ? dY : dZ), s0, dtd[d], false, h.P.smallr, qr);
+      hyd::face_state(qn, d == 0 ? nX : (d == 1 ? nY : nZ), ns0, dtd[d], true, h.P.smallr, ql);
+      swap_normal(ql, d);
+      swap_normal(qr, d);
+      hyd::riemann(ql, qr, f, h.P);
+      swap_normal(f, d);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) F[(long long)(d * NV + v) * h.vstride + c] = f[v] * dtd[d];
+    }
+  }
+}
+
+// ---- staged update (UpdateFunctor2D/3D, HydroRunFunctors2D.h:776-840, 3D.h:740-815)
+template <int DIM, int NV>
+__global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
+                                                    const double *__restrict__ F, HydroDev h,
+                                                    const double *ctrl) {
+  if (!active(ctrl)) return;
+  const int nz = DIM == 3 ? h.nz : 1;
+  const long long total = (long long)h.nx * h.ny * nz;
+  const long long vs = h.vstride;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = h.gw + (int)(t % h.nx);
+    const long long r = t / h.nx;
+    const int j = h.gw + (int)(r % h.ny);
+    const int k = DIM == 3 ? h.gw + (int)(r / h.ny) : 0;
+    const long long c = ((long long)k * h.jsize + j) * h.isize + i;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double *fx = F + (long long)v * vs;
+      const double *fy = F + (long long)(NV + v) * vs;
+      double o = U[c + v * vs];
+      o += fx[c];
+      o -= fx[c + 1];
+      o += fy[c];
+      o -= fy[c + h.isize];
+      if constexpr (DIM == 3) {
+        const double *fz = F + (long long)(2 * NV + v) * vs;
+        o += fz[c];
+        o -= fz[c + h.plane];
+      }
+      U[c + v * vs] = o;
+    }
+  }
+}
+
+// ---- fused 2D: one wave per strip of 60 output columns x strip_rows output rows ----------
+// Lane l holds column i = gw + 60 * sx + l - 2; lanes 2..61 are the strip's output.  The wave
+// walks the slope rows s = r0-1 .. r1: at row s it has Q(s-1), Q(s), Q(s+1) in registers, the
+// x neighbours' Q from lanes l -/+ 1, computes slopes and sources of (i, s), solves the y face
+// below row s (upper state of row s-1 kept from the previous row), the x face left of (i, s)
+// (left state from lane l-1), and finishes the update of (i, s-1) with the flux of that y face.
+constexpr int kWaves2 = 4;
+
+template <int NV>
+__global__ void __launch_bounds__(kWaves2 * 64) k_hfused2d(const double *__restrict__ Uin,
+                                                           double *__restrict__ Uout, HydroDev h,
+                                                           double *__restrict__ parts,
+                                                           const double *ctrl, int fold_cfl) {
+  if (!active(ctrl)) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * kWaves2 + (threadIdx.x >> 6);
+  const int nsx = (h.nx + kHydroCols - 1) / kHydroCols;
+  const int H = h.strip_rows;
+  const int nsy = (h.ny + H - 1) / H;
+  if (wave >= nsx * nsy) return;  // whole waves only
+  const int sx = wave % nsx, sy = wave / nsx;
+  const int i = h.gw + sx * kHydroCols + lane - 2;
+  const int ic = i < h.isize ? i : h.isize - 1;  // lanes past the box: clamped, never output
+  const int r0 = h.gw + sy * H;
+  const int r1 = min(r0 + H, h.gw + h.ny);
+  const bool out_lane = lane >= 2 && lane < 2 + kHydroCols && i < h.gw + h.nx;
+  const long long vs = h.vstride;
+  const double dt = ctrl[CTRL_DT];
+  const double dtdx = dt / h.dx, dtdy = dt / h.dy;
+  const hyd::HydroPar &P = h.P;
+
+  HSt<NV> qm, q0, u0, t;
+  double c;
+  load(Uin, vs, (long long)(r0 - 2) * h.isize + ic, t);
+  hyd::prims(t, P, c, qm);
+  load(Uin, vs, (long long)(r0 - 1) * h.isize + ic, u0);
+  hyd::prims(u0, P, c, q0);
+  HSt<NV> ymax_prev, o_prev;
+  double best = 0.0;
+#pragma unroll 1
+  for (int s = r0 - 1; s <= r1; ++s) {
+    HSt<NV> un, qp;
+    load(Uin, vs, (long long)(s + 1) * h.isize + ic, un);
+    hyd::prims(un, P, c, qp);
+    HSt<NV> qxm, qxp, dX, dY, dZ, s0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      qxm[v] = __shfl_up(q0[v], 1, 64);
+      qxp[v] = __shfl_down(q0[v], 1, 64);
+    }
+    hyd::slopes(q0, qxp, qxm, qp, qm, q0, q0, P, dX, dY, dZ);
+    hyd::sources(q0, dX, dY, dZ, P.gamma0, s0);
+    if (s >= r0) {
+      HSt<NV> ql = ymax_prev, qr, fy;
+      hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
+      swap_normal(ql, 1);
+      swap_normal(qr, 1);
+      hyd::riemann(ql, qr, fy, P);
+      swap_normal(fy, 1);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fy[v] = fy[v] * dtdy;
+      if (s > r0 && out_lane) {  // finish (i, s-1): its upper y face is this one
+        const long long cp = (long long)(s - 1) * h.isize + i;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          o_prev[v] -= fy[v];
+          Uout[cp + v * vs] = o_prev[v];
+        }
+        if (fold_cfl) best = fmax(best, inv_dt_cell(o_prev, h));
+      }
+      if (s < r1) {
+        HSt<NV> xmin, xmax, xl, fx;
+        hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
+        hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
+        hyd::riemann(xl, xmin, fx, P);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          fx[v] = fx[v] * dtdx;
+          const double fxr = __shfl_down(fx[v], 1, 64);
+          double o = u0[v];
+          o += fx[v];
+          o -= fxr;
+          o += fy[v];
+          o_prev[v] = o;
+        }
+      }
+    }
+    hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax_prev);
+    qm = q0;
+    q0 = qp;
+    u0 = un;
+  }
+  if (fold_cfl) {
+    for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off, 64));
+    if (lane == 0) parts[wave] = best;
+  }
+}
+
+// ---- fused 3D: a block of 8 waves stacked along y marches along z -----------------------
+// Wave w holds row j = gw + 4 * by + w - 2, lane l column i = gw + 60 * bx + l - 2; rows 2..5
+// and lanes 2..61 are the block's output.  At slope plane s each thread keeps Q(s-1), Q(s),
+// Q(s+1) of its column in registers; plane s of Q goes through LDS for the x and y
+// neighbours, the upper y-face states of each row through LDS to the row above, and the y
+// fluxes back through LDS to the row below.  The z face below plane s uses the upper z state
+// of plane s-1 kept in registers, so the update of (i, j, s-1) completes at plane s.
+constexpr int kW3 = kHydroWaves3;
+
+template <int NV>
+__global__ void __launch_bounds__(kW3 * 64) k_hfused3d(const double *__restrict__ Uin,
+                                                       double *__restrict__ Uout, HydroDev h,
+                                                       double *__restrict__ parts,
+                                                       const double *ctrl, int fold_cfl) {
+  __shared__ double sQ[2][NV][kW3][64];  // Q of plane s (parity s & 1), later the y fluxes
+  __shared__ double sT[NV][kW3][64];     // upper y-face states of plane s
+  __shared__ double red[kW3];
+  if (!active(ctrl)) return;
+  const int lane = threadIdx.x & 63;
+  const int ly = threadIdx.x >> 6;
+  const int nsx = (h.nx + kHydroCols - 1) / kHydroCols;
+  const int nsy = (h.ny + kHydroRows3 - 1) / kHydroRows3;
+  const int KH = h.strip_rows;
+  const int bx = blockIdx.x % nsx;
+  const int by = (blockIdx.x / nsx) % nsy;
+  const int bz = blockIdx.x / (nsx * nsy);
+  const int i = h.gw + bx * kHydroCols + lane - 2;
+  const int j = h.gw + by * kHydroRows3 + ly - 2;
+  const int ic = i < h.isize ? i : h.isize - 1;  // clamped halo, never output
+  const int jc = j < h.jsize ? j : h.jsize - 1;
+  const int k0 = h.gw + bz * KH;
+  const int k1 = min(k0 + KH, h.gw + h.nz);
+  const bool slope_row = ly >= 1 && ly <= kW3 - 2;
+  const bool yface_row = ly >= 2 && ly <= kW3 - 2;
+  const bool out_row = ly >= 2 && ly < 2 + kHydroRows3;
+  const bool out = out_row && lane >= 2 && lane < 2 + kHydroCols && i < h.gw + h.nx &&
+                   j < h.gw + h.ny;
+  const long long vs = h.vstride;
+  const long long col = (long long)jc * h.isize + ic;
+  const double dt = ctrl[CTRL_DT];
+  const double dtdx = dt / h.dx, dtdy = dt / h.dy, dtdz = dt / h.dz;
+  const hyd::HydroPar &P = h.P;
+  const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63;
+
+  HSt<NV> qm, q0, u0, t;
+  double c;
+  load(Uin, vs, (long long)(k0 - 2) * h.plane + col, t);
+  hyd::prims(t, P, c, qm);
+  load(Uin, vs, (long long)(k0 - 1) * h.plane + col, u0);
+  hyd::prims(u0, P, c, q0);
+  HSt<NV> zmax_prev, o_prev;
+  double best = 0.0;
+#pragma unroll 1
+  for (int s = k0 - 1; s <= k1; ++s) {
+    const int b = s & 1;
+    HSt<NV> un, qp;
+    load(Uin, vs, (long long)(s + 1) * h.plane + col, un);
+    hyd::prims(un, P, c, qp);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sQ[b][v][ly][lane] = q0[v];
+    __syncthreads();
+    HSt<NV> dX, dY, dZ, s0;
+    if (slope_row) {
+      HSt<NV> qxm, qxp, qym, qyp;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        qxm[v] = sQ[b][v][ly][lm];
+        qxp[v] = sQ[b][v][ly][lp];
+        qym[v] = sQ[b][v][ly - 1][lane];
+        qyp[v] = sQ[b][v][ly + 1][lane];
+      }
+      hyd::slopes(q0, qxp, qxm, qyp, qym, qp, qm, P, dX, dY, dZ);
+      hyd::sources(q0, dX, dY, dZ, P.gamma0, s0);
+      HSt<NV> ymax;
+      hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sT[v][ly][lane] = ymax[v];
+    }
+    __syncthreads();
+    const bool plane_out = s >= k0 && s < k1;
+    HSt<NV> fx, fz;
+    if (s >= k0 && out_row) {  // z face below plane s (also the top face of plane k1 - 1)
+      HSt<NV> ql = zmax_prev, qr;
+      hyd::face_state(q0, dZ, s0, dtdz, false, P.smallr, qr);
+      swap_normal(ql, 2);
+      swap_normal(qr, 2);
+      hyd::riemann(ql, qr, fz, P);
+      swap_normal(fz, 2);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fz[v] = fz[v] * dtdz;
+    }
+    if (plane_out && yface_row) {  // y face below row j, flux into the Q slot of plane s
+      HSt<NV> ql, qr, fy;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ql[v] = sT[v][ly - 1][lane];
+      hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
+      swap_normal(ql, 1);
+      swap_normal(qr, 1);
+      hyd::riemann(ql, qr, fy, P);
+      swap_normal(fy, 1);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sQ[b][v][ly][lane] = fy[v] * dtdy;
+    }
+    if (plane_out && out_row) {  // x face left of column i
+      HSt<NV> xmin, xmax, xl;
+      hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
+      hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
+      hyd::riemann(xl, xmin, fx, P);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fx[v] = fx[v] * dtdx;
+    }
+    __syncthreads();
+    if (out_row && s >= k0) {
+      if (s > k0 && out) {  // finish (i, j, s-1): its upper z face is this one
+        const long long cp = (long long)(s - 1) * h.plane + (long long)j * h.isize + i;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          o_prev[v] -= fz[v];
+          Uout[cp + v * vs] = o_prev[v];
+        }
+        if (fold_cfl) best = fmax(best, inv_dt_cell(o_prev, h));
+      }
+      if (plane_out) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const double fxr = __shfl_down(fx[v], 1, 64);
+          double o = u0[v];
+          o += fx[v];
+          o -= fxr;
+          o += sQ[b][v][ly][lane];
+          o -= sQ[b][v][ly + 1][lane];
+          o += fz[v];
+          o_prev[v] = o;
+        }
+      }
+    }
+    if (slope_row) hyd::face_state(q0, dZ, s0, dtdz, true, P.smallr, zmax_prev);
+    qm = q0;
+    q0 = qp;
+    u0 = un;
+  }
+  if (fold_cfl) {
+    best = block_max(best, red);
+    if (threadIdx.x == 0) parts[blockIdx.x] = best;
+  }
+}
+
+int fused2d_waves(const HydroDev &h) {
+  const int nsx = (h.nx + kHydroCols - 1) / kHydroCols;
+  const int nsy = (h.ny + h.strip_rows - 1) / h.strip_rows;
+  return nsx * nsy;
+}
+int fused3d_blocks(const HydroDev &h) {
+  const int nsx = (h.nx + kHydroCols - 1) / kHydroCols;
+  const int nsy = (h.ny + kHydroRows3 - 1) / kHydroRows3;
+  const int nsz = (h.nz + h.strip_rows - 1) / h.strip_rows;
+  return nsx * nsy * nsz;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------------------------
+// launchers
+// -------------------------------------------------------------------------------------------
+hipError_t hydro_boundaries(const HydroDev &h, double *U, double *copy, const double *ctrl,
+                            hipStream_t s) {
+  for (int dir = 0; dir < h.dim; ++dir) {
+    const int s1 = dir == 0 ? h.jsize : h.isize;
+    const int s2 = dir == 2 ? h.jsize : h.ksize;
+    const long long n = 2LL * h.gw * s1 * s2;
+    if (h.dim == 3)
+      hipLaunchKernelGGL(k_hbnd<5>, dim3(grid_for(n)), dim3(kBlock), 0, s, U, copy, h, dir, ctrl);
+    else
+      hipLaunchKernelGGL(k_hbnd<4>, dim3(grid_for(n)), dim3(kBlock), 0, s, U, copy, h, dir, ctrl);
+  }
+  return hipGetLastError();
+}
+
+int hydro_prim_parts(const HydroDev &h) {
+  return grid_for((long long)h.isize * h.jsize * h.ksize) < 4096
+             ? grid_for((long long)h.isize * h.jsize * h.ksize)
+             : 4096;
+}
+
+hipError_t hydro_prim_dt(const HydroDev &h, const double *U, double *Q, double *parts,
+                         const double *ctrl, int do_dt, hipStream_t s) {
+  const dim3 g(hydro_prim_parts(h)), b(kBlock);
+  if (h.dim == 3)
+    hipLaunchKernelGGL(k_hprim<5>, g, b, 0, s, U, Q, h, parts, ctrl, do_dt);
+  else
+    hipLaunchKernelGGL(k_hprim<4>, g, b, 0, s, U, Q, h, parts, ctrl, do_dt);
+  return hipGetLastError();
+}
+
+hipError_t hydro_staged(const HydroDev &h, const double *Q, double *F, double *Uout,
+                        const double *ctrl, hipStream_t s) {
+  const dim3 b(kBlock);
+  if (h.dim == 3) {
+    const long long nf = (long long)(h.nx + 1) * (h.ny + 1) * (h.nz + 1);
+    const long long nu = (long long)h.nx * h.ny * h.nz;
+    hipLaunchKernelGGL((k_hflux<3, 5>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
+    hipLaunchKernelGGL((k_hupdate<3, 5>), dim3(grid_for(nu)), b, 0, s, Uout, F, h, ctrl);
+  } else {
+    const long long nf = (long long)(h.nx + 1) * (h.ny + 1);
+    const long long nu = (long long)h.nx * h.ny;
+    hipLaunchKernelGGL((k_hflux<2, 4>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
+    hipLaunchKernelGGL((k_hupdate<2, 4>), dim3(grid_for(nu)), b, 0, s, Uout, F, h, ctrl);
+  }
+  return hipGetLastError();
+}
+
+int hydro_fused_parts(const HydroDev &h) {
+  return h.dim == 3 ? fused3d_blocks(h) : fused2d_waves(h);
+}
+
+hipError_t hydro_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
+                       const double *ctrl, int fold_cfl, hipStream_t s) {
+  if (h.dim == 3) {
+    hipLaunchKernelGGL(k_hfused3d<5>, dim3(fused3d_blocks(h)), dim3(kW3 * 64), 0, s, Uin, Uout,
+                       h, parts, ctrl, fold_cfl);
+  } else {
+    const int waves = fused2d_waves(h);
+    hipLaunchKernelGGL(k_hfused2d<4>, dim3((waves + kWaves2 - 1) / kWaves2), dim3(kWaves2 * 64),
+                       0, s, Uin, Uout, h, parts, ctrl, fold_cfl);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace PPK_NS
+}  // namespace ppk

@@ -1,0 +1,496 @@
+// C ABI of the Euler (hydro) solver: ppk_hydro_* (include/ppkmhd_amd.h).  Same handle model
+// and device-resident time loop as the MHD solver (ctrl block, per-block CFL partial maxima),
+// with the hydro kernels of hydro.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "hydro_kernels.hpp"
+#include "ppkmhd_amd.h"
+
+namespace ppk {
+namespace {
+
+const HydroTable kHydroFast = {fast::hydro_boundaries, fast::hydro_prim_dt,
+                               fast::hydro_prim_parts, fast::hydro_staged,
+                               fast::hydro_fused,      fast::hydro_fused_parts};
+const HydroTable kHydroStrict = {strict::hydro_boundaries, strict::hydro_prim_dt,
+                                 strict::hydro_prim_parts, strict::hydro_staged,
+                                 strict::hydro_fused,      strict::hydro_fused_parts};
+
+int validate_hydro(const ppk_mhd_params *p, std::string &why) {
+  if (!p) { why = "params is NULL"; return PPK_ERR_INVALID_ARGUMENT; }
+  if (p->dim != 2 && p->dim != 3) { why = "dim must be 2 or 3"; return PPK_ERR_INVALID_ARGUMENT; }
+  if (p->ghost_width != 2) {
+    why = "ghost_width must be 2 for the hydro solvers (HydroParams.cpp:42-55)";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2)) {
+    why = "nx, ny (and nz in 3D) must be >= ghost_width";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->dim == 2 && p->nz != 1) { why = "nz must be 1 in 2D"; return PPK_ERR_INVALID_ARGUMENT; }
+  const long long cells = (long long)(p->nx + 4) * (p->ny + 4) * (p->dim == 3 ? p->nz + 4 : 1);
+  if (cells > (1LL << 31)) { why = "grid too large (more than 2^31 padded cells)"; return PPK_ERR_INVALID_ARGUMENT; }
+  // riemann_hydro (RiemannSolvers.h:573-633): approx, hll (2D only), hllc, llf
+  const bool ok_riemann = p->riemann == PPK_RIEMANN_APPROX || p->riemann == PPK_RIEMANN_LLF ||
+                          p->riemann == PPK_RIEMANN_HLLC ||
+                          (p->riemann == PPK_RIEMANN_HLL && p->dim == 2);
+  if (!ok_riemann) {
+    why = p->riemann == PPK_RIEMANN_HLL
+              ? "riemann hll has no 3D hydro solver in the reference (RiemannSolvers.h:609-633)"
+              : "riemann must be approx, llf, hllc (or hll in 2D) for the hydro solvers";
+    return PPK_ERR_UNSUPPORTED;
+  }
+  if (p->slope_type != 0.0 && p->slope_type != 1.0 && p->slope_type != 2.0) {
+    why = "slope_type must be 0, 1 or 2 (HydroBaseFunctor2D.h:470-510)";
+    return PPK_ERR_UNSUPPORTED;
+  }
+  const int nf = p->dim == 3 ? 6 : 4;
+  for (int f = 0; f < nf; ++f)
+    if (p->bc[f] != PPK_BC_DIRICHLET && p->bc[f] != PPK_BC_NEUMANN && p->bc[f] != PPK_BC_PERIODIC) {
+      why = "boundary types must be 1 (Dirichlet), 2 (Neumann) or 3 (periodic)";
+      return PPK_ERR_INVALID_ARGUMENT;
+    }
+  if (p->precision != PPK_FP64 && p->precision != PPK_FP64_STRICT) {
+    why = p->precision == PPK_FP32_BULK ? "the hydro solvers compute in FP64 only"
+                                        : "unknown precision";
+    return p->precision == PPK_FP32_BULK ? PPK_ERR_UNSUPPORTED : PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->kernel_path < PPK_PATH_AUTO || p->kernel_path > PPK_PATH_FUSED) {
+    why = "unknown kernel_path";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (!(p->gamma0 > 1.0) || !(p->dx > 0) || !(p->dy > 0) || (p->dim == 3 && !(p->dz > 0))) {
+    why = "gamma0 must be > 1 and dx, dy, dz > 0 (call ppk_mhd_params_init)";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  return PPK_OK;
+}
+
+HydroDev make_hydro_dev(const ppk_mhd_params &p) {
+  HydroDev d{};
+  d.dim = p.dim;
+  d.nv = p.dim == 3 ? 5 : 4;
+  d.nx = p.nx;
+  d.ny = p.ny;
+  d.nz = p.dim == 3 ? p.nz : 1;
+  d.gw = p.ghost_width;
+  d.isize = p.nx + 2 * d.gw;
+  d.jsize = p.ny + 2 * d.gw;
+  d.ksize = p.dim == 3 ? p.nz + 2 * d.gw : 1;
+  for (int f = 0; f < 6; ++f) d.bc[f] = p.bc[f];
+  d.plane = (long long)d.isize * d.jsize;
+  d.vstride = d.plane * d.ksize;
+  d.dx = p.dx;
+  d.dy = p.dy;
+  d.dz = p.dz;
+  d.P.gamma0 = p.gamma0;
+  d.P.gamma6 = p.gamma6;
+  d.P.smallr = p.smallr;
+  d.P.smallc = p.smallc;
+  d.P.smallp = p.smallp;
+  d.P.smallpp = p.smallpp;
+  d.P.slope_type = p.slope_type;
+  d.P.riemann = p.riemann;
+  // tuning knob: output rows (2D) / planes (3D) of one marching wave / block
+  const char *strip = std::getenv("PPKMHD_HYDRO_STRIP");
+  const int rows = strip ? std::atoi(strip) : 0;
+  d.strip_rows = rows > 0 ? rows : 64;
+  return d;
+}
+
+// initial conditions of the hydro problems in the reference layout (host)
+// InitImplodeFunctor2D/3D (src/muscl/HydroInitFunctors2D.h:35-125, HydroInitFunctors3D.h:66-165),
+// InitBlastFunctor2D/3D (HydroInitFunctors2D.h:129-212, HydroInitFunctors3D.h:170-262)
+void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
+  const int gw = p.ghost_width, dim = p.dim, nv = dim == 3 ? 5 : 4;
+  const int is = p.nx + 2 * gw, js = p.ny + 2 * gw, ks = dim == 3 ? p.nz + 2 * gw : 1;
+  const long long vs = (long long)is * js * ks;
+  std::fill(U, U + nv * vs, 0.0);
+  const double g0 = p.gamma0;
+  for (int k = 0; k < ks; ++k)
+    for (int j = 0; j < js; ++j)
+      for (int i = 0; i < is; ++i) {
+        const long long c = ((long long)k * js + j) * is + i;
+        const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx;
+        const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy;
+        const double z = dim == 3 ? p.zmin + p.dz / 2 + (k - gw) * p.dz : 0.0;
+        double s[5];
+        if (pr.kind == PPK_PROBLEM_BLAST) {
+          double d2 = (x - pr.blast_center_x) * (x - pr.blast_center_x) +
+                      (y - pr.blast_center_y) * (y - pr.blast_center_y);
+          if (dim == 3) d2 = d2 + (z - pr.blast_center_z) * (z - pr.blast_center_z);
+          const bool in = d2 < pr.blast_radius * pr.blast_radius;
+          s[ID] = in ? pr.blast_density_in : pr.blast_density_out;
+          s[IP] = (in ? pr.blast_pressure_in : pr.blast_pressure_out) / (g0 - 1.0);
+          s[IU] = s[IV] = s[IW] = 0.0;
+        } else {
+          bool outer;
+          if (dim == 2)
+            outer = pr.implode_shape == 1 ? (x + y * y > 0.5 && x + y * y < 1.5)
+                                          : (x + y > (p.xmin + p.xmax) / 2. + p.ymin);
+          else
+            outer = pr.implode_shape == 1
+                        ? (x + y + z > 0.5 && x + y + z < 2.5)
+                        : (x + y + z > (p.xmin + p.xmax) / 2. + p.ymin + p.zmin);
+          const double *st = outer ? pr.implode_out : pr.implode_in;  // rho, p, u, v, w
+          const double rho = st[0], pre = st[1], u = st[2], v = st[3], w = st[4];
+          s[ID] = rho;
+          s[IP] = dim == 2 ? pre / (g0 - 1.0) + 0.5 * rho * (u * u + v * v)
+                           : pre / (g0 - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+          // the reference stores the velocities in the momentum slots
+          s[IU] = u;
+          s[IV] = v;
+          s[IW] = w;
+        }
+        for (int n = 0; n < nv; ++n) U[c + n * vs] = s[n];
+      }
+}
+
+}  // namespace
+}  // namespace ppk
+
+struct ppk_hydro_solver {
+  ppk_mhd_params p{};
+  ppk::HydroDev hd{};
+  hipStream_t stream = nullptr;
+  double *U[2] = {nullptr, nullptr};
+  double *Q = nullptr;  // staged form: primitives
+  double *F = nullptr;  // staged form: dim x nv flux planes
+  double *ctrl = nullptr;
+  double *ctrl_host = nullptr;
+  double *parts = nullptr;
+  bool fused = false;
+  bool cfl_valid = false;  // parts holds the fused kernel's CFL maxima of cur()
+  const ppk::KernelTable *K = nullptr;
+  const ppk::HydroTable *H = nullptr;
+  long long iter_host = 0;
+  std::string err;
+
+  double *cur() { return U[iter_host & 1]; }
+  double *nxt() { return U[(iter_host + 1) & 1]; }
+  long long cells() const { return (long long)hd.isize * hd.jsize * hd.ksize; }
+  size_t values() const { return (size_t)cells() * hd.nv; }
+  size_t state_bytes() const { return values() * sizeof(double); }
+};
+
+namespace {
+
+int hfail(ppk_hydro_t *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    ppk::set_thread_error(msg);
+  return code;
+}
+
+#define HHIP(h, expr)                                                                     \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return hfail((h), PPK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+  } while (0)
+
+int hwrite_ctrl(ppk_hydro_t *h) {
+  HHIP(h, hipMemcpyAsync(h->ctrl, h->ctrl_host, sizeof(double) * ppk::CTRL_SLOTS,
+                         hipMemcpyHostToDevice, h->stream));
+  HHIP(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int hread_ctrl(ppk_hydro_t *h) {
+  HHIP(h, hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(double) * ppk::CTRL_SLOTS,
+                         hipMemcpyDeviceToHost, h->stream));
+  HHIP(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+void hfree_all(ppk_hydro_t *h) {
+  for (void *ptr : {(void *)h->U[0], (void *)h->U[1], (void *)h->Q, (void *)h->F,
+                    (void *)h->ctrl, (void *)h->parts})
+    if (ptr) (void)hipFree(ptr);
+  if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// SolverBase::next_iteration -> SolverHydroMuscl::godunov_unsplit_impl, one step cur -> nxt
+int hlaunch_step(ppk_hydro_t *h, double dt, int dt_mode, int check_finished) {
+  const ppk::KernelTable &K = *h->K;
+  const ppk::HydroTable &H = *h->H;
+  const ppk::HydroDev &d = h->hd;
+  hipStream_t s = h->stream;
+  double *in = h->cur(), *out = h->nxt();
+  const bool cfl = dt_mode == PPK_DT_CFL;
+  const int mode = cfl ? ppk::SETDT_CFL : ppk::SETDT_FIXED;
+  HHIP(h, K.begin(h->ctrl, check_finished, s));
+  if (h->fused) {
+    HHIP(h, H.boundaries(d, in, out, h->ctrl, s));
+    int nparts = H.fused_parts(d);
+    if (cfl && !h->cfl_valid) {
+      HHIP(h, H.prim_dt(d, in, nullptr, h->parts, h->ctrl, 1, s));
+      nparts = H.prim_parts(d);
+    }
+    HHIP(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s));
+    HHIP(h, H.fused(d, in, out, h->parts, h->ctrl, cfl ? 1 : 0, s));
+    h->cfl_valid = cfl;
+  } else {
+    HHIP(h, H.boundaries(d, in, nullptr, h->ctrl, s));
+    HHIP(h, H.prim_dt(d, in, h->Q, h->parts, h->ctrl, cfl ? 1 : 0, s));
+    HHIP(h, K.set_dt(h->ctrl, h->parts, H.prim_parts(d), dt, mode, h->p.cfl, s));
+    HHIP(h, hipMemcpyAsync(out, in, h->state_bytes(), hipMemcpyDeviceToDevice, s));
+    HHIP(h, H.staged(d, h->Q, h->F, out, h->ctrl, s));
+  }
+  HHIP(h, K.end(h->ctrl, s));
+  return PPK_OK;
+}
+
+// the rest of the constructor after init(U): make_boundaries(U), U2 = U, compute_dt
+// (src/muscl/SolverHydroMuscl.h:289-300)
+int hfinish_init(ppk_hydro_t *h) {
+  h->ctrl_host[ppk::CTRL_T] = 0.0;
+  h->ctrl_host[ppk::CTRL_ITER] = 0.0;
+  h->ctrl_host[ppk::CTRL_ACTIVE] = 1.0;
+  int rc = hwrite_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  rc = ppk_hydro_make_boundaries(h);
+  if (rc != PPK_OK) return rc;
+  double dt = 0.0;
+  rc = ppk_hydro_compute_dt(h, &dt);
+  if (rc != PPK_OK) return rc;
+  HHIP(h, hipMemcpyAsync(h->nxt(), h->cur(), h->state_bytes(), hipMemcpyDeviceToDevice,
+                         h->stream));
+  h->ctrl_host[ppk::CTRL_DT] = (0.0 + dt > h->p.tEnd) ? h->p.tEnd - 0.0 : dt;
+  return hwrite_ctrl(h);
+}
+
+bool hydro_problem_ok(int kind) { return kind == PPK_PROBLEM_IMPLODE || kind == PPK_PROBLEM_BLAST; }
+
+}  // namespace
+
+extern "C" {
+
+const char *ppk_hydro_last_error(const ppk_hydro_t *h) {
+  return h ? h->err.c_str() : ppk::thread_error();
+}
+
+int ppk_hydro_create(const ppk_mhd_params *params, ppk_hydro_t **handle) {
+  if (!handle) return hfail(nullptr, PPK_ERR_INVALID_ARGUMENT, "handle is NULL");
+  *handle = nullptr;
+  std::string why;
+  int rc = ppk::validate_hydro(params, why);
+  if (rc != PPK_OK) return hfail(nullptr, rc, why);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return hfail(nullptr, PPK_ERR_HIP, "no HIP device available");
+  ppk_hydro_t *h = new (std::nothrow) ppk_hydro_t();
+  if (!h) return hfail(nullptr, PPK_ERR_OUT_OF_MEMORY, "host allocation failed");
+  h->p = *params;
+  h->hd = ppk::make_hydro_dev(*params);
+  const bool strict = params->precision == PPK_FP64_STRICT;
+  h->K = strict ? &ppk::kStrict : &ppk::kFast;
+  h->H = strict ? &ppk::kHydroStrict : &ppk::kHydroFast;
+  h->fused = params->kernel_path != PPK_PATH_STAGED;
+  auto bail = [&](int code, const std::string &m) {
+    hfree_all(h);
+    delete h;
+    return hfail(nullptr, code, m);
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(PPK_ERR_HIP, "hipStreamCreateWithFlags failed");
+  const size_t nvals = h->values();
+  auto dalloc = [&](double **ptr, size_t count) {
+    return hipMalloc((void **)ptr, count * sizeof(double)) == hipSuccess;
+  };
+  const size_t nparts = (size_t)std::max(h->H->prim_parts(h->hd), h->H->fused_parts(h->hd));
+  bool ok = dalloc(&h->U[0], nvals) && dalloc(&h->U[1], nvals) &&
+            dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->parts, nparts);
+  if (ok && !h->fused) ok = dalloc(&h->Q, nvals) && dalloc(&h->F, (size_t)params->dim * nvals);
+  if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
+  if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * ppk::CTRL_SLOTS) != hipSuccess)
+    return bail(PPK_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+  if (hipMemsetAsync(h->U[0], 0, nvals * sizeof(double), h->stream) != hipSuccess ||
+      hipMemsetAsync(h->U[1], 0, nvals * sizeof(double), h->stream) != hipSuccess)
+    return bail(PPK_ERR_HIP, "hipMemsetAsync failed");
+  std::fill(h->ctrl_host, h->ctrl_host + ppk::CTRL_SLOTS, 0.0);
+  h->ctrl_host[ppk::CTRL_DT] = params->tEnd;
+  h->ctrl_host[ppk::CTRL_TEND] = params->tEnd;
+  h->ctrl_host[ppk::CTRL_NSTEPMAX] = (double)params->n_step_max;
+  h->ctrl_host[ppk::CTRL_ACTIVE] = 1.0;
+  if (hwrite_ctrl(h) != PPK_OK) {
+    std::string m = h->err;
+    return bail(PPK_ERR_HIP, m);
+  }
+  *handle = h;
+  return PPK_OK;
+}
+
+void ppk_hydro_destroy(ppk_hydro_t *h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->stream);
+  hfree_all(h);
+  delete h;
+}
+
+int ppk_hydro_nbvar(const ppk_hydro_t *h) { return h ? h->hd.nv : 0; }
+
+int ppk_hydro_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem *problem,
+                            double *U, size_t count) {
+  std::string why;
+  int rc = ppk::validate_hydro(params, why);
+  if (rc != PPK_OK) return hfail(nullptr, rc, why);
+  if (!problem || !U) return hfail(nullptr, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!hydro_problem_ok(problem->kind))
+    return hfail(nullptr, PPK_ERR_UNSUPPORTED, "hydro problems: implode or blast");
+  const int gw = params->ghost_width;
+  const size_t cells = (size_t)(params->nx + 2 * gw) * (params->ny + 2 * gw) *
+                       (params->dim == 3 ? params->nz + 2 * gw : 1);
+  if (count != (size_t)(params->dim == 3 ? 5 : 4) * cells)
+    return hfail(nullptr, PPK_ERR_INVALID_ARGUMENT, "count must be nbvar*isize*jsize*ksize");
+  ppk::hydro_ics(*params, *problem, U);
+  return PPK_OK;
+}
+
+int ppk_hydro_init_problem(ppk_hydro_t *h, const ppk_mhd_problem *problem) {
+  if (!h || !problem) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!hydro_problem_ok(problem->kind))
+    return hfail(h, PPK_ERR_UNSUPPORTED, "hydro problems: implode or blast");
+  std::vector<double> host(h->values());
+  ppk::hydro_ics(h->p, *problem, host.data());
+  h->iter_host = 0;
+  h->cfl_valid = false;
+  HHIP(h, hipMemcpyAsync(h->cur(), host.data(), h->state_bytes(), hipMemcpyHostToDevice,
+                         h->stream));
+  HHIP(h, hipStreamSynchronize(h->stream));
+  return hfinish_init(h);
+}
+
+int ppk_hydro_upload(ppk_hydro_t *h, const double *U, size_t count) {
+  if (!h || !U) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (count != h->values())
+    return hfail(h, PPK_ERR_INVALID_ARGUMENT, "count must be nbvar*isize*jsize*ksize");
+  h->cfl_valid = false;
+  HHIP(h, hipMemcpyAsync(h->cur(), U, h->state_bytes(), hipMemcpyHostToDevice, h->stream));
+  HHIP(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int ppk_hydro_download(ppk_hydro_t *h, double *U, size_t count) {
+  if (!h || !U) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (count != h->values())
+    return hfail(h, PPK_ERR_INVALID_ARGUMENT, "count must be nbvar*isize*jsize*ksize");
+  HHIP(h, hipMemcpyAsync(U, h->cur(), h->state_bytes(), hipMemcpyDeviceToHost, h->stream));
+  HHIP(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int ppk_hydro_device_state(ppk_hydro_t *h, double **dU) {
+  if (!h || !dU) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  h->cfl_valid = false;  // the caller may write the state
+  *dU = h->cur();
+  return PPK_OK;
+}
+
+int ppk_hydro_make_boundaries(ppk_hydro_t *h) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  HHIP(h, h->K->begin(h->ctrl, 0, h->stream));
+  HHIP(h, h->H->boundaries(h->hd, h->cur(), nullptr, h->ctrl, h->stream));
+  return PPK_OK;
+}
+
+int ppk_hydro_compute_dt(ppk_hydro_t *h, double *dt) {
+  if (!h || !dt) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  // SolverHydroMuscl::compute_dt_local (SolverHydroMuscl.h:650-691): interior cells only
+  HHIP(h, h->K->begin(h->ctrl, 0, h->stream));
+  HHIP(h, h->H->prim_dt(h->hd, h->cur(), nullptr, h->parts, h->ctrl, 1, h->stream));
+  HHIP(h, h->K->set_dt(h->ctrl, h->parts, h->H->prim_parts(h->hd), 0.0,
+                       ppk::SETDT_REDUCE_ONLY, h->p.cfl, h->stream));
+  h->cfl_valid = false;
+  int rc = hread_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  *dt = h->p.cfl / h->ctrl_host[ppk::CTRL_INVDT];
+  return PPK_OK;
+}
+
+int ppk_hydro_step(ppk_hydro_t *h, double dt, int dt_mode) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (dt_mode != PPK_DT_FIXED && dt_mode != PPK_DT_CFL)
+    return hfail(h, PPK_ERR_INVALID_ARGUMENT, "dt_mode must be PPK_DT_FIXED or PPK_DT_CFL");
+  int rc = hlaunch_step(h, dt, dt_mode, 0);
+  if (rc != PPK_OK) return rc;
+  h->iter_host += 1;
+  return PPK_OK;
+}
+
+int ppk_hydro_run(ppk_hydro_t *h, int nsteps, double dt, int dt_mode) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (nsteps < 0) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  if (dt_mode != PPK_DT_FIXED && dt_mode != PPK_DT_CFL)
+    return hfail(h, PPK_ERR_INVALID_ARGUMENT, "dt_mode must be PPK_DT_FIXED or PPK_DT_CFL");
+  const long long start = h->iter_host;
+  const bool cfl_valid0 = h->cfl_valid;
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = hlaunch_step(h, dt, dt_mode, 1);
+    if (rc != PPK_OK) return rc;
+    h->iter_host += 1;
+  }
+  int rc = hread_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  h->iter_host = (long long)h->ctrl_host[ppk::CTRL_ITER];
+  if (h->iter_host == start) h->cfl_valid = cfl_valid0;
+  return PPK_OK;
+}
+
+int ppk_hydro_get_time(ppk_hydro_t *h, double *t, double *dt, int64_t *iteration) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  int rc = hread_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  if (t) *t = h->ctrl_host[ppk::CTRL_T];
+  if (dt) *dt = h->ctrl_host[ppk::CTRL_DT];
+  if (iteration) *iteration = (int64_t)h->ctrl_host[ppk::CTRL_ITER];
+  return PPK_OK;
+}
+
+int ppk_hydro_set_time(ppk_hydro_t *h, double t, int64_t iteration) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (iteration < 0) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "iteration must be >= 0");
+  int rc = hread_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  if (((long long)iteration & 1) != (h->iter_host & 1)) std::swap(h->U[0], h->U[1]);
+  h->cfl_valid = false;
+  h->iter_host = iteration;
+  h->ctrl_host[ppk::CTRL_T] = t;
+  h->ctrl_host[ppk::CTRL_ITER] = (double)iteration;
+  return hwrite_ctrl(h);
+}
+
+int ppk_hydro_finished(ppk_hydro_t *h, int *finished) {
+  if (!h || !finished) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = hread_ctrl(h);
+  if (rc != PPK_OK) return rc;
+  *finished = (h->ctrl_host[ppk::CTRL_T] >= (h->p.tEnd - 1e-14) ||
+               h->ctrl_host[ppk::CTRL_ITER] >= (double)h->p.n_step_max)
+                  ? 1
+                  : 0;
+  return PPK_OK;
+}
+
+int ppk_hydro_synchronize(ppk_hydro_t *h) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  HHIP(h, hipStreamSynchronize(h->stream));
+  return PPK_OK;
+}
+
+int ppk_hydro_stream(ppk_hydro_t *h, void **stream) {
+  if (!h || !stream) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *stream = (void *)h->stream;
+  return PPK_OK;
+}
+
+}  // extern "C"

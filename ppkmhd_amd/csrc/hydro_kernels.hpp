@@ -1,0 +1,65 @@
+// Interface between the hydro (Euler) driver (hydro_capi.cpp) and its kernels (hydro.hip),
+// compiled once per arithmetic variant (ppk::fast, ppk::strict) like the MHD kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hydro_body.hpp"
+#include "kernels.hpp"
+
+namespace ppk {
+
+// Everything the hydro kernels read, passed by value.  nv = 4 in 2D (ID, IP, IU, IV), 5 in 3D.
+struct HydroDev {
+  int dim, nv, nx, ny, nz, gw;
+  int isize, jsize, ksize;
+  int bc[6];
+  int strip_rows;  // fused kernels: output rows (2D) / planes (3D) one wave or block marches
+  int pad;
+  long long plane, vstride;
+  double dx, dy, dz;
+  hyd::HydroPar P;
+};
+
+// fused marching kernels: a wave covers 64 columns of x, of which the middle kHydroCols are its
+// output (two columns of halo on each side)
+constexpr int kHydroLanes = 64;
+constexpr int kHydroCols = kHydroLanes - 4;
+// 3D: waves of a block stacked along y; the middle kHydroRows3 rows are the block's output
+constexpr int kHydroWaves3 = 8;
+constexpr int kHydroRows3 = kHydroWaves3 - 4;
+
+#define PPK_DECLARE_HYDRO(NS)                                                                    \
+  namespace NS {                                                                                 \
+  /* ghost fill of U (x, then y, then z); with copy != nullptr every ghost value is also        \
+     written there (the next state's ghosts = this state's, as the reference's deep copy) */    \
+  hipError_t hydro_boundaries(const HydroDev &h, double *U, double *copy, const double *ctrl,    \
+                              hipStream_t s);                                                    \
+  /* primitives of every cell into Q (staged form) and per-block CFL maxima of the interior */   \
+  hipError_t hydro_prim_dt(const HydroDev &h, const double *U, double *Q, double *parts,         \
+                           const double *ctrl, int do_dt, hipStream_t s);                        \
+  int hydro_prim_parts(const HydroDev &h);                                                       \
+  /* staged form: fluxes of every left face of [gw, size-gw], then the update of Uout (which    \
+     holds a copy of Uin) */                                                                     \
+  hipError_t hydro_staged(const HydroDev &h, const double *Q, double *F, double *Uout,           \
+                          const double *ctrl, hipStream_t s);                                    \
+  /* fused form: one marching kernel from Uin to the interior of Uout; with fold_cfl the        \
+     per-wave / per-block CFL maxima of the new state go to parts */                             \
+  hipError_t hydro_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,      \
+                         const double *ctrl, int fold_cfl, hipStream_t s);                       \
+  int hydro_fused_parts(const HydroDev &h);                                                      \
+  }
+
+PPK_DECLARE_HYDRO(fast)
+PPK_DECLARE_HYDRO(strict)
+
+struct HydroTable {
+  decltype(&fast::hydro_boundaries) boundaries;
+  decltype(&fast::hydro_prim_dt) prim_dt;
+  decltype(&fast::hydro_prim_parts) prim_parts;
+  decltype(&fast::hydro_staged) staged;
+  decltype(&fast::hydro_fused) fused;
+  decltype(&fast::hydro_fused_parts) fused_parts;
+};
+
+}  // namespace ppk

@@ -51,6 +51,14 @@ def make_params(dim, nx, ny, nz=1, *, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zm
     return p
 
 
+def make_hydro_params(dim, nx, ny, nz=1, *, riemann=A.RIEMANN_APPROX, **kw):
+    """Parameters of the Euler solvers Hydro_Muscl_2D/3D: ghostWidth 2 and the reference's
+    default Riemann solver "approx" (src/shared/HydroParams.cpp:42-55, 145-160)."""
+    p = make_params(dim, nx, ny, nz, riemann=riemann, **kw)
+    p.ghost_width = 2
+    return p
+
+
 def init_params(p):
     """Derived fields, HydroParams::init (src/shared/HydroParams.cpp:421-453)."""
     p.dx = (p.xmax - p.xmin) / p.nx

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi as A
-from ._lib import PPKError, check, check_batch, lib
+from ._lib import PPKError, check, check_batch, check_with, lib
 from .params import default_problem
 
 
@@ -182,6 +182,125 @@ class SolverMHDMuscl:
                                            int(istep), 1 if ascii else 0))
 
 
+class SolverHydroMuscl:
+    """Euler MUSCL-Hancock solver on one GPU (the reference's Hydro_Muscl_2D/3D,
+    src/muscl/SolverHydroMuscl.{h,cpp}); state arrays are (nbvar, ksize, jsize, isize) with
+    nbvar 4 (2D) or 5 (3D)."""
+
+    def __init__(self, params, problem=None):
+        self.params = params
+        h = C.c_void_p()
+        L = lib()
+        check_with(L.ppk_hydro_create(C.byref(params), C.byref(h)), None, L.ppk_hydro_last_error)
+        self._h = h
+        self.nbvar = L.ppk_hydro_nbvar(h)
+        self.m_nCells = params.isize * params.jsize * params.ksize
+        if problem is not None:
+            self.init_problem(problem)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ppk_hydro_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        return check_with(rc, self._h, lib().ppk_hydro_last_error)
+
+    def init_problem(self, problem):
+        self._check(lib().ppk_hydro_init_problem(self._h, C.byref(problem)))
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.size != self.params.count:
+            raise ValueError(f"state must have {self.params.count} doubles, got {U.size}")
+        self._check(lib().ppk_hydro_upload(self._h, U.ctypes.data_as(C.POINTER(C.c_double)),
+                                           U.size))
+
+    def download(self):
+        U = np.empty(self.params.shape, dtype=np.float64)
+        self._check(lib().ppk_hydro_download(self._h, U.ctypes.data_as(C.POINTER(C.c_double)),
+                                             U.size))
+        return U
+
+    def device_state_ptr(self):
+        ptr = C.POINTER(C.c_double)()
+        self._check(lib().ppk_hydro_device_state(self._h, C.byref(ptr)))
+        return C.cast(ptr, C.c_void_p).value
+
+    def stream_ptr(self):
+        s = C.c_void_p()
+        self._check(lib().ppk_hydro_stream(self._h, C.byref(s)))
+        return s.value
+
+    def make_boundaries(self):
+        self._check(lib().ppk_hydro_make_boundaries(self._h))
+
+    def compute_dt_local(self):
+        dt = C.c_double()
+        self._check(lib().ppk_hydro_compute_dt(self._h, C.byref(dt)))
+        return dt.value
+
+    def step(self, dt=0.0, dt_mode=A.DT_CFL):
+        self._check(lib().ppk_hydro_step(self._h, float(dt), int(dt_mode)))
+
+    def next_iteration(self):
+        """SolverBase::next_iteration with reference dt semantics."""
+        self.step(0.0, A.DT_CFL)
+
+    def run(self, nsteps, dt=0.0, dt_mode=A.DT_CFL):
+        self._check(lib().ppk_hydro_run(self._h, int(nsteps), float(dt), int(dt_mode)))
+
+    def time(self):
+        t, dt, it = C.c_double(), C.c_double(), C.c_int64()
+        self._check(lib().ppk_hydro_get_time(self._h, C.byref(t), C.byref(dt), C.byref(it)))
+        return t.value, dt.value, it.value
+
+    def set_time(self, t, iteration):
+        self._check(lib().ppk_hydro_set_time(self._h, float(t), int(iteration)))
+
+    @property
+    def m_t(self):
+        return self.time()[0]
+
+    @property
+    def m_dt(self):
+        return self.time()[1]
+
+    @property
+    def m_iteration(self):
+        return self.time()[2]
+
+    def finished(self):
+        f = C.c_int()
+        self._check(lib().ppk_hydro_finished(self._h, C.byref(f)))
+        return bool(f.value)
+
+    def synchronize(self):
+        self._check(lib().ppk_hydro_synchronize(self._h))
+
+
+def hydro_initial_state(params, problem):
+    """The initial state of a hydro problem (implode, blast) computed on the host, no GPU."""
+    U = np.empty(params.shape, dtype=np.float64)
+    L = lib()
+    check_with(L.ppk_hydro_initial_state(C.byref(params), C.byref(problem),
+                                         U.ctypes.data_as(C.POINTER(C.c_double)), U.size),
+               None, L.ppk_hydro_last_error)
+    return U
+
+
 def initial_state(params, problem):
     """The initial state of a problem computed on the host (reference layout), no GPU."""
     U = np.empty(params.shape, dtype=np.float64)
@@ -291,12 +410,14 @@ class BatchSolver:
 
 class SolverFactory:
     """Registry of solver constructors keyed by the reference's solver names
-    (src/shared/SolverFactory.cpp:29-30)."""
+    (src/shared/SolverFactory.cpp:26-30)."""
 
     _instance = None
 
     def __init__(self):
         self._map = {}
+        self.registerSolver("Hydro_Muscl_2D", SolverHydroMuscl)
+        self.registerSolver("Hydro_Muscl_3D", SolverHydroMuscl)
         self.registerSolver("MHD_Muscl_2D", SolverMHDMuscl)
         self.registerSolver("MHD_Muscl_3D", SolverMHDMuscl)
 
@@ -314,5 +435,7 @@ class SolverFactory:
         if fn is None:
             raise PPKError(A.PPK_OK + 5, f"{solver_name}: is not recognized as a valid application "
                            f"name key; valid names: {sorted(self._map)}")
-        return fn(params, problem if problem is not None else default_problem(A.PROBLEM_ORSZAG_TANG,
-                                                                               params))
+        if problem is None:  # the solvers' fallbacks: Orszag-Tang (MHD), implode (hydro)
+            kind = A.PROBLEM_IMPLODE if solver_name.startswith("Hydro") else A.PROBLEM_ORSZAG_TANG
+            problem = default_problem(kind, params)
+        return fn(params, problem)

@@ -21,7 +21,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(ppk_(?:mhd|mlp|faceset)_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(ppk_(?:mhd|mlp|faceset|hydro)_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -136,9 +136,9 @@ density_out=1.0
 
 def test_ini_unknown_solver_is_an_error():
     with pytest.raises(PPKError) as e:
-        params_from_ini(text="[run]\nsolver_name=Hydro_Muscl_2D\n")
+        params_from_ini(text="[run]\nsolver_name=Hydro_SDM_2D_degree2\n")
     assert e.value.code == 5
-    assert "Hydro_Muscl_2D" in str(e.value)
+    assert "Hydro_SDM_2D_degree2" in str(e.value)
 
 
 def test_ini_missing_file_is_an_error(tmp_path):

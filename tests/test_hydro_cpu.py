@@ -1,0 +1,364 @@
+"""Pin the CPU hydro oracle (Hydro_Muscl_2D/3D, implementation 0) and the library's host side.
+
+The reference's own tests hold no golden vectors for the hydro solvers, so the oracle is pinned
+by what the mathematics fixes:
+  riemann            consistency F(q, q) = physical Euler flux for approx, llf, hll, hllc;
+                     supersonic upwinding; a stationary contact kept exactly by hllc / approx;
+                     the approx (two-shock) solver against the exact Riemann solution
+  boundaries         the textbook ghost fill (Dirichlet negates only the normal velocity)
+  full step          uniform state preserved; conservation with periodic and reflecting walls;
+                     the x <-> y mirror of a diagonal-symmetric problem; 3D with a z-uniform
+                     state equals the 2D step; Sod shock tube against the exact solution
+                     (first order at the discontinuities, error falls with resolution)
+  host library       initial conditions bit-identical to the oracle; .ini parsing of the
+                     reference's hydro settings; parameter checks before any device call
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import textbook as T
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd.params import default_problem, make_hydro_params
+
+
+@pytest.fixture(scope="module")
+def H():
+    from oracle import pyhydro
+
+    pyhydro.build()
+    return pyhydro
+
+
+SOLVERS2 = [A.RIEMANN_APPROX, A.RIEMANN_LLF, A.RIEMANN_HLL, A.RIEMANN_HLLC]
+SOLVERS3 = [A.RIEMANN_APPROX, A.RIEMANN_LLF, A.RIEMANN_HLLC]
+
+
+def params(dim=2, n=(16, 12, 8), **kw):
+    kw.setdefault("gamma0", 1.4)
+    kw.setdefault("cfl", 0.8)
+    kw.setdefault("slope_type", 2.0)
+    kw.setdefault("bc", A.BC_PERIODIC)
+    kw.setdefault("tEnd", 10.0)
+    return make_hydro_params(dim, *n, float_round=False, **kw)
+
+
+def rand_prim(rng, nv):
+    return np.array([rng.uniform(0.2, 3.0), rng.uniform(0.1, 5.0), *rng.normal(0, 0.8, nv - 2)])
+
+
+# ---- Riemann solvers -------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_riemann_consistency(H, dim):
+    """F(q, q) is the physical flux for every solver (the defining property of a consistent
+    numerical flux)."""
+    rng = np.random.default_rng(1)
+    nv = 5 if dim == 3 else 4
+    for rs in (SOLVERS3 if dim == 3 else SOLVERS2):
+        p = params(dim, riemann=rs)
+        for _ in range(50):
+            q = rand_prim(rng, nv)
+            f = H.riemann(p, q, q)
+            np.testing.assert_allclose(f, T.euler_flux(q, p.gamma0), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("rs", [A.RIEMANN_APPROX, A.RIEMANN_HLL, A.RIEMANN_HLLC])
+def test_riemann_supersonic_upwind(H, rs):
+    """All waves moving right (u - c > 0 on both sides): the Godunov-type solvers return the
+    left physical flux; moving left: the right flux."""
+    rng = np.random.default_rng(2)
+    p = params(2, riemann=rs)
+    for _ in range(40):
+        ql, qr = rand_prim(rng, 4), rand_prim(rng, 4)
+        cmax = max(np.sqrt(1.4 * ql[1] / ql[0]), np.sqrt(1.4 * qr[1] / qr[0]))
+        ql[2] = qr[2] = 3.0 * cmax
+        np.testing.assert_allclose(H.riemann(p, ql, qr), T.euler_flux(ql, 1.4), rtol=1e-12)
+        ql[2] = qr[2] = -3.0 * cmax
+        np.testing.assert_allclose(H.riemann(p, ql, qr), T.euler_flux(qr, 1.4), rtol=1e-12)
+
+
+def test_riemann_llf_is_central_plus_dissipation(H):
+    """LLF = mean of the physical fluxes minus the largest |u| + c times half the jump of the
+    conservative states (Toro 10.43)."""
+    rng = np.random.default_rng(3)
+    p = params(2, riemann=A.RIEMANN_LLF)
+    g = p.gamma0
+    for _ in range(40):
+        ql, qr = rand_prim(rng, 4), rand_prim(rng, 4)
+        def cons(q):
+            return np.array([q[0], q[1] / (g - 1) + 0.5 * q[0] * (q[2] ** 2 + q[3] ** 2),
+                             q[0] * q[2], q[0] * q[3]])
+        smax = max(abs(ql[2]) + np.sqrt(g * ql[1] / ql[0]), abs(qr[2]) + np.sqrt(g * qr[1] / qr[0]))
+        ref = 0.5 * (T.euler_flux(ql, g) + T.euler_flux(qr, g)) - 0.5 * smax * (cons(qr) - cons(ql))
+        np.testing.assert_allclose(H.riemann(p, ql, qr), ref, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("rs", [A.RIEMANN_APPROX, A.RIEMANN_HLLC])
+def test_riemann_stationary_contact(H, rs):
+    """A contact at rest (equal pressure and zero normal velocity, any density / tangential
+    jump) has the flux (0, 0, p, 0): exact for the contact-resolving solvers."""
+    rng = np.random.default_rng(4)
+    p = params(2, riemann=rs)
+    for _ in range(30):
+        pr = rng.uniform(0.1, 3.0)
+        ql = np.array([rng.uniform(0.2, 3), pr, 0.0, rng.normal()])
+        qr = np.array([rng.uniform(0.2, 3), pr, 0.0, rng.normal()])
+        f = H.riemann(p, ql, qr)
+        np.testing.assert_allclose(f, [0.0, 0.0, pr, 0.0], atol=1e-12 * pr)
+
+
+def test_riemann_approx_against_exact_solution(H):
+    """The two-shock approximate solver against the exact Godunov flux (Toro ch. 4) on
+    shock-shock problems, where the two-shock approximation is exact up to the Newton
+    tolerance, and on Sod, where it is close."""
+    p = params(2, riemann=A.RIEMANN_APPROX, gamma0=1.4)
+    g = 1.4
+    cases = [((1.0, 1.0, 2.0), (1.0, 1.0, -2.0), 1e-5),    # colliding flows: two shocks
+             ((3.0, 5.0, 0.5), (1.0, 1.0, -1.5), 1e-5),
+             ((1.0, 1.0, 0.0), (0.125, 0.1, 0.0), 0.02)]   # Sod: left rarefaction
+    for (rl, pl, ul), (rr, pr, ur), tol in cases:
+        r, u, pp = T.exact_riemann(rl, ul, pl, rr, ur, pr, g, 0.0)
+        ref = T.euler_flux(np.array([r, pp, u, 0.0]), g)
+        f = H.riemann(p, np.array([rl, pl, ul, 0.0]), np.array([rr, pr, ur, 0.0]))
+        np.testing.assert_allclose(f, ref, rtol=tol, atol=tol)
+
+
+# ---- boundaries -----------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("bc", [A.BC_PERIODIC, A.BC_NEUMANN, A.BC_DIRICHLET, "mixed"])
+def test_boundaries_textbook(H, dim, bc):
+    rng = np.random.default_rng(5)
+    bcs = [1, 2, 3, 3, 2, 1] if bc == "mixed" else [bc] * 6
+    p = params(dim, (7, 6, 5), bc=bcs)
+    U = rng.normal(size=H.shape(p))
+    ref = T.fill_ghosts_textbook(U, p.nx, p.ny, p.nz, 2, bcs, dim, mhd=False)
+    got = H.fill_boundaries(p, U.copy())
+    np.testing.assert_array_equal(got, ref)
+
+
+# ---- full step ------------------------------------------------------------------------------
+def smooth_state(p, rng):
+    sh = H_shape = (p.ksize, p.jsize, p.isize)
+    k, j, i = np.meshgrid(*[np.arange(s) for s in sh], indexing="ij")
+    def f(a, b, c, ph):
+        return np.sin(2 * np.pi * (a * i / p.isize + b * j / p.jsize + c * k / max(p.ksize, 1)) + ph)
+    r = 1.0 + 0.3 * f(1, 2, 1, 0.3)
+    pr = 1.0 + 0.2 * f(2, 1, 1, 1.1)
+    nv = 5 if p.dim == 3 else 4
+    vel = [0.4 * f(1, 1, 2, 0.7 + m) for m in range(nv - 2)]
+    U = np.zeros((nv,) + H_shape)
+    U[0] = r
+    for m in range(nv - 2):
+        U[2 + m] = r * vel[m]
+    U[1] = pr / (p.gamma0 - 1) + 0.5 * r * sum(v * v for v in vel)
+    return U
+
+
+def interior(p):
+    g = 2
+    ks = slice(g, g + p.nz) if p.dim == 3 else slice(0, 1)
+    return (slice(None), ks, slice(g, g + p.ny), slice(g, g + p.nx))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_uniform_state_preserved(H, dim):
+    nv = 5 if dim == 3 else 4
+    for rs in (SOLVERS3 if dim == 3 else SOLVERS2):
+        p = params(dim, riemann=rs)
+        U = np.zeros(H.shape(p))
+        q = np.array([1.3, 0.7, 0.2, -0.1, 0.3][:nv])
+        U[0] = q[0]
+        U[2:] = (q[0] * q[2:])[:, None, None, None]
+        U[1] = q[1] / (p.gamma0 - 1) + 0.5 * q[0] * np.sum(q[2:] ** 2)
+        s = H.Solver(p)
+        s.set_state(U)
+        for _ in range(3):
+            s.step(0, A.DT_CFL)
+        np.testing.assert_allclose(s.get_state(), U, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("slope_type", [0.0, 1.0, 2.0])
+def test_conservation_periodic(H, dim, slope_type):
+    rng = np.random.default_rng(6)
+    for rs in (SOLVERS3 if dim == 3 else SOLVERS2):
+        p = params(dim, riemann=rs, slope_type=slope_type)
+        U = smooth_state(p, rng)
+        s = H.Solver(p)
+        s.set_state(U)
+        tot0 = U[interior(p)].sum(axis=(1, 2, 3))
+        for _ in range(4):
+            s.step(0, A.DT_CFL)
+        tot = s.get_state()[interior(p)].sum(axis=(1, 2, 3))
+        scale = np.abs(U[interior(p)]).sum(axis=(1, 2, 3))
+        assert np.all(np.abs(tot - tot0) <= 1e-13 * scale), (rs, tot - tot0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reflecting_walls_conserve_mass_and_energy(H, dim):
+    rng = np.random.default_rng(7)
+    p = params(dim, bc=A.BC_DIRICHLET, riemann=A.RIEMANN_HLLC)
+    U = smooth_state(p, rng)
+    s = H.Solver(p)
+    s.set_state(U)
+    tot0 = U[interior(p)].sum(axis=(1, 2, 3))
+    for _ in range(4):
+        s.step(0, A.DT_CFL)
+    tot = s.get_state()[interior(p)].sum(axis=(1, 2, 3))
+    for v in (0, 1):  # mass, energy: no flux through a reflecting wall
+        assert abs(tot[v] - tot0[v]) <= 1e-12 * abs(tot0[v]), v
+
+
+@pytest.mark.parametrize("rs", SOLVERS2)
+def test_diagonal_mirror_symmetry_2d(H, rs):
+    """The implode problem on a square is symmetric under x <-> y; the scheme keeps the mirror
+    (transpose with the two momenta exchanged) up to the order of the x and y updates."""
+    p = params(2, (24, 24), riemann=rs, bc=A.BC_DIRICHLET, gamma0=1.666)
+    s = H.Solver(p)
+    s.init_problem(default_problem(A.PROBLEM_IMPLODE, p, float_round=False))
+    for _ in range(6):
+        s.step(0, A.DT_CFL)
+    U = s.get_state()[:, 0]
+    M = U.transpose(0, 2, 1)[[0, 1, 3, 2]]
+    np.testing.assert_allclose(U, M, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("rs", SOLVERS3)
+def test_3d_z_uniform_equals_2d(H, rs):
+    """A 3D state independent of z with w = 0 evolves as the 2D state (pins the z terms of the
+    3D trace, solver and update against the 2D scheme)."""
+    rng = np.random.default_rng(8)
+    p2 = params(2, (20, 14), riemann=rs)
+    p3 = params(3, (20, 14, 6), riemann=rs)
+    U2 = smooth_state(p2, rng)
+    U3 = np.zeros(H.shape(p3))
+    U3[:4] = U2[:, 0][:, None]
+    s2, s3 = H.Solver(p2), H.Solver(p3)
+    s2.set_state(U2)
+    s3.set_state(U3)
+    dt = 0.2 * min(p2.dx, p2.dy)
+    for _ in range(4):
+        s2.step(dt, A.DT_FIXED)
+        s3.step(dt, A.DT_FIXED)
+    a = s2.get_state()[interior(p2)][:, 0]
+    b = s3.get_state()[interior(p3)]
+    for k in range(p3.nz):
+        np.testing.assert_allclose(b[:4, k], a, rtol=1e-13, atol=1e-14)
+    assert np.all(b[4] == 0.0)
+
+
+def sod_run(H, n, rs, t_end=0.2):
+    p = params(2, (n, 4), riemann=rs, bc=[A.BC_NEUMANN] * 2 + [A.BC_PERIODIC] * 4,
+               gamma0=1.4, cfl=0.5, tEnd=t_end, slope_type=1.0)
+    U = np.zeros(H.shape(p))
+    x = p.xmin + p.dx / 2 + (np.arange(p.isize) - 2) * p.dx
+    left = x < 0.5
+    U[0] = np.where(left, 1.0, 0.125)[None, None, :]
+    U[1] = (np.where(left, 1.0, 0.1) / 0.4)[None, None, :]
+    s = H.Solver(p)
+    s.set_state(U)
+    for _ in range(10000):
+        t, _, _ = s.time()
+        if t >= t_end - 1e-14:
+            break
+        s.step(0, A.DT_CFL)
+    rho = s.get_state()[0, 0, 2 + 1, 2:2 + n]
+    xc = x[2:2 + n]
+    ex = np.array([T.exact_riemann(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4, (xi - 0.5) / t_end)[0]
+                   for xi in xc])
+    return np.mean(np.abs(rho - ex))
+
+
+@pytest.mark.parametrize("rs", SOLVERS2)
+def test_sod_shock_tube_converges(H, rs):
+    e1 = sod_run(H, 100, rs)
+    e2 = sod_run(H, 200, rs)
+    assert e1 < 0.02, e1
+    assert e2 < 0.65 * e1, (e1, e2)
+
+
+# ---- host side of the library (no GPU) ---------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kind", [A.PROBLEM_IMPLODE, A.PROBLEM_BLAST])
+def test_initial_state_bit_identical_to_oracle(H, dim, kind):
+    from ppkmhd_amd.solver import hydro_initial_state
+
+    for shape_region in (0, 1):
+        p = make_hydro_params(dim, 30, 22, 14, xmax=2.0, ymax=1.5, gamma0=1.666)
+        pr = default_problem(kind, p)
+        pr.implode_shape = shape_region
+        pr.implode_in[2] = 0.3  # a moving inner state exercises the energy's kinetic term
+        np.testing.assert_array_equal(hydro_initial_state(p, pr), H.init_array(p, pr))
+
+
+def test_ini_reference_hydro_settings():
+    """The reference's settings/test_implode_2D.ini and test_blast_3D.ini."""
+    from ppkmhd_amd.solver import params_from_ini
+    from ppkmhd_amd.params import f32
+
+    txt = """[run]
+solver_name=Hydro_Muscl_2D
+tEnd=0.025
+nStepmax=100
+nOutput=10
+[mesh]
+nx=1024
+ny=512
+xmin=0.0
+xmax=2.0
+ymin=0.0
+ymax=1.0
+boundary_type_xmin=1
+boundary_type_xmax=1
+boundary_type_ymin=1
+boundary_type_ymax=1
+[hydro]
+gamma0=1.666
+cfl=0.8
+niter_riemann=10
+iorder=2
+slope_type=2
+problem=implode
+riemann=hllc
+[other]
+implementationVersion=0
+"""
+    p, pr, name = params_from_ini(text=txt)
+    assert name == "Hydro_Muscl_2D" and p.dim == 2 and p.ghost_width == 2
+    assert p.riemann == A.RIEMANN_HLLC and pr.kind == A.PROBLEM_IMPLODE
+    assert p.gamma0 == f32(1.666) and p.cfl == f32(0.8) and p.nx == 1024 and p.ny == 512
+    assert p.shape == (4, 1, 516, 1028)
+    p3, pr3, name3 = params_from_ini(text=txt.replace("Hydro_Muscl_2D", "Hydro_Muscl_3D")
+                                     .replace("problem=implode", "problem=blast"))
+    assert p3.dim == 3 and p3.ghost_width == 2 and pr3.kind == A.PROBLEM_BLAST
+    assert p3.shape[0] == 5
+    # unknown problem names fall back to implode (SolverHydroMuscl.cpp:170-177); problems of
+    # the reference this library lacks are refused at init
+    _, pr4, _ = params_from_ini(text=txt.replace("problem=implode", "problem=nonsense"))
+    assert pr4.kind == A.PROBLEM_IMPLODE
+    _, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=gresho_vortex"))
+    assert pr5.kind == -1
+
+
+def test_hydro_create_rejects_bad_params_before_any_device_call():
+    from ppkmhd_amd._lib import lib
+
+    L = lib()
+    h = C.c_void_p()
+    cases = [
+        (dict(dim=3, riemann=A.RIEMANN_HLL), A.PPK_ERR_UNSUPPORTED, "3D"),
+        (dict(riemann=A.RIEMANN_HLLD), A.PPK_ERR_UNSUPPORTED, "riemann"),
+        (dict(slope_type=3.0), A.PPK_ERR_UNSUPPORTED, "slope_type"),
+        (dict(precision=A.FP32_BULK), A.PPK_ERR_UNSUPPORTED, "FP64"),
+    ]
+    for kw, code, word in cases:
+        dim = kw.pop("dim", 2)
+        p = make_hydro_params(dim, 16, 16, 8, **kw)
+        rc = L.ppk_hydro_create(C.byref(p), C.byref(h))
+        assert rc == code, (kw, rc)
+        assert not h.value
+        assert word in L.ppk_hydro_last_error(None).decode()
+    p = make_hydro_params(2, 16, 16)
+    p.ghost_width = 3
+    assert L.ppk_hydro_create(C.byref(p), C.byref(h)) == A.PPK_ERR_INVALID_ARGUMENT

@@ -199,10 +199,10 @@ def random_state(rng, bscale=1.0, vscale=1.0):
                      *rng.normal(0, bscale, 3)])
 
 
-def fill_ghosts_textbook(U, nx, ny, nz, gw, bc, dim):
+def fill_ghosts_textbook(U, nx, ny, nz, gw, bc, dim, mhd=True):
     """Ghost fill defined directly: faces in the order x, y, z, each over the whole transverse
     extent; periodic wraps, Neumann copies the first/last interior cell, Dirichlet mirrors
-    with the normal velocity and normal field negated."""
+    with the normal velocity (and, for MHD, the normal field) negated."""
     U = U.copy()
     ns = (nx, ny, nz)
     axes = {0: 3, 1: 2, 2: 1}  # numpy axis of x, y, z in (var, k, j, i)
@@ -222,8 +222,81 @@ def fill_ghosts_textbook(U, nx, ny, nz, gw, bc, dim):
                 src = np.take(U, c0, axis=ax).copy()
                 if kind == 1:
                     src[IU + d] *= -1
-                    src[IA + d] *= -1
+                    if mhd:
+                        src[IA + d] *= -1
                 idx = [slice(None)] * 4
                 idx[ax] = c
                 U[tuple(idx)] = src
     return U
+
+
+# ---- Euler equations ---------------------------------------------------------------------------
+def euler_flux(q, gamma):
+    """Physical x-flux of the Euler equations for a primitive state in the reference's variable
+    order (rho, p, u, v[, w]), returned in the same order (mass, energy, x, y[, z] momentum)."""
+    q = np.asarray(q, dtype=np.float64)
+    r, p, u = q[0], q[1], q[2]
+    vt = q[3:]
+    E = p / (gamma - 1) + 0.5 * r * (u * u + np.sum(vt * vt))
+    return np.array([r * u, u * (E + p), r * u * u + p, *(r * u * vt)])
+
+
+def exact_riemann(rl, ul, pl, rr, ur, pr, gamma, xi):
+    """Exact solution (rho, u, p) of the 1D Euler Riemann problem sampled at x/t = xi
+    (Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics, ch. 4: pressure function,
+    Newton iteration for p*, sampling of the wave pattern)."""
+    g = gamma
+    cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+
+    def fk(p, rk, pk, ck):
+        if p > pk:  # shock
+            A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
+            f = (p - pk) * np.sqrt(A / (p + B))
+            df = np.sqrt(A / (p + B)) * (1 - (p - pk) / (2 * (B + p)))
+        else:  # rarefaction
+            f = 2 * ck / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+            df = 1 / (rk * ck) * (p / pk) ** (-(g + 1) / (2 * g))
+        return f, df
+
+    p = max(1e-8, 0.5 * (pl + pr))
+    for _ in range(100):
+        f1, d1 = fk(p, rl, pl, cl)
+        f2, d2 = fk(p, rr, pr, cr)
+        dp = (f1 + f2 + ur - ul) / (d1 + d2)
+        p_new = max(1e-12, p - dp)
+        if abs(p_new - p) < 1e-14 * (p_new + p):
+            p = p_new
+            break
+        p = p_new
+    f1, _ = fk(p, rl, pl, cl)
+    f2, _ = fk(p, rr, pr, cr)
+    us = 0.5 * (ul + ur) + 0.5 * (f2 - f1)
+    if xi <= us:  # left of the contact
+        if p > pl:
+            sl = ul - cl * np.sqrt((g + 1) / (2 * g) * p / pl + (g - 1) / (2 * g))
+            if xi <= sl:
+                return rl, ul, pl
+            return rl * (p / pl + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p / pl + 1), us, p
+        csl = cl * (p / pl) ** ((g - 1) / (2 * g))
+        if xi <= ul - cl:
+            return rl, ul, pl
+        if xi >= us - csl:
+            return rl * (p / pl) ** (1 / g), us, p
+        c = 2 / (g + 1) * (cl + (g - 1) / 2 * (ul - xi))
+        u = 2 / (g + 1) * (cl + (g - 1) / 2 * ul + xi)
+        r = rl * (c / cl) ** (2 / (g - 1))
+        return r, u, pl * (c / cl) ** (2 * g / (g - 1))
+    if p > pr:
+        sr = ur + cr * np.sqrt((g + 1) / (2 * g) * p / pr + (g - 1) / (2 * g))
+        if xi >= sr:
+            return rr, ur, pr
+        return rr * (p / pr + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p / pr + 1), us, p
+    csr = cr * (p / pr) ** ((g - 1) / (2 * g))
+    if xi >= ur + cr:
+        return rr, ur, pr
+    if xi <= us + csr:
+        return rr * (p / pr) ** (1 / g), us, p
+    c = 2 / (g + 1) * (cr - (g - 1) / 2 * (ur - xi))
+    u = 2 / (g + 1) * (-cr + (g - 1) / 2 * ur + xi)
+    r = rr * (c / cr) ** (2 / (g - 1))
+    return r, u, pr * (c / cr) ** (2 * g / (g - 1))
