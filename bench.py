@@ -141,6 +141,146 @@ def cpu_baseline(n, steps=1):
                       f"({dt:.1f} s)"}
 
 
+# Euler (hydro) solvers.  FP64 add + mul + div + sqrt per cell-update of the fused algorithm
+# (primitives, slopes and Hancock sources once per cell, 2*dim face states, dim HLLC solves,
+# update, CFL of the new state), counted with the operation-counting build of the hydro oracle
+# (oracle/libhydro_oracle_count.so; HLLC has no data-dependent loop, so the count is exact).
+HYDRO_FLOP_PER_CELL = {2: 386.0, 3: 667.0}
+# the step: the nbvar FP64 conservative variables of a cell read once and written once
+HYDRO_STEP_BYTES = {2: 64.0, 3: 80.0}
+# HBM bytes per cell-update of the fused kernel, rocprofv3 FETCH_SIZE (x2) + WRITE_SIZE (x1) at
+# the bench sizes (profiles/hydro_fused_pmc.md): 3.615 GB / 33,554,432 cells (2D 8192 x 4096),
+# 7.463 GB / 50,331,648 cells (3D 512 x 384 x 256)
+HYDRO_MEASURED_BYTES = {2: 3.6149e9 / 33554432, 3: 7.4634e9 / 50331648}
+
+
+def hydro_workload(dim, n, path, precision):
+    """The reference's settings/test_implode_2D.ini (1024 x 512 on [0,2] x [0,1]) and
+    settings/test_blast_3D.ini (64 x 48 x 32 on [0,2] x [0,1.5] x [0,1]) at n times their
+    resolution: gamma 1.666, CFL 0.8, slope_type 2, HLLC, reflecting walls."""
+    from ppkmhd_amd.params import make_hydro_params
+
+    if dim == 2:
+        p = make_hydro_params(2, 1024 * n, 512 * n, xmax=2.0, ymax=1.0, gamma0=1.666, cfl=0.8,
+                              slope_type=2.0, riemann=A.RIEMANN_HLLC, bc=A.BC_DIRICHLET,
+                              tEnd=1.0e9, n_step_max=1 << 30, kernel_path=path,
+                              precision=precision)
+        pr = default_problem(A.PROBLEM_IMPLODE, p)
+    else:
+        p = make_hydro_params(3, 64 * n, 48 * n, 32 * n, xmax=2.0, ymax=1.5, zmax=1.0,
+                              gamma0=1.666, cfl=0.8, slope_type=2.0, riemann=A.RIEMANN_HLLC,
+                              bc=A.BC_DIRICHLET, tEnd=1.0e9, n_step_max=1 << 30,
+                              kernel_path=path, precision=precision)
+        pr = default_problem(A.PROBLEM_BLAST, p)
+    return p, pr
+
+
+def bench_hydro(a, dim):
+    """Hydro MUSCL-Hancock (Euler, HLLC) cell-updates/s on the reference's own hydro test
+    problems, scaled up (2D implode 8x: 8192 x 4096; 3D blast 8x: 512 x 384 x 256)."""
+    from ppkmhd_amd.solver import SolverHydroMuscl
+
+    scale = a.hydro_scale if a.hydro_scale else 8
+    path = {"auto": A.PATH_AUTO, "staged": A.PATH_STAGED, "fused": A.PATH_FUSED}[a.path]
+    prec = {"fp64": A.FP64, "fp64_strict": A.FP64_STRICT}.get(a.precision)
+    if prec is None:
+        raise SystemExit("the hydro solvers compute in FP64 (fp64 or fp64_strict)")
+    p, pr = hydro_workload(dim, scale, path, prec)
+    cells = p.nx * p.ny * (p.nz if dim == 3 else 1)
+    shape = f"{p.nx}x{p.ny}" + (f"x{p.nz}" if dim == 3 else "")
+    log(f"[bench] hydro {dim}D {shape} (path={a.path}, precision={a.precision})")
+    s = SolverHydroMuscl(p, pr)
+    if a.copy_calibration:
+        copy_calibration(s, p.count * 8)
+        log(f"[bench] copy calibration: {p.count * 8} bytes device to device")
+    s.run(a.warmup)
+    s.synchronize()
+    s.timers_enable(None)
+    s.run(3)
+    prof = s.timers_read()
+    log("[bench] stage ms/launch: " + ", ".join(f"{k} {v[0] / v[1]:.3f}" for k, v in prof.items()))
+    dom = max(prof, key=lambda k: prof[k][0] / prof[k][1] * (1 if k != "control" else 0))
+    s.timers_enable([dom])
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(a.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    dom_ms, dom_cnt = s.timers_read()[dom]
+    t, dt, it = s.time()
+    s.timers_enable([])
+    s.close()
+    elapsed = t1 - t0
+    value = cells * a.steps / elapsed
+    avg_ms = dom_ms / dom_cnt
+    log(f"[bench] {a.steps} steps in {elapsed:.3f} s: {elapsed / a.steps * 1e3:.3f} ms/step, "
+        f"{value / 1e9:.3f} G cell-updates/s; {dom} {avg_ms:.3f} ms/launch; t={t:.6g} it={it}")
+    roof = None
+    if dom == "fused":
+        flops = HYDRO_FLOP_PER_CELL[dim] * cells
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        gbs = HYDRO_STEP_BYTES[dim] * cells / (avg_ms * 1e-3) / 1e9
+        meas = HYDRO_MEASURED_BYTES[dim]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": meas * cells if (meas and prec == A.FP64 and scale == 8) else None,
+                "kernel": f"k_hfused{dim}d", "kernel_ms": avg_ms, "flop_per_launch": flops,
+                "hbm_achieved_GBs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS,
+                "traffic_GBs": (meas * cells / (avg_ms * 1e-3) / 1e9
+                                if (meas and prec == A.FP64 and scale == 8) else None),
+                "note": "FP64 VALU-bound (divisions and square roots); peak is the MI355X FP64 "
+                        "figure"}
+    cpu = None
+    if not a.no_cpu:
+        try:
+            # bounded sample: the same problem at up to 4x the reference resolution
+            cpu = hydro_cpu_baseline(dim, min(scale, 4), 3)
+        except Exception as e:  # reported, never fatal for the GPU figure
+            log(f"[bench] CPU baseline failed: {e!r}")
+    out = {
+        "metric": f"hydro cell-updates/s (MUSCL-Hancock HLLC, {dim}D FP64)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (analytic initial condition of the reference's test problem)",
+        "config": {"workload": (f"Hydro_Muscl_2D implode {shape} (settings/test_implode_2D.ini "
+                                f"x{scale})" if dim == 2 else
+                                f"Hydro_Muscl_3D blast {shape} (settings/test_blast_3D.ini "
+                                f"x{scale})") + ", HLLC, slope_type 2, CFL 0.8, walls",
+                   "global_batch": 1, "seq_len": None, "parallelism": "single GPU",
+                   "kernel_path": a.path, "precision": a.precision},
+        "step_hbm_GBs_at_state_rw": HYDRO_STEP_BYTES[dim] * value / 1e9,
+        "stage_ms": {k: v[0] / v[1] for k, v in prof.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def hydro_cpu_baseline(dim, scale, steps):
+    """The hydro oracle (C++ OpenMP restatement of the reference's impl-0 hydro step) on the
+    host cores, same workload: one warm-up step, then `steps` timed steps."""
+    from oracle import pyhydro as H
+
+    H.build()
+    p, pr = hydro_workload(dim, scale, A.PATH_AUTO, A.FP64)
+    cells = p.nx * p.ny * (p.nz if dim == 3 else 1)
+    s = H.Solver(p)
+    s.init_problem(pr)
+    s.step(0, A.DT_CFL)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(0, A.DT_CFL)
+    el = time.perf_counter() - t0
+    s.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    shape = f"{p.nx}x{p.ny}" + (f"x{p.nz}" if dim == 3 else "")
+    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"hydro oracle (C++ OpenMP restatement of the reference impl-0 step), same "
+                      f"{dim}D problem on {shape}, 1 warm-up + {steps} timed step(s) "
+                      f"({el:.1f} s)"}
+
+
 def bench_ensemble(a):
     """Batched ensemble: 64 2D Orszag-Tang runs of uneven sizes advanced together (one step
     graph for all); the same members stepped one solver after another are timed beside it."""
@@ -408,11 +548,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the state (profiler calibration)")
     ap.add_argument("--workload",
-                    choices=("blast3d", "ensemble2d", "surrogate2d", "surrogate3d"),
+                    choices=("blast3d", "ensemble2d", "surrogate2d", "surrogate3d", "hydro2d",
+                             "hydro3d"),
                     default="blast3d",
                     help="blast3d: the 3D blast step (default); ensemble2d: batched ensemble; "
                          "surrogate2d / surrogate3d: learned flux on Orszag-Tang 2D / "
-                         "Kelvin-Helmholtz 3D")
+                         "Kelvin-Helmholtz 3D; hydro2d / hydro3d: the Euler solvers on the "
+                         "reference's implode 2D / blast 3D tests, scaled up")
+    ap.add_argument("--hydro-scale", type=int, default=0,
+                    help="hydro workloads: multiple of the reference test resolution (default 8)")
     ap.add_argument("--samples", type=int, default=20_000_000)
     ap.add_argument("--train-steps", type=int, default=300000)
     ap.add_argument("--gate", type=float, default=0.005)
@@ -432,6 +576,8 @@ def main():
         return bench_surrogate(a)
     if a.workload == "surrogate3d":
         return bench_surrogate3d(a)
+    if a.workload in ("hydro2d", "hydro3d"):
+        return bench_hydro(a, 2 if a.workload == "hydro2d" else 3)
 
     from ppkmhd_amd.solver import SolverMHDMuscl
 

@@ -392,6 +392,13 @@ int ppk_hydro_set_time(ppk_hydro_t *h, double t, int64_t iteration);
 /* Replaces: SolverBase::finished (src/shared/SolverBase.cpp:195-201). */
 int ppk_hydro_finished(ppk_hydro_t *h, int *finished);
 int ppk_hydro_synchronize(ppk_hydro_t *h);
+/* Per-stage GPU timers (hipEvents on the handle's stream), as ppk_mhd_timers_*; stages:
+ * 0 boundaries, 1 primitives_dt, 2 fluxes, 3 update (staged form), 4 fused, 5 control.
+ * Replaces: the per-stage timers of SolverBase (src/shared/SolverBase.h:35-42, 150-157). */
+int ppk_hydro_timers_enable(ppk_hydro_t *h, uint32_t stage_mask);
+int ppk_hydro_timers_read(ppk_hydro_t *h, int stage, double *total_ms, int64_t *launches);
+int ppk_hydro_timer_count(void);
+const char *ppk_hydro_timer_name(int stage);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,54 @@
 
 namespace {
 
+// ---- scalar type: plain double, or (HORA_COUNT build) a double that counts its operations --
+#ifdef HORA_COUNT
+struct OpCounts {
+  long long add = 0, mul = 0, div = 0, sqrt = 0, sel = 0;
+};
+// stages of a step: 0 ghost fill + copy, 1 primitives of every cell, 2 slopes of a cell of the
+// flux range, 3 the same recomputed for its left neighbours, 4 Hancock sources (computed in
+// each of the 2 * dim traces of a cell), 5 face states + Riemann solve + dt/dx scaling,
+// 6 update, 7 CFL maximum
+constexpr int kStages = 8;
+OpCounts g_stage_ops[kStages];
+int g_stage = 0;
+#define g_ops (g_stage_ops[g_stage])
+struct real {
+  double v;
+  real() : v(0.0) {}
+  real(double x) : v(x) {}
+};
+inline real operator+(real a, real b) { ++g_ops.add; return real(a.v + b.v); }
+inline real operator-(real a, real b) { ++g_ops.add; return real(a.v - b.v); }
+inline real operator*(real a, real b) { ++g_ops.mul; return real(a.v * b.v); }
+inline real operator/(real a, real b) { ++g_ops.div; return real(a.v / b.v); }
+inline real operator-(real a) { return real(-a.v); }
+inline real &operator+=(real &a, real b) { a = a + b; return a; }
+inline real &operator-=(real &a, real b) { a = a - b; return a; }
+inline bool operator<(real a, real b) { return a.v < b.v; }
+inline bool operator>(real a, real b) { return a.v > b.v; }
+inline bool operator<=(real a, real b) { return a.v <= b.v; }
+inline bool operator>=(real a, real b) { return a.v >= b.v; }
+inline bool operator!=(real a, real b) { return a.v != b.v; }
+inline real SQRT(real a) { ++g_ops.sqrt; return real(std::sqrt(a.v)); }
+inline real FABS(real a) { ++g_ops.sel; return real(std::fabs(a.v)); }
+inline real FMAX(real a, real b) { ++g_ops.sel; return real(std::fmax(a.v, b.v)); }
+inline real FMIN(real a, real b) { ++g_ops.sel; return real(std::fmin(a.v, b.v)); }
+inline real COPYSIGN(real a, real b) { return real(std::copysign(a.v, b.v)); }
+inline double D(real a) { return a.v; }
+#define HORA_STAGE(n) (g_stage = (n))
+#else
+#define HORA_STAGE(n) ((void)0)
+typedef double real;
+inline double SQRT(double a) { return std::sqrt(a); }
+inline double FABS(double a) { return std::fabs(a); }
+inline double FMAX(double a, double b) { return std::fmax(a, b); }
+inline double FMIN(double a, double b) { return std::fmin(a, b); }
+inline double COPYSIGN(double a, double b) { return std::copysign(a, b); }
+inline double D(double a) { return a; }
+#endif
+
 enum { ID = 0, IP = 1, IU = 2, IV = 3, IW = 4 };
 
 struct Grid {
@@ -34,9 +82,9 @@ struct Grid {
 };
 
 struct S5 {
-  double v[5] = {0, 0, 0, 0, 0};
-  double &operator[](int i) { return v[i]; }
-  const double &operator[](int i) const { return v[i]; }
+  real v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  real &operator[](int i) { return v[i]; }
+  const real &operator[](int i) const { return v[i]; }
 };
 
 // ---- boundaries (MakeBoundariesFunctor2D/3D, non-MHD: Dirichlet flips the normal velocity) --
@@ -71,14 +119,14 @@ void boundaries(const ppk_mhd_params &p, double *U) {
 }
 
 // ---- primitives (computePrimitives) ---------------------------------------------------------
-void to_prim(const ppk_mhd_params &p, int nv, const S5 &u, double &c, S5 &q) {
-  const double d = std::fmax(u[ID], p.smallr);
-  const double ux = u[IU] / d, uy = u[IV] / d;
-  const double uz = nv == 5 ? u[IW] / d : 0.0;
-  const double eken = nv == 5 ? 0.5 * (ux * ux + uy * uy + uz * uz) : 0.5 * (ux * ux + uy * uy);
-  const double e = u[IP] / d - eken;
-  const double pr = std::fmax((p.gamma0 - 1.0) * d * e, d * p.smallp);
-  c = std::sqrt(p.gamma0 * (pr) / d);
+void to_prim(const ppk_mhd_params &p, int nv, const S5 &u, real &c, S5 &q) {
+  const real d = FMAX(u[ID], p.smallr);
+  const real ux = u[IU] / d, uy = u[IV] / d;
+  const real uz = nv == 5 ? u[IW] / d : 0.0;
+  const real eken = nv == 5 ? 0.5 * (ux * ux + uy * uy + uz * uz) : 0.5 * (ux * ux + uy * uy);
+  const real e = u[IP] / d - eken;
+  const real pr = FMAX((p.gamma0 - 1.0) * d * e, d * p.smallp);
+  c = SQRT(p.gamma0 * (pr) / d);
   q[ID] = d;
   q[IP] = pr;
   q[IU] = ux;
@@ -87,12 +135,12 @@ void to_prim(const ppk_mhd_params &p, int nv, const S5 &u, double &c, S5 &q) {
 }
 
 // ---- slopes (slope_unsplit_hydro_2d/3d_scalar) ----------------------------------------------
-double minmod_slope(double st, double q, double qp, double qm) {
-  const double dlft = st * (q - qm), drgt = st * (qp - q), dcen = 0.5 * (qp - qm);
-  const double dsgn = dcen >= 0.0 ? 1.0 : -1.0;
-  double dlim = std::fmin(std::fabs(dlft), std::fabs(drgt));
+real minmod_slope(real st, real q, real qp, real qm) {
+  const real dlft = st * (q - qm), drgt = st * (qp - q), dcen = 0.5 * (qp - qm);
+  const real dsgn = dcen >= 0.0 ? 1.0 : -1.0;
+  real dlim = FMIN(FABS(dlft), FABS(drgt));
   if ((dlft * drgt) <= 0.0) dlim = 0.0;
-  return dsgn * std::fmin(dlim, std::fabs(dcen));
+  return dsgn * FMIN(dlim, FABS(dcen));
 }
 
 // ---- Riemann solvers (RiemannSolvers.h) -----------------------------------------------------
@@ -101,46 +149,46 @@ void flux_of_gdnv(const ppk_mhd_params &p, int nv, const S5 &qg, S5 &f) {
   f[IU] = f[ID] * qg[IU] + qg[IP];
   f[IV] = f[ID] * qg[IV];
   if (nv == 5) f[IW] = f[ID] * qg[IW];
-  const double entho = 1.0 / (p.gamma0 - 1.0);
-  double ekin = 0.5 * qg[ID] * (qg[IU] * qg[IU] + qg[IV] * qg[IV]);
+  const real entho = 1.0 / (p.gamma0 - 1.0);
+  real ekin = 0.5 * qg[ID] * (qg[IU] * qg[IU] + qg[IV] * qg[IV]);
   if (nv == 5) ekin += 0.5 * qg[ID] * (qg[IW] * qg[IW]);
-  const double etot = qg[IP] * entho + ekin;
+  const real etot = qg[IP] * entho + ekin;
   f[IP] = qg[IU] * (etot + qg[IP]);
 }
 
 void solve_approx(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, S5 &f) {
-  const double g0 = p.gamma0, g6 = p.gamma6;
-  const double rl = std::fmax(ql[ID], p.smallr), ul = ql[IU], pl = std::fmax(ql[IP], rl * p.smallp);
-  const double rr = std::fmax(qr[ID], p.smallr), ur = qr[IU], pr = std::fmax(qr[IP], rr * p.smallp);
-  const double cl = g0 * pl * rl, cr = g0 * pr * rr;
-  double wl = std::sqrt(cl), wr = std::sqrt(cr);
-  double pstar = std::fmax(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), 0.0);
-  double pold = pstar, conv = 1.0;
+  const real g0 = p.gamma0, g6 = p.gamma6;
+  const real rl = FMAX(ql[ID], p.smallr), ul = ql[IU], pl = FMAX(ql[IP], rl * p.smallp);
+  const real rr = FMAX(qr[ID], p.smallr), ur = qr[IU], pr = FMAX(qr[IP], rr * p.smallp);
+  const real cl = g0 * pl * rl, cr = g0 * pr * rr;
+  real wl = SQRT(cl), wr = SQRT(cr);
+  real pstar = FMAX(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), 0.0);
+  real pold = pstar, conv = 1.0;
   for (int it = 0; it < 10 && conv > 1e-6; ++it) {
-    const double wwl = std::sqrt(cl * (1.0 + g6 * (pold - pl) / pl));
-    const double wwr = std::sqrt(cr * (1.0 + g6 * (pold - pr) / pr));
-    const double a = 2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
-    const double b = 2.0 * wwr * wwr * wwr / (wwr * wwr + cr);
-    const double usl = ul - (pold - pl) / wwl, usr = ur + (pold - pr) / wwr;
-    const double delp = std::fmax(b * a / (b + a) * (usl - usr), -pold);
+    const real wwl = SQRT(cl * (1.0 + g6 * (pold - pl) / pl));
+    const real wwr = SQRT(cr * (1.0 + g6 * (pold - pr) / pr));
+    const real a = 2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
+    const real b = 2.0 * wwr * wwr * wwr / (wwr * wwr + cr);
+    const real usl = ul - (pold - pl) / wwl, usr = ur + (pold - pr) / wwr;
+    const real delp = FMAX(b * a / (b + a) * (usl - usr), -pold);
     pold = pold + delp;
-    conv = std::fabs(delp / (pold + p.smallpp));
+    conv = FABS(delp / (pold + p.smallpp));
   }
   pstar = pold;
-  wl = std::sqrt(cl * (1.0 + g6 * (pstar - pl) / pl));
-  wr = std::sqrt(cr * (1.0 + g6 * (pstar - pr) / pr));
-  const double ustar = 0.5 * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr);
-  const double sgnm = std::copysign(1.0, ustar);
+  wl = SQRT(cl * (1.0 + g6 * (pstar - pl) / pl));
+  wr = SQRT(cr * (1.0 + g6 * (pstar - pr) / pr));
+  const real ustar = 0.5 * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr);
+  const real sgnm = COPYSIGN(1.0, ustar);
   const bool left = sgnm > 0.0;
-  const double ro = left ? rl : rr, uo = left ? ul : ur, po = left ? pl : pr, wo = left ? wl : wr;
-  const double co = std::fmax(p.smallc, std::sqrt(std::fabs(g0 * po / ro)));
-  const double rstar = std::fmax(ro / (1.0 + ro * (po - pstar) / (wo * wo)), p.smallr);
-  const double cstar = std::fmax(p.smallc, std::sqrt(std::fabs(g0 * pstar / rstar)));
-  double spout = co - sgnm * uo, spin = cstar - sgnm * ustar;
-  const double ushock = wo / ro - sgnm * uo;
+  const real ro = left ? rl : rr, uo = left ? ul : ur, po = left ? pl : pr, wo = left ? wl : wr;
+  const real co = FMAX(p.smallc, SQRT(FABS(g0 * po / ro)));
+  const real rstar = FMAX(ro / (1.0 + ro * (po - pstar) / (wo * wo)), p.smallr);
+  const real cstar = FMAX(p.smallc, SQRT(FABS(g0 * pstar / rstar)));
+  real spout = co - sgnm * uo, spin = cstar - sgnm * ustar;
+  const real ushock = wo / ro - sgnm * uo;
   if (pstar >= po) spin = spout = ushock;
-  const double scr = std::fmax(spout - spin, p.smallc + std::fabs(spout + spin));
-  double frac = 0.5 * (1.0 + (spout + spin) / scr);
+  const real scr = FMAX(spout - spin, p.smallc + FABS(spout + spin));
+  real frac = 0.5 * (1.0 + (spout + spin) / scr);
   frac = (frac != frac) ? 0.0 : (frac >= 1.0 ? 1.0 : frac <= 0.0 ? 0.0 : frac);
   S5 qg;
   qg[ID] = frac * rstar + (1.0 - frac) * ro;
@@ -153,7 +201,7 @@ void solve_approx(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, S
   flux_of_gdnv(p, nv, qg, f);
 }
 
-void cons_flux(int nv, const S5 &q, double entho, bool hll, S5 &u, S5 &f) {
+void cons_flux(int nv, const S5 &q, real entho, bool hll, S5 &u, S5 &f) {
   u[ID] = q[ID];
   u[IP] = q[IP] * entho + 0.5 * q[ID] * q[IU] * q[IU];
   u[IP] += 0.5 * q[ID] * q[IV] * q[IV];
@@ -170,48 +218,48 @@ void cons_flux(int nv, const S5 &q, double entho, bool hll, S5 &u, S5 &f) {
 
 void solve_llf_hll(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, bool hll,
                    S5 &f) {
-  const double g0 = p.gamma0, entho = 1.0 / (g0 - 1.0);
-  const double rl = std::fmax(ql[ID], p.smallr), ul = ql[IU], pl = std::fmax(ql[IP], rl * p.smallp);
-  const double rr = std::fmax(qr[ID], p.smallr), ur = qr[IU], pr = std::fmax(qr[IP], rr * p.smallp);
-  const double cl = std::sqrt(g0 * pl / rl), cr = std::sqrt(g0 * pr / rr);
+  const real g0 = p.gamma0, entho = 1.0 / (g0 - 1.0);
+  const real rl = FMAX(ql[ID], p.smallr), ul = ql[IU], pl = FMAX(ql[IP], rl * p.smallp);
+  const real rr = FMAX(qr[ID], p.smallr), ur = qr[IU], pr = FMAX(qr[IP], rr * p.smallp);
+  const real cl = SQRT(g0 * pl / rl), cr = SQRT(g0 * pr / rr);
   S5 uL, uR, fL, fR;
   cons_flux(nv, ql, entho, hll, uL, fL);
   cons_flux(nv, qr, entho, hll, uR, fR);
   if (!hll) {
-    const double cmax = std::fmax(std::fabs(ul) + cl, std::fabs(ur) + cr);
+    const real cmax = FMAX(FABS(ul) + cl, FABS(ur) + cr);
     for (int n = 0; n < nv; ++n) f[n] = 0.5 * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
   } else {
-    const double SL = std::fmin(std::fmin(ul, ur) - std::fmax(cl, cr), 0.0);
-    const double SR = std::fmax(std::fmax(ul, ur) + std::fmax(cl, cr), 0.0);
+    const real SL = FMIN(FMIN(ul, ur) - FMAX(cl, cr), 0.0);
+    const real SR = FMAX(FMAX(ul, ur) + FMAX(cl, cr), 0.0);
     for (int n = 0; n < nv; ++n)
       f[n] = (SR * fL[n] - SL * fR[n] + SR * SL * (uR[n] - uL[n])) / (SR - SL);
   }
 }
 
 void solve_hllc(const ppk_mhd_params &p, int nv, const S5 &ql, const S5 &qr, S5 &f) {
-  const double g0 = p.gamma0, entho = 1.0 / (g0 - 1.0);
-  const double rl = std::fmax(ql[ID], p.smallr), pl = std::fmax(ql[IP], rl * p.smallp), ul = ql[IU];
-  double ecinl = 0.5 * rl * ul * ul;
+  const real g0 = p.gamma0, entho = 1.0 / (g0 - 1.0);
+  const real rl = FMAX(ql[ID], p.smallr), pl = FMAX(ql[IP], rl * p.smallp), ul = ql[IU];
+  real ecinl = 0.5 * rl * ul * ul;
   ecinl += 0.5 * rl * ql[IV] * ql[IV];
   if (nv == 5) ecinl += 0.5 * rl * ql[IW] * ql[IW];
-  const double etotl = pl * entho + ecinl, ptotl = pl;
-  const double rr = std::fmax(qr[ID], p.smallr), pr = std::fmax(qr[IP], rr * p.smallp), ur = qr[IU];
-  double ecinr = 0.5 * rr * ur * ur;
+  const real etotl = pl * entho + ecinl, ptotl = pl;
+  const real rr = FMAX(qr[ID], p.smallr), pr = FMAX(qr[IP], rr * p.smallp), ur = qr[IU];
+  real ecinr = 0.5 * rr * ur * ur;
   ecinr += 0.5 * rr * qr[IV] * qr[IV];
   if (nv == 5) ecinr += 0.5 * rr * qr[IW] * qr[IW];
-  const double etotr = pr * entho + ecinr, ptotr = pr;
-  const double cfl_ = std::sqrt(std::fmax(g0 * pl / rl, p.smallc * p.smallc));
-  const double cfr = std::sqrt(std::fmax(g0 * pr / rr, p.smallc * p.smallc));
-  const double SL = std::fmin(ul, ur) - std::fmax(cfl_, cfr);
-  const double SR = std::fmax(ul, ur) + std::fmax(cfl_, cfr);
-  const double rcl = rl * (ul - SL), rcr = rr * (SR - ur);
-  const double ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
-  const double ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
-  const double rstarl = rl * (SL - ul) / (SL - ustar);
-  const double etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar) / (SL - ustar);
-  const double rstarr = rr * (SR - ur) / (SR - ustar);
-  const double etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar) / (SR - ustar);
-  double ro, uo, ptoto, etoto;
+  const real etotr = pr * entho + ecinr, ptotr = pr;
+  const real cfl_ = SQRT(FMAX(g0 * pl / rl, p.smallc * p.smallc));
+  const real cfr = SQRT(FMAX(g0 * pr / rr, p.smallc * p.smallc));
+  const real SL = FMIN(ul, ur) - FMAX(cfl_, cfr);
+  const real SR = FMAX(ul, ur) + FMAX(cfl_, cfr);
+  const real rcl = rl * (ul - SL), rcr = rr * (SR - ur);
+  const real ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
+  const real ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
+  const real rstarl = rl * (SL - ul) / (SL - ustar);
+  const real etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar) / (SL - ustar);
+  const real rstarr = rr * (SR - ur) / (SR - ustar);
+  const real etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar) / (SR - ustar);
+  real ro, uo, ptoto, etoto;
   if (SL > 0.0) { ro = rl; uo = ul; ptoto = ptotl; etoto = etotl; }
   else if (ustar > 0.0) { ro = rstarl; uo = ustar; ptoto = ptotstar; etoto = etotstarl; }
   else if (SR > 0.0) { ro = rstarr; uo = ustar; ptoto = ptotstar; etoto = etotstarr; }
@@ -249,20 +297,20 @@ struct HydroSolver {
     return s;
   }
   // compute_dt_local (SolverHydroMuscl.h:650-690): interior cells of the current state
-  double inv_dt(const std::vector<double> &A) const {
-    double inv = 0.0;
+  real inv_dt(const std::vector<double> &A) const {
+    real inv = 0.0;
     for (int k = g.dim == 3 ? g.gw : 0; k < (g.dim == 3 ? g.ks - g.gw : 1); ++k)
       for (int j = g.gw; j < g.js - g.gw; ++j)
         for (int i = g.gw; i < g.is - g.gw; ++i) {
           S5 q;
-          double c = 0.0;
+          real c = 0.0;
           to_prim(p, g.nv, cell(A, i, j, k), c, q);
-          const double vx = c + std::fabs(q[IU]), vy = c + std::fabs(q[IV]);
+          const real vx = c + FABS(q[IU]), vy = c + FABS(q[IV]);
           if (g.dim == 3) {
-            const double vz = c + std::fabs(q[IW]);
-            inv = std::fmax(inv, vx / p.dx + vy / p.dy + vz / p.dz);
+            const real vz = c + FABS(q[IW]);
+            inv = FMAX(inv, vx / p.dx + vy / p.dy + vz / p.dz);
           } else {
-            inv = std::fmax(inv, vx / p.dx + vy / p.dy);
+            inv = FMAX(inv, vx / p.dx + vy / p.dy);
           }
         }
     return inv;
@@ -284,10 +332,11 @@ struct HydroSolver {
   }
   // trace_unsplit_{2d,3d}_along_dir of a cell to face (dir, side: 0 min, 1 max)
   S5 trace(const S5 &q, const S5 &dx, const S5 &dy, const S5 &dz, int dir, int side,
-           double dtdx, double dtdy, double dtdz) const {
-    const double g0 = p.gamma0;
-    const double r = q[ID], pr = q[IP], u = q[IU], v = q[IV], w = g.dim == 3 ? q[IW] : 0.0;
-    double sr0, sp0, su0, sv0, sw0 = 0.0;
+           real dtdx, real dtdy, real dtdz) const {
+    HORA_STAGE(4);
+    const real g0 = p.gamma0;
+    const real r = q[ID], pr = q[IP], u = q[IU], v = q[IV], w = g.dim == 3 ? q[IW] : 0.0;
+    real sr0, sp0, su0, sv0, sw0 = 0.0;
     if (g.dim == 3) {
       sr0 = -u * dx[ID] - v * dy[ID] - w * dz[ID] - (dx[IU] + dy[IV] + dz[IW]) * r;
       sp0 = -u * dx[IP] - v * dy[IP] - w * dz[IP] - (dx[IU] + dy[IV] + dz[IW]) * g0 * pr;
@@ -300,31 +349,34 @@ struct HydroSolver {
       su0 = -u * dx[IU] - v * dy[IU] - (dx[IP]) / r;
       sv0 = -u * dx[IV] - v * dy[IV] - (dy[IP]) / r;
     }
+    HORA_STAGE(5);
     const S5 &d = dir == 0 ? dx : dir == 1 ? dy : dz;
-    const double dtd = dir == 0 ? dtdx : dir == 1 ? dtdy : dtdz;
-    const double s0[5] = {sr0, sp0, su0, sv0, sw0};
-    const double c0[5] = {r, pr, u, v, w};
+    const real dtd = dir == 0 ? dtdx : dir == 1 ? dtdy : dtdz;
+    const real s0[5] = {sr0, sp0, su0, sv0, sw0};
+    const real c0[5] = {r, pr, u, v, w};
     S5 out;
     for (int n = 0; n < g.nv; ++n)
       out[n] = side == 0 ? c0[n] - 0.5 * d[n] + s0[n] * dtd * 0.5
                          : c0[n] + 0.5 * d[n] + s0[n] * dtd * 0.5;
-    out[ID] = std::fmax(p.smallr, out[ID]);
+    out[ID] = FMAX(p.smallr, out[ID]);
     return out;
   }
   // godunov_unsplit_impl, implementation version 0 (SolverHydroMuscl.cpp:282-405)
   void godunov(std::vector<double> &in, std::vector<double> &out, double dt_) {
+    HORA_STAGE(0);
     boundaries(p, in.data());
     out = in;
+    HORA_STAGE(1);
     for (int k = 0; k < g.ks; ++k)
       for (int j = 0; j < g.js; ++j)
         for (int i = 0; i < g.is; ++i) {
           S5 q;
-          double c;
+          real c;
           to_prim(p, g.nv, cell(in, i, j, k), c, q);
-          for (int v = 0; v < g.nv; ++v) Q[g.at(i, j, k, v)] = q[v];
+          for (int v = 0; v < g.nv; ++v) Q[g.at(i, j, k, v)] = D(q[v]);
         }
-    const double dtdx = dt_ / p.dx, dtdy = dt_ / p.dy, dtdz = g.dim == 3 ? dt_ / p.dz : 0.0;
-    const double dtd[3] = {dtdx, dtdy, dtdz};
+    const real dtdx = dt_ / p.dx, dtdy = dt_ / p.dy, dtdz = g.dim == 3 ? dt_ / p.dz : 0.0;  // once a step
+    const real dtd[3] = {dtdx, dtdy, dtdz};
     const int k0 = g.dim == 3 ? g.gw : 0, k1 = g.dim == 3 ? g.ks - g.gw : 0;
     // ComputeAndStoreFluxesFunctor: the left face of every cell of [gw, size-gw]
 #pragma omp parallel for collapse(2) schedule(static)
@@ -332,11 +384,13 @@ struct HydroSolver {
       for (int j = g.gw; j <= g.js - g.gw; ++j)
         for (int i = g.gw; i <= g.is - g.gw; ++i) {
           S5 dx, dy, dz;
+          HORA_STAGE(2);
           cell_slopes(i, j, k, dx, dy, dz);
           const S5 qloc = cell(Q, i, j, k);
           for (int dir = 0; dir < g.dim; ++dir) {
             const int ni = i - (dir == 0), nj = j - (dir == 1), nk = k - (dir == 2);
             S5 ndx, ndy, ndz;
+            HORA_STAGE(3);
             cell_slopes(ni, nj, nk, ndx, ndy, ndz);
             S5 qright = trace(qloc, dx, dy, dz, dir, 0, dtdx, dtdy, dtdz);
             S5 qleft = trace(cell(Q, ni, nj, nk), ndx, ndy, ndz, dir, 1, dtdx, dtdy, dtdz);
@@ -346,15 +400,16 @@ struct HydroSolver {
             S5 f;
             solve(p, g.nv, qleft, qright, f);
             std::swap(f[IU], f[nrm]);  // and back
-            for (int v = 0; v < g.nv; ++v) F[dir][g.at(i, j, k, v)] = f[v] * dtd[dir];
+            for (int v = 0; v < g.nv; ++v) F[dir][g.at(i, j, k, v)] = D(f[v] * dtd[dir]);
           }
         }
     // UpdateFunctor: +F(i) - F(i+1) per direction, in that order, variable by variable
+    HORA_STAGE(6);
     for (int k = k0; k < (g.dim == 3 ? k1 : 1); ++k)
       for (int j = g.gw; j < g.js - g.gw; ++j)
         for (int i = g.gw; i < g.is - g.gw; ++i)
           for (int v = 0; v < g.nv; ++v) {
-            double &o = out[g.at(i, j, k, v)];
+            real o = out[g.at(i, j, k, v)];
             o += F[0][g.at(i, j, k, v)];
             o -= F[0][g.at(i + 1, j, k, v)];
             o += F[1][g.at(i, j, k, v)];
@@ -363,6 +418,7 @@ struct HydroSolver {
               o += F[2][g.at(i, j, k, v)];
               o -= F[2][g.at(i, j, k + 1, v)];
             }
+            out[g.at(i, j, k, v)] = D(o);
           }
   }
   // SolverBase::next_iteration (src/shared/SolverBase.cpp:161-219) with the hydro dt
@@ -370,7 +426,8 @@ struct HydroSolver {
     std::vector<double> &cur = (it % 2 == 0) ? U : U2;
     std::vector<double> &nxt = (it % 2 == 0) ? U2 : U;
     if (dt_mode == PPK_DT_CFL) {
-      dt = p.cfl / inv_dt(cur);
+      HORA_STAGE(7);
+      dt = p.cfl / D(inv_dt(cur));
       if (t + dt > p.tEnd) dt = p.tEnd - t;
     } else {
       dt = dt_fixed;
@@ -474,11 +531,37 @@ void hora_riemann(const ppk_mhd_params *p, const double *ql, const double *qr, d
     b[v] = qr[v];
   }
   solve(*p, nv, a, b, o);
-  for (int v = 0; v < nv; ++v) f[v] = o[v];
+  for (int v = 0; v < nv; ++v) f[v] = D(o[v]);
 }
 double hora_inv_dt(hora_solver *s) {
   const std::vector<double> &cur = (s->s.it % 2 == 0) ? s->s.U : s->s.U2;
-  return s->s.inv_dt(cur);
+  return D(s->s.inv_dt(cur));
+}
+
+/* operation counts of the steps since the last reset (HORA_COUNT build; zeros otherwise),
+   per stage: counts[5 * stage + {0 add/sub, 1 mul, 2 div, 3 sqrt, 4 min/max/abs}] */
+int hora_op_stages(void) {
+#ifdef HORA_COUNT
+  return kStages;
+#else
+  return 0;
+#endif
+}
+void hora_op_counts(long long *counts) {
+#ifdef HORA_COUNT
+  for (int st = 0; st < kStages; ++st) {
+    const OpCounts &o = g_stage_ops[st];
+    const long long v[5] = {o.add, o.mul, o.div, o.sqrt, o.sel};
+    for (int m = 0; m < 5; ++m) counts[5 * st + m] = v[m];
+  }
+#else
+  (void)counts;
+#endif
+}
+void hora_op_counts_reset(void) {
+#ifdef HORA_COUNT
+  for (auto &o : g_stage_ops) o = OpCounts{};
+#endif
 }
 
 }  // extern "C"

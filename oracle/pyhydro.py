@@ -14,22 +14,25 @@ import numpy as np
 from ppkmhd_amd import _abi as A
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = []
+_LIBS = {}
+_NAMES = ("libhydro_oracle.so", "libhydro_oracle_count.so")
 
 
 def build(force=False):
-    """Compile oracle/libhydro_oracle.so with the committed Makefile."""
-    out = os.path.join(_HERE, "libhydro_oracle.so")
-    if force or not os.path.exists(out):
-        cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["libhydro_oracle.so"]
-        subprocess.run(cmd, check=True)
-    return out
+    """Compile oracle/libhydro_oracle.so (and the operation-counting build) with the committed
+    Makefile."""
+    for name in _NAMES:
+        if force or not os.path.exists(os.path.join(_HERE, name)):
+            cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else []) + [name]
+            subprocess.run(cmd, check=True)
+    return os.path.join(_HERE, _NAMES[0])
 
 
-def _lib():
-    if _LIB:
-        return _LIB[0]
-    path = os.path.join(_HERE, "libhydro_oracle.so")
+def _lib(counting=False):
+    name = _NAMES[1] if counting else _NAMES[0]
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, name)
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)
@@ -50,8 +53,29 @@ def _lib():
     lib.hora_riemann.argtypes = [P, dp, dp, dp]
     lib.hora_inv_dt.argtypes = [C.c_void_p]
     lib.hora_inv_dt.restype = C.c_double
-    _LIB.append(lib)
+    lib.hora_op_stages.restype = C.c_int
+    lib.hora_op_counts.argtypes = [C.POINTER(C.c_longlong)]
+    lib.hora_op_counts_reset.argtypes = []
+    _LIBS[name] = lib
     return lib
+
+
+# stages of the counting build (hydro_oracle.cpp)
+OP_STAGES = ("boundaries", "primitives", "slopes", "slopes_neighbour", "sources",
+             "faces_riemann", "update", "cfl")
+
+
+def op_counts():
+    """{stage: [add, mul, div, sqrt, minmax]} of the counting build since the last reset."""
+    L = _lib(True)
+    n = L.hora_op_stages()
+    buf = (C.c_longlong * (5 * n))()
+    L.hora_op_counts(buf)
+    return {OP_STAGES[s]: list(buf[5 * s:5 * s + 5]) for s in range(n)}
+
+
+def op_counts_reset():
+    _lib(True).hora_op_counts_reset()
 
 
 def _dp(a):
@@ -99,13 +123,14 @@ def riemann(p, ql, qr):
 class Solver:
     """The reference's Hydro_Muscl time loop (U/U2 ping-pong, dt from the CFL or fixed)."""
 
-    def __init__(self, p):
+    def __init__(self, p, counting=False):
         self.p = p
-        self._h = _lib().hora_create(C.byref(p))
+        self._L = _lib(counting)
+        self._h = self._L.hora_create(C.byref(p))
 
     def close(self):
         if self._h:
-            _lib().hora_destroy(self._h)
+            self._L.hora_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -115,28 +140,28 @@ class Solver:
             pass
 
     def init_problem(self, prob):
-        _lib().hora_init_problem(self._h, C.byref(prob))
+        self._L.hora_init_problem(self._h, C.byref(prob))
 
     def set_state(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         assert U.shape == shape(self.p)
-        _lib().hora_set_state(self._h, _dp(U))
+        self._L.hora_set_state(self._h, _dp(U))
 
     def get_state(self):
         U = zeros(self.p)
-        _lib().hora_get_state(self._h, _dp(U))
+        self._L.hora_get_state(self._h, _dp(U))
         return U
 
     def step(self, dt=0.0, dt_mode=A.DT_FIXED):
-        _lib().hora_step(self._h, float(dt), int(dt_mode))
+        self._L.hora_step(self._h, float(dt), int(dt_mode))
 
     def time(self):
         t, dt, it = C.c_double(), C.c_double(), C.c_longlong()
-        _lib().hora_get_time(self._h, C.byref(t), C.byref(dt), C.byref(it))
+        self._L.hora_get_time(self._h, C.byref(t), C.byref(dt), C.byref(it))
         return t.value, dt.value, it.value
 
     def set_time(self, t, iteration):
-        _lib().hora_set_time(self._h, float(t), int(iteration))
+        self._L.hora_set_time(self._h, float(t), int(iteration))
 
     def inv_dt(self):
-        return _lib().hora_inv_dt(self._h)
+        return self._L.hora_inv_dt(self._h)

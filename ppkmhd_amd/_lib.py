@@ -107,6 +107,10 @@ SIGNATURES = {
     "ppk_hydro_set_time": (C.c_int, [_H, C.c_double, C.c_int64]),
     "ppk_hydro_finished": (C.c_int, [_H, C.POINTER(C.c_int)]),
     "ppk_hydro_synchronize": (C.c_int, [_H]),
+    "ppk_hydro_timers_enable": (C.c_int, [_H, C.c_uint32]),
+    "ppk_hydro_timers_read": (C.c_int, [_H, C.c_int, _DP, C.POINTER(C.c_int64)]),
+    "ppk_hydro_timer_count": (C.c_int, []),
+    "ppk_hydro_timer_name": (C.c_char_p, [C.c_int]),
 }
 
 _lib = None

@@ -61,9 +61,9 @@ __device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &
   const double vy = c + fabs(q[IV]);
   if constexpr (NV == 5) {
     const double vz = c + fabs(q[IW]);
-    return vx / h.dx + vy / h.dy + vz / h.dz;
+    return divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy) + divr(vz, h.dz, h.idz);
   } else {
-    return vx / h.dx + vy / h.dy;
+    return divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy);
   }
 }
 
@@ -147,9 +147,9 @@ __global__ void __launch_bounds__(kBlock) k_hprim(const double *__restrict__ U,
         double w;
         if constexpr (NV == 5) {
           const double vz = c + fabs(q[IW]);
-          w = vx / h.dx + vy / h.dy + vz / h.dz;
+          w = divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy) + divr(vz, h.dz, h.idz);
         } else {
-          w = vx / h.dx + vy / h.dy;
+          w = divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy);
         }
         best = fmax(best, w);
       }
@@ -553,20 +553,27 @@ hipError_t hydro_prim_dt(const HydroDev &h, const double *U, double *Q, double *
   return hipGetLastError();
 }
 
-hipError_t hydro_staged(const HydroDev &h, const double *Q, double *F, double *Uout,
-                        const double *ctrl, hipStream_t s) {
+hipError_t hydro_staged_fluxes(const HydroDev &h, const double *Q, double *F,
+                               const double *ctrl, hipStream_t s) {
   const dim3 b(kBlock);
   if (h.dim == 3) {
     const long long nf = (long long)(h.nx + 1) * (h.ny + 1) * (h.nz + 1);
-    const long long nu = (long long)h.nx * h.ny * h.nz;
     hipLaunchKernelGGL((k_hflux<3, 5>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
-    hipLaunchKernelGGL((k_hupdate<3, 5>), dim3(grid_for(nu)), b, 0, s, Uout, F, h, ctrl);
   } else {
     const long long nf = (long long)(h.nx + 1) * (h.ny + 1);
-    const long long nu = (long long)h.nx * h.ny;
     hipLaunchKernelGGL((k_hflux<2, 4>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
-    hipLaunchKernelGGL((k_hupdate<2, 4>), dim3(grid_for(nu)), b, 0, s, Uout, F, h, ctrl);
   }
+  return hipGetLastError();
+}
+
+hipError_t hydro_staged_update(const HydroDev &h, const double *F, double *Uout,
+                               const double *ctrl, hipStream_t s) {
+  const dim3 b(kBlock);
+  const long long nu = (long long)h.nx * h.ny * (h.dim == 3 ? h.nz : 1);
+  if (h.dim == 3)
+    hipLaunchKernelGGL((k_hupdate<3, 5>), dim3(grid_for(nu)), b, 0, s, Uout, F, h, ctrl);
+  else
+    hipLaunchKernelGGL((k_hupdate<2, 4>), dim3(grid_for(nu)), b, 0, s, Uout, F, h, ctrl);
   return hipGetLastError();
 }
 

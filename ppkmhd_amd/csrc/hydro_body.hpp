@@ -2,7 +2,10 @@
 // the trace of a cell to one of its faces, and the Riemann solvers (approximate two-shock,
 // LLF, HLL, HLLC).  Same operations in the same order as the reference
 // (src/muscl/HydroBaseFunctor{2D,3D}.h, src/shared/RiemannSolvers.h), so a build with
-// -ffp-contract=off reproduces the reference's double results bit for bit.
+// -ffp-contract=off and PPK_EXACT_DIV reproduces the reference's double results bit for bit.
+// Quotients and square roots go through divr / recip / msqrt (mhd_device.hpp): exactly a / b
+// and sqrt in that build; in the default build a denominator used several times is inverted
+// once and the GPU uses refined reciprocal / reciprocal-square-root sequences.
 // NV = 4 (2D: ID, IP, IU, IV) or 5 (3D: + IW).
 #pragma once
 
@@ -25,6 +28,7 @@ enum { FACE_XMIN = 0, FACE_XMAX = 1, FACE_YMIN = 2, FACE_YMAX = 3, FACE_ZMIN = 4
 // the parameters the hydro kernels read
 struct HydroPar {
   double gamma0, gamma6, smallr, smallc, smallp, smallpp, slope_type;
+  double entho;  // 1 / (gamma0 - 1), as each solver computes it (same IEEE value)
   int riemann;  // enum ppk_riemann
 };
 
@@ -32,19 +36,20 @@ struct HydroPar {
 template <int NV>
 PPK_HD void prims(const HSt<NV> &u, const HydroPar &P, double &c, HSt<NV> &q) {
   const double d = fmax(u[ID], P.smallr);
-  const double ux = u[IU] / d;
-  const double uy = u[IV] / d;
+  const double id = recip(d);
+  const double ux = divr(u[IU], d, id);
+  const double uy = divr(u[IV], d, id);
   double eken;
   double uz = 0.0;
   if constexpr (NV == 5) {
-    uz = u[IW] / d;
+    uz = divr(u[IW], d, id);
     eken = 0.5 * (ux * ux + uy * uy + uz * uz);
   } else {
     eken = 0.5 * (ux * ux + uy * uy);
   }
-  const double e = u[IP] / d - eken;
+  const double e = divr(u[IP], d, id) - eken;
   const double p = fmax((P.gamma0 - 1.0) * d * e, d * P.smallp);
-  c = sqrt(P.gamma0 * (p) / d);
+  c = msqrt(divr(P.gamma0 * (p), d, id));
   q[ID] = d;
   q[IP] = p;
   q[IU] = ux;
@@ -88,18 +93,19 @@ template <int NV>
 PPK_HD void sources(const HSt<NV> &q, const HSt<NV> &dqX, const HSt<NV> &dqY, const HSt<NV> &dqZ,
                     double gamma0, HSt<NV> &s0) {
   const double r = q[ID], p = q[IP], u = q[IU], v = q[IV];
+  const double ir = recip(r);
   if constexpr (NV == 5) {
     const double w = q[IW];
     s0[ID] = -u * dqX[ID] - v * dqY[ID] - w * dqZ[ID] - (dqX[IU] + dqY[IV] + dqZ[IW]) * r;
     s0[IP] = -u * dqX[IP] - v * dqY[IP] - w * dqZ[IP] - (dqX[IU] + dqY[IV] + dqZ[IW]) * gamma0 * p;
-    s0[IU] = -u * dqX[IU] - v * dqY[IU] - w * dqZ[IU] - (dqX[IP]) / r;
-    s0[IV] = -u * dqX[IV] - v * dqY[IV] - w * dqZ[IV] - (dqY[IP]) / r;
-    s0[IW] = -u * dqX[IW] - v * dqY[IW] - w * dqZ[IW] - (dqZ[IP]) / r;
+    s0[IU] = -u * dqX[IU] - v * dqY[IU] - w * dqZ[IU] - divr(dqX[IP], r, ir);
+    s0[IV] = -u * dqX[IV] - v * dqY[IV] - w * dqZ[IV] - divr(dqY[IP], r, ir);
+    s0[IW] = -u * dqX[IW] - v * dqY[IW] - w * dqZ[IW] - divr(dqZ[IP], r, ir);
   } else {
     s0[ID] = -u * dqX[ID] - v * dqY[ID] - (dqX[IU] + dqY[IV]) * r;
     s0[IP] = -u * dqX[IP] - v * dqY[IP] - (dqX[IU] + dqY[IV]) * gamma0 * p;
-    s0[IU] = -u * dqX[IU] - v * dqY[IU] - (dqX[IP]) / r;
-    s0[IV] = -u * dqX[IV] - v * dqY[IV] - (dqY[IP]) / r;
+    s0[IU] = -u * dqX[IU] - v * dqY[IU] - divr(dqX[IP], r, ir);
+    s0[IV] = -u * dqX[IV] - v * dqY[IV] - divr(dqY[IP], r, ir);
   }
 }
 
@@ -122,7 +128,7 @@ PPK_HD void cmpflx(const HSt<NV> &qg, HSt<NV> &flux, const HydroPar &P) {
   flux[IU] = flux[ID] * qg[IU] + qg[IP];
   flux[IV] = flux[ID] * qg[IV];
   if constexpr (NV == 5) flux[IW] = flux[ID] * qg[IW];
-  const double entho = 1.0 / (P.gamma0 - 1.0);
+  const double entho = P.entho;
   double ekin = 0.5 * qg[ID] * (qg[IU] * qg[IU] + qg[IV] * qg[IV]);
   if constexpr (NV == 5) ekin += 0.5 * qg[ID] * (qg[IW] * qg[IW]);
   const double etot = qg[IP] * entho + ekin;
@@ -139,26 +145,28 @@ PPK_HD void riemann_approx(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux,
   const double rr = fmax(qr[ID], smallr), ur = qr[IU], pr = fmax(qr[IP], rr * smallp);
   const double cl = gamma0 * pl * rl;
   const double cr = gamma0 * pr * rr;
-  double wl = sqrt(cl);
-  double wr = sqrt(cr);
-  double pstar = fmax(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), 0.0);
+  const double ipl = recip(pl), ipr = recip(pr);
+  double wl = msqrt(cl);
+  double wr = msqrt(cr);
+  double pstar = fmax(divr((wr * pl + wl * pr) + wl * wr * (ul - ur), wl + wr, recip(wl + wr)), 0.0);
   double pold = pstar;
   double conv = 1.0;
   for (int iter = 0; (iter < 10) && (conv > 1e-6); ++iter) {
-    const double wwl = sqrt(cl * (1.0 + gamma6 * (pold - pl) / pl));
-    const double wwr = sqrt(cr * (1.0 + gamma6 * (pold - pr) / pr));
-    const double qql = 2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
-    const double qqr = 2.0 * wwr * wwr * wwr / (wwr * wwr + cr);
-    const double usl = ul - (pold - pl) / wwl;
-    const double usr = ur + (pold - pr) / wwr;
-    const double delp = fmax(qqr * qql / (qqr + qql) * (usl - usr), -pold);
+    const double wwl = msqrt(cl * (1.0 + divr(gamma6 * (pold - pl), pl, ipl)));
+    const double wwr = msqrt(cr * (1.0 + divr(gamma6 * (pold - pr), pr, ipr)));
+    const double qql = divr(2.0 * wwl * wwl * wwl, wwl * wwl + cl, recip(wwl * wwl + cl));
+    const double qqr = divr(2.0 * wwr * wwr * wwr, wwr * wwr + cr, recip(wwr * wwr + cr));
+    const double usl = ul - divr(pold - pl, wwl, recip(wwl));
+    const double usr = ur + divr(pold - pr, wwr, recip(wwr));
+    const double delp = fmax(divr(qqr * qql, qqr + qql, recip(qqr + qql)) * (usl - usr), -pold);
     pold = pold + delp;
-    conv = fabs(delp / (pold + smallpp));
+    conv = fabs(divr(delp, pold + smallpp, recip(pold + smallpp)));
   }
   pstar = pold;
-  wl = sqrt(cl * (1.0 + gamma6 * (pstar - pl) / pl));
-  wr = sqrt(cr * (1.0 + gamma6 * (pstar - pr) / pr));
-  const double ustar = 0.5 * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr);
+  wl = msqrt(cl * (1.0 + divr(gamma6 * (pstar - pl), pl, ipl)));
+  wr = msqrt(cr * (1.0 + divr(gamma6 * (pstar - pr), pr, ipr)));
+  const double ustar =
+      0.5 * (ul + divr(pl - pstar, wl, recip(wl)) + ur - divr(pr - pstar, wr, recip(wr)));
   const double sgnm = copysign(1.0, ustar);
   double ro, uo, po, wo;
   if (sgnm > 0.0) {
@@ -166,18 +174,20 @@ PPK_HD void riemann_approx(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux,
   } else {
     ro = rr; uo = ur; po = pr; wo = wr;
   }
-  const double co = fmax(smallc, sqrt(fabs(gamma0 * po / ro)));
-  const double rstar = fmax((double)(ro / (1.0 + ro * (po - pstar) / (wo * wo))), (double)(smallr));
-  const double cstar = fmax(smallc, sqrt(fabs(gamma0 * pstar / rstar)));
+  const double iro = recip(ro);
+  const double co = fmax(smallc, msqrt(fabs(divr(gamma0 * po, ro, iro))));
+  const double den = 1.0 + divr(ro * (po - pstar), wo * wo, recip(wo * wo));
+  const double rstar = fmax(divr(ro, den, recip(den)), smallr);
+  const double cstar = fmax(smallc, msqrt(fabs(divr(gamma0 * pstar, rstar, recip(rstar)))));
   double spout = co - sgnm * uo;
   double spin = cstar - sgnm * ustar;
-  const double ushock = wo / ro - sgnm * uo;
+  const double ushock = divr(wo, ro, iro) - sgnm * uo;
   if (pstar >= po) {
     spin = ushock;
     spout = ushock;
   }
   const double scr = fmax(spout - spin, smallc + fabs(spout + spin));
-  double frac = 0.5 * (1.0 + (spout + spin) / scr);
+  double frac = 0.5 * (1.0 + divr(spout + spin, scr, recip(scr)));
   if (frac != frac)
     frac = 0.0;
   else
@@ -222,11 +232,11 @@ PPK_HD void cons_and_flux(const HSt<NV> &q, double entho, bool hll, HSt<NV> &u, 
 template <int NV>
 PPK_HD void riemann_llf(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, const HydroPar &P) {
   const double gamma0 = P.gamma0, smallr = P.smallr, smallp = P.smallp;
-  const double entho = 1.0 / (gamma0 - 1.0);
+  const double entho = P.entho;
   const double rl = fmax(ql[ID], smallr), ul = ql[IU], pl = fmax(ql[IP], rl * smallp);
   const double rr = fmax(qr[ID], smallr), ur = qr[IU], pr = fmax(qr[IP], rr * smallp);
-  const double cl = sqrt(gamma0 * pl / rl);
-  const double cr = sqrt(gamma0 * pr / rr);
+  const double cl = msqrt(divr(gamma0 * pl, rl, recip(rl)));
+  const double cr = msqrt(divr(gamma0 * pr, rr, recip(rr)));
   const double cmax = fmax(fabs(ul) + cl, fabs(ur) + cr);
   HSt<NV> uleft, uright, fleft, fright;
   cons_and_flux(ql, entho, false, uleft, fleft);
@@ -239,25 +249,26 @@ PPK_HD void riemann_llf(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, con
 template <int NV>
 PPK_HD void riemann_hll(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, const HydroPar &P) {
   const double gamma0 = P.gamma0, smallr = P.smallr, smallp = P.smallp;
-  const double entho = 1.0 / (gamma0 - 1.0);
+  const double entho = P.entho;
   const double rl = fmax(ql[ID], smallr), ul = ql[IU], pl = fmax(ql[IP], rl * smallp);
   const double rr = fmax(qr[ID], smallr), ur = qr[IU], pr = fmax(qr[IP], rr * smallp);
-  const double cl = sqrt(gamma0 * pl / rl);
-  const double cr = sqrt(gamma0 * pr / rr);
+  const double cl = msqrt(divr(gamma0 * pl, rl, recip(rl)));
+  const double cr = msqrt(divr(gamma0 * pr, rr, recip(rr)));
   const double SL = fmin(fmin(ul, ur) - fmax(cl, cr), 0.0);
   const double SR = fmax(fmax(ul, ur) + fmax(cl, cr), 0.0);
   HSt<NV> uleft, uright, fleft, fright;
   cons_and_flux(ql, entho, true, uleft, fleft);
   cons_and_flux(qr, entho, true, uright, fright);
+  const double iw = recip(SR - SL);
 #pragma unroll
   for (int n = 0; n < NV; ++n)
-    flux[n] = (SR * fleft[n] - SL * fright[n] + SR * SL * (uright[n] - uleft[n])) / (SR - SL);
+    flux[n] = divr(SR * fleft[n] - SL * fright[n] + SR * SL * (uright[n] - uleft[n]), SR - SL, iw);
 }
 
 template <int NV>
 PPK_HD void riemann_hllc(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, const HydroPar &P) {
   const double gamma0 = P.gamma0, smallr = P.smallr, smallp = P.smallp, smallc = P.smallc;
-  const double entho = 1.0 / (gamma0 - 1.0);
+  const double entho = P.entho;
   const double rl = fmax(ql[ID], smallr);
   const double pl = fmax(ql[IP], rl * smallp);
   const double ul = ql[IU];
@@ -274,18 +285,20 @@ PPK_HD void riemann_hllc(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, co
   if constexpr (NV == 5) ecinr += 0.5 * rr * qr[IW] * qr[IW];
   const double etotr = pr * entho + ecinr;
   const double ptotr = pr;
-  const double cfastl = sqrt(fmax(gamma0 * pl / rl, smallc * smallc));
-  const double cfastr = sqrt(fmax(gamma0 * pr / rr, smallc * smallc));
+  const double cfastl = msqrt(fmax(divr(gamma0 * pl, rl, recip(rl)), smallc * smallc));
+  const double cfastr = msqrt(fmax(divr(gamma0 * pr, rr, recip(rr)), smallc * smallc));
   const double SL = fmin(ul, ur) - fmax(cfastl, cfastr);
   const double SR = fmax(ul, ur) + fmax(cfastl, cfastr);
   const double rcl = rl * (ul - SL);
   const double rcr = rr * (SR - ur);
-  const double ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
-  const double ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
-  const double rstarl = rl * (SL - ul) / (SL - ustar);
-  const double etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar) / (SL - ustar);
-  const double rstarr = rr * (SR - ur) / (SR - ustar);
-  const double etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar) / (SR - ustar);
+  const double isum = recip(rcr + rcl);
+  const double ustar = divr(rcr * ur + rcl * ul + (ptotl - ptotr), rcr + rcl, isum);
+  const double ptotstar = divr(rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur), rcr + rcl, isum);
+  const double isl = recip(SL - ustar), isr = recip(SR - ustar);
+  const double rstarl = divr(rl * (SL - ul), SL - ustar, isl);
+  const double etotstarl = divr((SL - ul) * etotl - ptotl * ul + ptotstar * ustar, SL - ustar, isl);
+  const double rstarr = divr(rr * (SR - ur), SR - ustar, isr);
+  const double etotstarr = divr((SR - ur) * etotr - ptotr * ur + ptotstar * ustar, SR - ustar, isr);
   double ro, uo, ptoto, etoto;
   if (SL > 0.0) {
     ro = rl; uo = ul; ptoto = ptotl; etoto = etotl;

@@ -11,18 +11,28 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "event_timer.hpp"
 #include "hydro_kernels.hpp"
 #include "ppkmhd_amd.h"
 
 namespace ppk {
 namespace {
 
-const HydroTable kHydroFast = {fast::hydro_boundaries, fast::hydro_prim_dt,
-                               fast::hydro_prim_parts, fast::hydro_staged,
-                               fast::hydro_fused,      fast::hydro_fused_parts};
-const HydroTable kHydroStrict = {strict::hydro_boundaries, strict::hydro_prim_dt,
-                                 strict::hydro_prim_parts, strict::hydro_staged,
-                                 strict::hydro_fused,      strict::hydro_fused_parts};
+// stages of the hydro step bracketed by the optional timers (ppk_hydro_timers_*)
+enum HydroStage { HS_BOUNDARIES = 0, HS_PRIM_DT, HS_FLUXES, HS_UPDATE, HS_FUSED, HS_CONTROL,
+                  HS_COUNT };
+const char *const kHydroStageNames[HS_COUNT] = {"boundaries", "primitives_dt", "fluxes",
+                                                "update", "fused", "control"};
+static_assert(HS_COUNT <= ST_COUNT, "the event timer holds ST_COUNT stages");
+
+const HydroTable kHydroFast = {fast::hydro_boundaries,    fast::hydro_prim_dt,
+                               fast::hydro_prim_parts,    fast::hydro_staged_fluxes,
+                               fast::hydro_staged_update, fast::hydro_fused,
+                               fast::hydro_fused_parts};
+const HydroTable kHydroStrict = {strict::hydro_boundaries,    strict::hydro_prim_dt,
+                                 strict::hydro_prim_parts,    strict::hydro_staged_fluxes,
+                                 strict::hydro_staged_update, strict::hydro_fused,
+                                 strict::hydro_fused_parts};
 
 int validate_hydro(const ppk_mhd_params *p, std::string &why) {
   if (!p) { why = "params is NULL"; return PPK_ERR_INVALID_ARGUMENT; }
@@ -91,6 +101,9 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   d.dx = p.dx;
   d.dy = p.dy;
   d.dz = p.dz;
+  d.idx = 1.0 / p.dx;
+  d.idy = 1.0 / p.dy;
+  d.idz = p.dim == 3 ? 1.0 / p.dz : 0.0;
   d.P.gamma0 = p.gamma0;
   d.P.gamma6 = p.gamma6;
   d.P.smallr = p.smallr;
@@ -99,6 +112,7 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   d.P.smallpp = p.smallpp;
   d.P.slope_type = p.slope_type;
   d.P.riemann = p.riemann;
+  d.P.entho = 1.0 / (p.gamma0 - 1.0);  // RiemannSolvers.h: entho = ONE_F / (gamma0 - ONE_F)
   // tuning knob: output rows (2D) / planes (3D) of one marching wave / block
   const char *strip = std::getenv("PPKMHD_HYDRO_STRIP");
   const int rows = strip ? std::atoi(strip) : 0;
@@ -167,6 +181,7 @@ struct ppk_hydro_solver {
   double *ctrl = nullptr;
   double *ctrl_host = nullptr;
   double *parts = nullptr;
+  ppk::EventTimer *timer = nullptr;
   bool fused = false;
   bool cfl_valid = false;  // parts holds the fused kernel's CFL maxima of cur()
   const ppk::KernelTable *K = nullptr;
@@ -174,6 +189,7 @@ struct ppk_hydro_solver {
   long long iter_host = 0;
   std::string err;
 
+  ppk::StageTimer *tm() { return (timer && timer->mask()) ? timer : nullptr; }
   double *cur() { return U[iter_host & 1]; }
   double *nxt() { return U[(iter_host + 1) & 1]; }
   long long cells() const { return (long long)hd.isize * hd.jsize * hd.ksize; }
@@ -216,6 +232,8 @@ void hfree_all(ppk_hydro_t *h) {
   for (void *ptr : {(void *)h->U[0], (void *)h->U[1], (void *)h->Q, (void *)h->F,
                     (void *)h->ctrl, (void *)h->parts})
     if (ptr) (void)hipFree(ptr);
+  delete h->timer;
+  h->timer = nullptr;
   if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
   if (h->stream) (void)hipStreamDestroy(h->stream);
 }
@@ -229,23 +247,28 @@ int hlaunch_step(ppk_hydro_t *h, double dt, int dt_mode, int check_finished) {
   double *in = h->cur(), *out = h->nxt();
   const bool cfl = dt_mode == PPK_DT_CFL;
   const int mode = cfl ? ppk::SETDT_CFL : ppk::SETDT_FIXED;
+  ppk::StageTimer *tm = h->tm();
   HHIP(h, K.begin(h->ctrl, check_finished, s));
   if (h->fused) {
-    HHIP(h, H.boundaries(d, in, out, h->ctrl, s));
+    PPK_TIMED(tm, ppk::HS_BOUNDARIES, HHIP(h, H.boundaries(d, in, out, h->ctrl, s)));
     int nparts = H.fused_parts(d);
     if (cfl && !h->cfl_valid) {
-      HHIP(h, H.prim_dt(d, in, nullptr, h->parts, h->ctrl, 1, s));
+      PPK_TIMED(tm, ppk::HS_PRIM_DT, HHIP(h, H.prim_dt(d, in, nullptr, h->parts, h->ctrl, 1, s)));
       nparts = H.prim_parts(d);
     }
-    HHIP(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s));
-    HHIP(h, H.fused(d, in, out, h->parts, h->ctrl, cfl ? 1 : 0, s));
+    PPK_TIMED(tm, ppk::HS_CONTROL,
+              HHIP(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s)));
+    PPK_TIMED(tm, ppk::HS_FUSED, HHIP(h, H.fused(d, in, out, h->parts, h->ctrl, cfl ? 1 : 0, s)));
     h->cfl_valid = cfl;
   } else {
-    HHIP(h, H.boundaries(d, in, nullptr, h->ctrl, s));
-    HHIP(h, H.prim_dt(d, in, h->Q, h->parts, h->ctrl, cfl ? 1 : 0, s));
-    HHIP(h, K.set_dt(h->ctrl, h->parts, H.prim_parts(d), dt, mode, h->p.cfl, s));
+    PPK_TIMED(tm, ppk::HS_BOUNDARIES, HHIP(h, H.boundaries(d, in, nullptr, h->ctrl, s)));
+    PPK_TIMED(tm, ppk::HS_PRIM_DT,
+              HHIP(h, H.prim_dt(d, in, h->Q, h->parts, h->ctrl, cfl ? 1 : 0, s)));
+    PPK_TIMED(tm, ppk::HS_CONTROL,
+              HHIP(h, K.set_dt(h->ctrl, h->parts, H.prim_parts(d), dt, mode, h->p.cfl, s)));
     HHIP(h, hipMemcpyAsync(out, in, h->state_bytes(), hipMemcpyDeviceToDevice, s));
-    HHIP(h, H.staged(d, h->Q, h->F, out, h->ctrl, s));
+    PPK_TIMED(tm, ppk::HS_FLUXES, HHIP(h, H.staged_fluxes(d, h->Q, h->F, h->ctrl, s)));
+    PPK_TIMED(tm, ppk::HS_UPDATE, HHIP(h, H.staged_update(d, h->F, out, h->ctrl, s)));
   }
   HHIP(h, K.end(h->ctrl, s));
   return PPK_OK;
@@ -304,6 +327,8 @@ int ppk_hydro_create(const ppk_mhd_params *params, ppk_hydro_t **handle) {
   };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(PPK_ERR_HIP, "hipStreamCreateWithFlags failed");
+  h->timer = new (std::nothrow) ppk::EventTimer(h->stream);
+  if (!h->timer) return bail(PPK_ERR_OUT_OF_MEMORY, "host allocation failed");
   const size_t nvals = h->values();
   auto dalloc = [&](double **ptr, size_t count) {
     return hipMalloc((void **)ptr, count * sizeof(double)) == hipSuccess;
@@ -485,6 +510,29 @@ int ppk_hydro_synchronize(ppk_hydro_t *h) {
   if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
   HHIP(h, hipStreamSynchronize(h->stream));
   return PPK_OK;
+}
+
+int ppk_hydro_timers_enable(ppk_hydro_t *h, uint32_t stage_mask) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  HHIP(h, hipStreamSynchronize(h->stream));
+  h->timer->set_mask(stage_mask);
+  return PPK_OK;
+}
+
+int ppk_hydro_timers_read(ppk_hydro_t *h, int stage, double *total_ms, int64_t *launches) {
+  if (!h) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (stage < 0 || stage >= ppk::HS_COUNT) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "bad stage");
+  HHIP(h, h->timer->resolve());
+  if (total_ms) *total_ms = h->timer->total(stage);
+  if (launches) *launches = h->timer->count(stage);
+  return PPK_OK;
+}
+
+int ppk_hydro_timer_count(void) { return ppk::HS_COUNT; }
+
+const char *ppk_hydro_timer_name(int stage) {
+  if (stage < 0 || stage >= ppk::HS_COUNT) return "";
+  return ppk::kHydroStageNames[stage];
 }
 
 int ppk_hydro_stream(ppk_hydro_t *h, void **stream) {

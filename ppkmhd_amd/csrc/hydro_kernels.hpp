@@ -18,6 +18,7 @@ struct HydroDev {
   int pad;
   long long plane, vstride;
   double dx, dy, dz;
+  double idx, idy, idz;  // 1 / dx ... (used only where the build shares reciprocals)
   hyd::HydroPar P;
 };
 
@@ -41,8 +42,10 @@ constexpr int kHydroRows3 = kHydroWaves3 - 4;
   int hydro_prim_parts(const HydroDev &h);                                                       \
   /* staged form: fluxes of every left face of [gw, size-gw], then the update of Uout (which    \
      holds a copy of Uin) */                                                                     \
-  hipError_t hydro_staged(const HydroDev &h, const double *Q, double *F, double *Uout,           \
-                          const double *ctrl, hipStream_t s);                                    \
+  hipError_t hydro_staged_fluxes(const HydroDev &h, const double *Q, double *F,                  \
+                                 const double *ctrl, hipStream_t s);                             \
+  hipError_t hydro_staged_update(const HydroDev &h, const double *F, double *Uout,               \
+                                 const double *ctrl, hipStream_t s);                             \
   /* fused form: one marching kernel from Uin to the interior of Uout; with fold_cfl the        \
      per-wave / per-block CFL maxima of the new state go to parts */                             \
   hipError_t hydro_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,      \
@@ -57,7 +60,8 @@ struct HydroTable {
   decltype(&fast::hydro_boundaries) boundaries;
   decltype(&fast::hydro_prim_dt) prim_dt;
   decltype(&fast::hydro_prim_parts) prim_parts;
-  decltype(&fast::hydro_staged) staged;
+  decltype(&fast::hydro_staged_fluxes) staged_fluxes;
+  decltype(&fast::hydro_staged_update) staged_update;
   decltype(&fast::hydro_fused) fused;
   decltype(&fast::hydro_fused_parts) fused_parts;
 };

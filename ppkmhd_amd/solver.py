@@ -290,6 +290,32 @@ class SolverHydroMuscl:
     def synchronize(self):
         self._check(lib().ppk_hydro_synchronize(self._h))
 
+    @staticmethod
+    def timer_names():
+        L = lib()
+        return [L.ppk_hydro_timer_name(i).decode() for i in range(L.ppk_hydro_timer_count())]
+
+    def timers_enable(self, stages=None):
+        """Bracket the given stages (names or indexes; None = all, [] = off) with hipEvents."""
+        names = self.timer_names()
+        if stages is None:
+            mask = (1 << len(names)) - 1
+        else:
+            mask = 0
+            for st in stages:
+                mask |= 1 << (names.index(st) if isinstance(st, str) else int(st))
+        self._check(lib().ppk_hydro_timers_enable(self._h, mask))
+
+    def timers_read(self):
+        """{stage name: (total ms, launches)} for stages with at least one launch."""
+        out = {}
+        for i, n in enumerate(self.timer_names()):
+            ms, cnt = C.c_double(), C.c_int64()
+            self._check(lib().ppk_hydro_timers_read(self._h, i, C.byref(ms), C.byref(cnt)))
+            if cnt.value:
+                out[n] = (ms.value, cnt.value)
+        return out
+
 
 def hydro_initial_state(params, problem):
     """The initial state of a hydro problem (implode, blast) computed on the host, no GPU."""
