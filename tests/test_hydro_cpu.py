@@ -362,3 +362,32 @@ def test_hydro_create_rejects_bad_params_before_any_device_call():
     p = make_hydro_params(2, 16, 16)
     p.ghost_width = 3
     assert L.ppk_hydro_create(C.byref(p), C.byref(h)) == A.PPK_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_counted_flops_match_bench_constants(H, dim):
+    """The operation-counting build of the oracle gives the same states as the plain build, and
+    its counts per cell-update of the fused algorithm (HLLC) are the figures bench.py prices
+    the fused kernel's roofline with."""
+    import bench
+
+    n = (64, 48, 1) if dim == 2 else (24, 20, 16)
+    p = params(dim, n, riemann=A.RIEMANN_HLLC, bc=A.BC_DIRICHLET, gamma0=1.666)
+    pr = default_problem(A.PROBLEM_BLAST, p, float_round=False)
+    a, b = H.Solver(p), H.Solver(p, counting=True)
+    a.init_problem(pr)
+    b.init_problem(pr)
+    H.op_counts_reset()
+    for s in (a, b):
+        s.step(0, A.DT_CFL)
+    np.testing.assert_array_equal(a.get_state(), b.get_state())
+    c = H.op_counts()
+    nx, ny, nz = n
+    all_cells = (nx + 4) * (ny + 4) * ((nz + 4) if dim == 3 else 1)
+    nflux = (nx + 1) * (ny + 1) * ((nz + 1) if dim == 3 else 1)
+    nint = nx * ny * nz
+    f = {k: sum(v[:4]) for k, v in c.items()}
+    per_cell = (f["primitives"] / all_cells + f["slopes"] / nflux
+                + f["sources"] / (2 * dim * nflux) + f["faces_riemann"] / nflux
+                + f["update"] / nint + f["cfl"] / nint)
+    assert per_cell == pytest.approx(bench.HYDRO_FLOP_PER_CELL[dim], abs=0.5)
