@@ -35,3 +35,20 @@ def test_oracle_matches_golden(oracle, name):
         s.step(0, A.DT_CFL)
     assert np.array_equal(s.get_state(), z["U10"])
     assert tuple(s.time()[:2]) == tuple(z["time10"])
+
+
+@pytest.mark.parametrize("name", ["hydro_implode2d_32x16", "hydro_blast3d_16x12x8"])
+def test_hydro_oracle_matches_golden(name):
+    from oracle import pyhydro
+
+    pyhydro.build()
+    p, z = load_case(name)
+    s = pyhydro.Solver(p)
+    s.set_state(z["U0"])
+    s.step(0, A.DT_CFL)
+    assert np.array_equal(s.get_state(), z["U1"])
+    assert s.time()[0] == z["time1"][0]
+    for _ in range(9):
+        s.step(0, A.DT_CFL)
+    assert np.array_equal(s.get_state(), z["U10"])
+    assert tuple(s.time()[:2]) == tuple(z["time10"])

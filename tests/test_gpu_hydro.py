@@ -4,9 +4,9 @@ PPK_FP64_STRICT: every step bit-identical to the oracle (state incl. ghosts, t, 
 staged and the fused kernels, every Riemann solver, slope types 0/1/2, all boundary types,
 sizes that leave partial strips / blocks.  PPK_FP64 (FMA contraction): relative L-inf
 difference <= 1e-12 after a few steps.  Larger grids: fused and staged strict results equal.
+The committed golden fixtures (tests/golden/hydro_*.npz); at the benchmark sizes, mass and
+energy conserved by reflecting walls and the state physical.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -181,3 +181,57 @@ def test_solver_factory_keys(gpu):
         assert s.m_iteration == 1 and s.m_t > 0
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("name", ["hydro_implode2d_32x16", "hydro_blast3d_16x12x8"])
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("precision", [A.FP64_STRICT, A.FP64])
+def test_golden_fixtures(gpu, name, path, precision):
+    """The committed fixtures (made by tests/golden/make_golden_hydro.py from the oracle)."""
+    from test_golden_cpu import load_case
+
+    p, z = load_case(name)
+    p.precision = precision
+    p.kernel_path = path
+    print(f"hydro golden {name} path={path} precision={precision}", flush=True)
+    with SolverHydroMuscl(p) as g:
+        g.upload(z["U0"])
+        g.step(0, A.DT_CFL)
+        U1 = g.download()
+        for _ in range(9):
+            g.step(0, A.DT_CFL)
+        U10 = g.download()
+        t10 = g.time()[:2]
+    if precision == A.FP64_STRICT:
+        np.testing.assert_array_equal(U1, z["U1"])
+        np.testing.assert_array_equal(U10, z["U10"])
+        assert tuple(t10) == tuple(z["time10"])
+    else:
+        for a, b, tol in ((U1, z["U1"], 1e-12), (U10, z["U10"], 1e-11)):
+            assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bench_size_walls_conserve_mass_and_energy(gpu, dim):
+    """At the benchmark sizes (2D 8192 x 4096, 3D 512 x 384 x 256, the reference's hydro tests
+    at 8x their resolution) reflecting walls keep total mass and energy to roundoff; the state
+    stays physical (positive density and pressure)."""
+    import bench
+
+    p, pr = bench.hydro_workload(dim, 8, A.PATH_AUTO, A.FP64)
+    print(f"hydro bench-size conservation dim={dim} n=({p.nx},{p.ny},{p.nz})", flush=True)
+    g = 2
+    ks = slice(g, g + p.nz) if dim == 3 else slice(0, 1)
+    inner = (slice(None), ks, slice(g, g + p.ny), slice(g, g + p.nx))
+    with SolverHydroMuscl(p, pr) as s:
+        U = s.download()[inner]
+        m0, e0 = U[0].sum(dtype=np.float64), U[1].sum(dtype=np.float64)
+        del U
+        s.run(5)
+        U = s.download()[inner]
+    assert abs(U[0].sum(dtype=np.float64) - m0) <= 1e-12 * m0
+    assert abs(U[1].sum(dtype=np.float64) - e0) <= 1e-12 * e0
+    assert U[0].min() > 0
+    nv = U.shape[0]
+    kin = 0.5 * sum(U[v] ** 2 for v in range(2, nv)) / U[0]
+    assert (U[1] - kin).min() > 0
