@@ -195,6 +195,11 @@ ppk::DevParams make_dev(const ppk_mhd_params &p) {
   // streams, each at its own register budget (3D; measured 3 % faster than 2 at 256^3)
   const char *mix = std::getenv("PPKMHD_RIEMANN_MIX");
   d.riemann_mix = mix ? std::atoi(mix) : 2;
+  // tuning knob: PPKMHD_PIPE=S (2..16) pipelines the 3D step in S z-slabs on two streams
+  const char *pipe = std::getenv("PPKMHD_PIPE");
+  d.pipe = pipe ? std::atoi(pipe) : 0;
+  if (d.pipe > ppk::kMaxSlabs) d.pipe = ppk::kMaxSlabs;
+  if (d.pipe < 0) d.pipe = 0;
   return d;
 }
 
@@ -262,6 +267,8 @@ void free_all(ppk_mhd_t *h) {
   F(h->sur_flags);
   if (h->aux.fork) (void)hipEventDestroy(h->aux.fork);
   if (h->aux.join) (void)hipEventDestroy(h->aux.join);
+  for (hipEvent_t e : h->aux.slab)
+    if (e) (void)hipEventDestroy(e);
   if (h->aux.stream) (void)hipStreamDestroy(h->aux.stream);
   delete h->timer;
   h->timer = nullptr;
@@ -353,8 +360,11 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
          balloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n);
   }
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
-  if (h->fused && h->dp.dim == 3 && h->dp.riemann_mix >= 3) {
-    if (hipStreamCreateWithFlags(&h->aux.stream, hipStreamNonBlocking) != hipSuccess ||
+  if (h->fused && h->dp.dim == 3 && (h->dp.riemann_mix >= 3 || h->dp.pipe > 1)) {
+    bool ev_ok = true;
+    for (int i = 0; i < h->dp.pipe && i < ppk::kMaxSlabs; ++i)
+      ev_ok = ev_ok && hipEventCreateWithFlags(&h->aux.slab[i], hipEventDisableTiming) == hipSuccess;
+    if (!ev_ok || hipStreamCreateWithFlags(&h->aux.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->aux.fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->aux.join, hipEventDisableTiming) != hipSuccess)
       return bail(PPK_ERR_HIP, "side stream creation failed");

@@ -33,18 +33,21 @@ constexpr int kRiemannBlock = 256;
 
 __device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
 
+// A launch covers the tiles lbase + [0, gridDim.x) of the full numbering and, of those, the
+// tile rows tz0 <= tz < tz1 (a z-slab of the pipelined step; the whole box otherwise).
 template <class T, class R>
 __global__ void __launch_bounds__(T::NT) k_half3d(const double *__restrict__ U,
                                                   R *__restrict__ rec, DevParams p,
-                                                  const double *ctrl, int ntx, int nty, int ntz) {
+                                                  const double *ctrl, int ntx, int nty, int ntz,
+                                                  int lbase, int tz0, int tz1) {
   if (!active(ctrl)) return;
   __shared__ R lds[T::LDS];
   R *sF = lds, *sQ = lds + 3 * T::NF, *sE = sQ + 8 * T::NQ;
   // tiles: x fastest, (y, z) tile rows in patches, each XCD a contiguous range
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int L = lbase + xcd_remap(blockIdx.x, gridDim.x);
   int ty, tz;
   patch_row(L / ntx, kPatch, kPatch, (nty + kPatch - 1) / kPatch, ty, tz);
-  if (ty >= nty || tz >= ntz) return;  // padding tiles of the last patches
+  if (ty >= nty || tz >= ntz || tz < tz0 || tz >= tz1) return;  // padding / other slabs
   const int b0x = p.gw - 1 + (L % ntx) * T::TX;
   const int b0y = p.gw - 1 + ty * T::TY;
   const int b0z = p.gw - 1 + tz * T::TZ;
@@ -103,18 +106,25 @@ k_edge3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
 // per XCD first, so the six solves of a run execute on one XCD close in time: each record is
 // read from HBM about once and then from that XCD's L2, and every CU interleaves the
 // memory-bound face waves with the VALU-bound edge waves.
-template <int W, class R>
+// A launch covers the runs rbase + [0, gridDim.x / 6) and, of their cells, those whose z index
+// in the sub-box is in [kk0, kk1) (a z-slab of the pipelined step; the whole box otherwise).
+// RS: the face solver (enum ppk_riemann) as a compile-time constant, so the kernel holds only
+// the solver it runs.
+template <int W, int RS, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
 k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux,
-                R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl) {
+                R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl, int rbase,
+                int kk0, int kk1) {
   if (!active(ctrl)) return;
   const int B = xcd_remap(blockIdx.x, gridDim.x);
-  const long long c = sub_brick_cell(p, bk, B / 6, threadIdx.x);
+  const long long c = sub_brick_cell(p, bk, rbase + B / 6, threadIdx.x);
   if (c < 0) return;
+  const int kk = (int)(c / p.plane) - p.gw;
+  if (kk < kk0 || kk >= kk1) return;
   switch (B % 6) {
-    case 0: face3<0>(rec, flux, p, c); break;
-    case 1: face3<1>(rec, flux, p, c); break;
-    case 2: face3<2>(rec, flux, p, c); break;
+    case 0: face3<0, RS>(rec, flux, p, c); break;
+    case 1: face3<1, RS>(rec, flux, p, c); break;
+    case 2: face3<2, RS>(rec, flux, p, c); break;
     case 3: edge3<2>(rec, emf, p, c); break;
     case 4: edge3<1>(rec, emf, p, c); break;
     default: edge3<0>(rec, emf, p, c); break;
@@ -152,7 +162,7 @@ k_edges_mix3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick
   }
 }
 
-template <class R>
+template <int RS, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
                 R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl) {
@@ -161,8 +171,8 @@ k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
   const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
   if (c < 0) return;
   switch (B % 3) {
-    case 0: face2<0>(rec, flux, p, c); break;
-    case 1: face2<1>(rec, flux, p, c); break;
+    case 0: face2<0, RS>(rec, flux, p, c); break;
+    case 1: face2<1, RS>(rec, flux, p, c); break;
     default: edge2(rec, emf, p, c); break;
   }
 }
@@ -220,14 +230,120 @@ __global__ void __launch_bounds__(kCflBlock) k_cfl(const double *__restrict__ U,
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// the interleaved face/edge launch with the solver as a compile-time constant
+template <int W, class R>
+void launch_mix3d(const DevParams &p, unsigned grid, const R *rec, R *flux, R *emf, Brick bk,
+                  const double *ctrl, int rbase, int kk0, int kk1, hipStream_t s) {
+  const dim3 g(grid), b(kRiemannBlock);
+  switch (p.riemann) {
+    case PPK_RIEMANN_HLLD:
+      hipLaunchKernelGGL((k_riemann_mix3d<W, PPK_RIEMANN_HLLD, R>), g, b, 0, s, rec, flux, emf,
+                         p, bk, ctrl, rbase, kk0, kk1);
+      break;
+    case PPK_RIEMANN_HLL:
+      hipLaunchKernelGGL((k_riemann_mix3d<W, PPK_RIEMANN_HLL, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl, rbase, kk0, kk1);
+      break;
+    default:
+      hipLaunchKernelGGL((k_riemann_mix3d<W, PPK_RIEMANN_LLF, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl, rbase, kk0, kk1);
+      break;
+  }
+}
+template <class R>
+void launch_mix2d(const DevParams &p, unsigned grid, const R *rec, R *flux, R *emf, Brick bk,
+                  const double *ctrl, hipStream_t s) {
+  const dim3 g(grid), b(kRiemannBlock);
+  switch (p.riemann) {
+    case PPK_RIEMANN_HLLD:
+      hipLaunchKernelGGL((k_riemann_mix2d<PPK_RIEMANN_HLLD, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl);
+      break;
+    case PPK_RIEMANN_HLL:
+      hipLaunchKernelGGL((k_riemann_mix2d<PPK_RIEMANN_HLL, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl);
+      break;
+    default:
+      hipLaunchKernelGGL((k_riemann_mix2d<PPK_RIEMANN_LLF, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl);
+      break;
+  }
+}
+
+// tiles of rows tz0 <= tz < tz1 (tz1 < 0: all); launches the blocks of the z-patches they
+// fall in
 template <class T, class R>
 void launch_half3(const DevParams &p, const double *Uin, R *rec, const double *ctrl,
-                  hipStream_t s) {
+                  hipStream_t s, int tz0 = 0, int tz1 = -1) {
   const int ntx = ceil_div(p.nx + 2, T::TX), nty = ceil_div(p.ny + 2, T::TY);
   const int ntz = ceil_div(p.nz + 2, T::TZ);
-  const int rows = ceil_div(nty, kPatch) * kPatch * ceil_div(ntz, kPatch) * kPatch;
-  hipLaunchKernelGGL((k_half3d<T, R>), dim3(ntx * rows), dim3(T::NT), 0, s, Uin, rec, p, ctrl,
-                     ntx, nty, ntz);
+  if (tz1 < 0 || tz1 > ntz) tz1 = ntz;
+  if (tz0 >= tz1) return;
+  const int kp0 = tz0 / kPatch, kp1 = ceil_div(tz1, kPatch);
+  const int prow = ceil_div(nty, kPatch) * kPatch * kPatch;  // tile rows per z-patch
+  hipLaunchKernelGGL((k_half3d<T, R>), dim3(ntx * prow * (kp1 - kp0)), dim3(T::NT), 0, s, Uin,
+                     rec, p, ctrl, ntx, nty, ntz, ntx * prow * kp0, tz0, tz1);
+}
+
+template <class R>
+void launch_half3_any(const DevParams &p, const double *Uin, R *rec, const double *ctrl,
+                      hipStream_t s, int tz0 = 0, int tz1 = -1) {
+  switch (p.tile3) {
+    case 1: launch_half3<Tile3b>(p, Uin, rec, ctrl, s, tz0, tz1); break;
+    case 2: launch_half3<Tile3c>(p, Uin, rec, ctrl, s, tz0, tz1); break;
+    case 3: launch_half3<Tile3d>(p, Uin, rec, ctrl, s, tz0, tz1); break;
+    case 4: launch_half3<Tile3e>(p, Uin, rec, ctrl, s, tz0, tz1); break;
+    default: launch_half3<Tile3>(p, Uin, rec, ctrl, s, tz0, tz1); break;
+  }
+}
+// tile depth in z of the selected half-step tile
+inline int half3_tz(const DevParams &p) {
+  switch (p.tile3) {
+    case 1: return Tile3b::TZ;
+    case 2: return Tile3c::TZ;
+    case 3: return Tile3d::TZ;
+    case 4: return Tile3e::TZ;
+    default: return Tile3::TZ;
+  }
+}
+
+// The 3D step in z-slabs: the half step of slab s (its tile rows) on the side stream, then on
+// the main stream, once it is done, the face and edge solves of the sub-box cells whose
+// records lie in slabs <= s; the half step of slab s+1 overlaps those solves.  Slab s of the
+// solves covers z = [K_s, K_{s+1}) with K_s = TZ * T_s - TZ (T_s the first tile row of half
+// slab s): its records are the planes [K_s, K_{s+1} + 1) counted from gw - 1, all in tile rows
+// < T_{s+1}.
+template <class R>
+hipError_t pipelined_half_riemann(const DevParams &p, const double *Uin, R *rec, R *flux, R *emf,
+                                  const double *ctrl, const AuxStream &aux, hipStream_t s) {
+  const int TZ = half3_tz(p);
+  const int ntz = ceil_div(p.nz + 2, TZ);
+  const int S = p.pipe < ntz ? p.pipe : ntz;
+  const Brick bk = sub_bricks(p);  // flattened runs, rows in 4 x 4 (y, z) patches
+  const long long runs_per_zpatch =
+      (long long)ceil_div(p.ny + 1, kPatch) * kPatch * kPatch * (p.nx + 1);
+  hipError_t e = hipEventRecord(aux.fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(aux.stream, aux.fork, 0);
+  if (e != hipSuccess) return e;
+  for (int sl = 0; sl < S; ++sl) {
+    const int t0 = (int)((long long)ntz * sl / S), t1 = (int)((long long)ntz * (sl + 1) / S);
+    launch_half3_any<R>(p, Uin, rec, ctrl, aux.stream, t0, t1);
+    e = hipEventRecord(aux.slab[sl], aux.stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, aux.slab[sl], 0);
+    if (e != hipSuccess) return e;
+    const int k0 = sl == 0 ? 0 : TZ * t0 - TZ;
+    const int k1 = sl == S - 1 ? p.nz + 1 : TZ * t1 - TZ;
+    if (k0 >= k1) continue;
+    // runs of the z-patches [k0 / 4, ceil(k1 / 4)) of the flattened numbering
+    const long long n0 = (long long)(k0 / kPatch) * runs_per_zpatch;
+    const long long n1 = (long long)ceil_div(k1, kPatch) * runs_per_zpatch;
+    const long long r0 = n0 / 256;
+    long long r1 = (n1 + 255) / 256;
+    if (r1 > bk.count()) r1 = bk.count();
+    launch_mix3d<3, R>(p, (unsigned)(6 * (r1 - r0)), rec, flux, emf, bk, ctrl, (int)r0, k0, k1,
+                       s);
+  }
+  return hipGetLastError();
 }
 
 // the step's three stages in arithmetic type R (double, or float for the FP32-bulk form);
@@ -237,15 +353,16 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
                         R *emf, double *parts, const double *ctrl, int fold_cfl,
                         const SurrogateCfg *sur, const AuxStream *aux, hipStream_t s,
                         StageTimer *tm) {
-  if (p.dim == 3) {
+  if (p.dim == 3 && p.pipe > 1 && aux && aux->slab[0] && !sur && p.riemann_mix == 2 &&
+      p.brick == 0) {
+    // half step and face/edge solves overlapped by z-slabs; timed together
+    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
+    const hipError_t e = pipelined_half_riemann<R>(p, Uin, rec, flux, emf, ctrl, *aux, s);
+    if (e != hipSuccess) return e;
+    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
+  } else if (p.dim == 3) {
     if (tm) tm->mark(ST_FUSED_HALF, true);
-    switch (p.tile3) {
-      case 1: launch_half3<Tile3b>(p, Uin, rec, ctrl, s); break;
-      case 2: launch_half3<Tile3c>(p, Uin, rec, ctrl, s); break;
-      case 3: launch_half3<Tile3d>(p, Uin, rec, ctrl, s); break;
-      case 4: launch_half3<Tile3e>(p, Uin, rec, ctrl, s); break;
-      default: launch_half3<Tile3>(p, Uin, rec, ctrl, s); break;
-    }
+    launch_half3_any<R>(p, Uin, rec, ctrl, s);
     if (tm) tm->mark(ST_FUSED_HALF, false);
     const Brick bk = sub_bricks(p);
     const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
@@ -271,11 +388,11 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
       if (e == hipSuccess) e = hipStreamWaitEvent(s, aux->join, 0);
       if (e != hipSuccess) return e;
     } else if (p.riemann_mix == 2) {  // 3 waves per SIMD, no spill
-      hipLaunchKernelGGL((k_riemann_mix3d<3, R>), dim3((unsigned)(6 * bk.count())), b, 0, s, rec,
-                         flux, emf, p, bk, ctrl);
+      launch_mix3d<3, R>(p, (unsigned)(6 * bk.count()), rec, flux, emf, bk, ctrl, 0, 0, 1 << 30,
+                         s);
     } else if (p.riemann_mix) {
-      hipLaunchKernelGGL((k_riemann_mix3d<4, R>), dim3((unsigned)(6 * bk.count())), b, 0, s, rec,
-                         flux, emf, p, bk, ctrl);
+      launch_mix3d<4, R>(p, (unsigned)(6 * bk.count()), rec, flux, emf, bk, ctrl, 0, 0, 1 << 30,
+                         s);
     } else {
       hipLaunchKernelGGL((k_face3d<0, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
       hipLaunchKernelGGL((k_face3d<1, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
@@ -298,8 +415,7 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL((k_edge2d<R>), g, b, 0, s, rec, emf, p, bk, ctrl);
     } else if (p.riemann_mix) {
-      hipLaunchKernelGGL((k_riemann_mix2d<R>), dim3((unsigned)(3 * bk.count())), b, 0, s, rec,
-                         flux, emf, p, bk, ctrl);
+      launch_mix2d<R>(p, (unsigned)(3 * bk.count()), rec, flux, emf, bk, ctrl, s);
     } else {
       hipLaunchKernelGGL((k_face2d<0, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
       hipLaunchKernelGGL((k_face2d<1, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
@@ -336,13 +452,7 @@ hipError_t half_impl(const DevParams &p, const double *U, R *rec, const double *
                      hipStream_t s) {
   using namespace fusedk;
   if (p.dim == 3) {
-    switch (p.tile3) {
-      case 1: launch_half3<Tile3b>(p, U, rec, ctrl, s); break;
-      case 2: launch_half3<Tile3c>(p, U, rec, ctrl, s); break;
-      case 3: launch_half3<Tile3d>(p, U, rec, ctrl, s); break;
-      case 4: launch_half3<Tile3e>(p, U, rec, ctrl, s); break;
-      default: launch_half3<Tile3>(p, U, rec, ctrl, s); break;
-    }
+    launch_half3_any<R>(p, U, rec, ctrl, s);
   } else {
     const int ntx = ceil_div(p.nx + 2, Tile2::TX), nty = ceil_div(p.ny + 2, Tile2::TY);
     hipLaunchKernelGGL((k_half2d<R>), dim3(ntx * nty), dim3(Tile2::NT), 0, s, U, rec, p, ctrl,

@@ -507,12 +507,12 @@ PPK_HD void store_flux3(R *__restrict__ flux, const DevParams &p, long long c, c
   o[3 * vs] = f[IV];
   o[4 * vs] = f[IW];
 }
-template <int D, class R>
+template <int D, int RS = -1, class R>
 PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
                   long long c) {
   St<R> ql, qr, f;
   face_states3<D>(rec, p, c, ql, qr);
-  riemann_mhd(ql, qr, f, p);
+  riemann_mhd_rs<RS>(ql, qr, f, p);
   store_flux3<D>(flux, p, c, f);
 }
 
@@ -592,12 +592,12 @@ PPK_HD void store_flux2(R *__restrict__ flux, const DevParams &p, long long c, c
   o[4 * vs] = f[IW];
   o[5 * vs] = f[IC];
 }
-template <int D, class R>
+template <int D, int RS = -1, class R>
 PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevParams &p,
                   long long c) {
   St<R> ql, qr, f;
   face_states2<D>(rec, p, c, ql, qr);
-  riemann_mhd(ql, qr, f, p);
+  riemann_mhd_rs<RS>(ql, qr, f, p);
   store_flux2<D>(flux, p, c, f);
 }
 

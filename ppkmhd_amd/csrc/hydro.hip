@@ -184,7 +184,7 @@ __device__ __forceinline__ void cell_slopes(const double *__restrict__ Q, const 
   hyd::slopes(q, qpx, qmx, qpy, qmy, qpz, qmz, h.P, dX, dY, dZ);
 }
 
-template <int DIM, int NV>
+template <int DIM, int NV, int RS>
 __global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
                                                   double *__restrict__ F, HydroDev h,
                                                   const double *ctrl) {
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
       hyd::face_state(qn, d == 0 ? nX : (d == 1 ? nY : nZ), ns0, dtd[d], true, h.P.smallr, ql);
       swap_normal(ql, d);
       swap_normal(qr, d);
-      hyd::riemann(ql, qr, f, h.P);
+      hyd::riemann_rs<RS>(ql, qr, f, h.P);
       swap_normal(f, d);
 #pragma unroll
       for (int v = 0; v < NV; ++v) F[(long long)(d * NV + v) * h.vstride + c] = f[v] * dtd[d];
@@ -265,8 +265,8 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
 // (left state from lane l-1), and finishes the update of (i, s-1) with the flux of that y face.
 constexpr int kWaves2 = 4;
 
-template <int NV>
-__global__ void __launch_bounds__(kWaves2 * 64) k_hfused2d(const double *__restrict__ Uin,
+template <int NV, int RS, int MINW>
+__global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *__restrict__ Uin,
                                                            double *__restrict__ Uout, HydroDev h,
                                                            double *__restrict__ parts,
                                                            const double *ctrl, int fold_cfl) {
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kWaves2 * 64) k_hfused2d(const double *__restr
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
-      hyd::riemann(ql, qr, fy, P);
+      hyd::riemann_rs<RS>(ql, qr, fy, P);
       swap_normal(fy, 1);
 #pragma unroll
       for (int v = 0; v < NV; ++v) fy[v] = fy[v] * dtdy;
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(kWaves2 * 64) k_hfused2d(const double *__restr
         hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
-        hyd::riemann(xl, xmin, fx, P);
+        hyd::riemann_rs<RS>(xl, xmin, fx, P);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           fx[v] = fx[v] * dtdx;
@@ -366,8 +366,8 @@ __global__ void __launch_bounds__(kWaves2 * 64) k_hfused2d(const double *__restr
 // of plane s-1 kept in registers, so the update of (i, j, s-1) completes at plane s.
 constexpr int kW3 = kHydroWaves3;
 
-template <int NV>
-__global__ void __launch_bounds__(kW3 * 64) k_hfused3d(const double *__restrict__ Uin,
+template <int NV, int RS, int MINW>
+__global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__restrict__ Uin,
                                                        double *__restrict__ Uout, HydroDev h,
                                                        double *__restrict__ parts,
                                                        const double *ctrl, int fold_cfl) {
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(kW3 * 64) k_hfused3d(const double *__restrict_
       hyd::face_state(q0, dZ, s0, dtdz, false, P.smallr, qr);
       swap_normal(ql, 2);
       swap_normal(qr, 2);
-      hyd::riemann(ql, qr, fz, P);
+      hyd::riemann_rs<RS>(ql, qr, fz, P);
       swap_normal(fz, 2);
 #pragma unroll
       for (int v = 0; v < NV; ++v) fz[v] = fz[v] * dtdz;
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(kW3 * 64) k_hfused3d(const double *__restrict_
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
-      hyd::riemann(ql, qr, fy, P);
+      hyd::riemann_rs<RS>(ql, qr, fy, P);
       swap_normal(fy, 1);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sQ[b][v][ly][lane] = fy[v] * dtdy;
@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(kW3 * 64) k_hfused3d(const double *__restrict_
       hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
 #pragma unroll
       for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
-      hyd::riemann(xl, xmin, fx, P);
+      hyd::riemann_rs<RS>(xl, xmin, fx, P);
 #pragma unroll
       for (int v = 0; v < NV; ++v) fx[v] = fx[v] * dtdx;
     }
@@ -553,15 +553,26 @@ hipError_t hydro_prim_dt(const HydroDev &h, const double *U, double *Q, double *
   return hipGetLastError();
 }
 
-hipError_t hydro_staged_fluxes(const HydroDev &h, const double *Q, double *F,
-                               const double *ctrl, hipStream_t s) {
+template <int RS>
+void launch_staged_fluxes(const HydroDev &h, const double *Q, double *F, const double *ctrl,
+                          hipStream_t s) {
   const dim3 b(kBlock);
   if (h.dim == 3) {
     const long long nf = (long long)(h.nx + 1) * (h.ny + 1) * (h.nz + 1);
-    hipLaunchKernelGGL((k_hflux<3, 5>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
+    hipLaunchKernelGGL((k_hflux<3, 5, RS>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
   } else {
     const long long nf = (long long)(h.nx + 1) * (h.ny + 1);
-    hipLaunchKernelGGL((k_hflux<2, 4>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
+    hipLaunchKernelGGL((k_hflux<2, 4, RS>), dim3(grid_for(nf)), b, 0, s, Q, F, h, ctrl);
+  }
+}
+
+hipError_t hydro_staged_fluxes(const HydroDev &h, const double *Q, double *F,
+                               const double *ctrl, hipStream_t s) {
+  switch (h.P.riemann) {  // the solver as a compile-time constant of the kernel
+    case PPK_RIEMANN_APPROX: launch_staged_fluxes<PPK_RIEMANN_APPROX>(h, Q, F, ctrl, s); break;
+    case PPK_RIEMANN_HLL: launch_staged_fluxes<PPK_RIEMANN_HLL>(h, Q, F, ctrl, s); break;
+    case PPK_RIEMANN_HLLC: launch_staged_fluxes<PPK_RIEMANN_HLLC>(h, Q, F, ctrl, s); break;
+    default: launch_staged_fluxes<PPK_RIEMANN_LLF>(h, Q, F, ctrl, s); break;
   }
   return hipGetLastError();
 }
@@ -581,15 +592,41 @@ int hydro_fused_parts(const HydroDev &h) {
   return h.dim == 3 ? fused3d_blocks(h) : fused2d_waves(h);
 }
 
-hipError_t hydro_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
-                       const double *ctrl, int fold_cfl, hipStream_t s) {
+template <int RS>
+void launch_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
+                  const double *ctrl, int fold_cfl, hipStream_t s) {
+  // occupancy variant (tuning): 0 = the compiler's register budget, 1 = 4 waves per SIMD
   if (h.dim == 3) {
-    hipLaunchKernelGGL(k_hfused3d<5>, dim3(fused3d_blocks(h)), dim3(kW3 * 64), 0, s, Uin, Uout,
-                       h, parts, ctrl, fold_cfl);
+    if constexpr (RS != PPK_RIEMANN_HLL) {  // no 3D HLL (refused at create)
+      const dim3 g(fused3d_blocks(h)), b(kW3 * 64);
+      if (h.occupancy == 1)
+        hipLaunchKernelGGL((k_hfused3d<5, RS, 4>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+                           fold_cfl);
+      else
+        hipLaunchKernelGGL((k_hfused3d<5, RS, 1>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+                           fold_cfl);
+    }
   } else {
     const int waves = fused2d_waves(h);
-    hipLaunchKernelGGL(k_hfused2d<4>, dim3((waves + kWaves2 - 1) / kWaves2), dim3(kWaves2 * 64),
-                       0, s, Uin, Uout, h, parts, ctrl, fold_cfl);
+    const dim3 g((waves + kWaves2 - 1) / kWaves2), b(kWaves2 * 64);
+    if (h.occupancy == 1)
+      hipLaunchKernelGGL((k_hfused2d<4, RS, 4>), g, b, 0, s, Uin, Uout, h, parts, ctrl, fold_cfl);
+    else
+      hipLaunchKernelGGL((k_hfused2d<4, RS, 1>), g, b, 0, s, Uin, Uout, h, parts, ctrl, fold_cfl);
+  }
+}
+
+hipError_t hydro_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
+                       const double *ctrl, int fold_cfl, hipStream_t s) {
+  switch (h.P.riemann) {  // the solver as a compile-time constant of the kernel
+    case PPK_RIEMANN_APPROX:
+      launch_fused<PPK_RIEMANN_APPROX>(h, Uin, Uout, parts, ctrl, fold_cfl, s);
+      break;
+    case PPK_RIEMANN_HLL: launch_fused<PPK_RIEMANN_HLL>(h, Uin, Uout, parts, ctrl, fold_cfl, s); break;
+    case PPK_RIEMANN_HLLC:
+      launch_fused<PPK_RIEMANN_HLLC>(h, Uin, Uout, parts, ctrl, fold_cfl, s);
+      break;
+    default: launch_fused<PPK_RIEMANN_LLF>(h, Uin, Uout, parts, ctrl, fold_cfl, s); break;
   }
   return hipGetLastError();
 }

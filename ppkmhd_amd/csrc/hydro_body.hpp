@@ -74,16 +74,13 @@ template <int NV>
 PPK_HD void slopes(const HSt<NV> &q, const HSt<NV> &qpx, const HSt<NV> &qmx, const HSt<NV> &qpy,
                    const HSt<NV> &qmy, const HSt<NV> &qpz, const HSt<NV> &qmz,
                    const HydroPar &P, HSt<NV> &dx, HSt<NV> &dy, HSt<NV> &dz) {
-  if (P.slope_type == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) dx[v] = dy[v] = dz[v] = 0.0;
-    return;
-  }
+  // NV == 4 (2D): dz is not touched
+  const bool flat = P.slope_type == 0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    dx[v] = slope1(q[v], qpx[v], qmx[v], P.slope_type);
-    dy[v] = slope1(q[v], qpy[v], qmy[v], P.slope_type);
-    if constexpr (NV == 5) dz[v] = slope1(q[v], qpz[v], qmz[v], P.slope_type);
+    dx[v] = flat ? 0.0 : slope1(q[v], qpx[v], qmx[v], P.slope_type);
+    dy[v] = flat ? 0.0 : slope1(q[v], qpy[v], qmy[v], P.slope_type);
+    if constexpr (NV == 5) dz[v] = flat ? 0.0 : slope1(q[v], qpz[v], qmz[v], P.slope_type);
   }
 }
 
@@ -324,7 +321,22 @@ PPK_HD void riemann_hllc(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, co
   }
 }
 
-// riemann_hydro (RiemannSolvers.h:573-633); HLL exists in 2D only (3D rejects it at create)
+// riemann_hydro (RiemannSolvers.h:573-633) with the solver fixed at compile time (RS: enum
+// ppk_riemann), so each kernel holds one solver and its flux stays in registers
+template <int RS, int NV>
+PPK_HD void riemann_rs(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, const HydroPar &P) {
+  if constexpr (RS == PPK_RIEMANN_APPROX)
+    riemann_approx(ql, qr, flux, P);
+  else if constexpr (RS == PPK_RIEMANN_HLL)
+    riemann_hll(ql, qr, flux, P);
+  else if constexpr (RS == PPK_RIEMANN_HLLC)
+    riemann_hllc(ql, qr, flux, P);
+  else
+    riemann_llf(ql, qr, flux, P);
+}
+
+// riemann_hydro with the solver chosen at run time; HLL exists in 2D only (3D rejects it at
+// create)
 template <int NV>
 PPK_HD void riemann(const HSt<NV> &ql, const HSt<NV> &qr, HSt<NV> &flux, const HydroPar &P) {
   if (P.riemann == PPK_RIEMANN_APPROX)

@@ -117,6 +117,9 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   const char *strip = std::getenv("PPKMHD_HYDRO_STRIP");
   const int rows = strip ? std::atoi(strip) : 0;
   d.strip_rows = rows > 0 ? rows : 64;
+  // tuning knob: PPKMHD_HYDRO_OCC=1 selects the 4-waves-per-SIMD builds of the fused kernels
+  const char *occ = std::getenv("PPKMHD_HYDRO_OCC");
+  d.occupancy = occ ? std::atoi(occ) : 0;
   return d;
 }
 

@@ -77,9 +77,11 @@ struct SurrogateCfg {
   unsigned char *flags = nullptr;  // dim x cells: faces left to the HLLD pass (device scratch)
 };
 // a second stream of default priority and the two events that fork work onto it and join it
+constexpr int kMaxSlabs = 16;
 struct AuxStream {
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t slab[kMaxSlabs] = {};  // z-slab pipeline: half step of slab s done
 };
 // element type: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {

@@ -41,6 +41,8 @@ struct DevParams {
   int brick;            // block-to-cells shape of the fused per-cell kernels, tuning only
   int riemann_mix;      // 1, 2: all face and edge solves in one interleaved launch (4 / 3 waves)
   int bulk32;           // FP32-bulk form: half step, records, face/edge solves in float
+  int pipe;             // 3D: > 1 pipelines the step in that many z-slabs (half step of slab
+                        // s+1 on a side stream while the face/edge solves of slab s run)
 };
 
 template <class R>
@@ -126,6 +128,12 @@ PPK_HD float over_sqrt(float a, float r) { return a / sqrtf(r); }
 #endif
 template <class R>
 PPK_HD R dmin(R a, R b) { return fmin(a, b); }
+// The magnetosonic discriminant d2^2 - c2 bn^2 / rho is >= 0 in exact arithmetic and 0 when
+// the field is along the normal; rounded (and, in the default builds, FMA-contracted or float)
+// it can come out a few ulp below 0, and its square root NaN.  The default builds clamp it at
+// 0; the strict build keeps the reference's expression as written.
+template <class R>
+PPK_HD R nonneg(R x) { return kSharedRecip ? (x > R(0) ? x : R(0)) : x; }
 
 // compare-and-select maxima used by the 2D magnetic Riemann solver (FMAX4/FMIN4/FMAX5,
 // reference src/shared/mhd_utils.h:36-77)
@@ -161,7 +169,7 @@ PPK_HD R speed_fast(const St<R> &q, int dir, R gamma0) {
   R c2 = divr(gamma0 * p, d, id);
   R d2 = R(0.5) * (divr(b2, d, id) + c2);
   R bn = dir == IX ? a : (dir == IY ? b : c);
-  return msqrt(d2 + msqrt(d2 * d2 - divr(c2 * bn * bn, d, id)));
+  return msqrt(d2 + msqrt(nonneg(d2 * d2 - divr(c2 * bn * bn, d, id))));
 }
 
 // per-cell CFL term sum_d (c_f,d + |v_d|)/dx_d, reference src/shared/mhd_utils.h:319-366 and
@@ -172,12 +180,12 @@ PPK_HD double inv_dt_cell(const St<double> &q, const DevParams &p) {
   double b2 = a * a + b * b + c * c;
   double c2 = gamma0 * pp / d;
   double d2 = 0.5 * (b2 / d + c2);
-  double cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
+  double cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * a * a / d)));
   double vx = cf + dabs(q[IU]);
-  cf = sqrt(d2 + sqrt(d2 * d2 - c2 * b * b / d));
+  cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * b * b / d)));
   double vy = cf + dabs(q[IV]);
   if (p.dim == 3) {
-    cf = sqrt(d2 + sqrt(d2 * d2 - c2 * c * c / d));
+    cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * c * c / d)));
     double vz = cf + dabs(q[IW]);
     return vx / p.dx + vy / p.dy + vz / p.dz;
   }
@@ -426,7 +434,7 @@ PPK_HD void riemann_llf(St<R> &ql, St<R> &qr, St<R> &flux,
     R b2 = a * a + b * b + c * c;
     R c2 = R(par.gamma0) * pp / d;
     R d2 = R(0.5) * (b2 / d + c2);
-    R cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
+    R cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * a * a / d)));
     return cf + dabs(q[IU]);
   };
   R vel_info = dmax(info(ql), info(qr));
@@ -443,6 +451,19 @@ PPK_HD void riemann_mhd(St<R> &ql, St<R> &qr, St<R> &flux,
     riemann_hll(ql, qr, flux, p);
   else
     riemann_llf(ql, qr, flux, p);
+}
+// the same with the solver fixed at compile time (RS: enum ppk_riemann; RS < 0: p.riemann at
+// run time), so a kernel can hold only the solver it runs
+template <int RS, class R>
+PPK_HD void riemann_mhd_rs(St<R> &ql, St<R> &qr, St<R> &flux, const DevParams &p) {
+  if constexpr (RS == PPK_RIEMANN_HLLD)
+    riemann_hlld(ql, qr, flux, p);
+  else if constexpr (RS == PPK_RIEMANN_HLL)
+    riemann_hll(ql, qr, flux, p);
+  else if constexpr (RS == PPK_RIEMANN_LLF)
+    riemann_llf(ql, qr, flux, p);
+  else
+    riemann_mhd(ql, qr, flux, p);
 }
 
 // ------------------------------------------------------------------------------------------
