@@ -241,8 +241,9 @@ int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]);
 /* ---- output ---------------------------------------------------------------------------- */
 /* Write a state (host array, reference layout) as VTK ImageData <dir>/<prefix>_<istep>.vti
  * (istep zero-padded to 7 digits), byte for byte as the reference does: CellData arrays rho,
- * energy, rho_vx, rho_vy, rho_vz, bx, by, bz over the interior cells, appended raw binary
- * (ascii = 0) or ascii text (ascii = 1, the reference's [output] outputVtkAscii).
+ * energy, rho_vx, rho_vy, rho_vz, bx, by, bz over the interior cells (the first nbvar of them:
+ * 8 for MHD parameters, ghost_width 3; 4 / 5 for hydro parameters, ghost_width 2), appended raw
+ * binary (ascii = 0) or ascii text (ascii = 1, the reference's [output] outputVtkAscii).
  * Replaces: save_VTK_2D / save_VTK_3D (src/utils/io/IO_VTK.cpp:23-410) with the variable
  * names of src/shared/SolverBase.cpp:48-55. */
 int ppk_mhd_write_vtk(const ppk_mhd_params *params, const double *U, const char *dir,
@@ -392,6 +393,10 @@ int ppk_hydro_set_time(ppk_hydro_t *h, double t, int64_t iteration);
 /* Replaces: SolverBase::finished (src/shared/SolverBase.cpp:195-201). */
 int ppk_hydro_finished(ppk_hydro_t *h, int *finished);
 int ppk_hydro_synchronize(ppk_hydro_t *h);
+/* The handle's current state as a VTK file (ppk_mhd_write_vtk with nbvar 4 / 5).
+ * Replaces: SolverHydroMuscl::save_solution_impl (src/muscl/SolverHydroMuscl.h:809-820). */
+int ppk_hydro_save_vtk(ppk_hydro_t *h, const char *dir, const char *prefix, int istep,
+                       int ascii);
 /* Per-stage GPU timers (hipEvents on the handle's stream), as ppk_mhd_timers_*; stages:
  * 0 boundaries, 1 primitives_dt, 2 fluxes, 3 update (staged form), 4 fused, 5 control.
  * Replaces: the per-stage timers of SolverBase (src/shared/SolverBase.h:35-42, 150-157). */

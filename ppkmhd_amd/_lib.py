@@ -111,6 +111,7 @@ SIGNATURES = {
     "ppk_hydro_timers_read": (C.c_int, [_H, C.c_int, _DP, C.POINTER(C.c_int64)]),
     "ppk_hydro_timer_count": (C.c_int, []),
     "ppk_hydro_timer_name": (C.c_char_p, [C.c_int]),
+    "ppk_hydro_save_vtk": (C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_int, C.c_int]),
 }
 
 _lib = None

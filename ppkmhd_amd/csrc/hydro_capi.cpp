@@ -538,6 +538,17 @@ const char *ppk_hydro_timer_name(int stage) {
   return ppk::kHydroStageNames[stage];
 }
 
+int ppk_hydro_save_vtk(ppk_hydro_t *h, const char *dir, const char *prefix, int istep,
+                       int ascii) {
+  if (!h || !dir || !prefix) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::vector<double> host(h->values());
+  int rc = ppk_hydro_download(h, host.data(), host.size());
+  if (rc != PPK_OK) return rc;
+  rc = ppk_mhd_write_vtk(&h->p, host.data(), dir, prefix, istep, ascii);
+  if (rc != PPK_OK) return hfail(h, rc, ppk::thread_error());
+  return PPK_OK;
+}
+
 int ppk_hydro_stream(ppk_hydro_t *h, void **stream) {
   if (!h || !stream) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   *stream = (void *)h->stream;

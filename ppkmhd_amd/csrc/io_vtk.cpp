@@ -38,6 +38,9 @@ extern "C" int ppk_mhd_write_vtk(const ppk_mhd_params *p, const double *U, const
     return PPK_ERR_INVALID_ARGUMENT;
   }
   const int gw = p->ghost_width, nx = p->nx, ny = p->ny, nz = p->dim == 3 ? p->nz : 1;
+  // nbvar of the solver the parameters belong to: MHD (ghost width 3) 8, hydro (ghost width 2)
+  // 4 in 2D and 5 in 3D (HydroParams::setup, src/shared/HydroParams.cpp:42-71)
+  const int nbvar = gw == 2 ? (p->dim == 3 ? 5 : 4) : 8;
   const long long isize = nx + 2 * gw, jsize = ny + 2 * gw, ksize = p->dim == 3 ? nz + 2 * gw : 1;
   const long long vstride = isize * jsize * ksize;
 
@@ -81,7 +84,7 @@ extern "C" int ppk_mhd_write_vtk(const ppk_mhd_params *p, const double *U, const
 
   if (ascii) {
     out << "    <CellData>\n";
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < nbvar; ++v) {
       out << "    <DataArray type=\"Float64\" Name=\"" << kVarNames[v] << "\" format=\"ascii\" >\n";
       interior(v, [&](double x) { out << x << " "; });
       out << "\n    </DataArray>\n";
@@ -93,7 +96,7 @@ extern "C" int ppk_mhd_write_vtk(const ppk_mhd_params *p, const double *U, const
   } else {
     const uint64_t words = (uint64_t)nx * ny * nz * sizeof(double);
     out << "    <CellData>" << std::endl;
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < nbvar; ++v) {
       out << "     <DataArray type=\"Float64\" Name=\"" << kVarNames[v]
           << "\" format=\"appended\" offset=\"" << v * words + v * sizeof(uint64_t) << "\" />"
           << std::endl;
@@ -105,7 +108,7 @@ extern "C" int ppk_mhd_write_vtk(const ppk_mhd_params *p, const double *U, const
     out << "_";
     std::vector<double> buf;
     buf.reserve((size_t)nx * ny * nz);
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < nbvar; ++v) {
       out.write((const char *)&words, sizeof(uint64_t));
       buf.clear();
       interior(v, [&](double x) { buf.push_back(x); });

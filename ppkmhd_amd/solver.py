@@ -290,6 +290,14 @@ class SolverHydroMuscl:
     def synchronize(self):
         self._check(lib().ppk_hydro_synchronize(self._h))
 
+    def save_vtk(self, directory=".", prefix="output", istep=None, ascii=False):
+        """Write the current state as <directory>/<prefix>_<istep>.vti in the reference's VTK
+        format (rho, energy, rho_vx, rho_vy[, rho_vz]; istep defaults to the iteration)."""
+        if istep is None:
+            istep = self.m_iteration
+        self._check(lib().ppk_hydro_save_vtk(self._h, str(directory).encode(), prefix.encode(),
+                                             int(istep), 1 if ascii else 0))
+
     @staticmethod
     def timer_names():
         L = lib()
@@ -339,7 +347,7 @@ def write_vtk(params, U, directory=".", prefix="output", istep=0, ascii=False):
     """Write a host state array (reference layout) in the reference's VTK format (no GPU)."""
     U = np.ascontiguousarray(U, dtype=np.float64)
     if U.size != params.count:
-        raise ValueError("state must hold 8*isize*jsize*ksize doubles")
+        raise ValueError("state must hold nbvar*isize*jsize*ksize doubles")
     check(lib().ppk_mhd_write_vtk(C.byref(params), U.ctypes.data_as(C.POINTER(C.c_double)),
                                   str(directory).encode(), prefix.encode(), int(istep),
                                   1 if ascii else 0))

@@ -235,3 +235,15 @@ def test_bench_size_walls_conserve_mass_and_energy(gpu, dim):
     nv = U.shape[0]
     kin = 0.5 * sum(U[v] ** 2 for v in range(2, nv)) / U[0]
     assert (U[1] - kin).min() > 0
+
+
+def test_save_vtk_matches_host_writer(gpu, tmp_path):
+    """ppk_hydro_save_vtk writes the same bytes as the host writer on the downloaded state."""
+    from ppkmhd_amd.solver import write_vtk
+
+    p = hparams(3, (12, 10, 8), ymax=1.0, zmax=1.0)
+    with SolverHydroMuscl(p, problem(p, A.PROBLEM_BLAST)) as g:
+        g.run(3)
+        g.save_vtk(tmp_path, "dev", 3)
+        write_vtk(p, g.download(), tmp_path, "host", 3)
+    assert (tmp_path / "dev_0000003.vti").read_bytes() == (tmp_path / "host_0000003.vti").read_bytes()

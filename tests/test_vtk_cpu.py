@@ -90,3 +90,30 @@ def test_vtk_ascii(tmp_path):
         # default stream precision: 6 significant digits
         np.testing.assert_allclose(vals, interior[v].reshape(-1), rtol=1e-5, atol=0)
     assert text.endswith("    </CellData>\n  </Piece>\n  </ImageData>\n</VTKFile>\n")
+
+
+def test_vtk_hydro_nbvar(tmp_path):
+    """Hydro parameters (ghost width 2) write nbvar = 4 (2D) / 5 (3D) arrays, as the reference's
+    writer loops over the solver's nbvar variables (src/utils/io/IO_VTK.cpp:123, 321)."""
+    from ppkmhd_amd.params import make_hydro_params
+
+    for dim, n in ((2, (6, 5, 1)), (3, (4, 3, 2))):
+        p = make_hydro_params(dim, *n)
+        U = _state(p, 3)
+        assert U.shape[0] == (5 if dim == 3 else 4)
+        write_vtk(p, U, tmp_path, f"h{dim}", 1)
+        head, body = _split_binary((tmp_path / f"h{dim}_0000001.vti").read_bytes())
+        names = [ln.split('Name="')[1].split('"')[0] for ln in head.splitlines() if "Name=" in ln]
+        assert names == NAMES[:U.shape[0]]
+        g = 2
+        ks = slice(g, g + p.nz) if dim == 3 else slice(0, 1)
+        inner = U[:, ks, g:g + p.ny, g:g + p.nx]
+        ncell = inner[0].size
+        off = 0
+        for v in range(U.shape[0]):
+            (nbytes,) = struct.unpack("<Q", body[off:off + 8])
+            assert nbytes == 8 * ncell
+            vals = np.frombuffer(body[off + 8:off + 8 + nbytes], dtype="<f8")
+            np.testing.assert_array_equal(vals, inner[v].ravel())
+            off += 8 + nbytes
+        assert off == len(body)
