@@ -65,7 +65,9 @@ enum ppk_dt_mode {
 /* initial conditions (src/muscl/SolverMHDMuscl.h:649-713 problem names) */
 enum ppk_problem_kind { PPK_PROBLEM_ORSZAG_TANG = 0, PPK_PROBLEM_BLAST = 1,
                         PPK_PROBLEM_KELVIN_HELMHOLTZ = 2, PPK_PROBLEM_WAVE = 3,
-                        PPK_PROBLEM_IMPLODE = 4 };
+                        PPK_PROBLEM_IMPLODE = 4,
+                        /* hydro solvers only (src/muscl/SolverHydroMuscl.h:435-520) */
+                        PPK_PROBLEM_GRESHO_VORTEX = 5, PPK_PROBLEM_ISENTROPIC_VORTEX = 6 };
 
 /*
  * Solver parameters.  Mirrors the fields of HydroParams / HydroSettings that the MHD MUSCL
@@ -121,6 +123,11 @@ typedef struct ppk_mhd_problem {
   /* implode inner / outer state: rho, p, u, v, w, Bx, By, Bz
    * (src/shared/problems/ImplodeParams.h:40-58) */
   double implode_in[8], implode_out[8];
+  /* Gresho vortex rho0, Ma (src/shared/problems/GreshoParams.h) */
+  double gresho_rho0, gresho_ma;
+  /* isentropic vortex: ambient density, pressure, temperature, velocity, centre, strength
+   * (src/shared/problems/IsentropicVortexParams.h) */
+  double iv_rho_a, iv_p_a, iv_t_a, iv_u_a, iv_v_a, iv_w_a, iv_x, iv_y, iv_z, iv_beta;
 } ppk_mhd_problem;
 
 typedef struct ppk_mhd_solver ppk_mhd_t;

@@ -8,6 +8,7 @@
 // src/muscl/HydroInitFunctors{2D,3D}.h (implode, blast).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -438,7 +439,18 @@ struct HydroSolver {
   }
 };
 
-// ---- initial conditions (HydroInitFunctors2D.h:35-230, HydroInitFunctors3D.h) ---------------
+// ---- initial conditions (HydroInitFunctors2D.h:35-660, HydroInitFunctors3D.h:66-530) --------
+// counter generator of the random Kelvin-Helmholtz layer (splitmix64 of seed, padded-box cell,
+// draw), the library's documented stand-in for the reference's thread-mapped XorShift pool
+double kh_rand(uint64_t seed, uint64_t cell, uint64_t draw) {
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull + cell * 0xD1B54A32D192ED03ull + draw * 0x8CB92BA72F3D8DD7ull;
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x = x ^ (x >> 31);
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
 void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
   const Grid g(p);
   std::fill(U, U + g.nv * g.vs, 0.0);
@@ -448,7 +460,82 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
         const double x = p.xmin + p.dx / 2 + (i - g.gw) * p.dx;
         const double y = p.ymin + p.dy / 2 + (j - g.gw) * p.dy;
         const double z = g.dim == 3 ? p.zmin + p.dz / 2 + (k - g.gw) * p.dz : 0.0;
+        const double PI = 3.14159265358979323846;  // M_PI
         double d, e, u, v, w;
+        if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || pr.kind == PPK_PROBLEM_GRESHO_VORTEX ||
+            pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX) {
+          // these store momenta and total energy directly
+          double st[5] = {0, 0, 0, 0, 0};
+          if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ) {  // InitKelvinHelmholtzFunctor2D/3D
+            const double pres = pr.kh_pressure, gm1 = p.gamma0 - 1.0;
+            if (pr.kh_p_rand) {
+              const double sn = g.dim == 2 ? (y - p.ymin) / (p.ymax - p.ymin)
+                                           : (z - p.zmin) / (p.zmax - p.zmin);
+              double dd, uu, vv = 0.0, ww = 0.0;
+              if (sn < 0.25 || sn > 0.75) { dd = pr.kh_d_out; uu = pr.kh_vflow_out; }
+              else { dd = pr.kh_d_in; uu = pr.kh_vflow_in; }
+              const uint64_t cell = ((uint64_t)k * g.js + j) * g.is + i;
+              uu += pr.kh_amplitude * (kh_rand(pr.kh_seed, cell, 0) - 0.5);
+              vv += pr.kh_amplitude * (kh_rand(pr.kh_seed, cell, 1) - 0.5);
+              if (g.dim == 3) ww += pr.kh_amplitude * (kh_rand(pr.kh_seed, cell, 2) - 0.5);
+              st[ID] = dd;
+              st[IU] = dd * uu;
+              st[IV] = dd * vv;
+              st[IW] = dd * ww;
+              st[IP] = g.dim == 2 ? pres / gm1 + 0.5 * dd * (uu * uu + vv * vv)
+                                  : pres / gm1 + 0.5 * dd * (uu * uu + vv * vv + ww * ww);
+            } else if (pr.kh_p_sine_rob) {
+              const double c = g.dim == 2 ? y : z;
+              const double ramp = 1.0 / (1.0 + std::exp(2 * (c - 0.25) / pr.kh_delta)) +
+                                  1.0 / (1.0 + std::exp(2 * (0.75 - c) / pr.kh_delta));
+              const double dd = pr.kh_d_in + ramp * (pr.kh_d_out - pr.kh_d_in);
+              const double uu = pr.kh_vflow_in + ramp * (pr.kh_vflow_out - pr.kh_vflow_in);
+              const int n = pr.kh_mode;
+              if (g.dim == 2) {
+                const double vv = pr.kh_w0 * std::sin(n * PI * x);
+                st[ID] = dd; st[IU] = dd * uu; st[IV] = dd * vv;
+                st[IP] = pres / gm1 + 0.5 * dd * (uu * uu + vv * vv);
+              } else {
+                const double vv = pr.kh_vflow_in / 2 + ramp * (pr.kh_vflow_out / 2 - pr.kh_vflow_in / 2);
+                const double ww = pr.kh_w0 * std::sin(n * PI * x) * std::sin(n * PI * y);
+                st[ID] = dd; st[IU] = dd * uu; st[IV] = dd * vv; st[IW] = dd * ww;
+                st[IP] = pres / gm1 + 0.5 * dd * (uu * uu + vv * vv + ww * ww);
+              }
+            }
+          } else if (pr.kind == PPK_PROBLEM_GRESHO_VORTEX) {  // InitGreshoVortexFunctor2D/3D
+            const double rho0 = pr.gresho_rho0;
+            const double p0 = rho0 / (p.gamma0 * pr.gresho_ma * pr.gresho_ma);
+            const double r = std::sqrt(x * x + y * y), th = std::atan2(y, x);
+            double uphi, pp;
+            if (r < 0.2) { uphi = 5 * r; pp = p0 + 25 / 2.0 * r * r; }
+            else if (r < 0.4) {
+              uphi = 2 - 5 * r;
+              pp = p0 + 25 / 2.0 * r * r + 4 * (1 - 5 * r - std::log(0.2) + std::log(r));
+            } else { uphi = 0; pp = p0 - 2 + 4 * std::log(2.0); }
+            st[ID] = rho0;
+            st[IU] = rho0 * (-std::sin(th) * uphi);
+            st[IV] = rho0 * (std::cos(th) * uphi);
+            if (g.dim == 3) st[IW] = rho0 * (std::cos(th) * uphi);  // the reference's 3D choice
+            double m2 = st[IU] * st[IU] + st[IV] * st[IV];
+            if (g.dim == 3) m2 = m2 + st[IW] * st[IW];
+            st[IP] = pp / (p.gamma0 - 1.0) + 0.5 * m2 / st[ID];
+          } else {  // InitIsentropicVortexFunctor2D
+            const double xp = x - pr.iv_x, yp = y - pr.iv_y, r2 = xp * xp + yp * yp;
+            const double r = std::sqrt(r2);
+            const double du = -yp * pr.iv_beta / (2 * PI) * std::exp(0.5 * (1.0 - r * r));
+            const double dv = xp * pr.iv_beta / (2 * PI) * std::exp(0.5 * (1.0 - r * r));
+            const double T = pr.iv_t_a - (p.gamma0 - 1) * pr.iv_beta * pr.iv_beta /
+                                             (8 * p.gamma0 * PI * PI) * std::exp(1.0 - r * r);
+            const double rho = pr.iv_rho_a * std::pow(T / pr.iv_t_a, 1.0 / (p.gamma0 - 1));
+            st[ID] = rho;
+            st[IU] = rho * (pr.iv_u_a + du);
+            st[IV] = rho * (pr.iv_v_a + dv);
+            st[IP] = rho * T / (p.gamma0 - 1.0) + 0.5 * rho * (pr.iv_u_a + du) * (pr.iv_u_a + du) +
+                     0.5 * rho * (pr.iv_v_a + dv) * (pr.iv_v_a + dv);
+          }
+          for (int n = 0; n < g.nv; ++n) U[g.at(i, j, k, n)] = st[n];
+          continue;
+        }
         if (pr.kind == PPK_PROBLEM_BLAST) {
           double d2 = (x - pr.blast_center_x) * (x - pr.blast_center_x) +
                       (y - pr.blast_center_y) * (y - pr.blast_center_y);

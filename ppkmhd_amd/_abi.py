@@ -15,7 +15,8 @@ BC_UNDEFINED, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
 FP64, FP32_BULK, FP64_STRICT = 0, 1, 2
 PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
 DT_FIXED, DT_CFL = 0, 1
-PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE = range(5)
+(PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE,
+ PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX) = range(7)
 
 PPK_OK = 0
 PPK_ERR_INVALID_ARGUMENT, PPK_ERR_HIP, PPK_ERR_OUT_OF_MEMORY, PPK_ERR_UNSUPPORTED, PPK_ERR_CONFIG = 1, 2, 3, 4, 5
@@ -142,4 +143,16 @@ class Problem(C.Structure):
         ("wave_lz", C.c_double),
         ("implode_in", C.c_double * 8),
         ("implode_out", C.c_double * 8),
+        ("gresho_rho0", C.c_double),
+        ("gresho_ma", C.c_double),
+        ("iv_rho_a", C.c_double),
+        ("iv_p_a", C.c_double),
+        ("iv_t_a", C.c_double),
+        ("iv_u_a", C.c_double),
+        ("iv_v_a", C.c_double),
+        ("iv_w_a", C.c_double),
+        ("iv_x", C.c_double),
+        ("iv_y", C.c_double),
+        ("iv_z", C.c_double),
+        ("iv_beta", C.c_double),
     ]

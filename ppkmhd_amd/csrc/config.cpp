@@ -179,11 +179,18 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     else if (prob == "implode") R.kind = PPK_PROBLEM_IMPLODE;
     else if (prob == "kelvin_helmholtz") R.kind = PPK_PROBLEM_KELVIN_HELMHOLTZ;
     else if (prob == "wave") R.kind = PPK_PROBLEM_WAVE;
-    // hydro problems of the reference this library does not provide: kind -1, which
-    // ppk_hydro_init_problem rejects (SolverHydroMuscl.cpp:140-168)
+    // hydro problems (SolverHydroMuscl.cpp:106-256): Gresho vortex (2D, 3D), isentropic
+    // vortex (2D; the 3D solver falls back to implode); those this library does not provide
+    // (four quadrant, and the gravity problems) get kind -1, which ppk_hydro_init_problem
+    // rejects
+    else if (P.ghost_width == 2 && prob == "gresho_vortex") R.kind = PPK_PROBLEM_GRESHO_VORTEX;
+    else if (P.ghost_width == 2 && prob == "isentropic_vortex")
+      R.kind = P.dim == 2 ? PPK_PROBLEM_ISENTROPIC_VORTEX : PPK_PROBLEM_IMPLODE;
+    else if (P.ghost_width == 2 && P.dim == 3 && prob == "four_quadrant")
+      R.kind = PPK_PROBLEM_IMPLODE;
     else if (P.ghost_width == 2 &&
-             (prob == "gresho_vortex" || prob == "four_quadrant" || prob == "isentropic_vortex" ||
-              prob == "rayleigh_taylor" || prob == "rising_bubble" || prob == "disk"))
+             (prob == "four_quadrant" || prob == "rayleigh_taylor" || prob == "rising_bubble" ||
+              prob == "disk"))
       R.kind = -1;
     // unknown names: the MHD solver falls back to Orszag-Tang (SolverMHDMuscl.h:701-709),
     // the hydro solver to implode (SolverHydroMuscl.cpp:170-177)
@@ -247,6 +254,21 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
       R.implode_in[v] = m.get_float("implode", inames[v], idef[v]);
     }
     R.implode_shape = (int)m.get_integer("implode", "shape_region", 0);
+    // GreshoParams (src/shared/problems/GreshoParams.h)
+    R.gresho_rho0 = m.get_float("Gresho", "rho0", 1.0f);
+    R.gresho_ma = m.get_float("Gresho", "Ma", 0.1f);
+    // IsentropicVortexParams (src/shared/problems/IsentropicVortexParams.h): centre defaults
+    // from the mesh extents, computed in double and passed as float defaults
+    R.iv_rho_a = m.get_float("isentropic_vortex", "density_ambient", 1.0f);
+    R.iv_p_a = m.get_float("isentropic_vortex", "pressure_ambient", 1.0f);
+    R.iv_t_a = m.get_float("isentropic_vortex", "temperature_ambient", 1.0f);
+    R.iv_u_a = m.get_float("isentropic_vortex", "vx_ambient", 1.0f);
+    R.iv_v_a = m.get_float("isentropic_vortex", "vy_ambient", 1.0f);
+    R.iv_w_a = m.get_float("isentropic_vortex", "vz_ambient", 1.0f);
+    R.iv_x = m.get_float("isentropic_vortex", "center_x", (float)((xmin + xmax) / 2));
+    R.iv_y = m.get_float("isentropic_vortex", "center_y", (float)((ymin + ymax) / 2));
+    R.iv_z = m.get_float("isentropic_vortex", "center_z", (float)((zmin + zmax) / 2));
+    R.iv_beta = m.get_float("isentropic_vortex", "strength", 5.0f);
     *pr = R;
   }
   if (P.dim == 0) {

@@ -13,6 +13,7 @@
 #include "capi_internal.hpp"
 #include "event_timer.hpp"
 #include "hydro_kernels.hpp"
+#include "ics_body.hpp"
 #include "ppkmhd_amd.h"
 
 namespace ppk {
@@ -125,7 +126,10 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
 
 // initial conditions of the hydro problems in the reference layout (host)
 // InitImplodeFunctor2D/3D (src/muscl/HydroInitFunctors2D.h:35-125, HydroInitFunctors3D.h:66-165),
-// InitBlastFunctor2D/3D (HydroInitFunctors2D.h:129-212, HydroInitFunctors3D.h:170-262)
+// InitBlastFunctor2D/3D (HydroInitFunctors2D.h:129-212, HydroInitFunctors3D.h:170-262),
+// InitKelvinHelmholtzFunctor2D/3D (HydroInitFunctors2D.h:216-348, 3D.h:271-427),
+// InitGreshoVortexFunctor2D/3D (HydroInitFunctors2D.h:352-449, 3D.h:429-530),
+// InitIsentropicVortexFunctor2D (HydroInitFunctors2D.h:570-660)
 void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
   const int gw = p.ghost_width, dim = p.dim, nv = dim == 3 ? 5 : 4;
   const int is = p.nx + 2 * gw, js = p.ny + 2 * gw, ks = dim == 3 ? p.nz + 2 * gw : 1;
@@ -139,8 +143,95 @@ void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
         const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx;
         const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy;
         const double z = dim == 3 ? p.zmin + p.dz / 2 + (k - gw) * p.dz : 0.0;
-        double s[5];
-        if (pr.kind == PPK_PROBLEM_BLAST) {
+        double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ) {
+          // the random layer uses the library's counter generator (splitmix64 of seed, cell,
+          // draw), as the MHD Kelvin-Helmholtz problem does; the reference's XorShift pool
+          // depends on the thread mapping
+          const double d_in = pr.kh_d_in, d_out = pr.kh_d_out, vin = pr.kh_vflow_in;
+          const double vout = pr.kh_vflow_out, ampl = pr.kh_amplitude, pres = pr.kh_pressure;
+          const uint64_t cell = (uint64_t)c;
+          if (pr.kh_p_rand) {
+            const double sn = dim == 2 ? (y - p.ymin) / (p.ymax - p.ymin)
+                                       : (z - p.zmin) / (p.zmax - p.zmin);
+            const bool outer = sn < 0.25 || sn > 0.75;
+            const double d = outer ? d_out : d_in;
+            double u = outer ? vout : vin, v = 0.0, w = 0.0;
+            u += ampl * (icb::kh_uniform(pr.kh_seed, cell, 0) - 0.5);
+            v += ampl * (icb::kh_uniform(pr.kh_seed, cell, 1) - 0.5);
+            if (dim == 3) w += ampl * (icb::kh_uniform(pr.kh_seed, cell, 2) - 0.5);
+            s[ID] = d;
+            s[IU] = d * u;
+            s[IV] = d * v;
+            s[IW] = d * w;
+            s[IP] = dim == 2 ? pres / (g0 - 1.0) + 0.5 * d * (u * u + v * v)
+                             : pres / (g0 - 1.0) + 0.5 * d * (u * u + v * v + w * w);
+          } else if (pr.kh_p_sine_rob) {
+            const int n = pr.kh_mode;
+            const double w0 = pr.kh_w0, delta = pr.kh_delta, s1 = 0.25, s2 = 0.75;
+            const double sc = dim == 2 ? y : z;
+            const double ramp = 1.0 / (1.0 + std::exp(2 * (sc - s1) / delta)) +
+                                1.0 / (1.0 + std::exp(2 * (s2 - sc) / delta));
+            const double d = d_in + ramp * (d_out - d_in);
+            const double u = vin + ramp * (vout - vin);
+            if (dim == 2) {
+              const double v = w0 * std::sin(n * icb::kPi * x);
+              s[ID] = d;
+              s[IU] = d * u;
+              s[IV] = d * v;
+              s[IP] = pres / (g0 - 1.0) + 0.5 * d * (u * u + v * v);
+            } else {
+              const double v1y = vin / 2, v2y = vout / 2;
+              const double v = v1y + ramp * (v2y - v1y);
+              const double w = w0 * std::sin(n * icb::kPi * x) * std::sin(n * icb::kPi * y);
+              s[ID] = d;
+              s[IU] = d * u;
+              s[IV] = d * v;
+              s[IW] = d * w;
+              s[IP] = pres / (g0 - 1.0) + 0.5 * d * (u * u + v * v + w * w);
+            }
+          }  // neither perturbation: the reference leaves the cell at zero
+        } else if (pr.kind == PPK_PROBLEM_GRESHO_VORTEX) {
+          const double rho0 = pr.gresho_rho0, Ma = pr.gresho_ma;
+          const double p0 = rho0 / (g0 * Ma * Ma);
+          const double r = std::sqrt(x * x + y * y);
+          const double theta = std::atan2(y, x);
+          const double cosT = std::cos(theta), sinT = std::sin(theta);
+          double uphi, pp;
+          if (r < 0.2) {
+            uphi = 5 * r;
+            pp = p0 + 25 / 2.0 * r * r;
+          } else if (r < 0.4) {
+            uphi = 2 - 5 * r;
+            pp = p0 + 25 / 2.0 * r * r + 4 * (1 - 5 * r - std::log(0.2) + std::log(r));
+          } else {
+            uphi = 0;
+            pp = p0 - 2 + 4 * std::log(2.0);
+          }
+          s[ID] = rho0;
+          s[IU] = rho0 * (-sinT * uphi);
+          s[IV] = rho0 * (cosT * uphi);
+          if (dim == 2) {
+            s[IP] = pp / (g0 - 1.0) + 0.5 * (s[IU] * s[IU] + s[IV] * s[IV]) / s[ID];
+          } else {
+            s[IW] = rho0 * (cosT * uphi);  // as the reference's 3D functor writes it
+            s[IP] = pp / (g0 - 1.0) + 0.5 * (s[IU] * s[IU] + s[IV] * s[IV] + s[IW] * s[IW]) / s[ID];
+          }
+        } else if (pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX) {  // 2D
+          const double xp = x - pr.iv_x, yp = y - pr.iv_y;
+          const double r = std::sqrt(xp * xp + yp * yp);
+          const double beta = pr.iv_beta, T_a = pr.iv_t_a;
+          const double du = -yp * beta / (2 * icb::kPi) * std::exp(0.5 * (1.0 - r * r));
+          const double dv = xp * beta / (2 * icb::kPi) * std::exp(0.5 * (1.0 - r * r));
+          const double T = T_a - (g0 - 1) * beta * beta / (8 * g0 * icb::kPi * icb::kPi) *
+                                     std::exp(1.0 - r * r);
+          const double rho = pr.iv_rho_a * std::pow(T / T_a, 1.0 / (g0 - 1));
+          s[ID] = rho;
+          s[IU] = rho * (pr.iv_u_a + du);
+          s[IV] = rho * (pr.iv_v_a + dv);
+          s[IP] = rho * T / (g0 - 1.0) + 0.5 * rho * (pr.iv_u_a + du) * (pr.iv_u_a + du) +
+                  0.5 * rho * (pr.iv_v_a + dv) * (pr.iv_v_a + dv);
+        } else if (pr.kind == PPK_PROBLEM_BLAST) {
           double d2 = (x - pr.blast_center_x) * (x - pr.blast_center_x) +
                       (y - pr.blast_center_y) * (y - pr.blast_center_y);
           if (dim == 3) d2 = d2 + (z - pr.blast_center_z) * (z - pr.blast_center_z);
@@ -296,7 +387,13 @@ int hfinish_init(ppk_hydro_t *h) {
   return hwrite_ctrl(h);
 }
 
-bool hydro_problem_ok(int kind) { return kind == PPK_PROBLEM_IMPLODE || kind == PPK_PROBLEM_BLAST; }
+bool hydro_problem_ok(int kind, int dim) {
+  return kind == PPK_PROBLEM_IMPLODE || kind == PPK_PROBLEM_BLAST ||
+         kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || kind == PPK_PROBLEM_GRESHO_VORTEX ||
+         (kind == PPK_PROBLEM_ISENTROPIC_VORTEX && dim == 2);
+}
+const char *const kHydroProblems =
+    "hydro problems: implode, blast, kelvin_helmholtz, gresho_vortex, isentropic_vortex (2D)";
 
 }  // namespace
 
@@ -374,8 +471,8 @@ int ppk_hydro_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem 
   int rc = ppk::validate_hydro(params, why);
   if (rc != PPK_OK) return hfail(nullptr, rc, why);
   if (!problem || !U) return hfail(nullptr, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!hydro_problem_ok(problem->kind))
-    return hfail(nullptr, PPK_ERR_UNSUPPORTED, "hydro problems: implode or blast");
+  if (!hydro_problem_ok(problem->kind, params->dim))
+    return hfail(nullptr, PPK_ERR_UNSUPPORTED, kHydroProblems);
   const int gw = params->ghost_width;
   const size_t cells = (size_t)(params->nx + 2 * gw) * (params->ny + 2 * gw) *
                        (params->dim == 3 ? params->nz + 2 * gw : 1);
@@ -387,8 +484,8 @@ int ppk_hydro_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem 
 
 int ppk_hydro_init_problem(ppk_hydro_t *h, const ppk_mhd_problem *problem) {
   if (!h || !problem) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!hydro_problem_ok(problem->kind))
-    return hfail(h, PPK_ERR_UNSUPPORTED, "hydro problems: implode or blast");
+  if (!hydro_problem_ok(problem->kind, h->p.dim))
+    return hfail(h, PPK_ERR_UNSUPPORTED, kHydroProblems);
   std::vector<double> host(h->values());
   ppk::hydro_ics(h->p, *problem, host.data());
   h->iter_host = 0;

@@ -121,4 +121,14 @@ def default_problem(kind, params=None, float_round=True):
         pr.implode_out[v] = r(x)
     for v, x in enumerate((0.125, 0.14, 0, 0, 0, 0, 0, 0)):
         pr.implode_in[v] = r(x)
+    # GreshoParams, IsentropicVortexParams (centre defaults computed in double, passed as
+    # float defaults)
+    pr.gresho_rho0 = r(1.0)
+    pr.gresho_ma = r(0.1)
+    pr.iv_rho_a = pr.iv_p_a = pr.iv_t_a = r(1.0)
+    pr.iv_u_a = pr.iv_v_a = pr.iv_w_a = r(1.0)
+    pr.iv_x = r((xmin + xmax) / 2)
+    pr.iv_y = r((ymin + ymax) / 2)
+    pr.iv_z = r((zmin + zmax) / 2)
+    pr.iv_beta = r(5.0)
     return pr
