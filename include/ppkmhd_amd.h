@@ -67,7 +67,8 @@ enum ppk_problem_kind { PPK_PROBLEM_ORSZAG_TANG = 0, PPK_PROBLEM_BLAST = 1,
                         PPK_PROBLEM_KELVIN_HELMHOLTZ = 2, PPK_PROBLEM_WAVE = 3,
                         PPK_PROBLEM_IMPLODE = 4,
                         /* hydro solvers only (src/muscl/SolverHydroMuscl.h:435-520) */
-                        PPK_PROBLEM_GRESHO_VORTEX = 5, PPK_PROBLEM_ISENTROPIC_VORTEX = 6 };
+                        PPK_PROBLEM_GRESHO_VORTEX = 5, PPK_PROBLEM_ISENTROPIC_VORTEX = 6,
+                        PPK_PROBLEM_FOUR_QUADRANT = 7 /* 2D */ };
 
 /*
  * Solver parameters.  Mirrors the fields of HydroParams / HydroSettings that the MHD MUSCL
@@ -128,6 +129,10 @@ typedef struct ppk_mhd_problem {
   /* isentropic vortex: ambient density, pressure, temperature, velocity, centre, strength
    * (src/shared/problems/IsentropicVortexParams.h) */
   double iv_rho_a, iv_p_a, iv_t_a, iv_u_a, iv_v_a, iv_w_a, iv_x, iv_y, iv_z, iv_beta;
+  /* four-quadrant 2D Riemann problem: [riemann2d] config_number (0-based Lax-Liu
+   * configuration), quadrant corner x, y (src/muscl/SolverHydroMuscl.cpp:72-89) */
+  int fq_config, fq_reserved;
+  double fq_x, fq_y;
 } ppk_mhd_problem;
 
 typedef struct ppk_mhd_solver ppk_mhd_t;

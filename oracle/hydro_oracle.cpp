@@ -5,7 +5,8 @@
 // Reference: src/muscl/SolverHydroMuscl.cpp:282-405 (godunov_unsplit_impl),
 // src/muscl/HydroRunFunctors{2D,3D}.h, src/muscl/HydroBaseFunctor{2D,3D}.h,
 // src/shared/RiemannSolvers.h, src/shared/BoundariesFunctors.h:18-420,
-// src/muscl/HydroInitFunctors{2D,3D}.h (implode, blast).
+// src/muscl/HydroInitFunctors{2D,3D}.h (implode, blast, Kelvin-Helmholtz, Gresho vortex,
+// isentropic vortex, four quadrant).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -440,6 +441,53 @@ struct HydroSolver {
 };
 
 // ---- initial conditions (HydroInitFunctors2D.h:35-660, HydroInitFunctors3D.h:66-530) --------
+// four-quadrant states, (density, x velocity, y velocity, pressure) of the quadrants U0..U3
+// of src/shared/problems/initRiemannConfig2d.h:42-514 (U0 upper right, then counter-clockwise)
+struct Quad { double d, u, v, p; };
+const Quad kFourQuadrant[19][4] = {
+    {{1.0, 0.0, 0.0, 1.0}, {0.5197, -0.7259, 0.0, 0.4},
+     {0.1072, -0.7259, -1.4045, 0.0439}, {0.2579, 0.0, -1.4045, 0.15}},  // configuration 1
+    {{1.0, 0.0, 0.0, 1.0}, {0.5197, -0.7259, 0.0, 0.4},
+     {1.0, -0.7259, -0.7259, 1.0}, {0.5197, 0.0, -0.7259, 0.4}},  // configuration 2
+    {{1.5, 0.0, 0.0, 1.5}, {0.5323, 1.206, 0.0, 0.3},
+     {0.138, 1.206, 1.206, 0.029}, {0.5323, 0.0, 1.206, 0.3}},  // configuration 3
+    {{1.1, 0.0, 0.0, 1.1}, {0.5065, 0.8939, 0.0, 0.35},
+     {1.1, 0.8939, 0.8939, 1.1}, {0.5065, 0.0, 0.8939, 0.35}},  // configuration 4
+    {{1.0, -0.75, -0.5, 1.0}, {2.0, -0.75, 0.5, 1.0},
+     {1.0, 0.75, 0.5, 1.0}, {3.0, 0.75, -0.5, 1.0}},  // configuration 5
+    {{1.0, 0.75, -0.5, 1.0}, {2.0, 0.75, 0.5, 0.5},
+     {1.0, -0.75, 0.5, 1.0}, {3.0, -0.75, -0.5, 1.0}},  // configuration 6
+    {{1.0, 0.1, 0.1, 1.0}, {0.5197, -0.6259, 0.1, 0.4},
+     {0.8, 0.1, 0.1, 0.4}, {0.5197, 0.1, -0.6259, 0.4}},  // configuration 7
+    {{0.5197, 0.1, 0.1, 0.4}, {1.0, -0.6259, 0.1, 1.0},
+     {0.8, 0.1, 0.1, 1.0}, {1.0, 0.1, -0.6259, 1.0}},  // configuration 8
+    {{1.0, 0.0, 0.3, 1.0}, {2.0, 0.0, -0.3, 1.0},
+     {1.039, 0.0, -0.8133, 0.4}, {0.5197, 0.0, -0.4259, 0.4}},  // configuration 9
+    {{1.0, 0.0, 0.4297, 1.0}, {0.5, 0.0, 0.6076, 1.0},
+     {0.2281, 0.0, -0.6076, 0.3333}, {0.4562, 0.0, -0.4259, 0.3333}},  // configuration 10
+    {{1.0, 0.1, 0.0, 1.0}, {0.5313, 0.8276, 0.0, 0.4},
+     {0.8, 0.1, 0.0, 0.4}, {0.5313, 0.1, 0.7276, 0.4}},  // configuration 11
+    {{0.5313, 0.0, 0.0, 0.4}, {1.0, 0.7276, 0.0, 1.0},
+     {0.8, 0.0, 0.0, 1.0}, {1.0, 0.0, 0.7276, 1.0}},  // configuration 12
+    {{1.0, 0.0, -0.3, 1.0}, {2.0, 0.0, 0.3, 1.0},
+     {1.0625, 0.0, 0.8145, 0.4}, {0.5313, 0.0, 0.4276, 0.4}},  // configuration 13
+    {{2.0, 0.0, -0.5606, 8.0}, {1.0, 0.0, -1.2172, 8.0},
+     {0.4736, 0.0, 1.2172, 2.6667}, {0.9474, 0.0, 1.1606, 2.6667}},  // configuration 14
+    {{1.0, 0.1, -0.3, 1.0}, {0.5197, -0.6259, -0.3, 0.4},
+     {0.8, 0.1, -0.3, 0.4}, {0.5313, 0.1, 0.4276, 0.4}},  // configuration 15
+    {{0.5313, 0.1, 0.1, 0.4}, {1.0222, -0.6179, 0.1, 1.0},
+     {0.8, 0.1, 0.1, 1.0}, {1.0, 0.1, 0.8276, 1.0}},  // configuration 16
+    {{1.0, 0.0, -0.4, 1.0}, {2.0, 0.0, -0.3, 1.0},
+     {1.0625, 0.0, 0.2145, 0.4}, {0.5197, 0.0, -1.1259, 0.4}},  // configuration 17
+    {{1.0, 0.0, 1.0, 1.0}, {2.0, 0.0, -0.3, 1.0},
+     {1.0625, 0.0, 0.2145, 0.4}, {0.5197, 0.0, 0.2741, 0.4}},  // configuration 18
+    {{1.0, 0.0, 0.3, 1.0}, {2.0, 0.0, -0.3, 1.0},
+     {1.0625, 0.0, 0.2145, 0.4}, {0.5197, 0.0, -0.4259, 0.4}},  // configuration 19
+};
+// getRiemannConfig2d's switch falls from its last case (18) into the default (configuration
+// 1): only numbers 0..17 select their own row
+const Quad *four_quadrant_config(int n) { return kFourQuadrant[(n >= 0 && n < 18) ? n : 0]; }
+
 // counter generator of the random Kelvin-Helmholtz layer (splitmix64 of seed, padded-box cell,
 // draw), the library's documented stand-in for the reference's thread-mapped XorShift pool
 double kh_rand(uint64_t seed, uint64_t cell, uint64_t draw) {
@@ -463,7 +511,7 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
         const double PI = 3.14159265358979323846;  // M_PI
         double d, e, u, v, w;
         if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || pr.kind == PPK_PROBLEM_GRESHO_VORTEX ||
-            pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX) {
+            pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX || pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {
           // these store momenta and total energy directly
           double st[5] = {0, 0, 0, 0, 0};
           if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ) {  // InitKelvinHelmholtzFunctor2D/3D
@@ -519,6 +567,17 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
             double m2 = st[IU] * st[IU] + st[IV] * st[IV];
             if (g.dim == 3) m2 = m2 + st[IW] * st[IW];
             st[IP] = pp / (p.gamma0 - 1.0) + 0.5 * m2 / st[ID];
+          } else if (pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {
+            // InitFourQuadrantFunctor2D (HydroInitFunctors2D.h:496-559) of states converted by
+            // primToCons_2D (initRiemannConfig2d.h:25-38)
+            const Quad *cf = four_quadrant_config(pr.fq_config);
+            Quad q;
+            if (x < pr.fq_x) q = y < pr.fq_y ? cf[2] : cf[1];
+            else q = y < pr.fq_y ? cf[3] : cf[0];
+            st[ID] = q.d;
+            st[IU] = q.u * q.d;
+            st[IV] = q.v * q.d;
+            st[IP] = q.p / (p.gamma0 - 1.0) + q.d * (q.u * q.u + q.v * q.v) * 0.5;
           } else {  // InitIsentropicVortexFunctor2D
             const double xp = x - pr.iv_x, yp = y - pr.iv_y, r2 = xp * xp + yp * yp;
             const double r = std::sqrt(r2);

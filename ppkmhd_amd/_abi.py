@@ -16,7 +16,7 @@ FP64, FP32_BULK, FP64_STRICT = 0, 1, 2
 PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
 DT_FIXED, DT_CFL = 0, 1
 (PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE,
- PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX) = range(7)
+ PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX, PROBLEM_FOUR_QUADRANT) = range(8)
 
 PPK_OK = 0
 PPK_ERR_INVALID_ARGUMENT, PPK_ERR_HIP, PPK_ERR_OUT_OF_MEMORY, PPK_ERR_UNSUPPORTED, PPK_ERR_CONFIG = 1, 2, 3, 4, 5
@@ -155,4 +155,8 @@ class Problem(C.Structure):
         ("iv_y", C.c_double),
         ("iv_z", C.c_double),
         ("iv_beta", C.c_double),
+        ("fq_config", C.c_int),
+        ("fq_reserved", C.c_int),
+        ("fq_x", C.c_double),
+        ("fq_y", C.c_double),
     ]

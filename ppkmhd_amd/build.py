@@ -59,7 +59,7 @@ UNITS = [
     ("io_vtk.cpp", "io_vtk.o", []),
     ("batch_capi.cpp", "batch_capi.o", ["--offload-host-only"]),
     ("mlp_capi.cpp", "mlp_capi.o", ["--offload-host-only"]),
-    ("hydro_capi.cpp", "hydro_capi.o", ["--offload-host-only"]),
+    ("hydro_capi.cpp", "hydro_capi.o", ["--offload-host-only", "-ffp-contract=off"]),
 ]
 
 

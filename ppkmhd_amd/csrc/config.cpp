@@ -180,17 +180,16 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     else if (prob == "kelvin_helmholtz") R.kind = PPK_PROBLEM_KELVIN_HELMHOLTZ;
     else if (prob == "wave") R.kind = PPK_PROBLEM_WAVE;
     // hydro problems (SolverHydroMuscl.cpp:106-256): Gresho vortex (2D, 3D), isentropic
-    // vortex (2D; the 3D solver falls back to implode); those this library does not provide
-    // (four quadrant, and the gravity problems) get kind -1, which ppk_hydro_init_problem
-    // rejects
+    // vortex and four quadrant (2D; the 3D solver falls back to implode for both); the gravity
+    // problems, which this library does not provide, get kind -1, which
+    // ppk_hydro_init_problem rejects
     else if (P.ghost_width == 2 && prob == "gresho_vortex") R.kind = PPK_PROBLEM_GRESHO_VORTEX;
     else if (P.ghost_width == 2 && prob == "isentropic_vortex")
       R.kind = P.dim == 2 ? PPK_PROBLEM_ISENTROPIC_VORTEX : PPK_PROBLEM_IMPLODE;
-    else if (P.ghost_width == 2 && P.dim == 3 && prob == "four_quadrant")
-      R.kind = PPK_PROBLEM_IMPLODE;
+    else if (P.ghost_width == 2 && prob == "four_quadrant")
+      R.kind = P.dim == 2 ? PPK_PROBLEM_FOUR_QUADRANT : PPK_PROBLEM_IMPLODE;
     else if (P.ghost_width == 2 &&
-             (prob == "four_quadrant" || prob == "rayleigh_taylor" || prob == "rising_bubble" ||
-              prob == "disk"))
+             (prob == "rayleigh_taylor" || prob == "rising_bubble" || prob == "disk"))
       R.kind = -1;
     // unknown names: the MHD solver falls back to Orszag-Tang (SolverMHDMuscl.h:701-709),
     // the hydro solver to implode (SolverHydroMuscl.cpp:170-177)
@@ -269,6 +268,10 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     R.iv_y = m.get_float("isentropic_vortex", "center_y", (float)((ymin + ymax) / 2));
     R.iv_z = m.get_float("isentropic_vortex", "center_z", (float)((zmin + zmax) / 2));
     R.iv_beta = m.get_float("isentropic_vortex", "strength", 5.0f);
+    // four quadrant (SolverHydroMuscl.cpp:75-77)
+    R.fq_config = (int)m.get_integer("riemann2d", "config_number", 0);
+    R.fq_x = m.get_float("riemann2d", "x", 0.8f);
+    R.fq_y = m.get_float("riemann2d", "y", 0.8f);
     *pr = R;
   }
   if (P.dim == 0) {

@@ -124,12 +124,66 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   return d;
 }
 
+// The 19 four-quadrant 2D Riemann configurations of Lax & Liu (SIAM J. Sci. Comput. 19(2),
+// 1998), primitive (rho, u, v, p) per quadrant, quadrants numbered counter-clockwise from the
+// upper right (x >= xc, y >= yc) as the reference's U0..U3
+// (src/shared/problems/initRiemannConfig2d.h:42-514).  Values as the reference holds them,
+// including configuration 6's quadrant 1 at p = 0.5.
+constexpr double kLaxLiu[19][4][4] = {
+    {{1.0, 0.0, 0.0, 1.0}, {0.5197, -0.7259, 0.0, 0.4}, {0.1072, -0.7259, -1.4045, 0.0439},
+     {0.2579, 0.0, -1.4045, 0.15}},
+    {{1.0, 0.0, 0.0, 1.0}, {0.5197, -0.7259, 0.0, 0.4}, {1.0, -0.7259, -0.7259, 1.0},
+     {0.5197, 0.0, -0.7259, 0.4}},
+    {{1.5, 0.0, 0.0, 1.5}, {0.5323, 1.206, 0.0, 0.3}, {0.138, 1.206, 1.206, 0.029},
+     {0.5323, 0.0, 1.206, 0.3}},
+    {{1.1, 0.0, 0.0, 1.1}, {0.5065, 0.8939, 0.0, 0.35}, {1.1, 0.8939, 0.8939, 1.1},
+     {0.5065, 0.0, 0.8939, 0.35}},
+    {{1.0, -0.75, -0.5, 1.0}, {2.0, -0.75, 0.5, 1.0}, {1.0, 0.75, 0.5, 1.0},
+     {3.0, 0.75, -0.5, 1.0}},
+    {{1.0, 0.75, -0.5, 1.0}, {2.0, 0.75, 0.5, 0.5}, {1.0, -0.75, 0.5, 1.0},
+     {3.0, -0.75, -0.5, 1.0}},
+    {{1.0, 0.1, 0.1, 1.0}, {0.5197, -0.6259, 0.1, 0.4}, {0.8, 0.1, 0.1, 0.4},
+     {0.5197, 0.1, -0.6259, 0.4}},
+    {{0.5197, 0.1, 0.1, 0.4}, {1.0, -0.6259, 0.1, 1.0}, {0.8, 0.1, 0.1, 1.0},
+     {1.0, 0.1, -0.6259, 1.0}},
+    {{1.0, 0.0, 0.3, 1.0}, {2.0, 0.0, -0.3, 1.0}, {1.039, 0.0, -0.8133, 0.4},
+     {0.5197, 0.0, -0.4259, 0.4}},
+    {{1.0, 0.0, 0.4297, 1.0}, {0.5, 0.0, 0.6076, 1.0}, {0.2281, 0.0, -0.6076, 0.3333},
+     {0.4562, 0.0, -0.4259, 0.3333}},
+    {{1.0, 0.1, 0.0, 1.0}, {0.5313, 0.8276, 0.0, 0.4}, {0.8, 0.1, 0.0, 0.4},
+     {0.5313, 0.1, 0.7276, 0.4}},
+    {{0.5313, 0.0, 0.0, 0.4}, {1.0, 0.7276, 0.0, 1.0}, {0.8, 0.0, 0.0, 1.0},
+     {1.0, 0.0, 0.7276, 1.0}},
+    {{1.0, 0.0, -0.3, 1.0}, {2.0, 0.0, 0.3, 1.0}, {1.0625, 0.0, 0.8145, 0.4},
+     {0.5313, 0.0, 0.4276, 0.4}},
+    {{2.0, 0.0, -0.5606, 8.0}, {1.0, 0.0, -1.2172, 8.0}, {0.4736, 0.0, 1.2172, 2.6667},
+     {0.9474, 0.0, 1.1606, 2.6667}},
+    {{1.0, 0.1, -0.3, 1.0}, {0.5197, -0.6259, -0.3, 0.4}, {0.8, 0.1, -0.3, 0.4},
+     {0.5313, 0.1, 0.4276, 0.4}},
+    {{0.5313, 0.1, 0.1, 0.4}, {1.0222, -0.6179, 0.1, 1.0}, {0.8, 0.1, 0.1, 1.0},
+     {1.0, 0.1, 0.8276, 1.0}},
+    {{1.0, 0.0, -0.4, 1.0}, {2.0, 0.0, -0.3, 1.0}, {1.0625, 0.0, 0.2145, 0.4},
+     {0.5197, 0.0, -1.1259, 0.4}},
+    {{1.0, 0.0, 1.0, 1.0}, {2.0, 0.0, -0.3, 1.0}, {1.0625, 0.0, 0.2145, 0.4},
+     {0.5197, 0.0, 0.2741, 0.4}},
+    {{1.0, 0.0, 0.3, 1.0}, {2.0, 0.0, -0.3, 1.0}, {1.0625, 0.0, 0.2145, 0.4},
+     {0.5197, 0.0, -0.4259, 0.4}},
+};
+
+// [riemann2d] config_number -> row of kLaxLiu.  The reference's selection (a switch on the
+// number) has no break after its last case, so number 18 falls into the default, which is
+// configuration 1 like every number outside 0..17; configuration 19 is never selected.
+int lax_liu_row(int config_number) {
+  return config_number >= 0 && config_number <= 17 ? config_number : 0;
+}
+
 // initial conditions of the hydro problems in the reference layout (host)
 // InitImplodeFunctor2D/3D (src/muscl/HydroInitFunctors2D.h:35-125, HydroInitFunctors3D.h:66-165),
 // InitBlastFunctor2D/3D (HydroInitFunctors2D.h:129-212, HydroInitFunctors3D.h:170-262),
 // InitKelvinHelmholtzFunctor2D/3D (HydroInitFunctors2D.h:216-348, 3D.h:271-427),
 // InitGreshoVortexFunctor2D/3D (HydroInitFunctors2D.h:352-449, 3D.h:429-530),
-// InitIsentropicVortexFunctor2D (HydroInitFunctors2D.h:570-660)
+// InitIsentropicVortexFunctor2D (HydroInitFunctors2D.h:570-660),
+// InitFourQuadrantFunctor2D (HydroInitFunctors2D.h:453-565, SolverHydroMuscl.cpp:72-89)
 void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
   const int gw = p.ghost_width, dim = p.dim, nv = dim == 3 ? 5 : 4;
   const int is = p.nx + 2 * gw, js = p.ny + 2 * gw, ks = dim == 3 ? p.nz + 2 * gw : 1;
@@ -231,6 +285,13 @@ void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
           s[IV] = rho * (pr.iv_v_a + dv);
           s[IP] = rho * T / (g0 - 1.0) + 0.5 * rho * (pr.iv_u_a + du) * (pr.iv_u_a + du) +
                   0.5 * rho * (pr.iv_v_a + dv) * (pr.iv_v_a + dv);
+        } else if (pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {  // 2D
+          const int quad = x < pr.fq_x ? (y < pr.fq_y ? 2 : 1) : (y < pr.fq_y ? 3 : 0);
+          const double *q = kLaxLiu[lax_liu_row(pr.fq_config)][quad];  // rho, u, v, p
+          s[ID] = q[0];
+          s[IU] = q[1] * q[0];
+          s[IV] = q[2] * q[0];
+          s[IP] = q[3] / (g0 - 1.0) + q[0] * (q[1] * q[1] + q[2] * q[2]) * 0.5;
         } else if (pr.kind == PPK_PROBLEM_BLAST) {
           double d2 = (x - pr.blast_center_x) * (x - pr.blast_center_x) +
                       (y - pr.blast_center_y) * (y - pr.blast_center_y);
@@ -390,10 +451,12 @@ int hfinish_init(ppk_hydro_t *h) {
 bool hydro_problem_ok(int kind, int dim) {
   return kind == PPK_PROBLEM_IMPLODE || kind == PPK_PROBLEM_BLAST ||
          kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || kind == PPK_PROBLEM_GRESHO_VORTEX ||
-         (kind == PPK_PROBLEM_ISENTROPIC_VORTEX && dim == 2);
+         ((kind == PPK_PROBLEM_ISENTROPIC_VORTEX || kind == PPK_PROBLEM_FOUR_QUADRANT) &&
+          dim == 2);
 }
 const char *const kHydroProblems =
-    "hydro problems: implode, blast, kelvin_helmholtz, gresho_vortex, isentropic_vortex (2D)";
+    "hydro problems: implode, blast, kelvin_helmholtz, gresho_vortex, isentropic_vortex (2D), "
+    "four_quadrant (2D)";
 
 }  // namespace
 

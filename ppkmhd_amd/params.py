@@ -131,4 +131,7 @@ def default_problem(kind, params=None, float_round=True):
     pr.iv_y = r((ymin + ymax) / 2)
     pr.iv_z = r((zmin + zmax) / 2)
     pr.iv_beta = r(5.0)
+    pr.fq_config = 0
+    pr.fq_x = r(0.8)
+    pr.fq_y = r(0.8)
     return pr

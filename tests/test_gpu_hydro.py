@@ -275,6 +275,22 @@ def test_strict_bitwise_more_problems(gpu, H, strip, case):
     compare_steps(H, p, pr, 3, tag=case, oracle_state=not case.endswith("rand"))
 
 
+@pytest.mark.parametrize("config", [2, 5, 13])
+@pytest.mark.parametrize("path", [A.PATH_FUSED, A.PATH_STAGED])
+def test_strict_bitwise_four_quadrant(gpu, H, strip, config, path):
+    """Four-quadrant 2D Riemann problems on settings/test_four_quadrant_2D.ini's settings
+    (gamma 1.666, HLLC, slope type 2, absorbing walls; corner at (0.8, 0.8)) on a grid that
+    leaves partial strips: configurations 3, 6 (the one with the reference's p = 0.5
+    quadrant) and 14; strict, bit for bit over 4 steps."""
+    strip(8)
+    p = make_hydro_params(2, 90, 70, gamma0=1.666, cfl=0.8, slope_type=2.0,
+                          riemann=A.RIEMANN_HLLC, bc=A.BC_NEUMANN, tEnd=10.0,
+                          precision=A.FP64_STRICT, kernel_path=path)
+    pr = default_problem(A.PROBLEM_FOUR_QUADRANT, p)
+    pr.fq_config = config
+    compare_steps(H, p, pr, 4, tag=f"four_quadrant config_number={config}")
+
+
 def test_isentropic_vortex_second_order_at_scale(gpu):
     """The isentropic vortex moves with the ambient flow; on the GPU at 128^2 .. 512^2 (periodic
     [0,10]^2, t = 1) the density error against the translated initial field falls at second

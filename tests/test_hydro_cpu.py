@@ -281,19 +281,21 @@ def test_sod_shock_tube_converges(H, rs):
 # ---- host side of the library (no GPU) ---------------------------------------------------------
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("kind", [A.PROBLEM_IMPLODE, A.PROBLEM_BLAST, A.PROBLEM_KELVIN_HELMHOLTZ,
-                                  A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX])
+                                  A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX,
+                                  A.PROBLEM_FOUR_QUADRANT])
 def test_initial_state_bit_identical_to_oracle(H, dim, kind):
     from ppkmhd_amd.solver import hydro_initial_state
 
-    if kind == A.PROBLEM_ISENTROPIC_VORTEX and dim == 3:
-        pytest.skip("the reference has no 3D isentropic vortex (falls back to implode)")
-    for variant in (0, 1):
+    if kind in (A.PROBLEM_ISENTROPIC_VORTEX, A.PROBLEM_FOUR_QUADRANT) and dim == 3:
+        pytest.skip("2D problem in the reference (its 3D solver falls back to implode)")
+    for variant in ((-1, 0, 5, 13, 17, 18, 19) if kind == A.PROBLEM_FOUR_QUADRANT else (0, 1)):
         p = make_hydro_params(dim, 30, 22, 14, xmin=-0.5, xmax=2.0, ymin=-0.4, ymax=1.5,
                               gamma0=1.666)
         pr = default_problem(kind, p)
         pr.implode_shape = variant
         pr.implode_in[2] = 0.3  # a moving inner state exercises the energy's kinetic term
         pr.kh_p_rand = variant      # random layer, then the Robertson sine perturbation
+        pr.fq_config = variant
         a, b = hydro_initial_state(p, pr), H.init_array(p, pr)
         transcendental = kind in (A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX) or (
             kind == A.PROBLEM_KELVIN_HELMHOLTZ and not pr.kh_p_rand)
@@ -384,6 +386,96 @@ def test_kelvin_helmholtz_layers(H, dim):
     assert pert.max() <= 0.5 * pr.kh_amplitude + 1e-12
 
 
+def _quadrant_states(U, gamma, p):
+    """Primitive (rho, u, v, p) at the centres of the four quadrants of a four-quadrant state
+    whose corner is the box centre, U0 (upper right) first, counter-clockwise."""
+    g = p.ghost_width
+    lo_i, hi_i = g + p.nx // 4, g + 3 * p.nx // 4
+    lo_j, hi_j = g + p.ny // 4, g + 3 * p.ny // 4
+    out = []
+    for i, j in ((hi_i, hi_j), (lo_i, hi_j), (lo_i, lo_j), (hi_i, lo_j)):
+        d, e, mu, mv = U[:, 0, j, i]
+        u, v = mu / d, mv / d
+        out.append(np.array([d, u, v, (e - 0.5 * d * (u * u + v * v)) * (gamma - 1)]))
+    return out
+
+
+def _lax_liu_interfaces(Q, gamma=1.4):
+    """Classify the four interfaces of a Lax-Liu configuration: 'J' for a slip line (pressure
+    and normal velocity continuous), else the smaller of the residuals of a simple rarefaction
+    (isentropic, |du_n| = 2|dc| / (gamma - 1)) and of a shock (Rankine-Hugoniot mass/momentum
+    and Hugoniot energy relations), or inf when the tangential velocity jumps (no single 1D
+    wave connects the states)."""
+    res = []
+    for a, b, n in ((0, 1, 1), (2, 3, 1), (1, 2, 2), (3, 0, 2)):
+        t = 2 if n == 1 else 1
+        (da, pa), (db, pb) = (Q[a][0], Q[a][3]), (Q[b][0], Q[b][3])
+        ua, ub = Q[a][n], Q[b][n]
+        if abs(pa - pb) < 1e-12 and abs(ua - ub) < 1e-12:
+            res.append("J")
+            continue
+        if Q[a][t] != Q[b][t]:
+            res.append(np.inf)
+            continue
+        sa, sb = pa / da ** gamma, pb / db ** gamma
+        ca, cb = np.sqrt(gamma * pa / da), np.sqrt(gamma * pb / db)
+        rar = max(abs(sa - sb) / sa, abs(abs(ub - ua) - 2 / (gamma - 1) * abs(ca - cb)) / abs(ub - ua))
+        ea, eb = pa / ((gamma - 1) * da), pb / ((gamma - 1) * db)
+        shock = max(abs((ub - ua) ** 2 - (pb - pa) * (1 / da - 1 / db)) / (ub - ua) ** 2,
+                    abs(eb - ea + 0.5 * (pa + pb) * (1 / db - 1 / da)) / abs(eb - ea))
+        res.append(min(rar, shock))
+    return res
+
+
+def test_four_quadrant_states_are_lax_liu_configurations(H):
+    """Four-quadrant problem: each of the 18 selectable configurations puts, across each of its
+    four interfaces, states joined by one elementary 1D wave of gamma = 1.4 gas (a slip line, a
+    rarefaction or a shock), to the 4-digit rounding of the published table (residual <
+    5e-3).  This pins the quadrant order, the (u, v) order and the signs of the table through
+    the initial state.  Configuration 6 is the exception the reference holds: quadrant U1 at
+    p = 0.5 where its two interfaces are slip lines of p = 1 (restoring p = 1 makes all four
+    slip lines)."""
+    p = make_hydro_params(2, 16, 16, 1, gamma0=1.4)
+    for cfg in range(18):
+        pr = default_problem(A.PROBLEM_FOUR_QUADRANT, p)
+        pr.fq_config, pr.fq_x, pr.fq_y = cfg, 0.5, 0.5
+        Q = _quadrant_states(H.init_array(p, pr), p.gamma0, p)
+        res = _lax_liu_interfaces(Q)
+        if cfg == 5:
+            assert res[0] == np.inf and res[2] == np.inf and res[1] == res[3] == "J"
+            assert abs(Q[1][3] - 0.5) < 1e-14
+            Q[1][3] = 1.0
+            assert _lax_liu_interfaces(Q) == ["J"] * 4
+            continue
+        for r in res:
+            assert r == "J" or r < 5e-3, (cfg + 1, res)
+
+
+def test_four_quadrant_selection_and_corner(H):
+    """config_number 18 (the reference's switch falls from it into the default), negative and
+    out-of-range numbers all give configuration 1; cells exactly at the corner coordinate
+    belong to the upper / right quadrants (x < xt and y < yt select the left / lower ones)."""
+    p = make_hydro_params(2, 12, 10, 1, gamma0=1.4)
+    pr = default_problem(A.PROBLEM_FOUR_QUADRANT, p)
+    pr.fq_config = 0
+    base = H.init_array(p, pr)
+    for n in (18, 19, -1, 1000):
+        pr.fq_config = n
+        np.testing.assert_array_equal(H.init_array(p, pr), base)
+    pr.fq_config = 17
+    assert not np.array_equal(H.init_array(p, pr), base)
+    # corner on a cell centre: x = (i - 2 + 0.5) / 12 for i = 8 is 0.5416...
+    pr.fq_config = 4   # configuration 5: four distinct densities
+    pr.fq_x = p.xmin + p.dx / 2 + 6 * p.dx
+    pr.fq_y = p.ymin + p.dy / 2 + 5 * p.dy
+    U = H.init_array(p, pr)
+    g = p.ghost_width
+    assert U[0, 0, g + 5, g + 6] == 1.0            # (x, y) == corner: quadrant U0
+    assert U[0, 0, g + 5, g + 5] == 2.0            # left of it: U1
+    assert U[0, 0, g + 4, g + 5] == 1.0 and U[2, 0, g + 4, g + 5] == 0.75  # U2
+    assert U[0, 0, g + 4, g + 6] == 3.0            # U3
+
+
 def test_ini_reference_hydro_settings():
     """The reference's settings/test_implode_2D.ini and test_blast_3D.ini."""
     from ppkmhd_amd.solver import params_from_ini
@@ -429,9 +521,14 @@ implementationVersion=0
     # the reference this library lacks are refused at init
     _, pr4, _ = params_from_ini(text=txt.replace("problem=implode", "problem=nonsense"))
     assert pr4.kind == A.PROBLEM_IMPLODE
-    _, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=four_quadrant"))
+    _, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=rayleigh_taylor"))
     assert pr5.kind == -1
-    for name, kind2, kind3 in (("gresho_vortex", A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_GRESHO_VORTEX),
+    # settings/test_four_quadrant_2D.ini's problem section
+    _, pr7, _ = params_from_ini(text=txt.replace("problem=implode", "problem=four_quadrant")
+                                .replace("[other]", "[riemann2d]\nconfig_number=2\nx=0.8\ny=0.8\n[other]"))
+    assert pr7.kind == A.PROBLEM_FOUR_QUADRANT and pr7.fq_config == 2
+    assert pr7.fq_x == f32(0.8) and pr7.fq_y == f32(0.8)
+    for name, kind2, kind3 in (("four_quadrant", A.PROBLEM_FOUR_QUADRANT, A.PROBLEM_IMPLODE),("gresho_vortex", A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_GRESHO_VORTEX),
                                ("isentropic_vortex", A.PROBLEM_ISENTROPIC_VORTEX,
                                 A.PROBLEM_IMPLODE),
                                ("kelvin_helmholtz", A.PROBLEM_KELVIN_HELMHOLTZ,
