@@ -117,10 +117,9 @@ k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux,
                 int kk0, int kk1) {
   if (!active(ctrl)) return;
   const int B = xcd_remap(blockIdx.x, gridDim.x);
-  const long long c = sub_brick_cell(p, bk, rbase + B / 6, threadIdx.x);
-  if (c < 0) return;
-  const int kk = (int)(c / p.plane) - p.gw;
-  if (kk < kk0 || kk >= kk1) return;
+  int kk;
+  const long long c = sub_brick_cell(p, bk, rbase + B / 6, threadIdx.x, kk);
+  if (c < 0 || kk < kk0 || kk >= kk1) return;
   switch (B % 6) {
     case 0: face3<0, RS>(rec, flux, p, c); break;
     case 1: face3<1, RS>(rec, flux, p, c); break;

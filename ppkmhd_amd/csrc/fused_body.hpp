@@ -620,22 +620,35 @@ PPK_HD void edge2(const R *__restrict__ rec, R *__restrict__ emf, const DevParam
 }
 
 // cell (i,j,k) of thread tid of brick L; false past the covered range
-// row r of an enumeration of (j, k) rows in patches of pj x pk rows, npj patches along j
-PPK_HD void patch_row(long long r, int pj, int pk, int npj, int &j, int &k) {
-  const int lj = (int)(r % pj);
-  r /= pj;
-  const int lk = (int)(r % pk);
-  r /= pk;
-  j = (int)(r % npj) * pj + lj;
-  k = (int)(r / npj) * pk + lk;
+// row r of an enumeration of (j, k) rows in patches of pj x pk rows, npj patches along j.
+// Cell and row numbers are below 2^32 (grids are limited to 2^31 padded cells), so the
+// decoding is 32-bit unsigned arithmetic: a 64-bit division by a run-time value costs
+// several times more instructions, and every thread of the per-cell kernels decodes its cell.
+PPK_HD void patch_row(unsigned r, int pj, int pk, int npj, int &j, int &k) {
+  unsigned lj, lk;
+  if (pj == kPatch && pk == kPatch) {  // the 3D patch: shifts
+    static_assert(kPatch == 4, "patch decoding by shifts assumes 4 x 4 row patches");
+    lj = r & 3u;
+    lk = (r >> 2) & 3u;
+    r >>= 4;
+  } else {
+    lj = r % (unsigned)pj;
+    r /= (unsigned)pj;
+    lk = r % (unsigned)pk;
+    r /= (unsigned)pk;
+  }
+  const unsigned pr = r / (unsigned)npj;
+  j = (int)(r - pr * (unsigned)npj) * pj + (int)lj;
+  k = (int)pr * pk + (int)lk;
 }
 
 PPK_HD bool brick_cell(const Brick &b, int L, int tid, int &i, int &j, int &k) {
   if (b.bx == 0) {  // flattened runs of 256 cells, rows in patches
-    const long long n = (long long)L * 256 + tid;
-    i = (int)(n % b.mx);
-    patch_row(n / b.mx, b.by, b.bz, b.ny, j, k);
-    return j < b.my && k < b.mz && n < (long long)b.mx * b.ny * b.by * b.nz * b.bz;
+    const unsigned n = (unsigned)L * 256u + (unsigned)tid;
+    const unsigned row = n / (unsigned)b.mx;
+    i = (int)(n - row * (unsigned)b.mx);
+    patch_row(row, b.by, b.bz, b.ny, j, k);
+    return j < b.my && k < b.mz && row < (unsigned)(b.ny * b.by) * (unsigned)(b.nz * b.bz);
   }
   const int ux = L % b.nx, r = L / b.nx, uy = r % b.ny, uz = r / b.ny;
   i = ux * b.bx + tid % b.bx;
@@ -644,11 +657,16 @@ PPK_HD bool brick_cell(const Brick &b, int L, int tid, int &i, int &j, int &k) {
   return i < b.mx && j < b.my && k < b.mz;
 }
 
-// padded-box index of thread tid of sub-box brick L, or -1
-PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int tid) {
+// padded-box index of thread tid of sub-box brick L, or -1; kk: its z index in the sub-box
+PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int tid, int &kk) {
   int i, j, k;
   if (!brick_cell(b, L, tid, i, j, k)) return -1;
+  kk = p.dim == 3 ? k : 0;
   return cidx(p, p.gw + i, p.gw + j, p.dim == 3 ? p.gw + k : 0);
+}
+PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int tid) {
+  int kk;
+  return sub_brick_cell(p, b, L, tid, kk);
 }
 
 // ===========================================================================================

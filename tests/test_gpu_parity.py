@@ -82,6 +82,19 @@ def test_strict_is_bit_identical(oracle, gpu, case, path):
     assert (tg, dtg, itg) == (to, dto, ito)
 
 
+def test_strict_at_size_many_tiles(oracle, gpu):
+    """A 70 x 60 x 50 blast (MC slopes, an oblique field) whose 1280 half-step tiles outnumber
+    the blocks resident on the GPU, and whose sub-box bricks are partial in every direction:
+    strict bit-identical to the oracle over 3 CFL steps."""
+    p = make_params(3, 70, 60, 50, gamma0=5 / 3, cfl=0.8, tEnd=1.0, slope_type=2.0,
+                    precision=A.FP64_STRICT, kernel_path=A.PATH_FUSED)
+    pr = default_problem(A.PROBLEM_BLAST, p)
+    pr.blast_radius = 0.3
+    pr.blast_bx, pr.blast_by, pr.blast_bz = 0.6, -0.3, 0.2
+    o, g = run_both(oracle, p, pr, 3)
+    assert np.array_equal(g.download()[p.interior()], o.get_state()[p.interior()])
+
+
 @pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("variant", ["hll", "llf", "mc", "dirichlet_neumann", "kh_rand"])
 def test_strict_variants(oracle, gpu, variant, path):
