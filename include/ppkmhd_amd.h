@@ -68,7 +68,9 @@ enum ppk_problem_kind { PPK_PROBLEM_ORSZAG_TANG = 0, PPK_PROBLEM_BLAST = 1,
                         PPK_PROBLEM_IMPLODE = 4,
                         /* hydro solvers only (src/muscl/SolverHydroMuscl.h:435-520) */
                         PPK_PROBLEM_GRESHO_VORTEX = 5, PPK_PROBLEM_ISENTROPIC_VORTEX = 6,
-                        PPK_PROBLEM_FOUR_QUADRANT = 7 /* 2D */ };
+                        PPK_PROBLEM_FOUR_QUADRANT = 7 /* 2D */,
+                        /* with static gravity ([gravity] static=yes) */
+                        PPK_PROBLEM_RAYLEIGH_TAYLOR = 8, PPK_PROBLEM_RISING_BUBBLE = 9 };
 
 /*
  * Solver parameters.  Mirrors the fields of HydroParams / HydroSettings that the MHD MUSCL
@@ -90,6 +92,12 @@ typedef struct ppk_mhd_params {
   double dx, dy, dz;
   double gamma0, gamma6, cfl, slope_type, smallr, smallc, smallp, smallpp, cIso;
   double tEnd;
+  /* hydro solvers: [gravity] static (src/shared/SolverBase.cpp:138-141): 1 adds the gravity
+   * predictor to the face states, the gravity source term after the update and the
+   * gravity-corrected CFL dt; the field is the uniform one the problem sets (0 for problems
+   * that set none).  The MHD solvers ignore it, as the reference's do. */
+  int32_t gravity_static;
+  int32_t reserved2;     /* must be 0                                                  */
 } ppk_mhd_params;
 
 /*
@@ -133,6 +141,12 @@ typedef struct ppk_mhd_problem {
    * configuration), quadrant corner x, y (src/muscl/SolverHydroMuscl.cpp:72-89) */
   int fq_config, fq_reserved;
   double fq_x, fq_y;
+  /* Rayleigh-Taylor: perturbation amplitude, densities below / above, uniform gravity
+   * (src/shared/problems/RayleighTaylorInstabilityParams.h) */
+  double rti_amplitude, rti_d0, rti_d1, rti_gx, rti_gy, rti_gz;
+  /* rising bubble: radius, centre, densities in / out, uniform gravity
+   * (src/shared/problems/RisingBubbleParams.h) */
+  double rb_R, rb_x0, rb_y0, rb_z0, rb_din, rb_dout, rb_gx, rb_gy, rb_gz;
 } ppk_mhd_problem;
 
 typedef struct ppk_mhd_solver ppk_mhd_t;

@@ -287,6 +287,11 @@ struct HydroSolver {
   std::vector<double> U, U2, Q, F[3];
   double t = 0.0, dt = 0.0;
   long long it = 0;
+  // static gravity ([gravity] static or point: SolverBase.cpp:138-141) and its field, uniform
+  // here: the one the problem's initial condition writes, 0 for problems that write none
+  // (the reference's field is zero-initialised)
+  double grav[3] = {0.0, 0.0, 0.0};
+  bool gravity() const { return p.gravity_static != 0; }
   explicit HydroSolver(const ppk_mhd_params &pp) : p(pp), g(pp) {
     U.assign(g.nv * g.vs, 0.0);
     U2 = U;
@@ -298,9 +303,34 @@ struct HydroSolver {
     for (int v = 0; v < g.nv; ++v) s[v] = A[g.at(i, j, k, v)];
     return s;
   }
-  // compute_dt_local (SolverHydroMuscl.h:650-690): interior cells of the current state
+  // compute_dt_local (SolverHydroMuscl.h:650-690): interior cells of the current state; with
+  // gravity ComputeDtGravityFunctor2D/3D (HydroRunFunctors2D.h:120-170, 3D.h:130-185): the
+  // summed speeds scaled by k / (sqrt(1 + 2k) - 1), k = |g|_1 cfl dx / u^2 floored at 1e-4,
+  // over the smallest cell size
   real inv_dt(const std::vector<double> &A) const {
     real inv = 0.0;
+    if (gravity()) {
+      real dxm = FMIN(real(p.dx), real(p.dy));
+      if (g.dim == 3) dxm = FMIN(dxm, real(p.dz));
+      for (int k = g.dim == 3 ? g.gw : 0; k < (g.dim == 3 ? g.ks - g.gw : 1); ++k)
+        for (int j = g.gw; j < g.js - g.gw; ++j)
+          for (int i = g.gw; i < g.is - g.gw; ++i) {
+            S5 q;
+            real c = 0.0;
+            to_prim(p, g.nv, cell(A, i, j, k), c, q);
+            real velocity = 0.0;
+            velocity += c + FABS(q[IU]);
+            velocity += c + FABS(q[IV]);
+            if (g.dim == 3) velocity += c + FABS(q[IW]);
+            real kk = g.dim == 3 ? FABS(real(grav[0])) + FABS(real(grav[1])) + FABS(real(grav[2]))
+                                 : FABS(real(grav[0])) + FABS(real(grav[1]));
+            kk = kk * (p.cfl * dxm / (velocity * velocity));
+            kk = FMAX(kk, real(1e-4));
+            velocity = velocity * (kk / (SQRT(1.0 + 2.0 * kk) - 1.0));
+            inv = FMAX(inv, velocity / dxm);
+          }
+      return inv;
+    }
     for (int k = g.dim == 3 ? g.gw : 0; k < (g.dim == 3 ? g.ks - g.gw : 1); ++k)
       for (int j = g.gw; j < g.js - g.gw; ++j)
         for (int i = g.gw; i < g.is - g.gw; ++i) {
@@ -396,6 +426,12 @@ struct HydroSolver {
             cell_slopes(ni, nj, nk, ndx, ndy, ndz);
             S5 qright = trace(qloc, dx, dy, dz, dir, 0, dtdx, dtdy, dtdz);
             S5 qleft = trace(cell(Q, ni, nj, nk), ndx, ndy, ndz, dir, 1, dtdx, dtdy, dtdz);
+            if (gravity()) {  // gravity predictor, half a time step (HydroRunFunctors2D.h:650-660)
+              for (int d = 0; d < g.dim; ++d) {
+                qleft[IU + d] += 0.5 * dt_ * grav[d];
+                qright[IU + d] += 0.5 * dt_ * grav[d];
+              }
+            }
             const int nrm = IU + dir;  // rotate the normal velocity into IU
             std::swap(qleft[IU], qleft[nrm]);
             std::swap(qright[IU], qright[nrm]);
@@ -422,6 +458,31 @@ struct HydroSolver {
             }
             out[g.at(i, j, k, v)] = D(o);
           }
+    // GravitySourceTermFunctor2D/3D (HydroRunFunctors2D.h:1299-1331, 3D.h:1463-1500): momentum
+    // by dt g times the mean of the old and new densities, energy by the change of kinetic
+    // energy; the 2D functor floors the new density at smallr, the 3D one does not
+    if (gravity()) {
+      for (int k = k0; k < (g.dim == 3 ? k1 : 1); ++k)
+        for (int j = g.gw; j < g.js - g.gw; ++j)
+          for (int i = g.gw; i < g.is - g.gw; ++i) {
+            const real rhoOld = in[g.at(i, j, k, ID)];
+            const real rhoNew = g.dim == 3 ? real(out[g.at(i, j, k, ID)])
+                                           : FMAX(real(p.smallr), real(out[g.at(i, j, k, ID)]));
+            real m[3] = {out[g.at(i, j, k, IU)], out[g.at(i, j, k, IV)],
+                         g.dim == 3 ? out[g.at(i, j, k, IW)] : 0.0};
+            const real ekin_old = g.dim == 3
+                                      ? 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rhoNew
+                                      : 0.5 * (m[0] * m[0] + m[1] * m[1]) / rhoNew;
+            for (int d = 0; d < g.dim; ++d) m[d] += 0.5 * dt_ * grav[d] * (rhoOld + rhoNew);
+            for (int d = 0; d < g.dim; ++d) out[g.at(i, j, k, IU + d)] = D(m[d]);
+            const real ekin_new = g.dim == 3
+                                      ? 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rhoNew
+                                      : 0.5 * (m[0] * m[0] + m[1] * m[1]) / rhoNew;
+            real e = out[g.at(i, j, k, IP)];
+            e += (ekin_new - ekin_old);
+            out[g.at(i, j, k, IP)] = D(e);
+          }
+    }
   }
   // SolverBase::next_iteration (src/shared/SolverBase.cpp:161-219) with the hydro dt
   void step(double dt_fixed, int dt_mode) {
@@ -511,7 +572,8 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
         const double PI = 3.14159265358979323846;  // M_PI
         double d, e, u, v, w;
         if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || pr.kind == PPK_PROBLEM_GRESHO_VORTEX ||
-            pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX || pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {
+            pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX || pr.kind == PPK_PROBLEM_FOUR_QUADRANT ||
+            pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
           // these store momenta and total energy directly
           double st[5] = {0, 0, 0, 0, 0};
           if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ) {  // InitKelvinHelmholtzFunctor2D/3D
@@ -567,6 +629,43 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
             double m2 = st[IU] * st[IU] + st[IV] * st[IV];
             if (g.dim == 3) m2 = m2 + st[IW] * st[IW];
             st[IP] = pp / (p.gamma0 - 1.0) + 0.5 * m2 / st[ID];
+          } else if (pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR) {
+            // RayleighTaylorInstabilityFunctor2D/3D (HydroInitFunctors2D.h:695-780,
+            // 3D.h:578-676): heavy fluid above the mid-plane, a cosine bump in the vertical
+            // velocity (stored in the momentum slot as the reference does), hydrostatic
+            // pressure P0 + rho g.x with P0 = 1
+            const double Lx = p.xmax - p.xmin, Ly = p.ymax - p.ymin, P0 = 1.0;
+            if (g.dim == 2) {
+              st[ID] = y > (p.ymin + p.ymax) / 2 ? pr.rti_d1 : pr.rti_d0;
+              st[IU] = 0.0;
+              st[IV] = pr.rti_amplitude * (1 + std::cos(2 * PI * x / Lx)) *
+                       (1 + std::cos(2 * PI * y / Ly)) / 4;
+              st[IP] = (P0 + st[ID] * (pr.rti_gx * x + pr.rti_gy * y)) / (p.gamma0 - 1.0);
+            } else {
+              st[ID] = z > (p.zmin + p.zmax) / 2 ? pr.rti_d1 : pr.rti_d0;
+              st[IU] = 0.0;
+              st[IV] = 0.0;
+              st[IW] = pr.rti_amplitude * (1 + std::cos(2 * PI * x / Lx)) *
+                       (1 + std::cos(2 * PI * y / Ly)) / 4;
+              st[IP] = (P0 + st[ID] * (pr.rti_gx * x + pr.rti_gy * y + pr.rti_gz * z)) /
+                       (p.gamma0 - 1);
+            }
+          } else if (pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
+            // RisingBubbleFunctor2D/3D (HydroInitFunctors2D.h:840-900, 3D.h:715-800): a
+            // sphere of density din in dout at rest, hydrostatic pressure P0 + rho g.x
+            const double P0 = 1.0;
+            const double r = g.dim == 2
+                                 ? std::sqrt((x - pr.rb_x0) * (x - pr.rb_x0) +
+                                             (y - pr.rb_y0) * (y - pr.rb_y0))
+                                 : std::sqrt((x - pr.rb_x0) * (x - pr.rb_x0) +
+                                             (y - pr.rb_y0) * (y - pr.rb_y0) +
+                                             (z - pr.rb_z0) * (z - pr.rb_z0));
+            st[ID] = r < pr.rb_R ? pr.rb_din : pr.rb_dout;
+            st[IU] = st[IV] = st[IW] = 0.0;
+            st[IP] = g.dim == 2
+                         ? (P0 + st[ID] * (pr.rb_gx * x + pr.rb_gy * y)) / (p.gamma0 - 1.0)
+                         : (P0 + st[ID] * (pr.rb_gx * x + pr.rb_gy * y + pr.rb_gz * z)) /
+                               (p.gamma0 - 1.0);
           } else if (pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {
             // InitFourQuadrantFunctor2D (HydroInitFunctors2D.h:496-559) of states converted by
             // primToCons_2D (initRiemannConfig2d.h:25-38)
@@ -640,6 +739,10 @@ hora_solver *hora_create(const ppk_mhd_params *p) { return new hora_solver(*p); 
 void hora_destroy(hora_solver *s) { delete s; }
 void hora_init_problem(hora_solver *s, const ppk_mhd_problem *pr) {
   init(s->s.p, *pr, s->s.U.data());
+  const bool rti = pr->kind == PPK_PROBLEM_RAYLEIGH_TAYLOR, rb = pr->kind == PPK_PROBLEM_RISING_BUBBLE;
+  s->s.grav[0] = rti ? pr->rti_gx : rb ? pr->rb_gx : 0.0;
+  s->s.grav[1] = rti ? pr->rti_gy : rb ? pr->rb_gy : 0.0;
+  s->s.grav[2] = s->s.p.dim == 3 ? (rti ? pr->rti_gz : rb ? pr->rb_gz : 0.0) : 0.0;
   boundaries(s->s.p, s->s.U.data());
   s->s.U2 = s->s.U;
   s->s.t = 0.0;

@@ -16,7 +16,8 @@ FP64, FP32_BULK, FP64_STRICT = 0, 1, 2
 PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
 DT_FIXED, DT_CFL = 0, 1
 (PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE,
- PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX, PROBLEM_FOUR_QUADRANT) = range(8)
+ PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX, PROBLEM_FOUR_QUADRANT, PROBLEM_RAYLEIGH_TAYLOR,
+ PROBLEM_RISING_BUBBLE) = range(10)
 
 PPK_OK = 0
 PPK_ERR_INVALID_ARGUMENT, PPK_ERR_HIP, PPK_ERR_OUT_OF_MEMORY, PPK_ERR_UNSUPPORTED, PPK_ERR_CONFIG = 1, 2, 3, 4, 5
@@ -56,6 +57,8 @@ class Params(C.Structure):
         ("smallpp", C.c_double),
         ("cIso", C.c_double),
         ("tEnd", C.c_double),
+        ("gravity_static", C.c_int32),
+        ("reserved2", C.c_int32),
     ]
 
     @property
@@ -159,4 +162,19 @@ class Problem(C.Structure):
         ("fq_reserved", C.c_int),
         ("fq_x", C.c_double),
         ("fq_y", C.c_double),
+        ("rti_amplitude", C.c_double),
+        ("rti_d0", C.c_double),
+        ("rti_d1", C.c_double),
+        ("rti_gx", C.c_double),
+        ("rti_gy", C.c_double),
+        ("rti_gz", C.c_double),
+        ("rb_R", C.c_double),
+        ("rb_x0", C.c_double),
+        ("rb_y0", C.c_double),
+        ("rb_z0", C.c_double),
+        ("rb_din", C.c_double),
+        ("rb_dout", C.c_double),
+        ("rb_gx", C.c_double),
+        ("rb_gy", C.c_double),
+        ("rb_gz", C.c_double),
     ]

@@ -169,6 +169,12 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
   std::string path = lower(m.get_string("other", "kernel_path", "auto"));
   P.kernel_path = path == "staged" ? PPK_PATH_STAGED : (path == "fused" ? PPK_PATH_FUSED : PPK_PATH_AUTO);
   P.reserved = 0;
+  // SolverBase: gravity enabled when [gravity] static or point (SolverBase.cpp:138-141); the
+  // point-mass field is the disk problem's, which is not provided, so either key gives the
+  // uniform field of the problem (zero for problems that set none)
+  P.gravity_static =
+      (m.get_bool("gravity", "static", false) || m.get_bool("gravity", "point", false)) ? 1 : 0;
+  P.reserved2 = 0;
   ppk_mhd_params_init(&P);
   if (p) *p = P;
 
@@ -188,9 +194,9 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
       R.kind = P.dim == 2 ? PPK_PROBLEM_ISENTROPIC_VORTEX : PPK_PROBLEM_IMPLODE;
     else if (P.ghost_width == 2 && prob == "four_quadrant")
       R.kind = P.dim == 2 ? PPK_PROBLEM_FOUR_QUADRANT : PPK_PROBLEM_IMPLODE;
-    else if (P.ghost_width == 2 &&
-             (prob == "rayleigh_taylor" || prob == "rising_bubble" || prob == "disk"))
-      R.kind = -1;
+    else if (P.ghost_width == 2 && prob == "rayleigh_taylor") R.kind = PPK_PROBLEM_RAYLEIGH_TAYLOR;
+    else if (P.ghost_width == 2 && prob == "rising_bubble") R.kind = PPK_PROBLEM_RISING_BUBBLE;
+    else if (P.ghost_width == 2 && prob == "disk") R.kind = -1;
     // unknown names: the MHD solver falls back to Orszag-Tang (SolverMHDMuscl.h:701-709),
     // the hydro solver to implode (SolverHydroMuscl.cpp:170-177)
     else R.kind = P.ghost_width == 2 ? PPK_PROBLEM_IMPLODE : PPK_PROBLEM_ORSZAG_TANG;
@@ -268,6 +274,22 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     R.iv_y = m.get_float("isentropic_vortex", "center_y", (float)((ymin + ymax) / 2));
     R.iv_z = m.get_float("isentropic_vortex", "center_z", (float)((zmin + zmax) / 2));
     R.iv_beta = m.get_float("isentropic_vortex", "strength", 5.0f);
+    // RayleighTaylorInstabilityParams, RisingBubbleParams
+    R.rti_amplitude = m.get_float("rayleigh_taylor", "amplitude", 0.01f);
+    R.rti_d0 = m.get_float("rayleigh_taylor", "d0", 1.0f);
+    R.rti_d1 = m.get_float("rayleigh_taylor", "d1", 2.0f);
+    R.rti_gx = m.get_float("rayleigh_taylor", "gx", 0.0f);
+    R.rti_gy = m.get_float("rayleigh_taylor", "gy", -0.1f);
+    R.rti_gz = m.get_float("rayleigh_taylor", "gz", 0.0f);
+    R.rb_R = m.get_float("rising_bubble", "R", 0.1f);
+    R.rb_x0 = m.get_float("rising_bubble", "x0", 0.5f);
+    R.rb_y0 = m.get_float("rising_bubble", "y0", 0.25f);
+    R.rb_z0 = m.get_float("rising_bubble", "z0", 0.5f);
+    R.rb_din = m.get_float("rising_bubble", "din", 1.0f);
+    R.rb_dout = m.get_float("rising_bubble", "dout", 2.0f);
+    R.rb_gx = m.get_float("rising_bubble", "gx", 0.0f);
+    R.rb_gy = m.get_float("rising_bubble", "gy", -0.1f);
+    R.rb_gz = m.get_float("rising_bubble", "gz", 0.0f);
     // four quadrant (SolverHydroMuscl.cpp:75-77)
     R.fq_config = (int)m.get_integer("riemann2d", "config_number", 0);
     R.fq_x = m.get_float("riemann2d", "x", 0.8f);

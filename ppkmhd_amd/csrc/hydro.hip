@@ -51,12 +51,59 @@ __device__ __forceinline__ void swap_normal(HSt<NV> &q, int dir) {
   q[IU + dir] = t;
 }
 
-// CFL term of one cell (ComputeDtFunctor2D/3D, HydroRunFunctors2D.h:46-76, 3D.h:46-81)
+// gravity predictor of a face state, half a time step (HydroRunFunctors2D.h:650-660,
+// 3D.h:485-497): v += (0.5 dt) g, before the normal is rotated into IU
 template <int NV>
-__device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &h) {
-  HSt<NV> q;
-  double c;
-  hyd::prims(u, h.P, c, q);
+__device__ __forceinline__ void gravity_predict(HSt<NV> &q, const HydroDev &h, double hdt) {
+  if (!h.grav) return;
+  q[IU] += hdt * h.g[0];
+  q[IV] += hdt * h.g[1];
+  if constexpr (NV == 5) q[IW] += hdt * h.g[2];
+}
+
+// GravitySourceTermFunctor2D/3D (HydroRunFunctors2D.h:1299-1331, 3D.h:1463-1500) on an updated
+// cell o whose density before the step was rho_old: momentum += (0.5 dt) g (rho_old + rho_new),
+// energy += the change of kinetic energy.  The 2D functor floors the new density at smallr,
+// the 3D one does not.
+template <int NV>
+__device__ __forceinline__ void gravity_source(HSt<NV> &o, double rho_old, const HydroDev &h,
+                                               double hdt) {
+  if (!h.grav) return;
+  const double rho_new = NV == 4 ? fmax(h.P.smallr, o[ID]) : o[ID];
+  const double ir = recip(rho_new);
+  double mu = o[IU], mv = o[IV], mw = NV == 5 ? o[IW] : 0.0;
+  const double ek_old = NV == 5 ? divr(0.5 * (mu * mu + mv * mv + mw * mw), rho_new, ir)
+                                : divr(0.5 * (mu * mu + mv * mv), rho_new, ir);
+  mu += hdt * h.g[0] * (rho_old + rho_new);
+  mv += hdt * h.g[1] * (rho_old + rho_new);
+  if constexpr (NV == 5) mw += hdt * h.g[2] * (rho_old + rho_new);
+  o[IU] = mu;
+  o[IV] = mv;
+  if constexpr (NV == 5) o[IW] = mw;
+  const double ek_new = NV == 5 ? divr(0.5 * (mu * mu + mv * mv + mw * mw), rho_new, ir)
+                                : divr(0.5 * (mu * mu + mv * mv), rho_new, ir);
+  o[IP] += (ek_new - ek_old);
+}
+
+// CFL term of one cell (ComputeDtFunctor2D/3D, HydroRunFunctors2D.h:46-76, 3D.h:46-81); with
+// gravity ComputeDtGravityFunctor2D/3D (HydroRunFunctors2D.h:120-170, 3D.h:130-185): the
+// summed speeds u scaled by k / (sqrt(1 + 2k) - 1), k = |g|_1 cfl dx / u^2 floored at 1e-4,
+// which makes g dt^2 / (2 dx) + u dt / dx = cfl, over the smallest cell size
+template <int NV>
+__device__ __forceinline__ double inv_dt_prim(const HSt<NV> &q, double c, const HydroDev &h) {
+  if (h.grav) {
+    double vel = 0.0;
+    vel += c + fabs(q[IU]);
+    vel += c + fabs(q[IV]);
+    if constexpr (NV == 5) vel += c + fabs(q[IW]);
+    double k = NV == 5 ? fabs(h.g[0]) + fabs(h.g[1]) + fabs(h.g[2]) : fabs(h.g[0]) + fabs(h.g[1]);
+    const double v2 = vel * vel;
+    k = k * divr(h.cfl * h.dxmin, v2, recip(v2));
+    k = fmax(k, 1e-4);
+    const double den = msqrt(1.0 + 2.0 * k) - 1.0;
+    vel = vel * divr(k, den, recip(den));
+    return divr(vel, h.dxmin, h.idxmin);
+  }
   const double vx = c + fabs(q[IU]);
   const double vy = c + fabs(q[IV]);
   if constexpr (NV == 5) {
@@ -65,6 +112,13 @@ __device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &
   } else {
     return divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy);
   }
+}
+template <int NV>
+__device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &h) {
+  HSt<NV> q;
+  double c;
+  hyd::prims(u, h.P, c, q);
+  return inv_dt_prim(q, c, h);
 }
 
 // ---- ghost fill, one direction, both faces (MakeBoundariesFunctor2D/3D,
@@ -141,18 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_hprim(const double *__restrict__ U,
       const int g = h.gw;
       const bool inner = i >= g && i < h.isize - g && j >= g && j < h.jsize - g &&
                          (NV == 4 || (k >= g && k < h.ksize - g));
-      if (inner) {
-        const double vx = c + fabs(q[IU]);
-        const double vy = c + fabs(q[IV]);
-        double w;
-        if constexpr (NV == 5) {
-          const double vz = c + fabs(q[IW]);
-          w = divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy) + divr(vz, h.dz, h.idz);
-        } else {
-          w = divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy);
-        }
-        best = fmax(best, w);
-      }
+      if (inner) best = fmax(best, inv_dt_prim(q, c, h));
     }
   }
   if (!do_dt) return;
@@ -191,6 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
   if (!active(ctrl)) return;
   const double dt = ctrl[CTRL_DT];
   const double dtd[3] = {dt / h.dx, dt / h.dy, DIM == 3 ? dt / h.dz : 0.0};
+  const double hdt = 0.5 * dt;
   const int nsx = h.nx + 1, nsy = h.ny + 1, nsz = DIM == 3 ? h.nz + 1 : 1;
   const long long total = (long long)nsx * nsy * nsz;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
@@ -212,6 +256,8 @@ __global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
       HSt<NV> qr, ql, f;
       hyd::face_state(q, d == 0 ? dX : (d == 1 ? dY : dZ), s0, dtd[d], false, h.P.smallr, qr);
       hyd::face_state(qn, d == 0 ? nX : (d == 1 ? nY : nZ), ns0, dtd[d], true, h.P.smallr, ql);
+      gravity_predict(ql, h, hdt);
+      gravity_predict(qr, h, hdt);
       swap_normal(ql, d);
       swap_normal(qr, d);
       hyd::riemann_rs<RS>(ql, qr, f, h.P);
@@ -231,6 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
   const int nz = DIM == 3 ? h.nz : 1;
   const long long total = (long long)h.nx * h.ny * nz;
   const long long vs = h.vstride;
+  const double hdt = 0.5 * ctrl[CTRL_DT];
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int i = h.gw + (int)(t % h.nx);
@@ -238,6 +285,8 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
     const int j = h.gw + (int)(r % h.ny);
     const int k = DIM == 3 ? h.gw + (int)(r / h.ny) : 0;
     const long long c = ((long long)k * h.jsize + j) * h.isize + i;
+    const double rho_old = U[c];  // U holds a copy of the step's input state
+    HSt<NV> on;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double *fx = F + (long long)v * vs;
@@ -252,8 +301,11 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
         o += fz[c];
         o -= fz[c + h.plane];
       }
-      U[c + v * vs] = o;
+      on[v] = o;
     }
+    gravity_source(on, rho_old, h, hdt);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) U[c + v * vs] = on[v];
   }
 }
 
@@ -285,11 +337,11 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
   const bool out_lane = lane >= 2 && lane < 2 + kHydroCols && i < h.gw + h.nx;
   const long long vs = h.vstride;
   const double dt = ctrl[CTRL_DT];
-  const double dtdx = dt / h.dx, dtdy = dt / h.dy;
+  const double dtdx = dt / h.dx, dtdy = dt / h.dy, hdt = 0.5 * dt;
   const hyd::HydroPar &P = h.P;
 
   HSt<NV> qm, q0, u0, t;
-  double c;
+  double c, rho_prev = 0.0;  // density of (i, s-1) before the step, for the gravity source
   load(Uin, vs, (long long)(r0 - 2) * h.isize + ic, t);
   hyd::prims(t, P, c, qm);
   load(Uin, vs, (long long)(r0 - 1) * h.isize + ic, u0);
@@ -312,6 +364,7 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
     if (s >= r0) {
       HSt<NV> ql = ymax_prev, qr, fy;
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
+      gravity_predict(qr, h, hdt);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
       hyd::riemann_rs<RS>(ql, qr, fy, P);
@@ -321,16 +374,18 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
       if (s > r0 && out_lane) {  // finish (i, s-1): its upper y face is this one
         const long long cp = (long long)(s - 1) * h.isize + i;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          o_prev[v] -= fy[v];
-          Uout[cp + v * vs] = o_prev[v];
-        }
+        for (int v = 0; v < NV; ++v) o_prev[v] -= fy[v];
+        gravity_source(o_prev, rho_prev, h, hdt);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Uout[cp + v * vs] = o_prev[v];
         if (fold_cfl) best = fmax(best, inv_dt_cell(o_prev, h));
       }
       if (s < r1) {
         HSt<NV> xmin, xmax, xl, fx;
         hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
         hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
+        gravity_predict(xmin, h, hdt);
+        gravity_predict(xmax, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
         hyd::riemann_rs<RS>(xl, xmin, fx, P);
@@ -344,9 +399,11 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
           o += fy[v];
           o_prev[v] = o;
         }
+        rho_prev = u0[ID];
       }
     }
     hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax_prev);
+    gravity_predict(ymax_prev, h, hdt);
     qm = q0;
     q0 = qp;
     u0 = un;
@@ -397,12 +454,12 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
   const long long vs = h.vstride;
   const long long col = (long long)jc * h.isize + ic;
   const double dt = ctrl[CTRL_DT];
-  const double dtdx = dt / h.dx, dtdy = dt / h.dy, dtdz = dt / h.dz;
+  const double dtdx = dt / h.dx, dtdy = dt / h.dy, dtdz = dt / h.dz, hdt = 0.5 * dt;
   const hyd::HydroPar &P = h.P;
   const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63;
 
   HSt<NV> qm, q0, u0, t;
-  double c;
+  double c, rho_prev = 0.0;  // density of (i, j, s-1) before the step, for the gravity source
   load(Uin, vs, (long long)(k0 - 2) * h.plane + col, t);
   hyd::prims(t, P, c, qm);
   load(Uin, vs, (long long)(k0 - 1) * h.plane + col, u0);
@@ -432,6 +489,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       hyd::sources(q0, dX, dY, dZ, P.gamma0, s0);
       HSt<NV> ymax;
       hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax);
+      gravity_predict(ymax, h, hdt);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sT[v][ly][lane] = ymax[v];
     }
@@ -441,6 +499,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
     if (s >= k0 && out_row) {  // z face below plane s (also the top face of plane k1 - 1)
       HSt<NV> ql = zmax_prev, qr;
       hyd::face_state(q0, dZ, s0, dtdz, false, P.smallr, qr);
+      gravity_predict(qr, h, hdt);
       swap_normal(ql, 2);
       swap_normal(qr, 2);
       hyd::riemann_rs<RS>(ql, qr, fz, P);
@@ -453,6 +512,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
 #pragma unroll
       for (int v = 0; v < NV; ++v) ql[v] = sT[v][ly - 1][lane];
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
+      gravity_predict(qr, h, hdt);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
       hyd::riemann_rs<RS>(ql, qr, fy, P);
@@ -464,6 +524,8 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       HSt<NV> xmin, xmax, xl;
       hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
       hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
+      gravity_predict(xmin, h, hdt);
+      gravity_predict(xmax, h, hdt);
 #pragma unroll
       for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
       hyd::riemann_rs<RS>(xl, xmin, fx, P);
@@ -475,10 +537,10 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       if (s > k0 && out) {  // finish (i, j, s-1): its upper z face is this one
         const long long cp = (long long)(s - 1) * h.plane + (long long)j * h.isize + i;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          o_prev[v] -= fz[v];
-          Uout[cp + v * vs] = o_prev[v];
-        }
+        for (int v = 0; v < NV; ++v) o_prev[v] -= fz[v];
+        gravity_source(o_prev, rho_prev, h, hdt);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Uout[cp + v * vs] = o_prev[v];
         if (fold_cfl) best = fmax(best, inv_dt_cell(o_prev, h));
       }
       if (plane_out) {
@@ -493,9 +555,13 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
           o += fz[v];
           o_prev[v] = o;
         }
+        rho_prev = u0[ID];
       }
     }
-    if (slope_row) hyd::face_state(q0, dZ, s0, dtdz, true, P.smallr, zmax_prev);
+    if (slope_row) {
+      hyd::face_state(q0, dZ, s0, dtdz, true, P.smallr, zmax_prev);
+      gravity_predict(zmax_prev, h, hdt);
+    }
     qm = q0;
     q0 = qp;
     u0 = un;

@@ -114,6 +114,12 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   d.P.slope_type = p.slope_type;
   d.P.riemann = p.riemann;
   d.P.entho = 1.0 / (p.gamma0 - 1.0);  // RiemannSolvers.h: entho = ONE_F / (gamma0 - ONE_F)
+  // static gravity; the field is set by the problem (zero until then, as the reference's)
+  d.grav = p.gravity_static ? 1 : 0;
+  d.g[0] = d.g[1] = d.g[2] = 0.0;
+  d.cfl = p.cfl;
+  d.dxmin = p.dim == 3 ? std::fmin(std::fmin(p.dx, p.dy), p.dz) : std::fmin(p.dx, p.dy);
+  d.idxmin = 1.0 / d.dxmin;
   // tuning knob: output rows (2D) / planes (3D) of one marching wave / block
   const char *strip = std::getenv("PPKMHD_HYDRO_STRIP");
   const int rows = strip ? std::atoi(strip) : 0;
@@ -183,7 +189,9 @@ int lax_liu_row(int config_number) {
 // InitKelvinHelmholtzFunctor2D/3D (HydroInitFunctors2D.h:216-348, 3D.h:271-427),
 // InitGreshoVortexFunctor2D/3D (HydroInitFunctors2D.h:352-449, 3D.h:429-530),
 // InitIsentropicVortexFunctor2D (HydroInitFunctors2D.h:570-660),
-// InitFourQuadrantFunctor2D (HydroInitFunctors2D.h:453-565, SolverHydroMuscl.cpp:72-89)
+// InitFourQuadrantFunctor2D (HydroInitFunctors2D.h:453-565, SolverHydroMuscl.cpp:72-89),
+// RayleighTaylorInstabilityFunctor2D/3D (HydroInitFunctors2D.h:663-787, 3D.h:548-677),
+// RisingBubbleFunctor2D/3D (HydroInitFunctors2D.h:790-906, 3D.h:685-802)
 void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
   const int gw = p.ghost_width, dim = p.dim, nv = dim == 3 ? 5 : 4;
   const int is = p.nx + 2 * gw, js = p.ny + 2 * gw, ks = dim == 3 ? p.nz + 2 * gw : 1;
@@ -285,6 +293,35 @@ void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
           s[IV] = rho * (pr.iv_v_a + dv);
           s[IP] = rho * T / (g0 - 1.0) + 0.5 * rho * (pr.iv_u_a + du) * (pr.iv_u_a + du) +
                   0.5 * rho * (pr.iv_v_a + dv) * (pr.iv_v_a + dv);
+        } else if (pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR) {
+          // heavy fluid over light: density d1 above the mid-plane of the vertical axis (y in
+          // 2D, z in 3D), a cosine bump of the vertical velocity written into the momentum
+          // slot as the reference does, pressure P0 + rho (g . x) with P0 = 1 (hydrostatic)
+          const double P0 = 1.0, Lx = p.xmax - p.xmin, Ly = p.ymax - p.ymin;
+          const double bump = pr.rti_amplitude * (1 + std::cos(2 * icb::kPi * x / Lx)) *
+                              (1 + std::cos(2 * icb::kPi * y / Ly)) / 4;
+          if (dim == 2) {
+            s[ID] = y > (p.ymin + p.ymax) / 2 ? pr.rti_d1 : pr.rti_d0;
+            s[IU] = 0.0;
+            s[IV] = bump;
+            s[IP] = (P0 + s[ID] * (pr.rti_gx * x + pr.rti_gy * y)) / (g0 - 1.0);
+          } else {
+            s[ID] = z > (p.zmin + p.zmax) / 2 ? pr.rti_d1 : pr.rti_d0;
+            s[IU] = 0.0;
+            s[IV] = 0.0;
+            s[IW] = bump;
+            s[IP] = (P0 + s[ID] * (pr.rti_gx * x + pr.rti_gy * y + pr.rti_gz * z)) / (g0 - 1);
+          }
+        } else if (pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
+          // a ball of density din in dout, at rest, pressure P0 + rho (g . x) with P0 = 1
+          const double P0 = 1.0;
+          double r2 = (x - pr.rb_x0) * (x - pr.rb_x0) + (y - pr.rb_y0) * (y - pr.rb_y0);
+          if (dim == 3) r2 = r2 + (z - pr.rb_z0) * (z - pr.rb_z0);
+          s[ID] = std::sqrt(r2) < pr.rb_R ? pr.rb_din : pr.rb_dout;
+          s[IU] = s[IV] = s[IW] = 0.0;
+          s[IP] = dim == 2 ? (P0 + s[ID] * (pr.rb_gx * x + pr.rb_gy * y)) / (g0 - 1.0)
+                           : (P0 + s[ID] * (pr.rb_gx * x + pr.rb_gy * y + pr.rb_gz * z)) /
+                                 (g0 - 1.0);
         } else if (pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {  // 2D
           const int quad = x < pr.fq_x ? (y < pr.fq_y ? 2 : 1) : (y < pr.fq_y ? 3 : 0);
           const double *q = kLaxLiu[lax_liu_row(pr.fq_config)][quad];  // rho, u, v, p
@@ -451,12 +488,20 @@ int hfinish_init(ppk_hydro_t *h) {
 bool hydro_problem_ok(int kind, int dim) {
   return kind == PPK_PROBLEM_IMPLODE || kind == PPK_PROBLEM_BLAST ||
          kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || kind == PPK_PROBLEM_GRESHO_VORTEX ||
+         kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || kind == PPK_PROBLEM_RISING_BUBBLE ||
          ((kind == PPK_PROBLEM_ISENTROPIC_VORTEX || kind == PPK_PROBLEM_FOUR_QUADRANT) &&
           dim == 2);
 }
 const char *const kHydroProblems =
     "hydro problems: implode, blast, kelvin_helmholtz, gresho_vortex, isentropic_vortex (2D), "
-    "four_quadrant (2D)";
+    "four_quadrant (2D), rayleigh_taylor, rising_bubble";
+// the reference writes the gravity field of these problems into a view that exists only when
+// gravity is enabled
+bool hydro_problem_needs_gravity(int kind) {
+  return kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || kind == PPK_PROBLEM_RISING_BUBBLE;
+}
+const char *const kNeedsGravity =
+    "rayleigh_taylor and rising_bubble need static gravity ([gravity] static=yes)";
 
 }  // namespace
 
@@ -549,8 +594,16 @@ int ppk_hydro_init_problem(ppk_hydro_t *h, const ppk_mhd_problem *problem) {
   if (!h || !problem) return hfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!hydro_problem_ok(problem->kind, h->p.dim))
     return hfail(h, PPK_ERR_UNSUPPORTED, kHydroProblems);
+  if (hydro_problem_needs_gravity(problem->kind) && !h->p.gravity_static)
+    return hfail(h, PPK_ERR_INVALID_ARGUMENT, kNeedsGravity);
   std::vector<double> host(h->values());
   ppk::hydro_ics(h->p, *problem, host.data());
+  // the problem's uniform gravity field (zero for the problems that set none)
+  const bool rti = problem->kind == PPK_PROBLEM_RAYLEIGH_TAYLOR;
+  const bool rb = problem->kind == PPK_PROBLEM_RISING_BUBBLE;
+  h->hd.g[0] = rti ? problem->rti_gx : rb ? problem->rb_gx : 0.0;
+  h->hd.g[1] = rti ? problem->rti_gy : rb ? problem->rb_gy : 0.0;
+  h->hd.g[2] = h->p.dim == 3 ? (rti ? problem->rti_gz : rb ? problem->rb_gz : 0.0) : 0.0;
   h->iter_host = 0;
   h->cfl_valid = false;
   HHIP(h, hipMemcpyAsync(h->cur(), host.data(), h->state_bytes(), hipMemcpyHostToDevice,

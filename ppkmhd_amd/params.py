@@ -134,4 +134,10 @@ def default_problem(kind, params=None, float_round=True):
     pr.fq_config = 0
     pr.fq_x = r(0.8)
     pr.fq_y = r(0.8)
+    # RayleighTaylorInstabilityParams, RisingBubbleParams
+    pr.rti_amplitude, pr.rti_d0, pr.rti_d1 = r(0.01), r(1.0), r(2.0)
+    pr.rti_gx, pr.rti_gy, pr.rti_gz = r(0.0), r(-0.1), r(0.0)
+    pr.rb_R, pr.rb_x0, pr.rb_y0, pr.rb_z0 = r(0.1), r(0.5), r(0.25), r(0.5)
+    pr.rb_din, pr.rb_dout = r(1.0), r(2.0)
+    pr.rb_gx, pr.rb_gy, pr.rb_gz = r(0.0), r(-0.1), r(0.0)
     return pr

@@ -282,7 +282,8 @@ def test_sod_shock_tube_converges(H, rs):
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("kind", [A.PROBLEM_IMPLODE, A.PROBLEM_BLAST, A.PROBLEM_KELVIN_HELMHOLTZ,
                                   A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX,
-                                  A.PROBLEM_FOUR_QUADRANT])
+                                  A.PROBLEM_FOUR_QUADRANT, A.PROBLEM_RAYLEIGH_TAYLOR,
+                                  A.PROBLEM_RISING_BUBBLE])
 def test_initial_state_bit_identical_to_oracle(H, dim, kind):
     from ppkmhd_amd.solver import hydro_initial_state
 
@@ -297,7 +298,9 @@ def test_initial_state_bit_identical_to_oracle(H, dim, kind):
         pr.kh_p_rand = variant      # random layer, then the Robertson sine perturbation
         pr.fq_config = variant
         a, b = hydro_initial_state(p, pr), H.init_array(p, pr)
-        transcendental = kind in (A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX) or (
+        pr.rb_x0, pr.rb_y0, pr.rb_z0, pr.rb_R = 0.6, 0.5, 0.4, 0.35  # a bubble inside the box
+        transcendental = kind in (A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX,
+                                  A.PROBLEM_RAYLEIGH_TAYLOR) or (
             kind == A.PROBLEM_KELVIN_HELMHOLTZ and not pr.kh_p_rand)
         if transcendental:  # sin / cos / exp / log / pow of two compilers' math calls
             np.testing.assert_allclose(a, b, rtol=4e-16, atol=1e-300)
@@ -451,6 +454,118 @@ def test_four_quadrant_states_are_lax_liu_configurations(H):
             assert r == "J" or r < 5e-3, (cfg + 1, res)
 
 
+def _gravity_solver(H, dim, n, g, bc=A.BC_PERIODIC, **kw):
+    """Oracle solver with static gravity g: the rising-bubble problem sets the field."""
+    p = make_hydro_params(dim, *n, gamma0=1.4, cfl=0.7, slope_type=2.0, riemann=A.RIEMANN_HLLC,
+                          bc=bc, tEnd=100.0, **kw)
+    p.gravity_static = 1
+    pr = default_problem(A.PROBLEM_RISING_BUBBLE, p)
+    pr.rb_gx, pr.rb_gy, pr.rb_gz = g
+    s = H.Solver(p)
+    s.init_problem(pr)
+    return p, pr, s
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gravity_free_fall(H, dim):
+    """A uniform gas at rest under uniform gravity (periodic box) falls freely: the fluxes stay
+    uniform (the half-step predictor moves every face state alike), density and internal energy
+    do not change, the momentum after n steps of dt is rho g n dt and the total energy grows by
+    exactly the kinetic energy.  Pins the source term's momentum and energy updates."""
+    g = (0.3, -0.7, 0.2)[:dim] + ((0.0,) if dim == 2 else ())
+    p, pr, s = _gravity_solver(H, dim, (8, 6, 4), g)
+    U0 = H.zeros(p)
+    rho, e0 = 1.7, 2.5
+    U0[0], U0[1] = rho, e0
+    s.set_state(U0)
+    dt, n = 0.013, 12
+    for _ in range(n):
+        s.step(dt)
+    U = s.get_state()
+    st = U[:, 2:-2, 2:-2, 2:-2] if dim == 3 else U[:, :, 2:-2, 2:-2]
+    np.testing.assert_array_equal(st[0], rho)
+    for d in range(dim):
+        np.testing.assert_allclose(st[2 + d], rho * g[d] * n * dt, rtol=1e-13, atol=1e-15)
+    ekin = 0.5 * sum(st[2 + d] ** 2 for d in range(dim)) / rho
+    np.testing.assert_allclose(st[1], e0 + ekin, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gravity_cfl_solves_its_quadratic(H, dim):
+    """The gravity CFL dt of a uniform moving state satisfies g dt^2 / (2 dx) + u dt / dx = cfl
+    (u = sum of c + |v_d|, g = sum |g_d|, dx the smallest cell size), the condition the
+    reference derives it from; a vanishing g gives the plain CFL dt to 5e-5 (the 1e-4 floor)."""
+    for g in ((0.3, -0.7, 0.2), (1e-12, 0.0, 0.0)):
+        gg = g[:dim] + ((0.0,) if dim == 2 else ())
+        p, pr, s = _gravity_solver(H, dim, (8, 6, 4), gg, xmax=1.3)
+        U0 = H.zeros(p)
+        rho, pres, v = 1.3, 0.8, (0.2, -0.4, 0.1)[:dim]
+        U0[0] = rho
+        for d in range(dim):
+            U0[2 + d] = rho * v[d]
+        U0[1] = pres / (p.gamma0 - 1) + 0.5 * rho * sum(x * x for x in v)
+        s.set_state(U0)
+        c = np.sqrt(p.gamma0 * pres / rho)
+        u = sum(c + abs(x) for x in v)
+        dxm = min(p.dx, p.dy, p.dz if dim == 3 else np.inf)
+        dt = p.cfl / s.inv_dt()
+        G = sum(abs(x) for x in gg)
+        if G > 1e-6:
+            assert G * dt * dt / (2 * dxm) + u * dt / dxm == pytest.approx(p.cfl, rel=1e-12)
+        else:
+            assert u * dt / dxm == pytest.approx(p.cfl, rel=6e-5)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gravity_hydrostatic_column_and_rising_bubble(H, dim):
+    """Sign of gravity against the pressure gradient: the Rayleigh-Taylor column without its
+    perturbation (hydrostatic about the interface plane, reflecting walls across g) stays
+    nearly at rest, |v| below 25 % of the free-fall speed g t after 30 steps (measured 7 % in
+    2D, 11 % in 3D: the scheme is not well balanced), where the same state under -g reaches
+    200 % / 130 %; and a light bubble rises against g (its density-deficit centroid moves up)."""
+    n = (16, 24) if dim == 2 else (8, 8, 16)
+    up = 1 if dim == 2 else 2
+    bc = [A.BC_PERIODIC] * 6
+    bc[2 * up] = bc[2 * up + 1] = A.BC_DIRICHLET
+    p = make_hydro_params(dim, *n, gamma0=1.4, cfl=0.7, slope_type=2.0, riemann=A.RIEMANN_HLLC,
+                          bc=bc, tEnd=100.0, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5,
+                          zmin=-0.5, zmax=0.5)
+    p.gravity_static = 1
+    pr = default_problem(A.PROBLEM_RAYLEIGH_TAYLOR, p)
+    pr.rti_amplitude = 0.0
+    pr.rti_gx, pr.rti_gy, pr.rti_gz = (0.0, -0.1, 0.0) if dim == 2 else (0.0, 0.0, -0.1)
+    s = H.Solver(p)
+    s.init_problem(pr)
+    for _ in range(30):
+        s.step(0.0, A.DT_CFL)
+    t = s.time()[0]
+    U = s.get_state()
+    vmax = np.abs(U[2:2 + dim] / U[0]).max()
+    assert vmax < 0.25 * 0.1 * t, (vmax, t)
+    # rising bubble, light inside
+    p2 = make_hydro_params(dim, *n, gamma0=1.4, cfl=0.7, slope_type=2.0,
+                           riemann=A.RIEMANN_HLLC, bc=bc, tEnd=100.0,
+                           ymax=1.5 if dim == 2 else 1.0, zmax=1.5)
+    p2.gravity_static = 1
+    pr2 = default_problem(A.PROBLEM_RISING_BUBBLE, p2)
+    pr2.rb_R, pr2.rb_din, pr2.rb_dout = 0.2, 1.0, 2.0
+    pr2.rb_x0, pr2.rb_y0, pr2.rb_z0 = 0.5, 0.5, 0.5
+    pr2.rb_gx, pr2.rb_gy, pr2.rb_gz = (0.0, -0.5, 0.0) if dim == 2 else (0.0, 0.0, -0.5)
+    s2 = H.Solver(p2)
+    s2.init_problem(pr2)
+
+    def centroid(U):
+        deficit = np.clip(2.0 - U[0], 0.0, None)
+        ax = U.shape[2] if dim == 2 else U.shape[1]
+        prof = deficit.sum(axis=tuple(a for a in range(3) if a != (1 if dim == 2 else 0)))
+        return (prof * np.arange(prof.size)).sum() / prof.sum()
+
+    c0 = centroid(s2.get_state())
+    for _ in range(40):
+        s2.step(0.0, A.DT_CFL)
+    assert centroid(s2.get_state()) > c0 + 0.3
+
+
 def test_four_quadrant_selection_and_corner(H):
     """config_number 18 (the reference's switch falls from it into the default), negative and
     out-of-range numbers all give configuration 1; cells exactly at the corner coordinate
@@ -521,8 +636,26 @@ implementationVersion=0
     # the reference this library lacks are refused at init
     _, pr4, _ = params_from_ini(text=txt.replace("problem=implode", "problem=nonsense"))
     assert pr4.kind == A.PROBLEM_IMPLODE
-    _, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=rayleigh_taylor"))
-    assert pr5.kind == -1
+    _, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=disk"))
+    assert pr5.kind == -1  # point-mass gravity: not provided
+    # settings/test_rayleigh_taylor_2d.ini and test_muscl_rising_bubble_3d.ini sections
+    p8, pr8, _ = params_from_ini(text=txt.replace("problem=implode", "problem=rayleigh_taylor")
+                                 .replace("[other]", "[gravity]\nstatic=yes\n[rayleigh_taylor]\n"
+                                          "amplitude=0.01\nd0=1.0\nd1=2.0\ngx=0.0\ngy=-0.1\n"
+                                          "[other]"))
+    assert pr8.kind == A.PROBLEM_RAYLEIGH_TAYLOR and p8.gravity_static == 1
+    assert (pr8.rti_amplitude, pr8.rti_d1, pr8.rti_gy) == (f32(0.01), 2.0, f32(-0.1))
+    t9 = (txt.replace("problem=implode", "problem=rising_bubble")
+          .replace("Hydro_Muscl_2D", "Hydro_Muscl_3D")
+          .replace("[other]", "[gravity]\nstatic=yes\n[rising_bubble]\nR=0.1\ndin=1.0\n"
+                   "dout=2.0\ngx=0.0\ngy=0.0\ngz=-0.1\n[other]"))
+    p9, pr9, _ = params_from_ini(text=t9)
+    assert pr9.kind == A.PROBLEM_RISING_BUBBLE and p9.gravity_static == 1 and p9.dim == 3
+    assert (pr9.rb_R, pr9.rb_gy, pr9.rb_gz, pr9.rb_y0) == (f32(0.1), 0.0, f32(-0.1), 0.25)
+    # gravity off unless [gravity] static or point (SolverBase.cpp:138-141)
+    assert params_from_ini(text=txt)[0].gravity_static == 0
+    assert params_from_ini(text=txt.replace("[other]", "[gravity]\npoint=yes\n[other]"))[0] \
+        .gravity_static == 1
     # settings/test_four_quadrant_2D.ini's problem section
     _, pr7, _ = params_from_ini(text=txt.replace("problem=implode", "problem=four_quadrant")
                                 .replace("[other]", "[riemann2d]\nconfig_number=2\nx=0.8\ny=0.8\n[other]"))
