@@ -51,11 +51,19 @@ __device__ __forceinline__ void swap_normal(HSt<NV> &q, int dir) {
   q[IU + dir] = t;
 }
 
+// Gravity terms take a mode GR: 0 none, 1 static gravity, -1 decided at run time by h.grav
+// (the staged kernels); the fused kernels are compiled for GR = 0 and 1, so the step without
+// gravity carries none of its code.
+template <int GR>
+__device__ __forceinline__ bool gravity_on(const HydroDev &h) {
+  return GR == 1 || (GR < 0 && h.grav);
+}
+
 // gravity predictor of a face state, half a time step (HydroRunFunctors2D.h:650-660,
 // 3D.h:485-497): v += (0.5 dt) g, before the normal is rotated into IU
-template <int NV>
+template <int GR, int NV>
 __device__ __forceinline__ void gravity_predict(HSt<NV> &q, const HydroDev &h, double hdt) {
-  if (!h.grav) return;
+  if (!gravity_on<GR>(h)) return;
   q[IU] += hdt * h.g[0];
   q[IV] += hdt * h.g[1];
   if constexpr (NV == 5) q[IW] += hdt * h.g[2];
@@ -65,10 +73,10 @@ __device__ __forceinline__ void gravity_predict(HSt<NV> &q, const HydroDev &h, d
 // cell o whose density before the step was rho_old: momentum += (0.5 dt) g (rho_old + rho_new),
 // energy += the change of kinetic energy.  The 2D functor floors the new density at smallr,
 // the 3D one does not.
-template <int NV>
+template <int GR, int NV>
 __device__ __forceinline__ void gravity_source(HSt<NV> &o, double rho_old, const HydroDev &h,
                                                double hdt) {
-  if (!h.grav) return;
+  if (!gravity_on<GR>(h)) return;
   const double rho_new = NV == 4 ? fmax(h.P.smallr, o[ID]) : o[ID];
   const double ir = recip(rho_new);
   double mu = o[IU], mv = o[IV], mw = NV == 5 ? o[IW] : 0.0;
@@ -89,9 +97,9 @@ __device__ __forceinline__ void gravity_source(HSt<NV> &o, double rho_old, const
 // gravity ComputeDtGravityFunctor2D/3D (HydroRunFunctors2D.h:120-170, 3D.h:130-185): the
 // summed speeds u scaled by k / (sqrt(1 + 2k) - 1), k = |g|_1 cfl dx / u^2 floored at 1e-4,
 // which makes g dt^2 / (2 dx) + u dt / dx = cfl, over the smallest cell size
-template <int NV>
+template <int GR, int NV>
 __device__ __forceinline__ double inv_dt_prim(const HSt<NV> &q, double c, const HydroDev &h) {
-  if (h.grav) {
+  if (gravity_on<GR>(h)) {
     double vel = 0.0;
     vel += c + fabs(q[IU]);
     vel += c + fabs(q[IV]);
@@ -113,12 +121,12 @@ __device__ __forceinline__ double inv_dt_prim(const HSt<NV> &q, double c, const 
     return divr(vx, h.dx, h.idx) + divr(vy, h.dy, h.idy);
   }
 }
-template <int NV>
+template <int GR, int NV>
 __device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &h) {
   HSt<NV> q;
   double c;
   hyd::prims(u, h.P, c, q);
-  return inv_dt_prim(q, c, h);
+  return inv_dt_prim<GR>(q, c, h);
 }
 
 // ---- ghost fill, one direction, both faces (MakeBoundariesFunctor2D/3D,
@@ -195,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_hprim(const double *__restrict__ U,
       const int g = h.gw;
       const bool inner = i >= g && i < h.isize - g && j >= g && j < h.jsize - g &&
                          (NV == 4 || (k >= g && k < h.ksize - g));
-      if (inner) best = fmax(best, inv_dt_prim(q, c, h));
+      if (inner) best = fmax(best, inv_dt_prim<-1>(q, c, h));
     }
   }
   if (!do_dt) return;
@@ -256,8 +264,8 @@ __global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
       HSt<NV> qr, ql, f;
       hyd::face_state(q, d == 0 ? dX : (d == 1 ? dY : dZ), s0, dtd[d], false, h.P.smallr, qr);
       hyd::face_state(qn, d == 0 ? nX : (d == 1 ? nY : nZ), ns0, dtd[d], true, h.P.smallr, ql);
-      gravity_predict(ql, h, hdt);
-      gravity_predict(qr, h, hdt);
+      gravity_predict<-1>(ql, h, hdt);
+      gravity_predict<-1>(qr, h, hdt);
       swap_normal(ql, d);
       swap_normal(qr, d);
       hyd::riemann_rs<RS>(ql, qr, f, h.P);
@@ -303,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
       }
       on[v] = o;
     }
-    gravity_source(on, rho_old, h, hdt);
+    gravity_source<-1>(on, rho_old, h, hdt);
 #pragma unroll
     for (int v = 0; v < NV; ++v) U[c + v * vs] = on[v];
   }
@@ -317,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
 // (left state from lane l-1), and finishes the update of (i, s-1) with the flux of that y face.
 constexpr int kWaves2 = 4;
 
-template <int NV, int RS, int MINW>
+template <int NV, int RS, int MINW, int GR>
 __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *__restrict__ Uin,
                                                            double *__restrict__ Uout, HydroDev h,
                                                            double *__restrict__ parts,
@@ -364,7 +372,7 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
     if (s >= r0) {
       HSt<NV> ql = ymax_prev, qr, fy;
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
-      gravity_predict(qr, h, hdt);
+      gravity_predict<GR>(qr, h, hdt);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
       hyd::riemann_rs<RS>(ql, qr, fy, P);
@@ -375,17 +383,17 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
         const long long cp = (long long)(s - 1) * h.isize + i;
 #pragma unroll
         for (int v = 0; v < NV; ++v) o_prev[v] -= fy[v];
-        gravity_source(o_prev, rho_prev, h, hdt);
+        gravity_source<GR>(o_prev, rho_prev, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) Uout[cp + v * vs] = o_prev[v];
-        if (fold_cfl) best = fmax(best, inv_dt_cell(o_prev, h));
+        if (fold_cfl) best = fmax(best, inv_dt_cell<GR>(o_prev, h));
       }
       if (s < r1) {
         HSt<NV> xmin, xmax, xl, fx;
         hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
         hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
-        gravity_predict(xmin, h, hdt);
-        gravity_predict(xmax, h, hdt);
+        gravity_predict<GR>(xmin, h, hdt);
+        gravity_predict<GR>(xmax, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
         hyd::riemann_rs<RS>(xl, xmin, fx, P);
@@ -403,7 +411,7 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
       }
     }
     hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax_prev);
-    gravity_predict(ymax_prev, h, hdt);
+    gravity_predict<GR>(ymax_prev, h, hdt);
     qm = q0;
     q0 = qp;
     u0 = un;
@@ -423,7 +431,7 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
 // of plane s-1 kept in registers, so the update of (i, j, s-1) completes at plane s.
 constexpr int kW3 = kHydroWaves3;
 
-template <int NV, int RS, int MINW>
+template <int NV, int RS, int MINW, int GR>
 __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__restrict__ Uin,
                                                        double *__restrict__ Uout, HydroDev h,
                                                        double *__restrict__ parts,
@@ -489,7 +497,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       hyd::sources(q0, dX, dY, dZ, P.gamma0, s0);
       HSt<NV> ymax;
       hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax);
-      gravity_predict(ymax, h, hdt);
+      gravity_predict<GR>(ymax, h, hdt);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sT[v][ly][lane] = ymax[v];
     }
@@ -499,7 +507,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
     if (s >= k0 && out_row) {  // z face below plane s (also the top face of plane k1 - 1)
       HSt<NV> ql = zmax_prev, qr;
       hyd::face_state(q0, dZ, s0, dtdz, false, P.smallr, qr);
-      gravity_predict(qr, h, hdt);
+      gravity_predict<GR>(qr, h, hdt);
       swap_normal(ql, 2);
       swap_normal(qr, 2);
       hyd::riemann_rs<RS>(ql, qr, fz, P);
@@ -512,7 +520,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
 #pragma unroll
       for (int v = 0; v < NV; ++v) ql[v] = sT[v][ly - 1][lane];
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
-      gravity_predict(qr, h, hdt);
+      gravity_predict<GR>(qr, h, hdt);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
       hyd::riemann_rs<RS>(ql, qr, fy, P);
@@ -524,8 +532,8 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       HSt<NV> xmin, xmax, xl;
       hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
       hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
-      gravity_predict(xmin, h, hdt);
-      gravity_predict(xmax, h, hdt);
+      gravity_predict<GR>(xmin, h, hdt);
+      gravity_predict<GR>(xmax, h, hdt);
 #pragma unroll
       for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
       hyd::riemann_rs<RS>(xl, xmin, fx, P);
@@ -538,10 +546,10 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
         const long long cp = (long long)(s - 1) * h.plane + (long long)j * h.isize + i;
 #pragma unroll
         for (int v = 0; v < NV; ++v) o_prev[v] -= fz[v];
-        gravity_source(o_prev, rho_prev, h, hdt);
+        gravity_source<GR>(o_prev, rho_prev, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) Uout[cp + v * vs] = o_prev[v];
-        if (fold_cfl) best = fmax(best, inv_dt_cell(o_prev, h));
+        if (fold_cfl) best = fmax(best, inv_dt_cell<GR>(o_prev, h));
       }
       if (plane_out) {
 #pragma unroll
@@ -560,7 +568,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
     }
     if (slope_row) {
       hyd::face_state(q0, dZ, s0, dtdz, true, P.smallr, zmax_prev);
-      gravity_predict(zmax_prev, h, hdt);
+      gravity_predict<GR>(zmax_prev, h, hdt);
     }
     qm = q0;
     q0 = qp;
@@ -658,28 +666,40 @@ int hydro_fused_parts(const HydroDev &h) {
   return h.dim == 3 ? fused3d_blocks(h) : fused2d_waves(h);
 }
 
-template <int RS>
-void launch_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
-                  const double *ctrl, int fold_cfl, hipStream_t s) {
+template <int RS, int GR>
+void launch_fused_gr(const HydroDev &h, const double *Uin, double *Uout, double *parts,
+                     const double *ctrl, int fold_cfl, hipStream_t s) {
   // occupancy variant (tuning): 0 = the compiler's register budget, 1 = 4 waves per SIMD
   if (h.dim == 3) {
     if constexpr (RS != PPK_RIEMANN_HLL) {  // no 3D HLL (refused at create)
       const dim3 g(fused3d_blocks(h)), b(kW3 * 64);
       if (h.occupancy == 1)
-        hipLaunchKernelGGL((k_hfused3d<5, RS, 4>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+        hipLaunchKernelGGL((k_hfused3d<5, RS, 4, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
                            fold_cfl);
       else
-        hipLaunchKernelGGL((k_hfused3d<5, RS, 1>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+        hipLaunchKernelGGL((k_hfused3d<5, RS, 1, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
                            fold_cfl);
     }
   } else {
     const int waves = fused2d_waves(h);
     const dim3 g((waves + kWaves2 - 1) / kWaves2), b(kWaves2 * 64);
     if (h.occupancy == 1)
-      hipLaunchKernelGGL((k_hfused2d<4, RS, 4>), g, b, 0, s, Uin, Uout, h, parts, ctrl, fold_cfl);
+      hipLaunchKernelGGL((k_hfused2d<4, RS, 4, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+                         fold_cfl);
     else
-      hipLaunchKernelGGL((k_hfused2d<4, RS, 1>), g, b, 0, s, Uin, Uout, h, parts, ctrl, fold_cfl);
+      hipLaunchKernelGGL((k_hfused2d<4, RS, 1, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+                         fold_cfl);
   }
+}
+
+// the fused kernels with or without the gravity terms
+template <int RS>
+void launch_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
+                  const double *ctrl, int fold_cfl, hipStream_t s) {
+  if (h.grav)
+    launch_fused_gr<RS, 1>(h, Uin, Uout, parts, ctrl, fold_cfl, s);
+  else
+    launch_fused_gr<RS, 0>(h, Uin, Uout, parts, ctrl, fold_cfl, s);
 }
 
 hipError_t hydro_fused(const HydroDev &h, const double *Uin, double *Uout, double *parts,
