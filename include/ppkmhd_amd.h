@@ -70,7 +70,8 @@ enum ppk_problem_kind { PPK_PROBLEM_ORSZAG_TANG = 0, PPK_PROBLEM_BLAST = 1,
                         PPK_PROBLEM_GRESHO_VORTEX = 5, PPK_PROBLEM_ISENTROPIC_VORTEX = 6,
                         PPK_PROBLEM_FOUR_QUADRANT = 7 /* 2D */,
                         /* with static gravity ([gravity] static=yes) */
-                        PPK_PROBLEM_RAYLEIGH_TAYLOR = 8, PPK_PROBLEM_RISING_BUBBLE = 9 };
+                        PPK_PROBLEM_RAYLEIGH_TAYLOR = 8, PPK_PROBLEM_RISING_BUBBLE = 9,
+                        PPK_PROBLEM_DISK = 10 /* point-mass gravity */ };
 
 /*
  * Solver parameters.  Mirrors the fields of HydroParams / HydroSettings that the MHD MUSCL
@@ -92,10 +93,11 @@ typedef struct ppk_mhd_params {
   double dx, dy, dz;
   double gamma0, gamma6, cfl, slope_type, smallr, smallc, smallp, smallpp, cIso;
   double tEnd;
-  /* hydro solvers: [gravity] static (src/shared/SolverBase.cpp:138-141): 1 adds the gravity
-   * predictor to the face states, the gravity source term after the update and the
-   * gravity-corrected CFL dt; the field is the uniform one the problem sets (0 for problems
-   * that set none).  The MHD solvers ignore it, as the reference's do. */
+  /* hydro solvers: [gravity] static or point (src/shared/SolverBase.cpp:138-141): 1 adds the
+   * gravity predictor to the face states, the gravity source term after the update and the
+   * gravity-corrected CFL dt, with the per-cell field the problem sets (uniform for
+   * rayleigh_taylor / rising_bubble, the point mass for disk, 0 for problems that set none).
+   * The MHD solvers ignore it, as the reference's do. */
   int32_t gravity_static;
   int32_t reserved2;     /* must be 0                                                  */
 } ppk_mhd_params;
@@ -147,6 +149,11 @@ typedef struct ppk_mhd_problem {
   /* rising bubble: radius, centre, densities in / out, uniform gravity
    * (src/shared/problems/RisingBubbleParams.h) */
   double rb_R, rb_x0, rb_y0, rb_z0, rb_din, rb_dout, rb_gx, rb_gy, rb_gz;
+  /* disk around a point mass (src/shared/problems/DiskParams.h) and the softened point-mass
+   * field it sets ([gravity] x, y, z, G*M, soften; src/shared/problems/PointSourceGravity.h) */
+  double disk_radius, disk_radius_inner, disk_xc, disk_yc, disk_zc, disk_ref_density,
+      disk_contrast_density, disk_contrast_width, disk_ref_sound_speed;
+  double pg_x, pg_y, pg_z, pg_gm, pg_eps;
 } ppk_mhd_problem;
 
 typedef struct ppk_mhd_solver ppk_mhd_t;

@@ -287,13 +287,15 @@ struct HydroSolver {
   std::vector<double> U, U2, Q, F[3];
   double t = 0.0, dt = 0.0;
   long long it = 0;
-  // static gravity ([gravity] static or point: SolverBase.cpp:138-141) and its field, uniform
-  // here: the one the problem's initial condition writes, 0 for problems that write none
-  // (the reference's field is zero-initialised)
-  double grav[3] = {0.0, 0.0, 0.0};
+  // gravity ([gravity] static or point: SolverBase.cpp:138-141) and its per-cell field
+  // (VectorField gravity, dim components): the one the problem's initial condition writes, 0
+  // for problems that write none (the reference's view is zero-initialised)
+  std::vector<double> G;
   bool gravity() const { return p.gravity_static != 0; }
+  real gf(int i, int j, int k, int d) const { return G[(long long)d * g.vs + g.at(i, j, k, 0)]; }
   explicit HydroSolver(const ppk_mhd_params &pp) : p(pp), g(pp) {
     U.assign(g.nv * g.vs, 0.0);
+    G.assign(g.dim * g.vs, 0.0);
     U2 = U;
     Q = U;
     for (int d = 0; d < 3; ++d) F[d] = U;
@@ -322,8 +324,9 @@ struct HydroSolver {
             velocity += c + FABS(q[IU]);
             velocity += c + FABS(q[IV]);
             if (g.dim == 3) velocity += c + FABS(q[IW]);
-            real kk = g.dim == 3 ? FABS(real(grav[0])) + FABS(real(grav[1])) + FABS(real(grav[2]))
-                                 : FABS(real(grav[0])) + FABS(real(grav[1]));
+            real kk = g.dim == 3
+                          ? FABS(gf(i, j, k, 0)) + FABS(gf(i, j, k, 1)) + FABS(gf(i, j, k, 2))
+                          : FABS(gf(i, j, k, 0)) + FABS(gf(i, j, k, 1));
             kk = kk * (p.cfl * dxm / (velocity * velocity));
             kk = FMAX(kk, real(1e-4));
             velocity = velocity * (kk / (SQRT(1.0 + 2.0 * kk) - 1.0));
@@ -426,10 +429,11 @@ struct HydroSolver {
             cell_slopes(ni, nj, nk, ndx, ndy, ndz);
             S5 qright = trace(qloc, dx, dy, dz, dir, 0, dtdx, dtdy, dtdz);
             S5 qleft = trace(cell(Q, ni, nj, nk), ndx, ndy, ndz, dir, 1, dtdx, dtdy, dtdz);
-            if (gravity()) {  // gravity predictor, half a time step (HydroRunFunctors2D.h:650-660)
+            if (gravity()) {  // gravity predictor, half a time step (HydroRunFunctors2D.h:650-660):
+                              // each state with its own cell's field
               for (int d = 0; d < g.dim; ++d) {
-                qleft[IU + d] += 0.5 * dt_ * grav[d];
-                qright[IU + d] += 0.5 * dt_ * grav[d];
+                qleft[IU + d] += 0.5 * dt_ * gf(ni, nj, nk, d);
+                qright[IU + d] += 0.5 * dt_ * gf(i, j, k, d);
               }
             }
             const int nrm = IU + dir;  // rotate the normal velocity into IU
@@ -473,7 +477,7 @@ struct HydroSolver {
             const real ekin_old = g.dim == 3
                                       ? 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rhoNew
                                       : 0.5 * (m[0] * m[0] + m[1] * m[1]) / rhoNew;
-            for (int d = 0; d < g.dim; ++d) m[d] += 0.5 * dt_ * grav[d] * (rhoOld + rhoNew);
+            for (int d = 0; d < g.dim; ++d) m[d] += 0.5 * dt_ * gf(i, j, k, d) * (rhoOld + rhoNew);
             for (int d = 0; d < g.dim; ++d) out[g.at(i, j, k, IU + d)] = D(m[d]);
             const real ekin_new = g.dim == 3
                                       ? 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rhoNew
@@ -560,6 +564,35 @@ double kh_rand(uint64_t seed, uint64_t cell, uint64_t draw) {
   return (double)(x >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// the gravity field the problems' initial conditions write: uniform for Rayleigh-Taylor and
+// the rising bubble, PointSourceGravity::eval (src/shared/problems/PointSourceGravity.h)
+// at disk-relative coordinates for the disk (InitDiskFunctor2D/3D call it with x - xc ...),
+// zero for the others
+void gravity_field(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *G) {
+  const Grid g(p);
+  for (int k = 0; k < g.ks; ++k)
+    for (int j = 0; j < g.js; ++j)
+      for (int i = 0; i < g.is; ++i) {
+        double v[3] = {0, 0, 0};
+        if (pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR) {
+          v[0] = pr.rti_gx; v[1] = pr.rti_gy; v[2] = pr.rti_gz;
+        } else if (pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
+          v[0] = pr.rb_gx; v[1] = pr.rb_gy; v[2] = pr.rb_gz;
+        } else if (pr.kind == PPK_PROBLEM_DISK) {
+          const double x = p.xmin + p.dx / 2 + (i - g.gw) * p.dx - pr.disk_xc;
+          const double y = p.ymin + p.dy / 2 + (j - g.gw) * p.dy - pr.disk_yc;
+          const double z = g.dim == 3 ? p.zmin + p.dz / 2 + (k - g.gw) * p.dz - pr.disk_zc : 0.0;
+          const double dx = x - pr.pg_x, dy = y - pr.pg_y, dz = z - pr.pg_z;
+          const double r = std::sqrt(dx * dx + dy * dy + dz * dz + pr.pg_eps * pr.pg_eps);
+          const double GM2 = pr.pg_gm / (r * r * r);
+          v[0] = GM2 * (pr.pg_x - x);
+          v[1] = GM2 * (pr.pg_y - y);
+          v[2] = GM2 * (pr.pg_z - z);
+        }
+        for (int d = 0; d < g.dim; ++d) G[(long long)d * g.vs + g.at(i, j, k, 0)] = v[d];
+      }
+}
+
 void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
   const Grid g(p);
   std::fill(U, U + g.nv * g.vs, 0.0);
@@ -573,7 +606,8 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
         double d, e, u, v, w;
         if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || pr.kind == PPK_PROBLEM_GRESHO_VORTEX ||
             pr.kind == PPK_PROBLEM_ISENTROPIC_VORTEX || pr.kind == PPK_PROBLEM_FOUR_QUADRANT ||
-            pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
+            pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || pr.kind == PPK_PROBLEM_RISING_BUBBLE ||
+            pr.kind == PPK_PROBLEM_DISK) {
           // these store momenta and total energy directly
           double st[5] = {0, 0, 0, 0, 0};
           if (pr.kind == PPK_PROBLEM_KELVIN_HELMHOLTZ) {  // InitKelvinHelmholtzFunctor2D/3D
@@ -650,6 +684,50 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
               st[IP] = (P0 + st[ID] * (pr.rti_gx * x + pr.rti_gy * y + pr.rti_gz * z)) /
                        (p.gamma0 - 1);
             }
+          } else if (pr.kind == PPK_PROBLEM_DISK) {
+            // InitDiskFunctor2D/3D (HydroInitFunctors2D.h:907-1030, 3D.h:807-935) with
+            // DiskParams::radial_speed_of_sound
+            const double xr = x - pr.disk_xc, yr = y - pr.disk_yc;
+            const double zr = g.dim == 3 ? z - pr.disk_zc : 0.0;
+            const double GM = pr.pg_gm, eps = pr.pg_eps;
+            const double r2 = xr * xr + yr * yr;
+            const double r = std::sqrt(r2);
+            const double r_soft = std::sqrt(r2 + eps * eps);
+            const double r_sph = std::sqrt(r2 + eps * eps + zr * zr);
+            const double x_soft = xr * (r_soft / r);
+            const double y_soft = yr * (r_soft / r);
+            double csound;
+            if (r < pr.disk_radius_inner)
+              csound = pr.disk_ref_sound_speed / std::sqrt(pr.disk_radius_inner / pr.disk_radius);
+            else
+              csound = pr.disk_ref_sound_speed / std::sqrt(r / pr.disk_radius);
+            double omega2;
+            if (r >= pr.disk_radius_inner)
+              omega2 = GM / (r_soft * r_soft * r_sph) -
+                       3.5 * (csound / r_soft) * (csound / r_soft);
+            else
+              omega2 = GM / (r_soft * r_soft * r_soft) -
+                       2.5 * (csound / r_soft) * (csound / r_soft);
+            const double omega = std::sqrt(std::fmax(omega2, 0.0));
+            double rho = pr.disk_ref_density * std::pow(r_soft / pr.disk_radius, -2.5) *
+                         std::exp(GM / (csound * csound) * (1. / r_sph - 1. / r_soft));
+            if (r > pr.disk_radius || std::fabs(zr) > 0.5 * pr.disk_radius) {
+              if (r > pr.disk_radius + pr.disk_contrast_width ||
+                  std::fabs(zr) > 0.5 * pr.disk_radius + pr.disk_contrast_width) {
+                rho /= pr.disk_contrast_density;
+              } else {
+                const double cutdown =
+                    std::fmax((r - pr.disk_radius) / pr.disk_contrast_width,
+                              (std::fabs(zr) - 0.5 * pr.disk_radius) / pr.disk_contrast_width);
+                rho *= std::pow(pr.disk_contrast_density, -cutdown);
+              }
+            }
+            rho = std::fmax(rho, 1e-6);
+            st[ID] = rho;
+            st[IU] = -y_soft * omega * rho;
+            st[IV] = x_soft * omega * rho;
+            const double pressure = csound * csound * rho;
+            st[IP] = pressure / (p.gamma0 - 1.0) + 0.5 * rho * omega2 * (r2 + eps * eps);
           } else if (pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
             // RisingBubbleFunctor2D/3D (HydroInitFunctors2D.h:840-900, 3D.h:715-800): a
             // sphere of density din in dout at rest, hydrostatic pressure P0 + rho g.x
@@ -739,10 +817,7 @@ hora_solver *hora_create(const ppk_mhd_params *p) { return new hora_solver(*p); 
 void hora_destroy(hora_solver *s) { delete s; }
 void hora_init_problem(hora_solver *s, const ppk_mhd_problem *pr) {
   init(s->s.p, *pr, s->s.U.data());
-  const bool rti = pr->kind == PPK_PROBLEM_RAYLEIGH_TAYLOR, rb = pr->kind == PPK_PROBLEM_RISING_BUBBLE;
-  s->s.grav[0] = rti ? pr->rti_gx : rb ? pr->rb_gx : 0.0;
-  s->s.grav[1] = rti ? pr->rti_gy : rb ? pr->rb_gy : 0.0;
-  s->s.grav[2] = s->s.p.dim == 3 ? (rti ? pr->rti_gz : rb ? pr->rb_gz : 0.0) : 0.0;
+  gravity_field(s->s.p, *pr, s->s.G.data());
   boundaries(s->s.p, s->s.U.data());
   s->s.U2 = s->s.U;
   s->s.t = 0.0;
@@ -750,6 +825,9 @@ void hora_init_problem(hora_solver *s, const ppk_mhd_problem *pr) {
 }
 void hora_init_array(const ppk_mhd_params *p, const ppk_mhd_problem *pr, double *U) {
   init(*p, *pr, U);
+}
+void hora_gravity_field(const ppk_mhd_params *p, const ppk_mhd_problem *pr, double *G) {
+  gravity_field(*p, *pr, G);
 }
 void hora_set_state(hora_solver *s, const double *U) {
   std::vector<double> &cur = (s->s.it % 2 == 0) ? s->s.U : s->s.U2;

@@ -44,6 +44,7 @@ def _lib(counting=False):
     lib.hora_destroy.argtypes = [C.c_void_p]
     lib.hora_init_problem.argtypes = [C.c_void_p, PR]
     lib.hora_init_array.argtypes = [P, PR, dp]
+    lib.hora_gravity_field.argtypes = [P, PR, dp]
     lib.hora_set_state.argtypes = [C.c_void_p, dp]
     lib.hora_get_state.argtypes = [C.c_void_p, dp]
     lib.hora_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
@@ -102,6 +103,14 @@ def init_array(p, prob):
     U = zeros(p)
     _lib().hora_init_array(C.byref(p), C.byref(prob), _dp(U))
     return U
+
+
+def gravity_field(p, prob):
+    """The gravity field the problem's initial condition writes, shape (dim, ksize, jsize,
+    isize)."""
+    G = np.zeros((p.dim,) + shape(p)[1:], dtype=np.float64)
+    _lib().hora_gravity_field(C.byref(p), C.byref(prob), _dp(G))
+    return G
 
 
 def fill_boundaries(p, U):

@@ -17,7 +17,7 @@ PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
 DT_FIXED, DT_CFL = 0, 1
 (PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE,
  PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX, PROBLEM_FOUR_QUADRANT, PROBLEM_RAYLEIGH_TAYLOR,
- PROBLEM_RISING_BUBBLE) = range(10)
+ PROBLEM_RISING_BUBBLE, PROBLEM_DISK) = range(11)
 
 PPK_OK = 0
 PPK_ERR_INVALID_ARGUMENT, PPK_ERR_HIP, PPK_ERR_OUT_OF_MEMORY, PPK_ERR_UNSUPPORTED, PPK_ERR_CONFIG = 1, 2, 3, 4, 5
@@ -177,4 +177,18 @@ class Problem(C.Structure):
         ("rb_gx", C.c_double),
         ("rb_gy", C.c_double),
         ("rb_gz", C.c_double),
+        ("disk_radius", C.c_double),
+        ("disk_radius_inner", C.c_double),
+        ("disk_xc", C.c_double),
+        ("disk_yc", C.c_double),
+        ("disk_zc", C.c_double),
+        ("disk_ref_density", C.c_double),
+        ("disk_contrast_density", C.c_double),
+        ("disk_contrast_width", C.c_double),
+        ("disk_ref_sound_speed", C.c_double),
+        ("pg_x", C.c_double),
+        ("pg_y", C.c_double),
+        ("pg_z", C.c_double),
+        ("pg_gm", C.c_double),
+        ("pg_eps", C.c_double),
     ]

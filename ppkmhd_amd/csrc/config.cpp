@@ -196,7 +196,7 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
       R.kind = P.dim == 2 ? PPK_PROBLEM_FOUR_QUADRANT : PPK_PROBLEM_IMPLODE;
     else if (P.ghost_width == 2 && prob == "rayleigh_taylor") R.kind = PPK_PROBLEM_RAYLEIGH_TAYLOR;
     else if (P.ghost_width == 2 && prob == "rising_bubble") R.kind = PPK_PROBLEM_RISING_BUBBLE;
-    else if (P.ghost_width == 2 && prob == "disk") R.kind = -1;
+    else if (P.ghost_width == 2 && prob == "disk") R.kind = PPK_PROBLEM_DISK;
     // unknown names: the MHD solver falls back to Orszag-Tang (SolverMHDMuscl.h:701-709),
     // the hydro solver to implode (SolverHydroMuscl.cpp:170-177)
     else R.kind = P.ghost_width == 2 ? PPK_PROBLEM_IMPLODE : PPK_PROBLEM_ORSZAG_TANG;
@@ -290,6 +290,28 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     R.rb_gx = m.get_float("rising_bubble", "gx", 0.0f);
     R.rb_gy = m.get_float("rising_bubble", "gy", -0.1f);
     R.rb_gz = m.get_float("rising_bubble", "gz", 0.0f);
+    // DiskParams: its own mesh defaults (-0.5 .. 0.5), computed in double, passed as floats
+    {
+      const double dxmin = m.get_float("mesh", "xmin", -0.5f), dymin = m.get_float("mesh", "ymin", -0.5f);
+      const double dzmin = m.get_float("mesh", "zmin", -0.5f), dxmax = m.get_float("mesh", "xmax", 0.5f);
+      const double dymax = m.get_float("mesh", "ymax", 0.5f), dzmax = m.get_float("mesh", "zmax", 0.5f);
+      R.disk_radius = m.get_float("disk", "radius", (float)((dxmax - dxmin) * 0.125));
+      R.disk_radius_inner = m.get_float("disk", "radius_inner", (float)(R.disk_radius / 10));
+      R.disk_xc = m.get_float("disk", "xc", (float)((dxmin + dxmax) / 2));
+      R.disk_yc = m.get_float("disk", "yc", (float)((dymin + dymax) / 2));
+      R.disk_zc = m.get_float("disk", "zc", (float)((dzmin + dzmax) / 2));
+      R.disk_ref_density = m.get_float("disk", "ref_density", 1.0f);
+      R.disk_contrast_density = m.get_float("disk", "contrast_density", 100.0f);
+      R.disk_contrast_width = m.get_float("disk", "contrast_width", 0.01f);
+      R.disk_ref_sound_speed = m.get_float("disk", "ref_sound_speed", 0.2f);
+      // PointSourceGravity: GM = G * M of the two float values
+      R.pg_x = m.get_float("gravity", "x", 0.0f);
+      R.pg_y = m.get_float("gravity", "y", 0.0f);
+      R.pg_z = m.get_float("gravity", "z", 0.0f);
+      const double G = m.get_float("gravity", "G", 1.0f), M = m.get_float("gravity", "M", 1.0f);
+      R.pg_gm = G * M;
+      R.pg_eps = m.get_float("gravity", "soften", 0.01f);
+    }
     // four quadrant (SolverHydroMuscl.cpp:75-77)
     R.fq_config = (int)m.get_integer("riemann2d", "config_number", 0);
     R.fq_x = m.get_float("riemann2d", "x", 0.8f);

@@ -59,14 +59,31 @@ __device__ __forceinline__ bool gravity_on(const HydroDev &h) {
   return GR == 1 || (GR < 0 && h.grav);
 }
 
-// gravity predictor of a face state, half a time step (HydroRunFunctors2D.h:650-660,
-// 3D.h:485-497): v += (0.5 dt) g, before the normal is rotated into IU
+// the gravity field of padded-box cell c (zero without gravity, nothing loaded)
+struct G3 {
+  double x, y, z;
+};
 template <int GR, int NV>
-__device__ __forceinline__ void gravity_predict(HSt<NV> &q, const HydroDev &h, double hdt) {
+__device__ __forceinline__ G3 cell_g(const HydroDev &h, long long c) {
+  G3 g{0.0, 0.0, 0.0};
+  if (gravity_on<GR>(h)) {
+    g.x = h.gf[c];
+    g.y = h.gf[c + h.vstride];
+    if constexpr (NV == 5) g.z = h.gf[c + 2 * h.vstride];
+  }
+  return g;
+}
+
+// gravity predictor of a face state of a cell with field g, half a time step
+// (HydroRunFunctors2D.h:650-660, 3D.h:485-497): v += (0.5 dt) g, before the normal is
+// rotated into IU
+template <int GR, int NV>
+__device__ __forceinline__ void gravity_predict(HSt<NV> &q, const G3 &g, const HydroDev &h,
+                                                double hdt) {
   if (!gravity_on<GR>(h)) return;
-  q[IU] += hdt * h.g[0];
-  q[IV] += hdt * h.g[1];
-  if constexpr (NV == 5) q[IW] += hdt * h.g[2];
+  q[IU] += hdt * g.x;
+  q[IV] += hdt * g.y;
+  if constexpr (NV == 5) q[IW] += hdt * g.z;
 }
 
 // GravitySourceTermFunctor2D/3D (HydroRunFunctors2D.h:1299-1331, 3D.h:1463-1500) on an updated
@@ -74,17 +91,17 @@ __device__ __forceinline__ void gravity_predict(HSt<NV> &q, const HydroDev &h, d
 // energy += the change of kinetic energy.  The 2D functor floors the new density at smallr,
 // the 3D one does not.
 template <int GR, int NV>
-__device__ __forceinline__ void gravity_source(HSt<NV> &o, double rho_old, const HydroDev &h,
-                                               double hdt) {
+__device__ __forceinline__ void gravity_source(HSt<NV> &o, double rho_old, const G3 &g,
+                                               const HydroDev &h, double hdt) {
   if (!gravity_on<GR>(h)) return;
   const double rho_new = NV == 4 ? fmax(h.P.smallr, o[ID]) : o[ID];
   const double ir = recip(rho_new);
   double mu = o[IU], mv = o[IV], mw = NV == 5 ? o[IW] : 0.0;
   const double ek_old = NV == 5 ? divr(0.5 * (mu * mu + mv * mv + mw * mw), rho_new, ir)
                                 : divr(0.5 * (mu * mu + mv * mv), rho_new, ir);
-  mu += hdt * h.g[0] * (rho_old + rho_new);
-  mv += hdt * h.g[1] * (rho_old + rho_new);
-  if constexpr (NV == 5) mw += hdt * h.g[2] * (rho_old + rho_new);
+  mu += hdt * g.x * (rho_old + rho_new);
+  mv += hdt * g.y * (rho_old + rho_new);
+  if constexpr (NV == 5) mw += hdt * g.z * (rho_old + rho_new);
   o[IU] = mu;
   o[IV] = mv;
   if constexpr (NV == 5) o[IW] = mw;
@@ -98,13 +115,14 @@ __device__ __forceinline__ void gravity_source(HSt<NV> &o, double rho_old, const
 // summed speeds u scaled by k / (sqrt(1 + 2k) - 1), k = |g|_1 cfl dx / u^2 floored at 1e-4,
 // which makes g dt^2 / (2 dx) + u dt / dx = cfl, over the smallest cell size
 template <int GR, int NV>
-__device__ __forceinline__ double inv_dt_prim(const HSt<NV> &q, double c, const HydroDev &h) {
+__device__ __forceinline__ double inv_dt_prim(const HSt<NV> &q, double c, const G3 &g,
+                                              const HydroDev &h) {
   if (gravity_on<GR>(h)) {
     double vel = 0.0;
     vel += c + fabs(q[IU]);
     vel += c + fabs(q[IV]);
     if constexpr (NV == 5) vel += c + fabs(q[IW]);
-    double k = NV == 5 ? fabs(h.g[0]) + fabs(h.g[1]) + fabs(h.g[2]) : fabs(h.g[0]) + fabs(h.g[1]);
+    double k = NV == 5 ? fabs(g.x) + fabs(g.y) + fabs(g.z) : fabs(g.x) + fabs(g.y);
     const double v2 = vel * vel;
     k = k * divr(h.cfl * h.dxmin, v2, recip(v2));
     k = fmax(k, 1e-4);
@@ -122,11 +140,11 @@ __device__ __forceinline__ double inv_dt_prim(const HSt<NV> &q, double c, const 
   }
 }
 template <int GR, int NV>
-__device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const HydroDev &h) {
+__device__ __forceinline__ double inv_dt_cell(const HSt<NV> &u, const G3 &g, const HydroDev &h) {
   HSt<NV> q;
   double c;
   hyd::prims(u, h.P, c, q);
-  return inv_dt_prim<GR>(q, c, h);
+  return inv_dt_prim<GR>(q, c, g, h);
 }
 
 // ---- ghost fill, one direction, both faces (MakeBoundariesFunctor2D/3D,
@@ -203,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_hprim(const double *__restrict__ U,
       const int g = h.gw;
       const bool inner = i >= g && i < h.isize - g && j >= g && j < h.jsize - g &&
                          (NV == 4 || (k >= g && k < h.ksize - g));
-      if (inner) best = fmax(best, inv_dt_prim<-1>(q, c, h));
+      if (inner) best = fmax(best, inv_dt_prim<-1>(q, c, cell_g<-1, NV>(h, t), h));
     }
   }
   if (!do_dt) return;
@@ -264,8 +282,8 @@ __global__ void __launch_bounds__(kBlock) k_hflux(const double *__restrict__ Q,
       HSt<NV> qr, ql, f;
       hyd::face_state(q, d == 0 ? dX : (d == 1 ? dY : dZ), s0, dtd[d], false, h.P.smallr, qr);
       hyd::face_state(qn, d == 0 ? nX : (d == 1 ? nY : nZ), ns0, dtd[d], true, h.P.smallr, ql);
-      gravity_predict<-1>(ql, h, hdt);
-      gravity_predict<-1>(qr, h, hdt);
+      gravity_predict<-1>(ql, cell_g<-1, NV>(h, c - st), h, hdt);
+      gravity_predict<-1>(qr, cell_g<-1, NV>(h, c), h, hdt);
       swap_normal(ql, d);
       swap_normal(qr, d);
       hyd::riemann_rs<RS>(ql, qr, f, h.P);
@@ -311,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) k_hupdate(double *__restrict__ U,
       }
       on[v] = o;
     }
-    gravity_source<-1>(on, rho_old, h, hdt);
+    gravity_source<-1>(on, rho_old, cell_g<-1, NV>(h, c), h, hdt);
 #pragma unroll
     for (int v = 0; v < NV; ++v) U[c + v * vs] = on[v];
   }
@@ -354,6 +372,8 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
   hyd::prims(t, P, c, qm);
   load(Uin, vs, (long long)(r0 - 1) * h.isize + ic, u0);
   hyd::prims(u0, P, c, q0);
+  // gravity field of the row the face states are built for, and of the row being finished
+  G3 g0 = cell_g<GR, NV>(h, (long long)(r0 - 1) * h.isize + ic), g_prev = g0;
   HSt<NV> ymax_prev, o_prev;
   double best = 0.0;
 #pragma unroll 1
@@ -361,6 +381,7 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
     HSt<NV> un, qp;
     load(Uin, vs, (long long)(s + 1) * h.isize + ic, un);
     hyd::prims(un, P, c, qp);
+    const G3 gn = cell_g<GR, NV>(h, (long long)(s + 1) * h.isize + ic);
     HSt<NV> qxm, qxp, dX, dY, dZ, s0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -372,7 +393,7 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
     if (s >= r0) {
       HSt<NV> ql = ymax_prev, qr, fy;
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
-      gravity_predict<GR>(qr, h, hdt);
+      gravity_predict<GR>(qr, g0, h, hdt);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
       hyd::riemann_rs<RS>(ql, qr, fy, P);
@@ -383,17 +404,17 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
         const long long cp = (long long)(s - 1) * h.isize + i;
 #pragma unroll
         for (int v = 0; v < NV; ++v) o_prev[v] -= fy[v];
-        gravity_source<GR>(o_prev, rho_prev, h, hdt);
+        gravity_source<GR>(o_prev, rho_prev, g_prev, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) Uout[cp + v * vs] = o_prev[v];
-        if (fold_cfl) best = fmax(best, inv_dt_cell<GR>(o_prev, h));
+        if (fold_cfl) best = fmax(best, inv_dt_cell<GR>(o_prev, g_prev, h));
       }
       if (s < r1) {
         HSt<NV> xmin, xmax, xl, fx;
         hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
         hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
-        gravity_predict<GR>(xmin, h, hdt);
-        gravity_predict<GR>(xmax, h, hdt);
+        gravity_predict<GR>(xmin, g0, h, hdt);
+        gravity_predict<GR>(xmax, g0, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
         hyd::riemann_rs<RS>(xl, xmin, fx, P);
@@ -408,13 +429,15 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
           o_prev[v] = o;
         }
         rho_prev = u0[ID];
+        g_prev = g0;
       }
     }
     hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax_prev);
-    gravity_predict<GR>(ymax_prev, h, hdt);
+    gravity_predict<GR>(ymax_prev, g0, h, hdt);
     qm = q0;
     q0 = qp;
     u0 = un;
+    g0 = gn;
   }
   if (fold_cfl) {
     for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off, 64));
@@ -472,6 +495,8 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
   hyd::prims(t, P, c, qm);
   load(Uin, vs, (long long)(k0 - 1) * h.plane + col, u0);
   hyd::prims(u0, P, c, q0);
+  // gravity field of the plane the face states are built for, and of the plane being finished
+  G3 g0 = cell_g<GR, NV>(h, (long long)(k0 - 1) * h.plane + col), g_prev = g0;
   HSt<NV> zmax_prev, o_prev;
   double best = 0.0;
 #pragma unroll 1
@@ -480,6 +505,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
     HSt<NV> un, qp;
     load(Uin, vs, (long long)(s + 1) * h.plane + col, un);
     hyd::prims(un, P, c, qp);
+    const G3 gn = cell_g<GR, NV>(h, (long long)(s + 1) * h.plane + col);
 #pragma unroll
     for (int v = 0; v < NV; ++v) sQ[b][v][ly][lane] = q0[v];
     __syncthreads();
@@ -497,7 +523,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       hyd::sources(q0, dX, dY, dZ, P.gamma0, s0);
       HSt<NV> ymax;
       hyd::face_state(q0, dY, s0, dtdy, true, P.smallr, ymax);
-      gravity_predict<GR>(ymax, h, hdt);
+      gravity_predict<GR>(ymax, g0, h, hdt);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sT[v][ly][lane] = ymax[v];
     }
@@ -507,7 +533,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
     if (s >= k0 && out_row) {  // z face below plane s (also the top face of plane k1 - 1)
       HSt<NV> ql = zmax_prev, qr;
       hyd::face_state(q0, dZ, s0, dtdz, false, P.smallr, qr);
-      gravity_predict<GR>(qr, h, hdt);
+      gravity_predict<GR>(qr, g0, h, hdt);
       swap_normal(ql, 2);
       swap_normal(qr, 2);
       hyd::riemann_rs<RS>(ql, qr, fz, P);
@@ -520,7 +546,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
 #pragma unroll
       for (int v = 0; v < NV; ++v) ql[v] = sT[v][ly - 1][lane];
       hyd::face_state(q0, dY, s0, dtdy, false, P.smallr, qr);
-      gravity_predict<GR>(qr, h, hdt);
+      gravity_predict<GR>(qr, g0, h, hdt);
       swap_normal(ql, 1);
       swap_normal(qr, 1);
       hyd::riemann_rs<RS>(ql, qr, fy, P);
@@ -532,8 +558,8 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       HSt<NV> xmin, xmax, xl;
       hyd::face_state(q0, dX, s0, dtdx, false, P.smallr, xmin);
       hyd::face_state(q0, dX, s0, dtdx, true, P.smallr, xmax);
-      gravity_predict<GR>(xmin, h, hdt);
-      gravity_predict<GR>(xmax, h, hdt);
+      gravity_predict<GR>(xmin, g0, h, hdt);
+      gravity_predict<GR>(xmax, g0, h, hdt);
 #pragma unroll
       for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
       hyd::riemann_rs<RS>(xl, xmin, fx, P);
@@ -546,10 +572,10 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
         const long long cp = (long long)(s - 1) * h.plane + (long long)j * h.isize + i;
 #pragma unroll
         for (int v = 0; v < NV; ++v) o_prev[v] -= fz[v];
-        gravity_source<GR>(o_prev, rho_prev, h, hdt);
+        gravity_source<GR>(o_prev, rho_prev, g_prev, h, hdt);
 #pragma unroll
         for (int v = 0; v < NV; ++v) Uout[cp + v * vs] = o_prev[v];
-        if (fold_cfl) best = fmax(best, inv_dt_cell<GR>(o_prev, h));
+        if (fold_cfl) best = fmax(best, inv_dt_cell<GR>(o_prev, g_prev, h));
       }
       if (plane_out) {
 #pragma unroll
@@ -564,15 +590,17 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
           o_prev[v] = o;
         }
         rho_prev = u0[ID];
+        g_prev = g0;
       }
     }
     if (slope_row) {
       hyd::face_state(q0, dZ, s0, dtdz, true, P.smallr, zmax_prev);
-      gravity_predict<GR>(zmax_prev, h, hdt);
+      gravity_predict<GR>(zmax_prev, g0, h, hdt);
     }
     qm = q0;
     q0 = qp;
     u0 = un;
+    g0 = gn;
   }
   if (fold_cfl) {
     best = block_max(best, red);

@@ -114,9 +114,10 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   d.P.slope_type = p.slope_type;
   d.P.riemann = p.riemann;
   d.P.entho = 1.0 / (p.gamma0 - 1.0);  // RiemannSolvers.h: entho = ONE_F / (gamma0 - ONE_F)
-  // static gravity; the field is set by the problem (zero until then, as the reference's)
+  // gravity; the field (allocated by the handle) is set by the problem, zero until then as
+  // the reference's
   d.grav = p.gravity_static ? 1 : 0;
-  d.g[0] = d.g[1] = d.g[2] = 0.0;
+  d.gf = nullptr;
   d.cfl = p.cfl;
   d.dxmin = p.dim == 3 ? std::fmin(std::fmin(p.dx, p.dy), p.dz) : std::fmin(p.dx, p.dy);
   d.idxmin = 1.0 / d.dxmin;
@@ -322,6 +323,41 @@ void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
           s[IP] = dim == 2 ? (P0 + s[ID] * (pr.rb_gx * x + pr.rb_gy * y)) / (g0 - 1.0)
                            : (P0 + s[ID] * (pr.rb_gx * x + pr.rb_gy * y + pr.rb_gz * z)) /
                                  (g0 - 1.0);
+        } else if (pr.kind == PPK_PROBLEM_DISK) {
+          // a rotating disk around a softened point mass GM, in coordinates relative to the
+          // disk centre (xc, yc, zc): power-law density with a vertical exponential profile,
+          // divided by the contrast outside radius (with a ramp over contrast_width), sound
+          // speed c(r) = c_ref / sqrt(r / radius) (r_inner inside), rotation from the radial
+          // balance, pressure c^2 rho; the energy adds 0.5 rho omega2 (r^2 + eps^2)
+          const double xr = x - pr.disk_xc, yr = y - pr.disk_yc;
+          const double zr = dim == 3 ? z - pr.disk_zc : 0.0;
+          const double GM = pr.pg_gm, eps = pr.pg_eps, R = pr.disk_radius;
+          const double Ri = pr.disk_radius_inner, cw = pr.disk_contrast_width;
+          const double r2 = xr * xr + yr * yr, r = std::sqrt(r2);
+          const double r_soft = std::sqrt(r2 + eps * eps);
+          const double r_sph = std::sqrt(r2 + eps * eps + zr * zr);
+          const double x_soft = xr * (r_soft / r), y_soft = yr * (r_soft / r);
+          const double cs = r < Ri ? pr.disk_ref_sound_speed / std::sqrt(Ri / R)
+                                   : pr.disk_ref_sound_speed / std::sqrt(r / R);
+          const double omega2 =
+              r >= Ri ? GM / (r_soft * r_soft * r_sph) - 3.5 * (cs / r_soft) * (cs / r_soft)
+                      : GM / (r_soft * r_soft * r_soft) - 2.5 * (cs / r_soft) * (cs / r_soft);
+          const double omega = std::sqrt(std::fmax(omega2, 0.0));
+          double rho = pr.disk_ref_density * std::pow(r_soft / R, -2.5) *
+                       std::exp(GM / (cs * cs) * (1. / r_sph - 1. / r_soft));
+          if (r > R || std::fabs(zr) > 0.5 * R) {
+            if (r > R + cw || std::fabs(zr) > 0.5 * R + cw)
+              rho /= pr.disk_contrast_density;
+            else
+              rho *= std::pow(pr.disk_contrast_density,
+                              -std::fmax((r - R) / cw, (std::fabs(zr) - 0.5 * R) / cw));
+          }
+          rho = std::fmax(rho, 1e-6);
+          s[ID] = rho;
+          s[IU] = -y_soft * omega * rho;
+          s[IV] = x_soft * omega * rho;
+          s[IW] = 0.0;
+          s[IP] = cs * cs * rho / (g0 - 1.0) + 0.5 * rho * omega2 * (r2 + eps * eps);
         } else if (pr.kind == PPK_PROBLEM_FOUR_QUADRANT) {  // 2D
           const int quad = x < pr.fq_x ? (y < pr.fq_y ? 2 : 1) : (y < pr.fq_y ? 3 : 0);
           const double *q = kLaxLiu[lax_liu_row(pr.fq_config)][quad];  // rho, u, v, p
@@ -360,6 +396,38 @@ void hydro_ics(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
       }
 }
 
+// Gravity field of a problem, dim planes of the padded box: the uniform field of
+// rayleigh_taylor / rising_bubble, the softened point mass of disk
+// (PointSourceGravity::eval, src/shared/problems/PointSourceGravity.h, evaluated by the disk
+// functor at coordinates relative to the disk centre), zero otherwise.
+void hydro_gravity_field(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *G) {
+  const int gw = p.ghost_width, dim = p.dim;
+  const int is = p.nx + 2 * gw, js = p.ny + 2 * gw, ks = dim == 3 ? p.nz + 2 * gw : 1;
+  const long long vs = (long long)is * js * ks;
+  for (int k = 0; k < ks; ++k)
+    for (int j = 0; j < js; ++j)
+      for (int i = 0; i < is; ++i) {
+        const long long c = ((long long)k * js + j) * is + i;
+        double g[3] = {0.0, 0.0, 0.0};
+        if (pr.kind == PPK_PROBLEM_RAYLEIGH_TAYLOR) {
+          g[0] = pr.rti_gx; g[1] = pr.rti_gy; g[2] = pr.rti_gz;
+        } else if (pr.kind == PPK_PROBLEM_RISING_BUBBLE) {
+          g[0] = pr.rb_gx; g[1] = pr.rb_gy; g[2] = pr.rb_gz;
+        } else if (pr.kind == PPK_PROBLEM_DISK) {
+          const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx - pr.disk_xc;
+          const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy - pr.disk_yc;
+          const double z = dim == 3 ? p.zmin + p.dz / 2 + (k - gw) * p.dz - pr.disk_zc : 0.0;
+          const double r = std::sqrt((x - pr.pg_x) * (x - pr.pg_x) + (y - pr.pg_y) * (y - pr.pg_y) +
+                                     (z - pr.pg_z) * (z - pr.pg_z) + pr.pg_eps * pr.pg_eps);
+          const double GM2 = pr.pg_gm / (r * r * r);
+          g[0] = GM2 * (pr.pg_x - x);
+          g[1] = GM2 * (pr.pg_y - y);
+          g[2] = GM2 * (pr.pg_z - z);
+        }
+        for (int d = 0; d < dim; ++d) G[c + d * vs] = g[d];
+      }
+}
+
 }  // namespace
 }  // namespace ppk
 
@@ -370,6 +438,7 @@ struct ppk_hydro_solver {
   double *U[2] = {nullptr, nullptr};
   double *Q = nullptr;  // staged form: primitives
   double *F = nullptr;  // staged form: dim x nv flux planes
+  double *G = nullptr;  // gravity: dim planes of the per-cell field
   double *ctrl = nullptr;
   double *ctrl_host = nullptr;
   double *parts = nullptr;
@@ -422,7 +491,7 @@ int hread_ctrl(ppk_hydro_t *h) {
 
 void hfree_all(ppk_hydro_t *h) {
   for (void *ptr : {(void *)h->U[0], (void *)h->U[1], (void *)h->Q, (void *)h->F,
-                    (void *)h->ctrl, (void *)h->parts})
+                    (void *)h->G, (void *)h->ctrl, (void *)h->parts})
     if (ptr) (void)hipFree(ptr);
   delete h->timer;
   h->timer = nullptr;
@@ -489,19 +558,21 @@ bool hydro_problem_ok(int kind, int dim) {
   return kind == PPK_PROBLEM_IMPLODE || kind == PPK_PROBLEM_BLAST ||
          kind == PPK_PROBLEM_KELVIN_HELMHOLTZ || kind == PPK_PROBLEM_GRESHO_VORTEX ||
          kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || kind == PPK_PROBLEM_RISING_BUBBLE ||
+         kind == PPK_PROBLEM_DISK ||
          ((kind == PPK_PROBLEM_ISENTROPIC_VORTEX || kind == PPK_PROBLEM_FOUR_QUADRANT) &&
           dim == 2);
 }
 const char *const kHydroProblems =
     "hydro problems: implode, blast, kelvin_helmholtz, gresho_vortex, isentropic_vortex (2D), "
-    "four_quadrant (2D), rayleigh_taylor, rising_bubble";
+    "four_quadrant (2D), rayleigh_taylor, rising_bubble, disk";
 // the reference writes the gravity field of these problems into a view that exists only when
 // gravity is enabled
 bool hydro_problem_needs_gravity(int kind) {
-  return kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || kind == PPK_PROBLEM_RISING_BUBBLE;
+  return kind == PPK_PROBLEM_RAYLEIGH_TAYLOR || kind == PPK_PROBLEM_RISING_BUBBLE ||
+         kind == PPK_PROBLEM_DISK;
 }
 const char *const kNeedsGravity =
-    "rayleigh_taylor and rising_bubble need static gravity ([gravity] static=yes)";
+    "rayleigh_taylor, rising_bubble and disk need gravity ([gravity] static=yes or point=yes)";
 
 }  // namespace
 
@@ -545,7 +616,12 @@ int ppk_hydro_create(const ppk_mhd_params *params, ppk_hydro_t **handle) {
   bool ok = dalloc(&h->U[0], nvals) && dalloc(&h->U[1], nvals) &&
             dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->parts, nparts);
   if (ok && !h->fused) ok = dalloc(&h->Q, nvals) && dalloc(&h->F, (size_t)params->dim * nvals);
+  if (ok && params->gravity_static) ok = dalloc(&h->G, (size_t)params->dim * h->cells());
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
+  h->hd.gf = h->G;
+  if (h->G && hipMemsetAsync(h->G, 0, (size_t)params->dim * h->cells() * sizeof(double),
+                             h->stream) != hipSuccess)
+    return bail(PPK_ERR_HIP, "hipMemsetAsync failed");
   if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * ppk::CTRL_SLOTS) != hipSuccess)
     return bail(PPK_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
   if (hipMemsetAsync(h->U[0], 0, nvals * sizeof(double), h->stream) != hipSuccess ||
@@ -598,12 +674,13 @@ int ppk_hydro_init_problem(ppk_hydro_t *h, const ppk_mhd_problem *problem) {
     return hfail(h, PPK_ERR_INVALID_ARGUMENT, kNeedsGravity);
   std::vector<double> host(h->values());
   ppk::hydro_ics(h->p, *problem, host.data());
-  // the problem's uniform gravity field (zero for the problems that set none)
-  const bool rti = problem->kind == PPK_PROBLEM_RAYLEIGH_TAYLOR;
-  const bool rb = problem->kind == PPK_PROBLEM_RISING_BUBBLE;
-  h->hd.g[0] = rti ? problem->rti_gx : rb ? problem->rb_gx : 0.0;
-  h->hd.g[1] = rti ? problem->rti_gy : rb ? problem->rb_gy : 0.0;
-  h->hd.g[2] = h->p.dim == 3 ? (rti ? problem->rti_gz : rb ? problem->rb_gz : 0.0) : 0.0;
+  if (h->G) {  // the problem's gravity field (zero for the problems that set none)
+    std::vector<double> field((size_t)h->p.dim * h->cells());
+    ppk::hydro_gravity_field(h->p, *problem, field.data());
+    HHIP(h, hipMemcpyAsync(h->G, field.data(), field.size() * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream));
+    HHIP(h, hipStreamSynchronize(h->stream));
+  }
   h->iter_host = 0;
   h->cfl_valid = false;
   HHIP(h, hipMemcpyAsync(h->cur(), host.data(), h->state_bytes(), hipMemcpyHostToDevice,

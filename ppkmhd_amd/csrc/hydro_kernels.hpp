@@ -20,10 +20,11 @@ struct HydroDev {
   double dx, dy, dz;
   double idx, idy, idz;  // 1 / dx ... (used only where the build shares reciprocals)
   hyd::HydroPar P;
-  // static gravity: grav != 0 adds the half-step predictor to every face state, the source
-  // term to every updated cell and the gravity form of the CFL term; g is the uniform field
+  // gravity: grav != 0 adds the half-step predictor to every face state, the source term to
+  // every updated cell and the gravity form of the CFL term, each with its own cell's field
+  // gf (dim planes, vstride apart, padded-box layout)
   int grav;
-  double g[3];
+  const double *gf;
   double cfl, dxmin, idxmin;  // the gravity CFL term works on the smallest cell size
 };
 

@@ -140,4 +140,16 @@ def default_problem(kind, params=None, float_round=True):
     pr.rb_R, pr.rb_x0, pr.rb_y0, pr.rb_z0 = r(0.1), r(0.5), r(0.25), r(0.5)
     pr.rb_din, pr.rb_dout = r(1.0), r(2.0)
     pr.rb_gx, pr.rb_gy, pr.rb_gz = r(0.0), r(-0.1), r(0.0)
+    # DiskParams (its own mesh defaults -0.5 .. 0.5), PointSourceGravity
+    dx0, dx1 = (xmin, xmax) if params is not None else (-0.5, 0.5)
+    dy0, dy1 = (ymin, ymax) if params is not None else (-0.5, 0.5)
+    dz0, dz1 = (zmin, zmax) if params is not None else (-0.5, 0.5)
+    pr.disk_radius = r((dx1 - dx0) * 0.125)
+    pr.disk_radius_inner = r(pr.disk_radius / 10)
+    pr.disk_xc, pr.disk_yc, pr.disk_zc = r((dx0 + dx1) / 2), r((dy0 + dy1) / 2), r((dz0 + dz1) / 2)
+    pr.disk_ref_density, pr.disk_contrast_density = r(1.0), r(100.0)
+    pr.disk_contrast_width, pr.disk_ref_sound_speed = r(0.01), r(0.2)
+    pr.pg_x = pr.pg_y = pr.pg_z = r(0.0)
+    pr.pg_gm = r(1.0) * r(1.0)
+    pr.pg_eps = r(0.01)
     return pr

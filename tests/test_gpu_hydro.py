@@ -303,6 +303,17 @@ def gravity_case(case, **kw):
     p = make_hydro_params(dim, *n, gamma0=1.4, cfl=0.7, slope_type=2.0, bc=bc, tEnd=10.0,
                           xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5, **kw)
     p.gravity_static = 1
+    if case.startswith("disk"):  # settings/test_muscl_disk_2d.ini: walls, point gravity
+        p = make_hydro_params(dim, *n, gamma0=1.667, cfl=0.5, slope_type=1.0, bc=A.BC_NEUMANN,
+                              riemann=A.RIEMANN_HLLC, tEnd=10.0, xmin=-0.5, xmax=0.5,
+                              ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5, **kw)
+        p.gravity_static = 1
+        pr = default_problem(A.PROBLEM_DISK, p)
+        pr.disk_radius, pr.disk_ref_density, pr.disk_ref_sound_speed = 0.2, 100.0, 0.316227766
+        pr.disk_radius_inner = 0.02
+        pr.disk_xc, pr.disk_yc = 0.03, -0.02
+        pr.pg_x, pr.pg_y = 0.01, 0.0
+        return p, pr
     kind = A.PROBLEM_RAYLEIGH_TAYLOR if case.startswith("rti") else A.PROBLEM_RISING_BUBBLE
     pr = default_problem(kind, p)
     g = (0.02, -0.1, 0.0) if dim == 2 else (0.02, 0.015, -0.1)
@@ -314,21 +325,22 @@ def gravity_case(case, **kw):
 
 
 @pytest.mark.parametrize("path", PATHS)
-@pytest.mark.parametrize("case", ["rti2d", "rti3d", "rb2d", "rb3d"])
+@pytest.mark.parametrize("case", ["rti2d", "rti3d", "rb2d", "rb3d", "disk2d", "disk3d"])
 def test_strict_bitwise_gravity(gpu, H, strip, path, case):
-    """Static gravity: the half-step predictor on every face state, the source term after the
-    update (2D with the new density floored, 3D without) and the gravity CFL dt; strict, bit
-    for bit over 4 CFL steps (t and dt included), both kernel forms, partial strips."""
+    """Gravity: the half-step predictor on every face state (each with its own cell's field),
+    the source term after the update (2D with the new density floored, 3D without) and the
+    gravity CFL dt; uniform fields and the disk's point mass; strict, bit for bit over 4 CFL
+    steps (t and dt included), both kernel forms, partial strips."""
     strip(8 if "2d" in case else 4)
     p, pr = gravity_case(case, precision=A.FP64_STRICT, kernel_path=path)
-    compare_steps(H, p, pr, 4, tag=case, oracle_state=case.startswith("rti"))
+    compare_steps(H, p, pr, 4, tag=case, oracle_state=not case.startswith("rb"))
 
 
-@pytest.mark.parametrize("case", ["rti2d", "rb3d"])
+@pytest.mark.parametrize("case", ["rti2d", "rb3d", "disk2d"])
 def test_fast_gravity_within_tolerance(gpu, H, case):
     p, pr = gravity_case(case, kernel_path=A.PATH_FUSED)
     compare_steps(H, p, pr, 4, exact=False, tol=1e-12, tag=case,
-                  oracle_state=case.startswith("rti"))
+                  oracle_state=not case.startswith("rb"))
 
 
 def test_gravity_problems_need_static_gravity(gpu):

@@ -283,7 +283,7 @@ def test_sod_shock_tube_converges(H, rs):
 @pytest.mark.parametrize("kind", [A.PROBLEM_IMPLODE, A.PROBLEM_BLAST, A.PROBLEM_KELVIN_HELMHOLTZ,
                                   A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX,
                                   A.PROBLEM_FOUR_QUADRANT, A.PROBLEM_RAYLEIGH_TAYLOR,
-                                  A.PROBLEM_RISING_BUBBLE])
+                                  A.PROBLEM_RISING_BUBBLE, A.PROBLEM_DISK])
 def test_initial_state_bit_identical_to_oracle(H, dim, kind):
     from ppkmhd_amd.solver import hydro_initial_state
 
@@ -299,8 +299,10 @@ def test_initial_state_bit_identical_to_oracle(H, dim, kind):
         pr.fq_config = variant
         a, b = hydro_initial_state(p, pr), H.init_array(p, pr)
         pr.rb_x0, pr.rb_y0, pr.rb_z0, pr.rb_R = 0.6, 0.5, 0.4, 0.35  # a bubble inside the box
+        pr.disk_xc, pr.disk_yc, pr.disk_zc, pr.disk_radius = 0.71, 0.53, 0.37, 0.4
+        pr.disk_radius_inner = 0.04 + 0.1 * variant
         transcendental = kind in (A.PROBLEM_GRESHO_VORTEX, A.PROBLEM_ISENTROPIC_VORTEX,
-                                  A.PROBLEM_RAYLEIGH_TAYLOR) or (
+                                  A.PROBLEM_RAYLEIGH_TAYLOR, A.PROBLEM_DISK) or (
             kind == A.PROBLEM_KELVIN_HELMHOLTZ and not pr.kh_p_rand)
         if transcendental:  # sin / cos / exp / log / pow of two compilers' math calls
             np.testing.assert_allclose(a, b, rtol=4e-16, atol=1e-300)
@@ -566,6 +568,45 @@ def test_gravity_hydrostatic_column_and_rising_bubble(H, dim):
     assert centroid(s2.get_state()) > c0 + 0.3
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_disk_point_mass_field_and_state(H, dim):
+    """Disk problem: the field is a softened point mass, g = GM (P_s - P) / (|P - P_s|^2 +
+    eps^2)^(3/2), with the source at the disk centre plus [gravity] x, y, z (the reference's
+    functor evaluates it at disk-relative coordinates); the disk rotates counter-clockwise
+    (z angular momentum of every cell positive) and is at least contrast / 2 denser inside
+    its radius than beyond radius + contrast_width."""
+    p = make_hydro_params(dim, 24, 20, 16, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5,
+                          zmax=0.5, gamma0=1.667)
+    p.gravity_static = 1
+    pr = default_problem(A.PROBLEM_DISK, p)
+    pr.disk_xc, pr.disk_yc, pr.disk_zc = 0.05, -0.03, 0.02
+    pr.pg_x, pr.pg_y, pr.pg_z = 0.01, 0.02, -0.01
+    pr.pg_gm, pr.pg_eps = 2.0, 0.05
+    G = H.gravity_field(p, pr)
+    g = p.ghost_width
+    ii = np.arange(p.nx + 2 * g)
+    x = p.xmin + p.dx / 2 + (ii - g) * p.dx
+    y = p.ymin + p.dy / 2 + (np.arange(p.ny + 2 * g) - g) * p.dy
+    z = p.zmin + p.dz / 2 + (np.arange(p.nz + 2 * g) - g) * p.dz if dim == 3 else np.zeros(1)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    sx, sy = pr.disk_xc + pr.pg_x, pr.disk_yc + pr.pg_y
+    sz = (pr.disk_zc if dim == 3 else 0.0) + pr.pg_z
+    d = [sx - X, sy - Y, sz - Z]
+    den = (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + pr.pg_eps ** 2) ** 1.5
+    for c in range(dim):
+        np.testing.assert_allclose(G[c], pr.pg_gm * d[c] / den, rtol=1e-13, atol=1e-13)
+    U = H.init_array(p, pr)
+    xr, yr = X - pr.disk_xc, Y - pr.disk_yc
+    lz = xr * U[3] - yr * U[2]
+    assert (lz > 0).all()
+    r = np.sqrt(xr ** 2 + yr ** 2)
+    zr = np.abs(Z - pr.disk_zc) if dim == 3 else np.zeros_like(r)
+    inside = (r < 0.8 * pr.disk_radius) & (zr < 0.3 * pr.disk_radius)
+    outside = (r > pr.disk_radius + pr.disk_contrast_width) & (r < 1.5 * pr.disk_radius)
+    assert inside.any() and outside.any()
+    assert U[0][inside].min() > U[0][outside].max() * pr.disk_contrast_density / 2
+
+
 def test_four_quadrant_selection_and_corner(H):
     """config_number 18 (the reference's switch falls from it into the default), negative and
     out-of-range numbers all give configuration 1; cells exactly at the corner coordinate
@@ -636,8 +677,16 @@ implementationVersion=0
     # the reference this library lacks are refused at init
     _, pr4, _ = params_from_ini(text=txt.replace("problem=implode", "problem=nonsense"))
     assert pr4.kind == A.PROBLEM_IMPLODE
-    _, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=disk"))
-    assert pr5.kind == -1  # point-mass gravity: not provided
+    # settings/test_muscl_disk_2d.ini: point gravity, disk section
+    p5, pr5, _ = params_from_ini(text=txt.replace("problem=implode", "problem=disk")
+                                 .replace("[other]", "[gravity]\npoint=yes\nsoften=0.01\n[disk]\n"
+                                          "radius=0.125\nxc=0.0\nyc=0.0\nref_density=100.0\n"
+                                          "ref_sound_speed = 0.316227766\ncontrast_density = 100.0\n"
+                                          "contrast_width = 0.01\n[other]"))
+    assert pr5.kind == A.PROBLEM_DISK and p5.gravity_static == 1
+    assert (pr5.disk_radius, pr5.disk_ref_sound_speed, pr5.pg_eps, pr5.pg_gm) == \
+        (f32(0.125), f32(0.316227766), f32(0.01), 1.0)
+    assert pr5.disk_radius_inner == f32(f32(0.125) / 10)
     # settings/test_rayleigh_taylor_2d.ini and test_muscl_rising_bubble_3d.ini sections
     p8, pr8, _ = params_from_ini(text=txt.replace("problem=implode", "problem=rayleigh_taylor")
                                  .replace("[other]", "[gravity]\nstatic=yes\n[rayleigh_taylor]\n"
