@@ -71,7 +71,9 @@ enum ppk_problem_kind { PPK_PROBLEM_ORSZAG_TANG = 0, PPK_PROBLEM_BLAST = 1,
                         PPK_PROBLEM_FOUR_QUADRANT = 7 /* 2D */,
                         /* with static gravity ([gravity] static=yes) */
                         PPK_PROBLEM_RAYLEIGH_TAYLOR = 8, PPK_PROBLEM_RISING_BUBBLE = 9,
-                        PPK_PROBLEM_DISK = 10 /* point-mass gravity */ };
+                        PPK_PROBLEM_DISK = 10 /* point-mass gravity */,
+                        /* MHD solvers only (src/muscl/SolverMHDMuscl.h:686-695) */
+                        PPK_PROBLEM_ROTOR = 11, PPK_PROBLEM_FIELD_LOOP = 12 };
 
 /*
  * Solver parameters.  Mirrors the fields of HydroParams / HydroSettings that the MHD MUSCL
@@ -154,6 +156,11 @@ typedef struct ppk_mhd_problem {
   double disk_radius, disk_radius_inner, disk_xc, disk_yc, disk_zc, disk_ref_density,
       disk_contrast_density, disk_contrast_width, disk_ref_sound_speed;
   double pg_x, pg_y, pg_z, pg_gm, pg_eps;
+  /* MHD rotor: radii, spin speed, pressure, field (src/shared/problems/RotorParams.h) */
+  double rotor_r0, rotor_r1, rotor_u0, rotor_p0, rotor_b0;
+  /* MHD field loop: radius, density inside, vector-potential amplitude, flow speed
+   * (src/shared/problems/FieldLoopParams.h) */
+  double loop_radius, loop_density_in, loop_amplitude, loop_vflow;
 } ppk_mhd_problem;
 
 typedef struct ppk_mhd_solver ppk_mhd_t;

@@ -1801,6 +1801,123 @@ void init_problem_array(const ppk_mhd_params &p, const ppk_mhd_problem &pr, doub
           at(i, j, k, IB) = s[6];
           at(i, j, k, IC) = p.dim == 2 ? 0.0 : s[7];
         }
+  } else if (pr.kind == PPK_PROBLEM_ROTOR) {
+    // InitRotorFunctor2D_MHD (src/muscl/MHDInitFunctors2D.h:600-712) and InitRotorFunctor3D_MHD
+    // (src/muscl/MHDInitFunctors3D.h:640-756)
+    const double r0 = pr.rotor_r0, r1 = pr.rotor_r1, u0 = pr.rotor_u0, p0 = pr.rotor_p0;
+    const double xCenter = (p.xmax + p.xmin) / 2, yCenter = (p.ymax + p.ymin) / 2;
+    for (int k = 0; k < kend; ++k)
+      for (int j = 0; j < b.js; ++j)
+        for (int i = 0; i < b.is; ++i) {
+          const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx;
+          const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy;
+          const double r = std::sqrt((x - xCenter) * (x - xCenter) + (y - yCenter) * (y - yCenter));
+          const double f_r = (r1 - r) / (r1 - r0);
+          if (p.dim == 2) {
+            if (r <= r0) {
+              at(i, j, k, ID) = 10.0;
+              at(i, j, k, IU) = -at(i, j, k, ID) * f_r * u0 * (y - yCenter) / r0;
+              at(i, j, k, IV) = at(i, j, k, ID) * f_r * u0 * (x - xCenter) / r0;
+            } else if (r <= r1) {
+              at(i, j, k, ID) = 1 + 9 * f_r;
+              at(i, j, k, IU) = -at(i, j, k, ID) * f_r * u0 * (y - yCenter) / r;
+              at(i, j, k, IV) = at(i, j, k, ID) * f_r * u0 * (x - xCenter) / r;
+            } else {
+              at(i, j, k, ID) = 1.0;
+              at(i, j, k, IU) = 0.0;
+              at(i, j, k, IV) = 0.0;
+            }
+          } else {
+            if (r <= r0) {
+              at(i, j, k, ID) = 10.0;
+              at(i, j, k, IU) = -u0 * (y - yCenter) / r0;
+              at(i, j, k, IV) = u0 * (x - xCenter) / r0;
+            } else if (r <= r1) {
+              at(i, j, k, ID) = 1 + 9 * f_r;
+              at(i, j, k, IU) = -f_r * u0 * (y - yCenter) / r;
+              at(i, j, k, IV) = f_r * u0 * (x - xCenter) / r;
+            } else {
+              at(i, j, k, ID) = 1.0;
+              at(i, j, k, IU) = 0.0;
+              at(i, j, k, IV) = 0.0;
+            }
+          }
+          at(i, j, k, IW) = 0.0;
+          at(i, j, k, IA) = pr.rotor_b0;
+          at(i, j, k, IB) = 0.0;
+          at(i, j, k, IC) = 0.0;
+          const double U = at(i, j, k, IU), V = at(i, j, k, IV), W = at(i, j, k, IW);
+          const double Ab = at(i, j, k, IA);
+          if (p.dim == 2)
+            at(i, j, k, IP) = p0 / (gamma0 - 1.0) + 0.5 * (U * U + V * V + W * W) / at(i, j, k, ID) +
+                              0.5 * (Ab * Ab);
+          else
+            at(i, j, k, IP) = p0 / (gamma0 - 1.0) + (U * U + V * V + W * W) / 2 / at(i, j, k, ID) +
+                              (Ab * Ab) / 2;
+        }
+  } else if (pr.kind == PPK_PROBLEM_FIELD_LOOP) {
+    // InitFieldLoopFunctor2D_MHD (src/muscl/MHDInitFunctors2D.h:715-893) and
+    // InitFieldLoopFunctor3D_MHD (src/muscl/MHDInitFunctors3D.h:759-1010): vector potential on
+    // every cell, then the interior state with the curl of A, then the interior energy
+    std::vector<double> A((size_t)b.cells() * 3, 0.0);
+    auto Aat = [&](int i, int j, int k, int c) -> double & { return A[b.at(i, j, k, c)]; };
+    for (int k = 0; k < kend; ++k)
+      for (int j = 0; j < b.js; ++j)
+        for (int i = 0; i < b.is; ++i) {
+          const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx;
+          const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy;
+          const double r = std::sqrt(x * x + y * y);
+          if (r < pr.loop_radius) Aat(i, j, k, 2) = pr.loop_amplitude * (pr.loop_radius - r);
+        }
+    const double cos_theta = 2.0 / std::sqrt(5.0);
+    const double sin_theta = std::sqrt(1 - cos_theta * cos_theta);
+    const int k0 = p.dim == 3 ? gw : 0, k1 = p.dim == 3 ? b.ks - gw : 1;
+    for (int k = k0; k < k1; ++k)
+      for (int j = gw; j < b.js - gw; ++j)
+        for (int i = gw; i < b.is - gw; ++i) {
+          const double x = p.xmin + p.dx / 2 + (i - gw) * p.dx;
+          const double y = p.ymin + p.dy / 2 + (j - gw) * p.dy;
+          const double r = std::sqrt(x * x + y * y);
+          at(i, j, k, ID) = r < pr.loop_radius ? pr.loop_density_in : 1.0;
+          at(i, j, k, IU) = at(i, j, k, ID) * pr.loop_vflow * cos_theta;
+          at(i, j, k, IV) = at(i, j, k, ID) * pr.loop_vflow * sin_theta;
+          if (p.dim == 2) {
+            const double diag = std::sqrt(1.0 * (p.nx * p.nx + p.ny * p.ny + p.nz * p.nz));
+            at(i, j, k, IW) = at(i, j, k, ID) * pr.loop_vflow * p.nz / diag;
+            at(i, j, k, IA) = (Aat(i, j + 1, k, 2) - Aat(i, j, k, 2)) / p.dy;
+            at(i, j, k, IB) = -(Aat(i + 1, j, k, 2) - Aat(i, j, k, 2)) / p.dx;
+            at(i, j, k, IC) = 0.0;
+          } else {
+            at(i, j, k, IW) = at(i, j, k, ID) * pr.loop_vflow;
+            at(i, j, k, IA) = (Aat(i, j + 1, k, 2) - Aat(i, j, k, 2)) / p.dy -
+                              (Aat(i, j, k + 1, 1) - Aat(i, j, k, 1)) / p.dz;
+            at(i, j, k, IB) = (Aat(i, j, k + 1, 0) - Aat(i, j, k, 0)) / p.dz -
+                              (Aat(i + 1, j, k, 2) - Aat(i, j, k, 2)) / p.dx;
+            at(i, j, k, IC) = (Aat(i + 1, j, k, 1) - Aat(i, j, k, 1)) / p.dx -
+                              (Aat(i, j + 1, k, 0) - Aat(i, j, k, 0)) / p.dy;
+          }
+        }
+    for (int k = k0; k < k1; ++k)
+      for (int j = gw; j < b.js - gw; ++j)
+        for (int i = gw; i < b.is - gw; ++i) {
+          if (p.cIso > 0) {
+            at(i, j, k, IP) = 0.0;
+            continue;
+          }
+          auto sqr = [](double v) { return v * v; };
+          const double U = at(i, j, k, IU), V = at(i, j, k, IV), W = at(i, j, k, IW);
+          if (p.dim == 2)
+            at(i, j, k, IP) = 1.0f / (gamma0 - 1.0) +
+                              0.5 * (0.25 * sqr(at(i, j, k, IA) + at(i + 1, j, k, IA)) +
+                                     0.25 * sqr(at(i, j, k, IB) + at(i, j + 1, k, IB))) +
+                              0.5 * (U * U + V * V) / at(i, j, k, ID);
+          else
+            at(i, j, k, IP) = 1.0f / (gamma0 - 1.0) +
+                              0.5 * (0.25 * sqr(at(i, j, k, IA) + at(i + 1, j, k, IA)) +
+                                     0.25 * sqr(at(i, j, k, IB) + at(i, j + 1, k, IB)) +
+                                     0.25 * sqr(at(i, j, k, IC) + at(i, j, k + 1, IC))) +
+                              0.5 * (U * U + V * V + W * W) / at(i, j, k, ID);
+        }
   }
 }
 

@@ -17,7 +17,7 @@ PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
 DT_FIXED, DT_CFL = 0, 1
 (PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE,
  PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX, PROBLEM_FOUR_QUADRANT, PROBLEM_RAYLEIGH_TAYLOR,
- PROBLEM_RISING_BUBBLE, PROBLEM_DISK) = range(11)
+ PROBLEM_RISING_BUBBLE, PROBLEM_DISK, PROBLEM_ROTOR, PROBLEM_FIELD_LOOP) = range(13)
 
 PPK_OK = 0
 PPK_ERR_INVALID_ARGUMENT, PPK_ERR_HIP, PPK_ERR_OUT_OF_MEMORY, PPK_ERR_UNSUPPORTED, PPK_ERR_CONFIG = 1, 2, 3, 4, 5
@@ -191,4 +191,13 @@ class Problem(C.Structure):
         ("pg_z", C.c_double),
         ("pg_gm", C.c_double),
         ("pg_eps", C.c_double),
+        ("rotor_r0", C.c_double),
+        ("rotor_r1", C.c_double),
+        ("rotor_u0", C.c_double),
+        ("rotor_p0", C.c_double),
+        ("rotor_b0", C.c_double),
+        ("loop_radius", C.c_double),
+        ("loop_density_in", C.c_double),
+        ("loop_amplitude", C.c_double),
+        ("loop_vflow", C.c_double),
     ]

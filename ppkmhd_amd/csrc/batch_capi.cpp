@@ -238,7 +238,7 @@ const char *ppk_mhd_batch_last_error(const ppk_mhd_batch_t *h) {
 int ppk_mhd_batch_init_problem(ppk_mhd_batch_t *h, const ppk_mhd_problem *problems) {
   if (!h || !problems) return bfail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   for (int m = 0; m < h->nm; ++m)
-    if (problems[m].kind < PPK_PROBLEM_ORSZAG_TANG || problems[m].kind > PPK_PROBLEM_IMPLODE)
+    if (!ppk::mhd_problem_ok(problems[m].kind))
       return bfail(h, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind (member " +
                                                     std::to_string(m) + ")");
   int rc = read_ctrl(h);

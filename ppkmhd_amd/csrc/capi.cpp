@@ -280,6 +280,11 @@ void free_all(ppk_mhd_t *h) {
 
 namespace ppk {
 int validate_params(const ppk_mhd_params *p, std::string &why) { return validate(p, why); }
+// the MHD solvers' problems (SolverMHDMuscl.h:672-709): the hydro-only kinds are refused
+bool mhd_problem_ok(int kind) {
+  return (kind >= PPK_PROBLEM_ORSZAG_TANG && kind <= PPK_PROBLEM_IMPLODE) ||
+         kind == PPK_PROBLEM_ROTOR || kind == PPK_PROBLEM_FIELD_LOOP;
+}
 DevParams make_dev_params(const ppk_mhd_params &p) { return make_dev(p); }
 }  // namespace ppk
 
@@ -406,7 +411,7 @@ int finish_init(ppk_mhd_t *h);
 
 int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
   if (!h || !problem) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
+  if (!ppk::mhd_problem_ok(problem->kind))
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind");
   std::vector<double> host((size_t)h->cells() * 8);
   ppk::init_problem_host(h->p, *problem, host.data());
@@ -421,7 +426,7 @@ int ppk_mhd_init_problem(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
 
 int ppk_mhd_init_problem_device(ppk_mhd_t *h, const ppk_mhd_problem *problem) {
   if (!h || !problem) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
+  if (!ppk::mhd_problem_ok(problem->kind))
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind");
   h->iter_host = 0;
   h->cfl_valid = false;
@@ -436,7 +441,7 @@ int ppk_mhd_initial_state(const ppk_mhd_params *params, const ppk_mhd_problem *p
   int rc = ppk::validate_params(params, why);
   if (rc != PPK_OK) return fail(nullptr, rc, why);
   if (!problem || !U) return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (problem->kind < PPK_PROBLEM_ORSZAG_TANG || problem->kind > PPK_PROBLEM_IMPLODE)
+  if (!ppk::mhd_problem_ok(problem->kind))
     return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "unknown problem kind");
   const size_t cells = (size_t)(params->nx + 2 * params->ghost_width) *
                        (params->ny + 2 * params->ghost_width) *

@@ -18,6 +18,7 @@ const char *thread_error();
 
 // parameter checks and device-side parameter block of a solver (capi.cpp)
 int validate_params(const ppk_mhd_params *p, std::string &why);
+bool mhd_problem_ok(int kind);
 struct DevParams;
 DevParams make_dev_params(const ppk_mhd_params &p);
 

@@ -185,6 +185,10 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
     else if (prob == "implode") R.kind = PPK_PROBLEM_IMPLODE;
     else if (prob == "kelvin_helmholtz") R.kind = PPK_PROBLEM_KELVIN_HELMHOLTZ;
     else if (prob == "wave") R.kind = PPK_PROBLEM_WAVE;
+    // MHD only (SolverMHDMuscl.h:686-695); the hydro solver falls back to implode for them
+    else if (prob == "rotor") R.kind = P.ghost_width == 2 ? PPK_PROBLEM_IMPLODE : PPK_PROBLEM_ROTOR;
+    else if (prob == "field_loop" || prob == "field loop")
+      R.kind = P.ghost_width == 2 ? PPK_PROBLEM_IMPLODE : PPK_PROBLEM_FIELD_LOOP;
     // hydro problems (SolverHydroMuscl.cpp:106-256): Gresho vortex (2D, 3D), isentropic
     // vortex and four quadrant (2D; the 3D solver falls back to implode for both); the gravity
     // problems, which this library does not provide, get kind -1, which
@@ -312,6 +316,16 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
       R.pg_gm = G * M;
       R.pg_eps = m.get_float("gravity", "soften", 0.01f);
     }
+    // RotorParams (b0 default 5 / sqrt(4 pi) computed in double, passed as a float), FieldLoopParams
+    R.rotor_r0 = m.get_float("rotor", "r0", 0.1f);
+    R.rotor_r1 = m.get_float("rotor", "r1", 0.115f);
+    R.rotor_u0 = m.get_float("rotor", "u0", 2.0f);
+    R.rotor_p0 = m.get_float("rotor", "p0", 1.0f);
+    R.rotor_b0 = m.get_float("rotor", "b0", (float)(5.0 / std::sqrt(4 * M_PI)));
+    R.loop_radius = m.get_float("FieldLoop", "radius", 1.0f);
+    R.loop_density_in = m.get_float("FieldLoop", "density_in", 1.0f);
+    R.loop_amplitude = m.get_float("FieldLoop", "amplitude", 1.0f);
+    R.loop_vflow = m.get_float("FieldLoop", "vflow", 1.0f);
     // four quadrant (SolverHydroMuscl.cpp:75-77)
     R.fq_config = (int)m.get_integer("riemann2d", "config_number", 0);
     R.fq_x = m.get_float("riemann2d", "x", 0.8f);

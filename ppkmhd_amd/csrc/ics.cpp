@@ -2,7 +2,7 @@
 // (then uploaded), from the per-cell functions of ics_body.hpp that the on-device
 // initialisation (ics_device.hip) shares.  Restates src/muscl/MHDInitFunctors2D.h and
 // src/muscl/MHDInitFunctors3D.h (Orszag-Tang, blast, Kelvin-Helmholtz, linear wave,
-// implode); citations per branch in ics_body.hpp.
+// implode, rotor, field loop); citations per branch in ics_body.hpp.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -81,6 +81,20 @@ void init_problem_host(const ppk_mhd_params &p, const ppk_mhd_problem &pr, doubl
     for (int k = b.gw; k < b.ks - b.gw; ++k)
       for (int j = b.gw; j < b.js - b.gw; ++j)
         for (int i = b.gw; i < b.is - b.gw; ++i) icb::ic_wave_state(p, K, b, i, j, k, U);
+    return;
+  }
+  if (pr.kind == PPK_PROBLEM_FIELD_LOOP) {
+    std::vector<double> A((size_t)b.cells() * 3, 0.0);
+    for (int k = 0; k < b.ks; ++k)
+      for (int j = 0; j < b.js; ++j)
+        for (int i = 0; i < b.is; ++i) icb::ic_loop_potential(p, pr, b, i, j, k, A.data());
+    const int k0 = p.dim == 3 ? b.gw : 0, k1 = p.dim == 3 ? b.ks - b.gw : 1;
+    for (int k = k0; k < k1; ++k)
+      for (int j = b.gw; j < b.js - b.gw; ++j)
+        for (int i = b.gw; i < b.is - b.gw; ++i) icb::ic_loop_state(p, pr, b, i, j, k, A.data(), U);
+    for (int k = k0; k < k1; ++k)
+      for (int j = b.gw; j < b.js - b.gw; ++j)
+        for (int i = b.gw; i < b.is - b.gw; ++i) icb::ic_loop_energy(p, b, i, j, k, U);
     return;
   }
   for (int k = 0; k < b.ks; ++k)

@@ -217,6 +217,111 @@ PPK_HD void ic_cell(const ppk_mhd_params &p, const ppk_mhd_problem &pr, const Ic
     at(IA) = s[5];
     at(IB) = s[6];
     at(IC) = p.dim == 2 ? 0.0 : s[7];
+  } else if (pr.kind == PPK_PROBLEM_ROTOR) {
+    // InitRotorFunctor2D_MHD / 3D_MHD, src/muscl/MHDInitFunctors2D.h:588-712,
+    // src/muscl/MHDInitFunctors3D.h:627-756: a dense spinning disk (radius r0, a linear ramp to
+    // r1) in a uniform field b0 along x.  The 2D functor writes momenta with the ramp factor
+    // f_r also inside r0; the 3D one writes velocities in the momentum slots; both kept.
+    const double xc = (p.xmax + p.xmin) / 2, yc = (p.ymax + p.ymin) / 2;
+    const double r0 = pr.rotor_r0, r1 = pr.rotor_r1, u0 = pr.rotor_u0;
+    const double r = sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc));
+    const double f_r = (r1 - r) / (r1 - r0);
+    double d, mu, mv;
+    if (r <= r0) {
+      d = 10.0;
+      mu = p.dim == 2 ? -d * f_r * u0 * (y - yc) / r0 : -u0 * (y - yc) / r0;
+      mv = p.dim == 2 ? d * f_r * u0 * (x - xc) / r0 : u0 * (x - xc) / r0;
+    } else if (r <= r1) {
+      d = 1 + 9 * f_r;
+      mu = p.dim == 2 ? -d * f_r * u0 * (y - yc) / r : -f_r * u0 * (y - yc) / r;
+      mv = p.dim == 2 ? d * f_r * u0 * (x - xc) / r : f_r * u0 * (x - xc) / r;
+    } else {
+      d = 1.0;
+      mu = 0.0;
+      mv = 0.0;
+    }
+    const double mw = 0.0, a = pr.rotor_b0;
+    at(ID) = d;
+    at(IU) = mu;
+    at(IV) = mv;
+    at(IW) = mw;
+    at(IA) = a;
+    at(IB) = 0.0;
+    at(IC) = 0.0;
+    at(IP) = p.dim == 2 ? pr.rotor_p0 / (gamma0 - 1.0) + 0.5 * (mu * mu + mv * mv + mw * mw) / d +
+                              0.5 * (a * a)
+                        : pr.rotor_p0 / (gamma0 - 1.0) + (mu * mu + mv * mv + mw * mw) / 2 / d +
+                              (a * a) / 2;
+  }
+}
+
+// Field loop (InitFieldLoopFunctor2D_MHD / 3D_MHD, src/muscl/MHDInitFunctors2D.h:715-893,
+// src/muscl/MHDInitFunctors3D.h:759-1010), in three passes as the reference's functor: the
+// vector potential A_z = amplitude (radius - r) inside the loop (r from the origin, every
+// cell), ...
+PPK_HD void ic_loop_potential(const ppk_mhd_params &p, const ppk_mhd_problem &pr,
+                              const IcBox &b, int i, int j, int k, double *A) {
+  const double x = p.xmin + p.dx / 2 + (i - b.gw) * p.dx;
+  const double y = p.ymin + p.dy / 2 + (j - b.gw) * p.dy;
+  const double r = sqrt(x * x + y * y);
+  A[b.at(i, j, k, IX)] = 0.0;
+  A[b.at(i, j, k, IY)] = 0.0;
+  A[b.at(i, j, k, IZ)] = r < pr.loop_radius ? pr.loop_amplitude * (pr.loop_radius - r) : 0.0;
+}
+// ... the density, the flow along the diagonal and the face fields as the discrete curl of A,
+// interior cells (the 2D functor gives the flow a z component rho vflow nz / |(nx, ny, nz)|
+// that its energy leaves out) ...
+PPK_HD void ic_loop_state(const ppk_mhd_params &p, const ppk_mhd_problem &pr, const IcBox &b,
+                          int i, int j, int k, const double *A, double *U) {
+  auto Aat = [&](int ii, int jj, int kk, int c) { return A[b.at(ii, jj, kk, c)]; };
+  auto at = [&](int v) -> double & { return U[b.at(i, j, k, v)]; };
+  const double x = p.xmin + p.dx / 2 + (i - b.gw) * p.dx;
+  const double y = p.ymin + p.dy / 2 + (j - b.gw) * p.dy;
+  const double cos_theta = 2.0 / sqrt(5.0);
+  const double sin_theta = sqrt(1 - cos_theta * cos_theta);
+  const double r = sqrt(x * x + y * y);
+  const double d = r < pr.loop_radius ? pr.loop_density_in : 1.0;
+  const double vflow = pr.loop_vflow;
+  at(ID) = d;
+  at(IU) = d * vflow * cos_theta;
+  at(IV) = d * vflow * sin_theta;
+  if (p.dim == 2) {
+    const double diag = sqrt(1.0 * (p.nx * p.nx + p.ny * p.ny + p.nz * p.nz));
+    at(IW) = d * vflow * p.nz / diag;
+    at(IA) = (Aat(i, j + 1, k, IZ) - Aat(i, j, k, IZ)) / p.dy;
+    at(IB) = -(Aat(i + 1, j, k, IZ) - Aat(i, j, k, IZ)) / p.dx;
+    at(IC) = 0.0;
+  } else {
+    at(IW) = d * vflow;
+    at(IA) = (Aat(i, j + 1, k, 2) - Aat(i, j, k, 2)) / p.dy -
+             (Aat(i, j, k + 1, 1) - Aat(i, j, k, 1)) / p.dz;
+    at(IB) = (Aat(i, j, k + 1, 0) - Aat(i, j, k, 0)) / p.dz -
+             (Aat(i + 1, j, k, 2) - Aat(i, j, k, 2)) / p.dx;
+    at(IC) = (Aat(i + 1, j, k, 1) - Aat(i, j, k, 1)) / p.dx -
+             (Aat(i, j + 1, k, 0) - Aat(i, j, k, 0)) / p.dy;
+  }
+}
+// ... and the total energy of interior cells once every face field is set (the last interior
+// column / row averages with the +1 ghost face, still zero: the reference fills ghosts after)
+PPK_HD void ic_loop_energy(const ppk_mhd_params &p, const IcBox &b, int i, int j, int k,
+                           double *U) {
+  auto at = [&](int ii, int jj, int kk, int v) -> double & { return U[b.at(ii, jj, kk, v)]; };
+  if (p.cIso > 0) {
+    at(i, j, k, IP) = 0.0;
+    return;
+  }
+  const double gamma0 = p.gamma0;
+  const double sa = at(i, j, k, IA) + at(i + 1, j, k, IA);
+  const double sb = at(i, j, k, IB) + at(i, j + 1, k, IB);
+  const double u = at(i, j, k, IU), v = at(i, j, k, IV), w = at(i, j, k, IW);
+  if (p.dim == 2) {
+    at(i, j, k, IP) = 1.0f / (gamma0 - 1.0) + 0.5 * (0.25 * (sa * sa) + 0.25 * (sb * sb)) +
+                      0.5 * (u * u + v * v) / at(i, j, k, ID);
+  } else {
+    const double sc = at(i, j, k, IC) + at(i, j, k + 1, IC);
+    at(i, j, k, IP) = 1.0f / (gamma0 - 1.0) +
+                      0.5 * (0.25 * (sa * sa) + 0.25 * (sb * sb) + 0.25 * (sc * sc)) +
+                      0.5 * (u * u + v * v + w * w) / at(i, j, k, ID);
   }
 }
 

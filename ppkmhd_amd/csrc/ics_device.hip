@@ -59,11 +59,30 @@ __global__ void k_wave_state(ppk_mhd_params p, icb::IcConsts K, icb::IcBox b, Ra
     icb::ic_wave_state(p, K, b, i, j, k, U);
 }
 
+__global__ void k_loop_potential(ppk_mhd_params p, ppk_mhd_problem pr, icb::IcBox b, Range r,
+                                 double *A) {
+  int i, j, k;
+  if (r.cell((long long)blockIdx.x * kBlock + threadIdx.x, i, j, k))
+    icb::ic_loop_potential(p, pr, b, i, j, k, A);
+}
+__global__ void k_loop_state(ppk_mhd_params p, ppk_mhd_problem pr, icb::IcBox b, Range r,
+                             const double *A, double *U) {
+  int i, j, k;
+  if (r.cell((long long)blockIdx.x * kBlock + threadIdx.x, i, j, k))
+    icb::ic_loop_state(p, pr, b, i, j, k, A, U);
+}
+__global__ void k_loop_energy(ppk_mhd_params p, icb::IcBox b, Range r, double *U) {
+  int i, j, k;
+  if (r.cell((long long)blockIdx.x * kBlock + threadIdx.x, i, j, k))
+    icb::ic_loop_energy(p, b, i, j, k, U);
+}
+
 inline dim3 grid(long long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 }  // namespace icd
 
-// U: 8 x cells (the state), A: 3 x cells of scratch (linear wave only; may alias nothing else)
+// U: 8 x cells (the state), A: 3 x cells of scratch (linear wave and field loop only; may
+// alias nothing else)
 hipError_t init_problem_device(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U,
                                double *A, hipStream_t s) {
   using namespace icd;
@@ -81,6 +100,14 @@ hipError_t init_problem_device(const ppk_mhd_params &p, const ppk_mhd_problem &p
     hipLaunchKernelGGL(k_wave_field, grid(f.count()), blk, 0, s, p, b, f, (const double *)A, U);
     const Range in{b.gw, b.gw, b.gw, b.is - 2 * b.gw, b.js - 2 * b.gw, b.ks - 2 * b.gw};
     hipLaunchKernelGGL(k_wave_state, grid(in.count()), blk, 0, s, p, K, b, in, U);
+    return hipGetLastError();
+  }
+  if (pr.kind == PPK_PROBLEM_FIELD_LOOP) {
+    hipLaunchKernelGGL(k_loop_potential, grid(all.count()), blk, 0, s, p, pr, b, all, A);
+    const Range in{b.gw, b.gw, p.dim == 3 ? b.gw : 0, b.is - 2 * b.gw, b.js - 2 * b.gw,
+                   p.dim == 3 ? b.ks - 2 * b.gw : 1};
+    hipLaunchKernelGGL(k_loop_state, grid(in.count()), blk, 0, s, p, pr, b, in, (const double *)A, U);
+    hipLaunchKernelGGL(k_loop_energy, grid(in.count()), blk, 0, s, p, b, in, U);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_cell, grid(all.count()), blk, 0, s, p, pr, K, b, all, U);

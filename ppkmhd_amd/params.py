@@ -5,6 +5,8 @@ parses with ``strtof`` and returns a ``float`` (reference src/utils/config/Confi
 ``HydroParams`` then widens it to double.  ``f32`` reproduces that rounding so parameters built
 here equal those the C++ ``ppk_mhd_params_from_ini`` produces from the same .ini text.
 """
+import math
+
 import numpy as np
 
 from . import _abi as A
@@ -152,4 +154,8 @@ def default_problem(kind, params=None, float_round=True):
     pr.pg_x = pr.pg_y = pr.pg_z = r(0.0)
     pr.pg_gm = r(1.0) * r(1.0)
     pr.pg_eps = r(0.01)
+    # RotorParams, FieldLoopParams
+    pr.rotor_r0, pr.rotor_r1, pr.rotor_u0, pr.rotor_p0 = r(0.1), r(0.115), r(2.0), r(1.0)
+    pr.rotor_b0 = r(5.0 / math.sqrt(4 * math.pi))
+    pr.loop_radius, pr.loop_density_in, pr.loop_amplitude, pr.loop_vflow = r(1.0), r(1.0), r(1.0), r(1.0)
     return pr

@@ -134,6 +134,42 @@ density_out=1.0
     assert pr.blast_bx == 0.5
 
 
+def test_ini_rotor_and_field_loop():
+    """settings/test_mhd_rotor.ini's [rotor] section; "field_loop" and "field loop" select the
+    field loop, while "FieldLoop" (the spelling of settings/mhd_fieldloop2d.ini) matches none of
+    the reference's names and falls back to Orszag-Tang as the reference does; the hydro solver
+    falls back to implode for both."""
+    import math
+    base = """[run]
+solver_name=MHD_Muscl_2D
+[mesh]
+nx=8
+ny=8
+[hydro]
+problem=%s
+[rotor]
+r0=0.1
+r1=0.115
+u0=1.0
+p0=1.5
+[FieldLoop]
+radius=0.3
+amplitude=0.001
+vflow=3
+"""
+    p, pr, _ = params_from_ini(text=base % "rotor")
+    assert pr.kind == A.PROBLEM_ROTOR
+    assert (pr.rotor_r1, pr.rotor_u0, pr.rotor_p0) == (f32(0.115), 1.0, 1.5)
+    assert pr.rotor_b0 == f32(5.0 / math.sqrt(4 * math.pi))
+    for name in ("field_loop", "field loop"):
+        _, pr2, _ = params_from_ini(text=base % name)
+        assert pr2.kind == A.PROBLEM_FIELD_LOOP
+        assert (pr2.loop_radius, pr2.loop_amplitude, pr2.loop_vflow) == (f32(0.3), f32(0.001), 3.0)
+    assert params_from_ini(text=base % "FieldLoop")[1].kind == A.PROBLEM_ORSZAG_TANG
+    hydro = (base % "rotor").replace("MHD_Muscl_2D", "Hydro_Muscl_2D")
+    assert params_from_ini(text=hydro)[1].kind == A.PROBLEM_IMPLODE
+
+
 def test_ini_unknown_solver_is_an_error():
     with pytest.raises(PPKError) as e:
         params_from_ini(text="[run]\nsolver_name=Hydro_SDM_2D_degree2\n")

@@ -11,7 +11,7 @@ from test_ics_cpu import cases
 
 pytestmark = pytest.mark.gpu
 
-EXACT = (A.PROBLEM_BLAST, A.PROBLEM_IMPLODE)
+EXACT = (A.PROBLEM_BLAST, A.PROBLEM_IMPLODE, A.PROBLEM_ROTOR, A.PROBLEM_FIELD_LOOP)  # sqrt, / only
 
 
 @pytest.mark.parametrize("dim,n,kind,variant",
@@ -32,6 +32,9 @@ def test_device_initial_state_matches_host(gpu, dim, n, kind, variant):
         pr.ot_kt = 1.0
     elif variant and variant.startswith("wave"):
         pr.wave_type = int(variant[-1])
+    elif variant == "small":
+        pr.rotor_r0, pr.rotor_r1 = 0.12, 0.3
+        pr.loop_radius, pr.loop_density_in, pr.loop_vflow = 0.3, 2.5, 0.7
     a = SolverMHDMuscl(p)
     a.init_problem(pr)
     b = SolverMHDMuscl(p)

@@ -20,6 +20,11 @@ def cases():
     out.append((3, (12, 10, 8), A.PROBLEM_ORSZAG_TANG, "kt"))
     for t in (0, 1, 2):
         out.append((3, (12, 10, 8), A.PROBLEM_WAVE, f"wave{t}"))
+    for dim in (2, 3):
+        n = (12, 10, 8) if dim == 3 else (20, 14)
+        for kind in (A.PROBLEM_ROTOR, A.PROBLEM_FIELD_LOOP):
+            out.append((dim, n, kind, None))
+            out.append((dim, n, kind, "small"))
     return out
 
 
@@ -39,7 +44,45 @@ def test_host_initial_state_equals_oracle(oracle, dim, n, kind, variant):
         pr.ot_b0_scale = 1.3
     elif variant and variant.startswith("wave"):
         pr.wave_type = int(variant[-1])
+    elif variant == "small":  # the rotor's ramp and the loop's edge inside the box
+        pr.rotor_r0, pr.rotor_r1 = 0.12, 0.3
+        pr.loop_radius, pr.loop_density_in, pr.loop_vflow = 0.3, 2.5, 0.7
     U = initial_state(p, pr)
     ref = oracle.init_problem_array(p, pr)
     assert np.array_equal(U, ref)
     assert np.isfinite(U).all()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_field_loop_is_divergence_free_and_rotor_spins(oracle, dim):
+    """Field loop: the face fields are the discrete curl of the vector potential, so their
+    discrete divergence vanishes to roundoff in every interior cell whose upper faces are
+    interior; rotor: the disk spins counter-clockwise (z angular momentum about the centre
+    positive inside r1, zero outside) in the uniform field b0 along x."""
+    n = (24, 20, 16) if dim == 3 else (24, 20)
+    p = make_params(dim, *n, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5,
+                    gamma0=1.4)
+    pr = default_problem(A.PROBLEM_FIELD_LOOP, p)
+    pr.loop_radius, pr.loop_amplitude = 0.3, 1e-3
+    U = initial_state(p, pr)
+    g = p.ghost_width
+    ks = slice(g, -g - 1) if dim == 3 else slice(None)
+    dA = (U[5][ks, g:-g - 1, g + 1:-g] - U[5][ks, g:-g - 1, g:-g - 1]) / p.dx
+    dB = (U[6][ks, g + 1:-g, g:-g - 1] - U[6][ks, g:-g - 1, g:-g - 1]) / p.dy
+    divb = dA + dB
+    if dim == 3:
+        divb = divb + (U[7][g + 1:-g, g:-g - 1, g:-g - 1] - U[7][g:-g - 1, g:-g - 1, g:-g - 1]) / p.dz
+    scale = np.abs(U[5:8]).max() / min(p.dx, p.dy)
+    assert scale > 0 and np.abs(divb).max() <= 1e-12 * scale
+    assert np.abs(U[5]).max() > 0  # the loop is there
+    pr = default_problem(A.PROBLEM_ROTOR, p)
+    pr.rotor_r0, pr.rotor_r1 = 0.12, 0.3
+    U = initial_state(p, pr)
+    x = p.xmin + p.dx / 2 + (np.arange(U.shape[3]) - g) * p.dx
+    y = p.ymin + p.dy / 2 + (np.arange(U.shape[2]) - g) * p.dy
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    lz = X * U[3] - Y * U[2]
+    r = np.sqrt(X * X + Y * Y)
+    assert (lz[:, r < pr.rotor_r1 * 0.999] > 0).all()
+    assert (lz[:, r > pr.rotor_r1 * 1.001] == 0).all()
+    assert (U[5] == pr.rotor_b0).all() and (U[6] == 0).all()
