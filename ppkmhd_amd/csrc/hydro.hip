@@ -445,9 +445,12 @@ __global__ void __launch_bounds__(kWaves2 * 64, MINW) k_hfused2d(const double *_
   }
 }
 
-// ---- fused 3D: a block of 8 waves stacked along y marches along z -----------------------
-// Wave w holds row j = gw + 4 * by + w - 2, lane l column i = gw + 60 * bx + l - 2; rows 2..5
-// and lanes 2..61 are the block's output.  At slope plane s each thread keeps Q(s-1), Q(s),
+// ---- fused 3D: a block of 8 waves, 16 rows of 32 columns stacked along y, marches along z --
+// Thread t holds row r = t / 32 (two rows per wave), j = gw + 12 * by + r - 2, and column
+// c = t % 32, i = gw + 28 * bx + c - 2; rows 2..13 and columns 2..29 are the block's output
+// (32-wide rows: 336 output cells per block and plane, against 240 with one 64-wide row per
+// wave).  x neighbours come from LDS and shuffles within the 32 lanes of a row.  At slope
+// plane s each thread keeps Q(s-1), Q(s),
 // Q(s+1) of its column in registers; plane s of Q goes through LDS for the x and y
 // neighbours, the upper y-face states of each row through LDS to the row above, and the y
 // fluxes back through LDS to the row below.  The z face below plane s uses the upper z state
@@ -459,35 +462,36 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
                                                        double *__restrict__ Uout, HydroDev h,
                                                        double *__restrict__ parts,
                                                        const double *ctrl, int fold_cfl) {
-  __shared__ double sQ[2][NV][kW3][64];  // Q of plane s (parity s & 1), later the y fluxes
-  __shared__ double sT[NV][kW3][64];     // upper y-face states of plane s
+  constexpr int RW = kHydroRowW3, NR = kHydroBlockRows3;
+  __shared__ double sQ[2][NV][NR][RW];  // Q of plane s (parity s & 1), later the y fluxes
+  __shared__ double sT[NV][NR][RW];     // upper y-face states of plane s
   __shared__ double red[kW3];
   if (!active(ctrl)) return;
-  const int lane = threadIdx.x & 63;
-  const int ly = threadIdx.x >> 6;
-  const int nsx = (h.nx + kHydroCols - 1) / kHydroCols;
+  const int lane = threadIdx.x % RW;  // column within the row
+  const int ly = threadIdx.x / RW;    // row of the block
+  const int nsx = (h.nx + kHydroCols3 - 1) / kHydroCols3;
   const int nsy = (h.ny + kHydroRows3 - 1) / kHydroRows3;
   const int KH = h.strip_rows;
   const int bx = blockIdx.x % nsx;
   const int by = (blockIdx.x / nsx) % nsy;
   const int bz = blockIdx.x / (nsx * nsy);
-  const int i = h.gw + bx * kHydroCols + lane - 2;
+  const int i = h.gw + bx * kHydroCols3 + lane - 2;
   const int j = h.gw + by * kHydroRows3 + ly - 2;
   const int ic = i < h.isize ? i : h.isize - 1;  // clamped halo, never output
   const int jc = j < h.jsize ? j : h.jsize - 1;
   const int k0 = h.gw + bz * KH;
   const int k1 = min(k0 + KH, h.gw + h.nz);
-  const bool slope_row = ly >= 1 && ly <= kW3 - 2;
-  const bool yface_row = ly >= 2 && ly <= kW3 - 2;
+  const bool slope_row = ly >= 1 && ly <= NR - 2;
+  const bool yface_row = ly >= 2 && ly <= NR - 2;
   const bool out_row = ly >= 2 && ly < 2 + kHydroRows3;
-  const bool out = out_row && lane >= 2 && lane < 2 + kHydroCols && i < h.gw + h.nx &&
+  const bool out = out_row && lane >= 2 && lane < 2 + kHydroCols3 && i < h.gw + h.nx &&
                    j < h.gw + h.ny;
   const long long vs = h.vstride;
   const long long col = (long long)jc * h.isize + ic;
   const double dt = ctrl[CTRL_DT];
   const double dtdx = dt / h.dx, dtdy = dt / h.dy, dtdz = dt / h.dz, hdt = 0.5 * dt;
   const hyd::HydroPar &P = h.P;
-  const int lm = lane > 0 ? lane - 1 : 0, lp = lane < 63 ? lane + 1 : 63;
+  const int lm = lane > 0 ? lane - 1 : 0, lp = lane < RW - 1 ? lane + 1 : RW - 1;
 
   HSt<NV> qm, q0, u0, t;
   double c, rho_prev = 0.0;  // density of (i, j, s-1) before the step, for the gravity source
@@ -561,7 +565,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       gravity_predict<GR>(xmin, g0, h, hdt);
       gravity_predict<GR>(xmax, g0, h, hdt);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, 64);
+      for (int v = 0; v < NV; ++v) xl[v] = __shfl_up(xmax[v], 1, RW);
       hyd::riemann_rs<RS>(xl, xmin, fx, P);
 #pragma unroll
       for (int v = 0; v < NV; ++v) fx[v] = fx[v] * dtdx;
@@ -580,7 +584,7 @@ __global__ void __launch_bounds__(kW3 * 64, MINW) k_hfused3d(const double *__res
       if (plane_out) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          const double fxr = __shfl_down(fx[v], 1, 64);
+          const double fxr = __shfl_down(fx[v], 1, RW);
           double o = u0[v];
           o += fx[v];
           o -= fxr;
@@ -614,7 +618,7 @@ int fused2d_waves(const HydroDev &h) {
   return nsx * nsy;
 }
 int fused3d_blocks(const HydroDev &h) {
-  const int nsx = (h.nx + kHydroCols - 1) / kHydroCols;
+  const int nsx = (h.nx + kHydroCols3 - 1) / kHydroCols3;
   const int nsy = (h.ny + kHydroRows3 - 1) / kHydroRows3;
   const int nsz = (h.nz + h.strip_rows - 1) / h.strip_rows;
   return nsx * nsy * nsz;

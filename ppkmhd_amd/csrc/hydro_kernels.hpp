@@ -32,9 +32,15 @@ struct HydroDev {
 // output (two columns of halo on each side)
 constexpr int kHydroLanes = 64;
 constexpr int kHydroCols = kHydroLanes - 4;
-// 3D: waves of a block stacked along y; the middle kHydroRows3 rows are the block's output
+// 3D: a block of kHydroWaves3 waves; each wave holds 64 / kHydroRowW3 rows of kHydroRowW3
+// columns, the rows of the block stacked along y.  The middle kHydroCols3 columns and
+// kHydroRows3 rows are the block's output (two columns / rows of halo on each side): 32-wide
+// rows give 28 x 12 output cells per block and plane where 64-wide ones gave 60 x 4.
 constexpr int kHydroWaves3 = 8;
-constexpr int kHydroRows3 = kHydroWaves3 - 4;
+constexpr int kHydroRowW3 = 32;
+constexpr int kHydroBlockRows3 = kHydroWaves3 * (kHydroLanes / kHydroRowW3);
+constexpr int kHydroCols3 = kHydroRowW3 - 4;
+constexpr int kHydroRows3 = kHydroBlockRows3 - 4;
 
 #define PPK_DECLARE_HYDRO(NS)                                                                    \
   namespace NS {                                                                                 \
