@@ -151,7 +151,7 @@ HYDRO_STEP_BYTES = {2: 64.0, 3: 80.0}
 # HBM bytes per cell-update of the fused kernel, rocprofv3 FETCH_SIZE (x2) + WRITE_SIZE (x1) at
 # the bench sizes (profiles/hydro_fused_pmc.md): 2.320 GB / 33,554,432 cells (2D 8192 x 4096),
 # 7.026 GB / 50,331,648 cells (3D 512 x 384 x 256)
-HYDRO_MEASURED_BYTES = {2: 2.3199e9 / 33554432, 3: 7.0263e9 / 50331648}
+HYDRO_MEASURED_BYTES = {2: 2.3199e9 / 33554432, 3: 6.9917e9 / 50331648}
 
 
 def hydro_workload(dim, n, path, precision):

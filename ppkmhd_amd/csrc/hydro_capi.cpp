@@ -124,7 +124,7 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   // tuning knob: output rows (2D) / planes (3D) of one marching wave / block
   const char *strip = std::getenv("PPKMHD_HYDRO_STRIP");
   const int rows = strip ? std::atoi(strip) : 0;
-  d.strip_rows = rows > 0 ? rows : 64;
+  d.strip_rows = rows > 0 ? rows : (p.dim == 3 ? 48 : 64);  // measured best at 8192 x 4096 / 512 x 384 x 256
   // tuning knob: PPKMHD_HYDRO_OCC=1 selects the 4-waves-per-SIMD builds of the fused kernels
   const char *occ = std::getenv("PPKMHD_HYDRO_OCC");
   d.occupancy = occ ? std::atoi(occ) : 0;
