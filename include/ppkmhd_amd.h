@@ -105,10 +105,11 @@ typedef struct ppk_mhd_params {
 } ppk_mhd_params;
 
 /*
- * Initial-condition parameters (problems used by the MHD path: Orszag-Tang, blast,
- * Kelvin-Helmholtz, linear wave, implode).  Field defaults are the reference's
- * (src/shared/problems/{OrszagTang,Blast,KH,Wave,Implode}Params.h).  Extensions beyond the
- * reference are marked "ext" and default to the reference behaviour.
+ * Initial-condition parameters of every problem: MHD (Orszag-Tang, blast, Kelvin-Helmholtz,
+ * linear wave, implode, rotor, field loop) and hydro (implode, blast, Kelvin-Helmholtz,
+ * Gresho vortex, isentropic vortex, four quadrant, Rayleigh-Taylor, rising bubble, disk).
+ * Field defaults are the reference's (src/shared/problems/*Params.h).  Extensions beyond
+ * the reference are marked "ext" and default to the reference behaviour.
  */
 typedef struct ppk_mhd_problem {
   int32_t kind;          /* enum ppk_problem_kind                                      */
