@@ -108,7 +108,7 @@ typedef struct ppk_mhd_params {
  * Initial-condition parameters of every problem: MHD (Orszag-Tang, blast, Kelvin-Helmholtz,
  * linear wave, implode, rotor, field loop) and hydro (implode, blast, Kelvin-Helmholtz,
  * Gresho vortex, isentropic vortex, four quadrant, Rayleigh-Taylor, rising bubble, disk).
- * Field defaults are the reference's (src/shared/problems/*Params.h).  Extensions beyond
+ * Field defaults are the reference's (the ...Params.h files of src/shared/problems).  Extensions beyond
  * the reference are marked "ext" and default to the reference behaviour.
  */
 typedef struct ppk_mhd_problem {
