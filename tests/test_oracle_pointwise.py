@@ -2,7 +2,7 @@
 
 Oracle functions and the checks that pin them:
   riemann_hlld         textbook HLLD (Miyoshi & Kusano 2005) via jump conditions, consistency,
-                       upwinding, mirror symmetry
+                       upwinding, mirror symmetry, the B = 0 contact
   riemann_hll / llf    textbook HLL / LLF formulas, consistency
   mag_riemann2d_hlld   branch form of the same solver (independent restatement), uniform
                        states, supersonic corners
@@ -79,6 +79,28 @@ def test_hlld_mirror_symmetry(oracle):
         f = oracle.riemann(p, qL, qR)
         g = oracle.riemann(p, M * qR, M * qL)
         assert rel(g, S * f) < 1e-12
+
+
+def test_hlld_b0_resolves_contact(oracle):
+    """B = 0, equal pressure and normal velocity u0: the exact solution is a contact (with a
+    shear of v, w) moving at u0, and HLLD, like HLLC, returns the upwind side's physical flux;
+    at u0 = 0 the mass flux is zero.  HLL smears the same contact (the check can tell)."""
+    rng = np.random.default_rng(14)
+    p, phll = P(), P(riemann=A.RIEMANN_HLL)
+    hll_worst = 0.0
+    for n in range(200):
+        u0 = 0.0 if n < 50 else rng.uniform(-0.5, 0.5)
+        pr = rng.uniform(0.5, 3.0)
+        qL, qR = T.random_state(rng, bscale=0.0), T.random_state(rng, bscale=0.0)
+        qL[T.IP] = qR[T.IP] = pr
+        qL[T.IU] = qR[T.IU] = u0
+        f = oracle.riemann(p, qL, qR)
+        _, F = T.physical_flux(qL if u0 >= 0 else qR, p.gamma0)
+        assert rel(f, F) < 1e-13, (n, f, F)
+        if u0 == 0.0:
+            assert f[T.ID] == 0.0
+        hll_worst = max(hll_worst, rel(oracle.riemann(phll, qL, qR), F))
+    assert hll_worst > 1e-2
 
 
 @pytest.mark.parametrize("riemann", [A.RIEMANN_HLL, A.RIEMANN_LLF])

@@ -266,7 +266,9 @@ def _wave_l1(oracle, nx, wave_type, tend):
     return float(np.sqrt((d ** 2).sum()))
 
 
-@pytest.mark.parametrize("wave_type,tend", [(0, 0.5), (1, 1.0)])
+# one period of each wave, the periods of test/convergence/fulltest.sh:34 (fast, Alfven,
+# slow, and the entropy wave advected at V0 = 1)
+@pytest.mark.parametrize("wave_type,tend", [(0, 0.5), (1, 1.0), (2, 2.0), (3, 1.0)])
 def test_linear_wave_second_order(oracle, wave_type, tend):
     e16 = _wave_l1(oracle, 16, wave_type, tend)
     e32 = _wave_l1(oracle, 32, wave_type, tend)
