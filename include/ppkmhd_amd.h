@@ -58,7 +58,7 @@ enum ppk_kernel_path {
 };
 /* dt semantics of ppk_mhd_step */
 enum ppk_dt_mode {
-  PPK_DT_FIXED = 0, /* use the dt argument as given (parity runs at fixed dt)                 */
+  PPK_DT_FIXED = 0, /* use the dt argument (parity runs at fixed dt), clamped to tEnd - t    */
   PPK_DT_CFL = 1    /* reference semantics: dt = cfl / max(sum_d (c_f,d+|v_d|)/dx_d) from the
                        current state, clamped to tEnd - t (src/shared/SolverBase.cpp:149-179) */
 };
@@ -90,7 +90,10 @@ typedef struct ppk_mhd_params {
   int32_t n_step_max;    /* [run] nStepmax                                             */
   int32_t precision;     /* enum ppk_precision                                         */
   int32_t kernel_path;   /* enum ppk_kernel_path                                       */
-  int32_t reserved;      /* must be 0                                                  */
+  /* [OTHER] implementationVersion (src/shared/HydroParams.cpp:446-451): 0 or 1 (both run
+   * the version-0 stages: version 1 only accumulates the same fluxes with atomics); 2 (MHD's
+   * updated-primitive scheme, SolverMHDMuscl.cpp:430-456) is refused with PPK_ERR_UNSUPPORTED */
+  int32_t implementation_version;
   double xmin, xmax, ymin, ymax, zmin, zmax;
   double dx, dy, dz;
   double gamma0, gamma6, cfl, slope_type, smallr, smallc, smallp, smallpp, cIso;

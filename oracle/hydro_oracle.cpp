@@ -496,8 +496,9 @@ struct HydroSolver {
       HORA_STAGE(7);
       dt = p.cfl / D(inv_dt(cur));
       if (t + dt > p.tEnd) dt = p.tEnd - t;
-    } else {
+    } else {  // the library clamps a fixed dt to tEnd as the CFL one
       dt = dt_fixed;
+      if (t + dt > p.tEnd) dt = p.tEnd - t;
     }
     godunov(cur, nxt, dt);
     ++it;

@@ -1178,10 +1178,12 @@ struct Solver {
     std::memcpy(dout, din, sizeof(double) * b.cells() * 8);
     ORA_STAGE(1);
     primitives(p, din, Q.data());
-    if (dt_mode == PPK_DT_CFL)
+    if (dt_mode == PPK_DT_CFL) {
       compute_dt();
-    else
+    } else {  // the library clamps a fixed dt to tEnd as SolverBase::compute_dt clamps its own
       dt = fixed_dt;
+      if (t + dt > tEnd) dt = tEnd - t;
+    }
     const real dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = p.dim == 3 ? dt / p.dz : 0.0;
     if (p.dim == 2)
       step2d(din, dout, dtdx, dtdy);

@@ -37,7 +37,7 @@ class Params(C.Structure):
         ("n_step_max", C.c_int32),
         ("precision", C.c_int32),
         ("kernel_path", C.c_int32),
-        ("reserved", C.c_int32),
+        ("implementation_version", C.c_int32),
         ("xmin", C.c_double),
         ("xmax", C.c_double),
         ("ymin", C.c_double),

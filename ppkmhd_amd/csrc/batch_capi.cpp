@@ -187,7 +187,6 @@ int ppk_mhd_batch_create(const ppk_mhd_params *params, int nmembers, ppk_mhd_bat
     bm.p.bulk32 = h->bulk32;
     bm.p.brick = 0;
     bm.p.tile3 = 0;
-    bm.p.riemann_mix = 1;
     const size_t n = (size_t)bm.p.vstride;
     const int d = bm.p.dim;
     bm.U[0] = static_cast<double *>(dalloc(8 * n * sizeof(double)));

@@ -78,7 +78,6 @@ struct ppk_mhd_solver {
   ppk::StagedScratch w;
   ppk::FusedScratch fw;
   ppk::SurrogateCfg sur;                   // surrogate face solves (fw.sur points here when on)
-  ppk::AuxStream aux;                      // side stream of the two-stream face / edge mode
   unsigned long long *sur_counts = nullptr;  // device: faces by the network, by HLLD
   unsigned char *sur_flags = nullptr;        // device: faces left to the HLLD pass
   bool fused = false;
@@ -120,6 +119,15 @@ int validate(const ppk_mhd_params *p, std::string &why) {
   if (p->ghost_width != 3) {
     why = "ghost_width must be 3 for MHD (HydroParams.cpp:56-71)";
     return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->implementation_version < 0 || p->implementation_version > 2) {
+    why = "implementationVersion must be 0, 1 or 2 (HydroParams.cpp:446-451)";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->implementation_version == 2) {
+    why = "implementationVersion 2 (the updated-primitive scheme of SolverMHDMuscl.cpp:430-456) "
+          "is not provided; versions 0 and 1 run the version-0 scheme";
+    return PPK_ERR_UNSUPPORTED;
   }
   if (p->nx < 3 || p->ny < 3 || (p->dim == 3 && p->nz < 3)) {
     why = "nx, ny (and nz in 3D) must be >= ghost_width";
@@ -183,23 +191,11 @@ ppk::DevParams make_dev(const ppk_mhd_params &p) {
   d.smallp = p.smallp;
   d.cIso = p.cIso;
   d.cfl = p.cfl;
-  // tuning knob: PPKMHD_HALF_TILE selects one of the compiled 3D half-step tile shapes
-  const char *tile = std::getenv("PPKMHD_HALF_TILE");
-  d.tile3 = tile ? std::atoi(tile) : 0;
-  // tuning knob: PPKMHD_BRICK selects the block shape of the fused per-cell kernels
-  const char *brick = std::getenv("PPKMHD_BRICK");
-  d.brick = brick ? std::atoi(brick) : 0;
-  // tuning knob: PPKMHD_RIEMANN_MIX=0 launches the six face / edge kernels one by one, 1 and 2
-  // (default) interleave them in one launch at 4 or 3 waves per SIMD, 3 (4: with 5-wave faces)
-  // runs the three face solves and the three edge solves as two launches side by side on two
-  // streams, each at its own register budget (3D; measured 3 % faster than 2 at 256^3)
-  const char *mix = std::getenv("PPKMHD_RIEMANN_MIX");
-  d.riemann_mix = mix ? std::atoi(mix) : 2;
-  // tuning knob: PPKMHD_PIPE=S (2..16) pipelines the 3D step in S z-slabs on two streams
-  const char *pipe = std::getenv("PPKMHD_PIPE");
-  d.pipe = pipe ? std::atoi(pipe) : 0;
-  if (d.pipe > ppk::kMaxSlabs) d.pipe = ppk::kMaxSlabs;
-  if (d.pipe < 0) d.pipe = 0;
+  // the single-grid fused step runs one configuration: the 16 x 4 x 4 half-step tile, runs of
+  // 256 cells in 4 x 4 row patches for the per-cell kernels, all face and edge solves in one
+  // launch (the batch selects its own tile and brick shapes, ppk_mhd_batch_autotune)
+  d.tile3 = 0;
+  d.brick = 0;
   return d;
 }
 
@@ -265,11 +261,6 @@ void free_all(ppk_mhd_t *h) {
   F(h->fw.emf);
   F(h->sur_counts);
   F(h->sur_flags);
-  if (h->aux.fork) (void)hipEventDestroy(h->aux.fork);
-  if (h->aux.join) (void)hipEventDestroy(h->aux.join);
-  for (hipEvent_t e : h->aux.slab)
-    if (e) (void)hipEventDestroy(e);
-  if (h->aux.stream) (void)hipStreamDestroy(h->aux.stream);
   delete h->timer;
   h->timer = nullptr;
   if (h->ctrl_host) (void)hipHostFree(h->ctrl_host);
@@ -365,16 +356,6 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
          balloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n);
   }
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
-  if (h->fused && h->dp.dim == 3 && (h->dp.riemann_mix >= 3 || h->dp.pipe > 1)) {
-    bool ev_ok = true;
-    for (int i = 0; i < h->dp.pipe && i < ppk::kMaxSlabs; ++i)
-      ev_ok = ev_ok && hipEventCreateWithFlags(&h->aux.slab[i], hipEventDisableTiming) == hipSuccess;
-    if (!ev_ok || hipStreamCreateWithFlags(&h->aux.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&h->aux.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->aux.join, hipEventDisableTiming) != hipSuccess)
-      return bail(PPK_ERR_HIP, "side stream creation failed");
-    h->fw.aux = &h->aux;
-  }
   if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * (ppk::CTRL_SLOTS + 1)) != hipSuccess)
     return bail(PPK_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
   if (hipMemsetAsync(h->U[0], 0, 8 * n * sizeof(double), h->stream) != hipSuccess ||
@@ -559,6 +540,9 @@ int ppk_mhd_dump_faces(ppk_mhd_t *h, ppk_faceset_t *set, int stride, int phase,
 
 int ppk_mhd_set_surrogate(ppk_mhd_t *h, const ppk_mlp_t *net, double gate_rho, double gate_p) {
   if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
+  // steps already queued keep the network (and gate) they were launched with: let them finish
+  // before the configuration changes, so the caller may then free the previous network
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
   if (!net) {
     h->fw.sur = nullptr;
     h->sur = ppk::SurrogateCfg{};

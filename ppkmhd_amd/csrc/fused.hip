@@ -4,14 +4,14 @@
 //                              face-field slopes and the Hancock half step of a tile of cells;
 //                              one compact record per cell (kRec3 = 38 / kRec2 = 26 doubles)
 //                              instead of the 18 (3D) / 8 (2D) full face and edge states
-//   k_face{3,2}d<D>, k_edge{3,2}d<E>
-//                              per cell, one launch per direction: the Riemann problem of its
-//                              lower face, the 2D HLLD EMF of its lower edge, the states
-//                              rebuilt from the records of the cell and its lower neighbours
-//                              (one solve per thread keeps register use low).  The right-side
-//                              face values of a cell (AR, dARy, ...) are bit for bit the
-//                              left-side values of its +1 neighbour, so records hold only the
-//                              left ones.
+//   k_riemann_mix{3,2}d         per cell of the sub-box, every face and edge solve of the step in
+//                              one launch (one solve per thread keeps register use low): the
+//                              Riemann problem of the cell's lower face in each direction, the 2D
+//                              HLLD EMF of its lower edges, the states rebuilt from the records
+//                              of the cell and its lower neighbours.  The right-side face values
+//                              of a cell (AR, dARy, ...) are bit for bit the left-side values of
+//                              its +1 neighbour, so records hold only the left ones.  With the
+//                              surrogate, k_edge{3,2}d solve the edges after the learned faces.
 //   k_update                   flux update + constrained transport, ghost copy, and the CFL
 //                              maximum of the new state (per-block partials) that the next
 //                              step's dt is taken from
@@ -33,21 +33,18 @@ constexpr int kRiemannBlock = 256;
 
 __device__ __forceinline__ bool active(const double *ctrl) { return ctrl[CTRL_ACTIVE] != 0.0; }
 
-// A launch covers the tiles lbase + [0, gridDim.x) of the full numbering and, of those, the
-// tile rows tz0 <= tz < tz1 (a z-slab of the pipelined step; the whole box otherwise).
 template <class T, class R>
 __global__ void __launch_bounds__(T::NT) k_half3d(const double *__restrict__ U,
                                                   R *__restrict__ rec, DevParams p,
-                                                  const double *ctrl, int ntx, int nty, int ntz,
-                                                  int lbase, int tz0, int tz1) {
+                                                  const double *ctrl, int ntx, int nty, int ntz) {
   if (!active(ctrl)) return;
   __shared__ R lds[T::LDS];
   R *sF = lds, *sQ = lds + 3 * T::NF, *sE = sQ + 8 * T::NQ;
   // tiles: x fastest, (y, z) tile rows in patches, each XCD a contiguous range
-  const int L = lbase + xcd_remap(blockIdx.x, gridDim.x);
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
   int ty, tz;
   patch_row(L / ntx, kPatch, kPatch, (nty + kPatch - 1) / kPatch, ty, tz);
-  if (ty >= nty || tz >= ntz || tz < tz0 || tz >= tz1) return;  // padding / other slabs
+  if (ty >= nty || tz >= ntz) return;  // padding of the last patches
   const int b0x = p.gw - 1 + (L % ntx) * T::TX;
   const int b0y = p.gw - 1 + ty * T::TY;
   const int b0z = p.gw - 1 + tz * T::TZ;
@@ -83,15 +80,6 @@ __global__ void __launch_bounds__(Tile2::NT) k_half2d(const double *__restrict__
   half2_record<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
 }
 
-// one thread per cell of the sub-box (bricks, XCD-contiguous); a launch covers one direction
-template <int D, class R>
-__global__ void __launch_bounds__(kRiemannBlock) k_face3d(const R *__restrict__ rec,
-                                                          R *__restrict__ flux, DevParams p,
-                                                          Brick bk, const double *ctrl) {
-  if (!active(ctrl)) return;
-  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
-  if (c >= 0) face3<D>(rec, flux, p, c);
-}
 // 4 waves per SIMD: the EMF solve fits 128 VGPRs without spilling
 template <int E, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
@@ -101,62 +89,26 @@ k_edge3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
   if (c >= 0) edge3<E>(rec, emf, p, c);
 }
-// All solves of the step in one launch: block B does solve type B % NTYPE (3D: faces x, y, z,
-// edges z, y, x; 2D: faces x, y, edge z) for the cells of run B / NTYPE.  Blocks are renumbered
-// per XCD first, so the six solves of a run execute on one XCD close in time: each record is
-// read from HBM about once and then from that XCD's L2, and every CU interleaves the
-// memory-bound face waves with the VALU-bound edge waves.
-// A launch covers the runs rbase + [0, gridDim.x / 6) and, of their cells, those whose z index
-// in the sub-box is in [kk0, kk1) (a z-slab of the pipelined step; the whole box otherwise).
-// RS: the face solver (enum ppk_riemann) as a compile-time constant, so the kernel holds only
-// the solver it runs.
-template <int W, int RS, class R>
-__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
-k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux,
-                R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl, int rbase,
-                int kk0, int kk1) {
+// All solves of the step in one launch: block B does solve type B % 6 (faces x, y, z, edges
+// z, y, x) for the cells of run B / 6.  Blocks are renumbered per XCD first, so the six solves
+// of a run execute on one XCD close in time: each record is read from HBM about once and then
+// from that XCD's L2, and every CU interleaves the memory-bound face waves with the VALU-bound
+// edge waves.  At 3 waves per SIMD the EMF solve needs no spill.  RS: the face solver (enum
+// ppk_riemann) as a compile-time constant, so the kernel holds only the solver it runs.
+template <int RS, class R>
+__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux, R *__restrict__ emf,
+                DevParams p, Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
   const int B = xcd_remap(blockIdx.x, gridDim.x);
-  int kk;
-  const long long c = sub_brick_cell(p, bk, rbase + B / 6, threadIdx.x, kk);
-  if (c < 0 || kk < kk0 || kk >= kk1) return;
+  const long long c = sub_brick_cell(p, bk, B / 6, threadIdx.x);
+  if (c < 0) return;
   switch (B % 6) {
     case 0: face3<0, RS>(rec, flux, p, c); break;
     case 1: face3<1, RS>(rec, flux, p, c); break;
     case 2: face3<2, RS>(rec, flux, p, c); break;
     case 3: edge3<2>(rec, emf, p, c); break;
     case 4: edge3<1>(rec, emf, p, c); break;
-    default: edge3<0>(rec, emf, p, c); break;
-  }
-}
-// The three face solves (one launch) and the three edge solves (another), for running the two
-// launches side by side on two streams: each keeps its own register budget (the faces need
-// fewer registers than the EMF solves, so more face waves fit on a SIMD).
-template <int W, class R>
-__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
-k_faces_mix3d(const R *__restrict__ rec, R *__restrict__ flux, DevParams p, Brick bk,
-              const double *ctrl) {
-  if (!active(ctrl)) return;
-  const int B = xcd_remap(blockIdx.x, gridDim.x);
-  const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
-  if (c < 0) return;
-  switch (B % 3) {
-    case 0: face3<0>(rec, flux, p, c); break;
-    case 1: face3<1>(rec, flux, p, c); break;
-    default: face3<2>(rec, flux, p, c); break;
-  }
-}
-template <int W, class R>
-__global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(W)))
-k_edges_mix3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
-              const double *ctrl) {
-  if (!active(ctrl)) return;
-  const int B = xcd_remap(blockIdx.x, gridDim.x);
-  const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
-  if (c < 0) return;
-  switch (B % 3) {
-    case 0: edge3<2>(rec, emf, p, c); break;
-    case 1: edge3<1>(rec, emf, p, c); break;
     default: edge3<0>(rec, emf, p, c); break;
   }
 }
@@ -176,14 +128,6 @@ k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
   }
 }
 
-template <int D, class R>
-__global__ void __launch_bounds__(kRiemannBlock) k_face2d(const R *__restrict__ rec,
-                                                          R *__restrict__ flux, DevParams p,
-                                                          Brick bk, const double *ctrl) {
-  if (!active(ctrl)) return;
-  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
-  if (c >= 0) face2<D>(rec, flux, p, c);
-}
 template <class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_edge2d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
@@ -230,22 +174,23 @@ __global__ void __launch_bounds__(kCflBlock) k_cfl(const double *__restrict__ U,
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // the interleaved face/edge launch with the solver as a compile-time constant
-template <int W, class R>
-void launch_mix3d(const DevParams &p, unsigned grid, const R *rec, R *flux, R *emf, Brick bk,
-                  const double *ctrl, int rbase, int kk0, int kk1, hipStream_t s) {
-  const dim3 g(grid), b(kRiemannBlock);
+template <class R>
+void launch_mix3d(const DevParams &p, const R *rec, R *flux, R *emf, const double *ctrl,
+                  hipStream_t s) {
+  const Brick bk = sub_bricks(p);
+  const dim3 g((unsigned)(6 * bk.count())), b(kRiemannBlock);
   switch (p.riemann) {
     case PPK_RIEMANN_HLLD:
-      hipLaunchKernelGGL((k_riemann_mix3d<W, PPK_RIEMANN_HLLD, R>), g, b, 0, s, rec, flux, emf,
-                         p, bk, ctrl, rbase, kk0, kk1);
+      hipLaunchKernelGGL((k_riemann_mix3d<PPK_RIEMANN_HLLD, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl);
       break;
     case PPK_RIEMANN_HLL:
-      hipLaunchKernelGGL((k_riemann_mix3d<W, PPK_RIEMANN_HLL, R>), g, b, 0, s, rec, flux, emf, p,
-                         bk, ctrl, rbase, kk0, kk1);
+      hipLaunchKernelGGL((k_riemann_mix3d<PPK_RIEMANN_HLL, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl);
       break;
     default:
-      hipLaunchKernelGGL((k_riemann_mix3d<W, PPK_RIEMANN_LLF, R>), g, b, 0, s, rec, flux, emf, p,
-                         bk, ctrl, rbase, kk0, kk1);
+      hipLaunchKernelGGL((k_riemann_mix3d<PPK_RIEMANN_LLF, R>), g, b, 0, s, rec, flux, emf, p,
+                         bk, ctrl);
       break;
   }
 }
@@ -269,80 +214,16 @@ void launch_mix2d(const DevParams &p, unsigned grid, const R *rec, R *flux, R *e
   }
 }
 
-// tiles of rows tz0 <= tz < tz1 (tz1 < 0: all); launches the blocks of the z-patches they
-// fall in
-template <class T, class R>
+// the half step over every record tile: x fastest, (y, z) tile rows in 4 x 4 patches
+template <class R>
 void launch_half3(const DevParams &p, const double *Uin, R *rec, const double *ctrl,
-                  hipStream_t s, int tz0 = 0, int tz1 = -1) {
+                  hipStream_t s) {
+  using T = Tile3;
   const int ntx = ceil_div(p.nx + 2, T::TX), nty = ceil_div(p.ny + 2, T::TY);
   const int ntz = ceil_div(p.nz + 2, T::TZ);
-  if (tz1 < 0 || tz1 > ntz) tz1 = ntz;
-  if (tz0 >= tz1) return;
-  const int kp0 = tz0 / kPatch, kp1 = ceil_div(tz1, kPatch);
   const int prow = ceil_div(nty, kPatch) * kPatch * kPatch;  // tile rows per z-patch
-  hipLaunchKernelGGL((k_half3d<T, R>), dim3(ntx * prow * (kp1 - kp0)), dim3(T::NT), 0, s, Uin,
-                     rec, p, ctrl, ntx, nty, ntz, ntx * prow * kp0, tz0, tz1);
-}
-
-template <class R>
-void launch_half3_any(const DevParams &p, const double *Uin, R *rec, const double *ctrl,
-                      hipStream_t s, int tz0 = 0, int tz1 = -1) {
-  switch (p.tile3) {
-    case 1: launch_half3<Tile3b>(p, Uin, rec, ctrl, s, tz0, tz1); break;
-    case 2: launch_half3<Tile3c>(p, Uin, rec, ctrl, s, tz0, tz1); break;
-    case 3: launch_half3<Tile3d>(p, Uin, rec, ctrl, s, tz0, tz1); break;
-    case 4: launch_half3<Tile3e>(p, Uin, rec, ctrl, s, tz0, tz1); break;
-    default: launch_half3<Tile3>(p, Uin, rec, ctrl, s, tz0, tz1); break;
-  }
-}
-// tile depth in z of the selected half-step tile
-inline int half3_tz(const DevParams &p) {
-  switch (p.tile3) {
-    case 1: return Tile3b::TZ;
-    case 2: return Tile3c::TZ;
-    case 3: return Tile3d::TZ;
-    case 4: return Tile3e::TZ;
-    default: return Tile3::TZ;
-  }
-}
-
-// The 3D step in z-slabs: the half step of slab s (its tile rows) on the side stream, then on
-// the main stream, once it is done, the face and edge solves of the sub-box cells whose
-// records lie in slabs <= s; the half step of slab s+1 overlaps those solves.  Slab s of the
-// solves covers z = [K_s, K_{s+1}) with K_s = TZ * T_s - TZ (T_s the first tile row of half
-// slab s): its records are the planes [K_s, K_{s+1} + 1) counted from gw - 1, all in tile rows
-// < T_{s+1}.
-template <class R>
-hipError_t pipelined_half_riemann(const DevParams &p, const double *Uin, R *rec, R *flux, R *emf,
-                                  const double *ctrl, const AuxStream &aux, hipStream_t s) {
-  const int TZ = half3_tz(p);
-  const int ntz = ceil_div(p.nz + 2, TZ);
-  const int S = p.pipe < ntz ? p.pipe : ntz;
-  const Brick bk = sub_bricks(p);  // flattened runs, rows in 4 x 4 (y, z) patches
-  const long long runs_per_zpatch =
-      (long long)ceil_div(p.ny + 1, kPatch) * kPatch * kPatch * (p.nx + 1);
-  hipError_t e = hipEventRecord(aux.fork, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(aux.stream, aux.fork, 0);
-  if (e != hipSuccess) return e;
-  for (int sl = 0; sl < S; ++sl) {
-    const int t0 = (int)((long long)ntz * sl / S), t1 = (int)((long long)ntz * (sl + 1) / S);
-    launch_half3_any<R>(p, Uin, rec, ctrl, aux.stream, t0, t1);
-    e = hipEventRecord(aux.slab[sl], aux.stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, aux.slab[sl], 0);
-    if (e != hipSuccess) return e;
-    const int k0 = sl == 0 ? 0 : TZ * t0 - TZ;
-    const int k1 = sl == S - 1 ? p.nz + 1 : TZ * t1 - TZ;
-    if (k0 >= k1) continue;
-    // runs of the z-patches [k0 / 4, ceil(k1 / 4)) of the flattened numbering
-    const long long n0 = (long long)(k0 / kPatch) * runs_per_zpatch;
-    const long long n1 = (long long)ceil_div(k1, kPatch) * runs_per_zpatch;
-    const long long r0 = n0 / 256;
-    long long r1 = (n1 + 255) / 256;
-    if (r1 > bk.count()) r1 = bk.count();
-    launch_mix3d<3, R>(p, (unsigned)(6 * (r1 - r0)), rec, flux, emf, bk, ctrl, (int)r0, k0, k1,
-                       s);
-  }
-  return hipGetLastError();
+  hipLaunchKernelGGL((k_half3d<T, R>), dim3(ntx * prow * ceil_div(ntz, kPatch)), dim3(T::NT), 0,
+                     s, Uin, rec, p, ctrl, ntx, nty, ntz);
 }
 
 // the step's three stages in arithmetic type R (double, or float for the FP32-bulk form);
@@ -350,21 +231,11 @@ hipError_t pipelined_half_riemann(const DevParams &p, const double *Uin, R *rec,
 template <class R>
 hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *rec, R *flux,
                         R *emf, double *parts, const double *ctrl, int fold_cfl,
-                        const SurrogateCfg *sur, const AuxStream *aux, hipStream_t s,
-                        StageTimer *tm) {
-  if (p.dim == 3 && p.pipe > 1 && aux && aux->slab[0] && !sur && p.riemann_mix == 2 &&
-      p.brick == 0) {
-    // half step and face/edge solves overlapped by z-slabs; timed together
-    if (tm) tm->mark(ST_FUSED_RIEMANN, true);
-    const hipError_t e = pipelined_half_riemann<R>(p, Uin, rec, flux, emf, ctrl, *aux, s);
-    if (e != hipSuccess) return e;
-    if (tm) tm->mark(ST_FUSED_RIEMANN, false);
-  } else if (p.dim == 3) {
-    if (tm) tm->mark(ST_FUSED_HALF, true);
-    launch_half3_any<R>(p, Uin, rec, ctrl, s);
-    if (tm) tm->mark(ST_FUSED_HALF, false);
-    const Brick bk = sub_bricks(p);
-    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
+                        const SurrogateCfg *sur, hipStream_t s, StageTimer *tm) {
+  const Brick bk = sub_bricks(p);
+  const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
+  if (p.dim == 3) {
+    PPK_TIMED(tm, ST_FUSED_HALF, launch_half3<R>(p, Uin, rec, ctrl, s));
     if (tm) tm->mark(ST_FUSED_RIEMANN, true);
     if (sur) {  // learned face fluxes, then the exact edges
       const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s);
@@ -372,33 +243,8 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
       hipLaunchKernelGGL((k_edge3d<2, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
       hipLaunchKernelGGL((k_edge3d<1, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
       hipLaunchKernelGGL((k_edge3d<0, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
-    } else if (p.riemann_mix >= 3 && aux) {  // faces and edges side by side on two streams
-      const dim3 g3((unsigned)(3 * bk.count()));
-      hipError_t e = hipEventRecord(aux->fork, s);
-      if (e == hipSuccess) e = hipStreamWaitEvent(aux->stream, aux->fork, 0);
-      if (e != hipSuccess) return e;
-      // faces at 4 waves per SIMD (3: 5, spilling 60 bytes), edges at 3 (no spill)
-      if (p.riemann_mix == 3)
-        hipLaunchKernelGGL((k_faces_mix3d<4, R>), g3, b, 0, aux->stream, rec, flux, p, bk, ctrl);
-      else
-        hipLaunchKernelGGL((k_faces_mix3d<5, R>), g3, b, 0, aux->stream, rec, flux, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edges_mix3d<3, R>), g3, b, 0, s, rec, emf, p, bk, ctrl);
-      e = hipEventRecord(aux->join, aux->stream);
-      if (e == hipSuccess) e = hipStreamWaitEvent(s, aux->join, 0);
-      if (e != hipSuccess) return e;
-    } else if (p.riemann_mix == 2) {  // 3 waves per SIMD, no spill
-      launch_mix3d<3, R>(p, (unsigned)(6 * bk.count()), rec, flux, emf, bk, ctrl, 0, 0, 1 << 30,
-                         s);
-    } else if (p.riemann_mix) {
-      launch_mix3d<4, R>(p, (unsigned)(6 * bk.count()), rec, flux, emf, bk, ctrl, 0, 0, 1 << 30,
-                         s);
     } else {
-      hipLaunchKernelGGL((k_face3d<0, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
-      hipLaunchKernelGGL((k_face3d<1, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
-      hipLaunchKernelGGL((k_face3d<2, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edge3d<2, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edge3d<1, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edge3d<0, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      launch_mix3d<R>(p, rec, flux, emf, ctrl, s);
     }
     if (tm) tm->mark(ST_FUSED_RIEMANN, false);
   } else {
@@ -406,19 +252,13 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
     PPK_TIMED(tm, ST_FUSED_HALF,
               hipLaunchKernelGGL((k_half2d<R>), dim3(ntx * nty), dim3(Tile2::NT), 0, s, Uin, rec,
                                  p, ctrl, ntx));
-    const Brick bk = sub_bricks(p);
-    const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
     if (tm) tm->mark(ST_FUSED_RIEMANN, true);
     if (sur) {
       const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL((k_edge2d<R>), g, b, 0, s, rec, emf, p, bk, ctrl);
-    } else if (p.riemann_mix) {
-      launch_mix2d<R>(p, (unsigned)(3 * bk.count()), rec, flux, emf, bk, ctrl, s);
     } else {
-      hipLaunchKernelGGL((k_face2d<0, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
-      hipLaunchKernelGGL((k_face2d<1, R>), g, b, 0, s, rec, flux, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edge2d<R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      launch_mix2d<R>(p, (unsigned)(3 * bk.count()), rec, flux, emf, bk, ctrl, s);
     }
     if (tm) tm->mark(ST_FUSED_RIEMANN, false);
   }
@@ -440,10 +280,10 @@ hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout, Fu
   if (p.bulk32)
     return godunov_impl<float>(p, Uin, Uout, static_cast<float *>(w.rec),
                                static_cast<float *>(w.flux), static_cast<float *>(w.emf), parts,
-                               ctrl, fold_cfl, w.sur, w.aux, s, tm);
+                               ctrl, fold_cfl, w.sur, s, tm);
   return godunov_impl<double>(p, Uin, Uout, static_cast<double *>(w.rec),
                               static_cast<double *>(w.flux), static_cast<double *>(w.emf), parts,
-                              ctrl, fold_cfl, w.sur, w.aux, s, tm);
+                              ctrl, fold_cfl, w.sur, s, tm);
 }
 
 template <class R>
@@ -451,7 +291,7 @@ hipError_t half_impl(const DevParams &p, const double *U, R *rec, const double *
                      hipStream_t s) {
   using namespace fusedk;
   if (p.dim == 3) {
-    launch_half3_any<R>(p, U, rec, ctrl, s);
+    launch_half3<R>(p, U, rec, ctrl, s);
   } else {
     const int ntx = ceil_div(p.nx + 2, Tile2::TX), nty = ceil_div(p.ny + 2, Tile2::TY);
     hipLaunchKernelGGL((k_half2d<R>), dim3(ntx * nty), dim3(Tile2::NT), 0, s, U, rec, p, ctrl,

@@ -128,13 +128,11 @@ struct HalfTile3 {
   PPK_HD static int q(int x, int y, int z) { return x + QX * (y + QY * z); }
   PPK_HD static int e(int x, int y, int z) { return x + EX * (y + EY * z); }
 };
-// compiled 3D tile shapes, selected by DevParams::tile3 (0 is the default)
+// compiled 3D tile shapes: Tile3 for single grids, all three for the batch's autotuner
 using Tile3 = HalfTile3<16, 4, 4>;   // 256 threads, 74 KB LDS
-using Tile3b = HalfTile3<8, 8, 8>;   // 512 threads, 113 KB
-using Tile3c = HalfTile3<16, 8, 4>;  // 512 threads, 123 KB
 using Tile3d = HalfTile3<8, 8, 4>;   // 256 threads, 68 KB
 using Tile3e = HalfTile3<32, 4, 2>;  // 256 threads, 94 KB
-static_assert(Tile3c::LDS * sizeof(double) <= 160 * 1024, "3D tile exceeds the LDS of a CU");
+static_assert(Tile3e::LDS * sizeof(double) <= 160 * 1024, "3D tile exceeds the LDS of a CU");
 
 // the global loads of phase 1, held in registers until stashed in LDS (so a persistent block
 // can issue the next tile's loads before it computes the current tile's records)

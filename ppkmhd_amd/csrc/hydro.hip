@@ -701,26 +701,17 @@ int hydro_fused_parts(const HydroDev &h) {
 template <int RS, int GR>
 void launch_fused_gr(const HydroDev &h, const double *Uin, double *Uout, double *parts,
                      const double *ctrl, int fold_cfl, hipStream_t s) {
-  // occupancy variant (tuning): 0 = the compiler's register budget, 1 = 4 waves per SIMD
   if (h.dim == 3) {
     if constexpr (RS != PPK_RIEMANN_HLL) {  // no 3D HLL (refused at create)
       const dim3 g(fused3d_blocks(h)), b(kW3 * 64);
-      if (h.occupancy == 1)
-        hipLaunchKernelGGL((k_hfused3d<5, RS, 4, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
-                           fold_cfl);
-      else
-        hipLaunchKernelGGL((k_hfused3d<5, RS, 1, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
-                           fold_cfl);
+      hipLaunchKernelGGL((k_hfused3d<5, RS, 1, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+                         fold_cfl);
     }
   } else {
     const int waves = fused2d_waves(h);
     const dim3 g((waves + kWaves2 - 1) / kWaves2), b(kWaves2 * 64);
-    if (h.occupancy == 1)
-      hipLaunchKernelGGL((k_hfused2d<4, RS, 4, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
-                         fold_cfl);
-    else
-      hipLaunchKernelGGL((k_hfused2d<4, RS, 1, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
-                         fold_cfl);
+    hipLaunchKernelGGL((k_hfused2d<4, RS, 1, GR>), g, b, 0, s, Uin, Uout, h, parts, ctrl,
+                       fold_cfl);
   }
 }
 

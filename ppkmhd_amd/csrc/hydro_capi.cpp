@@ -38,6 +38,14 @@ const HydroTable kHydroStrict = {strict::hydro_boundaries,    strict::hydro_prim
 int validate_hydro(const ppk_mhd_params *p, std::string &why) {
   if (!p) { why = "params is NULL"; return PPK_ERR_INVALID_ARGUMENT; }
   if (p->dim != 2 && p->dim != 3) { why = "dim must be 2 or 3"; return PPK_ERR_INVALID_ARGUMENT; }
+  if (p->implementation_version < 0 || p->implementation_version > 2) {
+    why = "implementationVersion must be 0, 1 or 2 (HydroParams.cpp:446-451)";
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  if (p->implementation_version == 2) {  // SolverHydroMuscl has versions 0 and 1 only
+    why = "implementationVersion 2 has no hydro scheme (SolverHydroMuscl.cpp:277-319)";
+    return PPK_ERR_UNSUPPORTED;
+  }
   if (p->ghost_width != 2) {
     why = "ghost_width must be 2 for the hydro solvers (HydroParams.cpp:42-55)";
     return PPK_ERR_INVALID_ARGUMENT;
@@ -125,9 +133,6 @@ HydroDev make_hydro_dev(const ppk_mhd_params &p) {
   const char *strip = std::getenv("PPKMHD_HYDRO_STRIP");
   const int rows = strip ? std::atoi(strip) : 0;
   d.strip_rows = rows > 0 ? rows : (p.dim == 3 ? 48 : 64);  // measured best at 8192 x 4096 / 512 x 384 x 256
-  // tuning knob: PPKMHD_HYDRO_OCC=1 selects the 4-waves-per-SIMD builds of the fused kernels
-  const char *occ = std::getenv("PPKMHD_HYDRO_OCC");
-  d.occupancy = occ ? std::atoi(occ) : 0;
   return d;
 }
 

@@ -15,7 +15,6 @@ struct HydroDev {
   int isize, jsize, ksize;
   int bc[6];
   int strip_rows;  // fused kernels: output rows (2D) / planes (3D) one wave or block marches
-  int occupancy;   // fused kernels: 0 compiler's register budget, 1 four waves per SIMD
   long long plane, vstride;
   double dx, dy, dz;
   double idx, idy, idz;  // 1 / dx ... (used only where the build shares reciprocals)

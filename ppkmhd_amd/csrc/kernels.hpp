@@ -76,20 +76,12 @@ struct SurrogateCfg {
   unsigned long long *counters = nullptr;
   unsigned char *flags = nullptr;  // dim x cells: faces left to the HLLD pass (device scratch)
 };
-// a second stream of default priority and the two events that fork work onto it and join it
-constexpr int kMaxSlabs = 16;
-struct AuxStream {
-  hipStream_t stream = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-  hipEvent_t slab[kMaxSlabs] = {};  // z-slab pipeline: half step of slab s done
-};
 // element type: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {
   void *rec = nullptr;   // kRec{2,3} x cells
   void *flux = nullptr;  // dim x kFluxComp{2,3} x cells
   void *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
   const SurrogateCfg *sur = nullptr;  // surrogate face solves when set
-  const AuxStream *aux = nullptr;     // side stream for concurrent face / edge launches
 };
 
 // dt modes of the set_dt kernel
@@ -112,7 +104,8 @@ inline int cfl_blocks(const DevParams &p) {
 // Shape 0 (default) is a run of 256 consecutive cells of the flattened range (one wave = 64
 // cells of one x row); its rows are numbered in patches of 4 x 4 (y, z) rows in 3D, so the
 // rows a cell's y and z neighbours live in were visited a few rows earlier (L2 hits).
-// Shapes 1-3 are 3D bricks 64 x 2 x 2, 16 x 4 x 4, 32 x 4 x 2 (2D: 64 x 4), kept for tuning.
+// Shapes 1-3 are 3D bricks 64 x 2 x 2, 16 x 4 x 4, 32 x 4 x 2 (2D: 64 x 4), among which the
+// batch's autotuner picks; single grids use shape 0.
 constexpr int kPatch = 4;
 struct Brick {
   int bx, by, bz;  // brick extents; bx = 0: flattened runs, by x bz = row patch

@@ -37,12 +37,9 @@ struct DevParams {
   long long vstride;    // isize * jsize * ksize (distance between variables)
   double dx, dy, dz;
   double gamma0, slope_type, smallr, smallc, smallp, cIso, cfl;
-  int tile3;            // 3D half-step tile shape (index into the compiled shapes), tuning only
-  int brick;            // block-to-cells shape of the fused per-cell kernels, tuning only
-  int riemann_mix;      // 1, 2: all face and edge solves in one interleaved launch (4 / 3 waves)
+  int tile3;            // 3D half-step tile shape (batch tables; 0 for single grids)
+  int brick;            // block-to-cells shape of the fused per-cell kernels (batch tables)
   int bulk32;           // FP32-bulk form: half step, records, face/edge solves in float
-  int pipe;             // 3D: > 1 pipelines the step in that many z-slabs (half step of slab
-                        // s+1 on a side stream while the face/edge solves of slab s run)
 };
 
 template <class R>

@@ -1,6 +1,8 @@
 // C ABI of the surrogate network and its training samples (include/ppkmhd_amd.h, ppk_mlp_*,
 // ppk_faceset_*).  Every call is synchronous: it returns once its GPU work has finished, so a
-// solver that uses the network on its own stream always sees a complete parameter set.
+// solver that uses the network on its own stream always sees a complete parameter set.  A call
+// that rewrites the parameters first waits for all work on the device (solver steps on other
+// streams may still be reading the packed weights it is about to overwrite).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,7 +50,15 @@ void free_net(ppk_mlp_t *n) {
   if (n->stream) (void)hipStreamDestroy(n->stream);
 }
 
+// wait for every stream of the device: a solver step in flight may read the packed weights
+int quiesce_readers(ppk_mlp_t *n) {
+  NHIP(n, hipDeviceSynchronize());
+  return PPK_OK;
+}
+
 int upload_params(ppk_mlp_t *n, const float *P) {
+  int rc = quiesce_readers(n);
+  if (rc != PPK_OK) return rc;
   NHIP(n, hipMemcpyAsync(n->P, P, sizeof(float) * ppk::kMlpParams, hipMemcpyHostToDevice,
                          n->stream));
   NHIP(n, ppk::mlp::pack(n->P, n->packed, n->stream));
@@ -258,6 +268,7 @@ int ppk_mlp_train(ppk_mlp_t *net, const ppk_faceset_t *set, int steps, int batch
                  "steps >= 0, batch a positive multiple of 256 and lr > 0 are required");
   if (set->n < 1) return nfail(net, PPK_ERR_INVALID_ARGUMENT, "face set is empty");
   int rc = ensure_scratch(net, batch);
+  if (rc == PPK_OK) rc = quiesce_readers(net);
   if (rc != PPK_OK) return rc;
   for (int i = 0; i < steps; ++i) {
     NHIP(net, ppk::mlp::train_step(net->packed, net->P, net->m, net->v, set->X, set->T, nullptr,
@@ -276,6 +287,7 @@ int ppk_mlp_gradient(ppk_mlp_t *net, const float *X, const float *T, size_t n, f
   if (count != (size_t)ppk::kMlpTrained)
     return nfail(net, PPK_ERR_INVALID_ARGUMENT, "count must be the trained-parameter count");
   int rc = ensure_scratch(net, (long long)n);
+  if (rc == PPK_OK) rc = quiesce_readers(net);
   if (rc != PPK_OK) return rc;
   float *dX = nullptr, *dT = nullptr, *dG = nullptr;
   long long *idx = nullptr;

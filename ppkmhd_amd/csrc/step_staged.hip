@@ -416,7 +416,13 @@ __global__ void __launch_bounds__(kSetDtBlock) k_set_dt(double *ctrl,
   __shared__ double red[kSetDtBlock / 64];
   if (ctrl[CTRL_ACTIVE] == 0.0) return;
   if (mode == SETDT_FIXED) {
-    if (threadIdx.x == 0) ctrl[CTRL_DT] = fixed_dt;
+    // a fixed dt is clamped to tEnd like the CFL one (SolverBase.cpp:173-177), so a run of
+    // fixed steps stops exactly at tEnd
+    if (threadIdx.x == 0) {
+      double dt = fixed_dt;
+      if (ctrl[CTRL_T] + dt > ctrl[CTRL_TEND]) dt = ctrl[CTRL_TEND] - ctrl[CTRL_T];
+      ctrl[CTRL_DT] = dt;
+    }
     return;
   }
   // eight independent loads in flight per thread: the partials can number ~10^5

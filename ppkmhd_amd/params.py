@@ -39,7 +39,7 @@ def make_params(dim, nx, ny, nz=1, *, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zm
     p.n_step_max = n_step_max
     p.precision = precision
     p.kernel_path = kernel_path
-    p.reserved = 0
+    p.implementation_version = 0
     p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = (r(v) for v in (xmin, xmax, ymin, ymax,
                                                                       zmin, zmax))
     p.gamma0 = r(gamma0)

@@ -44,6 +44,8 @@ class SolverMHDMuscl:
         if getattr(self, "_h", None):
             lib().ppk_mhd_destroy(self._h)
             self._h = None
+        # an attached surrogate network may be freed only once the handle is gone
+        self._sur_net = None
 
     def __del__(self):
         try:

@@ -157,9 +157,13 @@ def dump_faces(solver, faceset, stride=1, phase=0):
 
 
 def attach(solver, net, gate_rho=0.1, gate_p=0.1):
-    """Use `net` for the face solves of `solver` (None: plain HLLD)."""
+    """Use `net` for the face solves of `solver` (None: plain HLLD).
+
+    The library keeps only the network's device weights, so the solver holds a reference to
+    `net` until it is detached: the weights cannot be freed while steps may still read them."""
     solver._check(lib().ppk_mhd_set_surrogate(solver._h, net.handle if net is not None else None,
                                               float(gate_rho), float(gate_p)))
+    solver._sur_net = net
 
 
 def counts(solver, reset=False):

@@ -216,6 +216,18 @@ implementationVersion=2
 """
     p, pr, name = params_from_ini(text=txt)
     assert p.slope_type == 2.0 and p.gamma0 == f32(1.666)
+    # settings/test_mhd_orszag_tang_2D.ini:32 selects implementation version 2, a different
+    # scheme (SolverMHDMuscl.cpp:430-456) that this library does not provide: creating the
+    # solver is refused (parameter checks run before any device call), never silently run as
+    # version 0
+    assert p.implementation_version == 2
+    h = C.c_void_p()
+    assert lib().ppk_mhd_create(C.byref(p), C.byref(h)) == A.PPK_ERR_UNSUPPORTED
+    assert not h.value
+    assert "implementationVersion 2" in lib().ppk_mhd_last_error(None).decode()
+    p.implementation_version = 3  # outside 0..2: the reference exits (HydroParams.cpp:446-451)
+    assert lib().ppk_mhd_create(C.byref(p), C.byref(h)) == A.PPK_ERR_INVALID_ARGUMENT
+    assert params_from_ini(text=OT_INI)[0].implementation_version == 0
 
 
 def test_batch_create_rejects_bad_members_before_any_device_call():

@@ -253,6 +253,11 @@ def test_run_stops_at_finished(gpu):
     g2.run(1000)
     t2, dt2, it2 = g2.time()
     assert t2 == pytest.approx(0.01, abs=1e-15) and g2.finished()
+    # a fixed dt is clamped the same way: 0.003 x 4 would pass tEnd = 0.01
+    g3 = solver(p2, pr2)
+    g3.run(10, dt=0.003, dt_mode=A.DT_FIXED)
+    t3, dt3, it3 = g3.time()
+    assert it3 == 4 and t3 == pytest.approx(0.01, abs=1e-15) and g3.finished()
 
 
 def test_compute_dt_matches_oracle(oracle, gpu):

@@ -36,7 +36,9 @@ def test_mlp_inference_matches_oracle(gpu):
     err = np.abs(Y - ref).max() / np.abs(ref).max()
     err_exact = np.abs(Y - exact).max() / np.abs(exact).max()
     print(f"inference: max error vs bf16 oracle {err:.2e}, vs exact {err_exact:.2e}")
-    assert err <= 1e-2, err
+    # measured 9.5e-5: the bar is 10x that, tight enough that a dropped bias or a 1 % slip in
+    # one layer's weights fails (test_surrogate_cpu.py shows those move the output by more)
+    assert err <= 1e-3, err
     assert err_exact <= 5e-2, err_exact
 
 
@@ -58,9 +60,9 @@ def test_mlp_gradient_matches_oracle(gpu):
     for name, (a, b) in blocks.items():
         r = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
         print(f"gradient {name}: relative L2 error vs bf16 oracle {r:.2e}")
-        assert r <= 2e-2, (name, r)
+        assert r <= 2e-4, (name, r)  # measured 1.1e-5
     print(f"gradient: relative L2 vs bf16 oracle {rel:.2e}, vs exact {rel_exact:.2e}")
-    assert rel <= 1e-2 and rel_exact <= 5e-2
+    assert rel <= 2e-4 and rel_exact <= 5e-2
     # the parameters did not move
     assert np.array_equal(net.get_params(), P)
 

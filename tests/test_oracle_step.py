@@ -274,3 +274,19 @@ def test_linear_wave_second_order(oracle, wave_type, tend):
     e32 = _wave_l1(oracle, 32, wave_type, tend)
     order = np.log2(e16 / e32)
     assert order > 1.7, (e16, e32, order)
+
+
+def test_fixed_dt_is_clamped_to_tend(oracle):
+    """A fixed dt is clamped to tEnd - t as the CFL dt is (SolverBase.cpp:173-177): four steps
+    of 0.003 towards tEnd = 0.01 take 0.003, 0.003, 0.003, 0.001."""
+    p = make_params(2, 16, 16, gamma0=5 / 3, cfl=0.8, slope_type=1.0, tEnd=0.01)
+    o = oracle.Solver(p)
+    o.init_problem(default_problem(A.PROBLEM_ORSZAG_TANG, p))
+    for _ in range(3):
+        o.step(0.003, A.DT_FIXED)
+    assert not o.finished()
+    o.step(0.003, A.DT_FIXED)
+    t, dt, it = o.time()
+    # tEnd holds the float-rounded 0.01 of an .ini real (ConfigMap.cpp:37-46)
+    assert it == 4 and dt == pytest.approx(p.tEnd - 0.009, abs=1e-15) and t == p.tEnd
+    assert o.finished()

@@ -142,3 +142,67 @@ def test_llf_flux_is_admissible_and_a_bad_flux_is_not():
         bad = F.copy()
         bad[S.ID] += 2.0 * s * max(ql[S.ID], qr[S.ID])
         assert not S.admissible(ql, qr, bad, g)
+
+
+def test_attached_network_outlives_its_last_user_reference(monkeypatch):
+    """attach() keeps the network alive while the solver may use its device weights: the library
+    stores only a raw pointer to them, so dropping the caller's last reference must not free
+    the network; detaching (or closing the solver) releases it."""
+    import gc
+    import weakref
+
+    from ppkmhd_amd import surrogate as SG
+
+    class StubLib:
+        def ppk_mhd_set_surrogate(self, h, net, gate_rho, gate_p):
+            return 0
+
+    class StubNet:
+        handle = object()
+
+    class StubSolver:
+        _h = object()
+
+        def _check(self, rc):
+            assert rc == 0
+
+    monkeypatch.setattr(SG, "lib", lambda: StubLib())
+    solver, net = StubSolver(), StubNet()
+    ref = weakref.ref(net)
+    SG.attach(solver, net, 0.01, 0.01)
+    del net
+    gc.collect()
+    assert ref() is not None, "the attached network was released while still in use"
+    SG.attach(solver, None)
+    gc.collect()
+    assert ref() is None, "detaching should release the network"
+
+
+def test_layer_slips_exceed_the_gpu_parity_bars():
+    """The GPU network is held to 1e-3 of the bf16 oracle (inference) and 2e-4 (gradient
+    blocks).  Plausible slips of one layer move the oracle's own output and gradient by more
+    than those bars, so a kernel carrying one of them would fail the GPU tests."""
+    P = random_params(11)
+    X = np.random.default_rng(12).normal(size=(1000, 16)).astype(np.float32)
+    T = np.random.default_rng(14).normal(size=(1000, 8)).astype(np.float32)[:512]
+    ref, _ = S.mlp_forward(P, X, rounding=True)
+    gref = S.mlp_grad(P, X[:512], T, rounding=True)
+
+    def slips():
+        for a, b in ((S.B1, S.W2), (S.B2, S.W3), (S.B3, S.N_TRAINED)):
+            q = P.copy()
+            q[a:b] = 0.0  # a dropped bias
+            yield q
+        for a, b in ((S.W1, S.B1), (S.W2, S.B2), (S.W3, S.B3)):
+            q = P.copy()
+            q[a:b] *= 1.01  # one layer's weights 1 % off
+            yield q
+        q = P.copy()  # layer 2 applied transposed
+        q[S.W2:S.B2] = P[S.W2:S.B2].reshape(128, 128).T.ravel()
+        yield q
+
+    for q in slips():
+        y, _ = S.mlp_forward(q, X, rounding=True)
+        assert np.abs(y - ref).max() / np.abs(ref).max() > 1e-3
+        g = S.mlp_grad(q, X[:512], T, rounding=True)
+        assert np.linalg.norm(g - gref) / np.linalg.norm(gref) > 2e-4
