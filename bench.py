@@ -119,14 +119,62 @@ def workload_params(n, path, precision):
     return p, pr
 
 
-def cpu_baseline(n, steps=1):
-    """The oracle (C++ OpenMP restatement of the reference's step) on the host cores, on a
-    bounded sample of the same workload: one warm-up step, then `steps` timed steps."""
-    from oracle import pyoracle as O
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
-    O.build()
-    p, pr = workload_params(n, A.PATH_AUTO, A.FP64)
-    s = O.Solver(p)
+
+def baseline_env():
+    """OpenMP settings of the CPU baseline: every host core the process may use (or the
+    host's OMP_NUM_THREADS), threads spread over cores and bound to them."""
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
+    env["OMP_PROC_BIND"] = "spread"
+    env["OMP_PLACES"] = "cores"
+    return env
+
+
+def run_baseline_child(args):
+    """The CPU baseline runs in a child process, so that its OpenMP settings apply from the
+    start of its OpenMP runtime; returns the child's JSON record."""
+    import subprocess
+    env = baseline_env()
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"] +
+                       [str(a) for a in args], env=env, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed: {r.stderr[-2000:]}")
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    rec["sample"] += (f"; {rec.pop('threads')} OpenMP threads (OMP_PROC_BIND={env['OMP_PROC_BIND']}, "
+                      f"OMP_PLACES={env['OMP_PLACES']}) on {cpu_model()}, g++ -O3 -march=native "
+                      f"-ffp-contract=off")
+    return rec
+
+
+def cpu_baseline_child(kind, n, steps):
+    """Child side: the oracle built for this host, one warm-up step, then `steps` timed steps."""
+    if kind == "mhd":
+        from oracle import pyoracle as O
+        p, pr = workload_params(n, A.PATH_AUTO, A.FP64)
+        s = O.Solver(p, native=True)
+        threads = O.threads(native=True)
+        cells = n ** 3
+        sample = (f"oracle (C++ OpenMP restatement of the reference impl-0 step), same {n}^3 "
+                  f"blast workload, 1 warm-up + {steps} timed step(s)")
+    else:
+        from oracle import pyhydro as H
+        dim = int(kind[-2])
+        p, pr = hydro_workload(dim, n, A.PATH_AUTO, A.FP64)
+        s = H.Solver(p, native=True)
+        threads = H.threads(native=True)
+        cells = p.nx * p.ny * (p.nz if dim == 3 else 1)
+        sample = (f"hydro oracle on the {p.nx} x {p.ny}" + (f" x {p.nz}" if dim == 3 else "") +
+                  f" problem, 1 warm-up + {steps} timed step(s)")
     s.init_problem(pr)
     s.step(0, A.DT_CFL)
     t0 = time.perf_counter()
@@ -134,11 +182,14 @@ def cpu_baseline(n, steps=1):
         s.step(0, A.DT_CFL)
     dt = time.perf_counter() - t0
     s.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (C++ OpenMP restatement of the reference impl-0 step), same "
-                      f"{n}^3 blast workload, 1 warm-up + {steps} timed step(s) "
-                      f"({dt:.1f} s)"}
+    print(json.dumps({"value": cells * steps / dt, "unit": "cell-updates/s", "cores": threads,
+                      "threads": threads, "kind": "port", "sample": f"{sample} ({dt:.1f} s)"}))
+
+
+def cpu_baseline(n, steps=1):
+    """The oracle (C++ OpenMP restatement of the reference's step) on the host cores, on a
+    bounded sample of the same workload."""
+    return run_baseline_child(["mhd", n, steps])
 
 
 # Euler (hydro) solvers.  FP64 add + mul + div + sqrt per cell-update of the fused algorithm
@@ -259,26 +310,8 @@ def bench_hydro(a, dim):
 
 def hydro_cpu_baseline(dim, scale, steps):
     """The hydro oracle (C++ OpenMP restatement of the reference's impl-0 hydro step) on the
-    host cores, same workload: one warm-up step, then `steps` timed steps."""
-    from oracle import pyhydro as H
-
-    H.build()
-    p, pr = hydro_workload(dim, scale, A.PATH_AUTO, A.FP64)
-    cells = p.nx * p.ny * (p.nz if dim == 3 else 1)
-    s = H.Solver(p)
-    s.init_problem(pr)
-    s.step(0, A.DT_CFL)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s.step(0, A.DT_CFL)
-    el = time.perf_counter() - t0
-    s.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    shape = f"{p.nx}x{p.ny}" + (f"x{p.nz}" if dim == 3 else "")
-    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"hydro oracle (C++ OpenMP restatement of the reference impl-0 step), same "
-                      f"{dim}D problem on {shape}, 1 warm-up + {steps} timed step(s) "
-                      f"({el:.1f} s)"}
+    host cores, same problem: one warm-up step, then `steps` timed steps."""
+    return run_baseline_child([f"hydro{dim}d", scale, steps])
 
 
 def bench_ensemble(a):
@@ -678,4 +711,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-baseline-child":
+        cpu_baseline_child(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))
+    else:
+        main()

@@ -191,6 +191,17 @@ int ppk_mhd_params_from_ini(const char *ini_path, ppk_mhd_params *params,
 /* Same, from an in-memory .ini text. */
 int ppk_mhd_params_from_ini_text(const char *ini_text, ppk_mhd_params *params,
                                  ppk_mhd_problem *problem, char *solver_name, int name_len);
+/* Same, through a caller's key lookup instead of a file: lookup(ctx, section, name, value,
+ * value_size) writes the raw value string of [section] name (NUL-terminated) and returns 1,
+ * or returns 0 when the key is absent; the typed parsing and defaults are the ones above.
+ * A reference-side binding passes the ConfigMap the solver was constructed with
+ * (SolverFactory::create(name, HydroParams &, ConfigMap &), src/shared/SolverFactory.h:43),
+ * so the problem parameters come from the same source as the reference's problem structs
+ * (BlastParams(configMap), ..., src/shared/problems/). */
+typedef int (*ppk_ini_lookup)(void *ctx, const char *section, const char *name, char *value,
+                              int value_size);
+int ppk_mhd_params_from_lookup(ppk_ini_lookup lookup, void *ctx, ppk_mhd_params *params,
+                               ppk_mhd_problem *problem, char *solver_name, int name_len);
 /* Fill the derived fields (dx, dy, dz, smallp, smallpp, gamma6) from the primary ones,
  * as HydroParams::init does (src/shared/HydroParams.cpp:421-453). */
 int ppk_mhd_params_init(ppk_mhd_params *params);
@@ -295,6 +306,22 @@ int ppk_mhd_write_vtk(const ppk_mhd_params *params, const double *U, const char 
 /* The same for the handle's current state (copies it to the host; synchronizes).
  * Replaces: SolverMHDMuscl::save_solution_impl (src/muscl/SolverMHDMuscl.h:894-906). */
 int ppk_mhd_save_vtk(ppk_mhd_t *h, const char *dir, const char *prefix, int istep, int ascii);
+
+/* Restart files: the state with its ghost cells, "time step" (the caller's output counter),
+ * the iteration and "total time", in one raw little-endian file (layout in io_restart.cpp).
+ * Replaces: the HDF5 restart output and reader (src/utils/io/IO_HDF5.h:430-484, attributes
+ * "time step" and "total time"; SolverMHDMuscl::init_restart, src/muscl/SolverMHDMuscl.h:
+ * 613-643).  A file whose grid differs from params is refused (PPK_ERR_INVALID_ARGUMENT). */
+int ppk_mhd_write_restart(const ppk_mhd_params *params, const double *U, double total_time,
+                          int64_t time_step, int64_t iteration, const char *path);
+int ppk_mhd_read_restart(const ppk_mhd_params *params, const char *path, double *U, size_t count,
+                         double *total_time, int64_t *time_step, int64_t *iteration);
+/* The handle's current state and counters to a restart file, and back (synchronize).  Loading
+ * restores t and the iteration; continuing gives the same results, bit for bit, as the run
+ * that was never stopped.  (The reference restarts its iteration count at 0; a caller that
+ * wants that calls ppk_mhd_set_time(h, t, 0) after loading.) */
+int ppk_mhd_save_state(ppk_mhd_t *h, const char *path, int64_t time_step);
+int ppk_mhd_load_state(ppk_mhd_t *h, const char *path, int64_t *time_step);
 
 /* ---- batched ensembles ----------------------------------------------------------------------
  * Many independent simulations of one dimension and precision (any sizes, gamma, cfl,

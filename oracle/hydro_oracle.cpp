@@ -11,6 +11,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <vector>
 
 #include "../include/ppkmhd_amd.h"
@@ -808,6 +811,15 @@ void init(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U) {
 }  // namespace
 
 extern "C" {
+
+// OpenMP threads of the parallel loops (the CPU baseline records it)
+int hora_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 struct hora_solver {
   HydroSolver s;

@@ -21,6 +21,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <vector>
 
 #ifdef ORA_COUNT
@@ -1937,6 +1940,15 @@ struct ora_solver {
 };
 
 extern "C" {
+
+// OpenMP threads the oracle's parallel loops use (the CPU baseline records it)
+int ora_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 int ora_params_size(void) { return (int)sizeof(ppk_mhd_params); }
 int ora_problem_size(void) { return (int)sizeof(ppk_mhd_problem); }

@@ -28,14 +28,23 @@ def build(force=False):
     return os.path.join(_HERE, _NAMES[0])
 
 
-def _lib(counting=False):
-    name = _NAMES[1] if counting else _NAMES[0]
+def build_native():
+    """The CPU baseline's build (-march=native for the host it runs on)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "libhydro_oracle_native.so"], check=True)
+    return os.path.join(_HERE, "libhydro_oracle_native.so")
+
+
+def _lib(counting=False, native=False):
+    name = _NAMES[1] if counting else ("libhydro_oracle_native.so" if native else _NAMES[0])
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_HERE, name)
-    if not os.path.exists(path):
+    if native:  # always for the host at hand: a copy built elsewhere may not run here
+        build_native()
+    elif not os.path.exists(path):
         build()
     lib = C.CDLL(path)
+    lib.hora_threads.restype = C.c_int
     P = C.POINTER(A.Params)
     PR = C.POINTER(A.Problem)
     dp = C.POINTER(C.c_double)
@@ -129,12 +138,17 @@ def riemann(p, ql, qr):
     return f
 
 
+def threads(native=False):
+    """OpenMP threads of the hydro oracle's parallel loops."""
+    return int(_lib(native=native).hora_threads())
+
+
 class Solver:
     """The reference's Hydro_Muscl time loop (U/U2 ping-pong, dt from the CFL or fixed)."""
 
-    def __init__(self, p, counting=False):
+    def __init__(self, p, counting=False, native=False):
         self.p = p
-        self._L = _lib(counting)
+        self._L = _lib(counting, native)
         self._h = self._L.hora_create(C.byref(p))
 
     def close(self):

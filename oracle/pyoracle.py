@@ -24,14 +24,23 @@ def build(force=False):
     return out
 
 
-def _lib(counting=False):
-    name = "liboracle_count.so" if counting else "liboracle.so"
+def build_native():
+    """The CPU baseline's build (-march=native for the host it runs on); returns its path."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle_native.so"], check=True)
+    return os.path.join(_HERE, "liboracle_native.so")
+
+
+def _lib(counting=False, native=False):
+    name = "liboracle_count.so" if counting else ("liboracle_native.so" if native else "liboracle.so")
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_HERE, name)
-    if not os.path.exists(path):
+    if native:  # always for the host at hand: a copy built elsewhere may not run here
+        build_native()
+    elif not os.path.exists(path):
         build()
     lib = C.CDLL(path)
+    lib.ora_threads.restype = C.c_int
     P = C.POINTER(A.Params)
     PR = C.POINTER(A.Problem)
     dp = C.POINTER(C.c_double)
@@ -134,9 +143,9 @@ def slope(p, q, qp, qm):
 class Solver:
     """SolverMHDMuscl<dim> time loop of the oracle (U / U2 ping-pong, t, dt, iteration)."""
 
-    def __init__(self, p, counting=False):
+    def __init__(self, p, counting=False, native=False):
         self.p = p
-        self._lib = _lib(counting)
+        self._lib = _lib(counting, native)
         self._h = self._lib.ora_create(C.byref(p))
 
     def close(self):
@@ -188,6 +197,11 @@ class Solver:
 
 STAGES = ("boundaries", "primitives_dt", "elec_field", "mag_slopes", "trace", "fluxes", "emf",
           "update_ct")
+
+
+def threads(native=False):
+    """OpenMP threads of the oracle's parallel loops."""
+    return int(_lib(native=native).ora_threads())
 
 
 def op_counts(stage=None):

@@ -18,6 +18,10 @@ _PR = C.POINTER(A.Problem)
 _H = C.c_void_p
 _DP = C.POINTER(C.c_double)
 _FP = C.POINTER(C.c_float)
+# ppk_ini_lookup(ctx, section, name, value, value_size) -> 1 found / 0 absent
+INI_LOOKUP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_char),
+                         C.c_int)
+
 SIGNATURES = {
     "ppk_mhd_params_size": (C.c_int, []),
     "ppk_mhd_problem_size": (C.c_int, []),
@@ -25,6 +29,8 @@ SIGNATURES = {
     "ppk_mhd_device_count": (C.c_int, []),
     "ppk_mhd_params_from_ini": (C.c_int, [C.c_char_p, _P, _PR, C.c_char_p, C.c_int]),
     "ppk_mhd_params_from_ini_text": (C.c_int, [C.c_char_p, _P, _PR, C.c_char_p, C.c_int]),
+    "ppk_mhd_params_from_lookup": (C.c_int, [INI_LOOKUP, C.c_void_p, _P, _PR, C.c_char_p,
+                                             C.c_int]),
     "ppk_mhd_params_init": (C.c_int, [_P]),
     "ppk_mhd_create": (C.c_int, [_P, C.POINTER(_H)]),
     "ppk_mhd_destroy": (None, [_H]),
@@ -53,6 +59,12 @@ SIGNATURES = {
     "ppk_mhd_write_vtk": (C.c_int, [C.POINTER(A.Params), _DP, C.c_char_p, C.c_char_p, C.c_int,
                                     C.c_int]),
     "ppk_mhd_save_vtk": (C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_int, C.c_int]),
+    "ppk_mhd_write_restart": (C.c_int, [C.POINTER(A.Params), _DP, C.c_double, C.c_int64,
+                                        C.c_int64, C.c_char_p]),
+    "ppk_mhd_read_restart": (C.c_int, [C.POINTER(A.Params), C.c_char_p, _DP, C.c_size_t, _DP,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ppk_mhd_save_state": (C.c_int, [_H, C.c_char_p, C.c_int64]),
+    "ppk_mhd_load_state": (C.c_int, [_H, C.c_char_p, C.POINTER(C.c_int64)]),
     "ppk_mhd_batch_create": (C.c_int, [_P, C.c_int, C.POINTER(_H)]),
     "ppk_mhd_batch_destroy": (None, [_H]),
     "ppk_mhd_batch_last_error": (C.c_char_p, [_H]),

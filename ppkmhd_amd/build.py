@@ -57,6 +57,7 @@ UNITS = [
     ("ics.cpp", "ics.o", ["-ffp-contract=off"]),
     ("ics_device.hip", "ics_device.o", ["-ffp-contract=off"]),
     ("io_vtk.cpp", "io_vtk.o", []),
+    ("io_restart.cpp", "io_restart.o", ["--offload-host-only"]),
     ("batch_capi.cpp", "batch_capi.o", ["--offload-host-only"]),
     ("mlp_capi.cpp", "mlp_capi.o", ["--offload-host-only"]),
     ("hydro_capi.cpp", "hydro_capi.o", ["--offload-host-only", "-ffp-contract=off"]),

@@ -165,7 +165,7 @@ k_briemann(const BatchMember *__restrict__ mem, int nm) {
   if (c < 0) return;
   const R *rec = static_cast<const R *>(m.rec);
   R *flux = static_cast<R *>(m.flux);
-  R *emf = static_cast<R *>(m.emf);
+  double *emf = static_cast<double *>(m.emf);
   if constexpr (DIM == 3) {
     switch ((int)(B % 6)) {
       case 0: face3<0>(rec, flux, p, c); break;
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(kCflBlock) k_bupdate(const BatchMember *__rest
   double best = 0.0;
   if (brick_cell(m.pad, L, threadIdx.x, i, j, k))
     best = update_cell(m.U[par], m.U[par ^ 1], static_cast<const R *>(m.flux),
-                       static_cast<const R *>(m.emf), p, cidx(p, i, j, k), dtdx, dtdy, dtdz, true);
+                       static_cast<const double *>(m.emf), p, cidx(p, i, j, k), dtdx, dtdy, dtdz, true);
   best = block_max(best, red);
   if (threadIdx.x == 0) m.parts[L] = best;
 }

@@ -193,7 +193,7 @@ int ppk_mhd_batch_create(const ppk_mhd_params *params, int nmembers, ppk_mhd_bat
     bm.U[1] = static_cast<double *>(dalloc(8 * n * sizeof(double)));
     bm.rec = dalloc((size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n * es);
     bm.flux = dalloc((size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n * es);
-    bm.emf = dalloc((size_t)(d == 3 ? 3 : 1) * n * es);
+    bm.emf = dalloc((size_t)(d == 3 ? 3 : 1) * n * sizeof(double));  // FP64 EMFs in every form
     int most_parts = 0;  // the partials of the largest brick table any shape gives
     for (int shape = 0; shape < kBrickShapes; ++shape) {
       ppk::DevParams q = bm.p;

@@ -277,6 +277,8 @@ bool mhd_problem_ok(int kind) {
          kind == PPK_PROBLEM_ROTOR || kind == PPK_PROBLEM_FIELD_LOOP;
 }
 DevParams make_dev_params(const ppk_mhd_params &p) { return make_dev(p); }
+ppk_mhd_params handle_params(const ppk_mhd_t *h) { return h->p; }
+void set_handle_error(ppk_mhd_t *h, const std::string &msg) { h->err = msg; }
 }  // namespace ppk
 
 extern "C" {
@@ -353,7 +355,7 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
     };
     ok = balloc(&h->fw.rec, (size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n) &&
          balloc(&h->fw.flux, (size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n) &&
-         balloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n);
+         hipMalloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n * sizeof(double)) == hipSuccess;
   }
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
   if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * (ppk::CTRL_SLOTS + 1)) != hipSuccess)

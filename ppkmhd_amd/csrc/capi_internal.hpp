@@ -21,6 +21,9 @@ int validate_params(const ppk_mhd_params *p, std::string &why);
 bool mhd_problem_ok(int kind);
 struct DevParams;
 DevParams make_dev_params(const ppk_mhd_params &p);
+// a solver handle's parameters, and its per-handle error message (capi.cpp)
+ppk_mhd_params handle_params(const ppk_mhd_t *h);
+void set_handle_error(ppk_mhd_t *h, const std::string &msg);
 
 // host initial conditions in the reference layout (ics.cpp)
 void init_problem_host(const ppk_mhd_params &p, const ppk_mhd_problem &pr, double *U);

@@ -39,9 +39,45 @@ std::string lower(std::string s) {
   return s;
 }
 
+// key lookups with the reference's typed accessors on top of one raw string lookup
 class IniMap {
  public:
-  explicit IniMap(const std::string &text) {
+  virtual ~IniMap() {}
+  // the raw value of [s] n, if the key exists
+  virtual bool raw(const std::string &s, const std::string &n, std::string &out) const = 0;
+
+  std::string get_string(const std::string &s, const std::string &n, const std::string &d) const {
+    std::string v;
+    return raw(s, n, v) ? v : d;
+  }
+  long get_integer(const std::string &s, const std::string &n, long d) const {
+    std::string v = get_string(s, n, "");
+    const char *c = v.c_str();
+    char *end = nullptr;
+    long r = std::strtol(c, &end, 0);
+    return end > c ? r : d;
+  }
+  float get_float(const std::string &s, const std::string &n, float d) const {
+    std::string v = get_string(s, n, "");
+    const char *c = v.c_str();
+    char *end = nullptr;
+    float r = std::strtof(c, &end);
+    return end > c ? r : d;
+  }
+  bool get_bool(const std::string &s, const std::string &n, bool d) const {
+    std::string v = get_string(s, n, "");
+    bool r = d;
+    if (v == "1" || v == "yes" || v == "true" || v == "on") r = true;
+    if (v == "0" || v == "no" || v == "false" || v == "off") r = false;
+    if (v.empty()) r = d;
+    return r;
+  }
+};
+
+// the text of an .ini file, parsed as the vendored inih reader does
+class IniText : public IniMap {
+ public:
+  explicit IniText(const std::string &text) {
     std::istringstream in(text);
     std::string line, section, prev_name;
     while (std::getline(in, line)) {
@@ -68,32 +104,11 @@ class IniMap {
       prev_name = name;
     }
   }
-
-  std::string get_string(const std::string &s, const std::string &n, const std::string &d) const {
+  bool raw(const std::string &s, const std::string &n, std::string &out) const override {
     auto it = values_.find(key(s, n));
-    return it == values_.end() ? d : it->second;
-  }
-  long get_integer(const std::string &s, const std::string &n, long d) const {
-    std::string v = get_string(s, n, "");
-    const char *c = v.c_str();
-    char *end = nullptr;
-    long r = std::strtol(c, &end, 0);
-    return end > c ? r : d;
-  }
-  float get_float(const std::string &s, const std::string &n, float d) const {
-    std::string v = get_string(s, n, "");
-    const char *c = v.c_str();
-    char *end = nullptr;
-    float r = std::strtof(c, &end);
-    return end > c ? r : d;
-  }
-  bool get_bool(const std::string &s, const std::string &n, bool d) const {
-    std::string v = get_string(s, n, "");
-    bool r = d;
-    if (v == "1" || v == "yes" || v == "true" || v == "on") r = true;
-    if (v == "0" || v == "no" || v == "false" || v == "off") r = false;
-    if (v.empty()) r = d;
-    return r;
+    if (it == values_.end()) return false;
+    out = it->second;
+    return true;
   }
 
  private:
@@ -108,6 +123,24 @@ class IniMap {
     return v;
   }
   std::map<std::string, std::string> values_;
+};
+
+// a caller's key lookup (the reference's ConfigMap behind a callback)
+class IniCallback : public IniMap {
+ public:
+  IniCallback(ppk_ini_lookup fn, void *ctx) : fn_(fn), ctx_(ctx) {}
+  bool raw(const std::string &s, const std::string &n, std::string &out) const override {
+    char buf[1024];
+    buf[0] = 0;
+    if (!fn_(ctx_, s.c_str(), n.c_str(), buf, (int)sizeof buf)) return false;
+    buf[sizeof buf - 1] = 0;
+    out = buf;
+    return true;
+  }
+
+ private:
+  ppk_ini_lookup fn_;
+  void *ctx_;
 };
 
 int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char *solver_name,
@@ -363,7 +396,7 @@ int ppk_mhd_params_from_ini_text(const char *ini_text, ppk_mhd_params *params,
     ppk::set_thread_error("ini_text is NULL");
     return PPK_ERR_INVALID_ARGUMENT;
   }
-  ppk::IniMap m{std::string(ini_text)};
+  ppk::IniText m{std::string(ini_text)};
   return ppk::fill_from_ini(m, params, problem, solver_name, name_len);
 }
 
@@ -380,7 +413,17 @@ int ppk_mhd_params_from_ini(const char *ini_path, ppk_mhd_params *params,
   }
   std::stringstream ss;
   ss << f.rdbuf();
-  ppk::IniMap m{ss.str()};
+  ppk::IniText m{ss.str()};
+  return ppk::fill_from_ini(m, params, problem, solver_name, name_len);
+}
+
+int ppk_mhd_params_from_lookup(ppk_ini_lookup lookup, void *ctx, ppk_mhd_params *params,
+                               ppk_mhd_problem *problem, char *solver_name, int name_len) {
+  if (!lookup) {
+    ppk::set_thread_error("lookup is NULL");
+    return PPK_ERR_INVALID_ARGUMENT;
+  }
+  ppk::IniCallback m{lookup, ctx};
   return ppk::fill_from_ini(m, params, problem, solver_name, name_len);
 }
 

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(Tile2::NT) k_half2d(const double *__restrict__
 // 4 waves per SIMD: the EMF solve fits 128 VGPRs without spilling
 template <int E, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_edge3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
+k_edge3d(const R *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
          const double *ctrl) {
   if (!active(ctrl)) return;
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
@@ -97,7 +97,7 @@ k_edge3d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
 // ppk_riemann) as a compile-time constant, so the kernel holds only the solver it runs.
 template <int RS, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(3)))
-k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux, R *__restrict__ emf,
+k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux, double *__restrict__ emf,
                 DevParams p, Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
   const int B = xcd_remap(blockIdx.x, gridDim.x);
@@ -116,7 +116,7 @@ k_riemann_mix3d(const R *__restrict__ rec, R *__restrict__ flux, R *__restrict__
 template <int RS, class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
-                R *__restrict__ emf, DevParams p, Brick bk, const double *ctrl) {
+                double *__restrict__ emf, DevParams p, Brick bk, const double *ctrl) {
   if (!active(ctrl)) return;
   const int B = xcd_remap(blockIdx.x, gridDim.x);
   const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
@@ -130,7 +130,7 @@ k_riemann_mix2d(const R *__restrict__ rec, R *__restrict__ flux,
 
 template <class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_edge2d(const R *__restrict__ rec, R *__restrict__ emf, DevParams p, Brick bk,
+k_edge2d(const R *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
          const double *ctrl) {
   if (!active(ctrl)) return;
   const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
@@ -142,7 +142,7 @@ template <class R>
 __global__ void __launch_bounds__(kCflBlock) k_update(const double *__restrict__ Uin,
                                                       double *__restrict__ Uout,
                                                       const R *__restrict__ flux,
-                                                      const R *__restrict__ emf,
+                                                      const double *__restrict__ emf,
                                                       double *__restrict__ parts, DevParams p,
                                                       Brick bk, const double *ctrl, int fold) {
   if (!active(ctrl)) return;
@@ -175,7 +175,7 @@ inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // the interleaved face/edge launch with the solver as a compile-time constant
 template <class R>
-void launch_mix3d(const DevParams &p, const R *rec, R *flux, R *emf, const double *ctrl,
+void launch_mix3d(const DevParams &p, const R *rec, R *flux, double *emf, const double *ctrl,
                   hipStream_t s) {
   const Brick bk = sub_bricks(p);
   const dim3 g((unsigned)(6 * bk.count())), b(kRiemannBlock);
@@ -195,7 +195,7 @@ void launch_mix3d(const DevParams &p, const R *rec, R *flux, R *emf, const doubl
   }
 }
 template <class R>
-void launch_mix2d(const DevParams &p, unsigned grid, const R *rec, R *flux, R *emf, Brick bk,
+void launch_mix2d(const DevParams &p, unsigned grid, const R *rec, R *flux, double *emf, Brick bk,
                   const double *ctrl, hipStream_t s) {
   const dim3 g(grid), b(kRiemannBlock);
   switch (p.riemann) {
@@ -227,10 +227,10 @@ void launch_half3(const DevParams &p, const double *Uin, R *rec, const double *c
 }
 
 // the step's three stages in arithmetic type R (double, or float for the FP32-bulk form);
-// rec, flux and emf hold R
+// rec and flux hold R, emf holds double
 template <class R>
 hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *rec, R *flux,
-                        R *emf, double *parts, const double *ctrl, int fold_cfl,
+                        double *emf, double *parts, const double *ctrl, int fold_cfl,
                         const SurrogateCfg *sur, hipStream_t s, StageTimer *tm) {
   const Brick bk = sub_bricks(p);
   const dim3 g((unsigned)bk.count()), b(kRiemannBlock);
@@ -279,7 +279,7 @@ hipError_t fused_godunov(const DevParams &p, const double *Uin, double *Uout, Fu
   using namespace fusedk;
   if (p.bulk32)
     return godunov_impl<float>(p, Uin, Uout, static_cast<float *>(w.rec),
-                               static_cast<float *>(w.flux), static_cast<float *>(w.emf), parts,
+                               static_cast<float *>(w.flux), static_cast<double *>(w.emf), parts,
                                ctrl, fold_cfl, w.sur, s, tm);
   return godunov_impl<double>(p, Uin, Uout, static_cast<double *>(w.rec),
                               static_cast<double *>(w.flux), static_cast<double *>(w.emf), parts,

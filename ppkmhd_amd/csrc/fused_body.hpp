@@ -516,13 +516,14 @@ PPK_HD void face3(const R *__restrict__ rec, R *__restrict__ flux, const DevPara
 
 // One edge of cell c (its lower edge along direction E = 0 x, 1 y, 2 z): the four edge states
 // of the cells around it, then compute_emf.  Reference src/muscl/MHDRunFunctors3D.h:2183-2230
-// (the y edge takes RB and LT swapped, :2212-2217).
+// (the y edge takes RB and LT swapped, :2212-2217).  The edge states and the EMF solve are
+// double whatever the record type (the FP32-bulk form keeps the CT EMFs in FP64).
 template <int E, class R>
-PPK_HD void edge3(const R *__restrict__ rec, R *__restrict__ emf, const DevParams &p,
+PPK_HD void edge3(const R *__restrict__ rec, double *__restrict__ emf, const DevParams &p,
                   long long c) {
   const long long vs = p.vstride, sx = 1, sy = p.isize, sz = p.plane;
-  auto RF = [&](long long t, int f) { return rec[t + f * vs]; };
-  Half3D<R> a, b, d, h;
+  auto RF = [&](long long t, int f) { return double(rec[t + f * vs]); };
+  Half3D<double> a, b, d, h;
   load_rec3(rec, vs, c, h);
   if (E == 2) {  // RT of c-x-y, RB of c-x, LT of c-y, LB of c
     load_rec3(rec, vs, c - sx - sy, a);
@@ -599,12 +600,13 @@ PPK_HD void face2(const R *__restrict__ rec, R *__restrict__ flux, const DevPara
   store_flux2<D>(flux, p, c, f);
 }
 
+// (double edge states and EMF whatever the record type, as edge3)
 template <class R>
-PPK_HD void edge2(const R *__restrict__ rec, R *__restrict__ emf, const DevParams &p,
+PPK_HD void edge2(const R *__restrict__ rec, double *__restrict__ emf, const DevParams &p,
                   long long c) {
   const long long vs = p.vstride, sx = 1, sy = p.isize;
-  auto RF = [&](long long t, int f) { return rec[t + f * vs]; };
-  Half2D<R> a, b, d, h;
+  auto RF = [&](long long t, int f) { return double(rec[t + f * vs]); };
+  Half2D<double> a, b, d, h;
   load_rec2(rec, vs, c, h);
   load_rec2(rec, vs, c - sx - sy, a);
   a.AR = RF(c - sy, R2_AL); a.dARy = RF(c - sy, R2_DALY);
@@ -671,36 +673,31 @@ PPK_HD long long sub_brick_cell(const DevParams &p, const Brick &b, int L, int t
 // update + CT + ghost copy + CFL of the new state
 // ===========================================================================================
 // CT of one face field (MHDRunFunctors3D.h:2600-2620); each component in the reference's order
-template <class R>
-PPK_HD double ct3_a(double a, const R *__restrict__ emf, long long vs, long long t,
+PPK_HD double ct3_a(double a, const double *__restrict__ emf, long long vs, long long t,
                     long long sy, long long sz, double dtdy, double dtdz) {
   a += (double(emf[t + sy + I_EMFZ * vs]) - double(emf[t + I_EMFZ * vs])) * dtdy;
   a -= (double(emf[t + sz + I_EMFY * vs]) - double(emf[t + I_EMFY * vs])) * dtdz;
   return a;
 }
-template <class R>
-PPK_HD double ct3_b(double b, const R *__restrict__ emf, long long vs, long long t,
+PPK_HD double ct3_b(double b, const double *__restrict__ emf, long long vs, long long t,
                     long long sx, long long sz, double dtdx, double dtdz) {
   b -= (double(emf[t + sx + I_EMFZ * vs]) - double(emf[t + I_EMFZ * vs])) * dtdx;
   b += (double(emf[t + sz + I_EMFX * vs]) - double(emf[t + I_EMFX * vs])) * dtdz;
   return b;
 }
-template <class R>
-PPK_HD double ct3_c(double c, const R *__restrict__ emf, long long vs, long long t,
+PPK_HD double ct3_c(double c, const double *__restrict__ emf, long long vs, long long t,
                     long long sx, long long sy, double dtdx, double dtdy) {
   c += (double(emf[t + sx + I_EMFY * vs]) - double(emf[t + I_EMFY * vs])) * dtdx;
   c -= (double(emf[t + sy + I_EMFX * vs]) - double(emf[t + I_EMFX * vs])) * dtdy;
   return c;
 }
 // 2D CT (MHDRunFunctors2D.h:1690-1691)
-template <class R>
-PPK_HD double ct2_a(double a, const R *__restrict__ emf, long long t, long long sy,
+PPK_HD double ct2_a(double a, const double *__restrict__ emf, long long t, long long sy,
                     double dtdy) {
   a += (double(emf[t + sy]) - double(emf[t])) * dtdy;
   return a;
 }
-template <class R>
-PPK_HD double ct2_b(double b, const R *__restrict__ emf, long long t, long long sx,
+PPK_HD double ct2_b(double b, const double *__restrict__ emf, long long t, long long sx,
                     double dtdx) {
   b -= (double(emf[t + sx]) - double(emf[t])) * dtdx;
   return b;
@@ -713,7 +710,7 @@ PPK_HD double ct2_b(double b, const R *__restrict__ emf, long long t, long long 
 // (periodic image, or +-own field, BoundariesFunctors.h).
 template <class R>
 PPK_HD double update_cell(const double *__restrict__ Uin, double *__restrict__ Uout,
-                          const R *__restrict__ flux, const R *__restrict__ emf,
+                          const R *__restrict__ flux, const double *__restrict__ emf,
                           const DevParams &p, long long t, double dtdx, double dtdy, double dtdz,
                           bool fold) {
   const long long vs = p.vstride, sx = 1, sy = p.isize, sz = p.plane;

@@ -76,11 +76,11 @@ struct SurrogateCfg {
   unsigned long long *counters = nullptr;
   unsigned char *flags = nullptr;  // dim x cells: faces left to the HLLD pass (device scratch)
 };
-// element type: double, or float in the FP32-bulk form (DevParams::bulk32)
+// element type of rec and flux: double, or float in the FP32-bulk form (DevParams::bulk32)
 struct FusedScratch {
   void *rec = nullptr;   // kRec{2,3} x cells
   void *flux = nullptr;  // dim x kFluxComp{2,3} x cells
-  void *emf = nullptr;   // 3 (3D) or 1 (2D) x cells
+  void *emf = nullptr;   // 3 (3D) or 1 (2D) x cells, double in every form
   const SurrogateCfg *sur = nullptr;  // surrogate face solves when set
 };
 

@@ -117,6 +117,17 @@ class SolverMHDMuscl:
     def set_time(self, t, iteration):
         self._check(lib().ppk_mhd_set_time(self._h, float(t), int(iteration)))
 
+    # ---- restart (SolverMHDMuscl::init_restart, IO_HDF5 "time step" / "total time") --------
+    def save_state(self, path, time_step=0):
+        """Write the state, t and the iteration to a restart file."""
+        self._check(lib().ppk_mhd_save_state(self._h, str(path).encode(), int(time_step)))
+
+    def load_state(self, path):
+        """Restore state, t and iteration from a restart file; returns its "time step"."""
+        ts = C.c_int64()
+        self._check(lib().ppk_mhd_load_state(self._h, str(path).encode(), C.byref(ts)))
+        return ts.value
+
     @property
     def m_t(self):
         return self.time()[0]
@@ -475,3 +486,23 @@ class SolverFactory:
             kind = A.PROBLEM_IMPLODE if solver_name.startswith("Hydro") else A.PROBLEM_ORSZAG_TANG
             problem = default_problem(kind, params)
         return fn(params, problem)
+
+
+def write_restart(path, params, U, total_time=0.0, time_step=0, iteration=0):
+    """Host-side restart writer (ppk_mhd_write_restart): no device involved."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    if U.size != params.count:
+        raise ValueError(f"state must have {params.count} doubles, got {U.size}")
+    check(lib().ppk_mhd_write_restart(C.byref(params), U.ctypes.data_as(C.POINTER(C.c_double)),
+                                      float(total_time), int(time_step), int(iteration),
+                                      str(path).encode()))
+
+
+def read_restart(path, params):
+    """Host-side restart reader: (state, total time, time step, iteration)."""
+    U = np.empty(params.shape, dtype=np.float64)
+    t, ts, it = C.c_double(), C.c_int64(), C.c_int64()
+    check(lib().ppk_mhd_read_restart(C.byref(params), str(path).encode(),
+                                     U.ctypes.data_as(C.POINTER(C.c_double)), U.size,
+                                     C.byref(t), C.byref(ts), C.byref(it)))
+    return U, t.value, ts.value, it.value

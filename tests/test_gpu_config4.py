@@ -1,0 +1,164 @@
+"""The 3D surrogate workload (MHD Kelvin-Helmholtz with a shear layer normal to y) on the GPU, at
+reduced size: the 3D surrogate face kernel against the plain step and against the oracle's LLF
+scheme, what a network trained on the solver's own 3D faces gains over no network, and the
+error of the FP32-bulk form against the FP64 oracle on this workload.
+
+Workload parameters: rho 2 for |y| <= 1/4 and 1 outside, vx = -/+ 0.5 with seeded uniform
+perturbations of amplitude 0.01 on vx and vy, p 2.5, B = (0.1, 0, 0), gamma 1.4, periodic
+[-1/2, 1/2]^3 (the reference's functor: src/muscl/MHDInitFunctors3D.h:420-613, random branch,
+here with the layer normal to y and a uniform field).
+"""
+import numpy as np
+import pytest
+
+from parity import rel_l1, rel_linf
+from ppkmhd_amd import _abi as A
+from ppkmhd_amd import default_problem, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(p, pr=None):
+    from ppkmhd_amd.solver import SolverMHDMuscl
+    return SolverMHDMuscl(p, pr)
+
+
+def S():
+    from oracle import surrogate_oracle
+    return surrogate_oracle
+
+
+def kh3d(n, precision=A.FP64, riemann=A.RIEMANN_HLLD, tEnd=10.0, seed=1):
+    p = make_params(3, n, n, n, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5,
+                    gamma0=1.4, cfl=0.8, slope_type=1.0, riemann=riemann, tEnd=tEnd,
+                    n_step_max=1 << 30, precision=precision)
+    pr = default_problem(A.PROBLEM_KELVIN_HELMHOLTZ, p)
+    pr.kh_p_rand, pr.kh_p_sine, pr.kh_p_sine_rob = 1, 0, 0
+    pr.kh_axis = 1
+    pr.kh_d_in, pr.kh_d_out = 2.0, 1.0
+    pr.kh_vflow_in, pr.kh_vflow_out = -0.5, 0.5
+    pr.kh_pressure = 2.5
+    pr.kh_amplitude = 0.01
+    pr.kh_seed = seed
+    pr.kh_bx, pr.kh_by, pr.kh_bz = 0.1, 0.0, 0.0
+    return p, pr
+
+
+def zero_network():
+    from ppkmhd_amd.surrogate import MLP
+    P = np.zeros(S().N_PARAMS, dtype=np.float32)
+    P[S().ISIG_IN:S().MU_OUT] = 1.0
+    P[S().SIG_OUT:S().N_PARAMS] = 1.0
+    net = MLP(0)
+    net.set_params(P)
+    return net
+
+
+@pytest.mark.parametrize("precision", [A.FP64_STRICT, A.FP64, A.FP32_BULK])
+def test_gate_zero_is_plain_hlld_3d(gpu, precision):
+    """Gate 0: every face goes to the HLLD fallback of the 3D surrogate kernel, which computes
+    the plain step's face states and flux."""
+    from ppkmhd_amd.surrogate import MLP, attach, counts
+    p, pr = kh3d(20, precision)
+    a, b = solver(p, pr), solver(p, pr)
+    net = MLP(5)
+    attach(b, net, 0.0, 0.0)
+    a.run(6)
+    b.run(6)
+    if precision == A.FP64_STRICT:
+        assert np.array_equal(a.download(), b.download())
+    else:  # FMA contraction may differ between the two face kernels
+        bar = 1e-13 if precision == A.FP64 else 1e-6
+        e = rel_linf(b.download(), a.download(), p)
+        print(f"gate 0, precision {precision}: relative L-inf to the plain step {e:.2e}")
+        assert e <= bar
+    n_net, n_hlld = counts(b)
+    assert n_net == 0 and n_hlld == 6 * 3 * 21**3
+
+
+def test_zero_network_is_the_llf_scheme_3d(oracle, gpu):
+    """A network that outputs 0 with an open gate gives F_LLF at every 3D face (its pieces in
+    float): the step follows the oracle's 3D step with riemann = LLF (edges 2D HLLD) to float
+    rounding of the face fluxes."""
+    from ppkmhd_amd.surrogate import attach, counts
+    p, pr = kh3d(16, A.FP64_STRICT)
+    g = solver(p, pr)
+    attach(g, zero_network(), 1e30, 1e30)
+    q, _ = kh3d(16, A.FP64_STRICT, riemann=A.RIEMANN_LLF)
+    o = oracle.Solver(q)
+    o.init_problem(pr)
+    for _ in range(6):
+        g.step(0, A.DT_CFL)
+        o.step(0, A.DT_CFL)
+    e = rel_linf(g.download(), o.get_state(), q)
+    print(f"3D zero network vs oracle LLF scheme after 6 steps: relative L-inf {e:.2e}")
+    assert e <= 1e-5
+    n_net, n_hlld = counts(g)
+    assert n_hlld == 0 and n_net == 6 * 3 * 17**3
+
+
+def test_trained_network_gains_over_llf_3d(gpu):
+    """Train on the 3D faces of the solver's own Kelvin-Helmholtz run (another seed), then
+    compare, against plain HLLD after 60 steps: the gated scheme with the trained network and
+    with no network (LLF at gated faces)."""
+    from ppkmhd_amd.surrogate import MLP, FaceSet, attach, counts, dump_faces
+    n = 32
+    p, pr = kh3d(n, seed=7)
+    src = solver(p, pr)
+    per = 3 * (n + 1) ** 3
+    fs = FaceSet(per * 20)
+    for _ in range(20):
+        dump_faces(src, fs, 1, 0)
+        src.run(4)
+    net = MLP(9)
+    net.standardize(fs)
+    mse0 = net.evaluate(fs)
+    net.train(fs, steps=3000, batch=4096, lr=1e-3)
+    mse1 = net.evaluate(fs)
+    print(f"3D: trained on {len(fs)} faces: standardised MSE {mse0.mean():.3e} -> "
+          f"{mse1.mean():.3e}")
+    q, qr = kh3d(n, seed=1)
+    ref = solver(q, qr)
+    ref.run(60)
+    out = {}
+    for name, m in (("trained", net), ("no network", zero_network())):
+        s = solver(q, qr)
+        attach(s, m, 0.1, 0.1)
+        s.run(60)
+        e = rel_l1(s.download(), ref.download(), q, var=0)
+        a, b = counts(s)
+        out[name] = e
+        print(f"3D {name}: density L1 vs HLLD {e:.3e}, faces by network {a / (a + b):.1%}")
+    assert out["trained"] < 0.5 * out["no network"], out
+
+
+def test_fp32_bulk_kh3d_error(oracle, gpu):
+    """FP32-bulk form (float trace and face solves; FP64 EMFs, CT, update and dt) against the
+    FP64 oracle on this workload at 32^3: relative L-inf per step at fixed dt, and relative L1
+    after 100 CFL steps."""
+    p, pr = kh3d(32, A.FP32_BULK)
+    o = oracle.Solver(p)
+    o.init_problem(pr)
+    g = solver(p, pr)
+    worst = 0.0
+    for _ in range(4):
+        g.upload(o.get_state())
+        g.set_time(*o.time()[::2])
+        o.step(2e-3, A.DT_FIXED)
+        g.step(2e-3, A.DT_FIXED)
+        worst = max(worst, rel_linf(g.download(), o.get_state(), p))
+    print(f"FP32 bulk, KH 3D 32^3: worst relative L-inf per step {worst:.3e}")
+    assert worst <= 1e-5, worst
+    q, _ = kh3d(32, A.FP32_BULK)
+    o2 = oracle.Solver(q)
+    o2.init_problem(pr)
+    g2 = solver(q, pr)
+    for _ in range(100):
+        o2.step(0, A.DT_CFL)
+    g2.run(100)
+    e = rel_l1(g2.download(), o2.get_state(), q)
+    ed = rel_l1(g2.download(), o2.get_state(), q, var=0)
+    print(f"FP32 bulk, KH 3D 32^3 after 100 CFL steps: relative L1 {e:.3e} (density {ed:.3e})")
+    assert e <= 1e-4, e
+    d = g2.divb_max()
+    assert d < 1e-12 * 0.1 / q.dx, d

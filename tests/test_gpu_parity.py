@@ -268,3 +268,23 @@ def test_compute_dt_matches_oracle(oracle, gpu):
     Q = oracle.primitives(p, oracle.fill_boundaries(p, o.get_state()))
     assert g.compute_dt_local() == p.cfl / oracle.inv_dt(p, Q)
     assert g.time()[1] == o.time()[1]
+
+
+@pytest.mark.parametrize("case", ["ot2d", "blast3d"])
+@pytest.mark.parametrize("precision", [A.FP64, A.FP64_STRICT])
+def test_restart_continues_bit_identically(gpu, tmp_path, case, precision):
+    """Stop after 4 CFL steps, save, load into a fresh solver and run 6 more: the state, t, dt
+    and iteration equal those of 10 uninterrupted steps, bit for bit."""
+    p, pr = ot2d(40, 36, precision=precision) if case == "ot2d" else blast3d(14, precision=precision)
+    a = solver(p, pr)
+    a.run(10)
+    b = solver(p, pr)
+    b.run(4)
+    f = tmp_path / "restart.bin"
+    b.save_state(f, time_step=2)
+    c = solver(p)
+    assert c.load_state(f) == 2
+    assert c.time()[0] == b.time()[0] and c.time()[2] == 4
+    c.run(6)
+    assert np.array_equal(c.download(), a.download())
+    assert c.time() == a.time()
