@@ -57,6 +57,10 @@ MEASURED_BYTES_PER_ITEM_3D = {"fused_riemann": 8.77e9 / 257 ** 3, "fused_half": 
                               "fused_update": 5.35e9 / 256 ** 3}
 
 
+# The single-pass kernel does the work of the three fused stages in one launch
+MARCH_STAGES = ("fused_half", "fused_riemann", "fused_update")
+
+
 def items(stage, n):
     return {"primitives_dt": (n + 5) ** 3, "elec_field": (n + 4) ** 3, "mag_slopes": (n + 4) ** 3,
             "trace": (n + 3) ** 3, "fluxes": (n + 1) ** 3, "emf": (n + 1) ** 3,
@@ -72,12 +76,16 @@ def step_flops_per_cell(n):
 
 
 def stage_flops(stage, n):
+    if stage == "fused_march":
+        return sum(FLOP_PER_ITEM_3D[k] * items(k, n) for k in MARCH_STAGES)
     if stage in FLOP_PER_ITEM_3D:
         return FLOP_PER_ITEM_3D[stage] * items(stage, n)
     return None
 
 
 def stage_bytes(stage, n):
+    if stage == "fused_march":  # the state read once and written once
+        return STEP_BYTES_PER_CELL * n ** 3
     if stage in BYTES_PER_ITEM_3D:
         return BYTES_PER_ITEM_3D[stage] * items(stage, n)
     return None
@@ -232,7 +240,9 @@ def bench_hydro(a, dim):
     from ppkmhd_amd.solver import SolverHydroMuscl
 
     scale = a.hydro_scale if a.hydro_scale else 8
-    path = {"auto": A.PATH_AUTO, "staged": A.PATH_STAGED, "fused": A.PATH_FUSED}[a.path]
+    path = {"auto": A.PATH_AUTO, "staged": A.PATH_STAGED, "fused": A.PATH_FUSED}.get(a.path)
+    if path is None:
+        raise SystemExit("the hydro solvers have the staged and fused paths")
     prec = {"fp64": A.FP64, "fp64_strict": A.FP64_STRICT}.get(a.precision)
     if prec is None:
         raise SystemExit("the hydro solvers compute in FP64 (fp64 or fp64_strict)")
@@ -574,7 +584,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--path", choices=("auto", "staged", "fused"), default="auto")
+    ap.add_argument("--path", choices=("auto", "staged", "fused", "fused_records"),
+                    default="auto")
     ap.add_argument("--precision", choices=("fp64", "fp64_strict", "fp32_bulk"), default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=8)
@@ -614,7 +625,8 @@ def main():
 
     from ppkmhd_amd.solver import SolverMHDMuscl
 
-    path = {"auto": A.PATH_AUTO, "staged": A.PATH_STAGED, "fused": A.PATH_FUSED}[a.path]
+    path = {"auto": A.PATH_AUTO, "staged": A.PATH_STAGED, "fused": A.PATH_FUSED,
+            "fused_records": A.PATH_FUSED_RECORDS}[a.path]
     prec = {"fp64": A.FP64, "fp64_strict": A.FP64_STRICT, "fp32_bulk": A.FP32_BULK}[a.precision]
     n = a.n
     p, pr = workload_params(n, path, prec)
@@ -660,7 +672,8 @@ def main():
     if flops is not None:
         achieved = flops / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": dom,
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "kernel": {"fused_march": "k_march3d"}.get(dom, dom),
                 "kernel_ms": avg_ms, "flop_per_launch": flops,
                 "note": "FP64 VALU-bound stage; peak is the MI355X FP64 figure"}
         if nbytes is not None:

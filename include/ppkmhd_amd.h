@@ -54,7 +54,10 @@ enum ppk_precision { PPK_FP64 = 0, PPK_FP32_BULK = 1, PPK_FP64_STRICT = 2 };
 enum ppk_kernel_path {
   PPK_PATH_AUTO = 0,   /* fused kernels where available, else staged                         */
   PPK_PATH_STAGED = 1, /* one kernel per reference stage, intermediates in HBM                */
-  PPK_PATH_FUSED = 2   /* LDS-tiled fused kernels                                             */
+  PPK_PATH_FUSED = 2,  /* fused kernels: in 3D FP64 the single-pass step (state read once and
+                        * written once), else the three-kernel form                          */
+  PPK_PATH_FUSED_RECORDS = 3 /* the three-kernel fused form in every case (half-step records,
+                        * face/edge solves, update; records and fluxes in HBM)               */
 };
 /* dt semantics of ppk_mhd_step */
 enum ppk_dt_mode {

@@ -13,7 +13,7 @@ VAR_NAMES = ("rho", "energy", "rho_vx", "rho_vy", "rho_vz", "bx", "by", "bz")
 RIEMANN_APPROX, RIEMANN_LLF, RIEMANN_HLL, RIEMANN_HLLC, RIEMANN_HLLD = range(5)
 BC_UNDEFINED, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
 FP64, FP32_BULK, FP64_STRICT = 0, 1, 2
-PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
+PATH_AUTO, PATH_STAGED, PATH_FUSED, PATH_FUSED_RECORDS = 0, 1, 2, 3
 DT_FIXED, DT_CFL = 0, 1
 (PROBLEM_ORSZAG_TANG, PROBLEM_BLAST, PROBLEM_KELVIN_HELMHOLTZ, PROBLEM_WAVE, PROBLEM_IMPLODE,
  PROBLEM_GRESHO_VORTEX, PROBLEM_ISENTROPIC_VORTEX, PROBLEM_FOUR_QUADRANT, PROBLEM_RAYLEIGH_TAYLOR,

@@ -43,6 +43,9 @@ UNITS = [
      ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("fused.hip", "fused_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("fused.hip", "fused_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
+    ("march3d.hip", "march3d_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
+    ("march3d.hip", "march3d_strict.o",
+     ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("batch.hip", "batch_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("batch.hip", "batch_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("surrogate.hip", "surrogate_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),

@@ -31,14 +31,16 @@ const KernelTable kFast = {fast::staged_boundaries, fast::staged_prim_dt,
                            fast::staged_end,        fast::staged_godunov,
                            fast::fused_godunov,     fast::fused_cfl,
                            fast::fused_half,        fast::surrogate_dump,
-                           fast::surrogate_dump_slots, fast::batch_block_counts,
+                           fast::surrogate_dump_slots, fast::march_godunov,
+                           fast::batch_block_counts,
                            fast::batch_step};
 const KernelTable kStrict = {strict::staged_boundaries, strict::staged_prim_dt,
                              strict::staged_begin,      strict::staged_set_dt,
                              strict::staged_end,        strict::staged_godunov,
                              strict::fused_godunov,     strict::fused_cfl,
                              strict::fused_half,        strict::surrogate_dump,
-                             strict::surrogate_dump_slots, strict::batch_block_counts,
+                             strict::surrogate_dump_slots, strict::march_godunov,
+                             strict::batch_block_counts,
                              strict::batch_step};
 
 void set_thread_error(const std::string &msg) { g_thread_error = msg; }
@@ -52,8 +54,9 @@ constexpr int kDiagBlocks = 512;
 
 const char *const kStageNames[ST_COUNT] = {
     "boundaries", "primitives_dt", "elec_field", "mag_slopes",    "trace",        "fluxes", "emf",
-    "update_ct",  "control",       "fused_half", "fused_riemann", "fused_update", "fused_cfl"};
-static_assert(ST_COUNT == 13, "stage names");
+    "update_ct",  "control",       "fused_half", "fused_riemann", "fused_update", "fused_cfl",
+    "fused_march"};
+static_assert(ST_COUNT == 14, "stage names");
 
 // the fused kernels serve every precision in 2D and 3D unless the staged path is asked for
 bool fused_supported(const DevParams &p, int precision, int kernel_path) {
@@ -84,6 +87,7 @@ struct ppk_mhd_solver {
   // parts holds the CFL maxima of cur() (ghosts as the next fill makes them): set by a fused
   // CFL-mode step or compute_dt, cleared by anything that changes cur() behind the kernels
   bool cfl_valid = false;
+  int nparts = 0;  // how many partial maxima the kernel that last wrote parts produced
   const ppk::KernelTable *K = nullptr;
   ppk::EventTimer *timer = nullptr;
   long long iter_host = 0;  // iteration count as far as the host knows (buffer parity)
@@ -155,7 +159,7 @@ int validate(const ppk_mhd_params *p, std::string &why) {
     why = "unknown precision";
     return PPK_ERR_INVALID_ARGUMENT;
   }
-  if (p->kernel_path < PPK_PATH_AUTO || p->kernel_path > PPK_PATH_FUSED) {
+  if (p->kernel_path < PPK_PATH_AUTO || p->kernel_path > PPK_PATH_FUSED_RECORDS) {
     why = "unknown kernel_path";
     return PPK_ERR_INVALID_ARGUMENT;
   }
@@ -213,6 +217,18 @@ int read_ctrl(ppk_mhd_t *h) {
   return PPK_OK;
 }
 
+// device buffers of the three-kernel fused form (records, fluxes, EMFs)
+bool alloc_records(ppk_mhd_t *h) {
+  if (h->fw.rec) return true;
+  const int d = h->dp.dim;
+  const size_t n = (size_t)h->cells();
+  const size_t es = h->dp.bulk32 ? sizeof(float) : sizeof(double);
+  return hipMalloc(&h->fw.rec, (size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n * es) == hipSuccess &&
+         hipMalloc(&h->fw.flux,
+                   (size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n * es) == hipSuccess &&
+         hipMalloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n * sizeof(double)) == hipSuccess;
+}
+
 // one time step from cur() into nxt() (godunov_unsplit_impl), optionally gated on finished()
 int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
   const ppk::KernelTable &K = *h->K;
@@ -221,17 +237,26 @@ int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
   double *in = h->cur(), *out = h->nxt();
   ppk::StageTimer *tm = h->tm();
   const bool cfl = dt_mode == PPK_DT_CFL;
-  const int nparts = h->fused ? ppk::fused_cfl_parts(p) : ppk::cfl_blocks(p);
   const int mode = cfl ? ppk::SETDT_CFL : ppk::SETDT_FIXED;
   HIP_CHECK(h, K.begin(h->ctrl, check_finished, s));
   HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s, tm));
   if (h->fused) {
     // the previous fused step left the CFL maxima of its output (= in) in parts
-    if (cfl && !h->cfl_valid) HIP_CHECK(h, K.fused_cfl(p, in, h->parts, h->ctrl, s, tm));
-    HIP_CHECK(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s));
-    HIP_CHECK(h, K.fused_godunov(p, in, out, h->fw, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
+    if (cfl && !h->cfl_valid) {
+      HIP_CHECK(h, K.fused_cfl(p, in, h->parts, h->ctrl, s, tm));
+      h->nparts = ppk::fused_cfl_parts(p);
+    }
+    HIP_CHECK(h, K.set_dt(h->ctrl, h->parts, h->nparts, dt, mode, h->p.cfl, s));
+    if (p.march > 0 && !h->fw.sur) {  // the single-pass 3D step
+      HIP_CHECK(h, K.march_godunov(p, in, out, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
+      if (cfl) h->nparts = (int)ppk::march_blocks(p);
+    } else {
+      HIP_CHECK(h, K.fused_godunov(p, in, out, h->fw, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
+      if (cfl) h->nparts = ppk::fused_cfl_parts(p);
+    }
     h->cfl_valid = cfl;
   } else {
+    const int nparts = ppk::cfl_blocks(p);
     HIP_CHECK(h, K.prim_dt(p, in, h->Q, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
     HIP_CHECK(h, K.set_dt(h->ctrl, h->parts, nparts, dt, mode, h->p.cfl, s));
     HIP_CHECK(h, K.godunov(p, in, out, h->Q, h->w, h->ctrl, s, tm));
@@ -314,6 +339,12 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   h->K = params->precision == PPK_FP64_STRICT ? &ppk::kStrict : &ppk::kFast;
   h->dp.bulk32 = params->precision == PPK_FP32_BULK ? 1 : 0;
   h->fused = ppk::fused_supported(h->dp, params->precision, params->kernel_path);
+  // FP64 3D: the single-pass kernel, unless the three-kernel form is asked for (the FP32-bulk
+  // form and the surrogate run the three-kernel form)
+  h->dp.march = (params->dim == 3 && h->fused && params->precision != PPK_FP32_BULK &&
+                 params->kernel_path != PPK_PATH_FUSED_RECORDS)
+                    ? ppk::kMarchPlanes
+                    : 0;
   if (params->kernel_path == PPK_PATH_FUSED && !h->fused) {
     delete h;
     return fail(nullptr, PPK_ERR_UNSUPPORTED, "fused kernels do not support these parameters");
@@ -339,7 +370,9 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   };
   bool ok = dalloc(&h->U[0], 8 * n) && dalloc(&h->U[1], 8 * n) &&
             dalloc(&h->ctrl, ppk::CTRL_SLOTS) && dalloc(&h->diag, ppk::kDiagBlocks * 8 + 8) &&
-            dalloc(&h->parts, std::max<size_t>(ppk::kCflMaxBlocks, ppk::fused_cfl_parts(h->dp)));
+            dalloc(&h->parts, std::max<size_t>({(size_t)ppk::kCflMaxBlocks,
+                                                (size_t)ppk::fused_cfl_parts(h->dp),
+                                                (size_t)ppk::march_blocks(h->dp)}));
   if (ok && !h->fused) {
     const int d = params->dim;
     ok = dalloc(&h->Q, 8 * n) && dalloc(&h->w.faces, (size_t)2 * d * 8 * n) &&
@@ -347,16 +380,9 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
          dalloc(&h->w.flux, (size_t)d * 8 * n) && dalloc(&h->w.emf, (size_t)(d == 3 ? 3 : 1) * n);
     if (ok && d == 3) ok = dalloc(&h->w.elec, 3 * n) && dalloc(&h->w.dbf, 6 * n);
   }
-  if (ok && h->fused) {
-    const int d = params->dim;
-    const size_t es = h->dp.bulk32 ? sizeof(float) : sizeof(double);
-    auto balloc = [&](void **ptr, size_t count) {
-      return hipMalloc(ptr, count * es) == hipSuccess;
-    };
-    ok = balloc(&h->fw.rec, (size_t)(d == 3 ? ppk::kRec3 : ppk::kRec2) * n) &&
-         balloc(&h->fw.flux, (size_t)d * (d == 3 ? ppk::kFluxComp3 : ppk::kFluxComp2) * n) &&
-         hipMalloc(&h->fw.emf, (size_t)(d == 3 ? 3 : 1) * n * sizeof(double)) == hipSuccess;
-  }
+  // the three-kernel form's records, fluxes and EMFs; the single-pass 3D step needs none of
+  // them (they are allocated when the surrogate or a face dump first asks for them)
+  if (ok && h->fused && h->dp.march == 0) ok = alloc_records(h);
   if (!ok) return bail(PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
   if (hipHostMalloc((void **)&h->ctrl_host, sizeof(double) * (ppk::CTRL_SLOTS + 1)) != hipSuccess)
     return bail(PPK_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
@@ -503,6 +529,7 @@ int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
     HIP_CHECK(h, h->K->prim_dt(h->dp, h->cur(), h->Q, h->parts, h->ctrl, 1, h->stream, h->tm()));
   }
   const int nparts = h->fused ? ppk::fused_cfl_parts(h->dp) : ppk::cfl_blocks(h->dp);
+  if (h->fused) h->nparts = nparts;
   HIP_CHECK(h, h->K->set_dt(h->ctrl, h->parts, nparts, 0.0,
                             ppk::SETDT_REDUCE_ONLY, h->p.cfl, h->stream));
   int rc = read_ctrl(h);
@@ -524,6 +551,8 @@ int ppk_mhd_dump_faces(ppk_mhd_t *h, ppk_faceset_t *set, int stride, int phase,
   const long long n = h->K->surrogate_dump_slots(h->dp, stride, phase);
   if ((size_t)n > set->capacity - set->n)
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "face set too small for this dump");
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  if (!alloc_records(h)) return fail(h, PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
   // the faces the next CFL step will solve: ghost fill, CFL dt, half-step records
   HIP_CHECK(h, h->K->begin(h->ctrl, 1, h->stream));
   HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
@@ -535,6 +564,7 @@ int ppk_mhd_dump_faces(ppk_mhd_t *h, ppk_faceset_t *set, int stride, int phase,
                                     set->T + set->n * ppk::kMlpOut, stride, phase, h->stream));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));
   h->cfl_valid = true;
+  h->nparts = ppk::fused_cfl_parts(h->dp);
   set->n += (size_t)n;
   if (appended) *appended = (size_t)n;
   return PPK_OK;
@@ -555,6 +585,7 @@ int ppk_mhd_set_surrogate(ppk_mhd_t *h, const ppk_mlp_t *net, double gate_rho, d
     return fail(h, PPK_ERR_UNSUPPORTED, "the surrogate replaces HLLD: riemann must be hlld");
   if (!(gate_rho >= 0.0) || !(gate_p >= 0.0))
     return fail(h, PPK_ERR_INVALID_ARGUMENT, "gate thresholds must be >= 0");
+  if (!alloc_records(h)) return fail(h, PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
   if (!h->sur_counts) {
     const size_t bytes = 2 * ppk::kSurrogateBlocks * sizeof(unsigned long long);
     HIP_CHECK(h, hipMalloc(&h->sur_counts, bytes));
@@ -612,6 +643,7 @@ int ppk_mhd_run(ppk_mhd_t *h, int nsteps, double dt, int dt_mode) {
   // steps after finished() are no-ops on the device; the buffer parity is re-read at the end
   const long long start = h->iter_host;
   const bool cfl_valid0 = h->cfl_valid;
+  const int nparts0 = h->nparts;
   for (int s = 0; s < nsteps; ++s) {
     int rc = launch_step(h, dt, dt_mode, 1);
     if (rc != PPK_OK) return rc;
@@ -621,7 +653,10 @@ int ppk_mhd_run(ppk_mhd_t *h, int nsteps, double dt, int dt_mode) {
   if (rc != PPK_OK) return rc;
   h->iter_host = (long long)h->ctrl_host[ppk::CTRL_ITER];
   // no step ran (already finished): the CFL partials are as they were before the call
-  if (h->iter_host == start) h->cfl_valid = cfl_valid0;
+  if (h->iter_host == start) {
+    h->cfl_valid = cfl_valid0;
+    h->nparts = nparts0;
+  }
   return PPK_OK;
 }
 

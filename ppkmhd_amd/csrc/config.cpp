@@ -200,7 +200,10 @@ int fill_from_ini(const IniMap &m, ppk_mhd_params *p, ppk_mhd_problem *pr, char 
   P.precision = prec == "fp64_strict" ? PPK_FP64_STRICT
                 : (prec == "fp32_bulk" ? PPK_FP32_BULK : PPK_FP64);
   std::string path = lower(m.get_string("other", "kernel_path", "auto"));
-  P.kernel_path = path == "staged" ? PPK_PATH_STAGED : (path == "fused" ? PPK_PATH_FUSED : PPK_PATH_AUTO);
+  P.kernel_path = path == "staged"           ? PPK_PATH_STAGED
+                  : path == "fused"          ? PPK_PATH_FUSED
+                  : path == "fused_records" ? PPK_PATH_FUSED_RECORDS
+                                             : PPK_PATH_AUTO;
   P.implementation_version = (int)m.get_integer("OTHER", "implementationVersion", 0);
   // SolverBase: gravity enabled when [gravity] static or point (SolverBase.cpp:138-141); the
   // point-mass field is the disk problem's, which is not provided, so either key gives the

@@ -37,6 +37,7 @@ enum Stage {
   ST_FUSED_RIEMANN,  // fused: Riemann problems at faces + EMFs at edges
   ST_FUSED_UPDATE,   // fused: update + CT + ghost copy (+ CFL of the new state)
   ST_FUSED_CFL,      // fused: standalone CFL reduction
+  ST_FUSED_MARCH,    // single-pass 3D step (half step, solves, update, CT, CFL in one kernel)
   ST_COUNT
 };
 // host-side hook the launchers call around each stage (implemented by the driver)
@@ -150,6 +151,17 @@ inline Brick sub_bricks(const DevParams &p) {
   return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1, p.brick);
 }
 
+// The single-pass 3D step (march3d.hip): blocks of 14 x 14 output columns marching through
+// chunks of p.march planes; one CFL partial per block
+constexpr int kMarchOut = 14;
+constexpr int kMarchPlanes = 32;  // default planes per chunk
+inline long long march_blocks(const DevParams &p) {
+  if (p.march <= 0) return 0;  // the single-pass kernel is not used
+  const long long tx = (p.nx + kMarchOut - 1) / kMarchOut, ty = (p.ny + kMarchOut - 1) / kMarchOut;
+  const long long tz = (p.nz + p.march - 1) / p.march;
+  return tx * ty * tz;
+}
+
 // half-step tile shapes of a batch (batch_block_counts / batch_step `tile`): 2D 32x8, 64x4,
 // 16x16; 3D 16x4x4, 8x8x4, 32x4x2
 constexpr int kBatchTiles = 3;
@@ -201,6 +213,9 @@ struct BatchMember {
   hipError_t surrogate_dump(const DevParams &p, const void *rec, const double *ctrl, float *X,   \
                             float *T, long long stride, long long phase, hipStream_t s);         \
   long long surrogate_dump_slots(const DevParams &p, long long stride, long long phase);         \
+  /* single-pass 3D step (p.march > 0): CFL partials, one per block, when fold_cfl */           \
+  hipError_t march_godunov(const DevParams &p, const double *Uin, double *Uout, double *parts,   \
+                           const double *ctrl, int fold_cfl, hipStream_t s, StageTimer *tm);    \
   /* batched ensemble: counts of blocks per stage for one member, then the batched step */      \
   void batch_block_counts(const DevParams &p, int tile, long long counts[BS_COUNT]);            \
   hipError_t batch_step(const BatchMember *members, int nmembers, int dim, int bulk32, int tile, \
@@ -223,6 +238,7 @@ struct KernelTable {
   decltype(&fast::fused_half) fused_half;
   decltype(&fast::surrogate_dump) surrogate_dump;
   decltype(&fast::surrogate_dump_slots) surrogate_dump_slots;
+  decltype(&fast::march_godunov) march_godunov;
   decltype(&fast::batch_block_counts) batch_block_counts;
   decltype(&fast::batch_step) batch_step;
 };

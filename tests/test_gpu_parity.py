@@ -15,7 +15,9 @@ from test_golden_cpu import load_case
 
 pytestmark = pytest.mark.gpu
 
-PATHS = [A.PATH_STAGED, A.PATH_AUTO]
+# staged; the default (3D FP64: the single-pass kernel; 2D: the three-kernel fused form); the
+# three-kernel fused form (half-step records in HBM) in 3D too
+PATHS = [A.PATH_STAGED, A.PATH_AUTO, A.PATH_FUSED_RECORDS]
 
 
 def solver(p, pr=None):
@@ -82,12 +84,14 @@ def test_strict_is_bit_identical(oracle, gpu, case, path):
     assert (tg, dtg, itg) == (to, dto, ito)
 
 
-def test_strict_at_size_many_tiles(oracle, gpu):
-    """A 70 x 60 x 50 blast (MC slopes, an oblique field) whose 1280 half-step tiles outnumber
-    the blocks resident on the GPU, and whose sub-box bricks are partial in every direction:
-    strict bit-identical to the oracle over 3 CFL steps."""
+@pytest.mark.parametrize("path", [A.PATH_FUSED, A.PATH_FUSED_RECORDS])
+def test_strict_at_size_many_tiles(oracle, gpu, path):
+    """A 70 x 60 x 50 blast (MC slopes, an oblique field) with more tiles than blocks resident
+    on the GPU, partial tiles and chunks in every direction (single-pass kernel: 5 x 5 tiles of
+    14 x 14 columns, chunks of 32 planes; three-kernel form: 1280 half-step tiles, partial
+    bricks): strict bit-identical to the oracle over 3 CFL steps."""
     p = make_params(3, 70, 60, 50, gamma0=5 / 3, cfl=0.8, tEnd=1.0, slope_type=2.0,
-                    precision=A.FP64_STRICT, kernel_path=A.PATH_FUSED)
+                    precision=A.FP64_STRICT, kernel_path=path)
     pr = default_problem(A.PROBLEM_BLAST, p)
     pr.blast_radius = 0.3
     pr.blast_bx, pr.blast_by, pr.blast_bz = 0.6, -0.3, 0.2
