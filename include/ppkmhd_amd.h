@@ -256,6 +256,19 @@ int ppk_mhd_make_boundaries(ppk_mhd_t *h);
  * Replaces: SolverMHDMuscl::compute_dt_local (src/muscl/SolverMHDMuscl.h:722-741). */
 int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops): ppk_mhd_compute_dt on a caller-owned device
+ * state dU (this library's layout, 8*isize*jsize*ksize doubles) without a solver handle and
+ * without any allocation.  Fills dU's ghosts in place, then writes dt = cfl / invDt (the
+ * value ppk_mhd_compute_dt returns for the same state) to the device double *dt, all on
+ * `stream` (a hipStream_t; NULL = the null stream); does not synchronize.  The kernels' control
+ * block and CFL partial maxima live in one 8 MiB device array per GPU, kept until the process
+ * ends; grids of more than 2^28 padded cells are refused with PPK_ERR_INVALID_ARGUMENT.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other streams or threads)
+ * race and must be serialised by the caller. */
+int ppk_mhd_compute_dt_noalloc(const ppk_mhd_params *params, double *dU, double *dt,
+                               void *stream);
+
 /*
  * One time step (godunov_unsplit_impl, src/muscl/SolverMHDMuscl.cpp:373-572): ghost fill,
  * primitives, dt (per dt_mode), trace, fluxes, EMFs, flux update and CT update, written to

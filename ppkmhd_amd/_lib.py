@@ -44,6 +44,7 @@ SIGNATURES = {
     "ppk_mhd_stream": (C.c_int, [_H, C.POINTER(C.c_void_p)]),
     "ppk_mhd_make_boundaries": (C.c_int, [_H]),
     "ppk_mhd_compute_dt": (C.c_int, [_H, _DP]),
+    "ppk_mhd_compute_dt_noalloc": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ppk_mhd_step": (C.c_int, [_H, C.c_double, C.c_int]),
     "ppk_mhd_run": (C.c_int, [_H, C.c_int, C.c_double, C.c_int]),
     "ppk_mhd_get_time": (C.c_int, [_H, _DP, _DP, C.POINTER(C.c_int64)]),

@@ -55,6 +55,7 @@ UNITS = [
     ("hydro.hip", "hydro_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("mlp.hip", "mlp.o", []),
     ("diag.hip", "diag.o", []),
+    ("noalloc.hip", "noalloc.o", ["-ffp-contract=off"]),
     ("capi.cpp", "capi.o", ["--offload-host-only"]),
     ("config.cpp", "config.o", []),
     ("ics.cpp", "ics.o", ["-ffp-contract=off"]),
