@@ -54,7 +54,9 @@ STEP_BYTES_PER_CELL = 128.0
 # FP64 (profiles/fused_blast3d_256_pmc.md): per launch 8.77 GB (riemann), 7.99 GB (half),
 # 5.35 GB (update), over the items of the table above
 MEASURED_BYTES_PER_ITEM_3D = {"fused_riemann": 8.77e9 / 257 ** 3, "fused_half": 7.99e9 / 258 ** 3,
-                              "fused_update": 5.35e9 / 256 ** 3}
+                              "fused_update": 5.35e9 / 256 ** 3,
+                              # the single-pass kernel (profiles/march_blast3d_256_pmc.md)
+                              "fused_march": 3.37e9 / 256 ** 3}
 
 
 # The single-pass kernel does the work of the three fused stages in one launch
@@ -65,7 +67,7 @@ def items(stage, n):
     return {"primitives_dt": (n + 5) ** 3, "elec_field": (n + 4) ** 3, "mag_slopes": (n + 4) ** 3,
             "trace": (n + 3) ** 3, "fluxes": (n + 1) ** 3, "emf": (n + 1) ** 3,
             "update_ct": n ** 3, "fused_half": (n + 2) ** 3, "fused_riemann": (n + 1) ** 3,
-            "fused_update": n ** 3}.get(stage)
+            "fused_update": n ** 3, "fused_march": n ** 3}.get(stage)
 
 
 def step_flops_per_cell(n):
