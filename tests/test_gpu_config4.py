@@ -98,22 +98,28 @@ def test_zero_network_is_the_llf_scheme_3d(oracle, gpu):
 
 
 def test_trained_network_gains_over_llf_3d(gpu):
-    """Train on the 3D faces of the solver's own Kelvin-Helmholtz run (another seed), then
-    compare, against plain HLLD after 60 steps: the gated scheme with the trained network and
-    with no network (LLF at gated faces)."""
-    from ppkmhd_amd.surrogate import MLP, FaceSet, attach, counts, dump_faces
+    """Train on the 3D faces of the solver's own Kelvin-Helmholtz run (another seed) that pass
+    the gate (the faces the network is asked about), then compare, against plain HLLD after 60
+    steps: the gated scheme with the trained network and with no network (LLF at gated faces).
+    Trained on every face instead, the few faces with strong jumps set the output scales and
+    the network fits the smooth faces' small corrections too coarsely: the run then drifts
+    from HLLD (measured: density L1 0.13 after 60 steps, 61 % of the faces refused)."""
+    from ppkmhd_amd.surrogate import (MLP, FaceSet, attach, counts, dump_faces, select_gated,
+                                      train_schedule)
     n = 32
     p, pr = kh3d(n, seed=7)
     src = solver(p, pr)
     per = 3 * (n + 1) ** 3
-    fs = FaceSet(per * 20)
+    every = FaceSet(per * 20)
     for _ in range(20):
-        dump_faces(src, fs, 1, 0)
+        dump_faces(src, every, 1, 0)
         src.run(4)
+    fs = select_gated(every, 0.1, 0.1)
+    every.close()
     net = MLP(9)
     net.standardize(fs)
     mse0 = net.evaluate(fs)
-    net.train(fs, steps=3000, batch=4096, lr=1e-3)
+    train_schedule(net, fs, 12000, batch=4096, lr=1e-3, seed=1)
     mse1 = net.evaluate(fs)
     print(f"3D: trained on {len(fs)} faces: standardised MSE {mse0.mean():.3e} -> "
           f"{mse1.mean():.3e}")
