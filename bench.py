@@ -393,16 +393,18 @@ def bench_ensemble(a):
     print(json.dumps(out), flush=True)
 
 
-def train_surrogate(a):
+def train_surrogate(a, kh3d=False):
     """The surrogate network trained on faces dumped from the solver's own runs (the
-    Orszag-Tang 256^2 run to t = 0.5, ensemble members, a 3D blast), restricted to the faces
-    that pass the gate; returns (network, statistics)."""
+    Orszag-Tang 256^2 run to t = 0.5, ensemble members, a 3D blast; with kh3d also a 64^3
+    Kelvin-Helmholtz run of another seed, a quarter of the samples more), restricted to the
+    faces that pass the gate; returns (network, statistics)."""
     from ppkmhd_amd.ensemble import ot_ensemble
     from ppkmhd_amd.solver import SolverMHDMuscl
     from ppkmhd_amd import surrogate as SG
 
     t0 = time.perf_counter()
-    fs = SG.FaceSet(a.samples + (1 << 22))
+    kh_budget = a.samples // 4 if kh3d else 0
+    fs = SG.FaceSet(a.samples + kh_budget + (1 << 22))
     # Orszag-Tang 256^2 to t = 0.5, every 4th step
     p = make_params(2, 256, 256, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0,
                     riemann=A.RIEMANN_HLLD, bc=A.BC_PERIODIC, tEnd=0.5, n_step_max=1 << 30)
@@ -428,6 +430,17 @@ def train_surrogate(a):
             m.run(60)
         m.close()
     n_ens = len(fs) - n_ot
+    if kh3d:  # the 3D workload's own kind of flow: Kelvin-Helmholtz 64^3, another seed
+        q, qr = kh3d_problem(64, a.seed + 100, A.FP32_BULK, 10 ** 9)
+        k3 = SolverMHDMuscl(q, qr)
+        faces3 = 3 * 65 ** 3
+        stride = max(1, -(-faces3 * 60 // kh_budget))  # ~60 snapshots, 5 steps apart
+        start = len(fs)
+        while len(fs) + faces3 // stride + 1 <= start + kh_budget and not k3.finished():
+            SG.dump_faces(k3, fs, stride, 0)
+            k3.run(5)
+        k3.close()
+    n_kh = len(fs) - n_ot - n_ens
     # 3D blast 128^3, faces rotated to x (the network sees the normal direction as x)
     q = make_params(3, 128, 128, 128, gamma0=5.0 / 3.0, cfl=0.8, tEnd=1.0)
     qr = default_problem(A.PROBLEM_BLAST, q)
@@ -438,7 +451,7 @@ def train_surrogate(a):
     b3 = SolverMHDMuscl(q, qr)
     b3.run(40)
     faces3 = 3 * 129 ** 3
-    room = a.samples - len(fs)
+    room = a.samples - (len(fs) - n_kh)
     if room > 0:
         SG.dump_faces(b3, fs, max(1, faces3 // room + 1), 0)
     b3.close()
@@ -446,8 +459,9 @@ def train_surrogate(a):
     train_set = SG.select_gated(fs, a.gate, a.gate)
     fs.close()
     t_dump = time.perf_counter() - t0
-    log(f"[bench] samples: {n_all} dumped (Orszag-Tang {n_ot}, ensemble {n_ens}, blast "
-        f"{n_all - n_ot - n_ens}), {len(train_set)} pass the gate; {t_dump:.1f} s")
+    log(f"[bench] samples: {n_all} dumped (Orszag-Tang {n_ot}, ensemble {n_ens}, "
+        f"Kelvin-Helmholtz 3D {n_kh}, blast {n_all - n_ot - n_ens - n_kh}), {len(train_set)} "
+        f"pass the gate; {t_dump:.1f} s")
     net = SG.MLP(a.seed)
     net.standardize(train_set)
     mse0 = net.evaluate(train_set).mean()
@@ -457,7 +471,8 @@ def train_surrogate(a):
     mse1 = net.evaluate(train_set).mean()
     log(f"[bench] training: {a.train_steps} steps of 4096 in {t_train:.1f} s, standardised MSE "
         f"{mse0:.3e} -> {mse1:.3e}")
-    stats = {"samples_dumped": n_all, "samples_gated": len(train_set), "steps": a.train_steps,
+    stats = {"samples_dumped": n_all, "samples_kelvin_helmholtz_3d": n_kh,
+             "samples_gated": len(train_set), "steps": a.train_steps,
              "batch": 4096, "seconds": t_train, "mse_standardised_before": mse0,
              "mse_standardised_after": mse1}
     train_set.close()
@@ -529,29 +544,36 @@ def bench_surrogate(a):
     print(json.dumps(out), flush=True)
 
 
+def kh3d_problem(n, seed, precision, n_step_max):
+    """The 3D surrogate workload's initial condition: MHD Kelvin-Helmholtz on [-1/2, 1/2]^3,
+    shear layer normal to y (rho 2 for |y| <= 1/4, 1 outside; vx = -/+ 0.5), seeded uniform
+    perturbations of amplitude 0.01 on vx and vy, p 2.5, B = (0.1, 0, 0), gamma 1.4."""
+    p = make_params(3, n, n, n, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5,
+                    zmax=0.5, gamma0=1.4, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
+                    bc=A.BC_PERIODIC, tEnd=1.0e9, n_step_max=n_step_max, precision=precision)
+    pr = default_problem(A.PROBLEM_KELVIN_HELMHOLTZ, p)
+    pr.kh_p_rand, pr.kh_p_sine, pr.kh_p_sine_rob = 1, 0, 0
+    pr.kh_axis = 1                       # shear layer normal to y
+    pr.kh_d_in, pr.kh_d_out = 2.0, 1.0   # rho = 2 for |y| <= 0.25, 1 outside
+    pr.kh_vflow_in, pr.kh_vflow_out = -0.5, 0.5
+    pr.kh_pressure = 2.5
+    pr.kh_amplitude = 0.01
+    pr.kh_seed = seed
+    pr.kh_bx, pr.kh_by, pr.kh_bz = 0.1, 0.0, 0.0
+    return p, pr
+
+
 def bench_surrogate3d(a):
     """Surrogate form, 3D (config 4): MHD Kelvin-Helmholtz at n^3 (default 512^3) on
     [-1/2, 1/2]^3, FP32 bulk with FP64 state and CT, 300 steps, plain HLLD against the gated
-    surrogate; time and density L1 against plain."""
-    net, stats = train_surrogate(a)
+    surrogate (trained with 64^3 Kelvin-Helmholtz faces of another seed as well); time and
+    density L1 against plain."""
+    net, stats = train_surrogate(a, kh3d=True)
     n = a.n if a.n != 256 else 512
     steps = 300
 
     def kh():
-        p = make_params(3, n, n, n, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5,
-                        zmax=0.5, gamma0=1.4, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
-                        bc=A.BC_PERIODIC, tEnd=1.0e9, n_step_max=steps + a.warmup,
-                        precision=A.FP32_BULK)
-        pr = default_problem(A.PROBLEM_KELVIN_HELMHOLTZ, p)
-        pr.kh_p_rand, pr.kh_p_sine, pr.kh_p_sine_rob = 1, 0, 0
-        pr.kh_axis = 1                       # shear layer normal to y
-        pr.kh_d_in, pr.kh_d_out = 2.0, 1.0   # rho = 2 for |y| <= 0.25, 1 outside
-        pr.kh_vflow_in, pr.kh_vflow_out = -0.5, 0.5
-        pr.kh_pressure = 2.5
-        pr.kh_amplitude = 0.01
-        pr.kh_seed = a.seed
-        pr.kh_bx, pr.kh_by, pr.kh_bz = 0.1, 0.0, 0.0
-        return p, pr
+        return kh3d_problem(n, a.seed, A.FP32_BULK, steps + a.warmup)
 
     t0 = time.perf_counter()
     runs = run_pair(a, kh, n ** 3, net)

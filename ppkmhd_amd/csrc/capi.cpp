@@ -341,9 +341,13 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
   h->fused = ppk::fused_supported(h->dp, params->precision, params->kernel_path);
   // FP64 3D: the single-pass kernel, unless the three-kernel form is asked for (the FP32-bulk
   // form and the surrogate run the three-kernel form)
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 0;
   h->dp.march = (params->dim == 3 && h->fused && params->precision != PPK_FP32_BULK &&
                  params->kernel_path != PPK_PATH_FUSED_RECORDS)
-                    ? ppk::kMarchPlanes
+                    ? ppk::march_planes(h->dp, cus)
                     : 0;
   if (params->kernel_path == PPK_PATH_FUSED && !h->fused) {
     delete h;
