@@ -27,6 +27,9 @@ from ppkmhd_amd import default_problem, make_params  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E, GB/s
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix), TFLOP/s
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, TFLOP/s
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 matrix, TFLOP/s
+MLP_FLOP_PER_FACE = 2 * (16 * 128 + 128 * 128 + 128 * 8)  # 16 -> 128 -> 128 -> 8
 
 # Algorithmic FP64 operations (add + mul + div + sqrt, counted by the instrumented oracle) per
 # work item of each stage, 3D; the item ranges are those of the reference's functors.
@@ -479,9 +482,47 @@ def train_surrogate(a, kh3d=False):
     return net, stats
 
 
+def surrogate_net(a, kh3d=False):
+    """train_surrogate, or with --net-file a network read from that file (a flat FP32
+    parameter vector in .npy form) when it exists, else trained and then written there."""
+    from ppkmhd_amd import surrogate as SG
+
+    if a.net_file and os.path.exists(a.net_file):
+        net = SG.MLP(a.seed)
+        net.set_params(np.load(a.net_file))
+        log(f"[bench] network read from {a.net_file}")
+        return net, {"net_file": a.net_file}
+    net, stats = train_surrogate(a, kh3d)
+    if a.net_file:
+        np.save(a.net_file, net.get_params())
+    return net, stats
+
+
+PROBE_STEPS = 3
+
+
+def surrogate_roofline(dim, faces_per_dir, frac_net, probe):
+    """Roofline of the network's face kernel (k_face_sur, one launch per direction): the
+    network's bf16 FLOPs for the faces it served (faces of a direction x the fraction by the
+    network) over the kernel's HIP-event time per launch, against the dense bf16 peak."""
+    ms = probe.get("surrogate_faces")
+    if not ms:
+        return None
+    per_launch = ms / dim
+    flops = MLP_FLOP_PER_FACE * faces_per_dir * frac_net
+    achieved = flops / (per_launch * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_face_sur",
+            "kernel_ms": per_launch, "flop_per_launch": flops,
+            "note": "network FLOPs of the faces it served; the kernel also computes every "
+                    "face's LLF pieces, features and admissibility check on the VALU"}
+
+
 def run_pair(a, make, n_cells, net):
     """The same run with plain HLLD and with the gated surrogate: (states, seconds, steps,
-    counts) per form; the timed part excludes `a.warmup` steps."""
+    counts, stage probe) per form.  After `a.warmup` steps, PROBE_STEPS steps run with every
+    stage timer on (HIP events on the solver's stream: {stage: ms per launch}); the timed part
+    is the rest of the run, timers off."""
     from ppkmhd_amd.solver import SolverMHDMuscl
     from ppkmhd_amd import surrogate as SG
 
@@ -493,6 +534,12 @@ def run_pair(a, make, n_cells, net):
             SG.attach(s, net, a.gate, a.gate)
         s.run(a.warmup)
         s.synchronize()
+        s.timers_enable(None)
+        s.run(PROBE_STEPS)
+        probe = {k: v[0] / v[1] for k, v in s.timers_read().items()}
+        s.timers_enable([])
+        s.synchronize()
+        it0 = s.time()[2]
         SG.counts(s, reset=True)
         t0 = time.perf_counter()
         while not s.finished():
@@ -500,8 +547,9 @@ def run_pair(a, make, n_cells, net):
         s.synchronize()
         el = time.perf_counter() - t0
         t, dt, it = s.time()
-        steps = it - a.warmup
-        runs[name] = (s.download(), el, steps, SG.counts(s) if name == "surrogate" else None)
+        steps = it - it0
+        runs[name] = (s.download(), el, steps, SG.counts(s) if name == "surrogate" else None,
+                      probe)
         log(f"[bench] {name}: {steps} steps to t={t:.4f} in {el:.3f} s, "
             f"{el / steps * 1e3:.3f} ms/step, {n_cells * steps / el / 1e9:.3f} G cell-updates/s")
         s.close()
@@ -511,7 +559,7 @@ def run_pair(a, make, n_cells, net):
 def bench_surrogate(a):
     """Surrogate form, 2D (config 3): Orszag-Tang at n^2 to t = 0.5 with plain HLLD and with
     the gated surrogate; time and density L1 against plain."""
-    net, stats = train_surrogate(a)
+    net, stats = surrogate_net(a)
 
     def ot(n, tEnd):
         p = make_params(2, n, n, gamma0=5.0 / 3.0, cfl=0.8, slope_type=1.0, riemann=A.RIEMANN_HLLD,
@@ -520,8 +568,8 @@ def bench_surrogate(a):
 
     n = a.n if a.n != 256 else 1024
     runs = run_pair(a, lambda: ot(n, 0.5), n * n, net)
-    Up, ep, sp, _ = runs["plain"]
-    Us, es, ss, (by_net, by_hlld) = runs["surrogate"]
+    Up, ep, sp, _, probe_p = runs["plain"]
+    Us, es, ss, (by_net, by_hlld), probe_s = runs["surrogate"]
     p, _ = ot(n, 0.5)
     ii = p.interior()
     l1 = float(np.abs(Us[ii][0] - Up[ii][0]).sum() / np.abs(Up[ii][0]).sum())
@@ -539,7 +587,10 @@ def bench_surrogate(a):
         "plain_value": v_plain, "speed_ratio": v_sur / v_plain, "density_l1_vs_plain": l1,
         "faces_by_network": by_net / max(1, by_net + by_hlld),
         "training": stats,
-        "roofline": None, "cpu_baseline": None,
+        "stage_ms_plain": probe_p, "stage_ms_surrogate": probe_s,
+        "roofline": surrogate_roofline(2, (n + 1) ** 2, by_net / max(1, by_net + by_hlld),
+                                       probe_s),
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
@@ -568,7 +619,7 @@ def bench_surrogate3d(a):
     [-1/2, 1/2]^3, FP32 bulk with FP64 state and CT, 300 steps, plain HLLD against the gated
     surrogate (trained with 64^3 Kelvin-Helmholtz faces of another seed as well); time and
     density L1 against plain."""
-    net, stats = train_surrogate(a, kh3d=True)
+    net, stats = surrogate_net(a, kh3d=True)
     n = a.n if a.n != 256 else 512
     steps = 300
 
@@ -578,8 +629,8 @@ def bench_surrogate3d(a):
     t0 = time.perf_counter()
     runs = run_pair(a, kh, n ** 3, net)
     log(f"[bench] both runs (with initial conditions) {time.perf_counter() - t0:.1f} s")
-    Up, ep, sp, _ = runs["plain"]
-    Us, es, ss, (by_net, by_hlld) = runs["surrogate"]
+    Up, ep, sp, _, probe_p = runs["plain"]
+    Us, es, ss, (by_net, by_hlld), probe_s = runs["surrogate"]
     p, _ = kh()
     ii = p.interior()
     l1 = float(np.abs(Us[ii][0] - Up[ii][0]).sum() / np.abs(Up[ii][0]).sum())
@@ -597,7 +648,10 @@ def bench_surrogate3d(a):
                    "precision": "fp32_bulk"},
         "plain_value": v_plain, "speed_ratio": v_sur / v_plain, "density_l1_vs_plain": l1,
         "faces_by_network": by_net / max(1, by_net + by_hlld), "training": stats,
-        "roofline": None, "cpu_baseline": None,
+        "stage_ms_plain": probe_p, "stage_ms_surrogate": probe_s,
+        "roofline": surrogate_roofline(3, (n + 1) ** 3, by_net / max(1, by_net + by_hlld),
+                                       probe_s),
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
@@ -630,6 +684,9 @@ def main():
     ap.add_argument("--gate", type=float, default=0.005)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--net-file", default="",
+                    help="surrogate workloads: read the network from this .npy file if it "
+                         "exists, else train it and write it there")
     ap.add_argument("--tune-steps", type=int, default=3)
     ap.add_argument("--seq-steps", type=int, default=10)
     a = ap.parse_args()
@@ -695,11 +752,16 @@ def main():
     roof = None
     if flops is not None:
         achieved = flops / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+        # FP32 bulk: the half step is float arithmetic; the solve stage's FLOPs are mostly the
+        # FP64 EMF solves, so it stays priced at the FP64 peak
+        f32 = prec == A.FP32_BULK and dom == "fused_half"
+        peak = PEAK_FP32_TFLOPS if f32 else PEAK_FP64_TFLOPS
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None,
                 "kernel": {"fused_march": "k_march3d"}.get(dom, dom),
                 "kernel_ms": avg_ms, "flop_per_launch": flops,
-                "note": "FP64 VALU-bound stage; peak is the MI355X FP64 figure"}
+                "note": ("FP32 VALU stage; peak is the MI355X FP32 vector figure" if f32 else
+                         "FP64 VALU-bound stage; peak is the MI355X FP64 figure")}
         if nbytes is not None:
             gbs = nbytes / (avg_ms * 1e-3) / 1e9
             roof["hbm_achieved_GBs"] = gbs
@@ -707,6 +769,7 @@ def main():
         if dom in MEASURED_BYTES_PER_ITEM_3D and prec == A.FP64:
             roof["traffic"] = MEASURED_BYTES_PER_ITEM_3D[dom] * items(dom, n)
             roof["traffic_GBs"] = roof["traffic"] / (avg_ms * 1e-3) / 1e9
+    prec_label = "FP32 bulk" if prec == A.FP32_BULK else "FP64"
     step_hbm = STEP_BYTES_PER_CELL * value / 1e9
     step_tflops = step_flops_per_cell(n) * value / 1e12
 
@@ -719,7 +782,7 @@ def main():
             log(f"[bench] CPU baseline failed: {e!r}")
 
     out = {
-        "metric": "MHD cell-updates/s (MUSCL-HLLD, 3D 256^3 FP64)",
+        "metric": f"MHD cell-updates/s (MUSCL-HLLD, 3D {n}^3 {prec_label})",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -731,7 +794,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec != A.FP32_BULK else "f32 bulk (f64 state, update, CT, dt)",
         "data": "synthetic (analytic blast initial condition)",
-        "config": {"workload": f"3D MHD blast {n}^3 FP64, periodic, HLLD+CT, minmod, CFL 0.8",
+        "config": {"workload": f"3D MHD blast {n}^3 {prec_label}, periodic, HLLD+CT, minmod, "
+                               f"CFL 0.8",
                    "global_batch": 1, "seq_len": None, "parallelism": "single GPU",
                    "kernel_path": a.path, "precision": a.precision},
         "value_ghost_inclusive": (n + 6) ** 3 * a.steps / elapsed,

@@ -48,9 +48,12 @@ UNITS = [
      ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("batch.hip", "batch_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("batch.hip", "batch_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
-    ("surrogate.hip", "surrogate_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
+    # the network's accumulators in VGPRs (no copies out of AGPRs before each ReLU)
+    ("surrogate.hip", "surrogate_fast.o",
+     ["-DPPK_NS=fast", "-ffp-contract=fast", "-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("surrogate.hip", "surrogate_strict.o",
-     ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
+     ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off", "-mllvm",
+      "-amdgpu-mfma-vgpr-form"]),
     ("hydro.hip", "hydro_fast.o", ["-DPPK_NS=fast", "-ffp-contract=fast"]),
     ("hydro.hip", "hydro_strict.o", ["-DPPK_NS=strict", "-DPPK_EXACT_DIV", "-ffp-contract=off"]),
     ("mlp.hip", "mlp.o", []),

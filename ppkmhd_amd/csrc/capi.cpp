@@ -55,8 +55,8 @@ constexpr int kDiagBlocks = 512;
 const char *const kStageNames[ST_COUNT] = {
     "boundaries", "primitives_dt", "elec_field", "mag_slopes",    "trace",        "fluxes", "emf",
     "update_ct",  "control",       "fused_half", "fused_riemann", "fused_update", "fused_cfl",
-    "fused_march"};
-static_assert(ST_COUNT == 14, "stage names");
+    "fused_march", "surrogate_faces", "surrogate_fallback"};
+static_assert(ST_COUNT == 16, "stage names");
 
 // the fused kernels serve every precision in 2D and 3D unless the staged path is asked for
 bool fused_supported(const DevParams &p, int precision, int kernel_path) {

@@ -11,7 +11,8 @@
 //                              of the cell and its lower neighbours.  The right-side face values
 //                              of a cell (AR, dARy, ...) are bit for bit the left-side values of
 //                              its +1 neighbour, so records hold only the left ones.  With the
-//                              surrogate, k_edge{3,2}d solve the edges after the learned faces.
+//                              surrogate, k_edges3d / k_edge2d solve the edges after the
+//                              learned faces.
 //   k_update                   flux update + constrained transport, ghost copy, and the CFL
 //                              maximum of the new state (per-block partials) that the next
 //                              step's dt is taken from
@@ -80,14 +81,23 @@ __global__ void __launch_bounds__(Tile2::NT) k_half2d(const double *__restrict__
   half2_record<Tile2, R>(p, b0x, b0y, tid, sF, sQ, sE, dt, rec);
 }
 
-// 4 waves per SIMD: the EMF solve fits 128 VGPRs without spilling
-template <int E, class R>
+// The three edge solves in one launch (the surrogate step, whose faces are solved apart):
+// block B does edge B % 3 (z, y, x) of run B / 3, renumbered per XCD like k_riemann_mix3d so
+// that the three edges of a run read their records from one XCD's L2.  4 waves per SIMD: the
+// EMF solve fits 128 VGPRs without spilling.
+template <class R>
 __global__ void __launch_bounds__(kRiemannBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_edge3d(const R *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
-         const double *ctrl) {
+k_edges3d(const R *__restrict__ rec, double *__restrict__ emf, DevParams p, Brick bk,
+          const double *ctrl) {
   if (!active(ctrl)) return;
-  const long long c = sub_brick_cell(p, bk, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x);
-  if (c >= 0) edge3<E>(rec, emf, p, c);
+  const int B = xcd_remap(blockIdx.x, gridDim.x);
+  const long long c = sub_brick_cell(p, bk, B / 3, threadIdx.x);
+  if (c < 0) return;
+  switch (B % 3) {
+    case 0: edge3<2>(rec, emf, p, c); break;
+    case 1: edge3<1>(rec, emf, p, c); break;
+    default: edge3<0>(rec, emf, p, c); break;
+  }
 }
 // All solves of the step in one launch: block B does solve type B % 6 (faces x, y, z, edges
 // z, y, x) for the cells of run B / 6.  Blocks are renumbered per XCD first, so the six solves
@@ -238,11 +248,10 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
     PPK_TIMED(tm, ST_FUSED_HALF, launch_half3<R>(p, Uin, rec, ctrl, s));
     if (tm) tm->mark(ST_FUSED_RIEMANN, true);
     if (sur) {  // learned face fluxes, then the exact edges
-      const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s);
+      const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s, tm);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((k_edge3d<2, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edge3d<1, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
-      hipLaunchKernelGGL((k_edge3d<0, R>), g, b, 0, s, rec, emf, p, bk, ctrl);
+      hipLaunchKernelGGL((k_edges3d<R>), dim3((unsigned)(3 * bk.count())), b, 0, s, rec, emf, p,
+                         bk, ctrl);
     } else {
       launch_mix3d<R>(p, rec, flux, emf, ctrl, s);
     }
@@ -254,7 +263,7 @@ hipError_t godunov_impl(const DevParams &p, const double *Uin, double *Uout, R *
                                  p, ctrl, ntx));
     if (tm) tm->mark(ST_FUSED_RIEMANN, true);
     if (sur) {
-      const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s);
+      const hipError_t e = surrogate_faces(p, rec, flux, ctrl, *sur, s, tm);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL((k_edge2d<R>), g, b, 0, s, rec, emf, p, bk, ctrl);
     } else {

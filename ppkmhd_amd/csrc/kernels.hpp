@@ -38,6 +38,8 @@ enum Stage {
   ST_FUSED_UPDATE,   // fused: update + CT + ghost copy (+ CFL of the new state)
   ST_FUSED_CFL,      // fused: standalone CFL reduction
   ST_FUSED_MARCH,    // single-pass 3D step (half step, solves, update, CT, CFL in one kernel)
+  ST_SUR_FACES,      // surrogate form: the network's face solves (inside ST_FUSED_RIEMANN)
+  ST_SUR_FALLBACK,   // surrogate form: exact HLLD of the refused faces (same)
   ST_COUNT
 };
 // host-side hook the launchers call around each stage (implemented by the driver)
@@ -217,7 +219,8 @@ struct BatchMember {
                         hipStream_t s);                                                          \
   /* surrogate form: face solves, and the training-sample dump of the current records */        \
   hipError_t surrogate_faces(const DevParams &p, const void *rec, void *flux,                    \
-                             const double *ctrl, const SurrogateCfg &cfg, hipStream_t s);        \
+                             const double *ctrl, const SurrogateCfg &cfg, hipStream_t s,         \
+                             StageTimer *tm);                                                    \
   hipError_t surrogate_dump(const DevParams &p, const void *rec, const double *ctrl, float *X,   \
                             float *T, long long stride, long long phase, hipStream_t s);         \
   long long surrogate_dump_slots(const DevParams &p, long long stride, long long phase);         \

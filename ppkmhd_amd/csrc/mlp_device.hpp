@@ -13,6 +13,10 @@ namespace mlpd {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v2s __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -72,18 +76,34 @@ __device__ __forceinline__ v8bf ld_a32(const uint16_t *A, int tile, int lane) {
   return *reinterpret_cast<const v8bf *>(A + ((size_t)tile * 64 + lane) * 8);
 }
 
+// two floats rounded to bf16 (nearest even) in one 32-bit word, a in the low half (one
+// v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned bf16x2(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((v2f){a, b}, v2bf));
+}
+
 // bf16 B operand of a 16x16x32 step q from two accumulator tiles (features 32q .. 32q+31)
 __device__ __forceinline__ v8bf pack_pair(v4f lo, v4f hi) {
-  v8bf r;
-  r[0] = bf16b(lo[0]);
-  r[1] = bf16b(lo[1]);
-  r[2] = bf16b(lo[2]);
-  r[3] = bf16b(lo[3]);
-  r[4] = bf16b(hi[0]);
-  r[5] = bf16b(hi[1]);
-  r[6] = bf16b(hi[2]);
-  r[7] = bf16b(hi[3]);
-  return r;
+  const v4u r = {bf16x2(lo[0], lo[1]), bf16x2(lo[2], lo[3]), bf16x2(hi[0], hi[1]),
+                 bf16x2(hi[2], hi[3])};
+  return __builtin_bit_cast(v8bf, r);
+}
+
+// ReLU of the two bf16 values of a word, as 16-bit integers (one v_pk_max_i16)
+__device__ __forceinline__ unsigned relu_bf16x2(unsigned w) {
+  const v2s zero = {0, 0};
+  return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(v2s, w), zero));
+}
+
+// ReLU of two accumulator tiles, packed like pack_pair.  The tiles are rounded to bf16 first
+// and the ReLU taken on the bf16 bit patterns as 16-bit integers (one v_pk_max_i16 per pair):
+// a negative value or -0 has the sign bit set, so its pattern is a negative integer and the
+// maximum with 0 is +0; rounding to nearest even keeps the sign, so the result equals
+// bf16(max(x, 0)) for every non-NaN x.
+__device__ __forceinline__ v8bf pack_relu(v4f lo, v4f hi) {
+  const v4u r = {relu_bf16x2(bf16x2(lo[0], lo[1])), relu_bf16x2(bf16x2(lo[2], lo[3])),
+                 relu_bf16x2(bf16x2(hi[0], hi[1])), relu_bf16x2(bf16x2(hi[2], hi[3]))};
+  return __builtin_bit_cast(v8bf, r);
 }
 
 // Forward pass of one block of 16 samples.  xb: B operand of layer 1 (lane l: the bf16
@@ -164,14 +184,10 @@ __device__ __forceinline__ void mlp_infer_blocks(const MlpView &w, const uint16_
       for (int j = 0; j < 4; ++j) bias[j] = w.b1[16 * r + 4 * g + j];
       const v4s a = ld_a16(w.A1, r, lane);
 #pragma unroll
-      for (int f = 0; f < NB; ++f) {
-        const v4f c = mfma_k16(a, xb[f], bias);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) t[f][h][j] = relu(c[j]);
-      }
+      for (int f = 0; f < NB; ++f) t[f][h] = mfma_k16(a, xb[f], bias);
     }
 #pragma unroll
-    for (int f = 0; f < NB; ++f) h1b[f][q] = pack_pair(t[f][0], t[f][1]);
+    for (int f = 0; f < NB; ++f) h1b[f][q] = pack_relu(t[f][0], t[f][1]);
   }
 #pragma unroll
   for (int q2 = 0; q2 < 4; ++q2) {
@@ -191,12 +207,10 @@ __device__ __forceinline__ void mlp_infer_blocks(const MlpView &w, const uint16_
         for (int f = 0; f < NB; ++f) acc[f] = mfma_k32(a, h1b[f][q], acc[f]);
       }
 #pragma unroll
-      for (int f = 0; f < NB; ++f)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) t[f][h][j] = relu(acc[f][j]);
+      for (int f = 0; f < NB; ++f) t[f][h] = acc[f];
     }
 #pragma unroll
-    for (int f = 0; f < NB; ++f) h2b[f][q2] = pack_pair(t[f][0], t[f][1]);
+    for (int f = 0; f < NB; ++f) h2b[f][q2] = pack_relu(t[f][0], t[f][1]);
   }
   v4f y[NB];
 #pragma unroll

@@ -222,25 +222,32 @@ __global__ void __launch_bounds__(kBlock) k_dump(const R *__restrict__ rec, DevP
 
 template <class R>
 hipError_t faces_impl(const DevParams &p, const R *rec, R *flux, const double *ctrl,
-                      const SurrogateCfg &cfg, hipStream_t s) {
+                      const SurrogateCfg &cfg, hipStream_t s, StageTimer *tm) {
   const Brick bk = sub_bricks(p);
   const long long nb = bk.count();
   const dim3 g((unsigned)(nb < kSurrogateBlocks ? nb : kSurrogateBlocks)), b(kBlock);
   const dim3 gf((unsigned)nb);
   const unsigned char *fl = cfg.flags;
+  if (tm) tm->mark(ST_SUR_FACES, true);
   if (p.dim == 3) {
     hipLaunchKernelGGL((k_face_sur<3, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
     hipLaunchKernelGGL((k_face_sur<3, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
     hipLaunchKernelGGL((k_face_sur<3, 2, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+  } else {
+    hipLaunchKernelGGL((k_face_sur<2, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+    hipLaunchKernelGGL((k_face_sur<2, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+  }
+  if (tm) tm->mark(ST_SUR_FACES, false);
+  if (tm) tm->mark(ST_SUR_FALLBACK, true);
+  if (p.dim == 3) {
     hipLaunchKernelGGL((k_face_fallback<3, 0, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
     hipLaunchKernelGGL((k_face_fallback<3, 1, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
     hipLaunchKernelGGL((k_face_fallback<3, 2, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
   } else {
-    hipLaunchKernelGGL((k_face_sur<2, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
-    hipLaunchKernelGGL((k_face_sur<2, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
     hipLaunchKernelGGL((k_face_fallback<2, 0, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
     hipLaunchKernelGGL((k_face_fallback<2, 1, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
   }
+  if (tm) tm->mark(ST_SUR_FALLBACK, false);
   return hipGetLastError();
 }
 
@@ -254,12 +261,12 @@ long long surrogate_dump_slots(const DevParams &p, long long stride, long long p
 }
 
 hipError_t surrogate_faces(const DevParams &p, const void *rec, void *flux, const double *ctrl,
-                           const SurrogateCfg &cfg, hipStream_t s) {
+                           const SurrogateCfg &cfg, hipStream_t s, StageTimer *tm) {
   if (p.bulk32)
     return surk::faces_impl<float>(p, static_cast<const float *>(rec), static_cast<float *>(flux),
-                                   ctrl, cfg, s);
+                                   ctrl, cfg, s, tm);
   return surk::faces_impl<double>(p, static_cast<const double *>(rec),
-                                  static_cast<double *>(flux), ctrl, cfg, s);
+                                  static_cast<double *>(flux), ctrl, cfg, s, tm);
 }
 
 hipError_t surrogate_dump(const DevParams &p, const void *rec, const double *ctrl, float *X,

@@ -35,6 +35,14 @@ constexpr double kGammaRef = 5.0 / 3.0;
 
 namespace PPK_NS {
 
+// 1/x: in the default device build the shared-reciprocal forms of mhd_device.hpp (float: one
+// v_rcp_f32, about 1 ulp, far below the network's own error; double: the refined reciprocal),
+// the IEEE quotient in the strict build and on the host
+template <class R>
+PPK_HD R inv(R x) {
+  return kSharedRecip ? recip(x) : R(1) / x;
+}
+
 // the LLF flux and its pieces (conservative states, physical fluxes, signal speed); same
 // operations as riemann_llf
 template <class R>
@@ -50,9 +58,10 @@ PPK_HD void llf_parts(St<R> &ql, St<R> &qr, St<R> &ul, St<R> &ur, St<R> &fl, St<
   auto info = [&](const St<R> &q) {
     R d = q[ID], pp = q[IP], a = q[IA], b = q[IB], c = q[IC];
     R b2 = a * a + b * b + c * c;
-    R c2 = R(par.gamma0) * pp / d;
-    R d2 = R(0.5) * (b2 / d + c2);
-    R cf = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d));
+    const R id = inv(d);
+    R c2 = divr(R(par.gamma0) * pp, d, id);
+    R d2 = R(0.5) * (divr(b2, d, id) + c2);
+    R cf = msqrt(d2 + msqrt(nonneg(d2 * d2 - divr(c2 * a * a, d, id))));
     return cf + dabs(q[IU]);
   };
   s = dmax(info(ql), info(qr));
@@ -67,8 +76,8 @@ template <class R>
 PPK_HD void face_features(const St<R> &ql, const St<R> &qr, R s, R gamma0, float x[kMlpIn],
                           R &rho_bar) {
   rho_bar = R(0.5) * (ql[ID] + qr[ID]);
-  const R iv = R(1) / s, ib = R(1) / (sqrt(rho_bar) * s), ip = R(1) / (rho_bar * s * s);
-  const R ir = R(1) / rho_bar, h = R(0.5);
+  const R iv = inv(s), ib = inv(msqrt(rho_bar) * s), ip = inv(rho_bar * s * s);
+  const R ir = inv(rho_bar), h = R(0.5);
   x[0] = float((qr[ID] - ql[ID]) * ir);
   x[1] = float(h * (ql[IU] + qr[IU]) * iv);
   x[2] = float(h * (ql[IV] + qr[IV]) * iv);
@@ -90,7 +99,7 @@ PPK_HD void face_features(const St<R> &ql, const St<R> &qr, R s, R gamma0, float
 // physical scale of each output component
 template <class R>
 PPK_HD void out_scales(R rho_bar, R s, R sc[kMlpOut]) {
-  const R m = rho_bar * s, b = sqrt(rho_bar) * s * s;
+  const R m = rho_bar * s, b = msqrt(rho_bar) * s * s;
   sc[0] = m;
   sc[1] = m * s * s;
   sc[2] = m * s;
@@ -112,7 +121,7 @@ PPK_HD bool smooth_face(const St<R> &ql, const St<R> &qr, double gate_rho, doubl
 template <class R>
 PPK_HD R pressure_of(const St<R> &u, R gamma0) {
   const R d = u[ID];
-  const R ke = R(0.5) * (u[IU] * u[IU] + u[IV] * u[IV] + u[IW] * u[IW]) / d;
+  const R ke = divr(R(0.5) * (u[IU] * u[IU] + u[IV] * u[IV] + u[IW] * u[IW]), d, inv(d));
   const R me = R(0.5) * (u[IA] * u[IA] + u[IB] * u[IB] + u[IC] * u[IC]);
   return (gamma0 - R(1)) * (u[IP] - ke - me);
 }
@@ -122,7 +131,7 @@ template <class R>
 PPK_HD bool admissible(const St<R> &ul, const St<R> &ur, const St<R> &fl, const St<R> &fr,
                        const St<R> &F, R s, R gamma0) {
   St<R> a, b;
-  const R is = R(1) / s;
+  const R is = inv(s);
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     a[v] = ul[v] - (F[v] - fl[v]) * is;
