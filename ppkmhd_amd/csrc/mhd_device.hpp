@@ -172,21 +172,33 @@ PPK_HD R speed_fast(const St<R> &q, int dir, R gamma0) {
 
 // per-cell CFL term sum_d (c_f,d + |v_d|)/dx_d, reference src/shared/mhd_utils.h:319-366 and
 // src/muscl/MHDRunFunctors{2D,3D}.h:73/76
+// In the default build the quotients are products with shared reciprocals; wherever both
+// operands of a sum would then be products, the fused multiply-add is written out, so that
+// every kernel that inlines this function rounds it the same way (the CFL dt of a step must
+// not depend on which kernel computed it: the folded maximum of the update and the
+// standalone pass agree bit for bit).
 PPK_HD double inv_dt_cell(const St<double> &q, const DevParams &p) {
   const double gamma0 = p.gamma0;
   double d = q[ID], pp = q[IP], a = q[IA], b = q[IB], c = q[IC];
   double b2 = a * a + b * b + c * c;
-  double c2 = gamma0 * pp / d;
-  double d2 = 0.5 * (b2 / d + c2);
-  double cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * a * a / d)));
+  const double id = recip(d);
+  double c2 = divr(gamma0 * pp, d, id);
+  double d2 = 0.5 * (kSharedRecip ? fma(b2, id, c2) : b2 / d + c2);
+  auto disc = [&](double bn) {  // d2^2 - c2 bn^2 / d
+    return kSharedRecip ? fma(d2, d2, -(c2 * bn * bn * id)) : d2 * d2 - c2 * bn * bn / d;
+  };
+  double cf = msqrt(d2 + msqrt(nonneg(disc(a))));
   double vx = cf + dabs(q[IU]);
-  cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * b * b / d)));
+  cf = msqrt(d2 + msqrt(nonneg(disc(b))));
   double vy = cf + dabs(q[IV]);
   if (p.dim == 3) {
-    cf = sqrt(d2 + sqrt(nonneg(d2 * d2 - c2 * c * c / d)));
+    cf = msqrt(d2 + msqrt(nonneg(disc(c))));
     double vz = cf + dabs(q[IW]);
+    if (kSharedRecip)
+      return fma(vz, recip(p.dz), fma(vy, recip(p.dy), vx * recip(p.dx)));
     return vx / p.dx + vy / p.dy + vz / p.dz;
   }
+  if (kSharedRecip) return fma(vy, recip(p.dy), vx * recip(p.dx));
   return vx / p.dx + vy / p.dy;
 }
 
@@ -236,7 +248,7 @@ template <class R>
 PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
                                              const DevParams &par) {
   const R gamma0 = R(par.gamma0);
-  const R entho = R(1.0) / (gamma0 - R(1.0));
+  const R entho = kSharedRecip ? recip(gamma0 - R(1.0)) : R(1.0) / (gamma0 - R(1.0));
 
   R a = R(0.5) * (qleft[IA] + qright[IA]);
   R sgnm = (a >= R(0)) ? R(1) : R(-1);
@@ -378,7 +390,7 @@ PPK_HD void mhd_flux(const St<R> &q, St<R> &cvar, St<R> &ff,
                                          const DevParams &par) {
   R gamma0 = R(par.gamma0);
   R p = par.cIso > 0 ? q[ID] * R(par.cIso) * R(par.cIso) : q[IP];
-  R entho = R(1.0) / (gamma0 - R(1.0));
+  R entho = kSharedRecip ? recip(gamma0 - R(1.0)) : R(1.0) / (gamma0 - R(1.0));
   R d = q[ID], u = q[IU], v = q[IV], w = q[IW], a = q[IA], b = q[IB], c = q[IC];
   R ecin = R(0.5) * (u * u + v * v + w * w) * d;
   R emag = R(0.5) * (a * a + b * b + c * c);
@@ -507,12 +519,13 @@ PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
   R rcRLy = rRL * (vRL - SB);
   R rcRRy = rRR * (ST - vRR);
 
-  R ustar = (rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR +
-             (PtotLL - PtotRL + PtotLR - PtotRR)) /
-            (rcLLx + rcLRx + rcRLx + rcRRx);
-  R vstar = (rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR +
-             (PtotLL - PtotLR + PtotRL - PtotRR)) /
-            (rcLLy + rcLRy + rcRLy + rcRRy);
+  const R rcx = rcLLx + rcLRx + rcRLx + rcRRx, rcy = rcLLy + rcLRy + rcRLy + rcRRy;
+  R ustar = divr(rcLLx * uLL + rcLRx * uLR + rcRLx * uRL + rcRRx * uRR +
+                     (PtotLL - PtotRL + PtotLR - PtotRR),
+                 rcx, recip(rcx));
+  R vstar = divr(rcLLy * vLL + rcLRy * vLR + rcRLy * vRL + rcRRy * vRR +
+                     (PtotLL - PtotLR + PtotRL - PtotRR),
+                 rcy, recip(rcy));
 
   const R iSLu = recip(SL - ustar), iSRu = recip(SR - ustar);
   const R iSBv = recip(SB - vstar), iSTv = recip(ST - vstar);
