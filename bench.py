@@ -397,8 +397,8 @@ def bench_ensemble(a):
 
 
 def train_surrogate(a, kh3d=False):
-    """The surrogate network trained on faces dumped from the solver's own runs (the
-    Orszag-Tang 256^2 run to t = 0.5, ensemble members, a 3D blast; with kh3d also a 64^3
+    """The surrogate network trained on faces dumped from the solver's own runs of configs
+    0-2 (the Orszag-Tang 256^2 run to t = 0.5, ensemble members, the 256^3 blast; with kh3d also a 64^3
     Kelvin-Helmholtz run of another seed, a quarter of the samples more), restricted to the
     faces that pass the gate; returns (network, statistics)."""
     from ppkmhd_amd.ensemble import ot_ensemble
@@ -444,16 +444,16 @@ def train_surrogate(a, kh3d=False):
             k3.run(5)
         k3.close()
     n_kh = len(fs) - n_ot - n_ens
-    # 3D blast 128^3, faces rotated to x (the network sees the normal direction as x)
-    q = make_params(3, 128, 128, 128, gamma0=5.0 / 3.0, cfl=0.8, tEnd=1.0)
+    # 3D blast 256^3 (config 1), faces rotated to x (the network sees the normal direction as x)
+    q = make_params(3, 256, 256, 256, gamma0=5.0 / 3.0, cfl=0.8, tEnd=1.0)
     qr = default_problem(A.PROBLEM_BLAST, q)
     qr.blast_radius = 0.1
     qr.blast_density_out = 1.0
     qr.blast_pressure_in, qr.blast_pressure_out = 10.0, 0.1
     qr.blast_bx = qr.blast_by = qr.blast_bz = 1.0 / np.sqrt(3.0)
     b3 = SolverMHDMuscl(q, qr)
-    b3.run(40)
-    faces3 = 3 * 129 ** 3
+    b3.run(80)
+    faces3 = 3 * 257 ** 3
     room = a.samples - (len(fs) - n_kh)
     if room > 0:
         SG.dump_faces(b3, fs, max(1, faces3 // room + 1), 0)
@@ -679,7 +679,7 @@ def main():
                          "reference's implode 2D / blast 3D tests, scaled up")
     ap.add_argument("--hydro-scale", type=int, default=0,
                     help="hydro workloads: multiple of the reference test resolution (default 8)")
-    ap.add_argument("--samples", type=int, default=20_000_000)
+    ap.add_argument("--samples", type=int, default=50_000_000)
     ap.add_argument("--train-steps", type=int, default=300000)
     ap.add_argument("--gate", type=float, default=0.005)
     ap.add_argument("--members", type=int, default=64)
