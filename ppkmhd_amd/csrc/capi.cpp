@@ -347,7 +347,7 @@ int ppk_mhd_create(const ppk_mhd_params *params, ppk_mhd_t **handle) {
     cus = 0;
   h->dp.march = (params->dim == 3 && h->fused && params->precision != PPK_FP32_BULK &&
                  params->kernel_path != PPK_PATH_FUSED_RECORDS)
-                    ? ppk::march_planes(h->dp, cus)
+                    ? ppk::march_nblk(h->dp, cus)
                     : 0;
   if (params->kernel_path == PPK_PATH_FUSED && !h->fused) {
     delete h;

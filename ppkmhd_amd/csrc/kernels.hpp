@@ -153,24 +153,19 @@ inline Brick sub_bricks(const DevParams &p) {
   return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1, p.brick);
 }
 
-// The single-pass 3D step (march3d.hip): blocks of 14 x 14 output columns marching through
-// chunks of p.march planes; one CFL partial per block
+// The single-pass 3D step (march3d.hip): blocks of 14 x 14 output columns marching along z
+// through equal shares of the grid's tile-planes; one CFL partial per block
 constexpr int kMarchOut = 14;
-// Planes per z chunk: 64 (measured at 256^3: 6.72 ms per step against 6.78 with 32 and 6.74
-// with 16), halved while the blocks would not fill every CU twice (small grids), not below 16
-inline int march_planes(const DevParams &p, int cus) {
-  const long long tiles =
-      (long long)((p.nx + kMarchOut - 1) / kMarchOut) * ((p.ny + kMarchOut - 1) / kMarchOut);
-  int nzb = 64;
-  while (nzb > 16 && tiles * ((p.nz + nzb - 1) / nzb) < 2LL * (cus > 0 ? cus : 256)) nzb /= 2;
-  return nzb;
+// Blocks of the launch: one per CU (each block needs a whole CU's LDS), fewer when a block
+// would get under 16 tile-planes (small grids)
+inline int march_nblk(const DevParams &p, int cus) {
+  const long long total =
+      (long long)((p.nx + kMarchOut - 1) / kMarchOut) * ((p.ny + kMarchOut - 1) / kMarchOut) * p.nz;
+  long long b = cus > 0 ? cus : 256;
+  if (b > total / 16) b = total / 16;
+  return (int)(b < 1 ? 1 : b);
 }
-inline long long march_blocks(const DevParams &p) {
-  if (p.march <= 0) return 0;  // the single-pass kernel is not used
-  const long long tx = (p.nx + kMarchOut - 1) / kMarchOut, ty = (p.ny + kMarchOut - 1) / kMarchOut;
-  const long long tz = (p.nz + p.march - 1) / p.march;
-  return tx * ty * tz;
-}
+inline long long march_blocks(const DevParams &p) { return p.march > 0 ? p.march : 0; }
 
 // half-step tile shapes of a batch (batch_block_counts / batch_step `tile`): 2D 32x8, 64x4,
 // 16x16; 3D 16x4x4, 8x8x4, 32x4x2

@@ -25,8 +25,9 @@ struct DevExec {
   __device__ void cell_cfl(const T &, int, double) {}
 };
 
-// One block per tile of 16 x 16 record columns and chunk of planes; 4 waves, one block per
-// CU (the LDS of one block is 153 KB), one wave per SIMD with up to 512 registers.
+// One block per CU, marching its share of the grid's tile-planes (tiles of 16 x 16 record
+// columns); 4 waves, one block per CU (the LDS of one block is 153 KB), one wave per SIMD with
+// up to 512 registers.
 template <int RS>
 __global__ void __launch_bounds__(kMN, 1) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_march3d(const double *__restrict__ Uin, double *__restrict__ Uout, double *__restrict__ parts,

@@ -1,7 +1,8 @@
 // Single-pass 3D MHD step (k_march3d): per-thread bodies, host+device.
 //
-// A block owns a 16 x 16 tile of record columns (x, y) and marches along z through a chunk of
-// planes.  One thread per column.  At plane r the block
+// A block owns a 16 x 16 tile of record columns (x, y) and marches along z through a range of
+// planes (a block's share of the grid may span several tiles: MarchGeom).  One thread per
+// column.  At plane r the block
 //   H  computes the half-step record of every column at plane r (primitives, edge electric
 //      field, limited slopes, Hancock predictor; the same arithmetic as k_half3d) and puts it in
 //      LDS,
@@ -60,27 +61,32 @@ static_assert(X_N * kMN <= kSlotE, "exchange area");
 // U -> CFL: the new A and B of plane r-1, in the E part of the free slot
 static_assert(2 * kMN <= 3 * kEN, "CFL exchange area");
 
-// block -> tile: x fastest, then y, then z chunk
+// block -> work: the tile-planes of the grid in tile order (every output plane of tile 0,
+// then of tile 1, ...; tiles x fastest) are cut into nblk ranges of `per` tile-planes, and
+// block L marches the planes of range L, one segment per tile it touches.  With one block per
+// CU every CU gets the same number of planes (a launch of fixed-length z chunks ends with a
+// partial round of blocks), at the cost of a two-plane start per segment.
 struct MarchGeom {
-  int ntx, nty, nzc, nzb;  // tiles along x and y, z chunks, planes per chunk
-  long long blocks() const { return (long long)ntx * nty * nzc; }
+  int ntx, nty;   // tiles along x and y
+  int nblk;       // blocks
+  long long per;  // tile-planes per block (the last ranges may be shorter or empty)
+  long long blocks() const { return nblk; }
 };
-inline MarchGeom march_geom(const DevParams &p, int nzb) {
+inline MarchGeom march_geom(const DevParams &p, int nblk) {
   MarchGeom g;
   g.ntx = (p.nx + kMO - 1) / kMO;
   g.nty = (p.ny + kMO - 1) / kMO;
-  g.nzb = nzb < 1 ? 1 : nzb;
-  g.nzc = (p.nz + g.nzb - 1) / g.nzb;
+  const long long total = (long long)g.ntx * g.nty * p.nz;
+  long long b = nblk < 1 ? 1 : nblk;
+  if (b > total) b = total;
+  g.nblk = (int)b;
+  g.per = (total + b - 1) / b;
   return g;
 }
-// tile L: record columns x0 .. x0+15 (output x0+1 .. x0+14), output planes [z0, z1)
-PPK_HD void march_tile(const DevParams &p, const MarchGeom &g, int L, int &x0, int &y0, int &z0,
-                       int &z1) {
-  const int tx = L % g.ntx, r = L / g.ntx, ty = r % g.nty, tz = r / g.nty;
-  x0 = p.gw - 1 + kMO * tx;
-  y0 = p.gw - 1 + kMO * ty;
-  z0 = p.gw + g.nzb * tz;
-  z1 = min(z0 + g.nzb, p.nz + p.gw);
+// tile t: record columns x0 .. x0+15 (output x0+1 .. x0+14), y0 .. y0+15
+PPK_HD void march_tile(const DevParams &p, const MarchGeom &g, int t, int &x0, int &y0) {
+  x0 = p.gw - 1 + kMO * (t % g.ntx);
+  y0 = p.gw - 1 + kMO * (t / g.ntx);
 }
 
 PPK_HD double *slot(double *lds, int s) { return lds + s * kSlot; }
@@ -156,7 +162,7 @@ struct MarchThread {
   // prefetched loads of plane r+2: Q cells (8 vars, A(i+1), B(j+1), C(k+1)) and face cells
   double pq[2][11], pf[2][3];
   bool pqok[2], pfok[2];
-  double cfl;        // maximum CFL term of this thread's output cells
+  double cfl;        // maximum CFL term of this thread's output cells (over every segment)
 
   PPK_HD void init(const DevParams &p, int t, int x0, int y0) {
     tid = t;
@@ -169,7 +175,6 @@ struct MarchThread {
     ocol = a >= 1 && a <= kMO && b >= 1 && b <= kMO && in;
     acol = rok && a >= 1 && b >= 1 && b <= kMO && j < p.jsize - p.gw;
     bcol = rok && b >= 1 && a >= 1 && a <= kMO && i < p.isize - p.gw;
-    cfl = 0.0;
   }
 
   // global loads of plane pl for its Q and face cells (issued early, stashed later)
@@ -519,15 +524,15 @@ struct MarchThread {
   }
 };
 
-// The block's program, written once for the kernel and for the host emulation.  X.each(f)
-// runs f(thread) for every thread of the block and then synchronizes the block;
+// One segment of a block's work: output planes [z0, z1) of tile `tile`.  X.each(f) runs
+// f(thread) for every thread of the block and then synchronizes the block;
 // X.cell_cfl(thread, plane, term) sees each output cell's CFL term (a no-op in the kernel).
 template <int RS, class Exec>
-PPK_HD void march_block(Exec &X, const double *__restrict__ Uin, double *__restrict__ Uout,
-                        const DevParams &p, const MarchGeom &g, int L, double dt, bool fold,
-                        double *lds) {
-  int x0, y0, z0, z1;
-  march_tile(p, g, L, x0, y0, z0, z1);
+PPK_HD void march_segment(Exec &X, const double *__restrict__ Uin, double *__restrict__ Uout,
+                          const DevParams &p, const MarchGeom &g, int tile, int z0, int z1,
+                          double dt, bool fold, double *lds) {
+  int x0, y0;
+  march_tile(p, g, tile, x0, y0);
   using T = MarchThread<RS>;
   X.each([&](T &t) {
     t.init(p, X.tid(t), x0, y0);
@@ -562,6 +567,28 @@ PPK_HD void march_block(Exec &X, const double *__restrict__ Uin, double *__restr
       if (ahead) t.stash(p, lds, sa);
     });
     if (ahead) X.each([&](T &t) { t.elec(p, lds, sa, sb, x0, y0); });
+  }
+}
+
+// The block's program, written once for the kernel and for the host emulation: the segments
+// of range L (see MarchGeom), one after the other; each thread's CFL maximum spans them all.
+template <int RS, class Exec>
+PPK_HD void march_block(Exec &X, const double *__restrict__ Uin, double *__restrict__ Uout,
+                        const DevParams &p, const MarchGeom &g, int L, double dt, bool fold,
+                        double *lds) {
+  using T = MarchThread<RS>;
+  X.each([&](T &t) { t.cfl = 0.0; });
+  const long long total = (long long)g.ntx * g.nty * p.nz;
+  long long lo = (long long)L * g.per;
+  const long long hi = lo + g.per < total ? lo + g.per : total;
+  while (lo < hi) {
+    const int tile = (int)(lo / p.nz);
+    const long long tend = (long long)(tile + 1) * p.nz;
+    const long long se = hi < tend ? hi : tend;
+    const long long zb = (long long)tile * p.nz;
+    march_segment<RS>(X, Uin, Uout, p, g, tile, p.gw + (int)(lo - zb), p.gw + (int)(se - zb),
+                      dt, fold, lds);
+    lo = se;
   }
 }
 

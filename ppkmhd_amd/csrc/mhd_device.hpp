@@ -40,7 +40,7 @@ struct DevParams {
   int tile3;            // 3D half-step tile shape (batch tables; 0 for single grids)
   int brick;            // block-to-cells shape of the fused per-cell kernels (batch tables)
   int bulk32;           // FP32-bulk form: half step, records, face/edge solves in float
-  int march;            // 3D single-pass step: planes per z chunk of a block (0: not used)
+  int march;            // 3D single-pass step: blocks of its launch (0: not used)
 };
 
 template <class R>

@@ -9,7 +9,7 @@
 // Usage: march_emu <dir>
 //   reads  dir/params.bin  ppk_mhd_params (3D)
 //          dir/u0.bin      state with its ghosts filled (8 x cells doubles)
-//          dir/meta.txt    "dt planes_per_chunk"
+//          dir/meta.txt    "dt blocks" (the number of blocks the grid's tile-planes are cut into)
 //   writes dir/u1.bin      new state (ghosts: the input's, as the ghost-copy kernel writes them)
 //          dir/cfl_cells.bin  the CFL term of each new interior cell (NaN elsewhere)
 //          dir/cfl.txt     the maximum CFL term of the new interior cells (%.17g)
@@ -68,10 +68,10 @@ DevParams dev_params(const ppk_mhd_params &p) {
 }
 
 template <int RS>
-double run(const DevParams &p, int nzb, const double *Uin, double *Uout, double dt,
+double run(const DevParams &p, int nblk, const double *Uin, double *Uout, double dt,
            std::vector<double> &cells) {
   using T = MarchThread<RS>;
-  const MarchGeom g = march_geom(p, nzb);
+  const MarchGeom g = march_geom(p, nblk);
   std::vector<double> lds(kMarchLds);
   std::vector<T> ts(kMN);
   double cfl = 0.0;
@@ -117,10 +117,10 @@ int main(int argc, char **argv) {
     return 2;
   }
   double dt = 0.0;
-  int nzb = 0;
+  int nblk = 0;
   {
     FILE *f = std::fopen((dir + "/meta.txt").c_str(), "r");
-    if (!f || std::fscanf(f, "%lf %d", &dt, &nzb) != 2) {
+    if (!f || std::fscanf(f, "%lf %d", &dt, &nblk) != 2) {
       std::fprintf(stderr, "bad meta.txt\n");
       return 2;
     }
@@ -136,11 +136,11 @@ int main(int argc, char **argv) {
   std::vector<double> cells((size_t)p.vstride, std::nan(""));
   double cfl;
   if (p.riemann == PPK_RIEMANN_HLLD)
-    cfl = run<PPK_RIEMANN_HLLD>(p, nzb, U0.data(), U1.data(), dt, cells);
+    cfl = run<PPK_RIEMANN_HLLD>(p, nblk, U0.data(), U1.data(), dt, cells);
   else if (p.riemann == PPK_RIEMANN_HLL)
-    cfl = run<PPK_RIEMANN_HLL>(p, nzb, U0.data(), U1.data(), dt, cells);
+    cfl = run<PPK_RIEMANN_HLL>(p, nblk, U0.data(), U1.data(), dt, cells);
   else
-    cfl = run<PPK_RIEMANN_LLF>(p, nzb, U0.data(), U1.data(), dt, cells);
+    cfl = run<PPK_RIEMANN_LLF>(p, nblk, U0.data(), U1.data(), dt, cells);
   FILE *f = std::fopen((dir + "/u1.bin").c_str(), "wb");
   if (!f || std::fwrite(U1.data(), sizeof(double), n, f) != n) return 2;
   std::fclose(f);

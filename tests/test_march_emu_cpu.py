@@ -3,8 +3,9 @@ kernel bodies of march_body.hpp built for the host in the strict arithmetic, wit
 AddressSanitizer and UndefinedBehaviorSanitizer, every LDS slot and register poisoned with NaN
 before each block.  One step at a fixed dt must equal the oracle's step bit for bit (ghost
 cells included), and the folded CFL maximum must equal the oracle's CFL maximum of the new
-state.  Cases cover partial tiles and z chunks, the three Riemann solvers, both slope limiters
-and every boundary type."""
+state.  Cases cover partial tiles, block ranges that start and end inside a tile's planes
+and span several tiles, the three Riemann solvers, both slope limiters and every boundary
+type."""
 import os
 import subprocess
 
@@ -31,13 +32,13 @@ def emu(tmp_path_factory):
     return exe
 
 
-def emulate(exe, tmp, p, U0, dt, planes):
+def emulate(exe, tmp, p, U0, dt, blocks):
     d = str(tmp)
     with open(os.path.join(d, "params.bin"), "wb") as f:
         f.write(bytes(p))
     np.ascontiguousarray(U0).tofile(os.path.join(d, "u0.bin"))
     with open(os.path.join(d, "meta.txt"), "w") as f:
-        f.write(f"{dt!r} {planes}\n")
+        f.write(f"{dt!r} {blocks}\n")
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="print_stacktrace=1")
     r = subprocess.run([exe, d], capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stdout + r.stderr[-4000:]
@@ -82,25 +83,28 @@ def kh(nx, ny, nz, **kw):
     return p, pr
 
 
+# (problem, size, parameters, blocks): the grid's tile-planes (tiles of 14 x 14 output
+# columns, all planes of a tile in a row) are cut into `blocks` equal ranges, so a block's
+# range starts and ends inside a tile's z range and may span two or three tiles
 CASES = {
-    # periodic, tiles of 14 cut at 30 / 29, z chunks of 8 cut at 20
-    "blast_hlld_minmod": (blast, (30, 29, 20), dict(slope_type=1.0, riemann=A.RIEMANN_HLLD), 8),
-    # every boundary type, MC slopes, HLL, odd sizes, chunks of 5
+    # periodic, tiles cut at 30 / 29: 9 tiles x 20 planes in 7 ranges of 26
+    "blast_hlld_minmod": (blast, (30, 29, 20), dict(slope_type=1.0, riemann=A.RIEMANN_HLLD), 7),
+    # every boundary type, MC slopes, HLL, odd sizes: 6 tiles x 13 planes in 5 ranges of 16
     "kh_hll_mc_bcs": (kh, (17, 31, 13), dict(slope_type=2.0, riemann=A.RIEMANN_HLL,
                                              bc=(A.BC_DIRICHLET, A.BC_NEUMANN, A.BC_NEUMANN,
                                                  A.BC_DIRICHLET, A.BC_PERIODIC,
                                                  A.BC_PERIODIC)), 5),
-    # exact multiples of the output tile, one chunk, LLF, non-periodic z
+    # exact multiples of the output tile, LLF, non-periodic z: one block runs both tiles
     "kh_llf_whole": (kh, (28, 14, 9), dict(slope_type=1.0, riemann=A.RIEMANN_LLF,
                                            bc=(A.BC_PERIODIC, A.BC_PERIODIC, A.BC_PERIODIC,
                                                A.BC_PERIODIC, A.BC_DIRICHLET,
-                                               A.BC_NEUMANN)), 32),
+                                               A.BC_NEUMANN)), 1),
 }
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_march_block_program_matches_oracle(emu, oracle, tmp_path, case):
-    make, size, kw, planes = CASES[case]
+    make, size, kw, blocks = CASES[case]
     p, pr = make(*size, precision=A.FP64_STRICT, **kw)
     o = oracle.Solver(p)
     o.init_problem(pr)
@@ -110,7 +114,7 @@ def test_march_block_program_matches_oracle(emu, oracle, tmp_path, case):
     o.set_state(U0)
     o.step(dt, A.DT_FIXED)
     ref = o.get_state()
-    U1, cells, cfl = emulate(emu, tmp_path, p, U0, dt, planes)
+    U1, cells, cfl = emulate(emu, tmp_path, p, U0, dt, blocks)
     bad = ~((U1 == ref) | (np.isnan(U1) & np.isnan(ref)))
     assert not bad.any(), f"{bad.sum()} values differ, first at {np.argwhere(bad)[:5]}"
     # the CFL term of every new interior cell, and their maximum, against the oracle's
