@@ -257,7 +257,7 @@ int ppk_mhd_batch_init_problem(ppk_mhd_batch_t *h, const ppk_mhd_problem *proble
     ppk::init_problem_host(h->params[m], problems[m], host.data());
     BHIP(h, hipMemcpyAsync(bm.U[0], host.data(), host.size() * sizeof(double),
                            hipMemcpyHostToDevice, h->stream));
-    BHIP(h, h->K->boundaries(bm.p, bm.U[0], bm.ctrl, h->stream, nullptr));
+    BHIP(h, h->K->boundaries(bm.p, bm.U[0], nullptr, bm.ctrl, h->stream, nullptr));
     BHIP(h, h->K->fused_cfl(bm.p, bm.U[0], bm.parts, bm.ctrl, h->stream, nullptr));
     BHIP(h, h->K->set_dt(bm.ctrl, bm.parts, bm.nparts, 0.0, ppk::SETDT_REDUCE_ONLY,
                          h->params[m].cfl, h->stream));

@@ -239,7 +239,10 @@ int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
   const bool cfl = dt_mode == PPK_DT_CFL;
   const int mode = cfl ? ppk::SETDT_CFL : ppk::SETDT_FIXED;
   HIP_CHECK(h, K.begin(h->ctrl, check_finished, s));
-  HIP_CHECK(h, K.boundaries(p, in, h->ctrl, s, tm));
+  // the single-pass step writes only the interior of its output: the ghost fill gives the
+  // output the filled input's ghosts (the reference's U2 = U copy) in the same pass
+  const bool march = h->fused && p.march > 0 && !h->fw.sur;
+  HIP_CHECK(h, K.boundaries(p, in, march ? out : nullptr, h->ctrl, s, tm));
   if (h->fused) {
     // the previous fused step left the CFL maxima of its output (= in) in parts
     if (cfl && !h->cfl_valid) {
@@ -247,7 +250,7 @@ int launch_step(ppk_mhd_t *h, double dt, int dt_mode, int check_finished) {
       h->nparts = ppk::fused_cfl_parts(p);
     }
     HIP_CHECK(h, K.set_dt(h->ctrl, h->parts, h->nparts, dt, mode, h->p.cfl, s));
-    if (p.march > 0 && !h->fw.sur) {  // the single-pass 3D step
+    if (march) {  // the single-pass 3D step
       HIP_CHECK(h, K.march_godunov(p, in, out, h->parts, h->ctrl, cfl ? 1 : 0, s, tm));
       if (cfl) h->nparts = (int)ppk::march_blocks(p);
     } else {
@@ -517,7 +520,7 @@ int ppk_mhd_stream(ppk_mhd_t *h, void **stream) {
 int ppk_mhd_make_boundaries(ppk_mhd_t *h) {
   if (!h) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL handle");
   HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
-  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), nullptr, h->ctrl, h->stream, h->tm()));
   return PPK_OK;
 }
 
@@ -525,7 +528,7 @@ int ppk_mhd_compute_dt(ppk_mhd_t *h, double *dt) {
   if (!h || !dt) return fail(h, PPK_ERR_INVALID_ARGUMENT, "NULL argument");
   // the dt the next step would take: ghost fill, primitives, CFL max (no tEnd clamp)
   HIP_CHECK(h, h->K->begin(h->ctrl, 0, h->stream));
-  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), nullptr, h->ctrl, h->stream, h->tm()));
   if (h->fused) {
     HIP_CHECK(h, h->K->fused_cfl(h->dp, h->cur(), h->parts, h->ctrl, h->stream, h->tm()));
     h->cfl_valid = true;
@@ -559,7 +562,7 @@ int ppk_mhd_dump_faces(ppk_mhd_t *h, ppk_faceset_t *set, int stride, int phase,
   if (!alloc_records(h)) return fail(h, PPK_ERR_OUT_OF_MEMORY, "device allocation failed");
   // the faces the next CFL step will solve: ghost fill, CFL dt, half-step records
   HIP_CHECK(h, h->K->begin(h->ctrl, 1, h->stream));
-  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), h->ctrl, h->stream, h->tm()));
+  HIP_CHECK(h, h->K->boundaries(h->dp, h->cur(), nullptr, h->ctrl, h->stream, h->tm()));
   HIP_CHECK(h, h->K->fused_cfl(h->dp, h->cur(), h->parts, h->ctrl, h->stream, h->tm()));
   HIP_CHECK(h, h->K->set_dt(h->ctrl, h->parts, ppk::fused_cfl_parts(h->dp), 0.0,
                             ppk::SETDT_CFL, h->p.cfl, h->stream));

@@ -194,8 +194,9 @@ struct BatchMember {
 
 #define PPK_DECLARE_KERNELS(NS)                                                                  \
   namespace NS {                                                                                 \
-  hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s, \
-                               StageTimer *tm);                                                  \
+  /* ghost fill of U; copy (may be null) receives the same ghost values */                     \
+  hipError_t staged_boundaries(const DevParams &p, double *U, double *copy, const double *ctrl,  \
+                               hipStream_t s, StageTimer *tm);                                   \
   hipError_t staged_prim_dt(const DevParams &p, const double *U, double *Q, double *parts,       \
                             const double *ctrl, int do_dt, hipStream_t s, StageTimer *tm);       \
   hipError_t staged_begin(double *ctrl, int check_finished, hipStream_t s);                      \
@@ -219,7 +220,8 @@ struct BatchMember {
   hipError_t surrogate_dump(const DevParams &p, const void *rec, const double *ctrl, float *X,   \
                             float *T, long long stride, long long phase, hipStream_t s);         \
   long long surrogate_dump_slots(const DevParams &p, long long stride, long long phase);         \
-  /* single-pass 3D step (p.march > 0): CFL partials, one per block, when fold_cfl */           \
+  /* single-pass 3D step (p.march > 0): the interior of Uout (its ghosts come from the ghost   \
+     fill's copy target); CFL partials, one per block, when fold_cfl */                         \
   hipError_t march_godunov(const DevParams &p, const double *Uin, double *Uout, double *parts,   \
                            const double *ctrl, int fold_cfl, hipStream_t s, StageTimer *tm);    \
   /* batched ensemble: counts of blocks per stage for one member, then the batched step */      \
@@ -256,7 +258,10 @@ extern const KernelTable kStrict;
 // ghost fill of one element of direction dir (both faces), thread index t over
 // 2 * gw * s1 * s2 elements; reference src/shared/BoundariesFunctors.h:534-1057 (x, then y,
 // then z; Dirichlet flips the normal velocity and normal field)
-PPK_HD void bnd_element(double *__restrict__ U, const DevParams &p, int dir, long long t) {
+// copy (optional): a second state that receives the same ghost value (the single-pass step's
+// output, whose ghosts are the reference's U2 = U copy of the filled input)
+PPK_HD void bnd_element(double *__restrict__ U, const DevParams &p, int dir, long long t,
+                        double *__restrict__ copy = nullptr) {
   const int gw = p.gw;
   const int n = dir == 0 ? p.nx : (dir == 1 ? p.ny : p.nz);
   const int s1 = dir == 0 ? p.jsize : p.isize;
@@ -286,7 +291,9 @@ PPK_HD void bnd_element(double *__restrict__ U, const DevParams &p, int dir, lon
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     double sign = (bc == PPK_BC_DIRICHLET && (v == IU + dir || v == IA + dir)) ? -1.0 : 1.0;
-    U[dst + v * p.vstride] = U[src + v * p.vstride] * sign;
+    const double x = U[src + v * p.vstride] * sign;
+    U[dst + v * p.vstride] = x;
+    if (copy) copy[dst + v * p.vstride] = x;
   }
 }
 PPK_HD long long bnd_elements(const DevParams &p, int dir) {

@@ -1,7 +1,8 @@
-// Single-pass 3D MHD step: one kernel reads the state once and writes the new state once
-// (bodies and data flow: march_body.hpp).  The ghost cells of the new state are the
-// reference's deep copy of the filled input ghosts (U2 = U before the update), copied by a
-// small kernel over the same elements the ghost fill writes.
+// Single-pass 3D MHD step: one kernel reads the state once and writes the interior of the new
+// state once (bodies and data flow: march_body.hpp).  The ghost cells of the new state are the
+// reference's deep copy of the filled input ghosts (U2 = U before the update): the step's ghost
+// fill (staged_boundaries with the new state as its copy target) writes each ghost value to
+// both states.
 #include <hip/hip_runtime.h>
 
 #include "march_body.hpp"
@@ -45,36 +46,6 @@ k_march3d(const double *__restrict__ Uin, double *__restrict__ Uout, double *__r
   }
 }
 
-// ghost cells of Uout = those of Uin, over the elements the ghost fill of direction dir writes
-// (bnd_element's enumeration)
-__global__ void __launch_bounds__(256) k_ghost_copy(const double *__restrict__ Uin,
-                                                   double *__restrict__ Uout, DevParams p,
-                                                   int dir, const double *ctrl) {
-  if (ctrl[CTRL_ACTIVE] == 0.0) return;
-  const long long n = bnd_elements(p, dir);
-  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n;
-       t += (long long)gridDim.x * 256) {
-    const int gw = p.gw;
-    const int nn = dir == 0 ? p.nx : (dir == 1 ? p.ny : p.nz);
-    const int s1 = dir == 0 ? p.jsize : p.isize;
-    const int s2 = dir == 2 ? p.jsize : p.ksize;
-    const int g = (int)(t % gw);
-    long long r = t / gw;
-    const int t1 = (int)(r % s1);
-    r /= s1;
-    const int t2 = (int)(r % s2);
-    const int upper = (int)(r / s2);
-    const int c = upper ? g + nn + gw : g;
-    int i, j, k;
-    if (dir == 0) { i = c; j = t1; k = t2; }
-    else if (dir == 1) { i = t1; j = c; k = t2; }
-    else { i = t1; j = t2; k = c; }
-    const long long d = cidx(p, i, j, k);
-#pragma unroll
-    for (int v = 0; v < 8; ++v) Uout[d + v * p.vstride] = Uin[d + v * p.vstride];
-  }
-}
-
 }  // namespace marchk
 
 hipError_t march_godunov(const DevParams &p, const double *Uin, double *Uout, double *parts,
@@ -98,13 +69,6 @@ hipError_t march_godunov(const DevParams &p, const double *Uin, double *Uout, do
       break;
   }
   if (tm) tm->mark(ST_FUSED_MARCH, false);
-  for (int dir = 0; dir < 3; ++dir) {
-    const long long n = bnd_elements(p, dir);
-    long long nb = (n + 255) / 256;
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(k_ghost_copy, dim3((unsigned)nb), dim3(256), 0, s, Uin, Uout, p, dir,
-                       ctrl);
-  }
   return hipGetLastError();
 }
 

@@ -91,7 +91,7 @@ extern "C" int ppk_mhd_compute_dt_noalloc(const ppk_mhd_params *params, double *
   const ppk::KernelTable &K = params->precision == PPK_FP64_STRICT ? ppk::kStrict : ppk::kFast;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = K.begin(ctrl, 0, st);
-  if (e == hipSuccess) e = K.boundaries(dp, dU, ctrl, st, nullptr);
+  if (e == hipSuccess) e = K.boundaries(dp, dU, nullptr, ctrl, st, nullptr);
   if (e == hipSuccess) e = K.fused_cfl(dp, dU, parts, ctrl, st, nullptr);
   if (e == hipSuccess)
     e = K.set_dt(ctrl, parts, (int)nparts, 0.0, ppk::SETDT_REDUCE_ONLY, params->cfl, st);

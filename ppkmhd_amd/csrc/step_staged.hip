@@ -44,13 +44,14 @@ __device__ __forceinline__ void decompose(const DevParams &p, long long t, int &
 }
 
 // ---- ghost fill, one direction (both faces): bnd_element (kernels.hpp) per thread ---------
-__global__ void __launch_bounds__(kBlock) k_boundaries(double *__restrict__ U, DevParams p,
+__global__ void __launch_bounds__(kBlock) k_boundaries(double *__restrict__ U,
+                                                       double *__restrict__ copy, DevParams p,
                                                        int dir, const double *ctrl) {
   if (!active(ctrl)) return;
   const long long total = bnd_elements(p, dir);
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x)
-    bnd_element(U, p, dir, t);
+    bnd_element(U, p, dir, t, copy);
 }
 
 // ---- primitives over [0, size-1) per direction + CFL max over the interior ----------------
@@ -463,12 +464,13 @@ int grid_for(long long n) {
 // -------------------------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------------------------
-hipError_t staged_boundaries(const DevParams &p, double *U, const double *ctrl, hipStream_t s,
-                             StageTimer *tm) {
+hipError_t staged_boundaries(const DevParams &p, double *U, double *copy, const double *ctrl,
+                             hipStream_t s, StageTimer *tm) {
   if (tm) tm->mark(ST_BOUNDARIES, true);
   for (int dir = 0; dir < p.dim; ++dir) {
     const long long n = bnd_elements(p, dir);
-    hipLaunchKernelGGL(k_boundaries, dim3(grid_for(n)), dim3(kBlock), 0, s, U, p, dir, ctrl);
+    hipLaunchKernelGGL(k_boundaries, dim3(grid_for(n)), dim3(kBlock), 0, s, U, copy, p, dir,
+                       ctrl);
   }
   if (tm) tm->mark(ST_BOUNDARIES, false);
   return hipGetLastError();
