@@ -302,6 +302,74 @@ PPK_HD long long bnd_elements(const DevParams &p, int dir) {
   return 2LL * p.gw * s1 * s2;
 }
 
+// The ghost fill in one pass.  The per-direction passes above fill x, then y, then z ghosts
+// over the whole transverse range, so a corner or edge ghost ends with the value its last pass
+// reads from a ghost the earlier passes filled: the interior cell reached by mapping each of
+// its ghost coordinates, times the product of the directions' Dirichlet signs (exact, each
+// sign is +-1).  ghost_shell_element computes that composition directly: the same values,
+// one write per ghost cell.
+// source coordinate of c in [0, n + 2 gw) along one direction; flip: a Dirichlet face mapped it
+PPK_HD int ghost_src(const DevParams &p, int d, int c, bool &flip) {
+  const int gw = p.gw, n = d == 0 ? p.nx : (d == 1 ? p.ny : p.nz);
+  flip = false;
+  if (c >= gw && c < n + gw) return c;
+  const bool upper = c >= n + gw;
+  const int bc = p.bc[2 * d + (upper ? 1 : 0)];
+  if (bc == PPK_BC_DIRICHLET) {
+    flip = true;
+    return upper ? 2 * n + 2 * gw - 1 - c : 2 * gw - 1 - c;
+  }
+  if (bc == PPK_BC_NEUMANN) return upper ? n + gw - 1 : gw;
+  return upper ? c - n : n + c;
+}
+// ghost cells of the padded box: the 2 gw z planes (3D), then per interior plane the 2 gw y
+// rows, then per interior row the 2 gw x cells
+PPK_HD long long ghost_shell_cells(const DevParams &p) {
+  const long long gw2 = 2LL * p.gw, kin = p.dim == 3 ? p.nz : 1;
+  return (p.dim == 3 ? gw2 * p.jsize * p.isize : 0) + kin * gw2 * p.isize + kin * p.ny * gw2;
+}
+PPK_HD void ghost_shell_element(double *__restrict__ U, const DevParams &p, long long t,
+                                double *__restrict__ copy) {
+  const int gw = p.gw;
+  const long long gw2 = 2LL * gw;
+  const long long nA = p.dim == 3 ? gw2 * p.jsize * p.isize : 0;
+  const long long nB = (p.dim == 3 ? p.nz : 1) * gw2 * p.isize;
+  const int koff = p.dim == 3 ? gw : 0;
+  int i, j, k;
+  if (t < nA) {
+    const int kk = (int)(t / ((long long)p.jsize * p.isize));
+    k = kk < gw ? kk : p.nz + kk;
+    j = (int)((t / p.isize) % p.jsize);
+    i = (int)(t % p.isize);
+  } else if (t < nA + nB) {
+    const long long u = t - nA;
+    k = (int)(u / (gw2 * p.isize)) + koff;
+    const int jj = (int)((u / p.isize) % gw2);
+    j = jj < gw ? jj : p.ny + jj;
+    i = (int)(u % p.isize);
+  } else {
+    const long long u = t - nA - nB;
+    k = (int)(u / ((long long)p.ny * gw2)) + koff;
+    const long long r = u % ((long long)p.ny * gw2);
+    j = gw + (int)(r / gw2);
+    const int ii = (int)(r % gw2);
+    i = ii < gw ? ii : p.nx + ii;
+  }
+  bool fx, fy, fz = false;
+  const int i0 = ghost_src(p, 0, i, fx), j0 = ghost_src(p, 1, j, fy);
+  const int k0 = p.dim == 3 ? ghost_src(p, 2, k, fz) : k;
+  const long long dst = ((long long)k * p.jsize + j) * p.isize + i;
+  const long long src = ((long long)k0 * p.jsize + j0) * p.isize + i0;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const bool neg = ((v == IU || v == IA) && fx) != (((v == IV || v == IB) && fy) !=
+                                                      ((v == IW || v == IC) && fz));
+    const double x = U[src + v * p.vstride] * (neg ? -1.0 : 1.0);
+    U[dst + v * p.vstride] = x;
+    if (copy) copy[dst + v * p.vstride] = x;
+  }
+}
+
 // block-wide maximum of one double per thread (any block size that is a multiple of 64, up
 // to 1024); every thread gets the result
 __device__ __forceinline__ double block_max(double v, double *smem32) {

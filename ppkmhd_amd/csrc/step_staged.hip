@@ -43,15 +43,17 @@ __device__ __forceinline__ void decompose(const DevParams &p, long long t, int &
   k = (int)(r / p.jsize);
 }
 
-// ---- ghost fill, one direction (both faces): bnd_element (kernels.hpp) per thread ---------
+// ---- ghost fill of every face, edge and corner in one pass: ghost_shell_element
+// (kernels.hpp) per ghost cell, the values of the reference's x, y, z passes
+// (src/shared/SolverBase.cpp:339-537, src/shared/BoundariesFunctors.h:534-1057) ------------
 __global__ void __launch_bounds__(kBlock) k_boundaries(double *__restrict__ U,
                                                        double *__restrict__ copy, DevParams p,
-                                                       int dir, const double *ctrl) {
+                                                       const double *ctrl) {
   if (!active(ctrl)) return;
-  const long long total = bnd_elements(p, dir);
+  const long long total = ghost_shell_cells(p);
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x)
-    bnd_element(U, p, dir, t, copy);
+    ghost_shell_element(U, p, t, copy);
 }
 
 // ---- primitives over [0, size-1) per direction + CFL max over the interior ----------------
@@ -467,11 +469,8 @@ int grid_for(long long n) {
 hipError_t staged_boundaries(const DevParams &p, double *U, double *copy, const double *ctrl,
                              hipStream_t s, StageTimer *tm) {
   if (tm) tm->mark(ST_BOUNDARIES, true);
-  for (int dir = 0; dir < p.dim; ++dir) {
-    const long long n = bnd_elements(p, dir);
-    hipLaunchKernelGGL(k_boundaries, dim3(grid_for(n)), dim3(kBlock), 0, s, U, copy, p, dir,
-                       ctrl);
-  }
+  hipLaunchKernelGGL(k_boundaries, dim3(grid_for(ghost_shell_cells(p))), dim3(kBlock), 0, s, U,
+                     copy, p, ctrl);
   if (tm) tm->mark(ST_BOUNDARIES, false);
   return hipGetLastError();
 }
