@@ -61,15 +61,21 @@ static_assert(X_N * kMN <= kSlotE, "exchange area");
 // U -> CFL: the new A and B of plane r-1, in the E part of the free slot
 static_assert(2 * kMN <= 3 * kEN, "CFL exchange area");
 
-// block -> work: the tile-planes of the grid in tile order (every output plane of tile 0,
-// then of tile 1, ...; tiles x fastest) are cut into nblk ranges of `per` tile-planes, and
-// block L marches the planes of range L, one segment per tile it touches.  With one block per
-// CU every CU gets the same number of planes (a launch of fixed-length z chunks ends with a
+// block -> work, with one block per CU (tiles x fastest, then y):
+//  - rounds of whole tiles: in round k block L marches every plane of tile k * nblk + L, so
+//    the blocks of one XCD (consecutive L) march neighbouring tiles side by side and share
+//    their halo columns in that XCD's L2;
+//  - the tiles left over (fewer than nblk) are shared out: their tile-planes in tile order
+//    (every plane of a tile, then the next tile) are cut into nblk ranges of `per`, and block L
+//    marches range L, one segment per tile it touches.
+// Every CU gets the same number of planes (a launch of fixed-length z chunks ends with a
 // partial round of blocks), at the cost of a two-plane start per segment.
 struct MarchGeom {
-  int ntx, nty;   // tiles along x and y
-  int nblk;       // blocks
-  long long per;  // tile-planes per block (the last ranges may be shorter or empty)
+  int ntx, nty;  // tiles along x and y
+  int nblk;      // blocks
+  int full;      // rounds of whole tiles
+  int per;       // tile-planes per block of the left-over tiles (the last ranges may be
+                 // shorter or empty)
   long long blocks() const { return nblk; }
 };
 inline MarchGeom march_geom(const DevParams &p, int nblk) {
@@ -80,7 +86,9 @@ inline MarchGeom march_geom(const DevParams &p, int nblk) {
   long long b = nblk < 1 ? 1 : nblk;
   if (b > total) b = total;
   g.nblk = (int)b;
-  g.per = (total + b - 1) / b;
+  g.full = g.ntx * g.nty / g.nblk;
+  const long long rest = total - (long long)g.full * g.nblk * p.nz;
+  g.per = (int)((rest + b - 1) / b);
   return g;
 }
 // tile t: record columns x0 .. x0+15 (output x0+1 .. x0+14), y0 .. y0+15
@@ -578,17 +586,27 @@ PPK_HD void march_block(Exec &X, const double *__restrict__ Uin, double *__restr
                         double *lds) {
   using T = MarchThread<RS>;
   X.each([&](T &t) { t.cfl = 0.0; });
-  const long long total = (long long)g.ntx * g.nty * p.nz;
-  long long lo = (long long)L * g.per;
-  const long long hi = lo + g.per < total ? lo + g.per : total;
-  while (lo < hi) {
-    const int tile = (int)(lo / p.nz);
-    const long long tend = (long long)(tile + 1) * p.nz;
-    const long long se = hi < tend ? hi : tend;
-    const long long zb = (long long)tile * p.nz;
-    march_segment<RS>(X, Uin, Uout, p, g, tile, p.gw + (int)(lo - zb), p.gw + (int)(se - zb),
-                      dt, fold, lds);
-    lo = se;
+  // 32-bit bookkeeping (a grid of 2^31 tile-planes would not fit a GPU's memory); one call
+  // site of the segment program, so that it is inlined once
+  const int t0 = g.full * g.nblk, rest = (g.ntx * g.nty - t0) * p.nz;
+  int lo = L * g.per;
+  const int hi = lo + g.per < rest ? lo + g.per : rest;
+  for (int k = 0;; ++k) {
+    int tile, z0, z1;
+    if (k < g.full) {  // whole tile
+      tile = k * g.nblk + L;
+      z0 = p.gw;
+      z1 = p.gw + p.nz;
+    } else {  // the next piece of this block's range of the left-over tiles
+      if (lo >= hi) break;
+      const int tr = lo / p.nz, zb = tr * p.nz;
+      const int se = hi < zb + p.nz ? hi : zb + p.nz;
+      tile = t0 + tr;
+      z0 = p.gw + lo - zb;
+      z1 = p.gw + se - zb;
+      lo = se;
+    }
+    march_segment<RS>(X, Uin, Uout, p, g, tile, z0, z1, dt, fold, lds);
   }
 }
 

@@ -83,13 +83,19 @@ def kh(nx, ny, nz, **kw):
     return p, pr
 
 
-# (problem, size, parameters, blocks): the grid's tile-planes (tiles of 14 x 14 output
-# columns, all planes of a tile in a row) are cut into `blocks` equal ranges, so a block's
-# range starts and ends inside a tile's z range and may span two or three tiles
+# (problem, size, parameters, blocks): tiles of 14 x 14 output columns; each block first
+# marches whole tiles (one per block per round), then the left-over tiles' tile-planes are
+# cut into `blocks` equal ranges, which start and end inside a tile's z range and may span
+# two tiles
 CASES = {
-    # periodic, tiles cut at 30 / 29: 9 tiles x 20 planes in 7 ranges of 26
+    # periodic, tiles cut at 30 / 29: 9 tiles x 20 planes; 7 whole tiles, then 2 tiles in 7
+    # ranges of 6 planes
     "blast_hlld_minmod": (blast, (30, 29, 20), dict(slope_type=1.0, riemann=A.RIEMANN_HLLD), 7),
-    # every boundary type, MC slopes, HLL, odd sizes: 6 tiles x 13 planes in 5 ranges of 16
+    # more blocks than tiles: every tile shared out in ranges of 17 planes
+    "blast_hlld_ranges_only": (blast, (30, 29, 20), dict(slope_type=1.0,
+                                                         riemann=A.RIEMANN_HLLD), 11),
+    # every boundary type, MC slopes, HLL, odd sizes: 6 tiles x 13 planes; 5 whole tiles,
+    # then one tile in 5 ranges of 3 planes
     "kh_hll_mc_bcs": (kh, (17, 31, 13), dict(slope_type=2.0, riemann=A.RIEMANN_HLL,
                                              bc=(A.BC_DIRICHLET, A.BC_NEUMANN, A.BC_NEUMANN,
                                                  A.BC_DIRICHLET, A.BC_PERIODIC,
