@@ -502,14 +502,15 @@ PROBE_STEPS = 3
 
 
 def surrogate_roofline(dim, faces_per_dir, frac_net, probe):
-    """Roofline of the network's face kernel (k_face_sur, one launch per direction): the
-    network's bf16 FLOPs for the faces it served (faces of a direction x the fraction by the
-    network) over the kernel's HIP-event time per launch, against the dense bf16 peak."""
+    """Roofline of the network's face kernel (k_face_sur, one launch for every direction):
+    the network's bf16 FLOPs for the faces it served (faces of a direction x dim x the
+    fraction by the network) over the kernel's HIP-event time per launch, against the dense
+    bf16 peak."""
     ms = probe.get("surrogate_faces")
     if not ms:
         return None
-    per_launch = ms / dim
-    flops = MLP_FLOP_PER_FACE * faces_per_dir * frac_net
+    per_launch = ms
+    flops = MLP_FLOP_PER_FACE * faces_per_dir * dim * frac_net
     achieved = flops / (per_launch * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_face_sur",

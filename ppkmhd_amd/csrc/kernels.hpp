@@ -70,7 +70,9 @@ constexpr int kRec2 = 26;   // half-step record per cell, 2D
 constexpr int kFluxComp3 = 5;  // ID IP IU IV IW per face direction, 3D
 constexpr int kFluxComp2 = 6;  // ID IP IU IV IW IC per face direction, 2D
 // the surrogate form of the face solves: network, gate, counters (device memory)
-constexpr int kSurrogateBlocks = 512;  // blocks of a surrogate face launch (at most)
+// blocks of a surrogate face launch (at most): three per CU of an MI355X, as many as its LDS
+// holds (measured: 2D step 0.344 -> 0.324 ms, 3D 57.8 -> 56.6 ms against two per CU)
+constexpr int kSurrogateBlocks = 768;
 struct SurrogateCfg {
   const void *net = nullptr;            // packed network (MlpPacked, device)
   double gate_rho = 0.0, gate_p = 0.0;  // max relative jumps |a_R - a_L| / min(a_L, a_R)

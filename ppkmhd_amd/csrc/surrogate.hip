@@ -1,11 +1,12 @@
 // Surrogate form of the face solves, and the face-sample dump its network is trained on.
 //
-//   k_face_sur<DIM, D>  the face fluxes of direction D with the learned flux where the gate and
-//                       the admissibility check allow (per face).  A block of 256 threads
+//   k_face_sur<DIM>     the face fluxes of every direction with the learned flux where the gate
+//                       and the admissibility check allow (per face).  A block of 256 threads
 //                       stages the network (40 KB of bf16 weights) in LDS once and then walks
-//                       bricks of the face sub-box; each wavefront runs the network on the
-//                       matrix cores for its 64 faces when any of them passed the gate, and
-//                       marks the faces that did not or whose flux was refused
+//                       bricks of the face sub-box, direction after direction; each wavefront
+//                       runs the network on the matrix cores for its 64 faces when any of them
+//                       passed the gate, and marks the faces that did not or whose flux was
+//                       refused
 //   k_face_fallback     exact HLLD for the marked faces
 //   k_dump<DIM>         features and training targets of every stride-th face (all directions)
 //
@@ -48,44 +49,18 @@ PPK_HD void store(R *flux, const DevParams &p, long long c, const St<R> &f) {
     store_flux2<D>(flux, p, c, f);
 }
 
+// the faces of direction D of the bricks this block walks (see k_face_sur); w: the network in
+// LDS, sX: this wave's LDS slice; counts of faces by the network / in total accumulate
 template <int DIM, int D, class R>
-__global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
-                                                     R *__restrict__ flux, DevParams p, Brick bk,
-                                                     const double *ctrl, SurrogateCfg cfg) {
-  if (!active(ctrl)) return;
-  __shared__ __attribute__((aligned(16))) uint16_t sW[kPackA1 + kPackA2 + kPackA3];
-  __shared__ float sFl[kFloatBlock];
-  // per wave: the bf16 features of its 64 faces, then (same bytes, 32 per face) the outputs:
-  // a pass of the network reads the features of its 16-face blocks before it writes their
-  // outputs, and later passes read only later blocks
-  __shared__ __attribute__((aligned(16))) uint16_t sXall[4][64 * 16];
-  const MlpPacked *net = static_cast<const MlpPacked *>(cfg.net);
-  {  // stage the inference weights: A1, A2, A3 are the first kMlpInferBytes of the net
-    const uint4 *src = reinterpret_cast<const uint4 *>(net);
-    uint4 *dst = reinterpret_cast<uint4 *>(sW);
-    for (int n = threadIdx.x; n < kMlpInferBytes / 16; n += kBlock) dst[n] = src[n];
-    const float *fsrc = net->b1;  // b1 .. sig_out are contiguous
-    for (int n = threadIdx.x; n < kFloatBlock; n += kBlock) sFl[n] = fsrc[n];
-  }
-  __syncthreads();
-  MlpView w;
-  w.A1 = sW;
-  w.A2 = sW + kPackA1;
-  w.A3 = sW + kPackA1 + kPackA2;
-  w.A3T = w.A2T = nullptr;
-  w.b1 = sFl;
-  w.b2 = sFl + 128;
-  w.b3 = sFl + 256;
-  w.mu_in = sFl + 272;
-  w.isig_in = sFl + 288;
-  w.mu_out = sFl + 304;
-  w.sig_out = sFl + 312;
-  const int wave = threadIdx.x >> 6, lane = lane_id();
-  uint16_t *sX = sXall[wave];
-  float *sY = reinterpret_cast<float *>(sXall[wave]);
+__device__ __forceinline__ void faces_dir(const R *__restrict__ rec, R *__restrict__ flux,
+                                          const DevParams &p, const Brick &bk,
+                                          const SurrogateCfg &cfg, const MlpView &w,
+                                          uint16_t *sX, unsigned long long &n_net,
+                                          unsigned long long &n_all) {
+  const int lane = lane_id();
+  float *sY = reinterpret_cast<float *>(sX);
   const float gamma0 = float(p.gamma0);
   const long long nb = bk.nblocks;
-  unsigned long long n_net = 0, n_all = 0;  // faces of this wave by the network, in total
 #pragma unroll 1
   for (long long L = blockIdx.x; L < nb; L += gridDim.x) {
     const long long c = sub_brick_cell(p, bk, (int)L, threadIdx.x);
@@ -141,6 +116,48 @@ __global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
     n_net += __popcll(__ballot(c >= 0 && use));
     n_all += __popcll(__ballot(c >= 0));
   }
+}
+
+// The learned face fluxes of every direction in one launch: a block stages the network in
+// LDS once and walks its bricks for direction x, then y (then z); see the file comment.
+template <int DIM, class R>
+__global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
+                                                     R *__restrict__ flux, DevParams p, Brick bk,
+                                                     const double *ctrl, SurrogateCfg cfg) {
+  if (!active(ctrl)) return;
+  __shared__ __attribute__((aligned(16))) uint16_t sW[kPackA1 + kPackA2 + kPackA3];
+  __shared__ float sFl[kFloatBlock];
+  // per wave: the bf16 features of its 64 faces, then (same bytes, 32 per face) the outputs:
+  // a pass of the network reads the features of its 16-face blocks before it writes their
+  // outputs, and later passes read only later blocks
+  __shared__ __attribute__((aligned(16))) uint16_t sXall[4][64 * 16];
+  const MlpPacked *net = static_cast<const MlpPacked *>(cfg.net);
+  {  // stage the inference weights: A1, A2, A3 are the first kMlpInferBytes of the net
+    const uint4 *src = reinterpret_cast<const uint4 *>(net);
+    uint4 *dst = reinterpret_cast<uint4 *>(sW);
+    for (int n = threadIdx.x; n < kMlpInferBytes / 16; n += kBlock) dst[n] = src[n];
+    const float *fsrc = net->b1;  // b1 .. sig_out are contiguous
+    for (int n = threadIdx.x; n < kFloatBlock; n += kBlock) sFl[n] = fsrc[n];
+  }
+  __syncthreads();
+  MlpView w;
+  w.A1 = sW;
+  w.A2 = sW + kPackA1;
+  w.A3 = sW + kPackA1 + kPackA2;
+  w.A3T = w.A2T = nullptr;
+  w.b1 = sFl;
+  w.b2 = sFl + 128;
+  w.b3 = sFl + 256;
+  w.mu_in = sFl + 272;
+  w.isig_in = sFl + 288;
+  w.mu_out = sFl + 304;
+  w.sig_out = sFl + 312;
+  const int wave = threadIdx.x >> 6, lane = lane_id();
+  uint16_t *sX = sXall[wave];
+  unsigned long long n_net = 0, n_all = 0;  // faces of this wave by the network, in total
+  faces_dir<DIM, 0, R>(rec, flux, p, bk, cfg, w, sX, n_net, n_all);
+  faces_dir<DIM, 1, R>(rec, flux, p, bk, cfg, w, sX, n_net, n_all);
+  if constexpr (DIM == 3) faces_dir<DIM, 2, R>(rec, flux, p, bk, cfg, w, sX, n_net, n_all);
   // per-block counts (no same-address atomics): this block's slots, in stream order
   if (cfg.counters) {
     __shared__ unsigned long long cnt[2][kBlock / 64];
@@ -229,14 +246,10 @@ hipError_t faces_impl(const DevParams &p, const R *rec, R *flux, const double *c
   const dim3 gf((unsigned)nb);
   const unsigned char *fl = cfg.flags;
   if (tm) tm->mark(ST_SUR_FACES, true);
-  if (p.dim == 3) {
-    hipLaunchKernelGGL((k_face_sur<3, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
-    hipLaunchKernelGGL((k_face_sur<3, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
-    hipLaunchKernelGGL((k_face_sur<3, 2, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
-  } else {
-    hipLaunchKernelGGL((k_face_sur<2, 0, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
-    hipLaunchKernelGGL((k_face_sur<2, 1, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
-  }
+  if (p.dim == 3)
+    hipLaunchKernelGGL((k_face_sur<3, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
+  else
+    hipLaunchKernelGGL((k_face_sur<2, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
   if (tm) tm->mark(ST_SUR_FACES, false);
   if (tm) tm->mark(ST_SUR_FALLBACK, true);
   if (p.dim == 3) {
