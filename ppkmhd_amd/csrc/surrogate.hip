@@ -7,7 +7,7 @@
 //                       runs the network on the matrix cores for its 64 faces when any of them
 //                       passed the gate, and marks the faces that did not or whose flux was
 //                       refused
-//   k_face_fallback     exact HLLD for the marked faces
+//   k_face_fallback     exact HLLD for the marked faces (every direction in one launch)
 //   k_dump<DIM>         features and training targets of every stride-th face (all directions)
 //
 // The face states are those of the plain fused step (face_states2/3), so a face that falls
@@ -180,17 +180,30 @@ __global__ void __launch_bounds__(kBlock) k_face_sur(const R *__restrict__ rec,
 
 // exact HLLD for the faces of direction D that the surrogate pass marked
 template <int DIM, int D, class R>
-__global__ void __launch_bounds__(kBlock) k_face_fallback(const R *__restrict__ rec,
-                                                          R *__restrict__ flux, DevParams p,
-                                                          Brick bk, const double *ctrl,
-                                                          const unsigned char *flags) {
-  if (!active(ctrl)) return;
-  const long long c = sub_brick_cell(p, bk, (int)blockIdx.x, threadIdx.x);
+__device__ __forceinline__ void fallback_dir(const R *__restrict__ rec, R *__restrict__ flux,
+                                             const DevParams &p, const Brick &bk, int L,
+                                             const unsigned char *flags) {
+  const long long c = sub_brick_cell(p, bk, L, threadIdx.x);
   if (c < 0 || !flags[(long long)D * p.vstride + c]) return;
   St<R> ql, qr, F;
   states<DIM, D>(rec, p, c, ql, qr);
   riemann_hlld(ql, qr, F, p);
   store<DIM, D>(flux, p, c, F);
+}
+// every direction in one launch: block B handles brick B / DIM of direction B % DIM
+template <int DIM, class R>
+__global__ void __launch_bounds__(kBlock) k_face_fallback(const R *__restrict__ rec,
+                                                          R *__restrict__ flux, DevParams p,
+                                                          Brick bk, const double *ctrl,
+                                                          const unsigned char *flags) {
+  if (!active(ctrl)) return;
+  const int L = (int)(blockIdx.x / DIM), d = (int)(blockIdx.x % DIM);
+  if (d == 0)
+    fallback_dir<DIM, 0, R>(rec, flux, p, bk, L, flags);
+  else if (d == 1)
+    fallback_dir<DIM, 1, R>(rec, flux, p, bk, L, flags);
+  else if constexpr (DIM == 3)
+    fallback_dir<DIM, 2, R>(rec, flux, p, bk, L, flags);
 }
 
 // one thread per selected face: slot s is face t = s * stride + phase of the enumeration
@@ -243,7 +256,7 @@ hipError_t faces_impl(const DevParams &p, const R *rec, R *flux, const double *c
   const Brick bk = sub_bricks(p);
   const long long nb = bk.count();
   const dim3 g((unsigned)(nb < kSurrogateBlocks ? nb : kSurrogateBlocks)), b(kBlock);
-  const dim3 gf((unsigned)nb);
+  const dim3 gf((unsigned)(p.dim * nb));
   const unsigned char *fl = cfg.flags;
   if (tm) tm->mark(ST_SUR_FACES, true);
   if (p.dim == 3)
@@ -252,14 +265,10 @@ hipError_t faces_impl(const DevParams &p, const R *rec, R *flux, const double *c
     hipLaunchKernelGGL((k_face_sur<2, R>), g, b, 0, s, rec, flux, p, bk, ctrl, cfg);
   if (tm) tm->mark(ST_SUR_FACES, false);
   if (tm) tm->mark(ST_SUR_FALLBACK, true);
-  if (p.dim == 3) {
-    hipLaunchKernelGGL((k_face_fallback<3, 0, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
-    hipLaunchKernelGGL((k_face_fallback<3, 1, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
-    hipLaunchKernelGGL((k_face_fallback<3, 2, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
-  } else {
-    hipLaunchKernelGGL((k_face_fallback<2, 0, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
-    hipLaunchKernelGGL((k_face_fallback<2, 1, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
-  }
+  if (p.dim == 3)
+    hipLaunchKernelGGL((k_face_fallback<3, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
+  else
+    hipLaunchKernelGGL((k_face_fallback<2, R>), gf, b, 0, s, rec, flux, p, bk, ctrl, fl);
   if (tm) tm->mark(ST_SUR_FALLBACK, false);
   return hipGetLastError();
 }
