@@ -86,10 +86,10 @@ def test_strict_is_bit_identical(oracle, gpu, case, path):
 
 @pytest.mark.parametrize("path", [A.PATH_FUSED, A.PATH_FUSED_RECORDS])
 def test_strict_at_size_many_tiles(oracle, gpu, path):
-    """A 70 x 60 x 50 blast (MC slopes, an oblique field) with more tiles than blocks resident
-    on the GPU, partial tiles and chunks in every direction (single-pass kernel: 5 x 5 tiles of
-    14 x 14 columns, chunks of 32 planes; three-kernel form: 1280 half-step tiles, partial
-    bricks): strict bit-identical to the oracle over 3 CFL steps."""
+    """A 70 x 60 x 50 blast (MC slopes, an oblique field) with partial tiles in every
+    direction (single-pass kernel: 5 x 5 tiles of 14 x 14 columns shared out to 78 blocks in
+    ranges of 17 planes, some spanning two tiles; three-kernel form: 1280 half-step tiles,
+    partial bricks): strict bit-identical to the oracle over 3 CFL steps."""
     p = make_params(3, 70, 60, 50, gamma0=5 / 3, cfl=0.8, tEnd=1.0, slope_type=2.0,
                     precision=A.FP64_STRICT, kernel_path=path)
     pr = default_problem(A.PROBLEM_BLAST, p)
@@ -194,6 +194,26 @@ def test_fast_1000_steps_l1(oracle, gpu, path):
     e = rel_l1(g.download(), o.get_state(), p)
     assert e <= 1e-10, e
     assert abs(g.m_t - o.time()[0]) <= 1e-12 * o.time()[0]
+
+
+@pytest.mark.parametrize("path", [A.PATH_AUTO, A.PATH_FUSED_RECORDS])
+def test_fast_1000_steps_l1_3d(oracle, gpu, path):
+    """3D blast 24^3, 1000 CFL steps (the single-pass kernel and the three-kernel form):
+    relative L1 <= 1e-10 against the oracle, div B at roundoff."""
+    p, pr = blast3d(24, precision=A.FP64, kernel_path=path, n_step_max=1000)
+    p.tEnd = 10.0
+    o = oracle.Solver(p)
+    o.init_problem(pr)
+    g = solver(p, pr)
+    for _ in range(1000):
+        o.step(0, A.DT_CFL)
+    g.run(1000)
+    assert g.m_iteration == 1000
+    e = rel_l1(g.download(), o.get_state(), p)
+    print(f"blast 24^3, path {path}, 1000 CFL steps: relative L1 {e:.3e}", flush=True)
+    assert e <= 1e-10, e
+    assert abs(g.m_t - o.time()[0]) <= 1e-12 * o.time()[0]
+    assert g.divb_max() < 1e-12 / p.dx
 
 
 @pytest.mark.parametrize("name", ["ot2d_16", "blast3d_8"])
