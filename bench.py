@@ -32,9 +32,12 @@ PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 matrix, TFLOP/s
 MLP_FLOP_PER_FACE = 2 * (16 * 128 + 128 * 128 + 128 * 8)  # 16 -> 128 -> 128 -> 8
 
 # Algorithmic FP64 operations (add + mul + div + sqrt, counted by the instrumented oracle) per
-# work item of each stage, 3D; the item ranges are those of the reference's functors.
+# work item of each stage, 3D; the item ranges are those of the reference's functors
+# (tests/test_oracle_step.py checks them against the counting oracle).  primitives_dt: the
+# primitive of every cell of [0, size-1)^3 (28) and the CFL term of every interior cell (42).
+PRIM_FLOP, CFL_FLOP = 28.0, 42.0
 FLOP_PER_ITEM_3D = {
-    "primitives_dt": 57.5, "elec_field": 45.0, "mag_slopes": 48.0, "trace": 639.0,
+    "elec_field": 45.0, "mag_slopes": 48.0, "trace": 639.0,
     "fluxes": 855.0, "emf": 1575.0, "update_ct": 78.0,
 }
 # Algorithmic HBM bytes per work item of each staged kernel (inputs read once + outputs
@@ -46,11 +49,13 @@ BYTES_PER_ITEM_3D = {
 }
 # Fused kernels, per work item (3D): the Riemann kernel solves 3 HLLD problems (fluxes stage,
 # 855) and 3 2D-HLLD EMFs (emf stage, 1575) per cell of [gw, size-gw]^3; the adds that rebuild
-# face and edge states from the records are not counted.  The half-step kernel: E field (45)
-# and face-field slopes (48) per cell plus the trace without the 210 adds of state assembly
-# (639 - 210).  The update kernel: update + CT (78) and the CFL term of the new state (57.5).
-FLOP_PER_ITEM_3D.update({"fused_riemann": 855.0 + 1575.0, "fused_half": 45.0 + 48.0 + 429.0,
-                         "fused_update": 78.0 + 57.5})
+# face and edge states from the records are not counted.  The half-step kernel: the primitive
+# (28), E field (45) and face-field slopes (48) per cell plus the trace without the 210 adds of
+# state assembly (639 - 210).  The update kernel: update + CT (78) and the primitive and CFL
+# term of the new state (28 + 42).
+FLOP_PER_ITEM_3D.update({"fused_riemann": 855.0 + 1575.0,
+                         "fused_half": PRIM_FLOP + 45.0 + 48.0 + 429.0,
+                         "fused_update": 78.0 + PRIM_FLOP + CFL_FLOP})
 # the step: each cell's 8 FP64 conservative variables read once and written once
 STEP_BYTES_PER_CELL = 128.0
 # HBM bytes per work item measured with rocprofv3 FETCH_SIZE (x2) + WRITE_SIZE (x1) at 256^3,
@@ -73,14 +78,19 @@ def items(stage, n):
             "fused_update": n ** 3, "fused_march": n ** 3}.get(stage)
 
 
+def primitives_dt_flops(n):
+    return PRIM_FLOP * (n + 5) ** 3 + CFL_FLOP * n ** 3
+
+
 def step_flops_per_cell(n):
     """Algorithmic FP64 operations of one step per interior cell-update (staged item ranges)."""
-    return sum(FLOP_PER_ITEM_3D[k] * items(k, n) for k in
-               ("primitives_dt", "elec_field", "mag_slopes", "trace", "fluxes", "emf",
-                "update_ct")) / n ** 3
+    return (primitives_dt_flops(n) + sum(FLOP_PER_ITEM_3D[k] * items(k, n) for k in
+            ("elec_field", "mag_slopes", "trace", "fluxes", "emf", "update_ct"))) / n ** 3
 
 
 def stage_flops(stage, n):
+    if stage == "primitives_dt":
+        return primitives_dt_flops(n)
     if stage == "fused_march":
         return sum(FLOP_PER_ITEM_3D[k] * items(k, n) for k in MARCH_STAGES)
     if stage in FLOP_PER_ITEM_3D:

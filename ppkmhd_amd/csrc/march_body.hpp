@@ -578,6 +578,30 @@ PPK_HD void march_segment(Exec &X, const double *__restrict__ Uin, double *__res
   }
 }
 
+// Segment k of block L (MarchGeom): output planes [z0, z1) of tile `tile`; false past the
+// last.  32-bit arithmetic (a grid of 2^31 tile-planes would not fit a GPU's memory).
+PPK_HD bool march_piece(const DevParams &p, const MarchGeom &g, int L, int k, int &tile, int &z0,
+                        int &z1) {
+  if (k < g.full) {  // whole tile
+    tile = k * g.nblk + L;
+    z0 = p.gw;
+    z1 = p.gw + p.nz;
+    return true;
+  }
+  // piece k - full of this block's range of the left-over tiles' tile-planes
+  const int t0 = g.full * g.nblk, rest = (g.ntx * g.nty - t0) * p.nz;
+  const int lo0 = L * g.per, hi = lo0 + g.per < rest ? lo0 + g.per : rest;
+  const int kk = k - g.full;
+  const int lo = kk == 0 ? lo0 : (lo0 / p.nz + kk) * p.nz;
+  if (lo >= hi) return false;
+  const int tr = lo / p.nz, zb = tr * p.nz;
+  const int se = hi < zb + p.nz ? hi : zb + p.nz;
+  tile = t0 + tr;
+  z0 = p.gw + lo - zb;
+  z1 = p.gw + se - zb;
+  return true;
+}
+
 // The block's program, written once for the kernel and for the host emulation: the segments
 // of range L (see MarchGeom), one after the other; each thread's CFL maximum spans them all.
 template <int RS, class Exec>
@@ -586,28 +610,11 @@ PPK_HD void march_block(Exec &X, const double *__restrict__ Uin, double *__restr
                         double *lds) {
   using T = MarchThread<RS>;
   X.each([&](T &t) { t.cfl = 0.0; });
-  // 32-bit bookkeeping (a grid of 2^31 tile-planes would not fit a GPU's memory); one call
-  // site of the segment program, so that it is inlined once
-  const int t0 = g.full * g.nblk, rest = (g.ntx * g.nty - t0) * p.nz;
-  int lo = L * g.per;
-  const int hi = lo + g.per < rest ? lo + g.per : rest;
-  for (int k = 0;; ++k) {
-    int tile, z0, z1;
-    if (k < g.full) {  // whole tile
-      tile = k * g.nblk + L;
-      z0 = p.gw;
-      z1 = p.gw + p.nz;
-    } else {  // the next piece of this block's range of the left-over tiles
-      if (lo >= hi) break;
-      const int tr = lo / p.nz, zb = tr * p.nz;
-      const int se = hi < zb + p.nz ? hi : zb + p.nz;
-      tile = t0 + tr;
-      z0 = p.gw + lo - zb;
-      z1 = p.gw + se - zb;
-      lo = se;
-    }
+  // one call site of the segment program, so that it is inlined once; segment k is computed
+  // from (L, k) alone, so that nothing but k is carried across a segment
+  int tile, z0, z1;
+  for (int k = 0; march_piece(p, g, L, k, tile, z0, z1); ++k)
     march_segment<RS>(X, Uin, Uout, p, g, tile, z0, z1, dt, fold, lds);
-  }
 }
 
 }  // namespace marchk

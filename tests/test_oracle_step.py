@@ -290,3 +290,31 @@ def test_fixed_dt_is_clamped_to_tend(oracle):
     # tEnd holds the float-rounded 0.01 of an .ini real (ConfigMap.cpp:37-46)
     assert it == 4 and dt == pytest.approx(p.tEnd - 0.009, abs=1e-15) and t == p.tEnd
     assert o.finished()
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_counted_flops_match_bench_constants(oracle, n):
+    """The instrumented oracle's operation counts (add + mul + div + sqrt) of one CFL step of
+    the 3D blast, per stage, are the per-item figures bench.py prices the rooflines with, over
+    the reference functors' item ranges: primitives of [0, size-1)^3 (28 each) and the CFL
+    term of the interior (42 each), E and face-field slopes over (n+4)^3, the trace over
+    (n+3)^3, fluxes and EMFs over (n+1)^3, the update over n^3.  Two grid sizes separate the
+    per-cell figures of stages whose ranges differ."""
+    import bench
+
+    p = make_params(3, n, n, n, gamma0=5 / 3, cfl=0.8, tEnd=1.0)
+    pr = default_problem(A.PROBLEM_BLAST, p)
+    pr.blast_radius = 0.1
+    pr.blast_density_out = 1.0
+    pr.blast_pressure_in, pr.blast_pressure_out = 10.0, 0.1
+    pr.blast_bx = pr.blast_by = pr.blast_bz = 1.0 / np.sqrt(3.0)
+    s = oracle.Solver(p, counting=True)
+    s.init_problem(pr)
+    s.step(0, A.DT_CFL)
+    oracle.op_counts_reset()
+    s.step(0, A.DT_CFL)
+    for stage in ("primitives_dt", "elec_field", "mag_slopes", "trace", "fluxes", "emf",
+                  "update_ct"):
+        c = oracle.op_counts(stage)
+        counted = c["add"] + c["mul"] + c["div"] + c["sqrt"]
+        assert counted == pytest.approx(bench.stage_flops(stage, n), rel=1e-12), stage
