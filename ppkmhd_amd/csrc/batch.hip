@@ -73,14 +73,13 @@ __global__ void k_bend(const BatchMember *__restrict__ mem, int nm) {
   ctrl[CTRL_T] += ctrl[CTRL_DT];
 }
 
-// ---- ghost fill of the current state, direction dir ---------------------------------------
-__global__ void __launch_bounds__(kBlock) k_bbnd(const BatchMember *__restrict__ mem, int nm,
-                                                 int dir) {
+// ---- ghost fill of the current state, every face, edge and corner in one pass -------------
+__global__ void __launch_bounds__(kBlock) k_bbnd(const BatchMember *__restrict__ mem, int nm) {
   const long long gb = blockIdx.x;
-  const BatchMember &m = mem[find_member(mem, nm, BS_BND0 + dir, gb)];
+  const BatchMember &m = mem[find_member(mem, nm, BS_BND, gb)];
   if (!member_active(m)) return;
-  const long long t = (gb - m.begin[BS_BND0 + dir]) * kBlock + threadIdx.x;
-  if (t < bnd_elements(m.p, dir)) bnd_element(m.U[parity(m)], m.p, dir, t);
+  const long long t = (gb - m.begin[BS_BND]) * kBlock + threadIdx.x;
+  if (t < ghost_shell_cells(m.p)) ghost_shell_element(m.U[parity(m)], m.p, t, nullptr);
 }
 
 // ---- standalone CFL partials of the current state ------------------------------------------
@@ -222,8 +221,7 @@ long long half_tiles3(const DevParams &p) {
 // blocks per stage of one member (the host builds the first-block tables from these)
 void batch_block_counts(const DevParams &p, int tile, long long counts[BS_COUNT]) {
   using namespace fusedk;
-  for (int d = 0; d < 3; ++d)
-    counts[BS_BND0 + d] = d < p.dim ? (bnd_elements(p, d) + batchk::kBlock - 1) / batchk::kBlock : 0;
+  counts[BS_BND] = (ghost_shell_cells(p) + batchk::kBlock - 1) / batchk::kBlock;
   counts[BS_CFL] = fused_cfl_parts(p);
   if (p.dim == 3)
     counts[BS_HALF] = tile == 1 ? half_tiles3<Tile3d>(p)
@@ -287,9 +285,8 @@ hipError_t batch_step(const BatchMember *members, int nm, int dim, int bulk32, i
   const dim3 b(kBlock), bm((nm + 63) / 64 * 64 > 1024 ? 1024 : (nm + 63) / 64 * 64);
   const dim3 gm((nm + (int)bm.x - 1) / (int)bm.x);
   hipLaunchKernelGGL(k_bbegin, gm, bm, 0, s, members, nm);
-  for (int d = 0; d < dim; ++d)
-    if (totals[BS_BND0 + d] > 0)
-      hipLaunchKernelGGL(k_bbnd, dim3((unsigned)totals[BS_BND0 + d]), b, 0, s, members, nm, d);
+  if (totals[BS_BND] > 0)
+    hipLaunchKernelGGL(k_bbnd, dim3((unsigned)totals[BS_BND]), b, 0, s, members, nm);
   if (with_cfl)
     hipLaunchKernelGGL(k_bcfl, dim3((unsigned)totals[BS_CFL]), dim3(kCflBlock), 0, s, members,
                        nm);

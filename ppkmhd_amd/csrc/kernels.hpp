@@ -177,7 +177,7 @@ constexpr int kBatchTiles = 3;
 // launches.  Each batched launch covers the blocks of every member back to back; a member
 // owns blocks [begin[stage], begin[stage] + its count) of that stage.
 enum BatchStage {
-  BS_BND0 = 0, BS_BND1, BS_BND2,  // ghost fill, one launch per direction
+  BS_BND = 0,                      // ghost fill of every face, edge and corner (one pass)
   BS_CFL,                          // standalone CFL of the current state (first step)
   BS_HALF, BS_RIEMANN, BS_UPDATE,  // the three fused stages
   BS_COUNT
@@ -257,59 +257,15 @@ struct KernelTable {
 extern const KernelTable kFast;
 extern const KernelTable kStrict;
 
-// ghost fill of one element of direction dir (both faces), thread index t over
-// 2 * gw * s1 * s2 elements; reference src/shared/BoundariesFunctors.h:534-1057 (x, then y,
-// then z; Dirichlet flips the normal velocity and normal field)
-// copy (optional): a second state that receives the same ghost value (the single-pass step's
-// output, whose ghosts are the reference's U2 = U copy of the filled input)
-PPK_HD void bnd_element(double *__restrict__ U, const DevParams &p, int dir, long long t,
-                        double *__restrict__ copy = nullptr) {
-  const int gw = p.gw;
-  const int n = dir == 0 ? p.nx : (dir == 1 ? p.ny : p.nz);
-  const int s1 = dir == 0 ? p.jsize : p.isize;
-  const int s2 = dir == 2 ? p.jsize : p.ksize;
-  int g = (int)(t % gw);
-  long long r = t / gw;
-  int t1 = (int)(r % s1);
-  r /= s1;
-  int t2 = (int)(r % s2);
-  int upper = (int)(r / s2);
-  int face = 2 * dir + upper;
-  int bc = p.bc[face];
-  int c = upper ? g + n + gw : g;
-  int c0;
-  if (bc == PPK_BC_DIRICHLET)
-    c0 = upper ? 2 * n + 2 * gw - 1 - c : 2 * gw - 1 - c;
-  else if (bc == PPK_BC_NEUMANN)
-    c0 = upper ? n + gw - 1 : gw;
-  else
-    c0 = upper ? c - n : n + c;
-  int i, j, k, i0, j0, k0;
-  if (dir == 0) { i = c; j = t1; k = t2; i0 = c0; j0 = j; k0 = k; }
-  else if (dir == 1) { i = t1; j = c; k = t2; i0 = i; j0 = c0; k0 = k; }
-  else { i = t1; j = t2; k = c; i0 = i; j0 = j; k0 = c0; }
-  const long long dst = ((long long)k * p.jsize + j) * p.isize + i;
-  const long long src = ((long long)k0 * p.jsize + j0) * p.isize + i0;
-#pragma unroll
-  for (int v = 0; v < 8; ++v) {
-    double sign = (bc == PPK_BC_DIRICHLET && (v == IU + dir || v == IA + dir)) ? -1.0 : 1.0;
-    const double x = U[src + v * p.vstride] * sign;
-    U[dst + v * p.vstride] = x;
-    if (copy) copy[dst + v * p.vstride] = x;
-  }
-}
-PPK_HD long long bnd_elements(const DevParams &p, int dir) {
-  const int s1 = dir == 0 ? p.jsize : p.isize;
-  const int s2 = dir == 2 ? p.jsize : p.ksize;
-  return 2LL * p.gw * s1 * s2;
-}
-
-// The ghost fill in one pass.  The per-direction passes above fill x, then y, then z ghosts
-// over the whole transverse range, so a corner or edge ghost ends with the value its last pass
-// reads from a ghost the earlier passes filled: the interior cell reached by mapping each of
-// its ghost coordinates, times the product of the directions' Dirichlet signs (exact, each
-// sign is +-1).  ghost_shell_element computes that composition directly: the same values,
-// one write per ghost cell.
+// The ghost fill in one pass.  The reference fills x, then y, then z ghosts over the whole
+// transverse range (src/shared/BoundariesFunctors.h:534-1057, SolverBase.cpp:339-537;
+// Dirichlet flips the normal velocity and normal field), so a corner or edge ghost ends with
+// the value its last pass reads from a ghost the earlier passes filled: the interior cell
+// reached by mapping each of its ghost coordinates, times the product of the directions'
+// Dirichlet signs (exact, each sign is +-1).  ghost_shell_element computes that composition
+// directly: the same values, one write per ghost cell (copy, optional: a second state that
+// receives the same ghost values, the single-pass step's output whose ghosts are the
+// reference's U2 = U copy of the filled input).
 // source coordinate of c in [0, n + 2 gw) along one direction; flip: a Dirichlet face mapped it
 PPK_HD int ghost_src(const DevParams &p, int d, int c, bool &flip) {
   const int gw = p.gw, n = d == 0 ? p.nx : (d == 1 ? p.ny : p.nz);
