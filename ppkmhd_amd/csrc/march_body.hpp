@@ -348,9 +348,16 @@ struct MarchThread {
 
   // S(r): the solves of the column.  mid: plane r is an output plane; cross: the z face and
   // the x / y edges between planes r-1 and r are needed; carry: the plane-r states of the
-  // solves between planes r and r+1 are needed.
-  PPK_HD void solves(const DevParams &p, double *lds, int sa, bool mid, bool cross, bool carry) {
-    if (!rok) return;
+  // solves between planes r and r+1 are needed.  pre() (the global loads of a later plane,
+  // which every thread issues) runs between the face and the edge solves, so that its
+  // registers are not held through the face solves.
+  template <class Pre>
+  PPK_HD void solves(const DevParams &p, double *lds, int sa, bool mid, bool cross, bool carry,
+                     Pre &&pre) {
+    if (!rok) {
+      pre();
+      return;
+    }
     const double *sR = lds + kRecOff;
     const bool inA = a >= 1, inB = b >= 1;
     if (mid && inA && inB && b <= kMO) {  // x face of c (its lower face)
@@ -378,6 +385,7 @@ struct MarchThread {
       riemann_mhd_rs<RS>(ql, qr, f, p);
       fz[0] = f[ID]; fz[1] = f[IP]; fz[2] = f[IU]; fz[3] = f[IV]; fz[4] = f[IW];
     }
+    pre();
     if (mid && inA && inB) {  // z edge: RT of c-x-y, RB of c-x, LT of c-y, LB of c
       Half3D<double> ra, rb, rd, t;
       lds_rec(sR, tid - 1 - kMT, ra);
@@ -564,11 +572,14 @@ PPK_HD void march_segment(Exec &X, const double *__restrict__ Uin, double *__res
     const bool mid = r >= z0 && r < z1, cross = r >= z0, carry = r < z1;
     const bool done = r > z0, top = r == z1;
     X.each([&](T &t) {
-      if (ahead) t.prefetch(Uin, p, x0, y0, r + 2);
       if (r >= z0) t.load_state(Uin, p, r);
       t.half(p, lds, sa, sb, dt);
     });
-    X.each([&](T &t) { t.solves(p, lds, sa, mid, cross, carry); });
+    X.each([&](T &t) {
+      t.solves(p, lds, sa, mid, cross, carry, [&] {
+        if (ahead) t.prefetch(Uin, p, x0, y0, r + 2);
+      });
+    });
     X.each([&](T &t) { t.update(p, lds, sa, dt, done, mid, top, Uout, r, fold); });
     X.each([&](T &t) {
       if (fold && done && t.ocol) X.cell_cfl(t, r - 1, t.cfl_term(p, lds, sa, r));

@@ -130,13 +130,19 @@ PPK_HD R dmin(R a, R b) { return fmin(a, b); }
 // the field is along the normal; rounded (and, in the default builds, FMA-contracted or float)
 // it can come out a few ulp below 0, and its square root NaN.  The default builds clamp it at
 // 0; the strict build keeps the reference's expression as written.
+// (fmax: one instruction, and 0 for a NaN input as the compare-and-select form)
 template <class R>
-PPK_HD R nonneg(R x) { return kSharedRecip ? (x > R(0) ? x : R(0)) : x; }
+PPK_HD R nonneg(R x) { return kSharedRecip ? dmax(x, R(0)) : x; }
 
 // compare-and-select maxima used by the 2D magnetic Riemann solver (FMAX4/FMIN4/FMAX5,
-// reference src/shared/mhd_utils.h:36-77)
+// reference src/shared/mhd_utils.h:36-77).  The default builds take fmax / fmin, one
+// instruction each instead of a compare and two selects; for operands that are not NaN the
+// value is the same: the maxima of fast and Alfven speeds have operands >= +0 only, and the
+// extrema of velocities are offset by a positive fast speed (S = min u - max c_f, max u +
+// max c_f), so the sign of a zero extremum does not reach the result.
 template <class R>
 PPK_HD R max4(R a0, R a1, R a2, R a3) {
+  if (kSharedRecip) return dmax(dmax(dmax(a0, a1), a2), a3);
   R r = a0;
   r = (a1 > r) ? a1 : r;
   r = (a2 > r) ? a2 : r;
@@ -145,6 +151,7 @@ PPK_HD R max4(R a0, R a1, R a2, R a3) {
 }
 template <class R>
 PPK_HD R min4(R a0, R a1, R a2, R a3) {
+  if (kSharedRecip) return dmin(dmin(dmin(a0, a1), a2), a3);
   R r = a0;
   r = (a1 < r) ? a1 : r;
   r = (a2 < r) ? a2 : r;
@@ -154,6 +161,7 @@ PPK_HD R min4(R a0, R a1, R a2, R a3) {
 template <class R>
 PPK_HD R max5(R a0, R a1, R a2, R a3, R a4) {
   R r = max4(a0, a1, a2, a3);
+  if (kSharedRecip) return dmax(r, a4);
   r = (a4 > r) ? a4 : r;
   return r;
 }
@@ -226,16 +234,19 @@ PPK_HD void constoprim(const St<R> &u, R magx, R magy, R magz, St<R> &q,
 }
 
 // minmod (slope_type 1) / MC (slope_type 2) limited slope,
-// reference src/muscl/MHDBaseFunctor2D.h:308-327
+// reference src/muscl/MHDBaseFunctor2D.h:308-327.  The default builds give the magnitude
+// the sign of dcen with copysign (one instruction for the compare, select and multiply by
+// +-1); the value is the same, only a zero slope at dcen = -0 keeps the sign of dcen.
 template <class R>
 PPK_HD R limited_slope(R st, R q, R qplus, R qminus) {
   R dlft = st * (q - qminus);
   R drgt = st * (qplus - q);
   R dcen = R(0.5) * (qplus - qminus);
-  R dsgn = (dcen >= R(0)) ? R(1) : R(-1);
   R slop = dmin(dabs(dlft), dabs(drgt));
   R dlim = slop;
   if ((dlft * drgt) <= R(0)) dlim = R(0);
+  if (kSharedRecip) return copysign(dmin(dlim, dabs(dcen)), dcen);
+  R dsgn = (dcen >= R(0)) ? R(1) : R(-1);
   return dsgn * dmin(dlim, dabs(dcen));
 }
 
