@@ -308,6 +308,12 @@ const char *ppk_mhd_timer_name(int stage);
  * Synchronizes. */
 int ppk_mhd_divb_max(ppk_mhd_t *h, double *divb_max);
 int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]);
+/* Device arithmetic self-check (no counterpart in the reference): for n host values x[i],
+ * out[3i], out[3i+1], out[3i+2] receive 1/x, sqrt(x) and 1/sqrt(x) as the default FP64 build
+ * computes them on the current device (v_rcp_f64 / v_rsq_f64 refined in FP64 arithmetic), so
+ * that a caller can compare them with the IEEE operations.  Allocates device memory and
+ * synchronizes. */
+int ppk_mhd_math_selfcheck(const double *x, int64_t n, double *out);
 
 /* ---- output ---------------------------------------------------------------------------- */
 /* Write a state (host array, reference layout) as VTK ImageData <dir>/<prefix>_<istep>.vti

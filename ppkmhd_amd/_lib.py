@@ -57,6 +57,7 @@ SIGNATURES = {
     "ppk_mhd_timer_name": (C.c_char_p, [C.c_int]),
     "ppk_mhd_divb_max": (C.c_int, [_H, _DP]),
     "ppk_mhd_conserved_sums": (C.c_int, [_H, _DP]),
+    "ppk_mhd_math_selfcheck": (C.c_int, [_DP, C.c_int64, _DP]),
     "ppk_mhd_write_vtk": (C.c_int, [C.POINTER(A.Params), _DP, C.c_char_p, C.c_char_p, C.c_int,
                                     C.c_int]),
     "ppk_mhd_save_vtk": (C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_int, C.c_int]),

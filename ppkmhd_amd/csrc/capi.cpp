@@ -48,6 +48,7 @@ const char *thread_error() { return g_thread_error.c_str(); }
 
 // diagnostics (diag.hip)
 hipError_t diag_divb(const DevParams &p, const double *U, double *out, hipStream_t s);
+hipError_t diag_math(const double *x, long long n, double *out, hipStream_t s);
 hipError_t diag_sums(const DevParams &p, const double *U, double *partials, int nblocks,
                      hipStream_t s);
 constexpr int kDiagBlocks = 512;
@@ -766,6 +767,24 @@ int ppk_mhd_conserved_sums(ppk_mhd_t *h, double sums[8]) {
     for (int b = 0; b < ppk::kDiagBlocks; ++b) s += part[(size_t)b * 8 + v];
     sums[v] = s;
   }
+  return PPK_OK;
+}
+
+int ppk_mhd_math_selfcheck(const double *x, int64_t n, double *out) {
+  if (n < 0 || (n > 0 && (!x || !out)))
+    return fail(nullptr, PPK_ERR_INVALID_ARGUMENT, "NULL array or negative count");
+  if (n == 0) return PPK_OK;
+  const size_t bytes = (size_t)n * sizeof(double);
+  double *dx = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&dx, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dout, 3 * bytes);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = ppk::diag_math(dx, n, dout, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 3 * bytes, hipMemcpyDeviceToHost);
+  if (dx) (void)hipFree(dx);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess)
+    return fail(nullptr, PPK_ERR_HIP, std::string("math self-check: ") + hipGetErrorString(e));
   return PPK_OK;
 }
 

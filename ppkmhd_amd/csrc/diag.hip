@@ -61,6 +61,18 @@ __global__ void __launch_bounds__(kDiagThreads) k_sums(const double *__restrict_
   }
 }
 
+// the default FP64 build's refined reciprocal, square root and inverse square root
+__global__ void __launch_bounds__(kDiagThreads) k_math(const double *__restrict__ x, long long n,
+                                                       double *__restrict__ out) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const double v = x[t];
+    out[3 * t] = fast_rcp(v);
+    out[3 * t + 1] = fast_sqrt(v);
+    out[3 * t + 2] = fast_rsq(v);
+  }
+}
+
 __global__ void k_zero1(double *x) {
   if (threadIdx.x == 0) x[0] = 0.0;
 }
@@ -70,6 +82,11 @@ __global__ void k_zero1(double *x) {
 hipError_t diag_divb(const DevParams &p, const double *U, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, out);
   hipLaunchKernelGGL(k_divb, dim3(512), dim3(kDiagThreads), 0, s, U, p, out);
+  return hipGetLastError();
+}
+
+hipError_t diag_math(const double *x, long long n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_math, dim3(1024), dim3(kDiagThreads), 0, s, x, n, out);
   return hipGetLastError();
 }
 

@@ -62,11 +62,12 @@ PPK_HD bool sign_bit(R x) { return __builtin_signbit(x); }
 // Division and square root.  Built with PPK_EXACT_DIV (the strict variant) every a / b and
 // sqrt(x) is the IEEE operation as written.  Otherwise (the default FP64 variant):
 //  - a denominator used several times is inverted once and each quotient is a * (1/b);
-//  - on the GPU, 1/b is v_rcp_f64 refined by two Newton steps, sqrt(x) is v_rsq_f64 refined
-//    and corrected with its residual, and a / sqrt(r) is a * (refined rsq).  Measured on
-//    MI355X over 1.7e7 log-uniform inputs in [2^-40, 2^40], the refined reciprocal and square
-//    root equal the IEEE results; each costs 5-9 instructions where the IEEE sequences with
-//    range scaling take 11-14.  Inputs here are positive normal numbers (densities, pressures,
+//  - on the GPU, 1/b is v_rcp_f64 refined by one third-order step, sqrt(x) is v_rsq_f64
+//    refined and corrected with its residual, and a / sqrt(r) is a * (refined rsq).  On
+//    MI355X, over log-uniform inputs in [2^-40, 2^40] (ppk_mhd_math_selfcheck), the refined
+//    reciprocal and square root equal the IEEE results and the inverse square root is within
+//    1 ulp; each costs 4-8 instructions where the IEEE sequences with range scaling take
+//    11-14.  Inputs here are positive normal numbers (densities, pressures,
 //    wave-speed differences); sqrt(0) is 0, other zero and non-finite inputs are not
 //    handled like IEEE.
 #ifdef PPK_EXACT_DIV
@@ -75,19 +76,18 @@ constexpr bool kSharedRecip = false;
 constexpr bool kSharedRecip = true;
 #endif
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(PPK_EXACT_DIV)
+// one third-order step from the hardware estimate r0 (relative error eps): with
+// e = 1 - b r0, 1/b = r0 (1 + e + e^2 + ...); with e = 1 - x y0^2, 1/sqrt(x) =
+// y0 (1 + e/2 + 3e^2/8 + ...): the truncation error is O(eps^3), below the final rounding
 PPK_HD double fast_rcp(double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  double e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-b, r, 1.0);
-  return fma(r, e, r);
+  const double r = __builtin_amdgcn_rcp(b);
+  const double e = fma(-b, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 }
 PPK_HD double fast_rsq(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double e = fma(-x * y, y, 1.0);
-  y = fma(0.5 * y, e, y);
-  e = fma(-x * y, y, 1.0);
-  return fma(0.5 * y, e, y);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 PPK_HD double fast_sqrt(double x) {
   const double q = __builtin_amdgcn_rsq(x);
