@@ -155,14 +155,15 @@ inline Brick sub_bricks(const DevParams &p) {
   return make_brick(p.dim, p.nx + 1, p.ny + 1, p.dim == 3 ? p.nz + 1 : 1, p.brick);
 }
 
-// The single-pass 3D step (march3d.hip): blocks of 14 x 14 output columns marching along z
+// The single-pass 3D step (march3d.hip): blocks of 16 x 12 output columns marching along z
 // through equal shares of the grid's tile-planes; one CFL partial per block
-constexpr int kMarchOut = 14;
+constexpr int kMarchOutX = 16, kMarchOutY = 12;
 // Blocks of the launch: one per CU (each block needs a whole CU's LDS), fewer when a block
 // would get under 16 tile-planes (small grids)
 inline int march_nblk(const DevParams &p, int cus) {
   const long long total =
-      (long long)((p.nx + kMarchOut - 1) / kMarchOut) * ((p.ny + kMarchOut - 1) / kMarchOut) * p.nz;
+      (long long)((p.nx + kMarchOutX - 1) / kMarchOutX) * ((p.ny + kMarchOutY - 1) / kMarchOutY) *
+      p.nz;
   long long b = cus > 0 ? cus : 256;
   if (b > total / 16) b = total / 16;
   return (int)(b < 1 ? 1 : b);

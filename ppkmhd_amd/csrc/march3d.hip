@@ -11,7 +11,7 @@ namespace ppk {
 namespace PPK_NS {
 namespace marchk {
 
-static_assert(kMO == kMarchOut, "output tile edge of the march kernel");
+static_assert(kMOX == kMarchOutX && kMOY == kMarchOutY, "output tile of the march kernel");
 
 // the block's program on the device: every thread runs each phase, then the block syncs
 template <class T>
@@ -26,8 +26,8 @@ struct DevExec {
   __device__ void cell_cfl(const T &, int, double) {}
 };
 
-// One block per CU, marching its share of the grid's tile-planes (tiles of 16 x 16 record
-// columns); 4 waves, one block per CU (the LDS of one block is 153 KB), one wave per SIMD with
+// One block per CU, marching its share of the grid's tile-planes (tiles of 18 x 14 record
+// columns); 4 waves, one block per CU (the LDS of one block is 152 KB), one wave per SIMD with
 // up to 512 registers.
 template <int RS>
 __global__ void __launch_bounds__(kMN, 1) __attribute__((amdgpu_waves_per_eu(1, 1)))

@@ -1,8 +1,8 @@
 // Single-pass 3D MHD step (k_march3d): per-thread bodies, host+device.
 //
-// A block owns a 16 x 16 tile of record columns (x, y) and marches along z through a range of
+// A block owns an 18 x 14 tile of record columns (x, y) and marches along z through a range of
 // planes (a block's share of the grid may span several tiles: MarchGeom).  One thread per
-// column.  At plane r the block
+// column (256 threads, the last 4 without a column).  At plane r the block
 //   H  computes the half-step record of every column at plane r (primitives, edge electric
 //      field, limited slopes, Hancock predictor; the same arithmetic as k_half3d) and puts it in
 //      LDS,
@@ -12,7 +12,9 @@
 //   U  finishes the update + CT of plane r-1 and starts that of plane r (the reference's order
 //      +F(i) -F(i+1) per direction and its CT order, one partial sum kept across the plane),
 //      writes plane r-1 of the new state and folds its CFL term into a per-block maximum.
-// The interior 14 x 14 columns are the block's output; the one-column ring around them
+// The interior 16 x 12 columns are the block's output (16 x 12 tiles cover a 256-wide grid
+// exactly along x and waste 3 % along y, where 14 x 14 tiles wasted 7 % of the work at 256^3);
+// the one-column ring around them
 // provides the records that the output cells' faces and edges need.  Each record field, each
 // face flux and each EMF of the output is computed once per block (faces and edges at the
 // ring are computed by both neighbouring blocks).
@@ -24,9 +26,9 @@
 // edges between planes r and r+1, solved one plane later) are computed by the cell itself, so
 // only one plane of records is held.
 //
-// LDS (doubles): two plane slots, each Q (8 x 18 x 18, origin x0-1), face fields (3 x 19 x 19,
-// origin x0-1) and E (3 x 17 x 17, origin x0), and the records of plane r (41 x 256):
-// 156,640 bytes, one block per CU.  Slot r & 1 holds plane r; after H(r) it is free and serves
+// LDS (doubles): two plane slots, each Q (8 x 20 x 16, origin x0-1), face fields (3 x 21 x 17,
+// origin x0-1) and E (3 x 19 x 15, origin x0), and the records of plane r (41 x 256):
+// 155,744 bytes, one block per CU.  Slot r & 1 holds plane r; after H(r) it is free and serves
 // as the exchange area of S/U, then receives plane r + 2.
 //
 // The arithmetic is that of the three-kernel fused path and of the staged kernels, so the
@@ -43,11 +45,15 @@ namespace marchk {
 
 using namespace fusedk;
 
-constexpr int kMT = 16;              // record tile edge (columns)
-constexpr int kMO = kMT - 2;         // output tile edge
-constexpr int kMN = kMT * kMT;       // threads per block
-constexpr int kQW = kMT + 2, kFW = kMT + 3, kEW = kMT + 1;
-constexpr int kQN = kQW * kQW, kFN = kFW * kFW, kEN = kEW * kEW;
+constexpr int kMTX = 18, kMTY = 14;      // record tile (columns along x, y)
+constexpr int kMOX = kMTX - 2, kMOY = kMTY - 2;  // output tile
+constexpr int kMC = kMTX * kMTY;         // record columns (threads kMC .. kMN-1 have none)
+constexpr int kMN = 256;                 // threads per block; also the stride of the per-column
+                                         // LDS arrays (records, exchange)
+static_assert(kMC <= kMN, "one thread per record column");
+constexpr int kQW = kMTX + 2, kFW = kMTX + 3, kEW = kMTX + 1;  // row widths of Q, faces, E
+constexpr int kQN = kQW * (kMTY + 2), kFN = kFW * (kMTY + 3), kEN = kEW * (kMTY + 1);
+static_assert(kQN <= 2 * kMN && kFN <= 2 * kMN && kEN <= 2 * kMN, "two cells per thread");
 constexpr int kSlotF = 8 * kQN, kSlotE = kSlotF + 3 * kFN, kSlot = kSlotE + 3 * kEN;
 // record fields in LDS: the 38 of a three-kernel record, then the cell's own CR and its
 // transverse slopes
@@ -80,8 +86,8 @@ struct MarchGeom {
 };
 inline MarchGeom march_geom(const DevParams &p, int nblk) {
   MarchGeom g;
-  g.ntx = (p.nx + kMO - 1) / kMO;
-  g.nty = (p.ny + kMO - 1) / kMO;
+  g.ntx = (p.nx + kMOX - 1) / kMOX;
+  g.nty = (p.ny + kMOY - 1) / kMOY;
   const long long total = (long long)g.ntx * g.nty * p.nz;
   long long b = nblk < 1 ? 1 : nblk;
   if (b > total) b = total;
@@ -91,10 +97,11 @@ inline MarchGeom march_geom(const DevParams &p, int nblk) {
   g.per = (int)((rest + b - 1) / b);
   return g;
 }
-// tile t: record columns x0 .. x0+15 (output x0+1 .. x0+14), y0 .. y0+15
+// tile t: record columns x0 .. x0+17 (output x0+1 .. x0+16), y0 .. y0+13 (output y0+1 ..
+// y0+12)
 PPK_HD void march_tile(const DevParams &p, const MarchGeom &g, int t, int &x0, int &y0) {
-  x0 = p.gw - 1 + kMO * (t % g.ntx);
-  y0 = p.gw - 1 + kMO * (t / g.ntx);
+  x0 = p.gw - 1 + kMOX * (t % g.ntx);
+  y0 = p.gw - 1 + kMOY * (t / g.ntx);
 }
 
 PPK_HD double *slot(double *lds, int s) { return lds + s * kSlot; }
@@ -174,15 +181,15 @@ struct MarchThread {
 
   PPK_HD void init(const DevParams &p, int t, int x0, int y0) {
     tid = t;
-    a = t % kMT;
-    b = t / kMT;
+    a = t % kMTX;
+    b = t / kMTX;
     i = x0 + a;
     j = y0 + b;
-    rok = i <= p.isize - p.gw && j <= p.jsize - p.gw;
+    rok = t < kMC && i <= p.isize - p.gw && j <= p.jsize - p.gw;
     const bool in = i < p.isize - p.gw && j < p.jsize - p.gw;
-    ocol = a >= 1 && a <= kMO && b >= 1 && b <= kMO && in;
-    acol = rok && a >= 1 && b >= 1 && b <= kMO && j < p.jsize - p.gw;
-    bcol = rok && b >= 1 && a >= 1 && a <= kMO && i < p.isize - p.gw;
+    ocol = t < kMC && a >= 1 && a <= kMOX && b >= 1 && b <= kMOY && in;
+    acol = rok && a >= 1 && b >= 1 && b <= kMOY && j < p.jsize - p.gw;
+    bcol = rok && b >= 1 && a >= 1 && a <= kMOX && i < p.isize - p.gw;
   }
 
   // global loads of plane pl for its Q and face cells (issued early, stashed later)
@@ -360,7 +367,7 @@ struct MarchThread {
     }
     const double *sR = lds + kRecOff;
     const bool inA = a >= 1, inB = b >= 1;
-    if (mid && inA && inB && b <= kMO) {  // x face of c (its lower face)
+    if (mid && inA && inB && b <= kMOY) {  // x face of c (its lower face)
       Half3D<double> hl;
       lds_rec(sR, tid - 1, hl);
       hl.AR = h.AL;
@@ -368,9 +375,9 @@ struct MarchThread {
       riemann_mhd_rs<RS>(ql, qr, f, p);
       fx[0] = f[ID]; fx[1] = f[IP]; fx[2] = f[IU]; fx[3] = f[IV]; fx[4] = f[IW];
     }
-    if (mid && inA && inB && a <= kMO) {  // y face
+    if (mid && inA && inB && a <= kMOX) {  // y face
       Half3D<double> hl;
-      lds_rec(sR, tid - kMT, hl);
+      lds_rec(sR, tid - kMTX, hl);
       hl.BR = h.BL;
       St<double> ql = face_state_3d(hl, 1, +1, p), qr = face_state_3d(h, 1, -1, p), f;
       swap2(ql, IU, IV); swap2(ql, IA, IB);
@@ -378,7 +385,7 @@ struct MarchThread {
       riemann_mhd_rs<RS>(ql, qr, f, p);
       fy[0] = f[ID]; fy[1] = f[IP]; fy[2] = f[IU]; fy[3] = f[IV]; fy[4] = f[IW];
     }
-    if (cross && inA && inB && a <= kMO && b <= kMO) {  // z face between r-1 and r
+    if (cross && inA && inB && a <= kMOX && b <= kMOY) {  // z face between r-1 and r
       St<double> ql = qmz, qr = face_state_3d(h, 2, -1, p), f;
       swap2(ql, IU, IW); swap2(ql, IA, IC);
       swap2(qr, IU, IW); swap2(qr, IA, IC);
@@ -388,8 +395,8 @@ struct MarchThread {
     pre();
     if (mid && inA && inB) {  // z edge: RT of c-x-y, RB of c-x, LT of c-y, LB of c
       Half3D<double> ra, rb, rd, t;
-      lds_rec(sR, tid - 1 - kMT, ra);
-      lds_rec(sR, tid - kMT, t);
+      lds_rec(sR, tid - 1 - kMTX, ra);
+      lds_rec(sR, tid - kMTX, t);
       ra.AR = t.AL; ra.dARy = t.dALy;
       lds_rec(sR, tid - 1, rb);
       ra.BR = rb.BL; ra.dBRx = rb.dBLx;
@@ -399,14 +406,14 @@ struct MarchThread {
       ez = compute_emf<EMFZ>(edge_state_3d(ra, 2, IRT, p), edge_state_3d(rb, 2, IRB, p),
                              edge_state_3d(rd, 2, ILT, p), edge_state_3d(h, 2, ILB, p), p);
     }
-    if (cross && inA && inB && a <= kMO) {  // x edge: RT of c-y-z, RB of c-y, LT of c-z, LB of c
+    if (cross && inA && inB && a <= kMOX) {  // x edge: RT of c-y-z, RB of c-y, LT of c-z, LB of c
       Half3D<double> rb;
-      lds_rec(sR, tid - kMT, rb);
+      lds_rec(sR, tid - kMTX, rb);
       rb.BR = h.BL; rb.dBRz = h.dBLz;
       ex = compute_emf<EMFX>(rtx, edge_state_3d(rb, 0, IRB, p), ltx, edge_state_3d(h, 0, ILB, p),
                              p);
     }
-    if (cross && inA && inB && b <= kMO) {  // y edge: RT of c-x-z, LT of c-z, RB of c-x, LB of c
+    if (cross && inA && inB && b <= kMOY) {  // y edge: RT of c-x-z, LT of c-z, RB of c-x, LB of c
       Half3D<double> rd;
       lds_rec(sR, tid - 1, rd);
       rd.AR = h.AL; rd.dARz = h.dALz;
@@ -414,15 +421,15 @@ struct MarchThread {
                              p);
     }
     if (carry) {  // plane-r states of the solves between planes r and r+1
-      if (inA && inB && a <= kMO && b <= kMO) qmz = face_state_3d(h, 2, +1, p);
-      if (inA && inB && a <= kMO) {
+      if (inA && inB && a <= kMOX && b <= kMOY) qmz = face_state_3d(h, 2, +1, p);
+      if (inA && inB && a <= kMOX) {
         ltx = edge_state_3d(h, 0, ILT, p);
         Half3D<double> ra;
-        lds_rec(sR, tid - kMT, ra);
+        lds_rec(sR, tid - kMTX, ra);
         ra.BR = h.BL; ra.dBRz = h.dBLz;
         rtx = edge_state_3d(ra, 0, IRT, p);
       }
-      if (inA && inB && b <= kMO) {
+      if (inA && inB && b <= kMOY) {
         lty = edge_state_3d(h, 1, ILT, p);
         Half3D<double> ra;
         lds_rec(sR, tid - 1, ra);
@@ -473,7 +480,7 @@ struct MarchThread {
     if (mid) {
       if (ocol) {
         double u[5] = {Uc[ID], Uc[IP], Uc[IU], Uc[IV], Uc[IW]};
-        const int nx = tid + 1, ny = tid + kMT;
+        const int nx = tid + 1, ny = tid + kMTX;
         // x: +F(i) -F(i+1); y: +F(j) -F(j+1) (y and z fluxes stored rotated); z: +F(k)
         u[0] += fx[0] * dtdx; u[1] += fx[1] * dtdx; u[2] += fx[2] * dtdx;
         u[3] += fx[3] * dtdx; u[4] += fx[4] * dtdx;
@@ -492,7 +499,7 @@ struct MarchThread {
       }
       if (ocol || acol) {  // A += (Ez(j+1) - Ez(j)) dt/dy, then later -= (Ey(k+1) - Ey(k)) dt/dz
         double A = Uc[IA];
-        A += (X(X_EZ, tid + kMT) - ez) * dtdy;
+        A += (X(X_EZ, tid + kMTX) - ez) * dtdy;
         Ap = A;
         Eym = ey;
       }
@@ -506,7 +513,7 @@ struct MarchThread {
     if ((mid || top) && ocol) {  // C += (Ey(i+1) - Ey(i)) dt/dx; C -= (Ex(j+1) - Ex(j)) dt/dy
       double C = Uc[IC];
       C += (X(X_EY, tid + 1) - ey) * dtdx;
-      C -= (X(X_EX, tid + kMT) - ex) * dtdy;
+      C -= (X(X_EX, tid + kMTX) - ex) * dtdy;
       if (mid) Cf = C;
       Cn = C;
     }
@@ -525,7 +532,7 @@ struct MarchThread {
     else
       mx = p.bc[1] == PPK_BC_DIRICHLET ? -un[IA] : un[IA];
     if (j + 1 < p.jsize - g || p.bc[3] == PPK_BC_PERIODIC)
-      my = sC[kMN + tid + kMT];
+      my = sC[kMN + tid + kMTX];
     else
       my = p.bc[3] == PPK_BC_DIRICHLET ? -un[IB] : un[IB];
     if (r < p.ksize - g || p.bc[5] == PPK_BC_PERIODIC)  // r: the plane above, r - 1 + 1

@@ -83,16 +83,16 @@ def kh(nx, ny, nz, **kw):
     return p, pr
 
 
-# (problem, size, parameters, blocks): tiles of 14 x 14 output columns; each block first
+# (problem, size, parameters, blocks): tiles of 16 x 12 output columns; each block first
 # marches whole tiles (one per block per round), then the left-over tiles' tile-planes are
 # cut into `blocks` equal ranges, which start and end inside a tile's z range and may span
 # two tiles
 CASES = {
-    # periodic, tiles cut at 30 / 29: 9 tiles x 20 planes; 7 whole tiles, then 2 tiles in 7
+    # periodic, tiles cut at 40 / 29: 9 tiles x 20 planes; 7 whole tiles, then 2 tiles in 7
     # ranges of 6 planes
-    "blast_hlld_minmod": (blast, (30, 29, 20), dict(slope_type=1.0, riemann=A.RIEMANN_HLLD), 7),
+    "blast_hlld_minmod": (blast, (40, 29, 20), dict(slope_type=1.0, riemann=A.RIEMANN_HLLD), 7),
     # more blocks than tiles: every tile shared out in ranges of 17 planes
-    "blast_hlld_ranges_only": (blast, (30, 29, 20), dict(slope_type=1.0,
+    "blast_hlld_ranges_only": (blast, (40, 29, 20), dict(slope_type=1.0,
                                                          riemann=A.RIEMANN_HLLD), 11),
     # every boundary type, MC slopes, HLL, odd sizes: 6 tiles x 13 planes; 5 whole tiles,
     # then one tile in 5 ranges of 3 planes
@@ -101,7 +101,7 @@ CASES = {
                                                  A.BC_DIRICHLET, A.BC_PERIODIC,
                                                  A.BC_PERIODIC)), 5),
     # exact multiples of the output tile, LLF, non-periodic z: one block runs both tiles
-    "kh_llf_whole": (kh, (28, 14, 9), dict(slope_type=1.0, riemann=A.RIEMANN_LLF,
+    "kh_llf_whole": (kh, (32, 12, 9), dict(slope_type=1.0, riemann=A.RIEMANN_LLF,
                                            bc=(A.BC_PERIODIC, A.BC_PERIODIC, A.BC_PERIODIC,
                                                A.BC_PERIODIC, A.BC_DIRICHLET,
                                                A.BC_NEUMANN)), 1),
