@@ -90,16 +90,18 @@ PPK_HD double fast_rsq(double x) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 PPK_HD double fast_sqrt(double x) {
-  // sqrt(+-0) = +-0 (the magnetosonic term d2^2 - c2 bn^2 / d can vanish): rsq(0) = inf is
-  // clamped to the largest double, so that g = x q = 0 instead of NaN (and every step below
-  // keeps the zero); positive inputs are unaffected, a NaN stays NaN
+  // Goldschmidt: g ~ sqrt(x) and h ~ 1/(2 sqrt(x)) refined together (r = 1/2 - g h), then
+  // one correction with the residual x - s^2.  sqrt(+-0) = +-0 (the magnetosonic term
+  // d2^2 - c2 bn^2 / d can vanish): rsq(0) = inf is clamped to the largest double, so that
+  // g = x q = 0 instead of NaN (and every step below keeps the zero); positive inputs are
+  // unaffected, a NaN stays NaN
   const double q = fmin(__builtin_amdgcn_rsq(fabs(x)), 1.7976931348623157e308);
   const double h = 0.5 * q;
   const double g = x * q;
   const double r = fma(-g, h, 0.5);
   const double s = fma(g, r, g);
   const double d = fma(-s, s, x);
-  return fma(d, h * (1.0 + 2.0 * r), s);
+  return fma(d, fma(h, r, h), s);
 }
 #else
 PPK_HD double fast_rcp(double b) { return 1.0 / b; }
