@@ -179,7 +179,9 @@ PPK_HD R speed_fast(const St<R> &q, int dir, R gamma0) {
   R c2 = divr(gamma0 * p, d, id);
   R d2 = R(0.5) * (divr(b2, d, id) + c2);
   R bn = dir == IX ? a : (dir == IY ? b : c);
-  return msqrt(d2 + msqrt(nonneg(d2 * d2 - divr(c2 * bn * bn, d, id))));
+  // c2 bn^2 / d: in the default builds (c2 / d) bn^2, c2 / d shared by the directions of a state
+  const R cbd = kSharedRecip ? (c2 * id) * (bn * bn) : c2 * bn * bn / d;
+  return msqrt(d2 + msqrt(nonneg(d2 * d2 - cbd)));
 }
 
 // per-cell CFL term sum_d (c_f,d + |v_d|)/dx_d, reference src/shared/mhd_utils.h:319-366 and
@@ -303,13 +305,18 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   R ustar = divr(rcr * ur + rcl * ul + (ptotl - ptotr), rcr + rcl, irc);
   R ptotstar = divr(rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur), rcr + rcl, irc);
 
-  const R ia2 = recip(a * a);
+  // the degenerate-state test |estar / a^2 - 1| <= 1e-8, in the default builds as
+  // |estar - a^2| <= 1e-8 a^2 (no reciprocal; the same test but for rounding at its edge)
+  auto degenerate = [&](R estar) {
+    return a * a > R(0) && (kSharedRecip ? dabs(estar - a * a) <= R(1e-8) * (a * a)
+                                         : dabs(estar / (a * a) - R(1)) <= R(1e-8));
+  };
   const R isl = recip(sl - ustar);
   R rstarl = divr(rl * (sl - ul), sl - ustar, isl);
   R estar = rl * (sl - ul) * (sl - ustar) - a * a;
   R el = rl * (sl - ul) * (sl - ul) - a * a;
   R vstarl, wstarl, bstarl, cstarl;
-  if (a * a > R(0) && dabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
+  if (degenerate(estar)) {
     vstarl = vl;
     bstarl = bl;
     wstarl = wl;
@@ -333,7 +340,7 @@ PPK_HD void riemann_hlld(St<R> &qleft, St<R> &qright, St<R> &flux,
   estar = rr * (sr - ur) * (sr - ustar) - a * a;
   R er = rr * (sr - ur) * (sr - ur) - a * a;
   R vstarr, wstarr, bstarr, cstarr;
-  if (a * a > R(0) && dabs(divr(estar, a * a, ia2) - R(1)) <= R(1e-8)) {
+  if (degenerate(estar)) {
     vstarr = vr;
     bstarr = br;
     wstarr = wr;
