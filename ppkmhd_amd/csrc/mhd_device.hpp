@@ -609,34 +609,40 @@ PPK_HD R mag_riemann2d_hlld(const St<R> &qLL, const St<R> &qRL,
   R BstarR = divr(SAT * BstarRR - SAB * BstarRL, SAT - SAB, iSAy);
   R BstarL = divr(SAT * BstarLR - SAB * BstarLL, SAT - SAB, iSAy);
 
-  // 0/1 region masks from the sign bit (copysign(1, S) > 0)
+  // the five region EMFs: strongly interacting (central), bottom, top, left, right
+  const R Ec = divr(divr(SAL * SAB * EstarRR - SAL * SAT * EstarRL - SAR * SAB * EstarLR +
+                             SAR * SAT * EstarLL,
+                         SAR - SAL, iSAx),
+                    SAT - SAB, iSAy) -
+               divr(SAT * SAB, SAT - SAB, iSAy) * (AstarT - AstarB) +
+               divr(SAR * SAL, SAR - SAL, iSAx) * (BstarR - BstarL);
+  const R Eb = divr(SAR * EstarLLx - SAL * EstarRLx + SAR * SAL * (bRL - bLL), SAR - SAL, iSAx);
+  const R Et = divr(SAR * EstarLRx - SAL * EstarRRx + SAR * SAL * (bRR - bLR), SAR - SAL, iSAx);
+  const R El = divr(SAT * EstarLLy - SAB * EstarLRy - SAT * SAB * (aLR - aLL), SAT - SAB, iSAy);
+  const R Er = divr(SAT * EstarRLy - SAB * EstarRRy - SAT * SAB * (aRR - aRL), SAT - SAB, iSAy);
+
+  if (kSharedRecip) {
+    // the default builds select the one region whose mask product is 1 (the masks partition
+    // the sign cases; the blend below adds that term to zeros, which gives the same value for
+    // finite region EMFs, the sign of a zero result aside)
+    const bool sbp = !sign_bit(SB), stp = !sign_bit(ST), slp = !sign_bit(SL), srp = !sign_bit(SR);
+    if (sbp) return slp ? ELL : (srp ? Eb : ERL);
+    if (!stp) return slp ? ELR : (srp ? Et : ERR);
+    if (slp) return El;
+    return srp ? Ec : Er;
+  }
+  // 0/1 region masks from the sign bit (copysign(1, S) > 0), blended as the reference does
   const R SB_pos = sign_bit(SB) ? R(0) : R(1), SB_neg = R(1) - SB_pos;
   const R ST_pos = sign_bit(ST) ? R(0) : R(1), ST_neg = R(1) - ST_pos;
   const R SL_pos = sign_bit(SL) ? R(0) : R(1), SL_neg = R(1) - SL_pos;
   const R SR_pos = sign_bit(SR) ? R(0) : R(1), SR_neg = R(1) - SR_pos;
 
-  R E = R(0), tmpE;
-  tmpE = divr(divr(SAL * SAB * EstarRR - SAL * SAT * EstarRL - SAR * SAB * EstarLR +
-                        SAR * SAT * EstarLL,
-                    SAR - SAL, iSAx),
-              SAT - SAB, iSAy) -
-         divr(SAT * SAB, SAT - SAB, iSAy) * (AstarT - AstarB) +
-         divr(SAR * SAL, SAR - SAL, iSAx) * (BstarR - BstarL);
-  E += (SB_neg * ST_pos * SL_neg * SR_pos) * tmpE;
-
-  tmpE = divr(SAR * EstarLLx - SAL * EstarRLx + SAR * SAL * (bRL - bLL), SAR - SAL, iSAx);
-  tmpE = SL_pos * ELL + SL_neg * SR_neg * ERL + SL_neg * SR_pos * tmpE;
-  E += SB_pos * tmpE;
-
-  tmpE = divr(SAR * EstarLRx - SAL * EstarRRx + SAR * SAL * (bRR - bLR), SAR - SAL, iSAx);
-  tmpE = SL_pos * ELR + SL_neg * SR_neg * ERR + SL_neg * SR_pos * tmpE;
-  E += (SB_neg * ST_neg) * tmpE;
-
-  tmpE = divr(SAT * EstarLLy - SAB * EstarLRy - SAT * SAB * (aLR - aLL), SAT - SAB, iSAy);
-  E += (SB_neg * ST_pos * SL_pos) * tmpE;
-
-  tmpE = divr(SAT * EstarRLy - SAB * EstarRRy - SAT * SAB * (aRR - aRL), SAT - SAB, iSAy);
-  E += (SB_neg * ST_pos * SL_neg * SR_neg) * tmpE;
+  R E = R(0);
+  E += (SB_neg * ST_pos * SL_neg * SR_pos) * Ec;
+  E += SB_pos * (SL_pos * ELL + SL_neg * SR_neg * ERL + SL_neg * SR_pos * Eb);
+  E += (SB_neg * ST_neg) * (SL_pos * ELR + SL_neg * SR_neg * ERR + SL_neg * SR_pos * Et);
+  E += (SB_neg * ST_pos * SL_pos) * El;
+  E += (SB_neg * ST_pos * SL_neg * SR_neg) * Er;
   return E;
 }
 
