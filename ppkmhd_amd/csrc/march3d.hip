@@ -29,17 +29,17 @@ struct DevExec {
 // One block per CU, marching its share of the grid's tile-planes (tiles of 18 x 14 record
 // columns); 4 waves, one block per CU (the LDS of one block is 152 KB), one wave per SIMD with
 // up to 512 registers.
-template <int RS>
+template <int RS, int ST>
 __global__ void __launch_bounds__(kMN, 1) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_march3d(const double *__restrict__ Uin, double *__restrict__ Uout, double *__restrict__ parts,
           DevParams p, MarchGeom g, const double *ctrl, int fold) {
   if (ctrl[CTRL_ACTIVE] == 0.0) return;
   __shared__ double lds[kMarchLds];
   __shared__ double red[kMN / 64];
-  MarchThread<RS> t;
-  DevExec<MarchThread<RS>> X{t};
+  MarchThread<RS, ST> t;
+  DevExec<MarchThread<RS, ST>> X{t};
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  march_block<RS>(X, Uin, Uout, p, g, L, ctrl[CTRL_DT], fold != 0, lds);
+  march_block<RS, ST>(X, Uin, Uout, p, g, L, ctrl[CTRL_DT], fold != 0, lds);
   if (fold) {
     const double m = block_max(t.cfl, red);
     if (threadIdx.x == 0) parts[blockIdx.x] = m;
@@ -54,18 +54,21 @@ hipError_t march_godunov(const DevParams &p, const double *Uin, double *Uout, do
   const MarchGeom g = march_geom(p, p.march);
   const dim3 grid((unsigned)g.blocks()), block(kMN);
   if (tm) tm->mark(ST_FUSED_MARCH, true);
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, Uin, Uout, parts, p, g, ctrl, fold_cfl);
+  };
+  // minmod slopes (slope_type 1) with the factor fixed at compile time (x * 1 is exact, so the
+  // result is the same; two multiplies fewer per limited slope); other slope types read it
+  const bool minmod = p.slope_type == 1.0;
   switch (p.riemann) {
     case PPK_RIEMANN_HLLD:
-      hipLaunchKernelGGL((k_march3d<PPK_RIEMANN_HLLD>), grid, block, 0, s, Uin, Uout, parts, p,
-                         g, ctrl, fold_cfl);
+      minmod ? launch(k_march3d<PPK_RIEMANN_HLLD, 1>) : launch(k_march3d<PPK_RIEMANN_HLLD, 0>);
       break;
     case PPK_RIEMANN_HLL:
-      hipLaunchKernelGGL((k_march3d<PPK_RIEMANN_HLL>), grid, block, 0, s, Uin, Uout, parts, p,
-                         g, ctrl, fold_cfl);
+      minmod ? launch(k_march3d<PPK_RIEMANN_HLL, 1>) : launch(k_march3d<PPK_RIEMANN_HLL, 0>);
       break;
     default:
-      hipLaunchKernelGGL((k_march3d<PPK_RIEMANN_LLF>), grid, block, 0, s, Uin, Uout, parts, p,
-                         g, ctrl, fold_cfl);
+      minmod ? launch(k_march3d<PPK_RIEMANN_LLF, 1>) : launch(k_march3d<PPK_RIEMANN_LLF, 0>);
       break;
   }
   if (tm) tm->mark(ST_FUSED_MARCH, false);

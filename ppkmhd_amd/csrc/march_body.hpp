@@ -151,7 +151,7 @@ PPK_HD void swap2(St<R> &s, int a, int b) {
 }
 
 // The registers of one thread (column) of a marching block.
-template <int RS>
+template <int RS, int ST = 0>
 struct MarchThread {
   int tid, a, b;     // thread, column position in the tile
   int i, j;          // column's cell indices
@@ -328,7 +328,7 @@ struct MarchThread {
     };
     const double bfNb[6] = {F(Fa, 0, 0, 0), F(Fa, 1, 0, 0), F(Fa, 0, 0, 1),
                             F(Fa, 0, 1, 1), F(Fa, 0, 0, 2), F(Fb, 0, 0, 2)};
-    const double xst = fmin(p.slope_type, 2.0);
+    const double xst = ST ? double(ST) : fmin(p.slope_type, 2.0);
     // the left-side transverse slopes, and the cell's own CR slopes (entries 10, 11)
     const double d12[12] = {limited_slope(xst, F(Fa, 0, 0, 0), F(Fa, 0, 1, 0), F(Fa, 0, -1, 0)),
                             limited_slope(xst, F(Fa, 0, 0, 0), F(Fb, 0, 0, 0), Am),
@@ -346,7 +346,7 @@ struct MarchThread {
     const double Ey[4] = {E(Ea, 0, 0, 1), E(Eb, 0, 0, 1), E(Ea, 1, 0, 1), E(Eb, 1, 0, 1)};
     const double Ez[4] = {E(Ea, 0, 0, 2), E(Ea, 0, 1, 2), E(Ea, 1, 0, 2), E(Ea, 1, 1, 2)};
     const double dtdx = dt / p.dx, dtdy = dt / p.dy, dtdz = dt / p.dz;
-    half_step_3d(q, qPX, qMX, qPY, qMY, qPZ, Qm, bfNb, d12, Ex, Ey, Ez, dtdx, dtdy, dtdz, p, h);
+    half_step_3d<ST>(q, qPX, qMX, qPY, qMY, qPZ, Qm, bfNb, d12, Ex, Ey, Ez, dtdx, dtdy, dtdz, p, h);
     lds_store_rec(lds + kRecOff, tid, h);
     Qm = q;
     Am = bfNb[0];
@@ -550,13 +550,13 @@ struct MarchThread {
 // One segment of a block's work: output planes [z0, z1) of tile `tile`.  X.each(f) runs
 // f(thread) for every thread of the block and then synchronizes the block;
 // X.cell_cfl(thread, plane, term) sees each output cell's CFL term (a no-op in the kernel).
-template <int RS, class Exec>
+template <int RS, int ST, class Exec>
 PPK_HD void march_segment(Exec &X, const double *__restrict__ Uin, double *__restrict__ Uout,
                           const DevParams &p, const MarchGeom &g, int tile, int z0, int z1,
                           double dt, bool fold, double *lds) {
   int x0, y0;
   march_tile(p, g, tile, x0, y0);
-  using T = MarchThread<RS>;
+  using T = MarchThread<RS, ST>;
   X.each([&](T &t) {
     t.init(p, X.tid(t), x0, y0);
     t.prefetch(Uin, p, x0, y0, z0 - 2);
@@ -622,17 +622,17 @@ PPK_HD bool march_piece(const DevParams &p, const MarchGeom &g, int L, int k, in
 
 // The block's program, written once for the kernel and for the host emulation: the segments
 // of range L (see MarchGeom), one after the other; each thread's CFL maximum spans them all.
-template <int RS, class Exec>
+template <int RS, int ST, class Exec>
 PPK_HD void march_block(Exec &X, const double *__restrict__ Uin, double *__restrict__ Uout,
                         const DevParams &p, const MarchGeom &g, int L, double dt, bool fold,
                         double *lds) {
-  using T = MarchThread<RS>;
+  using T = MarchThread<RS, ST>;
   X.each([&](T &t) { t.cfl = 0.0; });
   // one call site of the segment program, so that it is inlined once; segment k is computed
   // from (L, k) alone, so that nothing but k is carried across a segment
   int tile, z0, z1;
   for (int k = 0; march_piece(p, g, L, k, tile, z0, z1); ++k)
-    march_segment<RS>(X, Uin, Uout, p, g, tile, z0, z1, dt, fold, lds);
+    march_segment<RS, ST>(X, Uin, Uout, p, g, tile, z0, z1, dt, fold, lds);
 }
 
 }  // namespace marchk

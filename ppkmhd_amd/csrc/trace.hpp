@@ -189,16 +189,17 @@ struct Half3D {
 // inputs: q and its 6 neighbours, the 6 face fields of the cell, 12 transverse face slopes
 // (dbf, layout of MHDRunFunctors3D.h:726-738), edge electric fields E[x|y|z][2][2]
 // (Ex: ELL ELR ERL ERR, Ey: FLL FLR FRL FRR, Ez: GLL GLR GRL GRR)
-template <class R>
+// ST: the slope type when fixed at compile time (1: minmod), 0: p.slope_type
+template <int ST = 0, class R>
 PPK_HD void half_step_3d(const St<R> &q, const St<R> &qPX, const St<R> &qMX,
                                              const St<R> &qPY, const St<R> &qMY, const St<R> &qPZ,
                                              const St<R> &qMZ, const R (&bfNb)[6], const R (&dbf)[12],
                                              const R (&Ex)[4], const R (&Ey)[4], const R (&Ez)[4],
                                              R dtdx, R dtdy, R dtdz, const DevParams &p,
                                              Half3D<R> &h) {
-  const R st = R(p.slope_type);
+  const R st = ST ? R(ST) : R(p.slope_type);
   const R gamma = R(p.gamma0);
-  const bool sl = p.slope12;
+  const bool sl = ST ? true : p.slope12;
   auto S = [&](int v, const St<R> &plus, const St<R> &minus) {
     return sl ? R(0.5) * limited_slope(st, q[v], plus[v], minus[v]) : R(0);
   };

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "march_body.hpp"
@@ -67,10 +68,10 @@ DevParams dev_params(const ppk_mhd_params &p) {
   return d;
 }
 
-template <int RS>
+template <int RS, int ST>
 double run(const DevParams &p, int nblk, const double *Uin, double *Uout, double dt,
            std::vector<double> &cells) {
-  using T = MarchThread<RS>;
+  using T = MarchThread<RS, ST>;
   const MarchGeom g = march_geom(p, nblk);
   std::vector<double> lds(kMarchLds);
   std::vector<T> ts(kMN);
@@ -79,7 +80,7 @@ double run(const DevParams &p, int nblk, const double *Uin, double *Uout, double
     for (double &x : lds) x = std::nan("");
     std::memset(static_cast<void *>(ts.data()), 0xff, sizeof(T) * ts.size());
     HostExec<T> X{ts, p, cells};
-    march_block<RS>(X, Uin, Uout, p, g, (int)L, dt, true, lds.data());
+    march_block<RS, ST>(X, Uin, Uout, p, g, (int)L, dt, true, lds.data());
     for (const T &t : ts) {
       if (std::isnan(t.cfl)) return std::nan("");
       cfl = std::fmax(cfl, t.cfl);
@@ -134,13 +135,17 @@ int main(int argc, char **argv) {
         for (int i = p.gw; i < p.isize - p.gw; ++i)
           U1[(size_t)v * p.vstride + cidx(p, i, j, k)] = std::nan("");
   std::vector<double> cells((size_t)p.vstride, std::nan(""));
-  double cfl;
-  if (p.riemann == PPK_RIEMANN_HLLD)
-    cfl = run<PPK_RIEMANN_HLLD>(p, nblk, U0.data(), U1.data(), dt, cells);
-  else if (p.riemann == PPK_RIEMANN_HLL)
-    cfl = run<PPK_RIEMANN_HLL>(p, nblk, U0.data(), U1.data(), dt, cells);
-  else
-    cfl = run<PPK_RIEMANN_LLF>(p, nblk, U0.data(), U1.data(), dt, cells);
+  // the kernel's instantiations: minmod slopes with the factor fixed, others read it
+  auto go = [&](auto st) {
+    constexpr int ST = decltype(st)::value;
+    if (p.riemann == PPK_RIEMANN_HLLD)
+      return run<PPK_RIEMANN_HLLD, ST>(p, nblk, U0.data(), U1.data(), dt, cells);
+    if (p.riemann == PPK_RIEMANN_HLL)
+      return run<PPK_RIEMANN_HLL, ST>(p, nblk, U0.data(), U1.data(), dt, cells);
+    return run<PPK_RIEMANN_LLF, ST>(p, nblk, U0.data(), U1.data(), dt, cells);
+  };
+  const double cfl = p.slope_type == 1.0 ? go(std::integral_constant<int, 1>{})
+                                         : go(std::integral_constant<int, 0>{});
   FILE *f = std::fopen((dir + "/u1.bin").c_str(), "wb");
   if (!f || std::fwrite(U1.data(), sizeof(double), n, f) != n) return 2;
   std::fclose(f);
