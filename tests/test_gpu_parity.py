@@ -87,7 +87,7 @@ def test_strict_is_bit_identical(oracle, gpu, case, path):
 @pytest.mark.parametrize("path", [A.PATH_FUSED, A.PATH_FUSED_RECORDS])
 def test_strict_at_size_many_tiles(oracle, gpu, path):
     """A 70 x 60 x 50 blast (MC slopes, an oblique field) with partial tiles in every
-    direction (single-pass kernel: 5 x 5 tiles of 14 x 14 columns shared out to 78 blocks in
+    direction (single-pass kernel: 5 x 5 tiles of 16 x 12 columns shared out to 78 blocks in
     ranges of 17 planes, some spanning two tiles; three-kernel form: 1280 half-step tiles,
     partial bricks): strict bit-identical to the oracle over 3 CFL steps."""
     p = make_params(3, 70, 60, 50, gamma0=5 / 3, cfl=0.8, tEnd=1.0, slope_type=2.0,
@@ -97,6 +97,26 @@ def test_strict_at_size_many_tiles(oracle, gpu, path):
     pr.blast_bx, pr.blast_by, pr.blast_bz = 0.6, -0.3, 0.2
     o, g = run_both(oracle, p, pr, 3)
     assert np.array_equal(g.download()[p.interior()], o.get_state()[p.interior()])
+
+
+@pytest.mark.parametrize("path", [A.PATH_FUSED, A.PATH_FUSED_RECORDS])
+@pytest.mark.parametrize("slope", [1.0, 2.0])
+def test_strict_isothermal(oracle, gpu, slope, path):
+    """The isothermal equation of state (cIso > 0: p = rho cIso^2 in the primitives, the HLLD
+    and the 2D EMF solver, src/shared/RiemannSolvers_MHD.h), which the single-pass kernel runs
+    in its own instantiations: a 3D blast with partial tiles, strict bit-identical to the
+    oracle over 3 CFL steps, with minmod and MC slopes."""
+    p = make_params(3, 20, 15, 12, gamma0=5 / 3, cfl=0.8, tEnd=1.0, slope_type=slope,
+                    precision=A.FP64_STRICT, kernel_path=path)
+    p.cIso = 0.7
+    pr = default_problem(A.PROBLEM_BLAST, p)
+    pr.blast_radius = 0.3
+    pr.blast_bx, pr.blast_by, pr.blast_bz = 0.6, -0.3, 0.2
+    o, g = run_both(oracle, p, pr, 3)
+    U = g.download()
+    assert np.isfinite(U).all()
+    assert np.array_equal(U[p.interior()], o.get_state()[p.interior()])
+    assert g.time() == o.time()
 
 
 @pytest.mark.parametrize("path", PATHS)
