@@ -64,11 +64,22 @@ STEP_BYTES_PER_CELL = 128.0
 MEASURED_BYTES_PER_ITEM_3D = {"fused_riemann": 8.77e9 / 257 ** 3, "fused_half": 7.99e9 / 258 ** 3,
                               "fused_update": 5.35e9 / 256 ** 3,
                               # the single-pass kernel (profiles/march_blast3d_256_pmc.md)
-                              "fused_march": 4.35e9 / 256 ** 3}
+                              "fused_march": 4.36e9 / 256 ** 3}
 
 
 # The single-pass kernel does the work of the three fused stages in one launch
 MARCH_STAGES = ("fused_half", "fused_riemann", "fused_update")
+
+# 2D step (Orszag-Tang, HLLD + CT): algorithmic FP64 operations per item and the item range
+# of each stage as (count, k) over (nx + k) x (ny + k) cells, from the instrumented oracle
+# (tests/test_oracle_step.py): primitives over [0, size-1)^2, the CFL term and the update over
+# the interior, the trace over (n+3)^2, the two faces and the EMF over (n+1)^2
+FLOP_ITEMS_2D = {"primitives": (28.0, 5), "cfl": (31.0, 0), "trace": (454.0, 3),
+                 "fluxes": (570.0, 1), "emf": (525.0, 1), "update_ct": (54.0, 0)}
+
+
+def step_flops_2d(nx, ny):
+    return sum(f * (nx + k) * (ny + k) for f, k in FLOP_ITEMS_2D.values())
 
 
 def items(stage, n):
@@ -362,8 +373,10 @@ def bench_ensemble(a):
     its = [b.time(m)[2] for m in range(len(ps))]
     assert min(its) == a.warmup + a.steps, "a member finished early"
     value = cells * a.steps / elapsed
+    flops = sum(step_flops_2d(p.nx, p.ny) for p in ps)
     log(f"[bench] {a.steps} steps in {elapsed:.3f} s: {elapsed / a.steps * 1e3:.3f} ms/step, "
-        f"{value / 1e9:.3f} G cell-updates/s")
+        f"{value / 1e9:.3f} G cell-updates/s, {flops / (elapsed / a.steps) / 1e12:.2f} "
+        f"FP64 TFLOP/s")
     b.close()
 
     # the same members, one single-grid solver after another (a short sample of each)
@@ -400,7 +413,14 @@ def bench_ensemble(a):
                    "tile": tile, "brick": brick},
         "sequential_single_grid_value": seq_value,
         "batch_over_sequential": value / seq_value,
-        "roofline": None,
+        "roofline": {"bound": "mfma", "achieved": flops / (elapsed / a.steps) / 1e12,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / (elapsed / a.steps) / 1e12 / PEAK_FP64_TFLOPS,
+                     "traffic": None, "kernel": "captured ensemble step (all stage kernels)",
+                     "flop_per_launch": flops,
+                     "note": "FP64 VALU work of every member per ensemble step over the "
+                             "step's wall time (the stage kernels run inside one graph, "
+                             "so no single kernel is timed); peak is the MI355X FP64 figure"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -318,3 +318,35 @@ def test_counted_flops_match_bench_constants(oracle, n):
         c = oracle.op_counts(stage)
         counted = c["add"] + c["mul"] + c["div"] + c["sqrt"]
         assert counted == pytest.approx(bench.stage_flops(stage, n), rel=1e-12), stage
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 16), (24, 24), (20, 12)])
+def test_counted_flops_2d_match_bench_constants(oracle, nx, ny):
+    """The same for one CFL step of the 2D Orszag-Tang (the batched ensemble's workload): the
+    instrumented oracle's operation counts per stage equal bench.py's 2D per-item figures over
+    their (nx + k) x (ny + k) ranges (a non-square grid separates the two extents); the E and
+    face-field slope stages are part of the 2D trace."""
+    import bench
+
+    p = make_params(2, nx, ny, gamma0=5 / 3, cfl=0.8, tEnd=1.0)
+    pr = default_problem(A.PROBLEM_ORSZAG_TANG, p)
+    s = oracle.Solver(p, counting=True)
+    s.init_problem(pr)
+    s.step(0, A.DT_CFL)
+    oracle.op_counts_reset()
+    s.step(0, A.DT_CFL)
+
+    def counted(stage):
+        c = oracle.op_counts(stage)
+        return c["add"] + c["mul"] + c["div"] + c["sqrt"]
+
+    F = bench.FLOP_ITEMS_2D
+    items = lambda key: F[key][0] * (nx + F[key][1]) * (ny + F[key][1])
+    assert counted("primitives_dt") == items("primitives") + items("cfl")
+    assert counted("elec_field") == 0 and counted("mag_slopes") == 0
+    assert counted("trace") == items("trace")
+    assert counted("fluxes") == items("fluxes")
+    assert counted("emf") == items("emf")
+    assert counted("update_ct") == items("update_ct")
+    total = sum(counted(k) for k in ("primitives_dt", "trace", "fluxes", "emf", "update_ct"))
+    assert total == bench.step_flops_2d(nx, ny)
