@@ -133,8 +133,8 @@ PPK_HD R dmin(R a, R b) { return fmin(a, b); }
 // The magnetosonic discriminant d2^2 - c2 bn^2 / rho is >= 0 in exact arithmetic and 0 when
 // the field is along the normal; rounded (and, in the default builds, FMA-contracted or float)
 // it can come out a few ulp below 0, and its square root NaN.  The default builds clamp it at
-// 0; the strict build keeps the reference's expression as written.
-// (fmax: one instruction, and 0 for a NaN input as the compare-and-select form)
+// 0 (with fmax, one instruction, which like a compare and select gives 0 for a NaN input);
+// the strict build keeps the reference's expression as written.
 template <class R>
 PPK_HD R nonneg(R x) { return kSharedRecip ? dmax(x, R(0)) : x; }
 
