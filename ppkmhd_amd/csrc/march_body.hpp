@@ -59,7 +59,10 @@ constexpr int kSlotF = 8 * kQN, kSlotE = kSlotF + 3 * kFN, kSlot = kSlotE + 3 * 
 // transverse slopes
 enum { R3_CR = kRec3, R3_DCRX, R3_DCRY, R3X_N };
 constexpr int kRecOff = 2 * kSlot;
-constexpr int kMarchLds = kRecOff + R3X_N * kMN;
+// record field stride: one double more than the columns' stride, so that the fields of a
+// column fall on different LDS banks
+constexpr int kRS = kMN + 1;
+constexpr int kMarchLds = kRecOff + R3X_N * kRS;
 static_assert(kMarchLds * 8 <= 160 * 1024, "march tile exceeds the LDS of a CU");
 // exchange area of S -> U in the free slot: x flux (5), y flux (5), Ez, Ex, Ey per column
 enum { X_FX = 0, X_FY = 5, X_EZ = 10, X_EX = 11, X_EY = 12, X_N = 13 };
@@ -110,7 +113,7 @@ PPK_HD double *slot(double *lds, int s) { return lds + s * kSlot; }
 // load_rec3 leaves them
 PPK_HD void lds_rec(const double *sR, int n, Half3D<double> &h) {
   const double *a = sR + n;
-  constexpr int S = kMN;
+  constexpr int S = kRS;
   h.r = a[R3_R * S];     h.pp = a[R3_P * S];    h.u = a[R3_U * S];     h.v = a[R3_V * S];
   h.w = a[R3_W * S];     h.A = a[R3_A * S];     h.B = a[R3_B * S];     h.C = a[R3_C * S];
   h.drx = a[R3_DRX * S]; h.dpx = a[R3_DPX * S]; h.dux = a[R3_DUX * S]; h.dvx = a[R3_DVX * S];
@@ -128,7 +131,7 @@ PPK_HD void lds_rec(const double *sR, int n, Half3D<double> &h) {
 }
 PPK_HD void lds_store_rec(double *sR, int n, const Half3D<double> &h) {
   double *o = sR + n;
-  constexpr int S = kMN;
+  constexpr int S = kRS;
   o[R3_R * S] = h.r;     o[R3_P * S] = h.pp;    o[R3_U * S] = h.u;     o[R3_V * S] = h.v;
   o[R3_W * S] = h.w;     o[R3_A * S] = h.A;     o[R3_B * S] = h.B;     o[R3_C * S] = h.C;
   o[R3_DRX * S] = h.drx; o[R3_DPX * S] = h.dpx; o[R3_DUX * S] = h.dux; o[R3_DVX * S] = h.dvx;
