@@ -27,8 +27,8 @@
 // only one plane of records is held.
 //
 // LDS (doubles): two plane slots, each Q (8 x 20 x 16, origin x0-1), face fields (3 x 21 x 17,
-// origin x0-1) and E (3 x 19 x 15, origin x0), and the records of plane r (41 x 256):
-// 155,744 bytes, one block per CU.  Slot r & 1 holds plane r; after H(r) it is free and serves
+// origin x0-1) and E (3 x 19 x 15, origin x0), and the records of plane r (41 fields at a
+// stride of 257): 156,072 bytes, one block per CU.  Slot r & 1 holds plane r; after H(r) it is free and serves
 // as the exchange area of S/U, then receives plane r + 2.
 //
 // The arithmetic is that of the three-kernel fused path and of the staged kernels, so the
